@@ -1,0 +1,178 @@
+/* simple_gpu.h — C ABI of the MI355X (gfx950) SIMPLE lid-driven-cavity backend.
+ *
+ * This is the drop-in boundary for `backend = gpu`. It replaces the reference's C++ class
+ * `HipSolver` and `run_simple_hip` (include/simple/solver_hip.hpp:11-40): one entry point per
+ * solver phase, the same phase order and semantics as `outer_iteration` (src/solver.cpp:27-93),
+ * plus a whole-run entry point (`sg_advance`) that the host's outer loop
+ * (src/solver.cpp:106-132, `run_outer_loop`) calls in chunks.
+ *
+ * Conventions
+ *  - Plain C: PODs, pointers and sizes only. No exceptions cross this boundary.
+ *  - Every function returns 0 on success, a nonzero SG_E* code on failure; sg_last_error()
+ *    returns a thread-local message for the last failure ("GPU error: ..." for HIP errors,
+ *    mirroring src/hip/solver_hip.hip:17-23). The C++ host turns a nonzero status into
+ *    std::runtime_error, so `error (rank 0): ...` and exit code 1 are unchanged
+ *    (src/main.cpp:77-83).
+ *  - Host arrays are full fields with halos, nrows x ncols row-major, (i, j) -> i*ncols + j
+ *    (include/simple/field2d.hpp:36-65, docs/SPEC.md section 2).
+ *  - One host thread per solver; all device work is ordered on the solver's own HIP stream.
+ */
+#ifndef SIMPLE_GPU_H
+#define SIMPLE_GPU_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SG_ABI_VERSION 1
+
+/* Status codes. */
+enum {
+    SG_OK = 0,
+    SG_E_ARG = 1,       /* invalid argument (bad params, unknown field id, null pointer) */
+    SG_E_HIP = 2,       /* a HIP runtime call failed */
+    SG_E_UNSUPPORTED = 3,
+    SG_E_NOMEM = 4,
+    SG_E_STATE = 5      /* call not valid in the current state */
+};
+
+/* Enumerations, same values and spellings as include/simple/params.hpp:9-13. */
+enum { SG_UDS = 0, SG_QUICK = 1 };
+enum { SG_NORM_ABSOLUTE = 0, SG_NORM_RUNNING_MAX = 1 };
+enum { SG_BACKEND_SERIAL = 0, SG_BACKEND_OPENMP = 1, SG_BACKEND_GPU = 2 };
+enum { SG_MOM_GS_LEX = 0, SG_MOM_JACOBI = 1, SG_MOM_GS_MULTICOLOUR = 2 };
+enum { SG_P_GS_LEX = 0, SG_P_JACOBI = 1, SG_P_GS_REDBLACK = 2, SG_P_CG = 3, SG_P_MG = 4 };
+/* Additions with no reference counterpart. */
+enum { SG_MG_FP64 = 0, SG_MG_MIXED = 1 };      /* mg_precision */
+enum { SG_WARM_NONE = 0, SG_WARM_LEARNED = 1 }; /* pressure_warm_start */
+
+/* POD mirror of Params (include/simple/params.hpp:15-58): every numeric key of the config
+ * file, the derived sizes, and the additions of this backend. output_dir and the MPI fields
+ * stay on the host. */
+typedef struct sg_params {
+    int NX, NY, H;            /* N_cells_x (rows), N_cells_y (columns), halo width */
+    int N_halo_x, N_halo_y;
+    double Re;
+    int maxit_inner_u, maxit_inner_v, maxit_inner_p;
+    double error_tolerance;
+    int maxit;
+    double relaxation_factor_u, relaxation_factor_v, relaxation_factor_p;
+    int convection_scheme;      /* SG_UDS / SG_QUICK */
+    int diffusion_order;        /* 2 or 4 */
+    int Hybrid_switch;          /* must be 0 */
+    int residual_normalisation; /* SG_NORM_* */
+    int backend;                /* SG_BACKEND_* */
+    int momentum_solver;        /* SG_MOM_* */
+    int pressure_solver;        /* SG_P_* */
+    double pressure_tolerance;  /* cg: ||r||/||b|| stop */
+    int pressure_maxit;         /* cg: iteration cap */
+    int mg_cycles, mg_pre, mg_post;
+    double mg_omega;
+    int write_fields;
+    /* derived (src/params.cpp:174-178) */
+    int nrows, ncols;
+    double dx, dy, nu;
+    /* additions */
+    int mg_precision;           /* SG_MG_FP64 (reference arithmetic) or SG_MG_MIXED */
+    int mg_coarsest;            /* smallest coarse side kept (reference: 4) */
+    int mg_dc_maxit;            /* mixed: FP64 defect-correction steps per solve */
+    double mg_dc_tol;           /* mixed: stop when ||r||/||b|| <= this (0: run mg_dc_maxit) */
+    int pressure_warm_start;    /* SG_WARM_* */
+    int check_every;            /* device-side stop test: host looks every this many iterations */
+} sg_params;
+
+/* Array identifiers. Host layout of every id is nrows x ncols doubles. */
+enum {
+    SG_F_U = 0, SG_F_V, SG_F_P,
+    SG_F_CU_AE, SG_F_CU_AW, SG_F_CU_AN, SG_F_CU_AS,
+    SG_F_CU_AEE, SG_F_CU_AWW, SG_F_CU_ANN, SG_F_CU_ASS, SG_F_CU_AP,
+    SG_F_CV_AE, SG_F_CV_AW, SG_F_CV_AN, SG_F_CV_AS,
+    SG_F_CV_AEE, SG_F_CV_AWW, SG_F_CV_ANN, SG_F_CV_ASS, SG_F_CV_AP,
+    SG_F_AEP, SG_F_AWP, SG_F_ANP, SG_F_ASP, SG_F_APP, SG_F_B, SG_F_PC,
+    SG_F_COUNT
+};
+
+/* Phase indices of the per-phase timers, same order and names as include/simple/timer.hpp:6-20
+ * (assemble_u, bc_u, residual_u, sweep_u, assemble_v, bc_v, residual_v, sweep_v, assemble_p,
+ * solve_p, correct). */
+#define SG_NPHASES 11
+
+typedef struct sg_solver sg_solver;
+
+/* Last error message of this thread ("" if none). */
+const char* sg_last_error(void);
+int sg_abi_version(void);
+
+/* Number of visible GPUs (0 if none); name of device 0 into buf. */
+int sg_device_count(int* n);
+int sg_device_name(char* buf, size_t len);
+
+/* Validates p and fills the derived fields (nrows, ncols, dx, dy, nu), like the validation and
+ * derivation block of make_params (src/params.cpp:152-178). */
+int sg_params_finalize(sg_params* p);
+/* Reference defaults (include/simple/params.hpp:15-58) plus this backend's defaults. */
+void sg_params_default(sg_params* p);
+
+/* HipSolver(const Params&, const State&) (include/simple/solver_hip.hpp:13): allocates the
+ * device arrays and uploads u, v, p (any may be NULL: zero). Coefficient and p' arrays start 0. */
+int sg_create(const sg_params* p, const double* u, const double* v, const double* pf,
+              sg_solver** out);
+int sg_destroy(sg_solver* s);
+
+/* Upload / download one array (host layout nrows x ncols). download replaces
+ * HipSolver::download (include/simple/solver_hip.hpp:34). */
+int sg_upload(sg_solver* s, int field_id, const double* host);
+int sg_download(sg_solver* s, int field_id, double* host);
+
+/* Per-phase entry points, HipSolver::assemble_u .. correct (include/simple/solver_hip.hpp:19-29).
+ * Each is enqueued and completed before returning. Residuals use the reference's summation
+ * order (rows j ascending from 0.0, then rows ascending; src/decomp.cpp:176-183). */
+int sg_assemble_u(sg_solver* s);
+int sg_bc_u(sg_solver* s);
+int sg_residual_u(sg_solver* s, double* res);
+int sg_sweep_u(sg_solver* s, int nsweeps);
+int sg_assemble_v(sg_solver* s);
+int sg_bc_v(sg_solver* s);
+int sg_residual_v(sg_solver* s, double* res);
+int sg_sweep_v(sg_solver* s, int nsweeps);
+int sg_assemble_p(sg_solver* s, double* res_p);
+int sg_solve_p(sg_solver* s);
+int sg_correct(sg_solver* s);
+
+/* HipSolver::outer_iteration (include/simple/solver_hip.hpp:31): one outer iteration in eager
+ * mode, residuals into res[3] (u, v, p) and per-phase seconds (hipEvent-timed, SG_NPHASES) added
+ * into phase_s (may be NULL). */
+int sg_outer_iteration(sg_solver* s, double res[3], double* phase_s);
+
+/* Runs up to max_iters outer iterations as replays of one captured hipGraph, with the stopping
+ * rule of run_outer_loop (src/solver.cpp:112-128) evaluated on the device after every
+ * iteration: work stops at the converged iteration, never past it. Writes iters_done and, for
+ * each iteration done, [res_u, res_v, res_p] into log (3*max_iters doubles, may be NULL);
+ * *converged = 1 if the stop rule fired. Iteration numbers continue across calls. */
+int sg_advance(sg_solver* s, int max_iters, int* iters_done, double* log, int* converged);
+
+/* Iterations done so far by this solver (sg_outer_iteration and sg_advance). */
+int sg_iterations(sg_solver* s, int* iters);
+
+/* Mean seconds of one launch of the named kernel over the last sg_advance call that ran with
+ * timing on (sg_set_kernel_timing), measured with hipEvents on the solver's stream, and the
+ * algorithmic bytes of one launch (SG_K_* ids). */
+enum {
+    SG_K_SWEEP_U = 0,  /* one colour pass of the u multicolour sweep */
+    SG_K_SWEEP_V,
+    SG_K_MG_SMOOTH0,   /* one colour pass of the level-0 multigrid smoother */
+    SG_K_ASSEMBLE_U,
+    SG_K_COUNT
+};
+int sg_kernel_probe(sg_solver* s, int kernel_id, int reps, double* mean_seconds,
+                    double* bytes_per_launch, int* launches_per_iteration);
+
+/* Multigrid information: number of levels and statistics of the last solve. */
+int sg_mg_levels(sg_solver* s, int* nlevels);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SIMPLE_GPU_H */

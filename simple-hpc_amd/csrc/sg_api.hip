@@ -1,0 +1,618 @@
+// sg_api.hip — the C ABI of include/simple_gpu.h: solver lifetime, uploads/downloads, one entry
+// point per solver phase, the eager outer iteration with hipEvent phase timers, and sg_advance,
+// which replays one captured hipGraph per outer iteration with the stopping rule evaluated on the
+// device (the reference's HipSolver and run_simple_hip, include/simple/solver_hip.hpp:11-40, and
+// run_outer_loop, src/solver.cpp:106-132).
+#include "sg_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <new>
+
+namespace sg {
+
+namespace {
+thread_local std::string g_err;
+
+// Stopping rule of run_outer_loop (src/solver.cpp:112-128), one thread: log the residuals,
+// update the running maxima, test max(...) <= tol && iter > 1 with std::max's comparison order.
+// A residual that is not finite stops the loop (diverged), which the reference does not detect.
+__global__ void k_stop_rule(Ctl* ctl, const double* scal, double* log, int log_cap, int norm,
+                            double tol) {
+    if (threadIdx.x != 0 || ctl->stop)
+        return;
+    const int iter = ++ctl->iter;
+    const double ru = scal[S_RES_U], rv = scal[S_RES_V], rp = scal[S_RES_P];
+    if (ctl->log_pos < log_cap) {
+        double* e = log + 3 * ctl->log_pos;
+        e[0] = ru;
+        e[1] = rv;
+        e[2] = rp;
+    }
+    ++ctl->log_pos;
+    if (!isfinite(ru) || !isfinite(rv) || !isfinite(rp)) {
+        ctl->diverged = 1;
+        ctl->stop = 1;
+        return;
+    }
+    double tu = ru, tv = rv, tp = rp;
+    if (norm == SG_NORM_RUNNING_MAX) {
+        ctl->mu = ctl->mu < ru ? ru : ctl->mu;
+        ctl->mv = ctl->mv < rv ? rv : ctl->mv;
+        ctl->mp = ctl->mp < rp ? rp : ctl->mp;
+        tu = ctl->mu > 0.0 ? ru / ctl->mu : 0.0;
+        tv = ctl->mv > 0.0 ? rv / ctl->mv : 0.0;
+        tp = ctl->mp > 0.0 ? rp / ctl->mp : 0.0;
+    }
+    double m = tu;
+    if (m < tv)
+        m = tv;
+    if (m < tp)
+        m = tp;
+    if (m <= tol && iter > 1) {
+        ctl->converged = 1;
+        ctl->stop = 1;
+    }
+}
+
+__global__ void k_ctl_begin(Ctl* ctl) {
+    if (threadIdx.x == 0) {
+        ctl->stop = 0;
+        ctl->log_pos = 0;
+        ctl->converged = 0;
+    }
+}
+
+// Diffusive coefficients, src/coefficients.cpp:41-63 (host arithmetic, IEEE double, no
+// contraction: the same bits as the CPU).
+void diffusive(const sg_params& p, C9& d2, C9& d4) {
+    const double dx = p.dx, dy = p.dy, nu = p.nu;
+    d2.v[0] = d2.v[1] = dy * nu / dx;
+    d2.v[2] = d2.v[3] = dx * nu / dy;
+    d2.v[4] = d2.v[5] = d2.v[6] = d2.v[7] = 0.0;
+    d2.v[8] = -nu * (-2 * dx / dy - 2 * dy / dx);
+    d4.v[0] = d4.v[1] = 4.0 / 3 * dy * nu / dx;
+    d4.v[2] = d4.v[3] = 4.0 / 3 * dx * nu / dy;
+    d4.v[4] = d4.v[5] = -1.0 / 12 * dy * nu / dx;
+    d4.v[6] = d4.v[7] = -1.0 / 12 * dx * nu / dy;
+    d4.v[8] = -nu * (-2.5 * dx / dy - 2.5 * dy / dx);
+}
+
+template <class T> T* dalloc(Solver& s, size_t n) {
+    void* q = nullptr;
+    SG_HIP(hipMalloc(&q, sizeof(T) * std::max<size_t>(n, 1)));
+    s.owned.push_back(q);
+    SG_HIP(hipMemset(q, 0, sizeof(T) * std::max<size_t>(n, 1)));
+    return static_cast<T*>(q);
+}
+
+void ensure_log(Solver& s, int cap) {
+    if (cap <= s.log_cap)
+        return;
+    if (s.dlog) {
+        SG_HIP(hipFree(s.dlog));
+        s.owned.erase(std::remove(s.owned.begin(), s.owned.end(), static_cast<void*>(s.dlog)),
+                      s.owned.end());
+    }
+    s.log_cap = std::max(cap, 256);
+    s.dlog = dalloc<double>(s, 3 * static_cast<size_t>(s.log_cap));
+    if (s.graph) { // the stop-rule node holds the old log pointer
+        SG_HIP(hipGraphExecDestroy(s.graph));
+        s.graph = nullptr;
+    }
+}
+
+// One outer iteration (src/solver.cpp:27-93), enqueued. ev: optional phase events (12).
+void enqueue_iteration(Solver& s, const int* stop, bool eager, hipEvent_t* ev) {
+    const sg_params& p = s.p;
+    auto mark = [&](int k) {
+        if (ev)
+            SG_HIP(hipEventRecord(ev[k], s.stream));
+    };
+    mark(0);
+    launch_assemble(s, true, stop);
+    mark(1);
+    launch_bc(s, true, stop);
+    mark(2);
+    launch_mom_residual(s, true, s.scal + S_RES_U, stop);
+    mark(3);
+    launch_mom_sweeps(s, true, p.maxit_inner_u, stop);
+    mark(4);
+    launch_assemble(s, false, stop);
+    mark(5);
+    launch_bc(s, false, stop);
+    mark(6);
+    launch_mom_residual(s, false, s.scal + S_RES_V, stop);
+    mark(7);
+    launch_mom_sweeps(s, false, p.maxit_inner_v, stop);
+    mark(8);
+    launch_assemble_p(s, s.scal + S_RES_P, stop);
+    mark(9);
+    launch_solve_p(s, stop, eager);
+    mark(10);
+    launch_correct(s, stop);
+    mark(11);
+    hipLaunchKernelGGL(k_stop_rule, dim3(1), dim3(64), 0, s.stream, s.ctl, s.scal, s.dlog,
+                       s.log_cap, p.residual_normalisation, p.error_tolerance);
+    SG_HIP(hipGetLastError());
+}
+
+void capture_graph(Solver& s) {
+    if (s.graph)
+        return;
+    hipGraph_t g = nullptr;
+    SG_HIP(hipStreamBeginCapture(s.stream, hipStreamCaptureModeRelaxed));
+    try {
+        enqueue_iteration(s, &s.ctl->stop, false, nullptr);
+    } catch (...) {
+        hipGraph_t tmp = nullptr;
+        (void)hipStreamEndCapture(s.stream, &tmp);
+        if (tmp)
+            (void)hipGraphDestroy(tmp);
+        throw;
+    }
+    SG_HIP(hipStreamEndCapture(s.stream, &g));
+    hipError_t e = hipGraphInstantiate(&s.graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    SG_HIP(e);
+}
+
+void sync(Solver& s) { SG_HIP(hipStreamSynchronize(s.stream)); }
+
+Ctl read_ctl(Solver& s) {
+    Ctl c;
+    SG_HIP(hipMemcpyAsync(&c, s.ctl, sizeof c, hipMemcpyDeviceToHost, s.stream));
+    sync(s);
+    return c;
+}
+
+template <class F> int guard(F f) {
+    try {
+        g_err.clear();
+        f();
+        return SG_OK;
+    } catch (const HipError& e) {
+        g_err = e.what();
+        return SG_E_HIP;
+    } catch (const ArgError& e) {
+        g_err = e.what();
+        return SG_E_ARG;
+    } catch (const std::bad_alloc&) {
+        g_err = "out of host memory";
+        return SG_E_NOMEM;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return SG_E_STATE;
+    }
+}
+
+Solver& S(sg_solver* h) {
+    if (!h)
+        throw ArgError("null solver");
+    return *reinterpret_cast<Solver*>(h);
+}
+
+void validate(const sg_params& p) {
+    // The checks of src/params.cpp:152-172, same messages.
+    if (p.N_halo_x != p.N_halo_y)
+        throw ArgError("N_halo_x must equal N_halo_y");
+    if (p.N_halo_x < 2)
+        throw ArgError("halo width H must be >= 2");
+    if (p.diffusion_order != 2 && p.diffusion_order != 4)
+        throw ArgError("diffusion_order must be 2 or 4");
+    if (p.NX < 1 || p.NY < 1)
+        throw ArgError("N_cells_x and N_cells_y must be positive");
+    if (p.Hybrid_switch)
+        throw ArgError("Hybrid_switch = True is not supported");
+    if (!(p.pressure_tolerance > 0.0))
+        throw ArgError("pressure_tolerance must be positive");
+    if (p.pressure_maxit < 1)
+        throw ArgError("pressure_maxit must be >= 1");
+    if (p.mg_cycles < 1)
+        throw ArgError("mg_cycles must be >= 1");
+    if (p.mg_pre < 0 || p.mg_post < 0)
+        throw ArgError("mg_pre and mg_post must be >= 0");
+    if (!(p.mg_omega > 0.0 && p.mg_omega <= 2.0))
+        throw ArgError("mg_omega must be in (0, 2]");
+    // Additions.
+    if (p.mg_coarsest < 1)
+        throw ArgError("mg_coarsest must be >= 1");
+    if (p.mg_precision != SG_MG_FP64 && p.mg_precision != SG_MG_MIXED)
+        throw ArgError("mg_precision must be fp64 or mixed");
+    if (p.mg_dc_maxit < 1)
+        throw ArgError("mg_dc_maxit must be >= 1");
+    if (p.check_every < 1)
+        throw ArgError("check_every must be >= 1");
+}
+
+} // namespace
+
+void set_error(const std::string& m) { g_err = m; }
+
+} // namespace sg
+
+using namespace sg;
+
+extern "C" {
+
+const char* sg_last_error(void) { return g_err.c_str(); }
+int sg_abi_version(void) { return SG_ABI_VERSION; }
+
+int sg_device_count(int* n) {
+    return guard([&] {
+        if (!n)
+            throw ArgError("null pointer");
+        int c = 0;
+        if (hipGetDeviceCount(&c) != hipSuccess)
+            c = 0;
+        *n = c;
+    });
+}
+
+int sg_device_name(char* buf, size_t len) {
+    return guard([&] {
+        hipDeviceProp_t prop;
+        SG_HIP(hipGetDeviceProperties(&prop, 0));
+        if (buf && len) {
+            std::snprintf(buf, len, "%s (%s, %d CUs)", prop.name, prop.gcnArchName,
+                          prop.multiProcessorCount);
+        }
+    });
+}
+
+void sg_params_default(sg_params* p) {
+    std::memset(p, 0, sizeof *p);
+    // include/simple/params.hpp:15-36
+    p->NX = p->NY = 80;
+    p->H = p->N_halo_x = p->N_halo_y = 2;
+    p->Re = 1000.0;
+    p->maxit_inner_u = 4;
+    p->maxit_inner_v = 4;
+    p->maxit_inner_p = 8;
+    p->error_tolerance = 1e-4;
+    p->maxit = 15000;
+    p->relaxation_factor_u = 0.3;
+    p->relaxation_factor_v = 0.3;
+    p->relaxation_factor_p = 0.2;
+    p->convection_scheme = SG_QUICK;
+    p->diffusion_order = 4;
+    p->Hybrid_switch = 0;
+    p->residual_normalisation = SG_NORM_ABSOLUTE;
+    p->backend = SG_BACKEND_SERIAL;
+    p->momentum_solver = SG_MOM_GS_LEX;
+    p->pressure_solver = SG_P_GS_LEX;
+    p->pressure_tolerance = 1e-6;
+    p->pressure_maxit = 1000;
+    p->mg_cycles = 1;
+    p->mg_pre = 2;
+    p->mg_post = 2;
+    p->mg_omega = 1.8;
+    p->write_fields = 1;
+    // additions
+    p->mg_precision = SG_MG_FP64;
+    p->mg_coarsest = 4;
+    p->mg_dc_maxit = 1;
+    p->mg_dc_tol = 0.0;
+    p->pressure_warm_start = SG_WARM_NONE;
+    p->check_every = 200;
+}
+
+int sg_params_finalize(sg_params* p) {
+    return guard([&] {
+        if (!p)
+            throw ArgError("null params");
+        validate(*p);
+        // src/params.cpp:173-178
+        p->H = p->N_halo_x;
+        p->nrows = p->NX + 2 * p->H + 1;
+        p->ncols = p->NY + 2 * p->H + 1;
+        p->dx = 1.0 / p->NY;
+        p->dy = 1.0 / p->NX;
+        p->nu = 1.0 / p->Re;
+    });
+}
+
+int sg_create(const sg_params* pin, const double* u, const double* v, const double* pf,
+              sg_solver** out) {
+    Solver* sp = nullptr;
+    const int rc = guard([&] {
+        if (!pin || !out)
+            throw ArgError("null argument");
+        sg_params p = *pin;
+        validate(p);
+        if (p.nrows != p.NX + 2 * p.H + 1 || p.ncols != p.NY + 2 * p.H + 1)
+            throw ArgError("params not finalized (call sg_params_finalize)");
+        if (p.momentum_solver == SG_MOM_GS_LEX)
+            throw ArgError("backend = gpu runs momentum_solver = gs_multicolour or jacobi "
+                           "(gs_lex is a serial sweep)");
+        if (p.pressure_solver == SG_P_GS_LEX)
+            throw ArgError("backend = gpu runs pressure_solver = gs_redblack, jacobi, cg or mg "
+                           "(gs_lex is a serial sweep)");
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+            throw HipError("GPU error: no HIP device visible");
+        sp = new Solver;
+        Solver& s = *sp;
+        s.p = p;
+        SG_HIP(hipGetDevice(&s.device));
+        s.nr = p.nrows;
+        s.nc = p.ncols;
+        s.ld = (p.ncols + 15) / 16 * 16;
+        s.n = static_cast<size_t>(s.nr) * s.ld;
+        SG_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+        for (int k = 0; k < SG_F_COUNT; ++k)
+            s.f[k] = dalloc<double>(s, s.n);
+        s.scratch = dalloc<double>(s, s.n);
+        s.cell = dalloc<double>(s, 4 * static_cast<size_t>(std::max(p.NX + 2, 1)) * (p.NY + 2));
+        s.rowbuf = dalloc<double>(s, 4 * static_cast<size_t>(s.nr));
+        s.scal = dalloc<double>(s, S_COUNT);
+        s.ctl = dalloc<Ctl>(s, 1);
+        s.never = dalloc<int>(s, 1);
+        if (p.pressure_solver == SG_P_CG)
+            for (auto& c : s.cg)
+                c = dalloc<double>(s, s.n);
+        ensure_log(s, 256);
+        diffusive(p, s.diff2, s.diff4);
+        if (p.pressure_solver == SG_P_MG)
+            mg_setup(s);
+        for (auto& e : s.ev)
+            SG_HIP(hipEventCreate(&e));
+        SG_HIP(hipEventCreate(&s.kt0));
+        SG_HIP(hipEventCreate(&s.kt1));
+        const double* init[3] = {u, v, pf};
+        for (int k = 0; k < 3; ++k)
+            if (init[k])
+                SG_HIP(hipMemcpy2D(s.f[k], sizeof(double) * s.ld, init[k],
+                                   sizeof(double) * s.nc, sizeof(double) * s.nc, s.nr,
+                                   hipMemcpyHostToDevice));
+        SG_HIP(hipDeviceSynchronize());
+    });
+    if (rc != SG_OK) {
+        if (sp)
+            sg_destroy(reinterpret_cast<sg_solver*>(sp));
+        return rc;
+    }
+    *out = reinterpret_cast<sg_solver*>(sp);
+    return SG_OK;
+}
+
+int sg_destroy(sg_solver* h) {
+    if (!h)
+        return SG_OK;
+    Solver* s = reinterpret_cast<Solver*>(h);
+    if (s->stream)
+        (void)hipStreamSynchronize(s->stream);
+    if (s->graph)
+        (void)hipGraphExecDestroy(s->graph);
+    for (auto e : s->ev)
+        if (e)
+            (void)hipEventDestroy(e);
+    if (s->kt0)
+        (void)hipEventDestroy(s->kt0);
+    if (s->kt1)
+        (void)hipEventDestroy(s->kt1);
+    for (void* q : s->owned)
+        (void)hipFree(q);
+    if (s->stream)
+        (void)hipStreamDestroy(s->stream);
+    delete s;
+    return SG_OK;
+}
+
+int sg_upload(sg_solver* h, int id, const double* host) {
+    return guard([&] {
+        Solver& s = S(h);
+        if (id < 0 || id >= SG_F_COUNT || !host)
+            throw ArgError("bad field id or null pointer");
+        SG_HIP(hipMemcpy2DAsync(s.f[id], sizeof(double) * s.ld, host, sizeof(double) * s.nc,
+                                sizeof(double) * s.nc, s.nr, hipMemcpyHostToDevice, s.stream));
+        sync(s);
+    });
+}
+
+int sg_download(sg_solver* h, int id, double* host) {
+    return guard([&] {
+        Solver& s = S(h);
+        if (id < 0 || id >= SG_F_COUNT || !host)
+            throw ArgError("bad field id or null pointer");
+        SG_HIP(hipMemcpy2DAsync(host, sizeof(double) * s.nc, s.f[id], sizeof(double) * s.ld,
+                                sizeof(double) * s.nc, s.nr, hipMemcpyDeviceToHost, s.stream));
+        sync(s);
+    });
+}
+
+static int phase(sg_solver* h, void (*fn)(Solver&)) {
+    return guard([&] {
+        Solver& s = S(h);
+        fn(s);
+        sync(s);
+    });
+}
+
+static int phase_res(sg_solver* h, double* res, int slot, void (*fn)(Solver&)) {
+    return guard([&] {
+        Solver& s = S(h);
+        fn(s);
+        if (res) {
+            SG_HIP(hipMemcpyAsync(res, s.scal + slot, sizeof(double), hipMemcpyDeviceToHost,
+                                  s.stream));
+        }
+        sync(s);
+    });
+}
+
+int sg_assemble_u(sg_solver* h) {
+    return phase(h, [](Solver& s) { launch_assemble(s, true, s.never); });
+}
+int sg_bc_u(sg_solver* h) {
+    return phase(h, [](Solver& s) { launch_bc(s, true, s.never); });
+}
+int sg_residual_u(sg_solver* h, double* res) {
+    return phase_res(h, res, S_RES_U,
+                     [](Solver& s) { launch_mom_residual(s, true, s.scal + S_RES_U, s.never); });
+}
+int sg_sweep_u(sg_solver* h, int n) {
+    return guard([&] {
+        Solver& s = S(h);
+        launch_mom_sweeps(s, true, n, s.never);
+        sync(s);
+    });
+}
+int sg_assemble_v(sg_solver* h) {
+    return phase(h, [](Solver& s) { launch_assemble(s, false, s.never); });
+}
+int sg_bc_v(sg_solver* h) {
+    return phase(h, [](Solver& s) { launch_bc(s, false, s.never); });
+}
+int sg_residual_v(sg_solver* h, double* res) {
+    return phase_res(h, res, S_RES_V,
+                     [](Solver& s) { launch_mom_residual(s, false, s.scal + S_RES_V, s.never); });
+}
+int sg_sweep_v(sg_solver* h, int n) {
+    return guard([&] {
+        Solver& s = S(h);
+        launch_mom_sweeps(s, false, n, s.never);
+        sync(s);
+    });
+}
+int sg_assemble_p(sg_solver* h, double* res) {
+    return phase_res(h, res, S_RES_P,
+                     [](Solver& s) { launch_assemble_p(s, s.scal + S_RES_P, s.never); });
+}
+int sg_solve_p(sg_solver* h) {
+    return phase(h, [](Solver& s) { launch_solve_p(s, s.never, true); });
+}
+int sg_correct(sg_solver* h) {
+    return phase(h, [](Solver& s) { launch_correct(s, s.never); });
+}
+
+int sg_outer_iteration(sg_solver* h, double res[3], double* phase_s) {
+    return guard([&] {
+        Solver& s = S(h);
+        hipLaunchKernelGGL(k_ctl_begin, dim3(1), dim3(64), 0, s.stream, s.ctl);
+        enqueue_iteration(s, s.never, true, s.ev);
+        if (res)
+            SG_HIP(hipMemcpyAsync(res, s.scal + S_RES_U, 3 * sizeof(double),
+                                  hipMemcpyDeviceToHost, s.stream));
+        sync(s);
+        if (phase_s)
+            for (int k = 0; k < SG_NPHASES; ++k) {
+                float ms = 0.f;
+                SG_HIP(hipEventElapsedTime(&ms, s.ev[k], s.ev[k + 1]));
+                phase_s[k] += 1e-3 * ms;
+            }
+        s.iters = read_ctl(s).iter;
+    });
+}
+
+int sg_advance(sg_solver* h, int max_iters, int* iters_done, double* log, int* converged) {
+    return guard([&] {
+        Solver& s = S(h);
+        if (max_iters < 0)
+            throw ArgError("max_iters must be >= 0");
+        ensure_log(s, max_iters);
+        hipLaunchKernelGGL(k_ctl_begin, dim3(1), dim3(64), 0, s.stream, s.ctl);
+        SG_HIP(hipGetLastError());
+        if (s.p.pressure_solver == SG_P_CG) {
+            // CG needs the host between its iterations: eager iterations, stop flag checked.
+            for (int k = 0; k < max_iters; ++k) {
+                enqueue_iteration(s, &s.ctl->stop, true, nullptr);
+                if (read_ctl(s).stop)
+                    break;
+            }
+        } else {
+            capture_graph(s);
+            for (int k = 0; k < max_iters; ++k)
+                SG_HIP(hipGraphLaunch(s.graph, s.stream));
+        }
+        const Ctl c = read_ctl(s);
+        const int done = std::min(c.log_pos, max_iters);
+        if (log && done > 0) {
+            SG_HIP(hipMemcpyAsync(log, s.dlog, sizeof(double) * 3 * done, hipMemcpyDeviceToHost,
+                                  s.stream));
+            sync(s);
+        }
+        if (iters_done)
+            *iters_done = done;
+        if (converged)
+            *converged = c.converged ? 1 : (c.diverged ? -1 : 0);
+        s.iters = c.iter;
+    });
+}
+
+int sg_iterations(sg_solver* h, int* iters) {
+    return guard([&] {
+        Solver& s = S(h);
+        *iters = read_ctl(s).iter;
+    });
+}
+
+int sg_kernel_probe(sg_solver* h, int kid, int reps, double* mean_s, double* bytes,
+                    int* per_iter) {
+    return guard([&] {
+        Solver& s = S(h);
+        const sg_params& p = s.p;
+        if (reps < 1)
+            throw ArgError("reps must be >= 1");
+        const int ncol = (p.convection_scheme == SG_UDS && p.diffusion_order == 2) ? 2 : 5;
+        double cells = 0.0, bpc = 0.0;
+        int per = 0;
+        double* keep = nullptr; // field the probe modifies, restored afterwards
+        std::function<void(int)> one;
+        switch (kid) {
+        case SG_K_SWEEP_U:
+        case SG_K_SWEEP_V: {
+            const bool is_u = kid == SG_K_SWEEP_U;
+            if (p.momentum_solver != SG_MOM_GS_MULTICOLOUR)
+                throw ArgError("sweep probe needs momentum_solver = gs_multicolour");
+            keep = s.F(is_u ? SG_F_U : SG_F_V);
+            cells = is_u ? static_cast<double>(p.NX) * (p.NY - 1)
+                         : static_cast<double>(p.NX - 1) * p.NY;
+            cells /= ncol;
+            bpc = 96.0; // 9 coefficients + p + u read + u write, 8 B each, per updated cell
+            per = ncol * (is_u ? p.maxit_inner_u : p.maxit_inner_v);
+            one = [&s, is_u, ncol](int k) { launch_mom_colour_pass(s, is_u, k % ncol, s.never); };
+            break;
+        }
+        case SG_K_ASSEMBLE_U:
+            cells = static_cast<double>(p.NX) * (p.NY + 1);
+            bpc = 88.0; // u, v read; 9 coefficients written
+            per = 1;
+            one = [&s](int) { launch_assemble(s, true, s.never); };
+            break;
+        default:
+            throw ArgError("unknown kernel id");
+        }
+        if (keep)
+            SG_HIP(hipMemcpyAsync(s.scratch, keep, sizeof(double) * s.n, hipMemcpyDeviceToDevice,
+                                  s.stream));
+        one(0); // warm
+        SG_HIP(hipEventRecord(s.kt0, s.stream));
+        for (int k = 0; k < reps; ++k)
+            one(k);
+        SG_HIP(hipEventRecord(s.kt1, s.stream));
+        sync(s);
+        float ms = 0.f;
+        SG_HIP(hipEventElapsedTime(&ms, s.kt0, s.kt1));
+        if (keep) {
+            SG_HIP(hipMemcpyAsync(keep, s.scratch, sizeof(double) * s.n, hipMemcpyDeviceToDevice,
+                                  s.stream));
+            sync(s);
+        }
+        if (mean_s)
+            *mean_s = 1e-3 * ms / reps;
+        if (bytes)
+            *bytes = cells * bpc;
+        if (per_iter)
+            *per_iter = per;
+    });
+}
+
+int sg_mg_levels(sg_solver* h, int* n) {
+    return guard([&] { *n = static_cast<int>(S(h).mg.size()); });
+}
+
+} // extern "C"

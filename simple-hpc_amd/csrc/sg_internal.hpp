@@ -1,0 +1,134 @@
+// sg_internal.hpp — device state and kernel launchers of the gfx950 SIMPLE backend.
+//
+// Device layout: every level-0 field is an nrows x ld FP64 array (ld = ncols rounded up to 16
+// doubles, so each row starts on a 128-byte line), row-major, (i, j) -> i*ld + j, the same (i, j)
+// as the host arrays. Fields stay resident in HBM for the whole run; the host sees them only
+// through sg_upload / sg_download.
+//
+// Every kernel that runs inside the outer iteration takes `const int* stop` and returns at once
+// when *stop != 0: the device-side stopping rule sets it, so graph replays queued past the
+// converged iteration do no work.
+#pragma once
+#include "../../include/simple_gpu.h"
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+namespace sg {
+
+void set_error(const std::string& msg);
+
+#define SG_HIP(call)                                                                           \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            throw sg::HipError(std::string("GPU error: ") + #call + ": " +                     \
+                               hipGetErrorString(e_));                                         \
+    } while (0)
+
+struct HipError : std::exception {
+    std::string m;
+    explicit HipError(std::string s) : m(std::move(s)) {}
+    const char* what() const noexcept override { return m.c_str(); }
+};
+struct ArgError : std::exception {
+    std::string m;
+    explicit ArgError(std::string s) : m(std::move(s)) {}
+    const char* what() const noexcept override { return m.c_str(); }
+};
+
+// Nine coefficients in the order AE, AW, AN, AS, AEE, AWW, ANN, ASS, AP.
+struct C9 {
+    double v[9];
+};
+
+// A multigrid level on the device. Level 0 aliases the p' arrays (stride ld, offset of interior
+// cell (H, H+1)); coarse levels own (nx+2) x (ny+2) arrays with a zero halo.
+struct MgLevel {
+    int nx = 0, ny = 0, stride = 0;
+    double *AE = nullptr, *AW = nullptr, *AN = nullptr, *AS = nullptr, *AP = nullptr;
+    double *b = nullptr, *x = nullptr, *r = nullptr;
+    float *fAE = nullptr, *fAW = nullptr, *fAN = nullptr, *fAS = nullptr, *fAP = nullptr;
+    float *fb = nullptr, *fx = nullptr, *fr = nullptr; // FP32 copies (mixed precision)
+};
+
+// Device control block of the outer loop.
+struct Ctl {
+    int stop;       // nonzero: skip all work (set by the stop rule)
+    int iter;       // outer iterations completed
+    int converged;  // stop rule fired
+    int diverged;   // a residual was not finite
+    int log_pos;    // entries written into the log buffer in the current sg_advance call
+    int coarse_sweeps;
+    int pad[2];
+    double mu, mv, mp; // running maxima (residual_normalisation = running_max)
+};
+
+// Device scalar slots.
+enum {
+    S_RES_U = 0, S_RES_V, S_RES_P, S_SUM_A, S_SUM_B, S_SUM_C, S_SUM_D, S_COUNT = 32
+};
+
+struct Solver {
+    sg_params p{};
+    int nr = 0, nc = 0, ld = 0;
+    size_t n = 0; // nr * ld
+    hipStream_t stream = nullptr;
+    int device = 0;
+    double* f[SG_F_COUNT] = {};
+    double* scratch = nullptr;         // Jacobi previous iterate
+    double* cell = nullptr;            // per-cell values for ordered sums (4 planes of n)
+    double* rowbuf = nullptr;          // row sums (4 x nr)
+    double* scal = nullptr;            // S_COUNT device scalars
+    double* cg[4] = {};                // CG r, z, d, q
+    Ctl* ctl = nullptr;                // device control block
+    int* never = nullptr;              // device int that stays 0 (eager per-phase calls)
+    double* dlog = nullptr;            // device log, 3 doubles per iteration (log_cap entries)
+    int log_cap = 0;
+    std::vector<MgLevel> mg;
+    double* mg_r0 = nullptr;           // level-0 residual
+    std::vector<void*> owned;          // allocations to free
+    C9 diff2{}, diff4{};               // diffusive coefficients: order 2 (and band), order 4
+    hipGraphExec_t graph = nullptr;
+    int iters = 0;                     // host copy of completed iterations
+    long long cg_total = 0;
+    int cg_last = 0;
+    hipEvent_t ev[SG_NPHASES + 1] = {};
+    // kernel timing
+    hipEvent_t kt0 = nullptr, kt1 = nullptr;
+
+    const double* F(int id) const { return f[id]; }
+    double* F(int id) { return f[id]; }
+};
+
+// ---- launchers (each enqueues on s.stream; `stop` may be s.never) ----
+// momentum (sg_momentum.hip)
+void launch_assemble(Solver& s, bool is_u, const int* stop);
+void launch_bc(Solver& s, bool is_u, const int* stop);
+void launch_mom_residual(Solver& s, bool is_u, double* slot, const int* stop);
+void launch_mom_sweeps(Solver& s, bool is_u, int nsweeps, const int* stop);
+void launch_mom_colour_pass(Solver& s, bool is_u, int colour, const int* stop);
+// pressure (sg_pressure.hip)
+void launch_assemble_p(Solver& s, double* slot, const int* stop);
+void launch_solve_p(Solver& s, const int* stop, bool eager);
+void launch_correct(Solver& s, const int* stop);
+void launch_fill(Solver& s, double* a, size_t n, double v, const int* stop);
+// ordered sums (sg_reduce.hip): rows [i0, i1], columns [j0, j1] of a pitched array, row sums j
+// ascending from 0.0 into rowbuf, then rows ascending from 0.0 into *slot.
+void launch_ordered_sum(Solver& s, const double* a, int pitch, int i0, int i1, int j0, int j1,
+                        double* slot, const int* stop);
+// multigrid (sg_mg.hip)
+void mg_setup(Solver& s);
+void launch_mg_solve(Solver& s, const int* stop);
+void launch_remove_mean0(Solver& s, double* a, const int* stop);
+// outer loop control (sg_api.hip)
+void launch_stop_rule(Solver& s);
+
+inline unsigned cdiv(long a, long b) { return static_cast<unsigned>((a + b - 1) / b); }
+
+// Device helpers.
+__device__ __forceinline__ bool stopped(const int* stop) { return *stop != 0; }
+
+} // namespace sg

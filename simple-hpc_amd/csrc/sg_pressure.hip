@@ -1,0 +1,388 @@
+// sg_pressure.hip — pressure correction on gfx950: p' system assembly, the red-black / damped
+// Jacobi / CG solvers for p', and the u, v, p corrections (src/pressure.cpp). Multigrid is in
+// sg_mg.hip. Arithmetic is the reference's expression for expression.
+#include "sg_internal.hpp"
+
+#include <cmath>
+
+namespace sg {
+
+namespace {
+
+constexpr int BX = 64, BY = 4;
+
+struct PArgs {
+    double *AE, *AW, *AN, *AS, *AP, *b, *pc;
+    int ld, H, NX, NY;
+};
+
+PArgs p_args(Solver& s) {
+    return {s.F(SG_F_AEP), s.F(SG_F_AWP), s.F(SG_F_ANP), s.F(SG_F_ASP), s.F(SG_F_APP),
+            s.F(SG_F_B),   s.F(SG_F_PC),  s.ld,          s.p.H,         s.p.NX,
+            s.p.NY};
+}
+
+// assemble_p (src/pressure.cpp:14-54) for interior cell (i, j): links dy^2/APU, dx^2/APV, the
+// wall links zeroed, APP = AE + AW + AN + AS, mass source b, |b| for res_p.
+__global__ void __launch_bounds__(BX* BY)
+    k_assemble_p(PArgs a, const double* APU, const double* APV, const double* u, const double* v,
+                 double dx, double dy, double* cell, const int* stop) {
+    if (stopped(stop))
+        return;
+    const int H = a.H;
+    const int j = H + 1 + blockIdx.x * BX + threadIdx.x;
+    const int i = H + blockIdx.y * BY + threadIdx.y;
+    if (i > a.NX + H - 1 || j > a.NY + H)
+        return;
+    const long ld = a.ld;
+    const size_t q = static_cast<size_t>(i) * ld + j;
+    double ae = dy * dy / APU[q];
+    double aw = dy * dy / APU[q - 1];
+    double an = dx * dx / APV[q];
+    double as = dx * dx / APV[q + ld];
+    if (j == H + 1) aw = 0.0;
+    if (j == a.NY + H) ae = 0.0;
+    if (i == a.NX + H - 1) as = 0.0;
+    if (i == H) an = 0.0;
+    a.AE[q] = ae;
+    a.AW[q] = aw;
+    a.AN[q] = an;
+    a.AS[q] = as;
+    a.AP[q] = ae + aw + an + as;
+    const double bq = -dx * (-v[q + ld] + v[q]) - dy * (-u[q - 1] + u[q]);
+    a.b[q] = bq;
+    cell[static_cast<size_t>(i - H) * a.NY + (j - H - 1)] = fabs(bq);
+}
+
+// The wall-link zeroing of src/pressure.cpp:30-41 also covers halo rows and columns.
+__global__ void k_p_walls(PArgs a, int nr, int nc, const int* stop) {
+    if (stopped(stop))
+        return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const long ld = a.ld;
+    if (k < nr) {
+        a.AW[k * ld + a.H + 1] = 0.0;
+        a.AE[k * ld + a.NY + a.H] = 0.0;
+    }
+    if (k < nc) {
+        a.AS[(a.NX + a.H - 1) * ld + k] = 0.0;
+        a.AN[a.H * ld + k] = 0.0;
+    }
+}
+
+// pc_update (src/pressure.cpp:70-75): p' = b/APP; p' += (E + W + S + N)/APP, neighbours from src.
+__device__ __forceinline__ void pc_update(const PArgs& a, double* dst, const double* src, size_t q) {
+    const long ld = a.ld;
+    double t = a.b[q] / a.AP[q];
+    t += (a.AE[q] * src[q + 1] + a.AW[q] * src[q - 1] + a.AS[q] * src[q + ld] +
+          a.AN[q] * src[q - ld]) /
+         a.AP[q];
+    dst[q] = t;
+}
+
+// Red-black pass, colour (i + j) & 1 (src/pressure.cpp:114-134).
+__global__ void __launch_bounds__(BX* BY) k_pc_rb(PArgs a, int col, const int* stop) {
+    if (stopped(stop))
+        return;
+    const int H = a.H;
+    const int i = H + blockIdx.y * BY + threadIdx.y;
+    if (i > a.NX + H - 1)
+        return;
+    const int j = H + 1 + ((i + H + 1 + col) & 1) + 2 * (blockIdx.x * BX + threadIdx.x);
+    if (j > a.NY + H)
+        return;
+    pc_update(a, a.pc, a.pc, static_cast<size_t>(i) * a.ld + j);
+}
+
+// Damped Jacobi pass, weight 0.8 (src/pressure.cpp:98-104).
+__global__ void __launch_bounds__(BX* BY)
+    k_pc_jacobi(PArgs a, const double* src, double* dst, const int* stop) {
+    if (stopped(stop))
+        return;
+    const int H = a.H;
+    const int j = H + 1 + blockIdx.x * BX + threadIdx.x;
+    const int i = H + blockIdx.y * BY + threadIdx.y;
+    if (i > a.NX + H - 1 || j > a.NY + H)
+        return;
+    const size_t q = static_cast<size_t>(i) * a.ld + j;
+    const double w = 0.8;
+    pc_update(a, dst, src, q);
+    dst[q] = w * dst[q] + (1 - w) * src[q];
+}
+
+// correct (src/pressure.cpp:136-157).
+__global__ void __launch_bounds__(BX* BY)
+    k_correct(PArgs a, const double* APU, const double* APV, double* u, double* v, double* pf,
+              double dx, double dy, double ap, const int* stop) {
+    if (stopped(stop))
+        return;
+    const int H = a.H;
+    const int j = H + 1 + blockIdx.x * BX + threadIdx.x;
+    const int i = H + blockIdx.y * BY + threadIdx.y;
+    if (i > a.NX + H - 1 || j > a.NY + H)
+        return;
+    const long ld = a.ld;
+    const size_t q = static_cast<size_t>(i) * ld + j;
+    const double* pc = a.pc;
+    if (j <= a.NY + H - 1)
+        u[q] += dy * (pc[q] - pc[q + 1]) / APU[q];
+    if (i >= H + 1)
+        v[q] += dx * (pc[q] - pc[q - ld]) / APV[q];
+    pf[q] += ap * pc[q];
+}
+
+__global__ void k_fill(double* a, size_t n, double v, const int* stop) {
+    if (stopped(stop))
+        return;
+    for (size_t k = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; k < n;
+         k += static_cast<size_t>(gridDim.x) * blockDim.x)
+        a[k] = v;
+}
+
+// ---- CG (src/pressure.cpp:208-317); per-cell work on the device, the scalar recurrence on the
+// host from the ordered sums.
+
+struct CgArgs {
+    PArgs a;
+    double *r, *z, *d, *q, *cell;
+    size_t plane; // stride between the 4 value planes of cell
+};
+
+__device__ __forceinline__ size_t cidx(const PArgs& a, int i, int j) {
+    return static_cast<size_t>(i - a.H) * a.NY + (j - a.H - 1);
+}
+
+// r = b - mb, z = r / APP; values r*r, z, r*z, r.
+__global__ void __launch_bounds__(BX* BY) k_cg_init(CgArgs c, const double* sb, double n) {
+    const PArgs& a = c.a;
+    const int H = a.H;
+    const int j = H + 1 + blockIdx.x * BX + threadIdx.x;
+    const int i = H + blockIdx.y * BY + threadIdx.y;
+    if (i > a.NX + H - 1 || j > a.NY + H)
+        return;
+    const size_t q = static_cast<size_t>(i) * a.ld + j, k = cidx(a, i, j);
+    const double mb = *sb / n;
+    const double ri = a.b[q] - mb;
+    c.r[q] = ri;
+    const double zi = ri / a.AP[q];
+    c.z[q] = zi;
+    c.cell[k] = ri * ri;
+    c.cell[k + c.plane] = zi;
+    c.cell[k + 2 * c.plane] = ri * zi;
+    c.cell[k + 3 * c.plane] = ri;
+}
+
+// d = (z - mz) + beta d (first: beta unused, d = z - mz).
+__global__ void __launch_bounds__(BX* BY) k_cg_dir(CgArgs c, double mz, double beta, int first) {
+    const PArgs& a = c.a;
+    const int H = a.H;
+    const int j = H + 1 + blockIdx.x * BX + threadIdx.x;
+    const int i = H + blockIdx.y * BY + threadIdx.y;
+    if (i > a.NX + H - 1 || j > a.NY + H)
+        return;
+    const size_t q = static_cast<size_t>(i) * a.ld + j;
+    c.d[q] = first ? c.z[q] - mz : (c.z[q] - mz) + beta * c.d[q];
+}
+
+// q = A d (apply_A_p, src/pressure.cpp:165-173); value d*q.
+__global__ void __launch_bounds__(BX* BY) k_cg_matvec(CgArgs c) {
+    const PArgs& a = c.a;
+    const int H = a.H;
+    const int j = H + 1 + blockIdx.x * BX + threadIdx.x;
+    const int i = H + blockIdx.y * BY + threadIdx.y;
+    if (i > a.NX + H - 1 || j > a.NY + H)
+        return;
+    const long ld = a.ld;
+    const size_t q = static_cast<size_t>(i) * ld + j;
+    const double* d = c.d;
+    const double qi = a.AP[q] * d[q] - (a.AE[q] * d[q + 1] + a.AW[q] * d[q - 1] +
+                                        a.AS[q] * d[q + ld] + a.AN[q] * d[q - ld]);
+    c.q[q] = qi;
+    c.cell[cidx(a, i, j)] = d[q] * qi;
+}
+
+// x += alpha d; r -= alpha q; z = r / APP; values r*r, z, r*z, r.
+__global__ void __launch_bounds__(BX* BY) k_cg_update(CgArgs c, double alpha) {
+    const PArgs& a = c.a;
+    const int H = a.H;
+    const int j = H + 1 + blockIdx.x * BX + threadIdx.x;
+    const int i = H + blockIdx.y * BY + threadIdx.y;
+    if (i > a.NX + H - 1 || j > a.NY + H)
+        return;
+    const size_t q = static_cast<size_t>(i) * a.ld + j, k = cidx(a, i, j);
+    a.pc[q] += alpha * c.d[q];
+    const double ri = c.r[q] - alpha * c.q[q];
+    c.r[q] = ri;
+    const double zi = ri / a.AP[q];
+    c.z[q] = zi;
+    c.cell[k] = ri * ri;
+    c.cell[k + c.plane] = zi;
+    c.cell[k + 2 * c.plane] = ri * zi;
+    c.cell[k + 3 * c.plane] = ri;
+}
+
+// x -= sx / n.
+__global__ void __launch_bounds__(BX* BY) k_sub_mean(PArgs a, double* x, const double* sum, double n,
+                                                     const int* stop) {
+    if (stopped(stop))
+        return;
+    const int H = a.H;
+    const int j = H + 1 + blockIdx.x * BX + threadIdx.x;
+    const int i = H + blockIdx.y * BY + threadIdx.y;
+    if (i > a.NX + H - 1 || j > a.NY + H)
+        return;
+    const double m = *sum / n;
+    x[static_cast<size_t>(i) * a.ld + j] -= m;
+}
+
+dim3 pgrid(const sg_params& p) { return dim3(cdiv(p.NY, BX), cdiv(p.NX, BY)); }
+
+void solve_cg(Solver& s) {
+    const sg_params& p = s.p;
+    const PArgs a = p_args(s);
+    const size_t plane = static_cast<size_t>(p.NX) * p.NY;
+    CgArgs c{a, s.cg[0], s.cg[1], s.cg[2], s.cg[3], s.cell, plane};
+    const double n = static_cast<double>(p.NX) * p.NY;
+    const int* never = s.never;
+    double* slot = s.scal + S_SUM_A;
+    auto sum_plane = [&](int m, double* dst) {
+        launch_ordered_sum(s, s.cell + m * plane, p.NY, 0, p.NX - 1, 0, p.NY - 1, dst, never);
+    };
+    auto fetch = [&](double* h, int k) {
+        SG_HIP(hipMemcpyAsync(h, slot, sizeof(double) * k, hipMemcpyDeviceToHost, s.stream));
+        SG_HIP(hipStreamSynchronize(s.stream));
+    };
+    s.cg_last = 0;
+    launch_fill(s, a.pc, s.n, 0.0, never);
+    // sb = sum b, r = b - mb, z = r / APP.
+    launch_ordered_sum(s, a.b, s.ld, p.H, p.NX + p.H - 1, p.H + 1, p.NY + p.H, slot, never);
+    hipLaunchKernelGGL(k_cg_init, pgrid(p), dim3(BX, BY), 0, s.stream, c, slot, n);
+    SG_HIP(hipGetLastError());
+    for (int m = 0; m < 4; ++m)
+        sum_plane(m, slot + m);
+    double s4[4];
+    fetch(s4, 4);
+    double rr = s4[0], sz = s4[1], rz = s4[2], sr = s4[3];
+    if (rr == 0.0)
+        return;
+    const double nb = std::sqrt(rr);
+    double mz = sz / n;
+    rz -= mz * sr;
+    hipLaunchKernelGGL(k_cg_dir, pgrid(p), dim3(BX, BY), 0, s.stream, c, mz, 0.0, 1);
+    SG_HIP(hipGetLastError());
+    int k = 0;
+    while (k < p.pressure_maxit) {
+        hipLaunchKernelGGL(k_cg_matvec, pgrid(p), dim3(BX, BY), 0, s.stream, c);
+        SG_HIP(hipGetLastError());
+        sum_plane(0, slot);
+        double dq;
+        fetch(&dq, 1);
+        if (!(dq > 0.0))
+            break;
+        const double alpha = rz / dq;
+        hipLaunchKernelGGL(k_cg_update, pgrid(p), dim3(BX, BY), 0, s.stream, c, alpha);
+        SG_HIP(hipGetLastError());
+        for (int m = 0; m < 4; ++m)
+            sum_plane(m, slot + m);
+        fetch(s4, 4);
+        rr = s4[0];
+        sz = s4[1];
+        double rzn = s4[2];
+        sr = s4[3];
+        ++k;
+        const double rel = std::sqrt(rr) / nb;
+        if (rel <= p.pressure_tolerance)
+            break;
+        mz = sz / n;
+        rzn -= mz * sr;
+        const double beta = rzn / rz;
+        rz = rzn;
+        hipLaunchKernelGGL(k_cg_dir, pgrid(p), dim3(BX, BY), 0, s.stream, c, mz, beta, 0);
+        SG_HIP(hipGetLastError());
+    }
+    launch_ordered_sum(s, a.pc, s.ld, p.H, p.NX + p.H - 1, p.H + 1, p.NY + p.H, slot, never);
+    hipLaunchKernelGGL(k_sub_mean, pgrid(p), dim3(BX, BY), 0, s.stream, a, a.pc, slot, n, never);
+    SG_HIP(hipGetLastError());
+    s.cg_last = k;
+}
+
+} // namespace
+
+void launch_fill(Solver& s, double* a, size_t n, double v, const int* stop) {
+    hipLaunchKernelGGL(k_fill, dim3(std::min<size_t>(2048, (n + 255) / 256 + 1)), dim3(256), 0,
+                       s.stream, a, n, v, stop);
+    SG_HIP(hipGetLastError());
+}
+
+void launch_assemble_p(Solver& s, double* slot, const int* stop) {
+    const sg_params& p = s.p;
+    const PArgs a = p_args(s);
+    hipLaunchKernelGGL(k_p_walls, dim3(cdiv(std::max(s.nr, s.nc), 256)), dim3(256), 0, s.stream,
+                       a, s.nr, s.nc, stop);
+    SG_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_assemble_p, pgrid(p), dim3(BX, BY), 0, s.stream, a, s.F(SG_F_CU_AP),
+                       s.F(SG_F_CV_AP), s.F(SG_F_U), s.F(SG_F_V), p.dx, p.dy, s.cell, stop);
+    SG_HIP(hipGetLastError());
+    launch_ordered_sum(s, s.cell, p.NY, 0, p.NX - 1, 0, p.NY - 1, slot, stop);
+}
+
+void launch_correct(Solver& s, const int* stop) {
+    const sg_params& p = s.p;
+    hipLaunchKernelGGL(k_correct, pgrid(p), dim3(BX, BY), 0, s.stream, p_args(s),
+                       s.F(SG_F_CU_AP), s.F(SG_F_CV_AP), s.F(SG_F_U), s.F(SG_F_V), s.F(SG_F_P),
+                       p.dx, p.dy, p.relaxation_factor_p, stop);
+    SG_HIP(hipGetLastError());
+}
+
+void launch_remove_mean0(Solver& s, double* x, const int* stop) {
+    const sg_params& p = s.p;
+    double* slot = s.scal + S_SUM_D;
+    launch_ordered_sum(s, x, s.ld, p.H, p.NX + p.H - 1, p.H + 1, p.NY + p.H, slot, stop);
+    hipLaunchKernelGGL(k_sub_mean, pgrid(p), dim3(BX, BY), 0, s.stream, p_args(s), x, slot,
+                       static_cast<double>(p.NX) * p.NY, stop);
+    SG_HIP(hipGetLastError());
+}
+
+// solve_p for every GPU pressure solver. CG needs the host between iterations (eager only).
+void launch_solve_p(Solver& s, const int* stop, bool eager) {
+    const sg_params& p = s.p;
+    const PArgs a = p_args(s);
+    switch (p.pressure_solver) {
+    case SG_P_GS_REDBLACK:
+        launch_fill(s, a.pc, s.n, 0.0, stop);
+        for (int k = 0; k < p.maxit_inner_p; ++k)
+            for (int c = 0; c < 2; ++c) {
+                hipLaunchKernelGGL(k_pc_rb, dim3(cdiv(cdiv(p.NY, 2), BX), cdiv(p.NX, BY)),
+                                   dim3(BX, BY), 0, s.stream, a, c, stop);
+                SG_HIP(hipGetLastError());
+            }
+        return;
+    case SG_P_JACOBI: {
+        const int n = p.maxit_inner_p;
+        launch_fill(s, a.pc, s.n, 0.0, stop);
+        launch_fill(s, s.scratch, s.n, 0.0, stop);
+        for (int k = 0; k < n; ++k) {
+            const bool to_pc = (n - k) % 2 == 1;
+            const double* src = to_pc ? s.scratch : a.pc;
+            double* dst = to_pc ? a.pc : s.scratch;
+            hipLaunchKernelGGL(k_pc_jacobi, pgrid(p), dim3(BX, BY), 0, s.stream, a, src, dst,
+                               stop);
+            SG_HIP(hipGetLastError());
+        }
+        return;
+    }
+    case SG_P_CG:
+        if (!eager)
+            throw ArgError("pressure_solver = cg cannot be captured in a graph");
+        solve_cg(s);
+        s.cg_total += s.cg_last;
+        return;
+    case SG_P_MG:
+        launch_mg_solve(s, stop);
+        return;
+    default:
+        throw ArgError("backend = gpu does not run pressure_solver = gs_lex (a serial sweep)");
+    }
+}
+
+} // namespace sg

@@ -1,0 +1,247 @@
+"""ctypes binding of the C ABI in include/simple_gpu.h (libsimple_gpu.so) and of the host library
+(libsimple_host.so: config parsing, the C++ outer loop and output writers).
+
+The binding is plumbing for tests and bench.py; the product is the C ABI itself. Loading fails
+loudly if the built library is missing: there is no fallback path.
+"""
+import ctypes as C
+import os
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+BUILD = os.path.join(ROOT, "build")
+
+UDS, QUICK = 0, 1
+NORM_ABSOLUTE, NORM_RUNNING_MAX = 0, 1
+BACKEND_SERIAL, BACKEND_OPENMP, BACKEND_GPU = 0, 1, 2
+MOM_GS_LEX, MOM_JACOBI, MOM_GS_MULTICOLOUR = 0, 1, 2
+P_GS_LEX, P_JACOBI, P_GS_REDBLACK, P_CG, P_MG = 0, 1, 2, 3, 4
+MG_FP64, MG_MIXED = 0, 1
+WARM_NONE, WARM_LEARNED = 0, 1
+
+FIELDS = ["u", "v", "p",
+          "cu_AE", "cu_AW", "cu_AN", "cu_AS", "cu_AEE", "cu_AWW", "cu_ANN", "cu_ASS", "cu_AP",
+          "cv_AE", "cv_AW", "cv_AN", "cv_AS", "cv_AEE", "cv_AWW", "cv_ANN", "cv_ASS", "cv_AP",
+          "AEP", "AWP", "ANP", "ASP", "APP", "b", "pc"]
+FIELD_ID = {n: k for k, n in enumerate(FIELDS)}
+NPHASES = 11
+PHASES = ["assemble_u", "bc_u", "residual_u", "sweep_u", "assemble_v", "bc_v", "residual_v",
+          "sweep_v", "assemble_p", "solve_p", "correct"]
+K_SWEEP_U, K_SWEEP_V, K_MG_SMOOTH0, K_ASSEMBLE_U = 0, 1, 2, 3
+
+
+class SgParams(C.Structure):
+    """Mirror of struct sg_params (include/simple_gpu.h)."""
+    _fields_ = [
+        ("NX", C.c_int), ("NY", C.c_int), ("H", C.c_int),
+        ("N_halo_x", C.c_int), ("N_halo_y", C.c_int),
+        ("Re", C.c_double),
+        ("maxit_inner_u", C.c_int), ("maxit_inner_v", C.c_int), ("maxit_inner_p", C.c_int),
+        ("error_tolerance", C.c_double),
+        ("maxit", C.c_int),
+        ("relaxation_factor_u", C.c_double), ("relaxation_factor_v", C.c_double),
+        ("relaxation_factor_p", C.c_double),
+        ("convection_scheme", C.c_int), ("diffusion_order", C.c_int),
+        ("Hybrid_switch", C.c_int), ("residual_normalisation", C.c_int),
+        ("backend", C.c_int), ("momentum_solver", C.c_int), ("pressure_solver", C.c_int),
+        ("pressure_tolerance", C.c_double), ("pressure_maxit", C.c_int),
+        ("mg_cycles", C.c_int), ("mg_pre", C.c_int), ("mg_post", C.c_int),
+        ("mg_omega", C.c_double),
+        ("write_fields", C.c_int),
+        ("nrows", C.c_int), ("ncols", C.c_int),
+        ("dx", C.c_double), ("dy", C.c_double), ("nu", C.c_double),
+        ("mg_precision", C.c_int), ("mg_coarsest", C.c_int), ("mg_dc_maxit", C.c_int),
+        ("mg_dc_tol", C.c_double),
+        ("pressure_warm_start", C.c_int), ("check_every", C.c_int),
+    ]
+
+    def copy(self):
+        q = SgParams()
+        C.memmove(C.byref(q), C.byref(self), C.sizeof(SgParams))
+        return q
+
+
+_host = None
+_gpu = None
+
+
+def _load(name):
+    path = os.path.join(BUILD, name)
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built: run __graft_entry__.build() first")
+    return C.CDLL(path)
+
+
+def host():
+    """libsimple_host.so: config -> params, outputs (pure C++, loads without a GPU)."""
+    global _host
+    if _host is None:
+        h = _load("libsimple_host.so")
+        h.sh_params_from_config.argtypes = [C.c_char_p, C.POINTER(SgParams), C.c_char_p,
+                                            C.c_int, C.c_char_p, C.c_int]
+        h.sh_params_from_config.restype = C.c_int
+        _host = h
+    return _host
+
+
+def params_from_config(text):
+    """Parses config text with the host's parser (same keys and errors as the reference's
+    make_params). Returns (SgParams, output_dir); raises ValueError with the parser message."""
+    h = host()
+    p = SgParams()
+    out = C.create_string_buffer(4096)
+    err = C.create_string_buffer(4096)
+    rc = h.sh_params_from_config(text.encode(), C.byref(p), out, 4096, err, 4096)
+    if rc != 0:
+        raise ValueError(err.value.decode())
+    return p, out.value.decode()
+
+
+def params(**kv):
+    """Params from key = value overrides of the defaults (values as config strings)."""
+    text = "".join(f"{k} = {v}\n" for k, v in kv.items())
+    return params_from_config(text)[0]
+
+
+def gpu():
+    """libsimple_gpu.so, the C ABI. Raises if it is not built."""
+    global _gpu
+    if _gpu is None:
+        g = _load("libsimple_gpu.so")
+        P = C.POINTER(SgParams)
+        vp = C.c_void_p
+        dp = C.POINTER(C.c_double)
+        g.sg_last_error.restype = C.c_char_p
+        g.sg_abi_version.restype = C.c_int
+        g.sg_device_count.argtypes = [C.POINTER(C.c_int)]
+        g.sg_device_name.argtypes = [C.c_char_p, C.c_size_t]
+        g.sg_params_finalize.argtypes = [P]
+        g.sg_params_default.argtypes = [P]
+        g.sg_params_default.restype = None
+        g.sg_create.argtypes = [P, dp, dp, dp, C.POINTER(vp)]
+        g.sg_destroy.argtypes = [vp]
+        g.sg_upload.argtypes = [vp, C.c_int, dp]
+        g.sg_download.argtypes = [vp, C.c_int, dp]
+        for n in ["sg_assemble_u", "sg_bc_u", "sg_assemble_v", "sg_bc_v", "sg_solve_p",
+                  "sg_correct"]:
+            getattr(g, n).argtypes = [vp]
+        for n in ["sg_residual_u", "sg_residual_v", "sg_assemble_p"]:
+            getattr(g, n).argtypes = [vp, dp]
+        for n in ["sg_sweep_u", "sg_sweep_v"]:
+            getattr(g, n).argtypes = [vp, C.c_int]
+        g.sg_outer_iteration.argtypes = [vp, dp, dp]
+        g.sg_advance.argtypes = [vp, C.c_int, C.POINTER(C.c_int), dp, C.POINTER(C.c_int)]
+        g.sg_iterations.argtypes = [vp, C.POINTER(C.c_int)]
+        g.sg_kernel_probe.argtypes = [vp, C.c_int, C.c_int, dp, dp, C.POINTER(C.c_int)]
+        g.sg_mg_levels.argtypes = [vp, C.POINTER(C.c_int)]
+        _gpu = g
+    return _gpu
+
+
+class GpuError(RuntimeError):
+    pass
+
+
+def check(rc):
+    if rc != 0:
+        raise GpuError(gpu().sg_last_error().decode())
+
+
+def _dptr(a):
+    import numpy as np
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Solver:
+    """Thin owner of an sg_solver handle."""
+
+    def __init__(self, p, u=None, v=None, pf=None):
+        g = gpu()
+        self.p = p.copy()
+        h = C.c_void_p()
+        args = [(_dptr(a) if a is not None else None) for a in (u, v, pf)]
+        check(g.sg_create(C.byref(self.p), args[0], args[1], args[2], C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            gpu().sg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def shape(self):
+        return (self.p.nrows, self.p.ncols)
+
+    def get(self, name):
+        import numpy as np
+        a = np.empty(self.shape(), dtype=np.float64)
+        check(gpu().sg_download(self.h, FIELD_ID[name], _dptr(a)))
+        return a
+
+    def set(self, name, a):
+        import numpy as np
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.shape == self.shape()
+        check(gpu().sg_upload(self.h, FIELD_ID[name], _dptr(a)))
+
+    def _res(self, fn):
+        r = C.c_double()
+        check(fn(self.h, C.byref(r)))
+        return r.value
+
+    def assemble_u(self): check(gpu().sg_assemble_u(self.h))
+    def bc_u(self): check(gpu().sg_bc_u(self.h))
+    def residual_u(self): return self._res(gpu().sg_residual_u)
+    def sweep_u(self, n): check(gpu().sg_sweep_u(self.h, n))
+    def assemble_v(self): check(gpu().sg_assemble_v(self.h))
+    def bc_v(self): check(gpu().sg_bc_v(self.h))
+    def residual_v(self): return self._res(gpu().sg_residual_v)
+    def sweep_v(self, n): check(gpu().sg_sweep_v(self.h, n))
+    def assemble_p(self): return self._res(gpu().sg_assemble_p)
+    def solve_p(self): check(gpu().sg_solve_p(self.h))
+    def correct(self): check(gpu().sg_correct(self.h))
+
+    def outer_iteration(self, phase_s=None):
+        import numpy as np
+        r = np.zeros(3)
+        ph = None if phase_s is None else _dptr(phase_s)
+        check(gpu().sg_outer_iteration(self.h, _dptr(r), ph))
+        return r
+
+    def advance(self, n):
+        """Up to n iterations by graph replay with the device-side stop rule.
+        Returns (iterations done, log[done, 3], converged)."""
+        import numpy as np
+        log = np.zeros((n, 3))
+        done = C.c_int()
+        conv = C.c_int()
+        check(gpu().sg_advance(self.h, n, C.byref(done), _dptr(log), C.byref(conv)))
+        return done.value, log[:done.value].copy(), bool(conv.value)
+
+    def iterations(self):
+        k = C.c_int()
+        check(gpu().sg_iterations(self.h, C.byref(k)))
+        return k.value
+
+    def probe(self, kernel, reps=20):
+        t = C.c_double()
+        b = C.c_double()
+        n = C.c_int()
+        check(gpu().sg_kernel_probe(self.h, kernel, reps, C.byref(t), C.byref(b), C.byref(n)))
+        return t.value, b.value, n.value
+
+    def mg_levels(self):
+        n = C.c_int()
+        check(gpu().sg_mg_levels(self.h, C.byref(n)))
+        return n.value
+
+
+def device_count():
+    n = C.c_int(0)
+    rc = gpu().sg_device_count(C.byref(n))
+    return n.value if rc == 0 else 0
