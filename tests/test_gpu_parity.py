@@ -1,0 +1,157 @@
+"""GPU parity: the HIP backend against the CPU oracle, through the C ABI.
+
+Same shape as the reference's GPU test (tests/hip/test_hip.cpp:70-183): non-trivial starting
+fields from 3 CPU outer iterations on the 32x40 snapshot case, then each phase on the CPU and the
+GPU in turn, every array compared bit for bit; residuals compared to 1e-12 relative (they are in
+fact summed in the CPU's order, so they come out identical); whole runs compared bit for bit;
+and run-to-run determinism.
+"""
+import numpy as np
+import pytest
+
+from conftest import bits_equal, first_diff, golden_cfg, with_keys
+
+pytestmark = pytest.mark.gpu
+
+import pyoracle as po  # noqa: E402
+import simple_gpu as sg  # noqa: E402
+
+
+def _cfg(scheme, order, mom, pres, maxit, **extra):
+    return with_keys(golden_cfg("converged_32x40"), convection_scheme=scheme,
+                     diffusion_order=order, momentum_solver=mom, pressure_solver=pres,
+                     maxit=maxit, error_tolerance=0, backend="gpu", **extra)
+
+
+def _same(name, g, c):
+    assert bits_equal(g, c), f"{name}: {first_diff(g, c)}"
+
+
+def _rel_close(a, b, tol=1e-12):
+    return abs(a - b) <= tol * max(abs(b), 1e-300)
+
+
+COEFS = ["AE", "AW", "AN", "AS", "AEE", "AWW", "ANN", "ASS", "AP"]
+
+
+def _per_phase(scheme, order, mom, pres, **extra):
+    p, _ = sg.params_from_config(_cfg(scheme, order, mom, pres, 3, **extra))
+    cpu = po.Oracle(p)
+    for _ in range(3):
+        cpu.outer_iteration()
+    gpu = sg.Solver(p)
+    for name in sg.FIELDS:
+        gpu.set(name, cpu.get(name))
+    print(f"per-phase {scheme}/{order} {mom}/{pres}", flush=True)
+
+    cpu.assemble_u(); gpu.assemble_u()
+    for c in COEFS:
+        _same("assemble_u " + c, gpu.get("cu_" + c), cpu.get("cu_" + c))
+    cpu.bc_u(); gpu.bc_u()
+    _same("bc_u", gpu.get("u"), cpu.get("u"))
+    rc, rg = cpu.residual_u(), gpu.residual_u()
+    assert _rel_close(rg, rc), (rg, rc)
+    cpu.sweep_u(p.maxit_inner_u); gpu.sweep_u(p.maxit_inner_u)
+    _same("sweep_u", gpu.get("u"), cpu.get("u"))
+
+    cpu.assemble_v(); gpu.assemble_v()
+    for c in COEFS:
+        _same("assemble_v " + c, gpu.get("cv_" + c), cpu.get("cv_" + c))
+    cpu.bc_v(); gpu.bc_v()
+    _same("bc_v", gpu.get("v"), cpu.get("v"))
+    rc, rg = cpu.residual_v(), gpu.residual_v()
+    assert _rel_close(rg, rc), (rg, rc)
+    cpu.sweep_v(p.maxit_inner_v); gpu.sweep_v(p.maxit_inner_v)
+    _same("sweep_v", gpu.get("v"), cpu.get("v"))
+
+    rc, rg = cpu.assemble_p(), gpu.assemble_p()
+    assert _rel_close(rg, rc), (rg, rc)
+    for n in ["AEP", "AWP", "ANP", "ASP", "APP", "b"]:
+        _same("assemble_p " + n, gpu.get(n), cpu.get(n))
+    cpu.solve_p(); gpu.solve_p()
+    _same("solve_p pc", gpu.get("pc"), cpu.get("pc"))
+    _same("solve_p b", gpu.get("b"), cpu.get("b"))  # mg removes the mean of b in place
+    cpu.correct(); gpu.correct()
+    for n in ["u", "v", "p"]:
+        _same("correct " + n, gpu.get(n), cpu.get(n))
+    gpu.close()
+
+
+@pytest.mark.parametrize("scheme,order", [("QUICK", 4), ("UDS", 2), ("QUICK", 2), ("UDS", 4)])
+def test_per_phase_multicolour_redblack(scheme, order):
+    _per_phase(scheme, order, "gs_multicolour", "gs_redblack")
+
+
+@pytest.mark.parametrize("pres", ["mg", "cg", "jacobi"])
+def test_per_phase_pressure_solvers(pres):
+    _per_phase("QUICK", 4, "gs_multicolour", pres)
+
+
+def test_per_phase_jacobi_momentum():
+    _per_phase("QUICK", 4, "jacobi", "gs_redblack")
+
+
+@pytest.mark.parametrize("pres", ["gs_redblack", "mg", "cg"])
+def test_fixed_work_run_bit_identical(pres):
+    """200 outer iterations (graph replays) give the oracle's fields and residual log."""
+    n = 200
+    p, _ = sg.params_from_config(_cfg("QUICK", 4, "gs_multicolour", pres, n))
+    cpu = po.Oracle(p)
+    nc, logc, convc = cpu.run()
+    gpu = sg.Solver(p)
+    done, logg, convg = gpu.advance(n)
+    print(f"fixed work {pres}: {done} iterations", flush=True)
+    assert done == n and not convg
+    assert bits_equal(logg, logc[:, 1:]), first_diff(logg, logc[:, 1:])
+    for f in ["u", "v", "p"]:
+        _same(f, gpu.get(f), cpu.get(f))
+    gpu.close()
+
+
+def test_runs_are_deterministic():
+    p, _ = sg.params_from_config(with_keys(golden_cfg("short_16"), maxit=50,
+                                           momentum_solver="gs_multicolour",
+                                           pressure_solver="mg", backend="gpu"))
+    a, b = sg.Solver(p), sg.Solver(p)
+    _, la, _ = a.advance(50)
+    _, lb, _ = b.advance(50)
+    assert bits_equal(la, lb)
+    _same("u", a.get("u"), b.get("u"))
+
+
+def test_stop_rule_matches_oracle_to_convergence():
+    """Config 1 (32x32, Re 100, QUICK/2, multicolour + red-black, tol 1e-5) to tolerance: the
+    device-side stop rule stops at the oracle's iteration with the oracle's fields."""
+    cfg = ("N_cells_x = 32\nN_cells_y = 32\nRe = 100\nconvection_scheme = QUICK\n"
+           "diffusion_order = 2\nmomentum_solver = gs_multicolour\n"
+           "pressure_solver = gs_redblack\nerror_tolerance = 1e-5\nmaxit = 20000\nbackend = gpu\n")
+    p, _ = sg.params_from_config(cfg)
+    cpu = po.Oracle(p)
+    nc, logc, convc = cpu.run()
+    gpu = sg.Solver(p)
+    total, logs, conv = 0, [], False
+    while total < p.maxit and not conv:
+        k, lg, conv = gpu.advance(500)
+        total += k
+        logs.append(lg)
+    print(f"config-1: oracle {nc} iterations, gpu {total}", flush=True)
+    assert convc and conv and total == nc
+    assert bits_equal(np.concatenate(logs), logc[:, 1:])
+    for f in ["u", "v", "p"]:
+        _same(f, gpu.get(f), cpu.get(f))
+
+
+def test_running_max_normalisation_stop():
+    cfg = with_keys(golden_cfg("converged_32x40"), momentum_solver="gs_multicolour",
+                    pressure_solver="mg", maxit=3000, error_tolerance=1e-3, backend="gpu")
+    p, _ = sg.params_from_config(cfg)
+    cpu = po.Oracle(p)
+    nc, logc, convc = cpu.run()
+    gpu = sg.Solver(p)
+    r = gpu.outer_iteration()  # the host loop's first, eager iteration
+    total, conv = 1, False
+    while total < p.maxit and not conv:
+        k, lg, conv = gpu.advance(400)
+        total += k
+    assert convc and conv and total == nc, (nc, total)
+    _same("u", gpu.get("u"), cpu.get("u"))
