@@ -725,6 +725,13 @@ or_state* or_create(const sg_params* p) {
 }
 
 void or_destroy(or_state* s) { delete s; }
+// Replace the scalar parameters (sizes must not change): tests vary tolerances and counts.
+int or_set_params(or_state* s, const sg_params* p) {
+    if (p->NX != s->p.NX || p->NY != s->p.NY || p->H != s->p.H || p->mg_coarsest != s->p.mg_coarsest)
+        return 1;
+    s->p = *p;
+    return 0;
+}
 double* or_field(or_state* s, int id) { return (id >= 0 && id < SG_F_COUNT) ? s->a(id) : nullptr; }
 int or_nrows(or_state* s) { return s->nr; }
 int or_ncols(or_state* s) { return s->nc; }
@@ -938,6 +945,17 @@ double or_run_fixed(or_state* s, int n, double* last) {
             last[k] = r[k];
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
+
+// Per-cell coefficients of src/coefficients.cpp: which = 0 convective, 1 diffusive, 2 sum.
+void or_coefficients(const sg_params* p, double ue, double uw, double vn, double vs, int i, int j,
+                     int which, double out[9]) {
+    const C9 c = convective(*p, ue, uw, vn, vs, i, j), d = diffusive(*p, i, j);
+    const double cv[9] = {c.e, c.w, c.n, c.s, c.ee, c.ww, c.nn, c.ss, c.p};
+    const double dv[9] = {d.e, d.w, d.n, d.s, d.ee, d.ww, d.nn, d.ss, d.p};
+    for (int k = 0; k < 9; ++k)
+        out[k] = which == 0 ? cv[k] : which == 1 ? dv[k] : cv[k] + dv[k];
+}
+int or_in_band(const sg_params* p, int i, int j) { return band(*p, i, j) ? 1 : 0; }
 
 int or_mg_nlevels(or_state* s) { return static_cast<int>(s->mg.size()); }
 void or_mg_dims(or_state* s, int l, int* nx, int* ny) {

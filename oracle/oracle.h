@@ -27,6 +27,8 @@ typedef struct or_state or_state;
 
 or_state* or_create(const sg_params* p);
 void or_destroy(or_state* s);
+/* Replace the scalar parameters of a state (same grid); 0 on success. */
+int or_set_params(or_state* s, const sg_params* p);
 /* Pointer to the host array of a field (nrows x ncols doubles), SG_F_* ids. */
 double* or_field(or_state* s, int field_id);
 int or_nrows(or_state* s);
@@ -60,6 +62,12 @@ void or_solve_p_kind(or_state* s, int kind);              /* SG_P_* */
 int or_cg_last_iterations(or_state* s);
 double or_cg_last_rel(or_state* s);
 long long or_cg_total_iterations(or_state* s);
+
+/* Coefficients of one control volume: which = 0 convective, 1 diffusive, 2 their sum;
+ * out = AE, AW, AN, AS, AEE, AWW, ANN, ASS, AP. */
+void or_coefficients(const sg_params* p, double ue, double uw, double vn, double vs, int i, int j,
+                     int which, double out[9]);
+int or_in_band(const sg_params* p, int i, int j);
 
 /* Multigrid internals. */
 int or_mg_nlevels(or_state* s);

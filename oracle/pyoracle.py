@@ -27,6 +27,7 @@ def lib():
         L.or_create.restype = vp
         L.or_create.argtypes = [C.c_void_p]
         L.or_destroy.argtypes = [vp]
+        L.or_set_params.argtypes = [vp, C.c_void_p]
         L.or_field.restype = dp
         L.or_field.argtypes = [vp, C.c_int]
         for n in ["or_assemble_u", "or_bc_u", "or_assemble_v", "or_bc_v", "or_solve_p",
@@ -64,8 +65,24 @@ def lib():
         L.or_mg_last_coarse_sweeps.argtypes = [vp]
         L.or_apply_A_p.argtypes = [vp, dp, dp]
         L.or_apply_A_p.restype = None
+        L.or_coefficients.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double,
+                                      C.c_double, C.c_int, C.c_int, C.c_int, dp]
+        L.or_coefficients.restype = None
+        L.or_in_band.argtypes = [C.c_void_p, C.c_int, C.c_int]
         _lib = L
     return _lib
+
+
+def coefficients(params, ue, uw, vn, vs, i, j, which):
+    """Coefficients [AE, AW, AN, AS, AEE, AWW, ANN, ASS, AP]: which 0 convective, 1 diffusive,
+    2 sum."""
+    out = np.zeros(9)
+    lib().or_coefficients(C.byref(params), ue, uw, vn, vs, i, j, which, _dp(out))
+    return out
+
+
+def in_band(params, i, j):
+    return bool(lib().or_in_band(C.byref(params), i, j))
 
 
 FIELDS = ["u", "v", "p",
@@ -95,6 +112,10 @@ class Oracle:
                 self.h = None
         except Exception:
             pass
+
+    def set_params(self, params):
+        self.p = params.copy()
+        assert self.L.or_set_params(self.h, C.byref(self.p)) == 0
 
     def view(self, name):
         """Writable numpy view of a field (no copy)."""

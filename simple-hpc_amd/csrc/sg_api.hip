@@ -354,6 +354,10 @@ int sg_create(const sg_params* pin, const double* u, const double* v, const doub
         if (p.pressure_solver == SG_P_CG)
             for (auto& c : s.cg)
                 c = dalloc<double>(s, s.n);
+        s.pku = make_pack(p, true);
+        s.pkv = make_pack(p, false);
+        s.pku.base = dalloc<double>(s, 10 * s.pku.astride);
+        s.pkv.base = dalloc<double>(s, 10 * s.pkv.astride);
         ensure_log(s, 256);
         diffusive(p, s.diff2, s.diff4);
         if (p.pressure_solver == SG_P_MG)
@@ -409,6 +413,7 @@ int sg_upload(sg_solver* h, int id, const double* host) {
             throw ArgError("bad field id or null pointer");
         SG_HIP(hipMemcpy2DAsync(s.f[id], sizeof(double) * s.ld, host, sizeof(double) * s.nc,
                                 sizeof(double) * s.nc, s.nr, hipMemcpyHostToDevice, s.stream));
+        sync_packed(s, id, true);
         sync(s);
     });
 }
@@ -418,6 +423,7 @@ int sg_download(sg_solver* h, int id, double* host) {
         Solver& s = S(h);
         if (id < 0 || id >= SG_F_COUNT || !host)
             throw ArgError("bad field id or null pointer");
+        sync_packed(s, id, false);
         SG_HIP(hipMemcpy2DAsync(host, sizeof(double) * s.nc, s.f[id], sizeof(double) * s.ld,
                                 sizeof(double) * s.nc, s.nr, hipMemcpyDeviceToHost, s.stream));
         sync(s);

@@ -54,6 +54,16 @@ struct MgLevel {
     float *fb = nullptr, *fx = nullptr, *fr = nullptr; // FP32 copies (mixed precision)
 };
 
+// Colour-packed layout of one momentum component's coefficients (sg_momentum.hip): 10 arrays
+// (AE, AW, AN, AS, AEE, AWW, ANN, ASS, AP, pressure term), each ncol x rows x kp doubles.
+struct Pack {
+    int ncol;           // colours: 5, or 2 for UDS with 2nd-order diffusion
+    int ai0, ai1, aj0, aj1; // assembly range (inclusive)
+    int rows, kp;       // rows of the range, cells per colour per row (padded to 16)
+    size_t astride;     // doubles per array
+    double* base;
+};
+
 // Device control block of the outer loop.
 struct Ctl {
     int stop;       // nonzero: skip all work (set by the stop rule)
@@ -90,6 +100,7 @@ struct Solver {
     std::vector<MgLevel> mg;
     double* mg_r0 = nullptr;           // level-0 residual
     std::vector<void*> owned;          // allocations to free
+    Pack pku{}, pkv{};                 // packed momentum coefficients
     C9 diff2{}, diff4{};               // diffusive coefficients: order 2 (and band), order 4
     hipGraphExec_t graph = nullptr;
     int iters = 0;                     // host copy of completed iterations
@@ -110,6 +121,9 @@ void launch_bc(Solver& s, bool is_u, const int* stop);
 void launch_mom_residual(Solver& s, bool is_u, double* slot, const int* stop);
 void launch_mom_sweeps(Solver& s, bool is_u, int nsweeps, const int* stop);
 void launch_mom_colour_pass(Solver& s, bool is_u, int colour, const int* stop);
+int ncolours(const sg_params& p);
+Pack make_pack(const sg_params& p, bool is_u);
+void sync_packed(Solver& s, int field_id, bool to_packed);
 // pressure (sg_pressure.hip)
 void launch_assemble_p(Solver& s, double* slot, const int* stop);
 void launch_solve_p(Solver& s, const int* stop, bool eager);

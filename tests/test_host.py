@@ -1,0 +1,191 @@
+"""Host side (CPU): config parser, parameters, output files, CLI error paths and the C ABI's
+symbol table. Restates tests/unit/test_config.cpp, test_params.cpp and the formats of
+src/io.cpp; the output files are compared byte for byte with the reference's snapshot CSVs.
+"""
+import ctypes as C
+import json
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT, golden_cfg
+
+import pyoracle as po
+import simple_gpu as sg
+
+BUILD = os.path.join(ROOT, "build")
+
+
+def perr(text):
+    with pytest.raises(ValueError) as e:
+        sg.params_from_config(text)
+    return str(e.value)
+
+
+def test_parser_comments_whitespace_and_errors():
+    p, out = sg.params_from_config("# comment\n\n  N_cells_x = 12   # trailing\nRe=50\n")
+    assert p.NX == 12 and p.Re == 50.0
+    assert perr("N_cells_x = 1\nbad line\n") == "config line 2: malformed line (expected key = value)"
+    assert perr("N_cells_x =\n") == "config line 1: malformed line (empty key or value)"
+    assert perr("Re = 1\nRe = 2\n") == "config line 2: duplicate key 'Re'"
+
+
+def test_defaults_match_reference():
+    p, out = sg.params_from_config("")
+    assert (p.NX, p.NY, p.H, p.Re) == (80, 80, 2, 1000.0)
+    assert (p.maxit_inner_u, p.maxit_inner_v, p.maxit_inner_p) == (4, 4, 8)
+    assert p.error_tolerance == 1e-4 and p.maxit == 15000
+    assert (p.relaxation_factor_u, p.relaxation_factor_v, p.relaxation_factor_p) == (0.3, 0.3, 0.2)
+    assert p.convection_scheme == sg.QUICK and p.diffusion_order == 4
+    assert p.momentum_solver == sg.MOM_GS_LEX and p.pressure_solver == sg.P_GS_LEX
+    assert p.pressure_tolerance == 1e-6 and p.pressure_maxit == 1000
+    assert (p.mg_cycles, p.mg_pre, p.mg_post, p.mg_omega) == (1, 2, 2, 1.8)
+    assert out == "results/run"
+    assert (p.nrows, p.ncols, p.dx, p.dy, p.nu) == (85, 85, 1 / 80, 1 / 80, 1 / 1000)
+
+
+def test_derived_non_square():
+    p, _ = sg.params_from_config("N_cells_x = 32\nN_cells_y = 40\n")
+    assert (p.nrows, p.ncols) == (37, 45)
+    assert p.dx == 1.0 / 40 and p.dy == 1.0 / 32
+
+
+def test_key_and_value_errors():
+    assert perr("foo = 1\n") == "unknown config key: foo"
+    assert perr("N_cells_x = 1.5\n") == "cannot parse integer for 'N_cells_x': 1.5"
+    assert perr("Re = abc\n") == "cannot parse number for 'Re': abc"
+    assert perr("Hybrid_switch = yes\n") == \
+        "invalid boolean for 'Hybrid_switch' (use True/False): yes"
+    assert perr("convection_scheme = CDS\n") == "unknown value for 'convection_scheme': CDS"
+    assert perr("pressure_solver = sor\n") == "unknown value for 'pressure_solver': sor"
+    assert perr("N_halo_x = 3\n") == "N_halo_x must equal N_halo_y"
+    assert perr("N_halo_x = 1\nN_halo_y = 1\n") == "halo width H must be >= 2"
+    assert perr("diffusion_order = 3\n") == "diffusion_order must be 2 or 4"
+    assert perr("N_cells_x = 0\n") == "N_cells_x and N_cells_y must be positive"
+    assert perr("Hybrid_switch = True\n") == "Hybrid_switch = True is not supported"
+    assert perr("pressure_tolerance = 0\n") == "pressure_tolerance must be positive"
+    assert perr("pressure_maxit = 0\n") == "pressure_maxit must be >= 1"
+    assert perr("mg_cycles = 0\n") == "mg_cycles must be >= 1"
+    assert perr("mg_pre = -1\n") == "mg_pre and mg_post must be >= 0"
+    assert perr("mg_omega = 2.5\n") == "mg_omega must be in (0, 2]"
+    assert perr("mg_omega = 0\n") == "mg_omega must be in (0, 2]"
+    assert perr("mpi_grid = 2by2\n") == "mpi_grid must be auto or RxC (e.g. 2x2): 2by2"
+
+
+def test_every_reference_key_is_accepted():
+    keys = {
+        "N_cells_x": "8", "N_cells_y": "8", "N_halo_x": "2", "N_halo_y": "2", "Re": "100",
+        "maxit_inner_u": "4", "maxit_inner_v": "4", "maxit_inner_p": "8",
+        "error_tolerance": "1e-5", "maxit": "10", "relaxation_factor_u": "0.3",
+        "relaxation_factor_v": "0.3", "relaxation_factor_p": "0.2", "convection_scheme": "UDS",
+        "diffusion_order": "2", "Hybrid_switch": "False", "backend": "gpu",
+        "momentum_solver": "gs_multicolour", "pressure_solver": "mg", "output_dir": "x/y",
+        "residual_normalisation": "running_max", "pressure_tolerance": "1e-7",
+        "pressure_maxit": "5", "mg_cycles": "2", "mg_pre": "1", "mg_post": "3", "mg_omega": "1.5",
+        "mpi_grid": "auto", "halo_overlap": "True", "write_fields": "False",
+        # additions of this backend
+        "mg_precision": "mixed", "mg_coarsest": "8", "mg_dc_maxit": "3", "mg_dc_tol": "1e-3",
+        "pressure_warm_start": "learned", "warm_start_weights": "w.bin", "check_every": "50",
+    }
+    p, out = sg.params_from_config("".join(f"{k} = {v}\n" for k, v in keys.items()))
+    assert out == "x/y" and p.backend == sg.BACKEND_GPU and p.mg_precision == sg.MG_MIXED
+    assert p.residual_normalisation == sg.NORM_RUNNING_MAX and p.write_fields == 0
+    assert (p.mg_cycles, p.mg_pre, p.mg_post, p.mg_omega, p.mg_coarsest) == (2, 1, 3, 1.5, 8)
+
+
+def _write_outputs(p, outdir, o, log, conv):
+    h = sg.host()
+    f = h.sh_write_outputs
+    dp = C.POINTER(C.c_double)
+    f.argtypes = [C.POINTER(sg.SgParams), C.c_char_p, dp, dp, dp, dp, C.c_int, C.c_int,
+                  C.c_double, C.c_char_p, C.c_char_p, C.c_int]
+    u, v, pf = (np.ascontiguousarray(o.get(n)) for n in ("u", "v", "p"))
+    log = np.ascontiguousarray(log)
+    err = C.create_string_buffer(512)
+    rc = f(C.byref(p), outdir.encode(), *(a.ctypes.data_as(dp) for a in (u, v, pf, log)),
+           len(log), int(conv), 1.25, b"cfg.cfg", err, 512)
+    assert rc == 0, err.value
+
+
+def test_output_files_match_reference_snapshot_bytes(tmp_path):
+    """u/v/p.csv and alog.csv written by the host for the short_16 run are byte-identical to the
+    reference's snapshot files (same fields, same %.18e formatting)."""
+    p, _ = sg.params_from_config(golden_cfg("short_16"))
+    o = po.Oracle(p)
+    n, log, conv = o.run()
+    out = str(tmp_path / "run")
+    _write_outputs(p, out, o, log, conv)
+    d = os.path.join(GOLDEN, "snapshots", "short_16")
+    for f in ("u.csv", "v.csv", "p.csv", "alog.csv"):
+        with open(os.path.join(out, f), "rb") as a, open(os.path.join(d, f), "rb") as b:
+            assert a.read() == b.read(), f
+    ux = np.loadtxt(os.path.join(out, "ux_centre.csv"), delimiter=",")
+    u = o.get("u")
+    H = p.H
+    assert ux.shape == (p.NX, p.NY)
+    assert ux[3, 5] == (u[H + 3, H + 5] + u[H + 3, H + 6]) / 2
+    info = json.load(open(os.path.join(out, "run_info.json")))
+    ref = json.load(open(os.path.join(d, "run_info.json")))
+    # Every key of src/io.cpp:104-150 ("mode" is an older snapshot's key, no longer written).
+    keys = {"config", "git_hash", "compiler", "backend", "threads", "momentum_solver",
+            "pressure_solver", "N_cells_x", "N_cells_y", "N_halo_x", "N_halo_y", "Re",
+            "maxit_inner_u", "maxit_inner_v", "maxit_inner_p", "error_tolerance", "maxit",
+            "relaxation_factor_u", "relaxation_factor_v", "relaxation_factor_p",
+            "convection_scheme", "diffusion_order", "Hybrid_switch", "residual_normalisation",
+            "output_dir", "pressure_tolerance", "pressure_maxit", "mg_cycles", "mg_pre", "mg_post",
+            "mg_omega", "cg_iterations_total", "cg_iterations_mean", "ranks", "halo_seconds_mean",
+            "halo_seconds_max", "collective_seconds_mean", "collective_seconds_max", "iterations",
+            "converged", "seconds", "seconds_per_iteration"}
+    assert (set(ref) - {"mode"}) <= keys <= set(info)
+    assert info["iterations"] == n and info["converged"] == conv
+    timing = open(os.path.join(out, "timing.csv")).read().splitlines()
+    assert timing[0] == "phase,seconds,fraction"
+    assert [t.split(",")[0] for t in timing[1:]] == sg.PHASES + ["output", "total", "unaccounted"]
+
+
+def _cli(tmp_path, text):
+    cfg = tmp_path / "c.cfg"
+    cfg.write_text(text)
+    r = subprocess.run([os.path.join(BUILD, "simple"), str(cfg)], capture_output=True, text=True,
+                       timeout=60)
+    return r.returncode, r.stdout, r.stderr
+
+
+def test_cli_error_paths(tmp_path):
+    r = subprocess.run([os.path.join(BUILD, "simple")], capture_output=True, text=True)
+    assert r.returncode == 1 and "usage: simple <config>" in r.stderr
+    rc, _, err = _cli(tmp_path, "N_cells_x = 8\nbogus = 1\n")
+    assert rc == 1 and err.strip() == "error (rank 0): unknown config key: bogus"
+    rc, _, err = _cli(tmp_path, "N_cells_x = 8\nbackend = serial\n")
+    assert rc == 1 and "this build runs backend = gpu only" in err
+    rc, _, err = _cli(tmp_path, "backend = gpu\nmomentum_solver = gs_lex\n")
+    assert rc == 1 and err.startswith("error (rank 0): ")
+
+
+def test_abi_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, "include", "simple_gpu.h")).read()
+    names = set(re.findall(r"\b(sg_[a-z0-9_]+)\s*\(", hdr))
+    assert len(names) >= 25
+    lib = C.CDLL(os.path.join(BUILD, "libsimple_gpu.so"))
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_abi_errors_without_solver():
+    g = sg.gpu()
+    assert g.sg_abi_version() == 1
+    assert g.sg_destroy(None) == 0
+    rc = g.sg_assemble_u(None)
+    assert rc != 0 and g.sg_last_error().decode() == "null solver"
+    p = sg.SgParams()
+    g.sg_params_default(C.byref(p))
+    p.mg_omega = 3.0
+    assert g.sg_params_finalize(C.byref(p)) == sg_E_ARG()
+    assert g.sg_last_error().decode() == "mg_omega must be in (0, 2]"
+
+
+def sg_E_ARG():
+    return 1
