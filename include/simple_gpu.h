@@ -78,7 +78,6 @@ typedef struct sg_params {
     int mg_precision;           /* SG_MG_FP64 (reference arithmetic) or SG_MG_MIXED */
     int mg_coarsest;            /* smallest coarse side kept (reference: 4) */
     int mg_dc_maxit;            /* mixed: FP64 defect-correction steps per solve */
-    double mg_dc_tol;           /* mixed: stop when ||r||/||b|| <= this (0: run mg_dc_maxit) */
     int pressure_warm_start;    /* SG_WARM_* */
     int check_every;            /* device-side stop test: host looks every this many iterations */
 } sg_params;

@@ -119,7 +119,7 @@ void enqueue_iteration(Solver& s, const int* stop, bool eager, hipEvent_t* ev) {
     mark(2);
     launch_mom_residual(s, true, s.scal + S_RES_U, stop);
     mark(3);
-    launch_mom_sweeps(s, true, p.maxit_inner_u, stop);
+    launch_mom_sweeps(s, true, p.maxit_inner_u, stop, true);
     mark(4);
     launch_assemble(s, false, stop);
     mark(5);
@@ -127,7 +127,7 @@ void enqueue_iteration(Solver& s, const int* stop, bool eager, hipEvent_t* ev) {
     mark(6);
     launch_mom_residual(s, false, s.scal + S_RES_V, stop);
     mark(7);
-    launch_mom_sweeps(s, false, p.maxit_inner_v, stop);
+    launch_mom_sweeps(s, false, p.maxit_inner_v, stop, true);
     mark(8);
     launch_assemble_p(s, s.scal + S_RES_P, stop);
     mark(9);
@@ -295,7 +295,6 @@ void sg_params_default(sg_params* p) {
     p->mg_precision = SG_MG_FP64;
     p->mg_coarsest = 4;
     p->mg_dc_maxit = 1;
-    p->mg_dc_tol = 0.0;
     p->pressure_warm_start = SG_WARM_NONE;
     p->check_every = 200;
 }
@@ -338,6 +337,11 @@ int sg_create(const sg_params* pin, const double* u, const double* v, const doub
         Solver& s = *sp;
         s.p = p;
         SG_HIP(hipGetDevice(&s.device));
+        {
+            hipDeviceProp_t prop;
+            SG_HIP(hipGetDeviceProperties(&prop, s.device));
+            s.lds_budget = std::min<size_t>(prop.sharedMemPerBlock, 160 * 1024) - 1024;
+        }
         s.nr = p.nrows;
         s.nc = p.ncols;
         s.ld = (p.ncols + 15) / 16 * 16;
@@ -349,6 +353,7 @@ int sg_create(const sg_params* pin, const double* u, const double* v, const doub
         s.cell = dalloc<double>(s, 4 * static_cast<size_t>(std::max(p.NX + 2, 1)) * (p.NY + 2));
         s.rowbuf = dalloc<double>(s, 4 * static_cast<size_t>(s.nr));
         s.scal = dalloc<double>(s, S_COUNT);
+        s.ticket = dalloc<unsigned>(s, 1);
         s.ctl = dalloc<Ctl>(s, 1);
         s.never = dalloc<int>(s, 1);
         if (p.pressure_solver == SG_P_CG)
@@ -358,6 +363,8 @@ int sg_create(const sg_params* pin, const double* u, const double* v, const doub
         s.pkv = make_pack(p, false);
         s.pku.base = dalloc<double>(s, 10 * s.pku.astride);
         s.pkv.base = dalloc<double>(s, 10 * s.pkv.astride);
+        s.fpk = make_field_pack(p);
+        s.fbuf = dalloc<double>(s, s.fpk.astride);
         ensure_log(s, 256);
         diffusive(p, s.diff2, s.diff4);
         if (p.pressure_solver == SG_P_MG)
@@ -463,7 +470,7 @@ int sg_residual_u(sg_solver* h, double* res) {
 int sg_sweep_u(sg_solver* h, int n) {
     return guard([&] {
         Solver& s = S(h);
-        launch_mom_sweeps(s, true, n, s.never);
+        launch_mom_sweeps(s, true, n, s.never, false);
         sync(s);
     });
 }
@@ -480,7 +487,7 @@ int sg_residual_v(sg_solver* h, double* res) {
 int sg_sweep_v(sg_solver* h, int n) {
     return guard([&] {
         Solver& s = S(h);
-        launch_mom_sweeps(s, false, n, s.never);
+        launch_mom_sweeps(s, false, n, s.never, false);
         sync(s);
     });
 }
@@ -566,7 +573,6 @@ int sg_kernel_probe(sg_solver* h, int kid, int reps, double* mean_s, double* byt
         const int ncol = (p.convection_scheme == SG_UDS && p.diffusion_order == 2) ? 2 : 5;
         double cells = 0.0, bpc = 0.0;
         int per = 0;
-        double* keep = nullptr; // field the probe modifies, restored afterwards
         std::function<void(int)> one;
         switch (kid) {
         case SG_K_SWEEP_U:
@@ -574,7 +580,7 @@ int sg_kernel_probe(sg_solver* h, int kid, int reps, double* mean_s, double* byt
             const bool is_u = kid == SG_K_SWEEP_U;
             if (p.momentum_solver != SG_MOM_GS_MULTICOLOUR)
                 throw ArgError("sweep probe needs momentum_solver = gs_multicolour");
-            keep = s.F(is_u ? SG_F_U : SG_F_V);
+            launch_mom_pack(s, is_u, s.never); // the probe sweeps the packed copy only
             cells = is_u ? static_cast<double>(p.NX) * (p.NY - 1)
                          : static_cast<double>(p.NX - 1) * p.NY;
             cells /= ncol;
@@ -592,9 +598,6 @@ int sg_kernel_probe(sg_solver* h, int kid, int reps, double* mean_s, double* byt
         default:
             throw ArgError("unknown kernel id");
         }
-        if (keep)
-            SG_HIP(hipMemcpyAsync(s.scratch, keep, sizeof(double) * s.n, hipMemcpyDeviceToDevice,
-                                  s.stream));
         one(0); // warm
         SG_HIP(hipEventRecord(s.kt0, s.stream));
         for (int k = 0; k < reps; ++k)
@@ -603,11 +606,6 @@ int sg_kernel_probe(sg_solver* h, int kid, int reps, double* mean_s, double* byt
         sync(s);
         float ms = 0.f;
         SG_HIP(hipEventElapsedTime(&ms, s.kt0, s.kt1));
-        if (keep) {
-            SG_HIP(hipMemcpyAsync(keep, s.scratch, sizeof(double) * s.n, hipMemcpyDeviceToDevice,
-                                  s.stream));
-            sync(s);
-        }
         if (mean_s)
             *mean_s = 1e-3 * ms / reps;
         if (bytes)

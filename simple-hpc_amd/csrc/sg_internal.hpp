@@ -91,6 +91,7 @@ struct Solver {
     double* scratch = nullptr;         // Jacobi previous iterate
     double* cell = nullptr;            // per-cell values for ordered sums (4 planes of n)
     double* rowbuf = nullptr;          // row sums (4 x nr)
+    unsigned* ticket = nullptr;        // last-workgroup ticket of the ordered sums (kept 0)
     double* scal = nullptr;            // S_COUNT device scalars
     double* cg[4] = {};                // CG r, z, d, q
     Ctl* ctl = nullptr;                // device control block
@@ -99,8 +100,12 @@ struct Solver {
     int log_cap = 0;
     std::vector<MgLevel> mg;
     double* mg_r0 = nullptr;           // level-0 residual
+    long tail_cells = 4096;            // levels with at most this many cells run in one workgroup
+    size_t lds_budget = 65536 - 1024;  // dynamic LDS of the multigrid tail (set from the device)
     std::vector<void*> owned;          // allocations to free
     Pack pku{}, pkv{};                 // packed momentum coefficients
+    Pack fpk{};                        // whole-array packing of the swept field
+    double* fbuf = nullptr;            // the swept field, colour-packed (u, then v)
     C9 diff2{}, diff4{};               // diffusive coefficients: order 2 (and band), order 4
     hipGraphExec_t graph = nullptr;
     int iters = 0;                     // host copy of completed iterations
@@ -119,7 +124,9 @@ struct Solver {
 void launch_assemble(Solver& s, bool is_u, const int* stop);
 void launch_bc(Solver& s, bool is_u, const int* stop);
 void launch_mom_residual(Solver& s, bool is_u, double* slot, const int* stop);
-void launch_mom_sweeps(Solver& s, bool is_u, int nsweeps, const int* stop);
+void launch_mom_sweeps(Solver& s, bool is_u, int nsweeps, const int* stop, bool packed);
+void launch_mom_pack(Solver& s, bool is_u, const int* stop);
+Pack make_field_pack(const sg_params& p);
 void launch_mom_colour_pass(Solver& s, bool is_u, int colour, const int* stop);
 int ncolours(const sg_params& p);
 Pack make_pack(const sg_params& p, bool is_u);
@@ -135,7 +142,7 @@ void launch_ordered_sum(Solver& s, const double* a, int pitch, int i0, int i1, i
                         double* slot, const int* stop);
 // multigrid (sg_mg.hip)
 void mg_setup(Solver& s);
-void launch_mg_solve(Solver& s, const int* stop);
+void launch_mg_solve(Solver& s, const int* stop, bool has_guess);
 void launch_remove_mean0(Solver& s, double* a, const int* stop);
 // outer loop control (sg_api.hip)
 void launch_stop_rule(Solver& s);

@@ -1,7 +1,24 @@
 // sg_mg.hip — geometric multigrid for p' on gfx950 (src/multigrid.cpp): Galerkin 2x2
-// aggregation, red-black smoother, residual, restriction, prolongation with over-correction,
-// and the coarsest-level solve in one workgroup. FP64 arithmetic is the reference's expression
-// for expression and every sum keeps its fixed order, so p' is bit-identical to the CPU.
+// aggregation, red-black smoother, residual, restriction, prolongation with over-correction and
+// the coarsest-level solve.
+//
+// Two forms, one set of kernels templated on the element type:
+//  - fp64 (the reference's method): every expression in the reference's order and every sum in
+//    its fixed order, so p' is bit-identical to the CPU;
+//  - mixed (mg_precision = mixed): the V-cycle (smoothing, residual, restriction, prolongation,
+//    coarse solve) runs in FP32 on FP32 copies of the operators, inside an FP64 defect
+//    correction: FP64 stays in the mean removal of b, the fine-grid defect b - A p', the update
+//    p' += e and the final mean removal.
+//
+// Launch structure of one V-cycle:
+//  - levels above the tail (more than tail_cells cells): one launch per smoother colour pass;
+//    the residual is fused into the restriction (each coarse cell evaluates its 4 fine
+//    residuals and sums them in the reference's order), and the prolongation is fused into the
+//    first post-smoothing pass (a colour-0 update reads its colour-1 neighbours as x + omega*xc,
+//    the value the separate prolongation would have stored; colour-1 cells are overwritten by the
+//    next pass before anything reads them);
+//  - the tail (every level with at most tail_cells cells, down to the coarsest, and its coarse
+//    solve) runs in one workgroup, one launch per V-cycle, synchronising with barriers.
 #include "sg_internal.hpp"
 
 #include <algorithm>
@@ -11,30 +28,39 @@ namespace sg {
 namespace {
 
 constexpr int BX = 64, BY = 4;
+constexpr int TT = 1024;     // tail workgroup size
+constexpr int MAXLEV = 24;
 
-struct Lv {
+template <class T> struct Lv {
     int nx, ny, stride;
-    double *AE, *AW, *AN, *AS, *AP, *b, *x, *r;
+    T *AE, *AW, *AN, *AS, *AP, *b, *x, *r;
 };
 
-Lv lv(const MgLevel& L) {
+Lv<double> lv64(const MgLevel& L) {
     return {L.nx, L.ny, L.stride, L.AE, L.AW, L.AN, L.AS, L.AP, L.b, L.x, L.r};
 }
+Lv<float> lv32(const MgLevel& L) {
+    return {L.nx, L.ny, L.ny + 2, L.fAE, L.fAW, L.fAN, L.fAS, L.fAP, L.fb, L.fx, L.fr};
+}
 
-// mg_build (src/multigrid.cpp:232-261): coarse links are the sums of the two fine links crossing
-// each coarse face; AP = ae + aw + an + as.
-__global__ void __launch_bounds__(BX* BY) k_mg_build(Lv F, Lv C, const int* stop) {
-    if (stopped(stop))
-        return;
-    const int J = blockIdx.x * BX + threadIdx.x, I = blockIdx.y * BY + threadIdx.y;
-    if (I >= C.nx || J >= C.ny)
-        return;
+// Relative tolerance of the coarsest-level solve: the reference's 1e-10 in FP64; in FP32 a
+// tolerance the arithmetic can reach.
+template <class T> __device__ constexpr T coarse_tol();
+template <> __device__ constexpr double coarse_tol<double>() { return 1e-10; }
+template <> __device__ constexpr float coarse_tol<float>() { return 1e-6f; }
+
+// ---- per-cell pieces (src/multigrid.cpp) ----
+
+// Coarse links: sums of the two fine links crossing each coarse face; AP = ae + aw + an + as
+// (mg_build, :232-261).
+template <class T>
+__device__ __forceinline__ void build_cell(const Lv<T>& F, const Lv<T>& C, int I, int J) {
     const long fs = F.stride;
     const size_t f = 2 * I * fs + 2 * J, c = static_cast<size_t>(I) * C.stride + J;
-    const double ae = F.AE[f + 1] + F.AE[f + fs + 1];
-    const double aw = F.AW[f] + F.AW[f + fs];
-    const double an = F.AN[f] + F.AN[f + 1];
-    const double as = F.AS[f + fs] + F.AS[f + fs + 1];
+    const T ae = F.AE[f + 1] + F.AE[f + fs + 1];
+    const T aw = F.AW[f] + F.AW[f + fs];
+    const T an = F.AN[f] + F.AN[f + 1];
+    const T as = F.AS[f + fs] + F.AS[f + fs + 1];
     C.AE[c] = ae;
     C.AW[c] = aw;
     C.AN[c] = an;
@@ -42,48 +68,48 @@ __global__ void __launch_bounds__(BX* BY) k_mg_build(Lv F, Lv C, const int* stop
     C.AP[c] = ae + aw + an + as;
 }
 
-// Two-statement smoother update (src/multigrid.cpp:62-64).
-__device__ __forceinline__ void smooth_cell(const Lv& L, size_t q) {
+// Two-statement smoother update (smooth, :62-64).
+template <class T> __device__ __forceinline__ void smooth_cell(const Lv<T>& L, size_t q) {
     const long s = L.stride;
-    double* x = L.x;
-    double t = L.b[q] / L.AP[q];
+    const T* x = L.x;
+    T t = L.b[q] / L.AP[q];
     t += (L.AE[q] * x[q + 1] + L.AW[q] * x[q - 1] + L.AS[q] * x[q + s] + L.AN[q] * x[q - s]) /
          L.AP[q];
-    x[q] = t;
+    L.x[q] = t;
 }
 
-__device__ __forceinline__ double resid_cell(const Lv& L, size_t q) {
+// b - (AP x - (E + W + S + N)) (residual, :83-84).
+template <class T> __device__ __forceinline__ T resid_cell(const Lv<T>& L, size_t q) {
     const long s = L.stride;
-    const double* x = L.x;
-    return L.b[q] - (L.AP[q] * x[q] -
-                     (L.AE[q] * x[q + 1] + L.AW[q] * x[q - 1] + L.AS[q] * x[q + s] +
-                      L.AN[q] * x[q - s]));
+    const T* x = L.x;
+    return L.b[q] - (L.AP[q] * x[q] - (L.AE[q] * x[q + 1] + L.AW[q] * x[q - 1] +
+                                       L.AS[q] * x[q + s] + L.AN[q] * x[q - s]));
 }
 
-// One colour of the red-black smoother, colour (I + J) & 1 (src/multigrid.cpp:46-70).
-__global__ void __launch_bounds__(BX* BY) k_mg_smooth(Lv L, int col, const int* stop) {
+// ---- multi-workgroup kernels for the large levels ----
+
+template <class T> __global__ void __launch_bounds__(BX* BY) k_mg_build(Lv<T> F, Lv<T> C, const int* stop) {
+    if (stopped(stop))
+        return;
+    const int J = blockIdx.x * BX + threadIdx.x, I = blockIdx.y * BY + threadIdx.y;
+    if (I < C.nx && J < C.ny)
+        build_cell(F, C, I, J);
+}
+
+// One colour of the red-black smoother, colour (I + J) & 1 (smooth, :46-70).
+template <class T> __global__ void __launch_bounds__(BX* BY) k_mg_smooth(Lv<T> L, int col, const int* stop) {
     if (stopped(stop))
         return;
     const int I = blockIdx.y * BY + threadIdx.y;
     const int J = ((I + col) & 1) + 2 * (blockIdx.x * BX + threadIdx.x);
-    if (I >= L.nx || J >= L.ny)
-        return;
-    smooth_cell(L, static_cast<size_t>(I) * L.stride + J);
+    if (I < L.nx && J < L.ny)
+        smooth_cell(L, static_cast<size_t>(I) * L.stride + J);
 }
 
-// r = b - A x (src/multigrid.cpp:73-87).
-__global__ void __launch_bounds__(BX* BY) k_mg_resid(Lv L, const int* stop) {
-    if (stopped(stop))
-        return;
-    const int J = blockIdx.x * BX + threadIdx.x, I = blockIdx.y * BY + threadIdx.y;
-    if (I >= L.nx || J >= L.ny)
-        return;
-    const size_t q = static_cast<size_t>(I) * L.stride + J;
-    L.r[q] = resid_cell(L, q);
-}
-
-// Coarse b = sum of the 4 fine residuals, coarse x = 0 (src/multigrid.cpp:90-103).
-__global__ void __launch_bounds__(BX* BY) k_mg_restrict(Lv F, Lv C, const int* stop) {
+// Residual and restriction fused: coarse b = r(NW) + r(NE) + r(SW) + r(SE), coarse x = 0
+// (residual :73-87 then restrict_residual :90-103).
+template <class T>
+__global__ void __launch_bounds__(BX* BY) k_mg_resid_restrict(Lv<T> F, Lv<T> C, const int* stop) {
     if (stopped(stop))
         return;
     const int J = blockIdx.x * BX + threadIdx.x, I = blockIdx.y * BY + threadIdx.y;
@@ -91,105 +117,400 @@ __global__ void __launch_bounds__(BX* BY) k_mg_restrict(Lv F, Lv C, const int* s
         return;
     const long fs = F.stride;
     const size_t f = 2 * I * fs + 2 * J, c = static_cast<size_t>(I) * C.stride + J;
-    C.b[c] = F.r[f] + F.r[f + 1] + F.r[f + fs] + F.r[f + fs + 1];
-    C.x[c] = 0.0;
+    const T r0 = resid_cell(F, f), r1 = resid_cell(F, f + 1);
+    const T r2 = resid_cell(F, f + fs), r3 = resid_cell(F, f + fs + 1);
+    C.b[c] = r0 + r1 + r2 + r3;
+    C.x[c] = T(0);
 }
 
-// Fine x += omega * coarse x (src/multigrid.cpp:106-116).
-__global__ void __launch_bounds__(BX* BY) k_mg_prolong(Lv C, Lv F, double omega, const int* stop) {
+// Prolongation fused into the first post-smoothing pass (colour 0): neighbours inside the level
+// are read as x + omega * xc (prolong, :106-116).
+template <class T>
+__global__ void __launch_bounds__(BX* BY)
+    k_mg_prolong_smooth0(Lv<T> C, Lv<T> F, T omega, const int* stop) {
+    if (stopped(stop))
+        return;
+    const int I = blockIdx.y * BY + threadIdx.y;
+    const int J = (I & 1) + 2 * (blockIdx.x * BX + threadIdx.x);
+    if (I >= F.nx || J >= F.ny)
+        return;
+    const long s = F.stride;
+    const size_t q = static_cast<size_t>(I) * s + J;
+    const T* x = F.x;
+    const T* xc = C.x;
+    const long cs = C.stride;
+    auto px = [&](int i, int j, size_t qq) {
+        return (i >= 0 && i < F.nx && j >= 0 && j < F.ny)
+                   ? x[qq] + omega * xc[static_cast<size_t>(i / 2) * cs + j / 2]
+                   : x[qq];
+    };
+    const T xe = px(I, J + 1, q + 1), xw = px(I, J - 1, q - 1);
+    const T xs = px(I + 1, J, q + s), xn = px(I - 1, J, q - s);
+    T t = F.b[q] / F.AP[q];
+    t += (F.AE[q] * xe + F.AW[q] * xw + F.AS[q] * xs + F.AN[q] * xn) / F.AP[q];
+    F.x[q] = t;
+}
+
+// Plain prolongation (only when mg_post = 0).
+template <class T>
+__global__ void __launch_bounds__(BX* BY) k_mg_prolong(Lv<T> C, Lv<T> F, T omega, const int* stop) {
     if (stopped(stop))
         return;
     const int j = blockIdx.x * BX + threadIdx.x, i = blockIdx.y * BY + threadIdx.y;
-    if (i >= F.nx || j >= F.ny)
-        return;
-    F.x[static_cast<size_t>(i) * F.stride + j] +=
-        omega * C.x[static_cast<size_t>(i / 2) * C.stride + j / 2];
+    if (i < F.nx && j < F.ny)
+        F.x[static_cast<size_t>(i) * F.stride + j] +=
+            omega * C.x[static_cast<size_t>(i / 2) * C.stride + j / 2];
 }
 
-// coarse_solve (src/multigrid.cpp:120-140) in one workgroup: mean-free b, red-black sweeps until
-// ||r|| < 1e-10 ||b|| or 200 sweeps, mean-free x. det_sum order: each row by one thread (J
-// ascending from 0.0), then the rows by thread 0 (I ascending from 0.0).
-constexpr int CT = 1024;
+// ---- the tail: one workgroup, every level in LDS ----
+//
+// The tail levels t0..nl-1 live in LDS for the whole launch: level t0's operator is loaded
+// (t0 = 0) or built from level t0-1's global operator, the coarser ones are built in LDS, b and x
+// of level t0 come from global memory and x goes back at the end. Each level block holds 8
+// arrays (AE, AW, AN, AS, AP, b, x, r) of (nx+2) x (ny+2) elements with a zero halo.
 
-__device__ double wg_det_sum(const Lv& L, double* rows, int which) {
-    __shared__ double total;
-    const int t = threadIdx.x;
-    for (int I = t; I < L.nx; I += CT) {
-        double s = 0.0;
+template <class T> struct Tail {
+    Lv<T> G[MAXLEV]; // global views (level t0-1's operator, level t0's b and x)
+    int off[MAXLEV]; // LDS element offset of each tail level's block
+    int t0, nl;      // first tail level, number of levels
+    int pre, post;
+    T omega;
+    int lds_elems;   // elements of all level blocks; the row-sum scratch follows
+};
+
+template <class T> __device__ __forceinline__ Lv<T> lds_view(const Tail<T>& t, T* sm, int l) {
+    const int nx = t.G[l].nx, ny = t.G[l].ny, st = ny + 2;
+    const size_t n = static_cast<size_t>(nx + 2) * st;
+    T* b = sm + t.off[l] + st + 1;
+    return {nx, ny, st, b, b + n, b + 2 * n, b + 3 * n, b + 4 * n, b + 5 * n, b + 6 * n, b + 7 * n};
+}
+
+template <class T> __device__ void wg_smooth(const Lv<T>& L, int sweeps) {
+    const int half = (L.ny + 1) / 2, n = L.nx * half;
+    for (int k = 0; k < sweeps; ++k)
+        for (int c = 0; c < 2; ++c) {
+            for (int e = threadIdx.x; e < n; e += TT) {
+                const int I = e / half;
+                const int J = ((I + c) & 1) + 2 * (e - I * half);
+                if (J < L.ny)
+                    smooth_cell(L, static_cast<size_t>(I) * L.stride + J);
+            }
+            __syncthreads();
+        }
+}
+
+template <class T> __device__ void wg_resid(const Lv<T>& L) {
+    for (int e = threadIdx.x; e < L.nx * L.ny; e += TT) {
+        const int I = e / L.ny, J = e - I * L.ny;
+        const size_t q = static_cast<size_t>(I) * L.stride + J;
+        L.r[q] = resid_cell(L, q);
+    }
+    __syncthreads();
+}
+
+// det_sum (:18-34): each row by one thread (J ascending from 0.0), then the rows by thread 0
+// (I ascending from 0.0). which: 0 b, 1 b*b, 2 r*r, 3 x.
+template <class T> __device__ T wg_det_sum(const Lv<T>& L, T* rows, int which) {
+    __shared__ T total;
+    for (int I = threadIdx.x; I < L.nx; I += TT) {
+        T s = T(0);
         const size_t r0 = static_cast<size_t>(I) * L.stride;
         for (int J = 0; J < L.ny; ++J) {
             const size_t q = r0 + J;
-            const double v = which == 0 ? L.b[q] : which == 1 ? L.b[q] * L.b[q]
-                                         : which == 2 ? L.r[q] * L.r[q] : L.x[q];
+            const T v = which == 0 ? L.b[q] : which == 1 ? L.b[q] * L.b[q]
+                                    : which == 2 ? L.r[q] * L.r[q] : L.x[q];
             s += v;
         }
         rows[I] = s;
     }
     __syncthreads();
-    if (t == 0) {
-        double s = 0.0;
+    if (threadIdx.x == 0) {
+        T s = T(0);
         for (int I = 0; I < L.nx; ++I)
             s += rows[I];
         total = s;
     }
     __syncthreads();
-    const double v = total;
+    const T v = total;
     __syncthreads();
     return v;
 }
 
-__device__ void wg_sub(const Lv& L, double* x, double m) {
-    const int n = L.nx * L.ny;
-    for (int k = threadIdx.x; k < n; k += CT) {
-        const int I = k / L.ny, J = k - I * L.ny;
+template <class T> __device__ void wg_sub(const Lv<T>& L, T* x, T m) {
+    for (int e = threadIdx.x; e < L.nx * L.ny; e += TT) {
+        const int I = e / L.ny, J = e - I * L.ny;
         x[static_cast<size_t>(I) * L.stride + J] -= m;
     }
     __syncthreads();
 }
 
-__global__ void __launch_bounds__(CT) k_mg_coarse(Lv L, double* rows, Ctl* ctl, const int* stop) {
-    if (stopped(stop))
-        return;
-    const double ncell = static_cast<double>(L.nx) * L.ny;
+// coarse_solve (:120-140): mean-free b, red-black sweeps until ||r|| < tol ||b|| or 200 sweeps,
+// mean-free x.
+template <class T> __device__ int wg_coarse_solve(const Lv<T>& L, T* rows) {
+    const T ncell = static_cast<T>(static_cast<double>(L.nx) * L.ny);
     wg_sub(L, L.b, wg_det_sum(L, rows, 0) / ncell);
-    const double bb = wg_det_sum(L, rows, 1);
+    const T bb = wg_det_sum(L, rows, 1);
     int sweeps = 0;
-    if (bb > 0.0) {
-        const double bn = sqrt(bb);
-        const int half = (L.ny + 1) / 2;
+    if (bb > T(0)) {
+        const T bn = sqrt(bb);
         while (sweeps < 200) {
-            for (int c = 0; c < 2; ++c) {
-                for (int k = threadIdx.x; k < L.nx * half; k += CT) {
-                    const int I = k / half;
-                    const int J = ((I + c) & 1) + 2 * (k - I * half);
-                    if (J < L.ny)
-                        smooth_cell(L, static_cast<size_t>(I) * L.stride + J);
-                }
-                __syncthreads();
-            }
+            wg_smooth(L, 1);
             ++sweeps;
-            for (int k = threadIdx.x; k < L.nx * L.ny; k += CT) {
-                const int I = k / L.ny, J = k - I * L.ny;
-                const size_t q = static_cast<size_t>(I) * L.stride + J;
-                L.r[q] = resid_cell(L, q);
-            }
-            __syncthreads();
-            const double rr = wg_det_sum(L, rows, 2);
-            if (sqrt(rr) < 1e-10 * bn)
+            wg_resid(L);
+            const T rr = wg_det_sum(L, rows, 2);
+            if (sqrt(rr) < coarse_tol<T>() * bn)
                 break;
         }
     }
     wg_sub(L, L.x, wg_det_sum(L, rows, 3) / ncell);
+    return sweeps;
+}
+
+template <class T> __device__ void wg_build(const Lv<T>& F, const Lv<T>& C) {
+    for (int e = threadIdx.x; e < C.nx * C.ny; e += TT)
+        build_cell(F, C, e / C.ny, e % C.ny);
+    __syncthreads();
+}
+
+// Coarsest level too large for LDS (a grid that does not coarsen): the same solve on the global
+// arrays, still one workgroup.
+template <class T>
+__global__ void __launch_bounds__(TT) k_mg_coarse_global(Lv<T> L, T* rows, Ctl* ctl, const int* stop) {
+    if (stopped(stop))
+        return;
+    const int sweeps = wg_coarse_solve(L, rows);
     if (threadIdx.x == 0)
         ctl->coarse_sweeps = sweeps;
 }
 
+template <class T>
+__global__ void __launch_bounds__(TT) k_mg_tail(Tail<T> t, Ctl* ctl, const int* stop) {
+    if (stopped(stop))
+        return;
+    extern __shared__ __align__(16) unsigned char smem[];
+    T* sm = reinterpret_cast<T*>(smem);
+    for (int e = threadIdx.x; e < t.lds_elems; e += TT)
+        sm[e] = T(0);
+    __syncthreads();
+    T* rows = sm + t.lds_elems;
+    Lv<T> L[MAXLEV];
+    for (int l = t.t0; l < t.nl; ++l)
+        L[l] = lds_view(t, sm, l);
+    // Level t0: operator (loaded, or built from the global level above), b and x.
+    const Lv<T> &G0 = t.G[t.t0], &D = L[t.t0];
+    if (t.t0 > 0)
+        wg_build(t.G[t.t0 - 1], D);
+    for (int e = threadIdx.x; e < D.nx * D.ny; e += TT) {
+        const int I = e / D.ny, J = e - I * D.ny;
+        const size_t g = static_cast<size_t>(I) * G0.stride + J, q = static_cast<size_t>(I) * D.stride + J;
+        if (t.t0 == 0) {
+            D.AE[q] = G0.AE[g];
+            D.AW[q] = G0.AW[g];
+            D.AN[q] = G0.AN[g];
+            D.AS[q] = G0.AS[g];
+            D.AP[q] = G0.AP[g];
+        }
+        D.b[q] = G0.b[g];
+        D.x[q] = G0.x[g];
+    }
+    __syncthreads();
+    for (int l = t.t0 + 1; l < t.nl; ++l)
+        wg_build(L[l - 1], L[l]);
+    for (int l = t.t0; l + 1 < t.nl; ++l) {
+        const Lv<T>&F = L[l], &C = L[l + 1];
+        wg_smooth(F, t.pre);
+        wg_resid(F);
+        for (int e = threadIdx.x; e < C.nx * C.ny; e += TT) {
+            const int I = e / C.ny, J = e - I * C.ny;
+            const long fs = F.stride;
+            const size_t f = 2 * I * fs + 2 * J, c = static_cast<size_t>(I) * C.stride + J;
+            C.b[c] = F.r[f] + F.r[f + 1] + F.r[f + fs] + F.r[f + fs + 1];
+            C.x[c] = T(0);
+        }
+        __syncthreads();
+    }
+    const int sweeps = wg_coarse_solve(L[t.nl - 1], rows);
+    for (int l = t.nl - 2; l >= t.t0; --l) {
+        const Lv<T>&F = L[l], &C = L[l + 1];
+        for (int e = threadIdx.x; e < F.nx * F.ny; e += TT) {
+            const int i = e / F.ny, j = e - i * F.ny;
+            F.x[static_cast<size_t>(i) * F.stride + j] +=
+                t.omega * C.x[static_cast<size_t>(i / 2) * C.stride + j / 2];
+        }
+        __syncthreads();
+        wg_smooth(F, t.post);
+    }
+    // Level t0's x (and, for the coarsest level, its mean-free b) back to global memory.
+    for (int e = threadIdx.x; e < D.nx * D.ny; e += TT) {
+        const int I = e / D.ny, J = e - I * D.ny;
+        const size_t g = static_cast<size_t>(I) * G0.stride + J, q = static_cast<size_t>(I) * D.stride + J;
+        G0.x[g] = D.x[q];
+        if (t.t0 == t.nl - 1)
+            G0.b[g] = D.b[q];
+    }
+    if (threadIdx.x == 0)
+        ctl->coarse_sweeps = sweeps;
+}
+
+// ---- mixed precision: FP64 <-> FP32 level-0 transfers ----
+
+struct Mix {
+    int nx, ny, ld;          // level-0 interior size, FP64 stride
+    const double *AE, *AW, *AN, *AS, *AP, *b; // FP64 level 0 (at interior cell 0, 0)
+    double* x;               // FP64 p' (at interior cell 0, 0)
+    Lv<float> f;             // FP32 level 0
+};
+
+// FP32 copies of the level-0 operator; rhs = b (first step, p' = 0), x = 0.
+__global__ void __launch_bounds__(BX* BY) k_mx_init(Mix m, const int* stop) {
+    if (stopped(stop))
+        return;
+    const int J = blockIdx.x * BX + threadIdx.x, I = blockIdx.y * BY + threadIdx.y;
+    if (I >= m.nx || J >= m.ny)
+        return;
+    const size_t q = static_cast<size_t>(I) * m.ld + J, g = static_cast<size_t>(I) * m.f.stride + J;
+    m.f.AE[g] = static_cast<float>(m.AE[q]);
+    m.f.AW[g] = static_cast<float>(m.AW[q]);
+    m.f.AN[g] = static_cast<float>(m.AN[q]);
+    m.f.AS[g] = static_cast<float>(m.AS[q]);
+    m.f.AP[g] = static_cast<float>(m.AP[q]);
+    m.f.b[g] = static_cast<float>(m.b[q]);
+    m.f.x[g] = 0.0f;
+}
+
+// FP64 defect r = b - A p' (level-0 operator) as the FP32 right-hand side; x = 0.
+__global__ void __launch_bounds__(BX* BY) k_mx_defect(Mix m, const int* stop) {
+    if (stopped(stop))
+        return;
+    const int J = blockIdx.x * BX + threadIdx.x, I = blockIdx.y * BY + threadIdx.y;
+    if (I >= m.nx || J >= m.ny)
+        return;
+    const long s = m.ld;
+    const size_t q = static_cast<size_t>(I) * s + J, g = static_cast<size_t>(I) * m.f.stride + J;
+    const double* x = m.x;
+    const double r = m.b[q] - (m.AP[q] * x[q] - (m.AE[q] * x[q + 1] + m.AW[q] * x[q - 1] +
+                                                 m.AS[q] * x[q + s] + m.AN[q] * x[q - s]));
+    m.f.b[g] = static_cast<float>(r);
+    m.f.x[g] = 0.0f;
+}
+
+// p' += e (FP64 update).
+__global__ void __launch_bounds__(BX* BY) k_mx_update(Mix m, const int* stop) {
+    if (stopped(stop))
+        return;
+    const int J = blockIdx.x * BX + threadIdx.x, I = blockIdx.y * BY + threadIdx.y;
+    if (I >= m.nx || J >= m.ny)
+        return;
+    m.x[static_cast<size_t>(I) * m.ld + J] +=
+        static_cast<double>(m.f.x[static_cast<size_t>(I) * m.f.stride + J]);
+}
+
 dim3 lgrid(int nx, int ny) { return dim3(cdiv(ny, BX), cdiv(nx, BY)); }
+
+template <class T>
+void smooth_passes(Solver& s, const Lv<T>& L, int sweeps, int first_col, const int* stop) {
+    for (int k = first_col; k < 2 * sweeps; ++k) {
+        hipLaunchKernelGGL(k_mg_smooth<T>, lgrid(L.nx, cdiv(L.ny, 2)), dim3(BX, BY), 0, s.stream,
+                           L, k & 1, stop);
+        SG_HIP(hipGetLastError());
+    }
+}
+
+// LDS bytes of a tail starting at level t0 (level blocks plus the row-sum scratch).
+template <class T> size_t tail_bytes(const Solver& s, int t0) {
+    size_t e = 0;
+    int maxnx = 0;
+    for (size_t l = t0; l < s.mg.size(); ++l) {
+        e += 8 * static_cast<size_t>(s.mg[l].nx + 2) * (s.mg[l].ny + 2);
+        maxnx = std::max(maxnx, s.mg[l].nx);
+    }
+    return sizeof(T) * (e + maxnx + 2);
+}
+
+// First tail level: the largest tail whose levels fit the LDS budget and have at most
+// tail_cells cells each.
+template <class T> int tail_start(const Solver& s) {
+    const int nl = static_cast<int>(s.mg.size());
+    int t0 = nl - 1;
+    while (t0 > 0 && static_cast<long>(s.mg[t0 - 1].nx) * s.mg[t0 - 1].ny <= s.tail_cells &&
+           tail_bytes<T>(s, t0 - 1) <= s.lds_budget)
+        --t0;
+    return t0;
+}
+
+// Coarse operators of the levels above the tail (the tail builds its own in LDS).
+template <class T> void launch_builds(Solver& s, const Tail<T>& t, const int* stop) {
+    // When even the coarsest level does not fit in LDS, it is built here too.
+    const int last = tail_bytes<T>(s, t.t0) <= s.lds_budget ? t.t0 - 1 : t.t0;
+    for (int l = 1; l <= last; ++l) {
+        hipLaunchKernelGGL(k_mg_build<T>, lgrid(t.G[l].nx, t.G[l].ny), dim3(BX, BY), 0, s.stream,
+                           t.G[l - 1], t.G[l], stop);
+        SG_HIP(hipGetLastError());
+    }
+}
+
+// mg_cycles V-cycles (mg_vcycle, :299-323) on level 0 of t.
+template <class T> void launch_vcycles(Solver& s, const Tail<T>& t, const int* stop) {
+    const sg_params& p = s.p;
+    const size_t lds = tail_bytes<T>(s, t.t0);
+    for (int cyc = 0; cyc < p.mg_cycles; ++cyc) {
+        for (int l = 0; l < t.t0; ++l) {
+            smooth_passes(s, t.G[l], p.mg_pre, 0, stop);
+            hipLaunchKernelGGL(k_mg_resid_restrict<T>, lgrid(t.G[l + 1].nx, t.G[l + 1].ny),
+                               dim3(BX, BY), 0, s.stream, t.G[l], t.G[l + 1], stop);
+            SG_HIP(hipGetLastError());
+        }
+        if (lds <= s.lds_budget)
+            hipLaunchKernelGGL(k_mg_tail<T>, dim3(1), dim3(TT), lds, s.stream, t, s.ctl, stop);
+        else // only the coarsest level, and it does not fit: solve it in global memory
+            hipLaunchKernelGGL(k_mg_coarse_global<T>, dim3(1), dim3(TT), 0, s.stream,
+                               t.G[t.nl - 1], reinterpret_cast<T*>(s.rowbuf), s.ctl, stop);
+        SG_HIP(hipGetLastError());
+        for (int l = t.t0 - 1; l >= 0; --l) {
+            const Lv<T> F = t.G[l], C = t.G[l + 1];
+            if (p.mg_post > 0) {
+                hipLaunchKernelGGL(k_mg_prolong_smooth0<T>, lgrid(F.nx, cdiv(F.ny, 2)),
+                                   dim3(BX, BY), 0, s.stream, C, F, t.omega, stop);
+                SG_HIP(hipGetLastError());
+                smooth_passes(s, F, p.mg_post, 1, stop);
+            } else {
+                hipLaunchKernelGGL(k_mg_prolong<T>, lgrid(F.nx, F.ny), dim3(BX, BY), 0, s.stream,
+                                   C, F, t.omega, stop);
+                SG_HIP(hipGetLastError());
+            }
+        }
+    }
+}
+
+template <class T> Tail<T> make_tail(Solver& s, Lv<T> (*view)(const MgLevel&)) {
+    Tail<T> t{};
+    const int nl = static_cast<int>(s.mg.size());
+    for (int l = 0; l < nl; ++l)
+        t.G[l] = view(s.mg[l]);
+    t.t0 = tail_start<T>(s);
+    t.nl = nl;
+    t.pre = s.p.mg_pre;
+    t.post = s.p.mg_post;
+    t.omega = static_cast<T>(s.p.mg_omega);
+    int e = 0;
+    for (int l = t.t0; l < nl; ++l) {
+        t.off[l] = e;
+        e += 8 * (s.mg[l].nx + 2) * (s.mg[l].ny + 2);
+    }
+    t.lds_elems = e;
+    const size_t bytes = tail_bytes<T>(s, t.t0);
+    if (bytes > 65536 && bytes <= s.lds_budget)
+        SG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mg_tail<T>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   static_cast<int>(bytes)));
+    return t;
+}
 
 } // namespace
 
 // Levels (src/multigrid.cpp:143-185): halve while both sides are even and the halves stay at
-// least mg_coarsest (4 in the reference). Level 0 aliases the p' arrays.
+// least mg_coarsest (4 in the reference). Level 0 aliases the p' arrays. The mixed form adds
+// FP32 copies of every level ((nx+2) x (ny+2), zero halo).
 void mg_setup(Solver& s) {
     const sg_params& p = s.p;
     const int coarsest = p.mg_coarsest > 0 ? p.mg_coarsest : 4;
@@ -200,6 +521,9 @@ void mg_setup(Solver& s) {
         ny /= 2;
         dims.emplace_back(nx, ny);
     }
+    if (static_cast<int>(dims.size()) > MAXLEV)
+        throw ArgError("too many multigrid levels");
+    const bool mixed = p.mg_precision == SG_MG_MIXED;
     s.mg.assign(dims.size(), MgLevel{});
     SG_HIP(hipMalloc(reinterpret_cast<void**>(&s.mg_r0), sizeof(double) * s.n));
     s.owned.push_back(s.mg_r0);
@@ -209,6 +533,16 @@ void mg_setup(Solver& s) {
         MgLevel& L = s.mg[l];
         L.nx = dims[l].first;
         L.ny = dims[l].second;
+        const size_t n = static_cast<size_t>(L.nx + 2) * (L.ny + 2);
+        if (mixed) {
+            float* blk = nullptr;
+            SG_HIP(hipMalloc(reinterpret_cast<void**>(&blk), sizeof(float) * n * 8));
+            SG_HIP(hipMemset(blk, 0, sizeof(float) * n * 8));
+            s.owned.push_back(blk);
+            float** ptr[8] = {&L.fAE, &L.fAW, &L.fAN, &L.fAS, &L.fAP, &L.fb, &L.fx, &L.fr};
+            for (int k = 0; k < 8; ++k)
+                *ptr[k] = blk + k * n + (L.ny + 2) + 1;
+        }
         if (l == 0) {
             L.stride = s.ld;
             L.AE = s.F(SG_F_AEP) + base0;
@@ -222,7 +556,8 @@ void mg_setup(Solver& s) {
             continue;
         }
         L.stride = L.ny + 2;
-        const size_t n = static_cast<size_t>(L.nx + 2) * (L.ny + 2);
+        if (mixed)
+            continue; // the FP64 form's coarse levels are not used
         double* blk = nullptr;
         SG_HIP(hipMalloc(reinterpret_cast<void**>(&blk), sizeof(double) * n * 8));
         SG_HIP(hipMemset(blk, 0, sizeof(double) * n * 8));
@@ -233,48 +568,39 @@ void mg_setup(Solver& s) {
     }
 }
 
-// solve_pc_mg (src/multigrid.cpp:347-357).
-void launch_mg_solve(Solver& s, const int* stop) {
+// solve_pc_mg (src/multigrid.cpp:347-357): build, p' = 0, mean-free b, mg_cycles V-cycles,
+// mean-free p'. Mixed form: the same with FP32 V-cycles inside mg_dc_maxit FP64 defect
+// corrections (the first with p' = 0, so its defect is b itself; with has_guess, p' holds an
+// initial guess and every step starts from its FP64 defect).
+void launch_mg_solve(Solver& s, const int* stop, bool has_guess) {
     const sg_params& p = s.p;
-    const int nl = static_cast<int>(s.mg.size());
-    for (int l = 1; l < nl; ++l) {
-        const Lv F = lv(s.mg[l - 1]), C = lv(s.mg[l]);
-        hipLaunchKernelGGL(k_mg_build, lgrid(C.nx, C.ny), dim3(BX, BY), 0, s.stream, F, C, stop);
-        SG_HIP(hipGetLastError());
+    if (p.mg_precision != SG_MG_MIXED) {
+        const Tail<double> t = make_tail<double>(s, lv64);
+        launch_builds(s, t, stop);
+        launch_fill(s, s.F(SG_F_PC), s.n, 0.0, stop);
+        launch_remove_mean0(s, s.F(SG_F_B), stop);
+        launch_vcycles(s, t, stop);
+        launch_remove_mean0(s, s.F(SG_F_PC), stop);
+        return;
     }
-    launch_fill(s, s.F(SG_F_PC), s.n, 0.0, stop);
+    const Tail<float> t = make_tail<float>(s, lv32);
+    const MgLevel& L0 = s.mg[0];
+    const Mix m{L0.nx, L0.ny, s.ld, L0.AE, L0.AW, L0.AN, L0.AS, L0.AP, L0.b, L0.x, t.G[0]};
+    const dim3 g = lgrid(m.nx, m.ny);
+    if (!has_guess)
+        launch_fill(s, s.F(SG_F_PC), s.n, 0.0, stop);
     launch_remove_mean0(s, s.F(SG_F_B), stop);
-    for (int cyc = 0; cyc < p.mg_cycles; ++cyc) {
-        // mg_vcycle (src/multigrid.cpp:299-323)
-        for (int l = 0; l + 1 < nl; ++l) {
-            const Lv F = lv(s.mg[l]), C = lv(s.mg[l + 1]);
-            for (int k = 0; k < p.mg_pre; ++k)
-                for (int c = 0; c < 2; ++c) {
-                    hipLaunchKernelGGL(k_mg_smooth, lgrid(F.nx, cdiv(F.ny, 2)), dim3(BX, BY), 0,
-                                       s.stream, F, c, stop);
-                    SG_HIP(hipGetLastError());
-                }
-            hipLaunchKernelGGL(k_mg_resid, lgrid(F.nx, F.ny), dim3(BX, BY), 0, s.stream, F, stop);
-            SG_HIP(hipGetLastError());
-            hipLaunchKernelGGL(k_mg_restrict, lgrid(C.nx, C.ny), dim3(BX, BY), 0, s.stream, F, C,
-                               stop);
+    hipLaunchKernelGGL(k_mx_init, g, dim3(BX, BY), 0, s.stream, m, stop);
+    SG_HIP(hipGetLastError());
+    launch_builds(s, t, stop);
+    for (int k = 0; k < p.mg_dc_maxit; ++k) {
+        if (k > 0 || has_guess) {
+            hipLaunchKernelGGL(k_mx_defect, g, dim3(BX, BY), 0, s.stream, m, stop);
             SG_HIP(hipGetLastError());
         }
-        hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(CT), 0, s.stream, lv(s.mg[nl - 1]),
-                           s.rowbuf, s.ctl, stop);
+        launch_vcycles(s, t, stop);
+        hipLaunchKernelGGL(k_mx_update, g, dim3(BX, BY), 0, s.stream, m, stop);
         SG_HIP(hipGetLastError());
-        for (int l = nl - 2; l >= 0; --l) {
-            const Lv F = lv(s.mg[l]), C = lv(s.mg[l + 1]);
-            hipLaunchKernelGGL(k_mg_prolong, lgrid(F.nx, F.ny), dim3(BX, BY), 0, s.stream, C, F,
-                               p.mg_omega, stop);
-            SG_HIP(hipGetLastError());
-            for (int k = 0; k < p.mg_post; ++k)
-                for (int c = 0; c < 2; ++c) {
-                    hipLaunchKernelGGL(k_mg_smooth, lgrid(F.nx, cdiv(F.ny, 2)), dim3(BX, BY), 0,
-                                       s.stream, F, c, stop);
-                    SG_HIP(hipGetLastError());
-                }
-        }
     }
     launch_remove_mean0(s, s.F(SG_F_PC), stop);
 }

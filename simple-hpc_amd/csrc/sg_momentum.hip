@@ -11,6 +11,12 @@
 // it at neighbouring cells). The row-major copies of the other coefficients are refreshed only
 // when the host downloads them.
 //
+// The field itself (u or v) is also colour-packed while it is swept, over the whole array
+// (rows [0, nrows), columns [0, ncols)): the prep kernel packs it (and evaluates the residual from
+// the row-major copy in the same pass), the colour passes read and write only packed data (the
+// neighbours of a colour-c cell sit at the same or an adjacent packed index of another colour's
+// rows), and an unpack kernel restores the row-major field after the last pass.
+//
 // Arithmetic is the reference's expression for expression (compiled with -ffp-contract=off, IEEE
 // division), so every value is bit-identical to src/momentum_u.cpp / momentum_v.cpp; the
 // pressure term (p(i,j) - p(i,j+1)) * dy is evaluated once, at assembly, with the same operands.
@@ -156,13 +162,8 @@ __device__ __forceinline__ double nb9(const double* c, size_t as, const double* 
            (c[4 * as] * x[q + 2] + c[5 * as] * x[q - 2] + c[6 * as] * x[q - 2 * ld] +
             c[7 * as] * x[q + 2 * ld]);
 }
-// 5-point sum of the 2-colour case (src/momentum_u.cpp:127-132).
-__device__ __forceinline__ double nb5(const double* c, size_t as, const double* x, size_t q,
-                                      long ld) {
-    return c[0] * x[q + 1] + c[as] * x[q - 1] + c[2 * as] * x[q - ld] + c[3 * as] * x[q + ld];
-}
-
-// Cell (i, j) of colour blockIdx.z handled by this thread, or false.
+// Cell (i, j) of colour c handled by this thread in the sweep range, or false; o = its
+// coefficient-pack index.
 __device__ __forceinline__ bool packed_cell(const MomArgs& m, int c, int& i, int& j, size_t& o) {
     i = m.i0 + blockIdx.y * BY + threadIdx.y;
     if (i > m.i1)
@@ -176,37 +177,67 @@ __device__ __forceinline__ bool packed_cell(const MomArgs& m, int c, int& i, int
     return true;
 }
 
-// |R| per cell, R = (nb + pterm) - AP x (src/momentum_u.cpp:48-50), into a compact buffer;
-// colour = blockIdx.z.
-__global__ void __launch_bounds__(BX* BY) k_mom_residual(MomArgs m, double* cell, const int* stop) {
+// Packed index of any cell (i, j) in the whole-array field pack fp.
+__device__ __forceinline__ size_t fidx(const Pack& fp, int i, int j) {
+    const int c = colour_of(fp.ncol, i, j);
+    return pidx(fp, c, i, (j - jfirst(fp, c, i)) / fp.ncol);
+}
+
+// Prep, one thread per cell of the whole array: pack the field (row-major -> colour-packed) and,
+// inside the residual range, |R| with R = (nb + pterm) - AP x (src/momentum_u.cpp:48-50) from the
+// row-major field, into a compact buffer for the ordered sum.
+__global__ void __launch_bounds__(BX* BY)
+    k_mom_prep(MomArgs m, Pack fp, double* up, double* cell, int nr, int nc, int want_res,
+               const int* stop) {
     if (stopped(stop))
         return;
-    int i, j;
-    size_t o;
-    if (!packed_cell(m, blockIdx.z, i, j, o))
+    const int j = blockIdx.x * BX + threadIdx.x, i = blockIdx.y * BY + threadIdx.y;
+    if (i >= nr || j >= nc)
         return;
     const long ld = m.ld;
-    const size_t q = static_cast<size_t>(i) * ld + j, as = m.pk.astride;
-    const double* c = m.pk.base + o;
-    const double R = nb9(c, as, m.x, q, ld) + c[9 * as] - c[8 * as] * m.x[q];
+    const size_t q = static_cast<size_t>(i) * ld + j;
+    up[fidx(fp, i, j)] = m.x[q];
+    if (!want_res || i < m.i0 || i > m.i1 || j < m.j0 || j > m.j1)
+        return;
+    const Pack& pk = m.pk;
+    const int c = colour_of(pk.ncol, i, j);
+    const double* co = pk.base + pidx(pk, c, i, (j - jfirst(pk, c, i)) / pk.ncol);
+    const double R = nb9(co, pk.astride, m.x, q, ld) + co[9 * pk.astride] -
+                     co[8 * pk.astride] * m.x[q];
     cell[static_cast<size_t>(i - m.i0) * (m.j1 - m.j0 + 1) + (j - m.j0)] = fabs(R);
 }
 
-// One colour pass of the multicolour sweep (src/momentum_u.cpp:130-138). Cells of one colour are
-// never stencil neighbours, so the in-place update is race free and equals the CPU's.
+// Packed field -> row-major field over the whole array.
+__global__ void __launch_bounds__(BX* BY)
+    k_field_unpack(Pack fp, const double* up, double* x, int ld, int nr, int nc, const int* stop) {
+    if (stopped(stop))
+        return;
+    const int j = blockIdx.x * BX + threadIdx.x, i = blockIdx.y * BY + threadIdx.y;
+    if (i < nr && j < nc)
+        x[static_cast<size_t>(i) * ld + j] = up[fidx(fp, i, j)];
+}
+
+// One colour pass of the multicolour sweep (src/momentum_u.cpp:130-138) on the packed field.
+// Cells of one colour are never stencil neighbours, so the in-place update is race free and
+// equals the CPU's. Neighbour sums keep the order of nb_sum / near_sum.
 template <int NCOL>
-__global__ void __launch_bounds__(BX* BY) k_mom_pass(MomArgs m, int col, const int* stop) {
+__global__ void __launch_bounds__(BX* BY)
+    k_mom_pass(MomArgs m, Pack fp, double* up, int col, const int* stop) {
     if (stopped(stop))
         return;
     int i, j;
     size_t o;
     if (!packed_cell(m, col, i, j, o))
         return;
-    const long ld = m.ld;
-    const size_t q = static_cast<size_t>(i) * ld + j, as = m.pk.astride;
+    const size_t as = m.pk.astride;
     const double* c = m.pk.base + o;
-    const double nb = NCOL == 2 ? nb5(c, as, m.x, q, ld) : nb9(c, as, m.x, q, ld);
-    m.x[q] = m.alpha * (nb + c[9 * as]) / c[8 * as] + m.one_minus_alpha * m.x[q];
+    const size_t self = fidx(fp, i, j);
+    double nb = c[0] * up[fidx(fp, i, j + 1)] + c[as] * up[fidx(fp, i, j - 1)] +
+                c[2 * as] * up[fidx(fp, i - 1, j)] + c[3 * as] * up[fidx(fp, i + 1, j)];
+    if (NCOL != 2)
+        nb = nb + (c[4 * as] * up[fidx(fp, i, j + 2)] + c[5 * as] * up[fidx(fp, i, j - 2)] +
+                   c[6 * as] * up[fidx(fp, i - 2, j)] + c[7 * as] * up[fidx(fp, i + 2, j)]);
+    up[self] = m.alpha * (nb + c[9 * as]) / c[8 * as] + m.one_minus_alpha * up[self];
 }
 
 // One Jacobi pass, dst from src (src/momentum_u.cpp:105-113); colour = blockIdx.z.
@@ -338,17 +369,25 @@ void launch_bc(Solver& s, bool is_u, const int* stop) {
     SG_HIP(hipGetLastError());
 }
 
+// Pack the field and evaluate the residual (residual_u / residual_v, src/momentum_u.cpp:43-59)
+// in one pass, then the ordered sum into *slot.
 void launch_mom_residual(Solver& s, bool is_u, double* slot, const int* stop) {
     const MomArgs m = mom_args(s, is_u);
     const int nrow = m.i1 - m.i0 + 1, w = m.j1 - m.j0 + 1;
-    if (nrow <= 0 || w <= 0) {
-        launch_ordered_sum(s, s.cell, 1, 0, -1, 0, -1, slot, stop); // empty range: 0.0
-        return;
-    }
-    hipLaunchKernelGGL(k_mom_residual, pass_grid(m, m.pk.ncol), dim3(BX, BY), 0, s.stream, m,
-                       s.cell, stop);
+    hipLaunchKernelGGL(k_mom_prep, dim3(cdiv(s.nc, BX), cdiv(s.nr, BY)), dim3(BX, BY), 0,
+                       s.stream, m, s.fpk, s.fbuf, s.cell, s.nr, s.nc, 1, stop);
     SG_HIP(hipGetLastError());
-    launch_ordered_sum(s, s.cell, w, 0, nrow - 1, 0, w - 1, slot, stop);
+    if (nrow <= 0 || w <= 0)
+        launch_ordered_sum(s, s.cell, 1, 0, -1, 0, -1, slot, stop); // empty range: 0.0
+    else
+        launch_ordered_sum(s, s.cell, w, 0, nrow - 1, 0, w - 1, slot, stop);
+}
+
+void launch_mom_pack(Solver& s, bool is_u, const int* stop) {
+    const MomArgs m = mom_args(s, is_u);
+    hipLaunchKernelGGL(k_mom_prep, dim3(cdiv(s.nc, BX), cdiv(s.nr, BY)), dim3(BX, BY), 0,
+                       s.stream, m, s.fpk, s.fbuf, s.cell, s.nr, s.nc, 0, stop);
+    SG_HIP(hipGetLastError());
 }
 
 void launch_mom_colour_pass(Solver& s, bool is_u, int col, const int* stop) {
@@ -356,26 +395,35 @@ void launch_mom_colour_pass(Solver& s, bool is_u, int col, const int* stop) {
     if (m.i1 < m.i0 || m.j1 < m.j0)
         return;
     if (m.pk.ncol == 2)
-        hipLaunchKernelGGL(k_mom_pass<2>, pass_grid(m, 1), dim3(BX, BY), 0, s.stream, m, col, stop);
+        hipLaunchKernelGGL(k_mom_pass<2>, pass_grid(m, 1), dim3(BX, BY), 0, s.stream, m, s.fpk,
+                           s.fbuf, col, stop);
     else
-        hipLaunchKernelGGL(k_mom_pass<5>, pass_grid(m, 1), dim3(BX, BY), 0, s.stream, m, col, stop);
+        hipLaunchKernelGGL(k_mom_pass<5>, pass_grid(m, 1), dim3(BX, BY), 0, s.stream, m, s.fpk,
+                           s.fbuf, col, stop);
     SG_HIP(hipGetLastError());
 }
 
 // sweep_u_batch / sweep_v_batch (src/momentum_u.cpp:164-171) for the two parallel solvers.
-void launch_mom_sweeps(Solver& s, bool is_u, int nsweeps, const int* stop) {
+// Multicolour: the packed field must hold the current field (packed = the residual's prep ran
+// since the last change, else it is packed here); it is unpacked after the last pass.
+void launch_mom_sweeps(Solver& s, bool is_u, int nsweeps, const int* stop, bool packed) {
     const sg_params& p = s.p;
     if (nsweeps <= 0)
         return;
+    const MomArgs m = mom_args(s, is_u);
     if (p.momentum_solver == SG_MOM_GS_MULTICOLOUR) {
+        if (!packed)
+            launch_mom_pack(s, is_u, stop);
         const int ncol = ncolours(p);
         for (int k = 0; k < nsweeps; ++k)
             for (int c = 0; c < ncol; ++c)
                 launch_mom_colour_pass(s, is_u, c, stop);
+        hipLaunchKernelGGL(k_field_unpack, dim3(cdiv(s.nc, BX), cdiv(s.nr, BY)), dim3(BX, BY), 0,
+                           s.stream, s.fpk, s.fbuf, m.x, s.ld, s.nr, s.nc, stop);
+        SG_HIP(hipGetLastError());
         return;
     }
     // Jacobi batch: full copy, then alternate so that the last pass writes the field.
-    const MomArgs m = mom_args(s, is_u);
     hipLaunchKernelGGL(k_copy, dim3(1024), dim3(256), 0, s.stream, m.x, s.scratch, s.n, stop);
     SG_HIP(hipGetLastError());
     if (m.i1 < m.i0 || m.j1 < m.j0)
@@ -388,6 +436,20 @@ void launch_mom_sweeps(Solver& s, bool is_u, int nsweeps, const int* stop) {
                            src, dst, stop);
         SG_HIP(hipGetLastError());
     }
+}
+
+// Whole-array field pack geometry (all rows and columns).
+Pack make_field_pack(const sg_params& p) {
+    Pack k{};
+    k.ncol = ncolours(p);
+    k.ai0 = 0;
+    k.aj0 = 0;
+    k.ai1 = p.nrows - 1;
+    k.aj1 = p.ncols - 1;
+    k.rows = p.nrows;
+    k.kp = ((p.ncols + k.ncol - 1) / k.ncol + 15) / 16 * 16;
+    k.astride = static_cast<size_t>(k.ncol) * k.rows * k.kp;
+    return k;
 }
 
 // Row-major array id <-> packed layout (coefficient ids only); p uploads refresh the pressure

@@ -378,7 +378,7 @@ void launch_solve_p(Solver& s, const int* stop, bool eager) {
         s.cg_total += s.cg_last;
         return;
     case SG_P_MG:
-        launch_mg_solve(s, stop);
+        launch_mg_solve(s, stop, false);
         return;
     default:
         throw ArgError("backend = gpu does not run pressure_solver = gs_lex (a serial sweep)");

@@ -4,53 +4,105 @@
 // associative, so this order is what makes every sum that feeds back into a field (mean
 // removals, CG alpha/beta) and every residual bit-identical to the CPU.
 //
-// Each row is summed by one thread, with its columns staged through LDS in 64-wide chunks so
-// that global loads stay coalesced; the row sums are then added by one thread.
+// One launch. A workgroup of 256 threads owns RW rows: all threads load a RW x CW chunk with
+// coalesced, independent loads into LDS, then RW threads add their rows' chunk in order. The
+// additions of a row are inherently sequential (~1 FP64 add per few cycles); spreading the rows
+// over many workgroups keeps every CU loading while they run. The last workgroup to finish (a
+// device-scope atomic ticket) adds the row sums in order and re-arms the ticket.
 #include "sg_internal.hpp"
 
 namespace sg {
 
 namespace {
 
-constexpr int RT = 64; // rows per workgroup (one wave)
-constexpr int CW = 64; // columns per LDS chunk
+constexpr int NT = 256;  // threads per workgroup
+constexpr int RW = 4;    // rows per workgroup
+constexpr int CW = 1024; // columns per LDS chunk
+constexpr int PAD = 2;   // row padding (doubles) against bank conflicts
+constexpr int LOADS = RW * CW / NT;
 
-__global__ void __launch_bounds__(RT)
-    k_rowsum(const double* a, long pitch, int i0, int nrow, int j0, int w, double* rows,
-             const int* stop) {
+// s += v[0] + ... in order, from LDS, in batches of 8 independent reads.
+__device__ __forceinline__ double add_in_order(double s, const double* v, int n) {
+    int c = 0;
+    for (; c + 8 <= n; c += 8) {
+        const double a0 = v[c], a1 = v[c + 1], a2 = v[c + 2], a3 = v[c + 3];
+        const double a4 = v[c + 4], a5 = v[c + 5], a6 = v[c + 6], a7 = v[c + 7];
+        s += a0;
+        s += a1;
+        s += a2;
+        s += a3;
+        s += a4;
+        s += a5;
+        s += a6;
+        s += a7;
+    }
+    for (; c < n; ++c)
+        s += v[c];
+    return s;
+}
+
+__global__ void __launch_bounds__(NT)
+    k_ordered_sum(const double* a, long pitch, int i0, int nrow, int j0, int w, double* rows,
+                  unsigned* ticket, double* slot, const int* stop) {
     if (stopped(stop))
         return;
-    __shared__ double tile[RT][CW + 1];
-    const int r0 = blockIdx.x * RT;
+    __shared__ double tile[RW][CW + PAD];
+    __shared__ bool last;
+    const int r0 = blockIdx.x * RW;
     const int t = threadIdx.x;
     double s = 0.0;
     for (int c0 = 0; c0 < w; c0 += CW) {
         const int cw = min(CW, w - c0);
-        // Coalesced load: lane t reads column c0 + t of each of the block's rows.
-        for (int rr = 0; rr < RT; ++rr) {
-            const int row = r0 + rr;
-            if (row < nrow && t < cw)
-                tile[rr][t] = a[static_cast<size_t>(i0 + row) * pitch + j0 + c0 + t];
+        double v[LOADS];
+#pragma unroll
+        for (int k = 0; k < LOADS; ++k) {
+            const int e = k * NT + t, r = e / CW, c = e - r * CW;
+            v[k] = (c < cw && r0 + r < nrow)
+                       ? a[static_cast<size_t>(i0 + r0 + r) * pitch + j0 + c0 + c]
+                       : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < LOADS; ++k) {
+            const int e = k * NT + t, r = e / CW, c = e - r * CW;
+            tile[r][c] = v[k];
         }
         __syncthreads();
-        if (r0 + t < nrow)
-            for (int c = 0; c < cw; ++c)
-                s += tile[t][c];
+        if (t < RW && r0 + t < nrow)
+            s = add_in_order(s, tile[t], cw);
         __syncthreads();
     }
-    if (r0 + t < nrow)
+    if (t < RW && r0 + t < nrow)
         rows[r0 + t] = s;
+    __threadfence();
+    __syncthreads();
+    if (t == 0)
+        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!last)
+        return;
+    __threadfence();
+    // Rows ascending from 0.0, staged through LDS, added by thread 0.
+    double* buf = &tile[0][0];
+    const int chunk = RW * (CW + PAD);
+    double total = 0.0;
+    for (int b0 = 0; b0 < nrow; b0 += chunk) {
+        const int n = min(chunk, nrow - b0);
+        for (int k = t; k < n; k += NT)
+            buf[k] = __builtin_nontemporal_load(&rows[b0 + k]);
+        __syncthreads();
+        if (t == 0)
+            total = add_in_order(total, buf, n);
+        __syncthreads();
+    }
+    if (t == 0) {
+        *slot = total;
+        *ticket = 0u;
+    }
 }
 
-__global__ void k_rowtotal(const double* rows, int nrow, double* slot, const int* stop) {
-    if (stopped(stop))
-        return;
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        double s = 0.0;
-        for (int r = 0; r < nrow; ++r)
-            s += rows[r];
-        *slot = s;
-    }
+__global__ void k_zero_slot(double* slot, const int* stop) {
+    if (!stopped(stop) && threadIdx.x == 0)
+        *slot = 0.0;
 }
 
 } // namespace
@@ -58,13 +110,13 @@ __global__ void k_rowtotal(const double* rows, int nrow, double* slot, const int
 void launch_ordered_sum(Solver& s, const double* a, int pitch, int i0, int i1, int j0, int j1,
                         double* slot, const int* stop) {
     const int nrow = i1 - i0 + 1, w = j1 - j0 + 1;
-    if (nrow > 0 && w > 0) {
-        hipLaunchKernelGGL(k_rowsum, dim3(cdiv(nrow, RT)), dim3(RT), 0, s.stream, a,
-                           static_cast<long>(pitch), i0, nrow, j0, w, s.rowbuf, stop);
-        SG_HIP(hipGetLastError());
+    if (nrow <= 0 || w <= 0) {
+        hipLaunchKernelGGL(k_zero_slot, dim3(1), dim3(64), 0, s.stream, slot, stop);
+    } else {
+        hipLaunchKernelGGL(k_ordered_sum, dim3(cdiv(nrow, RW)), dim3(NT), 0, s.stream, a,
+                           static_cast<long>(pitch), i0, nrow, j0, w, s.rowbuf, s.ticket, slot,
+                           stop);
     }
-    hipLaunchKernelGGL(k_rowtotal, dim3(1), dim3(64), 0, s.stream, s.rowbuf,
-                       (nrow > 0 && w > 0) ? nrow : 0, slot, stop);
     SG_HIP(hipGetLastError());
 }
 

@@ -61,7 +61,7 @@ const std::set<std::string>& known_keys() {
         "pressure_tolerance", "pressure_maxit", "mg_cycles", "mg_pre", "mg_post", "mg_omega",
         "mpi_grid", "halo_overlap", "write_fields",
         // additions
-        "mg_precision", "mg_coarsest", "mg_dc_maxit", "mg_dc_tol", "pressure_warm_start",
+        "mg_precision", "mg_coarsest", "mg_dc_maxit", "pressure_warm_start",
         "warm_start_weights", "check_every"};
     return k;
 }
@@ -151,7 +151,6 @@ Params make_params(const ConfigMap& cfg) {
             p.mg_precision = as_enum(k, v, {{"fp64", SG_MG_FP64}, {"mixed", SG_MG_MIXED}});
         else if (k == "mg_coarsest") p.mg_coarsest = as_int(k, v);
         else if (k == "mg_dc_maxit") p.mg_dc_maxit = as_int(k, v);
-        else if (k == "mg_dc_tol") p.mg_dc_tol = as_double(k, v);
         else if (k == "pressure_warm_start")
             p.pressure_warm_start =
                 as_enum(k, v, {{"none", SG_WARM_NONE}, {"learned", SG_WARM_LEARNED}});

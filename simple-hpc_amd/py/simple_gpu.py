@@ -50,7 +50,6 @@ class SgParams(C.Structure):
         ("nrows", C.c_int), ("ncols", C.c_int),
         ("dx", C.c_double), ("dy", C.c_double), ("nu", C.c_double),
         ("mg_precision", C.c_int), ("mg_coarsest", C.c_int), ("mg_dc_maxit", C.c_int),
-        ("mg_dc_tol", C.c_double),
         ("pressure_warm_start", C.c_int), ("check_every", C.c_int),
     ]
 

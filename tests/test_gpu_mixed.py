@@ -1,0 +1,75 @@
+"""Mixed-precision multigrid (mg_precision = mixed): an FP32 V-cycle inside FP64 defect
+correction. Not bit-identical by design; the checks are against the FP64 oracle:
+ - one p' solve on a realistic system: the FP32 V-cycle's p' equals the FP64 V-cycle's to FP32
+   accuracy (relative 1e-5 of max |p'|), and a second defect-correction step tightens it;
+ - whole runs to tolerance converge, in an outer-iteration count within 2% of the FP64 run, to
+   fields within 1e-6 (u, v) and 1e-5 (p, after removing its mean) of the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+from conftest import with_keys
+
+pytestmark = pytest.mark.gpu
+
+import pyoracle as po  # noqa: E402
+import simple_gpu as sg  # noqa: E402
+
+
+def _system(n, re, **kv):
+    cfg = with_keys(f"N_cells_x = {n}\nN_cells_y = {n}\nRe = {re}\n",
+                    momentum_solver="gs_multicolour", pressure_solver="mg", backend="gpu", **kv)
+    return sg.params_from_config(cfg)[0]
+
+
+def _interior(p, a):
+    return a[p.H:p.NX + p.H, p.H + 1:p.NY + p.H + 1]
+
+
+@pytest.mark.parametrize("dc", [1, 2])
+def test_mixed_pc_solve_close_to_fp64(dc):
+    # dc defect-correction steps of one FP32 V-cycle each are dc V-cycles: compare with FP64
+    # mg_cycles = dc (a V-cycle is a linear stationary iteration x += B (b - A x)).
+    p64 = _system(128, 1000, mg_cycles=dc)
+    cpu = po.Oracle(p64)
+    for _ in range(5):
+        cpu.outer_iteration()
+    cpu.assemble_u(); cpu.bc_u(); cpu.residual_u(); cpu.sweep_u(4)
+    cpu.assemble_v(); cpu.bc_v(); cpu.residual_v(); cpu.sweep_v(4)
+    cpu.assemble_p()
+    state = {n: cpu.get(n) for n in sg.FIELDS}
+    cpu.solve_p()
+    x64 = _interior(p64, cpu.get("pc"))
+    pm = _system(128, 1000, mg_precision="mixed", mg_dc_maxit=dc)
+    g = sg.Solver(pm)
+    for n, a in state.items():
+        g.set(n, a)
+    g.solve_p()
+    xm = _interior(pm, g.get("pc"))
+    err = np.abs(xm - x64).max() / np.abs(x64).max()
+    print(f"mixed p' (dc={dc}) vs fp64 p': rel max err {err:.3e}", flush=True)
+    assert err < 1e-4
+    # FP64 pieces: b is made mean-free in FP64 exactly as in the reference.
+    assert np.array_equal(g.get("b"), cpu.get("b"))
+
+
+def test_mixed_run_to_tolerance_matches_fp64():
+    n, re, tol = 64, 1000, 1e-5
+    p64 = _system(n, re, error_tolerance=tol, maxit=20000)
+    cpu = po.Oracle(p64)
+    n64, log64, conv64 = cpu.run()
+    pm = _system(n, re, error_tolerance=tol, maxit=20000, mg_precision="mixed")
+    g = sg.Solver(pm)
+    total, conv = 0, False
+    while total < pm.maxit and not conv:
+        k, _, conv = g.advance(1000)
+        total += k
+    du = np.abs(g.get("u") - cpu.get("u")).max()
+    dv = np.abs(g.get("v") - cpu.get("v")).max()
+    pg, pc = _interior(pm, g.get("p")), _interior(pm, cpu.get("p"))
+    dp = np.abs((pg - pg.mean()) - (pc - pc.mean())).max()
+    print(f"\nmixed {n}^2: {total} iterations vs fp64 {n64}; max|du| {du:.2e} max|dv| {dv:.2e} "
+          f"max|dp| {dp:.2e}", flush=True)
+    assert conv64 and conv
+    assert abs(total - n64) <= 0.02 * n64
+    assert du < 1e-6 and dv < 1e-6 and dp < 1e-5

@@ -87,7 +87,7 @@ def test_every_reference_key_is_accepted():
         "pressure_maxit": "5", "mg_cycles": "2", "mg_pre": "1", "mg_post": "3", "mg_omega": "1.5",
         "mpi_grid": "auto", "halo_overlap": "True", "write_fields": "False",
         # additions of this backend
-        "mg_precision": "mixed", "mg_coarsest": "8", "mg_dc_maxit": "3", "mg_dc_tol": "1e-3",
+        "mg_precision": "mixed", "mg_coarsest": "8", "mg_dc_maxit": "3",
         "pressure_warm_start": "learned", "warm_start_weights": "w.bin", "check_every": "50",
     }
     p, out = sg.params_from_config("".join(f"{k} = {v}\n" for k, v in keys.items()))
