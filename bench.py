@@ -45,6 +45,16 @@ def log(msg):
 def run_form(sg, n, form, mode, steps, warmup):
     p, _ = sg.params_from_config(config_text(n, form))
     s = sg.Solver(p)
+    s.advance(3)
+    tune = s.autotune(reps=10)
+    log(f"{n}^2 {form}: autotune -> {tune.strip().splitlines()[-1]}")
+    s.close()
+    s = sg.Solver(p)  # fresh fields; keep the tuned launch shapes
+    for line in tune.strip().splitlines()[-1:]:
+        f = line.split()
+        s.set_tuning(sg.TUNE_TAIL_THREADS, int(f[2]))
+        s.set_tuning(sg.TUNE_MID_CELLS, int(f[4]))
+        s.set_tuning(sg.TUNE_LDS_CELLS, int(f[6]))
     if mode == "fixed":
         s.advance(warmup)
         log(f"{n}^2 {form}: timing {steps} outer iterations")
@@ -52,7 +62,8 @@ def run_form(sg, n, form, mode, steps, warmup):
         done, lg, conv = s.advance(steps)
         t = time.perf_counter() - t0
         return s, {"form": form, "iterations": done, "seconds": t, "ms_per_iter": 1e3 * t / done,
-                   "last_residuals": [float(x) for x in lg[-1]]}
+                   "last_residuals": [float(x) for x in lg[-1]],
+                   "autotune_best": tune.strip().splitlines()[-1]}
     log(f"{n}^2 {form}: to tolerance")
     t0 = time.perf_counter()
     total, conv, last = 0, False, None
@@ -68,7 +79,8 @@ def run_form(sg, n, form, mode, steps, warmup):
             break
     t = time.perf_counter() - t0
     return s, {"form": form, "iterations": total, "seconds": t, "ms_per_iter": 1e3 * t / total,
-               "converged": bool(conv), "last_residuals": [float(x) for x in last]}
+               "converged": bool(conv), "last_residuals": [float(x) for x in last],
+               "autotune_best": tune.strip().splitlines()[-1]}
 
 
 def cpu_baseline(sg, n, seconds=12.0):

@@ -163,13 +163,28 @@ enum {
     SG_K_SWEEP_V,
     SG_K_MG_SMOOTH0,   /* one colour pass of the level-0 multigrid smoother */
     SG_K_ASSEMBLE_U,
+    SG_K_PSOLVE,       /* one complete p' solve (pressure_solver of the params) */
     SG_K_COUNT
 };
 int sg_kernel_probe(sg_solver* s, int kernel_id, int reps, double* mean_seconds,
                     double* bytes_per_launch, int* launches_per_iteration);
 
-/* Multigrid information: number of levels and statistics of the last solve. */
+/* Multigrid information: number of levels, and the red-black sweeps of the last coarsest-level
+ * solve. */
 int sg_mg_levels(sg_solver* s, int* nlevels);
+int sg_mg_coarse_sweeps(sg_solver* s, int* sweeps);
+
+/* Launch-shape knobs (no effect on results). SG_TUNE_TAIL_THREADS: threads of the one-workgroup
+ * multigrid tail (64..1024, multiple of 64); SG_TUNE_MID_CELLS: levels with at most this many
+ * cells run inside that workgroup; SG_TUNE_LDS_CELLS: levels with at most this many cells (that
+ * fit) live in LDS there. */
+enum { SG_TUNE_TAIL_THREADS = 0, SG_TUNE_MID_CELLS = 1, SG_TUNE_LDS_CELLS = 2, SG_TUNE_COUNT };
+int sg_set_tuning(sg_solver* s, int key, long value);
+int sg_get_tuning(sg_solver* s, int key, long* value);
+/* Autotuner: times the p' solve on the device (hipEvents around graph replays, reps each) for a
+ * grid of the knobs above on the solver's current state, keeps the fastest, restores every
+ * array, and writes a one-line-per-candidate report into report (may be NULL). */
+int sg_autotune(sg_solver* s, int reps, char* report, size_t len);
 
 #ifdef __cplusplus
 }

@@ -106,8 +106,10 @@ void ensure_log(Solver& s, int cap) {
 }
 
 // One outer iteration (src/solver.cpp:27-93), enqueued. ev: optional phase events (12).
+// The residual sums go to the side stream (they feed only the stop rule, which joins them).
 void enqueue_iteration(Solver& s, const int* stop, bool eager, hipEvent_t* ev) {
     const sg_params& p = s.p;
+    const bool side = !ev; // per-phase timing keeps every phase on the main stream
     auto mark = [&](int k) {
         if (ev)
             SG_HIP(hipEventRecord(ev[k], s.stream));
@@ -117,7 +119,7 @@ void enqueue_iteration(Solver& s, const int* stop, bool eager, hipEvent_t* ev) {
     mark(1);
     launch_bc(s, true, stop);
     mark(2);
-    launch_mom_residual(s, true, s.scal + S_RES_U, stop);
+    launch_mom_residual(s, true, s.scal + S_RES_U, stop, side);
     mark(3);
     launch_mom_sweeps(s, true, p.maxit_inner_u, stop, true);
     mark(4);
@@ -125,16 +127,18 @@ void enqueue_iteration(Solver& s, const int* stop, bool eager, hipEvent_t* ev) {
     mark(5);
     launch_bc(s, false, stop);
     mark(6);
-    launch_mom_residual(s, false, s.scal + S_RES_V, stop);
+    launch_mom_residual(s, false, s.scal + S_RES_V, stop, side);
     mark(7);
     launch_mom_sweeps(s, false, p.maxit_inner_v, stop, true);
     mark(8);
-    launch_assemble_p(s, s.scal + S_RES_P, stop);
+    launch_assemble_p(s, s.scal + S_RES_P, stop, side);
     mark(9);
     launch_solve_p(s, stop, eager);
     mark(10);
     launch_correct(s, stop);
     mark(11);
+    if (side)
+        join_side(s);
     hipLaunchKernelGGL(k_stop_rule, dim3(1), dim3(64), 0, s.stream, s.ctl, s.scal, s.dlog,
                        s.log_cap, p.residual_normalisation, p.error_tolerance);
     SG_HIP(hipGetLastError());
@@ -353,7 +357,14 @@ int sg_create(const sg_params* pin, const double* u, const double* v, const doub
         s.cell = dalloc<double>(s, 4 * static_cast<size_t>(std::max(p.NX + 2, 1)) * (p.NY + 2));
         s.rowbuf = dalloc<double>(s, 4 * static_cast<size_t>(s.nr));
         s.scal = dalloc<double>(s, S_COUNT);
-        s.ticket = dalloc<unsigned>(s, 1);
+        s.ticket = dalloc<unsigned>(s, 4);
+        for (auto& c : s.rcell)
+            c = dalloc<double>(s, static_cast<size_t>(p.NX + 2) * (p.NY + 2));
+        SG_HIP(hipStreamCreateWithFlags(&s.side, hipStreamNonBlocking));
+        for (int k = 0; k < 3; ++k) {
+            SG_HIP(hipEventCreateWithFlags(&s.fork[k], hipEventDisableTiming));
+            SG_HIP(hipEventCreateWithFlags(&s.join[k], hipEventDisableTiming));
+        }
         s.ctl = dalloc<Ctl>(s, 1);
         s.never = dalloc<int>(s, 1);
         if (p.pressure_solver == SG_P_CG)
@@ -405,8 +416,18 @@ int sg_destroy(sg_solver* h) {
         (void)hipEventDestroy(s->kt0);
     if (s->kt1)
         (void)hipEventDestroy(s->kt1);
+    if (s->side)
+        (void)hipStreamSynchronize(s->side);
+    for (int k = 0; k < 3; ++k) {
+        if (s->fork[k])
+            (void)hipEventDestroy(s->fork[k]);
+        if (s->join[k])
+            (void)hipEventDestroy(s->join[k]);
+    }
     for (void* q : s->owned)
         (void)hipFree(q);
+    if (s->side)
+        (void)hipStreamDestroy(s->side);
     if (s->stream)
         (void)hipStreamDestroy(s->stream);
     delete s;
@@ -465,7 +486,7 @@ int sg_bc_u(sg_solver* h) {
 }
 int sg_residual_u(sg_solver* h, double* res) {
     return phase_res(h, res, S_RES_U,
-                     [](Solver& s) { launch_mom_residual(s, true, s.scal + S_RES_U, s.never); });
+                     [](Solver& s) { launch_mom_residual(s, true, s.scal + S_RES_U, s.never, false); });
 }
 int sg_sweep_u(sg_solver* h, int n) {
     return guard([&] {
@@ -482,7 +503,7 @@ int sg_bc_v(sg_solver* h) {
 }
 int sg_residual_v(sg_solver* h, double* res) {
     return phase_res(h, res, S_RES_V,
-                     [](Solver& s) { launch_mom_residual(s, false, s.scal + S_RES_V, s.never); });
+                     [](Solver& s) { launch_mom_residual(s, false, s.scal + S_RES_V, s.never, false); });
 }
 int sg_sweep_v(sg_solver* h, int n) {
     return guard([&] {
@@ -493,7 +514,7 @@ int sg_sweep_v(sg_solver* h, int n) {
 }
 int sg_assemble_p(sg_solver* h, double* res) {
     return phase_res(h, res, S_RES_P,
-                     [](Solver& s) { launch_assemble_p(s, s.scal + S_RES_P, s.never); });
+                     [](Solver& s) { launch_assemble_p(s, s.scal + S_RES_P, s.never, false); });
 }
 int sg_solve_p(sg_solver* h) {
     return phase(h, [](Solver& s) { launch_solve_p(s, s.never, true); });
@@ -589,6 +610,12 @@ int sg_kernel_probe(sg_solver* h, int kid, int reps, double* mean_s, double* byt
             one = [&s, is_u, ncol](int k) { launch_mom_colour_pass(s, is_u, k % ncol, s.never); };
             break;
         }
+        case SG_K_PSOLVE:
+            cells = static_cast<double>(p.NX) * p.NY;
+            bpc = 0.0;
+            per = 1;
+            one = [&s](int) { launch_solve_p(s, s.never, false); };
+            break;
         case SG_K_ASSEMBLE_U:
             cells = static_cast<double>(p.NX) * (p.NY + 1);
             bpc = 88.0; // u, v read; 9 coefficients written
@@ -617,6 +644,122 @@ int sg_kernel_probe(sg_solver* h, int kid, int reps, double* mean_s, double* byt
 
 int sg_mg_levels(sg_solver* h, int* n) {
     return guard([&] { *n = static_cast<int>(S(h).mg.size()); });
+}
+
+int sg_mg_coarse_sweeps(sg_solver* h, int* n) {
+    return guard([&] { *n = read_ctl(S(h)).coarse_sweeps; });
+}
+
+static long* knob(Solver& s, int key) {
+    static long threads;
+    switch (key) {
+    case SG_TUNE_MID_CELLS: return &s.mid_cells;
+    case SG_TUNE_LDS_CELLS: return &s.lds_cells;
+    case SG_TUNE_TAIL_THREADS: threads = s.tail_threads; return &threads;
+    default: throw ArgError("unknown tuning key");
+    }
+}
+
+int sg_set_tuning(sg_solver* h, int key, long value) {
+    return guard([&] {
+        Solver& s = S(h);
+        if (key == SG_TUNE_TAIL_THREADS) {
+            if (value < 64 || value > 1024 || value % 64)
+                throw ArgError("tail threads must be a multiple of 64 in [64, 1024]");
+            s.tail_threads = static_cast<int>(value);
+        } else {
+            if (value < 0)
+                throw ArgError("tuning value must be >= 0");
+            *knob(s, key) = value;
+        }
+        if (s.graph) { // launch shapes are baked into the captured graph
+            SG_HIP(hipGraphExecDestroy(s.graph));
+            s.graph = nullptr;
+        }
+    });
+}
+
+int sg_get_tuning(sg_solver* h, int key, long* value) {
+    return guard([&] { *value = *knob(S(h), key); });
+}
+
+// Device time of one p' solve with the current knobs: the solve captured once and replayed.
+static double time_psolve(Solver& s, int reps) {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    SG_HIP(hipStreamBeginCapture(s.stream, hipStreamCaptureModeRelaxed));
+    launch_solve_p(s, s.never, false);
+    SG_HIP(hipStreamEndCapture(s.stream, &g));
+    SG_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(g);
+    SG_HIP(hipGraphLaunch(ge, s.stream)); // warm
+    SG_HIP(hipEventRecord(s.kt0, s.stream));
+    for (int k = 0; k < reps; ++k)
+        SG_HIP(hipGraphLaunch(ge, s.stream));
+    SG_HIP(hipEventRecord(s.kt1, s.stream));
+    SG_HIP(hipStreamSynchronize(s.stream));
+    float ms = 0.f;
+    SG_HIP(hipEventElapsedTime(&ms, s.kt0, s.kt1));
+    (void)hipGraphExecDestroy(ge);
+    return 1e-3 * ms / reps;
+}
+
+int sg_autotune(sg_solver* h, int reps, char* report, size_t len) {
+    return guard([&] {
+        Solver& s = S(h);
+        if (s.p.pressure_solver != SG_P_MG)
+            throw ArgError("sg_autotune tunes the multigrid p' solve (pressure_solver = mg)");
+        if (reps < 1)
+            throw ArgError("reps must be >= 1");
+        // Keep b and p' (the solve rewrites both) to restore them afterwards.
+        double *kb = nullptr, *kp = nullptr;
+        SG_HIP(hipMalloc(reinterpret_cast<void**>(&kb), sizeof(double) * s.n));
+        SG_HIP(hipMalloc(reinterpret_cast<void**>(&kp), sizeof(double) * s.n));
+        SG_HIP(hipMemcpyAsync(kb, s.F(SG_F_B), sizeof(double) * s.n, hipMemcpyDeviceToDevice, s.stream));
+        SG_HIP(hipMemcpyAsync(kp, s.F(SG_F_PC), sizeof(double) * s.n, hipMemcpyDeviceToDevice, s.stream));
+        std::string rep;
+        double best = 1e30;
+        int bt = s.tail_threads;
+        long bm = s.mid_cells, bl = s.lds_cells;
+        for (int threads : {64, 256, 1024})
+            for (long mid : {0L, 4096L, 16384L, 65536L})
+                for (long lds : {256L, 1024L, 4096L}) {
+                    s.tail_threads = threads;
+                    s.mid_cells = mid;
+                    s.lds_cells = lds;
+                    const double t = time_psolve(s, reps);
+                    char line[160];
+                    std::snprintf(line, sizeof line,
+                                  "tail_threads %4d mid_cells %6ld lds_cells %5ld : %9.2f us\n",
+                                  threads, mid, lds, 1e6 * t);
+                    rep += line;
+                    if (t < best) {
+                        best = t;
+                        bt = threads, bm = mid, bl = lds;
+                    }
+                }
+        s.tail_threads = bt;
+        s.mid_cells = bm;
+        s.lds_cells = bl;
+        char line[160];
+        std::snprintf(line, sizeof line,
+                      "best: tail_threads %d mid_cells %ld lds_cells %ld : %.2f us\n", bt, bm, bl,
+                      1e6 * best);
+        rep += line;
+        SG_HIP(hipMemcpyAsync(s.F(SG_F_B), kb, sizeof(double) * s.n, hipMemcpyDeviceToDevice, s.stream));
+        SG_HIP(hipMemcpyAsync(s.F(SG_F_PC), kp, sizeof(double) * s.n, hipMemcpyDeviceToDevice, s.stream));
+        sync(s);
+        (void)hipFree(kb);
+        (void)hipFree(kp);
+        if (s.graph) {
+            SG_HIP(hipGraphExecDestroy(s.graph));
+            s.graph = nullptr;
+        }
+        if (report && len) {
+            std::strncpy(report, rep.c_str(), len - 1);
+            report[len - 1] = 0;
+        }
+    });
 }
 
 } // extern "C"

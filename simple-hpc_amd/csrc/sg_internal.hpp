@@ -90,8 +90,11 @@ struct Solver {
     double* f[SG_F_COUNT] = {};
     double* scratch = nullptr;         // Jacobi previous iterate
     double* cell = nullptr;            // per-cell values for ordered sums (4 planes of n)
-    double* rowbuf = nullptr;          // row sums (4 x nr)
-    unsigned* ticket = nullptr;        // last-workgroup ticket of the ordered sums (kept 0)
+    double* rowbuf = nullptr;          // row sums, one nr-slice per ordered-sum lane (4 lanes)
+    unsigned* ticket = nullptr;        // per lane: last-workgroup ticket of the ordered sums (kept 0)
+    double* rcell[3] = {};             // |R| cells of res_u, res_v, res_p (summed on the side stream)
+    hipStream_t side = nullptr;        // side stream: the residual sums, off the critical path
+    hipEvent_t fork[3] = {}, join[3] = {};
     double* scal = nullptr;            // S_COUNT device scalars
     double* cg[4] = {};                // CG r, z, d, q
     Ctl* ctl = nullptr;                // device control block
@@ -100,7 +103,10 @@ struct Solver {
     int log_cap = 0;
     std::vector<MgLevel> mg;
     double* mg_r0 = nullptr;           // level-0 residual
-    long tail_cells = 4096;            // levels with at most this many cells run in one workgroup
+    // Multigrid tail (tunable; sg_tune): levels with at most mid_cells cells run in one workgroup
+    // of tail_threads threads, those with at most lds_cells cells (and fitting lds_budget) in LDS.
+    long mid_cells = 16384, lds_cells = 4096;
+    int tail_threads = 256;
     size_t lds_budget = 65536 - 1024;  // dynamic LDS of the multigrid tail (set from the device)
     std::vector<void*> owned;          // allocations to free
     Pack pku{}, pkv{};                 // packed momentum coefficients
@@ -123,7 +129,7 @@ struct Solver {
 // momentum (sg_momentum.hip)
 void launch_assemble(Solver& s, bool is_u, const int* stop);
 void launch_bc(Solver& s, bool is_u, const int* stop);
-void launch_mom_residual(Solver& s, bool is_u, double* slot, const int* stop);
+void launch_mom_residual(Solver& s, bool is_u, double* slot, const int* stop, bool side);
 void launch_mom_sweeps(Solver& s, bool is_u, int nsweeps, const int* stop, bool packed);
 void launch_mom_pack(Solver& s, bool is_u, const int* stop);
 Pack make_field_pack(const sg_params& p);
@@ -132,14 +138,21 @@ int ncolours(const sg_params& p);
 Pack make_pack(const sg_params& p, bool is_u);
 void sync_packed(Solver& s, int field_id, bool to_packed);
 // pressure (sg_pressure.hip)
-void launch_assemble_p(Solver& s, double* slot, const int* stop);
+void launch_assemble_p(Solver& s, double* slot, const int* stop, bool side);
 void launch_solve_p(Solver& s, const int* stop, bool eager);
 void launch_correct(Solver& s, const int* stop);
 void launch_fill(Solver& s, double* a, size_t n, double v, const int* stop);
 // ordered sums (sg_reduce.hip): rows [i0, i1], columns [j0, j1] of a pitched array, row sums j
 // ascending from 0.0 into rowbuf, then rows ascending from 0.0 into *slot.
+// lane: which row-sum slice and ticket (0 on the main stream; 1..3 on the side stream); the sum
+// is enqueued on `stream` (the main stream if null).
 void launch_ordered_sum(Solver& s, const double* a, int pitch, int i0, int i1, int j0, int j1,
-                        double* slot, const int* stop);
+                        double* slot, const int* stop, int lane = 0, hipStream_t stream = nullptr);
+// Enqueue an ordered sum of a compact nrow x w cell buffer on the side stream (lane 1..3), forked
+// from and later joined (join_side) into the main stream.
+void side_sum(Solver& s, int lane, const double* cell, int nrow, int w, double* slot,
+              const int* stop);
+void join_side(Solver& s);
 // multigrid (sg_mg.hip)
 void mg_setup(Solver& s);
 void launch_mg_solve(Solver& s, const int* stop, bool has_guess);

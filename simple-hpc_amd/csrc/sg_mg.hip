@@ -28,7 +28,6 @@ namespace sg {
 namespace {
 
 constexpr int BX = 64, BY = 4;
-constexpr int TT = 1024;     // tail workgroup size
 constexpr int MAXLEV = 24;
 
 template <class T> struct Lv {
@@ -162,20 +161,23 @@ __global__ void __launch_bounds__(BX* BY) k_mg_prolong(Lv<T> C, Lv<T> F, T omega
             omega * C.x[static_cast<size_t>(i / 2) * C.stride + j / 2];
 }
 
-// ---- the tail: one workgroup, every level in LDS ----
+// ---- the tail: one workgroup for the small levels ----
 //
-// The tail levels t0..nl-1 live in LDS for the whole launch: level t0's operator is loaded
-// (t0 = 0) or built from level t0-1's global operator, the coarser ones are built in LDS, b and x
-// of level t0 come from global memory and x goes back at the end. Each level block holds 8
-// arrays (AE, AW, AN, AS, AP, b, x, r) of (nx+2) x (ny+2) elements with a zero halo.
+// Levels tm..nl-1 run in one workgroup, one launch per V-cycle, with barriers between steps:
+// levels tm..t0-1 in global memory (they stay in L2), levels t0..nl-1 in LDS for the whole
+// launch. A level in LDS has 8 arrays (AE, AW, AN, AS, AP, b, x, r) of (nx+2) x (ny+2) elements
+// with a zero halo; its operator is built in LDS from the level above (or loaded, for level 0),
+// and when the workgroup starts at an LDS level (tm = t0) that level's b and x come from global
+// memory and x goes back at the end. t0 = nl: no level in LDS.
 
 template <class T> struct Tail {
-    Lv<T> G[MAXLEV]; // global views (level t0-1's operator, level t0's b and x)
-    int off[MAXLEV]; // LDS element offset of each tail level's block
-    int t0, nl;      // first tail level, number of levels
+    Lv<T> G[MAXLEV]; // global views of every level
+    int off[MAXLEV]; // LDS element offset of each LDS level's block
+    int tm, t0, nl;  // first workgroup level, first LDS level, number of levels
     int pre, post;
     T omega;
-    int lds_elems;   // elements of all level blocks; the row-sum scratch follows
+    int lds_elems;   // elements of all LDS level blocks; the row-sum scratch follows
+    T* grows;        // row-sum scratch in global memory (t0 = nl)
 };
 
 template <class T> __device__ __forceinline__ Lv<T> lds_view(const Tail<T>& t, T* sm, int l) {
@@ -189,7 +191,7 @@ template <class T> __device__ void wg_smooth(const Lv<T>& L, int sweeps) {
     const int half = (L.ny + 1) / 2, n = L.nx * half;
     for (int k = 0; k < sweeps; ++k)
         for (int c = 0; c < 2; ++c) {
-            for (int e = threadIdx.x; e < n; e += TT) {
+            for (int e = threadIdx.x; e < n; e += blockDim.x) {
                 const int I = e / half;
                 const int J = ((I + c) & 1) + 2 * (e - I * half);
                 if (J < L.ny)
@@ -200,7 +202,7 @@ template <class T> __device__ void wg_smooth(const Lv<T>& L, int sweeps) {
 }
 
 template <class T> __device__ void wg_resid(const Lv<T>& L) {
-    for (int e = threadIdx.x; e < L.nx * L.ny; e += TT) {
+    for (int e = threadIdx.x; e < L.nx * L.ny; e += blockDim.x) {
         const int I = e / L.ny, J = e - I * L.ny;
         const size_t q = static_cast<size_t>(I) * L.stride + J;
         L.r[q] = resid_cell(L, q);
@@ -212,7 +214,7 @@ template <class T> __device__ void wg_resid(const Lv<T>& L) {
 // (I ascending from 0.0). which: 0 b, 1 b*b, 2 r*r, 3 x.
 template <class T> __device__ T wg_det_sum(const Lv<T>& L, T* rows, int which) {
     __shared__ T total;
-    for (int I = threadIdx.x; I < L.nx; I += TT) {
+    for (int I = threadIdx.x; I < L.nx; I += blockDim.x) {
         T s = T(0);
         const size_t r0 = static_cast<size_t>(I) * L.stride;
         for (int J = 0; J < L.ny; ++J) {
@@ -237,7 +239,7 @@ template <class T> __device__ T wg_det_sum(const Lv<T>& L, T* rows, int which) {
 }
 
 template <class T> __device__ void wg_sub(const Lv<T>& L, T* x, T m) {
-    for (int e = threadIdx.x; e < L.nx * L.ny; e += TT) {
+    for (int e = threadIdx.x; e < L.nx * L.ny; e += blockDim.x) {
         const int I = e / L.ny, J = e - I * L.ny;
         x[static_cast<size_t>(I) * L.stride + J] -= m;
     }
@@ -267,60 +269,58 @@ template <class T> __device__ int wg_coarse_solve(const Lv<T>& L, T* rows) {
 }
 
 template <class T> __device__ void wg_build(const Lv<T>& F, const Lv<T>& C) {
-    for (int e = threadIdx.x; e < C.nx * C.ny; e += TT)
+    for (int e = threadIdx.x; e < C.nx * C.ny; e += blockDim.x)
         build_cell(F, C, e / C.ny, e % C.ny);
     __syncthreads();
 }
 
-// Coarsest level too large for LDS (a grid that does not coarsen): the same solve on the global
-// arrays, still one workgroup.
-template <class T>
-__global__ void __launch_bounds__(TT) k_mg_coarse_global(Lv<T> L, T* rows, Ctl* ctl, const int* stop) {
-    if (stopped(stop))
-        return;
-    const int sweeps = wg_coarse_solve(L, rows);
-    if (threadIdx.x == 0)
-        ctl->coarse_sweeps = sweeps;
+template <class T> __device__ void wg_copy_level(const Lv<T>& D, const Lv<T>& S, bool coefs) {
+    for (int e = threadIdx.x; e < D.nx * D.ny; e += blockDim.x) {
+        const int I = e / D.ny, J = e - I * D.ny;
+        const size_t d = static_cast<size_t>(I) * D.stride + J, g = static_cast<size_t>(I) * S.stride + J;
+        if (coefs) {
+            D.AE[d] = S.AE[g];
+            D.AW[d] = S.AW[g];
+            D.AN[d] = S.AN[g];
+            D.AS[d] = S.AS[g];
+            D.AP[d] = S.AP[g];
+        }
+        D.b[d] = S.b[g];
+        D.x[d] = S.x[g];
+    }
+    __syncthreads();
 }
 
 template <class T>
-__global__ void __launch_bounds__(TT) k_mg_tail(Tail<T> t, Ctl* ctl, const int* stop) {
+__global__ void __launch_bounds__(1024) k_mg_tail(Tail<T> t, Ctl* ctl, const int* stop) {
     if (stopped(stop))
         return;
     extern __shared__ __align__(16) unsigned char smem[];
     T* sm = reinterpret_cast<T*>(smem);
-    for (int e = threadIdx.x; e < t.lds_elems; e += TT)
+    for (int e = threadIdx.x; e < t.lds_elems; e += blockDim.x)
         sm[e] = T(0);
     __syncthreads();
-    T* rows = sm + t.lds_elems;
+    T* rows = t.t0 < t.nl ? sm + t.lds_elems : t.grows;
     Lv<T> L[MAXLEV];
-    for (int l = t.t0; l < t.nl; ++l)
-        L[l] = lds_view(t, sm, l);
-    // Level t0: operator (loaded, or built from the global level above), b and x.
-    const Lv<T> &G0 = t.G[t.t0], &D = L[t.t0];
-    if (t.t0 > 0)
-        wg_build(t.G[t.t0 - 1], D);
-    for (int e = threadIdx.x; e < D.nx * D.ny; e += TT) {
-        const int I = e / D.ny, J = e - I * D.ny;
-        const size_t g = static_cast<size_t>(I) * G0.stride + J, q = static_cast<size_t>(I) * D.stride + J;
-        if (t.t0 == 0) {
-            D.AE[q] = G0.AE[g];
-            D.AW[q] = G0.AW[g];
-            D.AN[q] = G0.AN[g];
-            D.AS[q] = G0.AS[g];
-            D.AP[q] = G0.AP[g];
+    for (int l = t.tm; l < t.nl; ++l)
+        L[l] = l < t.t0 ? t.G[l] : lds_view(t, sm, l);
+    // Operators: global levels from their (global) parents, LDS levels from theirs; level 0 in
+    // LDS is loaded.
+    for (int l = t.tm; l < t.nl; ++l) {
+        if (l == 0) {
+            if (t.t0 == 0)
+                wg_copy_level(L[0], t.G[0], true);
+            continue;
         }
-        D.b[q] = G0.b[g];
-        D.x[q] = G0.x[g];
+        wg_build(l == t.tm ? t.G[l - 1] : L[l - 1], L[l]);
     }
-    __syncthreads();
-    for (int l = t.t0 + 1; l < t.nl; ++l)
-        wg_build(L[l - 1], L[l]);
-    for (int l = t.t0; l + 1 < t.nl; ++l) {
+    if (t.tm == t.t0 && t.t0 > 0)
+        wg_copy_level(L[t.tm], t.G[t.tm], false); // b and x of the first level
+    for (int l = t.tm; l + 1 < t.nl; ++l) {
         const Lv<T>&F = L[l], &C = L[l + 1];
         wg_smooth(F, t.pre);
         wg_resid(F);
-        for (int e = threadIdx.x; e < C.nx * C.ny; e += TT) {
+        for (int e = threadIdx.x; e < C.nx * C.ny; e += blockDim.x) {
             const int I = e / C.ny, J = e - I * C.ny;
             const long fs = F.stride;
             const size_t f = 2 * I * fs + 2 * J, c = static_cast<size_t>(I) * C.stride + J;
@@ -330,9 +330,9 @@ __global__ void __launch_bounds__(TT) k_mg_tail(Tail<T> t, Ctl* ctl, const int* 
         __syncthreads();
     }
     const int sweeps = wg_coarse_solve(L[t.nl - 1], rows);
-    for (int l = t.nl - 2; l >= t.t0; --l) {
+    for (int l = t.nl - 2; l >= t.tm; --l) {
         const Lv<T>&F = L[l], &C = L[l + 1];
-        for (int e = threadIdx.x; e < F.nx * F.ny; e += TT) {
+        for (int e = threadIdx.x; e < F.nx * F.ny; e += blockDim.x) {
             const int i = e / F.ny, j = e - i * F.ny;
             F.x[static_cast<size_t>(i) * F.stride + j] +=
                 t.omega * C.x[static_cast<size_t>(i / 2) * C.stride + j / 2];
@@ -340,13 +340,17 @@ __global__ void __launch_bounds__(TT) k_mg_tail(Tail<T> t, Ctl* ctl, const int* 
         __syncthreads();
         wg_smooth(F, t.post);
     }
-    // Level t0's x (and, for the coarsest level, its mean-free b) back to global memory.
-    for (int e = threadIdx.x; e < D.nx * D.ny; e += TT) {
-        const int I = e / D.ny, J = e - I * D.ny;
-        const size_t g = static_cast<size_t>(I) * G0.stride + J, q = static_cast<size_t>(I) * D.stride + J;
-        G0.x[g] = D.x[q];
-        if (t.t0 == t.nl - 1)
-            G0.b[g] = D.b[q];
+    if (t.tm == t.t0 && t.t0 < t.nl) {
+        // The first level was in LDS: its x (and, if it is the coarsest, its mean-free b) back.
+        const Lv<T>&D = L[t.tm], &G0 = t.G[t.tm];
+        for (int e = threadIdx.x; e < D.nx * D.ny; e += blockDim.x) {
+            const int I = e / D.ny, J = e - I * D.ny;
+            const size_t g = static_cast<size_t>(I) * G0.stride + J,
+                         q = static_cast<size_t>(I) * D.stride + J;
+            G0.x[g] = D.x[q];
+            if (t.tm == t.nl - 1)
+                G0.b[g] = D.b[q];
+        }
     }
     if (threadIdx.x == 0)
         ctl->coarse_sweeps = sweeps;
@@ -416,33 +420,43 @@ void smooth_passes(Solver& s, const Lv<T>& L, int sweeps, int first_col, const i
     }
 }
 
-// LDS bytes of a tail starting at level t0 (level blocks plus the row-sum scratch).
-template <class T> size_t tail_bytes(const Solver& s, int t0) {
+// LDS bytes of levels t0.. (level blocks plus the row-sum scratch).
+template <class T> size_t lds_bytes(const Solver& s, int t0) {
     size_t e = 0;
     int maxnx = 0;
     for (size_t l = t0; l < s.mg.size(); ++l) {
         e += 8 * static_cast<size_t>(s.mg[l].nx + 2) * (s.mg[l].ny + 2);
         maxnx = std::max(maxnx, s.mg[l].nx);
     }
-    return sizeof(T) * (e + maxnx + 2);
+    return t0 >= static_cast<int>(s.mg.size()) ? 0 : sizeof(T) * (e + maxnx + 2);
 }
 
-// First tail level: the largest tail whose levels fit the LDS budget and have at most
-// tail_cells cells each.
-template <class T> int tail_start(const Solver& s) {
+static long cells(const Solver& s, int l) { return static_cast<long>(s.mg[l].nx) * s.mg[l].ny; }
+
+// First LDS level: the longest run of coarsest levels that fit the LDS budget, each with at most
+// lds_cells cells; nl if not even the coarsest fits.
+template <class T> int lds_start(const Solver& s) {
     const int nl = static_cast<int>(s.mg.size());
+    if (lds_bytes<T>(s, nl - 1) > s.lds_budget)
+        return nl;
     int t0 = nl - 1;
-    while (t0 > 0 && static_cast<long>(s.mg[t0 - 1].nx) * s.mg[t0 - 1].ny <= s.tail_cells &&
-           tail_bytes<T>(s, t0 - 1) <= s.lds_budget)
+    while (t0 > 0 && cells(s, t0 - 1) <= s.lds_cells && lds_bytes<T>(s, t0 - 1) <= s.lds_budget)
         --t0;
     return t0;
 }
 
-// Coarse operators of the levels above the tail (the tail builds its own in LDS).
+// First level of the one-workgroup tail: the levels from t0 (or the coarsest) up with at most
+// mid_cells cells.
+int mid_start(const Solver& s, int t0) {
+    int tm = std::min(t0, static_cast<int>(s.mg.size()) - 1);
+    while (tm > 0 && cells(s, tm - 1) <= s.mid_cells)
+        --tm;
+    return tm;
+}
+
+// Coarse operators of the levels above the tail (the tail builds its own).
 template <class T> void launch_builds(Solver& s, const Tail<T>& t, const int* stop) {
-    // When even the coarsest level does not fit in LDS, it is built here too.
-    const int last = tail_bytes<T>(s, t.t0) <= s.lds_budget ? t.t0 - 1 : t.t0;
-    for (int l = 1; l <= last; ++l) {
+    for (int l = 1; l < t.tm; ++l) {
         hipLaunchKernelGGL(k_mg_build<T>, lgrid(t.G[l].nx, t.G[l].ny), dim3(BX, BY), 0, s.stream,
                            t.G[l - 1], t.G[l], stop);
         SG_HIP(hipGetLastError());
@@ -452,21 +466,18 @@ template <class T> void launch_builds(Solver& s, const Tail<T>& t, const int* st
 // mg_cycles V-cycles (mg_vcycle, :299-323) on level 0 of t.
 template <class T> void launch_vcycles(Solver& s, const Tail<T>& t, const int* stop) {
     const sg_params& p = s.p;
-    const size_t lds = tail_bytes<T>(s, t.t0);
+    const size_t lds = lds_bytes<T>(s, t.t0);
     for (int cyc = 0; cyc < p.mg_cycles; ++cyc) {
-        for (int l = 0; l < t.t0; ++l) {
+        for (int l = 0; l < t.tm; ++l) {
             smooth_passes(s, t.G[l], p.mg_pre, 0, stop);
             hipLaunchKernelGGL(k_mg_resid_restrict<T>, lgrid(t.G[l + 1].nx, t.G[l + 1].ny),
                                dim3(BX, BY), 0, s.stream, t.G[l], t.G[l + 1], stop);
             SG_HIP(hipGetLastError());
         }
-        if (lds <= s.lds_budget)
-            hipLaunchKernelGGL(k_mg_tail<T>, dim3(1), dim3(TT), lds, s.stream, t, s.ctl, stop);
-        else // only the coarsest level, and it does not fit: solve it in global memory
-            hipLaunchKernelGGL(k_mg_coarse_global<T>, dim3(1), dim3(TT), 0, s.stream,
-                               t.G[t.nl - 1], reinterpret_cast<T*>(s.rowbuf), s.ctl, stop);
+        hipLaunchKernelGGL(k_mg_tail<T>, dim3(1), dim3(s.tail_threads), lds, s.stream, t, s.ctl,
+                           stop);
         SG_HIP(hipGetLastError());
-        for (int l = t.t0 - 1; l >= 0; --l) {
+        for (int l = t.tm - 1; l >= 0; --l) {
             const Lv<T> F = t.G[l], C = t.G[l + 1];
             if (p.mg_post > 0) {
                 hipLaunchKernelGGL(k_mg_prolong_smooth0<T>, lgrid(F.nx, cdiv(F.ny, 2)),
@@ -487,19 +498,21 @@ template <class T> Tail<T> make_tail(Solver& s, Lv<T> (*view)(const MgLevel&)) {
     const int nl = static_cast<int>(s.mg.size());
     for (int l = 0; l < nl; ++l)
         t.G[l] = view(s.mg[l]);
-    t.t0 = tail_start<T>(s);
+    t.t0 = lds_start<T>(s);
+    t.tm = mid_start(s, t.t0);
     t.nl = nl;
     t.pre = s.p.mg_pre;
     t.post = s.p.mg_post;
     t.omega = static_cast<T>(s.p.mg_omega);
+    t.grows = reinterpret_cast<T*>(s.rowbuf);
     int e = 0;
     for (int l = t.t0; l < nl; ++l) {
         t.off[l] = e;
         e += 8 * (s.mg[l].nx + 2) * (s.mg[l].ny + 2);
     }
     t.lds_elems = e;
-    const size_t bytes = tail_bytes<T>(s, t.t0);
-    if (bytes > 65536 && bytes <= s.lds_budget)
+    const size_t bytes = lds_bytes<T>(s, t.t0);
+    if (bytes > 65536)
         SG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mg_tail<T>),
                                    hipFuncAttributeMaxDynamicSharedMemorySize,
                                    static_cast<int>(bytes)));

@@ -371,22 +371,26 @@ void launch_bc(Solver& s, bool is_u, const int* stop) {
 
 // Pack the field and evaluate the residual (residual_u / residual_v, src/momentum_u.cpp:43-59)
 // in one pass, then the ordered sum into *slot.
-void launch_mom_residual(Solver& s, bool is_u, double* slot, const int* stop) {
+// side: the sum runs on the side stream (joined before the stop rule).
+void launch_mom_residual(Solver& s, bool is_u, double* slot, const int* stop, bool side) {
     const MomArgs m = mom_args(s, is_u);
     const int nrow = m.i1 - m.i0 + 1, w = m.j1 - m.j0 + 1;
+    double* cell = s.rcell[is_u ? 0 : 1];
     hipLaunchKernelGGL(k_mom_prep, dim3(cdiv(s.nc, BX), cdiv(s.nr, BY)), dim3(BX, BY), 0,
-                       s.stream, m, s.fpk, s.fbuf, s.cell, s.nr, s.nc, 1, stop);
+                       s.stream, m, s.fpk, s.fbuf, cell, s.nr, s.nc, 1, stop);
     SG_HIP(hipGetLastError());
-    if (nrow <= 0 || w <= 0)
-        launch_ordered_sum(s, s.cell, 1, 0, -1, 0, -1, slot, stop); // empty range: 0.0
+    if (side)
+        side_sum(s, is_u ? 1 : 2, cell, nrow, w, slot, stop);
+    else if (nrow <= 0 || w <= 0)
+        launch_ordered_sum(s, cell, 1, 0, -1, 0, -1, slot, stop); // empty range: 0.0
     else
-        launch_ordered_sum(s, s.cell, w, 0, nrow - 1, 0, w - 1, slot, stop);
+        launch_ordered_sum(s, cell, w, 0, nrow - 1, 0, w - 1, slot, stop);
 }
 
 void launch_mom_pack(Solver& s, bool is_u, const int* stop) {
     const MomArgs m = mom_args(s, is_u);
     hipLaunchKernelGGL(k_mom_prep, dim3(cdiv(s.nc, BX), cdiv(s.nr, BY)), dim3(BX, BY), 0,
-                       s.stream, m, s.fpk, s.fbuf, s.cell, s.nr, s.nc, 0, stop);
+                       s.stream, m, s.fpk, s.fbuf, s.rcell[0], s.nr, s.nc, 0, stop);
     SG_HIP(hipGetLastError());
 }
 

@@ -314,16 +314,19 @@ void launch_fill(Solver& s, double* a, size_t n, double v, const int* stop) {
     SG_HIP(hipGetLastError());
 }
 
-void launch_assemble_p(Solver& s, double* slot, const int* stop) {
+void launch_assemble_p(Solver& s, double* slot, const int* stop, bool side) {
     const sg_params& p = s.p;
     const PArgs a = p_args(s);
     hipLaunchKernelGGL(k_p_walls, dim3(cdiv(std::max(s.nr, s.nc), 256)), dim3(256), 0, s.stream,
                        a, s.nr, s.nc, stop);
     SG_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_assemble_p, pgrid(p), dim3(BX, BY), 0, s.stream, a, s.F(SG_F_CU_AP),
-                       s.F(SG_F_CV_AP), s.F(SG_F_U), s.F(SG_F_V), p.dx, p.dy, s.cell, stop);
+                       s.F(SG_F_CV_AP), s.F(SG_F_U), s.F(SG_F_V), p.dx, p.dy, s.rcell[2], stop);
     SG_HIP(hipGetLastError());
-    launch_ordered_sum(s, s.cell, p.NY, 0, p.NX - 1, 0, p.NY - 1, slot, stop);
+    if (side)
+        side_sum(s, 3, s.rcell[2], p.NX, p.NY, slot, stop);
+    else
+        launch_ordered_sum(s, s.rcell[2], p.NY, 0, p.NX - 1, 0, p.NY - 1, slot, stop);
 }
 
 void launch_correct(Solver& s, const int* stop) {

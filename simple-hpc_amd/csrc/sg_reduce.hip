@@ -108,16 +108,31 @@ __global__ void k_zero_slot(double* slot, const int* stop) {
 } // namespace
 
 void launch_ordered_sum(Solver& s, const double* a, int pitch, int i0, int i1, int j0, int j1,
-                        double* slot, const int* stop) {
+                        double* slot, const int* stop, int lane, hipStream_t stream) {
     const int nrow = i1 - i0 + 1, w = j1 - j0 + 1;
+    hipStream_t st = stream ? stream : s.stream;
     if (nrow <= 0 || w <= 0) {
-        hipLaunchKernelGGL(k_zero_slot, dim3(1), dim3(64), 0, s.stream, slot, stop);
+        hipLaunchKernelGGL(k_zero_slot, dim3(1), dim3(64), 0, st, slot, stop);
     } else {
-        hipLaunchKernelGGL(k_ordered_sum, dim3(cdiv(nrow, RW)), dim3(NT), 0, s.stream, a,
-                           static_cast<long>(pitch), i0, nrow, j0, w, s.rowbuf, s.ticket, slot,
+        hipLaunchKernelGGL(k_ordered_sum, dim3(cdiv(nrow, RW)), dim3(NT), 0, st, a,
+                           static_cast<long>(pitch), i0, nrow, j0, w,
+                           s.rowbuf + static_cast<size_t>(lane) * s.nr, s.ticket + lane, slot,
                            stop);
     }
     SG_HIP(hipGetLastError());
+}
+
+void side_sum(Solver& s, int lane, const double* cell, int nrow, int w, double* slot,
+              const int* stop) {
+    SG_HIP(hipEventRecord(s.fork[lane - 1], s.stream));
+    SG_HIP(hipStreamWaitEvent(s.side, s.fork[lane - 1], 0));
+    launch_ordered_sum(s, cell, w > 0 ? w : 1, 0, nrow - 1, 0, w - 1, slot, stop, lane, s.side);
+    SG_HIP(hipEventRecord(s.join[lane - 1], s.side));
+}
+
+void join_side(Solver& s) {
+    for (int k = 0; k < 3; ++k)
+        SG_HIP(hipStreamWaitEvent(s.stream, s.join[k], 0));
 }
 
 } // namespace sg

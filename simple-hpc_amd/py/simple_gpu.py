@@ -26,7 +26,8 @@ FIELD_ID = {n: k for k, n in enumerate(FIELDS)}
 NPHASES = 11
 PHASES = ["assemble_u", "bc_u", "residual_u", "sweep_u", "assemble_v", "bc_v", "residual_v",
           "sweep_v", "assemble_p", "solve_p", "correct"]
-K_SWEEP_U, K_SWEEP_V, K_MG_SMOOTH0, K_ASSEMBLE_U = 0, 1, 2, 3
+K_SWEEP_U, K_SWEEP_V, K_MG_SMOOTH0, K_ASSEMBLE_U, K_PSOLVE = 0, 1, 2, 3, 4
+TUNE_TAIL_THREADS, TUNE_MID_CELLS, TUNE_LDS_CELLS = 0, 1, 2
 
 
 class SgParams(C.Structure):
@@ -132,6 +133,10 @@ def gpu():
         g.sg_iterations.argtypes = [vp, C.POINTER(C.c_int)]
         g.sg_kernel_probe.argtypes = [vp, C.c_int, C.c_int, dp, dp, C.POINTER(C.c_int)]
         g.sg_mg_levels.argtypes = [vp, C.POINTER(C.c_int)]
+        g.sg_mg_coarse_sweeps.argtypes = [vp, C.POINTER(C.c_int)]
+        g.sg_set_tuning.argtypes = [vp, C.c_int, C.c_long]
+        g.sg_get_tuning.argtypes = [vp, C.c_int, C.POINTER(C.c_long)]
+        g.sg_autotune.argtypes = [vp, C.c_int, C.c_char_p, C.c_size_t]
         _gpu = g
     return _gpu
 
@@ -233,6 +238,24 @@ class Solver:
         n = C.c_int()
         check(gpu().sg_kernel_probe(self.h, kernel, reps, C.byref(t), C.byref(b), C.byref(n)))
         return t.value, b.value, n.value
+
+    def coarse_sweeps(self):
+        n = C.c_int()
+        check(gpu().sg_mg_coarse_sweeps(self.h, C.byref(n)))
+        return n.value
+
+    def set_tuning(self, key, value):
+        check(gpu().sg_set_tuning(self.h, key, value))
+
+    def get_tuning(self, key):
+        v = C.c_long()
+        check(gpu().sg_get_tuning(self.h, key, C.byref(v)))
+        return v.value
+
+    def autotune(self, reps=20):
+        buf = C.create_string_buffer(8192)
+        check(gpu().sg_autotune(self.h, reps, buf, 8192))
+        return buf.value.decode()
 
     def mg_levels(self):
         n = C.c_int()

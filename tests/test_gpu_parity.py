@@ -181,3 +181,38 @@ def test_fixed_work_sizes_bit_identical(nx, ny, pres, scheme, order, iters):
     for f in ["u", "v", "p", "pc", "b", "cu_AP", "cv_AP", "APP"]:
         _same(f, gpu.get(f), cpu.get(f))
     gpu.close()
+
+
+@pytest.mark.parametrize("threads,mid,lds", [(64, 0, 256), (1024, 65536, 4096), (256, 4096, 0)])
+def test_tuning_knobs_do_not_change_results(threads, mid, lds):
+    cfg = ("N_cells_x = 256\nN_cells_y = 256\nRe = 1000\nmomentum_solver = gs_multicolour\n"
+           "pressure_solver = mg\nmaxit = 8\nerror_tolerance = 0\nbackend = gpu\n")
+    p, _ = sg.params_from_config(cfg)
+    cpu = po.Oracle(p)
+    _, logc, _ = cpu.run()
+    gpu = sg.Solver(p)
+    gpu.set_tuning(sg.TUNE_TAIL_THREADS, threads)
+    gpu.set_tuning(sg.TUNE_MID_CELLS, mid)
+    gpu.set_tuning(sg.TUNE_LDS_CELLS, lds)
+    done, logg, _ = gpu.advance(8)
+    assert bits_equal(logg, logc[:, 1:]), first_diff(logg, logc[:, 1:])
+    _same("p", gpu.get("p"), cpu.get("p"))
+
+
+def test_autotune_restores_state_and_keeps_parity():
+    cfg = ("N_cells_x = 128\nN_cells_y = 128\nRe = 1000\nmomentum_solver = gs_multicolour\n"
+           "pressure_solver = mg\nmaxit = 12\nerror_tolerance = 0\nbackend = gpu\n")
+    p, _ = sg.params_from_config(cfg)
+    cpu = po.Oracle(p)
+    _, logc, _ = cpu.run()
+    gpu = sg.Solver(p)
+    gpu.advance(6)
+    before = {n: gpu.get(n) for n in ("b", "pc", "u")}
+    report = gpu.autotune(reps=3)
+    print(report, flush=True)
+    assert "best:" in report
+    for n, a in before.items():
+        _same("after autotune " + n, gpu.get(n), a)
+    _, logg, _ = gpu.advance(6)
+    assert bits_equal(logg, logc[6:, 1:])
+    _same("u", gpu.get("u"), cpu.get("u"))
