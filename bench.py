@@ -42,19 +42,27 @@ def log(msg):
     print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
-def run_form(sg, n, form, mode, steps, warmup):
+def run_form(sg, n, form, mode, steps, warmup, knobs=None):
     p, _ = sg.params_from_config(config_text(n, form))
-    s = sg.Solver(p)
-    s.advance(3)
-    tune = s.autotune(reps=10)
-    log(f"{n}^2 {form}: autotune -> {tune.strip().splitlines()[-1]}")
-    s.close()
-    s = sg.Solver(p)  # fresh fields; keep the tuned launch shapes
-    for line in tune.strip().splitlines()[-1:]:
+    if knobs:  # given launch shapes: "tile,threads,mid,lds" (skips the autotuner)
+        t, th, mid, lds = (int(x) for x in knobs.split(","))
+        tune = (f"best momentum tile: given ({t})\n"
+                f"best: tail_threads {th} mid_cells {mid} lds_cells {lds} : given\n")
+    else:
+        s = sg.Solver(p)
+        s.advance(3)
+        tune = s.autotune(reps=10)
+        log(f"{n}^2 {form}: autotune -> {tune.strip().splitlines()[-1]}")
+        s.close()
+    s = sg.Solver(p)  # fresh fields; the tuned launch shapes
+    for line in tune.strip().splitlines():
         f = line.split()
-        s.set_tuning(sg.TUNE_TAIL_THREADS, int(f[2]))
-        s.set_tuning(sg.TUNE_MID_CELLS, int(f[4]))
-        s.set_tuning(sg.TUNE_LDS_CELLS, int(f[6]))
+        if line.startswith("best momentum tile"):
+            s.set_tuning(sg.TUNE_MOM_TILE, int(f[-1].strip("()")))
+        elif line.startswith("best:"):
+            s.set_tuning(sg.TUNE_TAIL_THREADS, int(f[2]))
+            s.set_tuning(sg.TUNE_MID_CELLS, int(f[4]))
+            s.set_tuning(sg.TUNE_LDS_CELLS, int(f[6]))
     if mode == "fixed":
         s.advance(warmup)
         log(f"{n}^2 {form}: timing {steps} outer iterations")
@@ -63,7 +71,8 @@ def run_form(sg, n, form, mode, steps, warmup):
         t = time.perf_counter() - t0
         return s, {"form": form, "iterations": done, "seconds": t, "ms_per_iter": 1e3 * t / done,
                    "last_residuals": [float(x) for x in lg[-1]],
-                   "autotune_best": tune.strip().splitlines()[-1]}
+                   "coarse_sweeps_last": s.coarse_sweeps(),
+                   "autotune": [l for l in tune.strip().splitlines() if l.startswith("best")]}
     log(f"{n}^2 {form}: to tolerance")
     t0 = time.perf_counter()
     total, conv, last = 0, False, None
@@ -80,7 +89,7 @@ def run_form(sg, n, form, mode, steps, warmup):
     t = time.perf_counter() - t0
     return s, {"form": form, "iterations": total, "seconds": t, "ms_per_iter": 1e3 * t / total,
                "converged": bool(conv), "last_residuals": [float(x) for x in last],
-               "autotune_best": tune.strip().splitlines()[-1]}
+               "autotune": [l for l in tune.strip().splitlines() if l.startswith("best")]}
 
 
 def cpu_baseline(sg, n, seconds=12.0):
@@ -110,6 +119,9 @@ def main():
     ap.add_argument("--forms", default="plain")
     ap.add_argument("--n", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--knobs", default=None,
+                    help="launch shapes 'tile,tail_threads,mid_cells,lds_cells' instead of "
+                         "autotuning (profiling runs)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU: the multi-GPU decomposition of the cavity is not built "
@@ -119,7 +131,7 @@ def main():
     results = []
     probe = None
     for form in a.forms.split(","):
-        s, r = run_form(sg, a.n, form, a.mode, a.steps, a.warmup)
+        s, r = run_form(sg, a.n, form, a.mode, a.steps, a.warmup, a.knobs)
         results.append(r)
         if probe is None:
             sec, bytes_, per = s.probe(sg.K_SWEEP_U, reps=200)
@@ -145,7 +157,7 @@ def main():
                    "forms": results},
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": gbs / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_mom_pass<5> (one colour pass of the u multicolour sweep)",
+                     "kernel": "k_mom_sweep<5> (one u multicolour sweep, all colours)",
                      "kernel_us": sec * 1e6, "bytes_per_launch": bytes_,
                      "launches_per_iteration": per},
     }

@@ -159,7 +159,7 @@ int sg_iterations(sg_solver* s, int* iters);
  * timing on (sg_set_kernel_timing), measured with hipEvents on the solver's stream, and the
  * algorithmic bytes of one launch (SG_K_* ids). */
 enum {
-    SG_K_SWEEP_U = 0,  /* one colour pass of the u multicolour sweep */
+    SG_K_SWEEP_U = 0,  /* one u multicolour sweep (every colour, one launch) */
     SG_K_SWEEP_V,
     SG_K_MG_SMOOTH0,   /* one colour pass of the level-0 multigrid smoother */
     SG_K_ASSEMBLE_U,
@@ -178,12 +178,19 @@ int sg_mg_coarse_sweeps(sg_solver* s, int* sweeps);
  * multigrid tail (64..1024, multiple of 64); SG_TUNE_MID_CELLS: levels with at most this many
  * cells run inside that workgroup; SG_TUNE_LDS_CELLS: levels with at most this many cells (that
  * fit) live in LDS there. */
-enum { SG_TUNE_TAIL_THREADS = 0, SG_TUNE_MID_CELLS = 1, SG_TUNE_LDS_CELLS = 2, SG_TUNE_COUNT };
+enum {
+    SG_TUNE_TAIL_THREADS = 0,
+    SG_TUNE_MID_CELLS = 1,
+    SG_TUNE_LDS_CELLS = 2,
+    SG_TUNE_MOM_TILE = 3, /* momentum sweep tile: 0 32x64, 1 64x64, 2 32x128, 3 16x128 */
+    SG_TUNE_COUNT
+};
 int sg_set_tuning(sg_solver* s, int key, long value);
 int sg_get_tuning(sg_solver* s, int key, long* value);
-/* Autotuner: times the p' solve on the device (hipEvents around graph replays, reps each) for a
- * grid of the knobs above on the solver's current state, keeps the fastest, restores every
- * array, and writes a one-line-per-candidate report into report (may be NULL). */
+/* Autotuner: on the solver's current state, times one momentum sweep per tile shape and the p'
+ * solve (hipEvents around graph replays, reps each) for a grid of the multigrid knobs, keeps the
+ * fastest of each, restores every array, and writes a one-line-per-candidate report into report
+ * (may be NULL); its last line names the multigrid choice. */
 int sg_autotune(sg_solver* s, int reps, char* report, size_t len);
 
 #ifdef __cplusplus

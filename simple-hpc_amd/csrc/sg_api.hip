@@ -119,17 +119,15 @@ void enqueue_iteration(Solver& s, const int* stop, bool eager, hipEvent_t* ev) {
     mark(1);
     launch_bc(s, true, stop);
     mark(2);
-    launch_mom_residual(s, true, s.scal + S_RES_U, stop, side);
     mark(3);
-    launch_mom_sweeps(s, true, p.maxit_inner_u, stop, true);
+    launch_mom_sweeps(s, true, p.maxit_inner_u, stop, s.scal + S_RES_U, side);
     mark(4);
     launch_assemble(s, false, stop);
     mark(5);
     launch_bc(s, false, stop);
     mark(6);
-    launch_mom_residual(s, false, s.scal + S_RES_V, stop, side);
     mark(7);
-    launch_mom_sweeps(s, false, p.maxit_inner_v, stop, true);
+    launch_mom_sweeps(s, false, p.maxit_inner_v, stop, s.scal + S_RES_V, side);
     mark(8);
     launch_assemble_p(s, s.scal + S_RES_P, stop, side);
     mark(9);
@@ -374,8 +372,6 @@ int sg_create(const sg_params* pin, const double* u, const double* v, const doub
         s.pkv = make_pack(p, false);
         s.pku.base = dalloc<double>(s, 10 * s.pku.astride);
         s.pkv.base = dalloc<double>(s, 10 * s.pkv.astride);
-        s.fpk = make_field_pack(p);
-        s.fbuf = dalloc<double>(s, s.fpk.astride);
         ensure_log(s, 256);
         diffusive(p, s.diff2, s.diff4);
         if (p.pressure_solver == SG_P_MG)
@@ -491,7 +487,7 @@ int sg_residual_u(sg_solver* h, double* res) {
 int sg_sweep_u(sg_solver* h, int n) {
     return guard([&] {
         Solver& s = S(h);
-        launch_mom_sweeps(s, true, n, s.never, false);
+        launch_mom_sweeps(s, true, n, s.never, nullptr, false);
         sync(s);
     });
 }
@@ -508,7 +504,7 @@ int sg_residual_v(sg_solver* h, double* res) {
 int sg_sweep_v(sg_solver* h, int n) {
     return guard([&] {
         Solver& s = S(h);
-        launch_mom_sweeps(s, false, n, s.never, false);
+        launch_mom_sweeps(s, false, n, s.never, nullptr, false);
         sync(s);
     });
 }
@@ -591,7 +587,6 @@ int sg_kernel_probe(sg_solver* h, int kid, int reps, double* mean_s, double* byt
         const sg_params& p = s.p;
         if (reps < 1)
             throw ArgError("reps must be >= 1");
-        const int ncol = (p.convection_scheme == SG_UDS && p.diffusion_order == 2) ? 2 : 5;
         double cells = 0.0, bpc = 0.0;
         int per = 0;
         std::function<void(int)> one;
@@ -601,13 +596,16 @@ int sg_kernel_probe(sg_solver* h, int kid, int reps, double* mean_s, double* byt
             const bool is_u = kid == SG_K_SWEEP_U;
             if (p.momentum_solver != SG_MOM_GS_MULTICOLOUR)
                 throw ArgError("sweep probe needs momentum_solver = gs_multicolour");
-            launch_mom_pack(s, is_u, s.never); // the probe sweeps the packed copy only
             cells = is_u ? static_cast<double>(p.NX) * (p.NY - 1)
                          : static_cast<double>(p.NX - 1) * p.NY;
-            cells /= ncol;
-            bpc = 96.0; // 9 coefficients + p + u read + u write, 8 B each, per updated cell
-            per = ncol * (is_u ? p.maxit_inner_u : p.maxit_inner_v);
-            one = [&s, is_u, ncol](int k) { launch_mom_colour_pass(s, is_u, k % ncol, s.never); };
+            // one launch is a whole sweep (every colour)
+            bpc = 96.0; // 9 coefficients + pressure term + u read + u write, 8 B each, per cell
+            per = is_u ? p.maxit_inner_u : p.maxit_inner_v;
+            // field -> scratch: the field itself is left unchanged
+            one = [&s, is_u](int) {
+                launch_mom_sweep_once(s, is_u, s.F(is_u ? SG_F_U : SG_F_V), s.scratch, false,
+                                      s.never);
+            };
             break;
         }
         case SG_K_PSOLVE:
@@ -651,11 +649,12 @@ int sg_mg_coarse_sweeps(sg_solver* h, int* n) {
 }
 
 static long* knob(Solver& s, int key) {
-    static long threads;
+    static thread_local long tmp;
     switch (key) {
     case SG_TUNE_MID_CELLS: return &s.mid_cells;
     case SG_TUNE_LDS_CELLS: return &s.lds_cells;
-    case SG_TUNE_TAIL_THREADS: threads = s.tail_threads; return &threads;
+    case SG_TUNE_TAIL_THREADS: tmp = s.tail_threads; return &tmp;
+    case SG_TUNE_MOM_TILE: tmp = s.mom_tile; return &tmp;
     default: throw ArgError("unknown tuning key");
     }
 }
@@ -667,6 +666,10 @@ int sg_set_tuning(sg_solver* h, int key, long value) {
             if (value < 64 || value > 1024 || value % 64)
                 throw ArgError("tail threads must be a multiple of 64 in [64, 1024]");
             s.tail_threads = static_cast<int>(value);
+        } else if (key == SG_TUNE_MOM_TILE) {
+            if (value < 0 || value > 3)
+                throw ArgError("momentum tile shape must be 0..3");
+            s.mom_tile = static_cast<int>(value);
         } else {
             if (value < 0)
                 throw ArgError("tuning value must be >= 0");
@@ -707,17 +710,50 @@ static double time_psolve(Solver& s, int reps) {
 int sg_autotune(sg_solver* h, int reps, char* report, size_t len) {
     return guard([&] {
         Solver& s = S(h);
-        if (s.p.pressure_solver != SG_P_MG)
-            throw ArgError("sg_autotune tunes the multigrid p' solve (pressure_solver = mg)");
         if (reps < 1)
             throw ArgError("reps must be >= 1");
+        std::string rep;
+        if (s.p.momentum_solver == SG_MOM_GS_MULTICOLOUR) {
+            // Momentum sweep tile: one u sweep (field -> scratch) per candidate shape.
+            static const char* shape[4] = {"32x64", "64x64", "32x128", "16x128"};
+            double best_t = 1e30;
+            int best_k = s.mom_tile;
+            for (int k = 0; k < 4; ++k) {
+                s.mom_tile = k;
+                launch_mom_sweep_once(s, true, s.F(SG_F_U), s.scratch, false, s.never);
+                SG_HIP(hipEventRecord(s.kt0, s.stream));
+                for (int r = 0; r < reps; ++r)
+                    launch_mom_sweep_once(s, true, s.F(SG_F_U), s.scratch, false, s.never);
+                SG_HIP(hipEventRecord(s.kt1, s.stream));
+                sync(s);
+                float ms = 0.f;
+                SG_HIP(hipEventElapsedTime(&ms, s.kt0, s.kt1));
+                const double t = 1e-3 * ms / reps;
+                char line[160];
+                std::snprintf(line, sizeof line, "momentum tile %-6s : %9.2f us per sweep\n",
+                              shape[k], 1e6 * t);
+                rep += line;
+                if (t < best_t)
+                    best_t = t, best_k = k;
+            }
+            s.mom_tile = best_k;
+            char line[160];
+            std::snprintf(line, sizeof line, "best momentum tile: %s (%d)\n", shape[best_k], best_k);
+            rep += line;
+        }
+        if (s.p.pressure_solver != SG_P_MG) {
+            if (report && len) {
+                std::strncpy(report, rep.c_str(), len - 1);
+                report[len - 1] = 0;
+            }
+            return;
+        }
         // Keep b and p' (the solve rewrites both) to restore them afterwards.
         double *kb = nullptr, *kp = nullptr;
         SG_HIP(hipMalloc(reinterpret_cast<void**>(&kb), sizeof(double) * s.n));
         SG_HIP(hipMalloc(reinterpret_cast<void**>(&kp), sizeof(double) * s.n));
         SG_HIP(hipMemcpyAsync(kb, s.F(SG_F_B), sizeof(double) * s.n, hipMemcpyDeviceToDevice, s.stream));
         SG_HIP(hipMemcpyAsync(kp, s.F(SG_F_PC), sizeof(double) * s.n, hipMemcpyDeviceToDevice, s.stream));
-        std::string rep;
         double best = 1e30;
         int bt = s.tail_threads;
         long bm = s.mid_cells, bl = s.lds_cells;

@@ -50,8 +50,10 @@ struct MgLevel {
     int nx = 0, ny = 0, stride = 0;
     double *AE = nullptr, *AW = nullptr, *AN = nullptr, *AS = nullptr, *AP = nullptr;
     double *b = nullptr, *x = nullptr, *r = nullptr;
+    double* xt = nullptr; // pre-smoothed iterate (tiled V-cycle), same layout as x, zero halo
     float *fAE = nullptr, *fAW = nullptr, *fAN = nullptr, *fAS = nullptr, *fAP = nullptr;
     float *fb = nullptr, *fx = nullptr, *fr = nullptr; // FP32 copies (mixed precision)
+    float* fxt = nullptr;
 };
 
 // Colour-packed layout of one momentum component's coefficients (sg_momentum.hip): 10 arrays
@@ -107,11 +109,11 @@ struct Solver {
     // of tail_threads threads, those with at most lds_cells cells (and fitting lds_budget) in LDS.
     long mid_cells = 16384, lds_cells = 4096;
     int tail_threads = 256;
+    bool mg_tiled = true;              // multigrid levels above the tail: down/up tile kernels
     size_t lds_budget = 65536 - 1024;  // dynamic LDS of the multigrid tail (set from the device)
     std::vector<void*> owned;          // allocations to free
     Pack pku{}, pkv{};                 // packed momentum coefficients
-    Pack fpk{};                        // whole-array packing of the swept field
-    double* fbuf = nullptr;            // the swept field, colour-packed (u, then v)
+    int mom_tile = 0;                  // momentum sweep tile shape (tunable; sg_momentum.hip)
     C9 diff2{}, diff4{};               // diffusive coefficients: order 2 (and band), order 4
     hipGraphExec_t graph = nullptr;
     int iters = 0;                     // host copy of completed iterations
@@ -130,10 +132,10 @@ struct Solver {
 void launch_assemble(Solver& s, bool is_u, const int* stop);
 void launch_bc(Solver& s, bool is_u, const int* stop);
 void launch_mom_residual(Solver& s, bool is_u, double* slot, const int* stop, bool side);
-void launch_mom_sweeps(Solver& s, bool is_u, int nsweeps, const int* stop, bool packed);
-void launch_mom_pack(Solver& s, bool is_u, const int* stop);
-Pack make_field_pack(const sg_params& p);
-void launch_mom_colour_pass(Solver& s, bool is_u, int colour, const int* stop);
+void launch_mom_sweeps(Solver& s, bool is_u, int nsweeps, const int* stop, double* res_slot,
+                       bool side);
+void launch_mom_sweep_once(Solver& s, bool is_u, const double* src, double* dst, bool first,
+                           const int* stop);
 int ncolours(const sg_params& p);
 Pack make_pack(const sg_params& p, bool is_u);
 void sync_packed(Solver& s, int field_id, bool to_packed);

@@ -33,13 +33,14 @@ constexpr int MAXLEV = 24;
 template <class T> struct Lv {
     int nx, ny, stride;
     T *AE, *AW, *AN, *AS, *AP, *b, *x, *r;
+    T* xt; // pre-smoothed iterate of the tiled V-cycle
 };
 
 Lv<double> lv64(const MgLevel& L) {
-    return {L.nx, L.ny, L.stride, L.AE, L.AW, L.AN, L.AS, L.AP, L.b, L.x, L.r};
+    return {L.nx, L.ny, L.stride, L.AE, L.AW, L.AN, L.AS, L.AP, L.b, L.x, L.r, L.xt};
 }
 Lv<float> lv32(const MgLevel& L) {
-    return {L.nx, L.ny, L.ny + 2, L.fAE, L.fAW, L.fAN, L.fAS, L.fAP, L.fb, L.fx, L.fr};
+    return {L.nx, L.ny, L.ny + 2, L.fAE, L.fAW, L.fAN, L.fAS, L.fAP, L.fb, L.fx, L.fr, L.fxt};
 }
 
 // Relative tolerance of the coarsest-level solve: the reference's 1e-10 in FP64; in FP32 a
@@ -161,6 +162,157 @@ __global__ void __launch_bounds__(BX* BY) k_mg_prolong(Lv<T> C, Lv<T> F, T omega
             omega * C.x[static_cast<size_t>(i / 2) * C.stride + j / 2];
 }
 
+// ---- tiled V-cycle kernels for the levels above the tail ----
+//
+// down: pre-smoothing of a TI x TJ tile in LDS (every colour pass of every pre-sweep; pass p runs
+// over the tile grown by 2*pre - p cells, so the tile and its one-cell ring end exactly as after
+// the global passes), then the residual and restriction of the tile's coarse cells (coarse x = 0)
+// and, on the first V-cycle, their Galerkin operator. The iterate is read from x and written to
+// xt, so no tile reads cells another tile has already updated.
+// up: the tile of xt prolonged (x + omega * xc, interior cells only) into LDS, every post-smoothing
+// pass (pass p over the tile grown by 2*post - 1 - p cells), written to x.
+
+constexpr int MT_I = 32, MT_J = 64, MT_N = 256; // tile rows, columns, threads
+constexpr int MT_MAXH = 8;                     // largest halo handled (2*pre + 1, 2*post)
+
+template <class T>
+__device__ __forceinline__ void tile_load(T* sx, const Lv<T>& L, const T* x, int gi0, int gj0,
+                                          int LH, int LW, const Lv<T>* C, T omega) {
+    constexpr int B = 8; // loads in flight per thread
+    for (int base = 0; base < LH * LW; base += B * MT_N) {
+        T v[B];
+#pragma unroll
+        for (int k = 0; k < B; ++k) {
+            const int e = base + k * MT_N + threadIdx.x, a = e / LW, b = e - a * LW;
+            const int I = gi0 + a, J = gj0 + b;
+            T val = T(0);
+            if (e < LH * LW && I >= -1 && I <= L.nx && J >= -1 && J <= L.ny) {
+                val = x[static_cast<long>(I) * L.stride + J];
+                if (C && I >= 0 && I < L.nx && J >= 0 && J < L.ny)
+                    val = val + omega * C->x[static_cast<size_t>(I / 2) * C->stride + J / 2];
+            }
+            v[k] = val;
+        }
+#pragma unroll
+        for (int k = 0; k < B; ++k) {
+            const int e = base + k * MT_N + threadIdx.x;
+            if (e < LH * LW)
+                sx[e] = v[k];
+        }
+    }
+    __syncthreads();
+}
+
+// Colour pass `col` over rows [ia, ib], columns [ja, jb] of the level, x in LDS. Each thread
+// gathers the operator of up to B cells (all loads in flight) before updating them.
+template <class T>
+__device__ __forceinline__ void tile_pass(T* sx, const Lv<T>& L, int gi0, int gj0, int LW, int col,
+                                          int ia, int ib, int ja, int jb) {
+    constexpr int B = 6;
+    if (ia <= ib && ja <= jb) {
+        const int cpr = (jb - ja + 2) / 2, n = (ib - ia + 1) * cpr;
+        for (int base = 0; base < n; base += B * MT_N) {
+            T cb[B], cp[B], ce[B], cw[B], cs[B], cn[B];
+            int off[B];
+#pragma unroll
+            for (int k = 0; k < B; ++k) {
+                const int e = base + k * MT_N + threadIdx.x;
+                const int r = e / cpr, I = ia + r;
+                const int J = ja + ((I + ja + col) & 1) + 2 * (e - r * cpr);
+                off[k] = -1;
+                if (e < n && J <= jb) {
+                    const size_t q = static_cast<size_t>(I) * L.stride + J;
+                    cb[k] = L.b[q];
+                    cp[k] = L.AP[q];
+                    ce[k] = L.AE[q];
+                    cw[k] = L.AW[q];
+                    cs[k] = L.AS[q];
+                    cn[k] = L.AN[q];
+                    off[k] = (I - gi0) * LW + (J - gj0);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < B; ++k)
+                if (off[k] >= 0) {
+                    T* c = sx + off[k];
+                    T t = cb[k] / cp[k];
+                    t += (ce[k] * c[1] + cw[k] * c[-1] + cs[k] * c[LW] + cn[k] * c[-LW]) / cp[k];
+                    *c = t;
+                }
+        }
+    }
+    __syncthreads();
+}
+
+template <class T>
+__global__ void __launch_bounds__(MT_N)
+    k_mg_down(Lv<T> F, Lv<T> C, const T* xin, T* xout, int pre, int build, const int* stop) {
+    if (stopped(stop))
+        return;
+    extern __shared__ __align__(16) unsigned char smem[];
+    T* sx = reinterpret_cast<T*>(smem);
+    const int H = 2 * pre + 1, LW = MT_J + 2 * H, LH = MT_I + 2 * H;
+    const int ti0 = blockIdx.y * MT_I, tj0 = blockIdx.x * MT_J;
+    const int gi0 = ti0 - H, gj0 = tj0 - H;
+    tile_load<T>(sx, F, xin, gi0, gj0, LH, LW, nullptr, T(0));
+    for (int p = 0; p < 2 * pre; ++p) {
+        const int d = 2 * pre - p;
+        tile_pass(sx, F, gi0, gj0, LW, p & 1, max(0, ti0 - d), min(F.nx - 1, ti0 + MT_I - 1 + d),
+                  max(0, tj0 - d), min(F.ny - 1, tj0 + MT_J - 1 + d));
+    }
+    // Residual and restriction of the tile's coarse cells; their operator on the first cycle.
+    const int ci0 = ti0 / 2, cj0 = tj0 / 2;
+    const int cn = min(MT_I / 2, C.nx - ci0), cm = min(MT_J / 2, C.ny - cj0);
+    const long fs = F.stride;
+    for (int e = threadIdx.x; e < cn * cm; e += MT_N) {
+        const int I = ci0 + e / cm, J = cj0 + e % cm;
+        const size_t f = 2 * I * fs + 2 * J, c = static_cast<size_t>(I) * C.stride + J;
+        T r4[4];
+        const size_t fq[4] = {f, f + 1, f + fs, f + fs + 1};
+        const int fi[4] = {2 * I, 2 * I, 2 * I + 1, 2 * I + 1}, fj[4] = {2 * J, 2 * J + 1, 2 * J, 2 * J + 1};
+        for (int k = 0; k < 4; ++k) {
+            const size_t q = fq[k];
+            const T* x = sx + (fi[k] - gi0) * LW + (fj[k] - gj0);
+            r4[k] = F.b[q] - (F.AP[q] * x[0] - (F.AE[q] * x[1] + F.AW[q] * x[-1] +
+                                                 F.AS[q] * x[LW] + F.AN[q] * x[-LW]));
+        }
+        C.b[c] = r4[0] + r4[1] + r4[2] + r4[3];
+        C.x[c] = T(0);
+        if (build)
+            build_cell(F, C, I, J);
+    }
+#pragma unroll
+    for (int k = 0; k < MT_I * MT_J / MT_N; ++k) {
+        const int e = k * MT_N + threadIdx.x, I = ti0 + e / MT_J, J = tj0 + e % MT_J;
+        if (I < F.nx && J < F.ny)
+            xout[static_cast<size_t>(I) * F.stride + J] = sx[(I - gi0) * LW + (J - gj0)];
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(MT_N)
+    k_mg_up(Lv<T> F, Lv<T> C, const T* xpre, T* xfin, T omega, int post, const int* stop) {
+    if (stopped(stop))
+        return;
+    extern __shared__ __align__(16) unsigned char smem[];
+    T* sx = reinterpret_cast<T*>(smem);
+    const int H = 2 * post, LW = MT_J + 2 * H, LH = MT_I + 2 * H;
+    const int ti0 = blockIdx.y * MT_I, tj0 = blockIdx.x * MT_J;
+    const int gi0 = ti0 - H, gj0 = tj0 - H;
+    tile_load<T>(sx, F, xpre, gi0, gj0, LH, LW, &C, omega);
+    for (int p = 0; p < 2 * post; ++p) {
+        const int d = 2 * post - 1 - p;
+        tile_pass(sx, F, gi0, gj0, LW, p & 1, max(0, ti0 - d), min(F.nx - 1, ti0 + MT_I - 1 + d),
+                  max(0, tj0 - d), min(F.ny - 1, tj0 + MT_J - 1 + d));
+    }
+#pragma unroll
+    for (int k = 0; k < MT_I * MT_J / MT_N; ++k) {
+        const int e = k * MT_N + threadIdx.x, I = ti0 + e / MT_J, J = tj0 + e % MT_J;
+        if (I < F.nx && J < F.ny)
+            xfin[static_cast<size_t>(I) * F.stride + J] = sx[(I - gi0) * LW + (J - gj0)];
+    }
+}
+
 // ---- the tail: one workgroup for the small levels ----
 //
 // Levels tm..nl-1 run in one workgroup, one launch per V-cycle, with barriers between steps:
@@ -184,7 +336,8 @@ template <class T> __device__ __forceinline__ Lv<T> lds_view(const Tail<T>& t, T
     const int nx = t.G[l].nx, ny = t.G[l].ny, st = ny + 2;
     const size_t n = static_cast<size_t>(nx + 2) * st;
     T* b = sm + t.off[l] + st + 1;
-    return {nx, ny, st, b, b + n, b + 2 * n, b + 3 * n, b + 4 * n, b + 5 * n, b + 6 * n, b + 7 * n};
+    return {nx, ny, st, b, b + n, b + 2 * n, b + 3 * n, b + 4 * n, b + 5 * n, b + 6 * n, b + 7 * n,
+            nullptr};
 }
 
 template <class T> __device__ void wg_smooth(const Lv<T>& L, int sweeps) {
@@ -301,23 +454,23 @@ __global__ void __launch_bounds__(1024) k_mg_tail(Tail<T> t, Ctl* ctl, const int
         sm[e] = T(0);
     __syncthreads();
     T* rows = t.t0 < t.nl ? sm + t.lds_elems : t.grows;
-    Lv<T> L[MAXLEV];
-    for (int l = t.tm; l < t.nl; ++l)
-        L[l] = l < t.t0 ? t.G[l] : lds_view(t, sm, l);
+    // Level views are rebuilt where needed (a local array indexed by the level would live in
+    // scratch memory).
+    auto V = [&](int l) { return l < t.t0 ? t.G[l] : lds_view(t, sm, l); };
     // Operators: global levels from their (global) parents, LDS levels from theirs; level 0 in
     // LDS is loaded.
     for (int l = t.tm; l < t.nl; ++l) {
         if (l == 0) {
             if (t.t0 == 0)
-                wg_copy_level(L[0], t.G[0], true);
+                wg_copy_level(V(0), t.G[0], true);
             continue;
         }
-        wg_build(l == t.tm ? t.G[l - 1] : L[l - 1], L[l]);
+        wg_build(l == t.tm ? t.G[l - 1] : V(l - 1), V(l));
     }
     if (t.tm == t.t0 && t.t0 > 0)
-        wg_copy_level(L[t.tm], t.G[t.tm], false); // b and x of the first level
+        wg_copy_level(V(t.tm), t.G[t.tm], false); // b and x of the first level
     for (int l = t.tm; l + 1 < t.nl; ++l) {
-        const Lv<T>&F = L[l], &C = L[l + 1];
+        const Lv<T> F = V(l), C = V(l + 1);
         wg_smooth(F, t.pre);
         wg_resid(F);
         for (int e = threadIdx.x; e < C.nx * C.ny; e += blockDim.x) {
@@ -329,9 +482,9 @@ __global__ void __launch_bounds__(1024) k_mg_tail(Tail<T> t, Ctl* ctl, const int
         }
         __syncthreads();
     }
-    const int sweeps = wg_coarse_solve(L[t.nl - 1], rows);
+    const int sweeps = wg_coarse_solve(V(t.nl - 1), rows);
     for (int l = t.nl - 2; l >= t.tm; --l) {
-        const Lv<T>&F = L[l], &C = L[l + 1];
+        const Lv<T> F = V(l), C = V(l + 1);
         for (int e = threadIdx.x; e < F.nx * F.ny; e += blockDim.x) {
             const int i = e / F.ny, j = e - i * F.ny;
             F.x[static_cast<size_t>(i) * F.stride + j] +=
@@ -342,7 +495,7 @@ __global__ void __launch_bounds__(1024) k_mg_tail(Tail<T> t, Ctl* ctl, const int
     }
     if (t.tm == t.t0 && t.t0 < t.nl) {
         // The first level was in LDS: its x (and, if it is the coarsest, its mean-free b) back.
-        const Lv<T>&D = L[t.tm], &G0 = t.G[t.tm];
+        const Lv<T> D = V(t.tm), G0 = t.G[t.tm];
         for (int e = threadIdx.x; e < D.nx * D.ny; e += blockDim.x) {
             const int I = e / D.ny, J = e - I * D.ny;
             const size_t g = static_cast<size_t>(I) * G0.stride + J,
@@ -454,8 +607,16 @@ int mid_start(const Solver& s, int t0) {
     return tm;
 }
 
-// Coarse operators of the levels above the tail (the tail builds its own).
+// The tiled down/up kernels apply when their halos are small enough.
+bool tiled_ok(const Solver& s) {
+    return s.mg_tiled && 2 * s.p.mg_pre + 1 <= MT_MAXH && 2 * s.p.mg_post <= MT_MAXH;
+}
+
+// Coarse operators of the levels above the tail (the tail builds its own; the tiled down kernel
+// builds them during the first V-cycle).
 template <class T> void launch_builds(Solver& s, const Tail<T>& t, const int* stop) {
+    if (tiled_ok(s))
+        return;
     for (int l = 1; l < t.tm; ++l) {
         hipLaunchKernelGGL(k_mg_build<T>, lgrid(t.G[l].nx, t.G[l].ny), dim3(BX, BY), 0, s.stream,
                            t.G[l - 1], t.G[l], stop);
@@ -467,6 +628,32 @@ template <class T> void launch_builds(Solver& s, const Tail<T>& t, const int* st
 template <class T> void launch_vcycles(Solver& s, const Tail<T>& t, const int* stop) {
     const sg_params& p = s.p;
     const size_t lds = lds_bytes<T>(s, t.t0);
+    if (tiled_ok(s)) {
+        const size_t dn = sizeof(T) * (MT_I + 2 * (2 * p.mg_pre + 1)) * (MT_J + 2 * (2 * p.mg_pre + 1));
+        const size_t up = sizeof(T) * (MT_I + 4 * p.mg_post) * (MT_J + 4 * p.mg_post);
+        for (int cyc = 0; cyc < p.mg_cycles; ++cyc) {
+            for (int l = 0; l < t.tm; ++l) {
+                const Lv<T> F = t.G[l], C = t.G[l + 1];
+                // The first cycle builds level l+1 here unless the tail builds it (l+1 = tm).
+                const int build = (cyc == 0 && l + 1 < t.tm) ? 1 : 0;
+                hipLaunchKernelGGL(k_mg_down<T>, dim3(cdiv(F.ny, MT_J), cdiv(F.nx, MT_I)),
+                                   dim3(MT_N), dn, s.stream, F, C, F.x, F.xt, p.mg_pre, build,
+                                   stop);
+                SG_HIP(hipGetLastError());
+            }
+            hipLaunchKernelGGL(k_mg_tail<T>, dim3(1), dim3(s.tail_threads), lds, s.stream, t,
+                               s.ctl, stop);
+            SG_HIP(hipGetLastError());
+            for (int l = t.tm - 1; l >= 0; --l) {
+                const Lv<T> F = t.G[l], C = t.G[l + 1];
+                hipLaunchKernelGGL(k_mg_up<T>, dim3(cdiv(F.ny, MT_J), cdiv(F.nx, MT_I)),
+                                   dim3(MT_N), up, s.stream, F, C, F.xt, F.x, t.omega, p.mg_post,
+                                   stop);
+                SG_HIP(hipGetLastError());
+            }
+        }
+        return;
+    }
     for (int cyc = 0; cyc < p.mg_cycles; ++cyc) {
         for (int l = 0; l < t.tm; ++l) {
             smooth_passes(s, t.G[l], p.mg_pre, 0, stop);
@@ -549,11 +736,11 @@ void mg_setup(Solver& s) {
         const size_t n = static_cast<size_t>(L.nx + 2) * (L.ny + 2);
         if (mixed) {
             float* blk = nullptr;
-            SG_HIP(hipMalloc(reinterpret_cast<void**>(&blk), sizeof(float) * n * 8));
-            SG_HIP(hipMemset(blk, 0, sizeof(float) * n * 8));
+            SG_HIP(hipMalloc(reinterpret_cast<void**>(&blk), sizeof(float) * n * 9));
+            SG_HIP(hipMemset(blk, 0, sizeof(float) * n * 9));
             s.owned.push_back(blk);
-            float** ptr[8] = {&L.fAE, &L.fAW, &L.fAN, &L.fAS, &L.fAP, &L.fb, &L.fx, &L.fr};
-            for (int k = 0; k < 8; ++k)
+            float** ptr[9] = {&L.fAE, &L.fAW, &L.fAN, &L.fAS, &L.fAP, &L.fb, &L.fx, &L.fr, &L.fxt};
+            for (int k = 0; k < 9; ++k)
                 *ptr[k] = blk + k * n + (L.ny + 2) + 1;
         }
         if (l == 0) {
@@ -566,17 +753,22 @@ void mg_setup(Solver& s) {
             L.b = s.F(SG_F_B) + base0;
             L.x = s.F(SG_F_PC) + base0;
             L.r = s.mg_r0 + base0;
+            double* xt = nullptr;
+            SG_HIP(hipMalloc(reinterpret_cast<void**>(&xt), sizeof(double) * s.n));
+            SG_HIP(hipMemset(xt, 0, sizeof(double) * s.n));
+            s.owned.push_back(xt);
+            L.xt = xt + base0;
             continue;
         }
         L.stride = L.ny + 2;
         if (mixed)
             continue; // the FP64 form's coarse levels are not used
         double* blk = nullptr;
-        SG_HIP(hipMalloc(reinterpret_cast<void**>(&blk), sizeof(double) * n * 8));
-        SG_HIP(hipMemset(blk, 0, sizeof(double) * n * 8));
+        SG_HIP(hipMalloc(reinterpret_cast<void**>(&blk), sizeof(double) * n * 9));
+        SG_HIP(hipMemset(blk, 0, sizeof(double) * n * 9));
         s.owned.push_back(blk);
-        double** ptr[8] = {&L.AE, &L.AW, &L.AN, &L.AS, &L.AP, &L.b, &L.x, &L.r};
-        for (int k = 0; k < 8; ++k)
+        double** ptr[9] = {&L.AE, &L.AW, &L.AN, &L.AS, &L.AP, &L.b, &L.x, &L.r, &L.xt};
+        for (int k = 0; k < 9; ++k)
             *ptr[k] = blk + k * n + L.stride + 1;
     }
 }

@@ -11,11 +11,13 @@
 // it at neighbouring cells). The row-major copies of the other coefficients are refreshed only
 // when the host downloads them.
 //
-// The field itself (u or v) is also colour-packed while it is swept, over the whole array
-// (rows [0, nrows), columns [0, ncols)): the prep kernel packs it (and evaluates the residual from
-// the row-major copy in the same pass), the colour passes read and write only packed data (the
-// neighbours of a colour-c cell sit at the same or an adjacent packed index of another colour's
-// rows), and an unpack kernel restores the row-major field after the last pass.
+// Sweeps. One launch per multicolour sweep (all colours): a workgroup loads a tile of the field
+// plus a halo of 2*ncol cells into LDS and runs the colour passes there, pass k over the tile
+// grown by 2*(ncol-1-k) cells (the cells whose new values later passes of the tile still read),
+// so it reproduces the colour-by-colour sweep exactly; the tile's cells are then written to a
+// second array (tiles read their halo from the first), and sweeps alternate between the field and
+// a scratch copy. The first sweep of an outer iteration also evaluates the residual from the
+// loaded field before any update.
 //
 // Arithmetic is the reference's expression for expression (compiled with -ffp-contract=off, IEEE
 // division), so every value is bit-identical to src/momentum_u.cpp / momentum_v.cpp; the
@@ -177,28 +179,16 @@ __device__ __forceinline__ bool packed_cell(const MomArgs& m, int c, int& i, int
     return true;
 }
 
-// Packed index of any cell (i, j) in the whole-array field pack fp.
-__device__ __forceinline__ size_t fidx(const Pack& fp, int i, int j) {
-    const int c = colour_of(fp.ncol, i, j);
-    return pidx(fp, c, i, (j - jfirst(fp, c, i)) / fp.ncol);
-}
-
-// Prep, one thread per cell of the whole array: pack the field (row-major -> colour-packed) and,
-// inside the residual range, |R| with R = (nb + pterm) - AP x (src/momentum_u.cpp:48-50) from the
-// row-major field, into a compact buffer for the ordered sum.
-__global__ void __launch_bounds__(BX* BY)
-    k_mom_prep(MomArgs m, Pack fp, double* up, double* cell, int nr, int nc, int want_res,
-               const int* stop) {
+// |R| per cell in the residual range, R = (nb + pterm) - AP x (src/momentum_u.cpp:48-50), into a
+// compact buffer (one thread per cell; the per-phase entry point and sweepless iterations).
+__global__ void __launch_bounds__(BX* BY) k_mom_residual(MomArgs m, double* cell, const int* stop) {
     if (stopped(stop))
         return;
-    const int j = blockIdx.x * BX + threadIdx.x, i = blockIdx.y * BY + threadIdx.y;
-    if (i >= nr || j >= nc)
+    const int j = m.j0 + blockIdx.x * BX + threadIdx.x, i = m.i0 + blockIdx.y * BY + threadIdx.y;
+    if (i > m.i1 || j > m.j1)
         return;
     const long ld = m.ld;
     const size_t q = static_cast<size_t>(i) * ld + j;
-    up[fidx(fp, i, j)] = m.x[q];
-    if (!want_res || i < m.i0 || i > m.i1 || j < m.j0 || j > m.j1)
-        return;
     const Pack& pk = m.pk;
     const int c = colour_of(pk.ncol, i, j);
     const double* co = pk.base + pidx(pk, c, i, (j - jfirst(pk, c, i)) / pk.ncol);
@@ -207,37 +197,94 @@ __global__ void __launch_bounds__(BX* BY)
     cell[static_cast<size_t>(i - m.i0) * (m.j1 - m.j0 + 1) + (j - m.j0)] = fabs(R);
 }
 
-// Packed field -> row-major field over the whole array.
-__global__ void __launch_bounds__(BX* BY)
-    k_field_unpack(Pack fp, const double* up, double* x, int ld, int nr, int nc, const int* stop) {
-    if (stopped(stop))
-        return;
-    const int j = blockIdx.x * BX + threadIdx.x, i = blockIdx.y * BY + threadIdx.y;
-    if (i < nr && j < nc)
-        x[static_cast<size_t>(i) * ld + j] = up[fidx(fp, i, j)];
-}
+constexpr int ST = 1024; // sweep-tile workgroup size
 
-// One colour pass of the multicolour sweep (src/momentum_u.cpp:130-138) on the packed field.
-// Cells of one colour are never stencil neighbours, so the in-place update is race free and
-// equals the CPU's. Neighbour sums keep the order of nb_sum / near_sum.
-template <int NCOL>
-__global__ void __launch_bounds__(BX* BY)
-    k_mom_pass(MomArgs m, Pack fp, double* up, int col, const int* stop) {
+// One multicolour sweep (src/momentum_u.cpp:123-143) on a TI x TJ tile, in LDS: src -> dst.
+// first: also |R| of the tile's cells from the loaded field, into cell.
+template <int NCOL, int TI, int TJ>
+__global__ void __launch_bounds__(ST)
+    k_mom_sweep(MomArgs m, const double* src, double* dst, double* cell, int nr, int nc,
+                int first, const int* stop) {
     if (stopped(stop))
         return;
-    int i, j;
-    size_t o;
-    if (!packed_cell(m, col, i, j, o))
-        return;
-    const size_t as = m.pk.astride;
-    const double* c = m.pk.base + o;
-    const size_t self = fidx(fp, i, j);
-    double nb = c[0] * up[fidx(fp, i, j + 1)] + c[as] * up[fidx(fp, i, j - 1)] +
-                c[2 * as] * up[fidx(fp, i - 1, j)] + c[3 * as] * up[fidx(fp, i + 1, j)];
-    if (NCOL != 2)
-        nb = nb + (c[4 * as] * up[fidx(fp, i, j + 2)] + c[5 * as] * up[fidx(fp, i, j - 2)] +
-                   c[6 * as] * up[fidx(fp, i - 2, j)] + c[7 * as] * up[fidx(fp, i + 2, j)]);
-    up[self] = m.alpha * (nb + c[9 * as]) / c[8 * as] + m.one_minus_alpha * up[self];
+    constexpr int R = NCOL == 2 ? 1 : 2; // stencil reach
+    constexpr int HL = R * NCOL;          // halo read by the first pass
+    constexpr int LW = TJ + 2 * HL, LH = TI + 2 * HL;
+    __shared__ double su[LH * LW];
+    const int ti0 = blockIdx.y * TI, tj0 = blockIdx.x * TJ;
+    const int gi0 = ti0 - HL, gj0 = tj0 - HL; // array cell of su[0]
+    const long ld = m.ld;
+    // Batched loads: every load of a chunk is in flight before the first LDS store.
+    constexpr int NL = (LH * LW + ST - 1) / ST;
+    {
+        double v[NL];
+#pragma unroll
+        for (int k = 0; k < NL; ++k) {
+            const int e = k * ST + threadIdx.x, a = e / LW, b = e - a * LW, i = gi0 + a, j = gj0 + b;
+            v[k] = (e < LH * LW && i >= 0 && i < nr && j >= 0 && j < nc)
+                       ? src[static_cast<size_t>(i) * ld + j]
+                       : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < NL; ++k) {
+            const int e = k * ST + threadIdx.x;
+            if (e < LH * LW)
+                su[e] = v[k];
+        }
+    }
+    __syncthreads();
+    const Pack& pk = m.pk;
+    const size_t as = pk.astride;
+#define SU(i, j) su[((i) - gi0) * LW + ((j) - gj0)]
+    if (first) {
+        const int ia = max(m.i0, ti0), ib = min(m.i1, ti0 + TI - 1);
+        const int ja = max(m.j0, tj0), jb = min(m.j1, tj0 + TJ - 1);
+        const int w = jb - ja + 1, n = (ib - ia + 1) * w;
+        for (int e = threadIdx.x; e < n; e += ST) {
+            const int i = ia + e / w, j = ja + e % w;
+            const int c = colour_of(NCOL, i, j);
+            const double* co = pk.base + pidx(pk, c, i, (j - jfirst(pk, c, i)) / NCOL);
+            const double nb =
+                (co[0] * SU(i, j + 1) + co[as] * SU(i, j - 1) + co[2 * as] * SU(i - 1, j) +
+                 co[3 * as] * SU(i + 1, j)) +
+                (co[4 * as] * SU(i, j + 2) + co[5 * as] * SU(i, j - 2) +
+                 co[6 * as] * SU(i - 2, j) + co[7 * as] * SU(i + 2, j));
+            const double Rv = nb + co[9 * as] - co[8 * as] * SU(i, j);
+            cell[static_cast<size_t>(i - m.i0) * (m.j1 - m.j0 + 1) + (j - m.j0)] = fabs(Rv);
+        }
+        __syncthreads(); // the passes below overwrite values the residual reads
+    }
+    for (int k = 0; k < NCOL; ++k) {
+        const int d = R * (NCOL - 1 - k);
+        const int ia = max(m.i0, ti0 - d), ib = min(m.i1, ti0 + TI - 1 + d);
+        const int ja = max(m.j0, tj0 - d), jb = min(m.j1, tj0 + TJ - 1 + d);
+        if (ia <= ib && ja <= jb) {
+            const int cpr = (jb - ja + NCOL) / NCOL;
+            const int n = (ib - ia + 1) * cpr;
+            for (int e = threadIdx.x; e < n; e += ST) {
+                const int r = e / cpr, t = e - r * cpr, i = ia + r;
+                const int c0 = colour_of(NCOL, i, ja);
+                const int j = ja + (NCOL == 2 ? ((k - c0) & 1) : (2 * (k - c0) + 10) % 5) + NCOL * t;
+                if (j > jb)
+                    continue;
+                const double* co = pk.base + pidx(pk, k, i, (j - jfirst(pk, k, i)) / NCOL);
+                double nb = co[0] * SU(i, j + 1) + co[as] * SU(i, j - 1) +
+                            co[2 * as] * SU(i - 1, j) + co[3 * as] * SU(i + 1, j);
+                if (NCOL != 2)
+                    nb = nb + (co[4 * as] * SU(i, j + 2) + co[5 * as] * SU(i, j - 2) +
+                               co[6 * as] * SU(i - 2, j) + co[7 * as] * SU(i + 2, j));
+                SU(i, j) = m.alpha * (nb + co[9 * as]) / co[8 * as] + m.one_minus_alpha * SU(i, j);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < (TI * TJ + ST - 1) / ST; ++k) {
+        const int e = k * ST + threadIdx.x, a = e / TJ, b = e - a * TJ, i = ti0 + a, j = tj0 + b;
+        if (e < TI * TJ && i < nr && j < nc)
+            dst[static_cast<size_t>(i) * ld + j] = SU(i, j);
+    }
+#undef SU
 }
 
 // One Jacobi pass, dst from src (src/momentum_u.cpp:105-113); colour = blockIdx.z.
@@ -369,16 +416,17 @@ void launch_bc(Solver& s, bool is_u, const int* stop) {
     SG_HIP(hipGetLastError());
 }
 
-// Pack the field and evaluate the residual (residual_u / residual_v, src/momentum_u.cpp:43-59)
-// in one pass, then the ordered sum into *slot.
-// side: the sum runs on the side stream (joined before the stop rule).
+// residual_u / residual_v (src/momentum_u.cpp:43-59): |R| per cell, then the ordered sum into
+// *slot (on the side stream if side).
 void launch_mom_residual(Solver& s, bool is_u, double* slot, const int* stop, bool side) {
     const MomArgs m = mom_args(s, is_u);
     const int nrow = m.i1 - m.i0 + 1, w = m.j1 - m.j0 + 1;
     double* cell = s.rcell[is_u ? 0 : 1];
-    hipLaunchKernelGGL(k_mom_prep, dim3(cdiv(s.nc, BX), cdiv(s.nr, BY)), dim3(BX, BY), 0,
-                       s.stream, m, s.fpk, s.fbuf, cell, s.nr, s.nc, 1, stop);
-    SG_HIP(hipGetLastError());
+    if (nrow > 0 && w > 0) {
+        hipLaunchKernelGGL(k_mom_residual, dim3(cdiv(w, BX), cdiv(nrow, BY)), dim3(BX, BY), 0,
+                           s.stream, m, cell, stop);
+        SG_HIP(hipGetLastError());
+    }
     if (side)
         side_sum(s, is_u ? 1 : 2, cell, nrow, w, slot, stop);
     else if (nrow <= 0 || w <= 0)
@@ -387,44 +435,76 @@ void launch_mom_residual(Solver& s, bool is_u, double* slot, const int* stop, bo
         launch_ordered_sum(s, cell, w, 0, nrow - 1, 0, w - 1, slot, stop);
 }
 
-void launch_mom_pack(Solver& s, bool is_u, const int* stop) {
-    const MomArgs m = mom_args(s, is_u);
-    hipLaunchKernelGGL(k_mom_prep, dim3(cdiv(s.nc, BX), cdiv(s.nr, BY)), dim3(BX, BY), 0,
-                       s.stream, m, s.fpk, s.fbuf, s.rcell[0], s.nr, s.nc, 0, stop);
+namespace {
+template <int NCOL>
+void sweep_tile(Solver& s, const MomArgs& m, const double* src, double* dst, double* cell,
+                int first, const int* stop) {
+    const dim3 b(ST);
+    switch (s.mom_tile) {
+    case 0:
+        hipLaunchKernelGGL((k_mom_sweep<NCOL, 32, 64>), dim3(cdiv(s.nc, 64), cdiv(s.nr, 32)), b, 0,
+                           s.stream, m, src, dst, cell, s.nr, s.nc, first, stop);
+        break;
+    case 1:
+        hipLaunchKernelGGL((k_mom_sweep<NCOL, 64, 64>), dim3(cdiv(s.nc, 64), cdiv(s.nr, 64)), b, 0,
+                           s.stream, m, src, dst, cell, s.nr, s.nc, first, stop);
+        break;
+    case 2:
+        hipLaunchKernelGGL((k_mom_sweep<NCOL, 32, 128>), dim3(cdiv(s.nc, 128), cdiv(s.nr, 32)), b,
+                           0, s.stream, m, src, dst, cell, s.nr, s.nc, first, stop);
+        break;
+    default:
+        hipLaunchKernelGGL((k_mom_sweep<NCOL, 16, 128>), dim3(cdiv(s.nc, 128), cdiv(s.nr, 16)), b,
+                           0, s.stream, m, src, dst, cell, s.nr, s.nc, first, stop);
+    }
     SG_HIP(hipGetLastError());
 }
+} // namespace
 
-void launch_mom_colour_pass(Solver& s, bool is_u, int col, const int* stop) {
+// One multicolour sweep of the field into dst (tile shape s.mom_tile); first: also the residual
+// cells.
+void launch_mom_sweep_once(Solver& s, bool is_u, const double* src, double* dst, bool first,
+                           const int* stop) {
     const MomArgs m = mom_args(s, is_u);
-    if (m.i1 < m.i0 || m.j1 < m.j0)
-        return;
+    double* cell = s.rcell[is_u ? 0 : 1];
     if (m.pk.ncol == 2)
-        hipLaunchKernelGGL(k_mom_pass<2>, pass_grid(m, 1), dim3(BX, BY), 0, s.stream, m, s.fpk,
-                           s.fbuf, col, stop);
+        sweep_tile<2>(s, m, src, dst, cell, first ? 1 : 0, stop);
     else
-        hipLaunchKernelGGL(k_mom_pass<5>, pass_grid(m, 1), dim3(BX, BY), 0, s.stream, m, s.fpk,
-                           s.fbuf, col, stop);
-    SG_HIP(hipGetLastError());
+        sweep_tile<5>(s, m, src, dst, cell, first ? 1 : 0, stop);
 }
 
-// sweep_u_batch / sweep_v_batch (src/momentum_u.cpp:164-171) for the two parallel solvers.
-// Multicolour: the packed field must hold the current field (packed = the residual's prep ran
-// since the last change, else it is packed here); it is unpacked after the last pass.
-void launch_mom_sweeps(Solver& s, bool is_u, int nsweeps, const int* stop, bool packed) {
+// sweep_u_batch / sweep_v_batch (src/momentum_u.cpp:164-171) for the two parallel solvers. With
+// res_slot (multicolour only), the residual is evaluated inside the first sweep and summed (on
+// the side stream if side) into *res_slot.
+void launch_mom_sweeps(Solver& s, bool is_u, int nsweeps, const int* stop, double* res_slot,
+                       bool side) {
     const sg_params& p = s.p;
+    const MomArgs m = mom_args(s, is_u);
+    const bool fuse = res_slot && p.momentum_solver == SG_MOM_GS_MULTICOLOUR && nsweeps > 0;
+    if (res_slot && !fuse)
+        launch_mom_residual(s, is_u, res_slot, stop, side);
     if (nsweeps <= 0)
         return;
-    const MomArgs m = mom_args(s, is_u);
     if (p.momentum_solver == SG_MOM_GS_MULTICOLOUR) {
-        if (!packed)
-            launch_mom_pack(s, is_u, stop);
-        const int ncol = ncolours(p);
-        for (int k = 0; k < nsweeps; ++k)
-            for (int c = 0; c < ncol; ++c)
-                launch_mom_colour_pass(s, is_u, c, stop);
-        hipLaunchKernelGGL(k_field_unpack, dim3(cdiv(s.nc, BX), cdiv(s.nr, BY)), dim3(BX, BY), 0,
-                           s.stream, s.fpk, s.fbuf, m.x, s.ld, s.nr, s.nc, stop);
-        SG_HIP(hipGetLastError());
+        // Alternate field -> scratch -> field ...; an odd count ends with a copy back.
+        for (int k = 0; k < nsweeps; ++k) {
+            const bool even = (k % 2) == 0;
+            launch_mom_sweep_once(s, is_u, even ? m.x : s.scratch, even ? s.scratch : m.x,
+                                  fuse && k == 0, stop);
+        }
+        if (nsweeps % 2) {
+            hipLaunchKernelGGL(k_copy, dim3(1024), dim3(256), 0, s.stream, s.scratch, m.x, s.n,
+                               stop);
+            SG_HIP(hipGetLastError());
+        }
+        if (fuse) {
+            const int nrow = m.i1 - m.i0 + 1, w = m.j1 - m.j0 + 1;
+            if (side)
+                side_sum(s, is_u ? 1 : 2, s.rcell[is_u ? 0 : 1], nrow, w, res_slot, stop);
+            else
+                launch_ordered_sum(s, s.rcell[is_u ? 0 : 1], w, 0, nrow - 1, 0, w - 1, res_slot,
+                                   stop);
+        }
         return;
     }
     // Jacobi batch: full copy, then alternate so that the last pass writes the field.
@@ -440,20 +520,6 @@ void launch_mom_sweeps(Solver& s, bool is_u, int nsweeps, const int* stop, bool 
                            src, dst, stop);
         SG_HIP(hipGetLastError());
     }
-}
-
-// Whole-array field pack geometry (all rows and columns).
-Pack make_field_pack(const sg_params& p) {
-    Pack k{};
-    k.ncol = ncolours(p);
-    k.ai0 = 0;
-    k.aj0 = 0;
-    k.ai1 = p.nrows - 1;
-    k.aj1 = p.ncols - 1;
-    k.rows = p.nrows;
-    k.kp = ((p.ncols + k.ncol - 1) / k.ncol + 15) / 16 * 16;
-    k.astride = static_cast<size_t>(k.ncol) * k.rows * k.kp;
-    return k;
 }
 
 // Row-major array id <-> packed layout (coefficient ids only); p uploads refresh the pressure

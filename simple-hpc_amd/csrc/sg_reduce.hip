@@ -21,22 +21,32 @@ constexpr int CW = 1024; // columns per LDS chunk
 constexpr int PAD = 2;   // row padding (doubles) against bank conflicts
 constexpr int LOADS = RW * CW / NT;
 
-// s += v[0] + ... in order, from LDS, in batches of 8 independent reads.
+// s += v[0] + v[1] + ... in order, from LDS. The reads of the next batch of 8 are issued before
+// the 8 dependent additions of the current one, so LDS latency overlaps the add chain.
 __device__ __forceinline__ double add_in_order(double s, const double* v, int n) {
-    int c = 0;
-    for (; c + 8 <= n; c += 8) {
-        const double a0 = v[c], a1 = v[c + 1], a2 = v[c + 2], a3 = v[c + 3];
-        const double a4 = v[c + 4], a5 = v[c + 5], a6 = v[c + 6], a7 = v[c + 7];
-        s += a0;
-        s += a1;
-        s += a2;
-        s += a3;
-        s += a4;
-        s += a5;
-        s += a6;
-        s += a7;
+    const int nb = n / 8;
+    if (nb > 0) {
+        double a[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            a[q] = v[q];
+        for (int b = 1; b < nb; ++b) {
+            double c[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                c[q] = v[8 * b + q];
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                s += a[q];
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                a[q] = c[q];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            s += a[q];
     }
-    for (; c < n; ++c)
+    for (int c = 8 * nb; c < n; ++c)
         s += v[c];
     return s;
 }

@@ -216,3 +216,22 @@ def test_autotune_restores_state_and_keeps_parity():
     _, logg, _ = gpu.advance(6)
     assert bits_equal(logg, logc[6:, 1:])
     _same("u", gpu.get("u"), cpu.get("u"))
+
+
+@pytest.mark.parametrize("tile", [0, 1, 2, 3])
+@pytest.mark.parametrize("scheme,order,inner", [("QUICK", 4, 4), ("UDS", 2, 3), ("QUICK", 2, 1)])
+def test_sweep_tile_shapes_bit_identical(tile, scheme, order, inner):
+    """Every momentum sweep tile shape, odd and even sweep counts, 5 and 2 colours."""
+    cfg = (f"N_cells_x = 130\nN_cells_y = 200\nRe = 1000\nconvection_scheme = {scheme}\n"
+           f"diffusion_order = {order}\nmomentum_solver = gs_multicolour\n"
+           f"maxit_inner_u = {inner}\nmaxit_inner_v = {inner + 1}\n"
+           "pressure_solver = mg\nmaxit = 6\nerror_tolerance = 0\nbackend = gpu\n")
+    p, _ = sg.params_from_config(cfg)
+    cpu = po.Oracle(p)
+    _, logc, _ = cpu.run()
+    gpu = sg.Solver(p)
+    gpu.set_tuning(sg.TUNE_MOM_TILE, tile)
+    done, logg, _ = gpu.advance(6)
+    assert bits_equal(logg, logc[:, 1:]), first_diff(logg, logc[:, 1:])
+    for f in ("u", "v", "p"):
+        _same(f, gpu.get(f), cpu.get(f))
