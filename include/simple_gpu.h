@@ -193,6 +193,34 @@ int sg_get_tuning(sg_solver* s, int key, long* value);
  * (may be NULL); its last line names the multigrid choice. */
 int sg_autotune(sg_solver* s, int reps, char* report, size_t len);
 
+/* ---- Batched cavities ----
+ * No counterpart in the reference, which runs one cavity per process (src/main.cpp:40-46). A
+ * batch holds n cavities (zero initial fields, lid velocity 1) that may differ in N_cells_x,
+ * N_cells_y, Re, error_tolerance and the relaxation factors; every other key must agree. Each
+ * phase of the outer iteration runs as one persistent launch over all cavities, and one outer
+ * iteration of the whole batch is replayed as a captured hipGraph. Every cavity keeps its own
+ * stopping rule (src/solver.cpp:112-128) and, once stopped, is skipped; its results are exactly
+ * those of the same cavity run alone by sg_advance. */
+typedef struct sg_batch sg_batch;
+int sg_batch_create(const sg_params* params, int n, sg_batch** out); /* params[n], finalized */
+int sg_batch_destroy(sg_batch* b);
+int sg_batch_size(sg_batch* b, int* n);
+/* Borrowed handle of cavity k (sg_download, sg_iterations, ...); do not destroy it. */
+int sg_batch_solver(sg_batch* b, int k, sg_solver** s);
+/* Up to max_iters outer iterations of every cavity not yet stopped (the stopping rules are read
+ * every check_every iterations of cavity 0). Per cavity (arrays of n, each may be NULL): total
+ * outer iterations, status (1 converged, -1 diverged, 0 running) and the last logged residuals
+ * (3 per cavity). */
+int sg_batch_advance(sg_batch* b, int max_iters, int* iters, int* status, double* last_res);
+/* Kernel launches of one batched outer iteration (independent of n). */
+int sg_batch_launches(sg_batch* b, int* per_iteration);
+int sg_batch_set_tuning(sg_batch* b, int key, long value); /* SG_TUNE_* for every cavity */
+int sg_batch_get_tuning(sg_batch* b, int key, long* value);
+/* Autotuner: times the batch's own outer iteration for each momentum tile shape and a grid of
+ * multigrid tail knobs on the current state, restores u, v, p and the loop state, keeps the
+ * fastest; report format as sg_autotune. */
+int sg_batch_autotune(sg_batch* b, int reps, char* report, size_t len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -20,8 +20,20 @@ thread_local std::string g_err;
 // Stopping rule of run_outer_loop (src/solver.cpp:112-128), one thread: log the residuals,
 // update the running maxima, test max(...) <= tol && iter > 1 with std::max's comparison order.
 // A residual that is not finite stops the loop (diverged), which the reference does not detect.
-__global__ void k_stop_rule(Ctl* ctl, const double* scal, double* log, int log_cap, int norm,
-                            double tol) {
+struct StopArgs {
+    Ctl* ctl;
+    const double* scal;
+    double* log;
+    int log_cap, norm;
+    double tol;
+};
+
+__device__ __forceinline__ void k_stop_rule(const StopArgs& a, const Blk&) {
+    Ctl* ctl = a.ctl;
+    const double* scal = a.scal;
+    double* log = a.log;
+    const int log_cap = a.log_cap, norm = a.norm;
+    const double tol = a.tol;
     if (threadIdx.x != 0 || ctl->stop)
         return;
     const int iter = ++ctl->iter;
@@ -89,6 +101,8 @@ template <class T> T* dalloc(Solver& s, size_t n) {
     return static_cast<T*>(q);
 }
 
+} // namespace
+
 void ensure_log(Solver& s, int cap) {
     if (cap <= s.log_cap)
         return;
@@ -137,10 +151,13 @@ void enqueue_iteration(Solver& s, const int* stop, bool eager, hipEvent_t* ev) {
     mark(11);
     if (side)
         join_side(s);
-    hipLaunchKernelGGL(k_stop_rule, dim3(1), dim3(64), 0, s.stream, s.ctl, s.scal, s.dlog,
-                       s.log_cap, p.residual_normalisation, p.error_tolerance);
-    SG_HIP(hipGetLastError());
+    emit<StopArgs, k_stop_rule, 64>(s, dim3(1), dim3(64), 0,
+                                    StopArgs{s.ctl, s.scal, s.dlog, s.log_cap,
+                                             p.residual_normalisation, p.error_tolerance},
+                                    &s.ctl->stop);
 }
+
+namespace {
 
 void capture_graph(Solver& s) {
     if (s.graph)
@@ -316,27 +333,26 @@ int sg_params_finalize(sg_params* p) {
     });
 }
 
-int sg_create(const sg_params* pin, const double* u, const double* v, const double* pf,
-              sg_solver** out) {
-    Solver* sp = nullptr;
-    const int rc = guard([&] {
-        if (!pin || !out)
-            throw ArgError("null argument");
-        sg_params p = *pin;
-        validate(p);
-        if (p.nrows != p.NX + 2 * p.H + 1 || p.ncols != p.NY + 2 * p.H + 1)
-            throw ArgError("params not finalized (call sg_params_finalize)");
-        if (p.momentum_solver == SG_MOM_GS_LEX)
-            throw ArgError("backend = gpu runs momentum_solver = gs_multicolour or jacobi "
-                           "(gs_lex is a serial sweep)");
-        if (p.pressure_solver == SG_P_GS_LEX)
-            throw ArgError("backend = gpu runs pressure_solver = gs_redblack, jacobi, cg or mg "
-                           "(gs_lex is a serial sweep)");
-        int ndev = 0;
-        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
-            throw HipError("GPU error: no HIP device visible");
-        sp = new Solver;
-        Solver& s = *sp;
+} // extern "C"
+
+namespace sg {
+
+Solver* create_solver(const sg_params& p, const double* u, const double* v, const double* pf) {
+    validate(p);
+    if (p.nrows != p.NX + 2 * p.H + 1 || p.ncols != p.NY + 2 * p.H + 1)
+        throw ArgError("params not finalized (call sg_params_finalize)");
+    if (p.momentum_solver == SG_MOM_GS_LEX)
+        throw ArgError("backend = gpu runs momentum_solver = gs_multicolour or jacobi "
+                       "(gs_lex is a serial sweep)");
+    if (p.pressure_solver == SG_P_GS_LEX)
+        throw ArgError("backend = gpu runs pressure_solver = gs_redblack, jacobi, cg or mg "
+                       "(gs_lex is a serial sweep)");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        throw HipError("GPU error: no HIP device visible");
+    Solver* sp = new Solver;
+    Solver& s = *sp;
+    try {
         s.p = p;
         SG_HIP(hipGetDevice(&s.device));
         {
@@ -387,20 +403,14 @@ int sg_create(const sg_params* pin, const double* u, const double* v, const doub
                                    sizeof(double) * s.nc, sizeof(double) * s.nc, s.nr,
                                    hipMemcpyHostToDevice));
         SG_HIP(hipDeviceSynchronize());
-    });
-    if (rc != SG_OK) {
-        if (sp)
-            sg_destroy(reinterpret_cast<sg_solver*>(sp));
-        return rc;
+    } catch (...) {
+        destroy_solver(sp);
+        throw;
     }
-    *out = reinterpret_cast<sg_solver*>(sp);
-    return SG_OK;
+    return sp;
 }
 
-int sg_destroy(sg_solver* h) {
-    if (!h)
-        return SG_OK;
-    Solver* s = reinterpret_cast<Solver*>(h);
+void destroy_solver(Solver* s) {
     if (s->stream)
         (void)hipStreamSynchronize(s->stream);
     if (s->graph)
@@ -427,6 +437,24 @@ int sg_destroy(sg_solver* h) {
     if (s->stream)
         (void)hipStreamDestroy(s->stream);
     delete s;
+}
+
+} // namespace sg
+
+extern "C" {
+
+int sg_create(const sg_params* pin, const double* u, const double* v, const double* pf,
+              sg_solver** out) {
+    return guard([&] {
+        if (!pin || !out)
+            throw ArgError("null argument");
+        *out = reinterpret_cast<sg_solver*>(create_solver(*pin, u, v, pf));
+    });
+}
+
+int sg_destroy(sg_solver* h) {
+    if (h)
+        destroy_solver(reinterpret_cast<Solver*>(h));
     return SG_OK;
 }
 
@@ -523,6 +551,7 @@ int sg_outer_iteration(sg_solver* h, double res[3], double* phase_s) {
     return guard([&] {
         Solver& s = S(h);
         hipLaunchKernelGGL(k_ctl_begin, dim3(1), dim3(64), 0, s.stream, s.ctl);
+        SG_HIP(hipGetLastError());
         enqueue_iteration(s, s.never, true, s.ev);
         if (res)
             SG_HIP(hipMemcpyAsync(res, s.scal + S_RES_U, 3 * sizeof(double),
@@ -545,6 +574,7 @@ int sg_advance(sg_solver* h, int max_iters, int* iters_done, double* log, int* c
             throw ArgError("max_iters must be >= 0");
         ensure_log(s, max_iters);
         hipLaunchKernelGGL(k_ctl_begin, dim3(1), dim3(64), 0, s.stream, s.ctl);
+        SG_HIP(hipGetLastError());
         SG_HIP(hipGetLastError());
         if (s.p.pressure_solver == SG_P_CG) {
             // CG needs the host between its iterations: eager iterations, stop flag checked.

@@ -10,6 +10,7 @@
 // converged iteration do no work.
 #pragma once
 #include "../../include/simple_gpu.h"
+#include "sg_launch.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -122,10 +123,41 @@ struct Solver {
     hipEvent_t ev[SG_NPHASES + 1] = {};
     // kernel timing
     hipEvent_t kt0 = nullptr, kt1 = nullptr;
+    // Recording (batched execution): launches are appended here instead of enqueued; tag is the
+    // structural label of the launches being enqueued (multigrid level and cycle, else 0).
+    std::vector<Rec>* rec = nullptr;
+    int tag = 0;
+    int vcount = 0; // V-cycles enqueued in the current p' solve
 
     const double* F(int id) const { return f[id]; }
     double* F(int id) { return f[id]; }
 };
+
+// Launch body B over `grid` for this cavity on `st` (the main stream if null), or record it.
+template <class A, void (*B)(const A&, const Blk&), int NT>
+void emit(Solver& s, dim3 grid, dim3 block, size_t lds, const A& a, const int* stop,
+          hipStream_t st = nullptr) {
+    if (grid.x == 0 || grid.y == 0 || grid.z == 0)
+        return;
+    if (s.rec) {
+        Rec r;
+        r.vt = &Op<A, B, NT>::vt;
+        r.args.resize(sizeof(A));
+        std::memcpy(r.args.data(), &a, sizeof(A));
+        r.grid = grid;
+        r.block = block;
+        r.lds = lds;
+        r.stop = stop;
+        r.side = st != nullptr && st != s.stream;
+        r.tag = s.tag;
+        for (const Rec& q : *s.rec)
+            r.occ += (q.vt == r.vt && q.side == r.side && q.tag == r.tag) ? 1 : 0;
+        s.rec->push_back(std::move(r));
+        return;
+    }
+    hipLaunchKernelGGL((k_one<A, B, NT>), grid, block, lds, st ? st : s.stream, a, stop);
+    SG_HIP(hipGetLastError());
+}
 
 // ---- launchers (each enqueues on s.stream; `stop` may be s.never) ----
 // momentum (sg_momentum.hip)
@@ -160,7 +192,12 @@ void mg_setup(Solver& s);
 void launch_mg_solve(Solver& s, const int* stop, bool has_guess);
 void launch_remove_mean0(Solver& s, double* a, const int* stop);
 // outer loop control (sg_api.hip)
-void launch_stop_rule(Solver& s);
+
+// One outer iteration (src/solver.cpp:27-93), enqueued or recorded. ev: optional phase events.
+void enqueue_iteration(Solver& s, const int* stop, bool eager, hipEvent_t* ev);
+void ensure_log(Solver& s, int cap);
+Solver* create_solver(const sg_params& p, const double* u, const double* v, const double* pf);
+void destroy_solver(Solver* s);
 
 inline unsigned cdiv(long a, long b) { return static_cast<unsigned>((a + b - 1) / b); }
 

@@ -88,31 +88,35 @@ template <class T> __device__ __forceinline__ T resid_cell(const Lv<T>& L, size_
 
 // ---- multi-workgroup kernels for the large levels ----
 
-template <class T> __global__ void __launch_bounds__(BX* BY) k_mg_build(Lv<T> F, Lv<T> C, const int* stop) {
-    if (stopped(stop))
-        return;
-    const int J = blockIdx.x * BX + threadIdx.x, I = blockIdx.y * BY + threadIdx.y;
-    if (I < C.nx && J < C.ny)
-        build_cell(F, C, I, J);
+// Two levels (fine F, coarse C) and a colour or over-correction weight.
+template <class T> struct FC {
+    Lv<T> F, C;
+    T omega;
+    int col;
+};
+
+template <class T> __device__ __forceinline__ void k_mg_build(const FC<T>& a, const Blk& bk) {
+    const int J = bk.x * BX + threadIdx.x, I = bk.y * BY + threadIdx.y;
+    if (I < a.C.nx && J < a.C.ny)
+        build_cell(a.F, a.C, I, J);
 }
 
 // One colour of the red-black smoother, colour (I + J) & 1 (smooth, :46-70).
-template <class T> __global__ void __launch_bounds__(BX* BY) k_mg_smooth(Lv<T> L, int col, const int* stop) {
-    if (stopped(stop))
-        return;
-    const int I = blockIdx.y * BY + threadIdx.y;
-    const int J = ((I + col) & 1) + 2 * (blockIdx.x * BX + threadIdx.x);
+template <class T> __device__ __forceinline__ void k_mg_smooth(const FC<T>& a, const Blk& bk) {
+    const Lv<T>& L = a.F;
+    const int col = a.col;
+    const int I = bk.y * BY + threadIdx.y;
+    const int J = ((I + col) & 1) + 2 * (bk.x * BX + threadIdx.x);
     if (I < L.nx && J < L.ny)
         smooth_cell(L, static_cast<size_t>(I) * L.stride + J);
 }
 
 // Residual and restriction fused: coarse b = r(NW) + r(NE) + r(SW) + r(SE), coarse x = 0
 // (residual :73-87 then restrict_residual :90-103).
-template <class T>
-__global__ void __launch_bounds__(BX* BY) k_mg_resid_restrict(Lv<T> F, Lv<T> C, const int* stop) {
-    if (stopped(stop))
-        return;
-    const int J = blockIdx.x * BX + threadIdx.x, I = blockIdx.y * BY + threadIdx.y;
+template <class T> __device__ __forceinline__ void k_mg_resid_restrict(const FC<T>& a, const Blk& bk) {
+    const Lv<T>& F = a.F;
+    const Lv<T>& C = a.C;
+    const int J = bk.x * BX + threadIdx.x, I = bk.y * BY + threadIdx.y;
     if (I >= C.nx || J >= C.ny)
         return;
     const long fs = F.stride;
@@ -125,13 +129,12 @@ __global__ void __launch_bounds__(BX* BY) k_mg_resid_restrict(Lv<T> F, Lv<T> C, 
 
 // Prolongation fused into the first post-smoothing pass (colour 0): neighbours inside the level
 // are read as x + omega * xc (prolong, :106-116).
-template <class T>
-__global__ void __launch_bounds__(BX* BY)
-    k_mg_prolong_smooth0(Lv<T> C, Lv<T> F, T omega, const int* stop) {
-    if (stopped(stop))
-        return;
-    const int I = blockIdx.y * BY + threadIdx.y;
-    const int J = (I & 1) + 2 * (blockIdx.x * BX + threadIdx.x);
+template <class T> __device__ __forceinline__ void k_mg_prolong_smooth0(const FC<T>& a, const Blk& bk) {
+    const Lv<T>& F = a.F;
+    const Lv<T>& C = a.C;
+    const T omega = a.omega;
+    const int I = bk.y * BY + threadIdx.y;
+    const int J = (I & 1) + 2 * (bk.x * BX + threadIdx.x);
     if (I >= F.nx || J >= F.ny)
         return;
     const long s = F.stride;
@@ -152,14 +155,13 @@ __global__ void __launch_bounds__(BX* BY)
 }
 
 // Plain prolongation (only when mg_post = 0).
-template <class T>
-__global__ void __launch_bounds__(BX* BY) k_mg_prolong(Lv<T> C, Lv<T> F, T omega, const int* stop) {
-    if (stopped(stop))
-        return;
-    const int j = blockIdx.x * BX + threadIdx.x, i = blockIdx.y * BY + threadIdx.y;
+template <class T> __device__ __forceinline__ void k_mg_prolong(const FC<T>& a, const Blk& bk) {
+    const Lv<T>& F = a.F;
+    const Lv<T>& C = a.C;
+    const int j = bk.x * BX + threadIdx.x, i = bk.y * BY + threadIdx.y;
     if (i < F.nx && j < F.ny)
         F.x[static_cast<size_t>(i) * F.stride + j] +=
-            omega * C.x[static_cast<size_t>(i / 2) * C.stride + j / 2];
+            a.omega * C.x[static_cast<size_t>(i / 2) * C.stride + j / 2];
 }
 
 // ---- tiled V-cycle kernels for the levels above the tail ----
@@ -244,15 +246,26 @@ __device__ __forceinline__ void tile_pass(T* sx, const Lv<T>& L, int gi0, int gj
     __syncthreads();
 }
 
-template <class T>
-__global__ void __launch_bounds__(MT_N)
-    k_mg_down(Lv<T> F, Lv<T> C, const T* xin, T* xout, int pre, int build, const int* stop) {
-    if (stopped(stop))
-        return;
+// Tiled down / up step of one level: F, C, the iterate read (xin) and written (xout), the sweep
+// count (pre or post), whether to build the coarse operator, the over-correction weight.
+template <class T> struct DU {
+    Lv<T> F, C;
+    const T* xin;
+    T* xout;
+    int sweeps, build;
+    T omega;
+};
+
+template <class T> __device__ __forceinline__ void k_mg_down(const DU<T>& a, const Blk& bk) {
+    const Lv<T>& F = a.F;
+    const Lv<T>& C = a.C;
+    const T* xin = a.xin;
+    T* xout = a.xout;
+    const int pre = a.sweeps, build = a.build;
     extern __shared__ __align__(16) unsigned char smem[];
     T* sx = reinterpret_cast<T*>(smem);
     const int H = 2 * pre + 1, LW = MT_J + 2 * H, LH = MT_I + 2 * H;
-    const int ti0 = blockIdx.y * MT_I, tj0 = blockIdx.x * MT_J;
+    const int ti0 = bk.y * MT_I, tj0 = bk.x * MT_J;
     const int gi0 = ti0 - H, gj0 = tj0 - H;
     tile_load<T>(sx, F, xin, gi0, gj0, LH, LW, nullptr, T(0));
     for (int p = 0; p < 2 * pre; ++p) {
@@ -289,15 +302,17 @@ __global__ void __launch_bounds__(MT_N)
     }
 }
 
-template <class T>
-__global__ void __launch_bounds__(MT_N)
-    k_mg_up(Lv<T> F, Lv<T> C, const T* xpre, T* xfin, T omega, int post, const int* stop) {
-    if (stopped(stop))
-        return;
+template <class T> __device__ __forceinline__ void k_mg_up(const DU<T>& a, const Blk& bk) {
+    const Lv<T>& F = a.F;
+    const Lv<T>& C = a.C;
+    const T* xpre = a.xin;
+    T* xfin = a.xout;
+    const T omega = a.omega;
+    const int post = a.sweeps;
     extern __shared__ __align__(16) unsigned char smem[];
     T* sx = reinterpret_cast<T*>(smem);
     const int H = 2 * post, LW = MT_J + 2 * H, LH = MT_I + 2 * H;
-    const int ti0 = blockIdx.y * MT_I, tj0 = blockIdx.x * MT_J;
+    const int ti0 = bk.y * MT_I, tj0 = bk.x * MT_J;
     const int gi0 = ti0 - H, gj0 = tj0 - H;
     tile_load<T>(sx, F, xpre, gi0, gj0, LH, LW, &C, omega);
     for (int p = 0; p < 2 * post; ++p) {
@@ -330,6 +345,7 @@ template <class T> struct Tail {
     T omega;
     int lds_elems;   // elements of all LDS level blocks; the row-sum scratch follows
     T* grows;        // row-sum scratch in global memory (t0 = nl)
+    Ctl* ctl;        // receives the coarse-solve sweep count
 };
 
 template <class T> __device__ __forceinline__ Lv<T> lds_view(const Tail<T>& t, T* sm, int l) {
@@ -444,10 +460,7 @@ template <class T> __device__ void wg_copy_level(const Lv<T>& D, const Lv<T>& S,
     __syncthreads();
 }
 
-template <class T>
-__global__ void __launch_bounds__(1024) k_mg_tail(Tail<T> t, Ctl* ctl, const int* stop) {
-    if (stopped(stop))
-        return;
+template <class T> __device__ __forceinline__ void k_mg_tail(const Tail<T>& t, const Blk&) {
     extern __shared__ __align__(16) unsigned char smem[];
     T* sm = reinterpret_cast<T*>(smem);
     for (int e = threadIdx.x; e < t.lds_elems; e += blockDim.x)
@@ -506,7 +519,7 @@ __global__ void __launch_bounds__(1024) k_mg_tail(Tail<T> t, Ctl* ctl, const int
         }
     }
     if (threadIdx.x == 0)
-        ctl->coarse_sweeps = sweeps;
+        t.ctl->coarse_sweeps = sweeps;
 }
 
 // ---- mixed precision: FP64 <-> FP32 level-0 transfers ----
@@ -519,10 +532,8 @@ struct Mix {
 };
 
 // FP32 copies of the level-0 operator; rhs = b (first step, p' = 0), x = 0.
-__global__ void __launch_bounds__(BX* BY) k_mx_init(Mix m, const int* stop) {
-    if (stopped(stop))
-        return;
-    const int J = blockIdx.x * BX + threadIdx.x, I = blockIdx.y * BY + threadIdx.y;
+__device__ __forceinline__ void k_mx_init(const Mix& m, const Blk& bk) {
+    const int J = bk.x * BX + threadIdx.x, I = bk.y * BY + threadIdx.y;
     if (I >= m.nx || J >= m.ny)
         return;
     const size_t q = static_cast<size_t>(I) * m.ld + J, g = static_cast<size_t>(I) * m.f.stride + J;
@@ -536,10 +547,8 @@ __global__ void __launch_bounds__(BX* BY) k_mx_init(Mix m, const int* stop) {
 }
 
 // FP64 defect r = b - A p' (level-0 operator) as the FP32 right-hand side; x = 0.
-__global__ void __launch_bounds__(BX* BY) k_mx_defect(Mix m, const int* stop) {
-    if (stopped(stop))
-        return;
-    const int J = blockIdx.x * BX + threadIdx.x, I = blockIdx.y * BY + threadIdx.y;
+__device__ __forceinline__ void k_mx_defect(const Mix& m, const Blk& bk) {
+    const int J = bk.x * BX + threadIdx.x, I = bk.y * BY + threadIdx.y;
     if (I >= m.nx || J >= m.ny)
         return;
     const long s = m.ld;
@@ -552,10 +561,8 @@ __global__ void __launch_bounds__(BX* BY) k_mx_defect(Mix m, const int* stop) {
 }
 
 // p' += e (FP64 update).
-__global__ void __launch_bounds__(BX* BY) k_mx_update(Mix m, const int* stop) {
-    if (stopped(stop))
-        return;
-    const int J = blockIdx.x * BX + threadIdx.x, I = blockIdx.y * BY + threadIdx.y;
+__device__ __forceinline__ void k_mx_update(const Mix& m, const Blk& bk) {
+    const int J = bk.x * BX + threadIdx.x, I = bk.y * BY + threadIdx.y;
     if (I >= m.nx || J >= m.ny)
         return;
     m.x[static_cast<size_t>(I) * m.ld + J] +=
@@ -564,13 +571,17 @@ __global__ void __launch_bounds__(BX* BY) k_mx_update(Mix m, const int* stop) {
 
 dim3 lgrid(int nx, int ny) { return dim3(cdiv(ny, BX), cdiv(nx, BY)); }
 
+// Launch labels of the multigrid steps, so that a batch of cavities with different numbers of
+// levels groups the same step of the same V-cycle: level l of V-cycle v, going down or up.
+int mg_tag(int v, int up, int l) { return 1 + (v * 2 + up) * 32 + l; }
+constexpr int TAG_TAIL = 31; // level slot of the tail
+constexpr int TAG_BUILD = 1 << 20;
+
 template <class T>
 void smooth_passes(Solver& s, const Lv<T>& L, int sweeps, int first_col, const int* stop) {
-    for (int k = first_col; k < 2 * sweeps; ++k) {
-        hipLaunchKernelGGL(k_mg_smooth<T>, lgrid(L.nx, cdiv(L.ny, 2)), dim3(BX, BY), 0, s.stream,
-                           L, k & 1, stop);
-        SG_HIP(hipGetLastError());
-    }
+    for (int k = first_col; k < 2 * sweeps; ++k)
+        emit<FC<T>, k_mg_smooth<T>, BX * BY>(s, lgrid(L.nx, cdiv(L.ny, 2)), dim3(BX, BY), 0,
+                                            FC<T>{L, L, T(0), k & 1}, stop);
 }
 
 // LDS bytes of levels t0.. (level blocks plus the row-sum scratch).
@@ -618,66 +629,59 @@ template <class T> void launch_builds(Solver& s, const Tail<T>& t, const int* st
     if (tiled_ok(s))
         return;
     for (int l = 1; l < t.tm; ++l) {
-        hipLaunchKernelGGL(k_mg_build<T>, lgrid(t.G[l].nx, t.G[l].ny), dim3(BX, BY), 0, s.stream,
-                           t.G[l - 1], t.G[l], stop);
-        SG_HIP(hipGetLastError());
+        s.tag = TAG_BUILD + l;
+        emit<FC<T>, k_mg_build<T>, BX * BY>(s, lgrid(t.G[l].nx, t.G[l].ny), dim3(BX, BY), 0,
+                                           FC<T>{t.G[l - 1], t.G[l], T(0), 0}, stop);
     }
+    s.tag = 0;
 }
 
 // mg_cycles V-cycles (mg_vcycle, :299-323) on level 0 of t.
 template <class T> void launch_vcycles(Solver& s, const Tail<T>& t, const int* stop) {
     const sg_params& p = s.p;
     const size_t lds = lds_bytes<T>(s, t.t0);
-    if (tiled_ok(s)) {
-        const size_t dn = sizeof(T) * (MT_I + 2 * (2 * p.mg_pre + 1)) * (MT_J + 2 * (2 * p.mg_pre + 1));
-        const size_t up = sizeof(T) * (MT_I + 4 * p.mg_post) * (MT_J + 4 * p.mg_post);
-        for (int cyc = 0; cyc < p.mg_cycles; ++cyc) {
-            for (int l = 0; l < t.tm; ++l) {
-                const Lv<T> F = t.G[l], C = t.G[l + 1];
+    const bool tiled = tiled_ok(s);
+    const size_t dn = sizeof(T) * (MT_I + 2 * (2 * p.mg_pre + 1)) * (MT_J + 2 * (2 * p.mg_pre + 1));
+    const size_t up = sizeof(T) * (MT_I + 4 * p.mg_post) * (MT_J + 4 * p.mg_post);
+    for (int cyc = 0; cyc < p.mg_cycles; ++cyc) {
+        const int v = s.vcount++;
+        for (int l = 0; l < t.tm; ++l) {
+            const Lv<T> F = t.G[l], C = t.G[l + 1];
+            s.tag = mg_tag(v, 0, l);
+            if (tiled) {
                 // The first cycle builds level l+1 here unless the tail builds it (l+1 = tm).
                 const int build = (cyc == 0 && l + 1 < t.tm) ? 1 : 0;
-                hipLaunchKernelGGL(k_mg_down<T>, dim3(cdiv(F.ny, MT_J), cdiv(F.nx, MT_I)),
-                                   dim3(MT_N), dn, s.stream, F, C, F.x, F.xt, p.mg_pre, build,
-                                   stop);
-                SG_HIP(hipGetLastError());
-            }
-            hipLaunchKernelGGL(k_mg_tail<T>, dim3(1), dim3(s.tail_threads), lds, s.stream, t,
-                               s.ctl, stop);
-            SG_HIP(hipGetLastError());
-            for (int l = t.tm - 1; l >= 0; --l) {
-                const Lv<T> F = t.G[l], C = t.G[l + 1];
-                hipLaunchKernelGGL(k_mg_up<T>, dim3(cdiv(F.ny, MT_J), cdiv(F.nx, MT_I)),
-                                   dim3(MT_N), up, s.stream, F, C, F.xt, F.x, t.omega, p.mg_post,
-                                   stop);
-                SG_HIP(hipGetLastError());
+                emit<DU<T>, k_mg_down<T>, MT_N>(s, dim3(cdiv(F.ny, MT_J), cdiv(F.nx, MT_I)),
+                                                dim3(MT_N), dn,
+                                                DU<T>{F, C, F.x, F.xt, p.mg_pre, build, T(0)},
+                                                stop);
+            } else {
+                smooth_passes(s, F, p.mg_pre, 0, stop);
+                emit<FC<T>, k_mg_resid_restrict<T>, BX * BY>(s, lgrid(C.nx, C.ny), dim3(BX, BY),
+                                                             0, FC<T>{F, C, T(0), 0}, stop);
             }
         }
-        return;
-    }
-    for (int cyc = 0; cyc < p.mg_cycles; ++cyc) {
-        for (int l = 0; l < t.tm; ++l) {
-            smooth_passes(s, t.G[l], p.mg_pre, 0, stop);
-            hipLaunchKernelGGL(k_mg_resid_restrict<T>, lgrid(t.G[l + 1].nx, t.G[l + 1].ny),
-                               dim3(BX, BY), 0, s.stream, t.G[l], t.G[l + 1], stop);
-            SG_HIP(hipGetLastError());
-        }
-        hipLaunchKernelGGL(k_mg_tail<T>, dim3(1), dim3(s.tail_threads), lds, s.stream, t, s.ctl,
-                           stop);
-        SG_HIP(hipGetLastError());
+        s.tag = mg_tag(v, 0, TAG_TAIL);
+        emit<Tail<T>, k_mg_tail<T>, 1024>(s, dim3(1), dim3(s.tail_threads), lds, t, stop);
         for (int l = t.tm - 1; l >= 0; --l) {
             const Lv<T> F = t.G[l], C = t.G[l + 1];
-            if (p.mg_post > 0) {
-                hipLaunchKernelGGL(k_mg_prolong_smooth0<T>, lgrid(F.nx, cdiv(F.ny, 2)),
-                                   dim3(BX, BY), 0, s.stream, C, F, t.omega, stop);
-                SG_HIP(hipGetLastError());
+            s.tag = mg_tag(v, 1, l);
+            if (tiled) {
+                emit<DU<T>, k_mg_up<T>, MT_N>(s, dim3(cdiv(F.ny, MT_J), cdiv(F.nx, MT_I)),
+                                              dim3(MT_N), up,
+                                              DU<T>{F, C, F.xt, F.x, p.mg_post, 0, t.omega}, stop);
+            } else if (p.mg_post > 0) {
+                emit<FC<T>, k_mg_prolong_smooth0<T>, BX * BY>(s, lgrid(F.nx, cdiv(F.ny, 2)),
+                                                              dim3(BX, BY), 0,
+                                                              FC<T>{F, C, t.omega, 0}, stop);
                 smooth_passes(s, F, p.mg_post, 1, stop);
             } else {
-                hipLaunchKernelGGL(k_mg_prolong<T>, lgrid(F.nx, F.ny), dim3(BX, BY), 0, s.stream,
-                                   C, F, t.omega, stop);
-                SG_HIP(hipGetLastError());
+                emit<FC<T>, k_mg_prolong<T>, BX * BY>(s, lgrid(F.nx, F.ny), dim3(BX, BY), 0,
+                                                      FC<T>{F, C, t.omega, 0}, stop);
             }
         }
     }
+    s.tag = 0;
 }
 
 template <class T> Tail<T> make_tail(Solver& s, Lv<T> (*view)(const MgLevel&)) {
@@ -692,6 +696,7 @@ template <class T> Tail<T> make_tail(Solver& s, Lv<T> (*view)(const MgLevel&)) {
     t.post = s.p.mg_post;
     t.omega = static_cast<T>(s.p.mg_omega);
     t.grows = reinterpret_cast<T*>(s.rowbuf);
+    t.ctl = s.ctl;
     int e = 0;
     for (int l = t.t0; l < nl; ++l) {
         t.off[l] = e;
@@ -700,7 +705,7 @@ template <class T> Tail<T> make_tail(Solver& s, Lv<T> (*view)(const MgLevel&)) {
     t.lds_elems = e;
     const size_t bytes = lds_bytes<T>(s, t.t0);
     if (bytes > 65536)
-        SG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mg_tail<T>),
+        SG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_one<Tail<T>, k_mg_tail<T>, 1024>),
                                    hipFuncAttributeMaxDynamicSharedMemorySize,
                                    static_cast<int>(bytes)));
     return t;
@@ -779,6 +784,7 @@ void mg_setup(Solver& s) {
 // initial guess and every step starts from its FP64 defect).
 void launch_mg_solve(Solver& s, const int* stop, bool has_guess) {
     const sg_params& p = s.p;
+    s.vcount = 0;
     if (p.mg_precision != SG_MG_MIXED) {
         const Tail<double> t = make_tail<double>(s, lv64);
         launch_builds(s, t, stop);
@@ -795,17 +801,14 @@ void launch_mg_solve(Solver& s, const int* stop, bool has_guess) {
     if (!has_guess)
         launch_fill(s, s.F(SG_F_PC), s.n, 0.0, stop);
     launch_remove_mean0(s, s.F(SG_F_B), stop);
-    hipLaunchKernelGGL(k_mx_init, g, dim3(BX, BY), 0, s.stream, m, stop);
-    SG_HIP(hipGetLastError());
+    emit<Mix, k_mx_init, BX * BY>(s, g, dim3(BX, BY), 0, m, stop);
     launch_builds(s, t, stop);
     for (int k = 0; k < p.mg_dc_maxit; ++k) {
         if (k > 0 || has_guess) {
-            hipLaunchKernelGGL(k_mx_defect, g, dim3(BX, BY), 0, s.stream, m, stop);
-            SG_HIP(hipGetLastError());
+            emit<Mix, k_mx_defect, BX * BY>(s, g, dim3(BX, BY), 0, m, stop);
         }
         launch_vcycles(s, t, stop);
-        hipLaunchKernelGGL(k_mx_update, g, dim3(BX, BY), 0, s.stream, m, stop);
-        SG_HIP(hipGetLastError());
+        emit<Mix, k_mx_update, BX * BY>(s, g, dim3(BX, BY), 0, m, stop);
     }
     launch_remove_mean0(s, s.F(SG_F_PC), stop);
 }

@@ -64,13 +64,10 @@ struct AsmArgs {
 // One control volume: convection (src/coefficients.cpp:15-39) plus diffusion (:41-63,
 // precomputed on the host as d2 / d4), summed element-wise (:5-8); face velocities of
 // src/momentum_u.cpp:13-16 or src/momentum_v.cpp:14-17.
-template <bool IS_U>
-__global__ void __launch_bounds__(BX* BY) k_assemble(AsmArgs a, const int* stop) {
-    if (stopped(stop))
-        return;
+template <bool IS_U> __device__ __forceinline__ void k_assemble(const AsmArgs& a, const Blk& bk) {
     const Pack& pk = a.pk;
-    const int j = pk.aj0 + blockIdx.x * BX + threadIdx.x;
-    const int i = pk.ai0 + blockIdx.y * BY + threadIdx.y;
+    const int j = pk.aj0 + bk.x * BX + threadIdx.x;
+    const int i = pk.ai0 + bk.y * BY + threadIdx.y;
     if (i > pk.ai1 || j > pk.aj1)
         return;
     const long ld = a.ld;
@@ -128,20 +125,26 @@ __global__ void __launch_bounds__(BX* BY) k_assemble(AsmArgs a, const int* stop)
 
 // Ghost cells: u lid 2 - u and south -u (src/momentum_u.cpp:24-34); v west/east -v
 // (src/momentum_v.cpp:24-34).
-__global__ void k_bc_u(double* u, int ld, int H, int NX, int NY, const int* stop) {
-    if (stopped(stop))
-        return;
-    const int j = H + 1 + blockIdx.x * blockDim.x + threadIdx.x;
+struct BcArgs {
+    double* x;
+    int ld, H, NX, NY;
+};
+constexpr int BC_T = 256;
+
+__device__ __forceinline__ void k_bc_u(const BcArgs& a, const Blk& bk) {
+    double* u = a.x;
+    const int ld = a.ld, H = a.H, NX = a.NX, NY = a.NY;
+    const int j = H + 1 + bk.x * BC_T + threadIdx.x;
     if (j > NY + H)
         return;
     u[static_cast<size_t>(H - 1) * ld + j] = 2 - u[static_cast<size_t>(H) * ld + j];
     u[static_cast<size_t>(NX + H) * ld + j] = -u[static_cast<size_t>(NX + H - 1) * ld + j];
 }
 
-__global__ void k_bc_v(double* v, int ld, int H, int NX, int NY, const int* stop) {
-    if (stopped(stop))
-        return;
-    const int i = H + blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_bc_v(const BcArgs& a, const Blk& bk) {
+    double* v = a.x;
+    const int ld = a.ld, H = a.H, NX = a.NX, NY = a.NY;
+    const int i = H + bk.x * BC_T + threadIdx.x;
     if (i > NX + H - 1)
         return;
     const size_t r = static_cast<size_t>(i) * ld;
@@ -166,12 +169,13 @@ __device__ __forceinline__ double nb9(const double* c, size_t as, const double* 
 }
 // Cell (i, j) of colour c handled by this thread in the sweep range, or false; o = its
 // coefficient-pack index.
-__device__ __forceinline__ bool packed_cell(const MomArgs& m, int c, int& i, int& j, size_t& o) {
-    i = m.i0 + blockIdx.y * BY + threadIdx.y;
+__device__ __forceinline__ bool packed_cell(const MomArgs& m, const Blk& bk, int c, int& i,
+                                            int& j, size_t& o) {
+    i = m.i0 + bk.y * BY + threadIdx.y;
     if (i > m.i1)
         return false;
     const int jf = jfirst(m.pk, c, i);
-    const int k = (jf < m.j0 ? 1 : 0) + blockIdx.x * BX + threadIdx.x;
+    const int k = (jf < m.j0 ? 1 : 0) + bk.x * BX + threadIdx.x;
     j = jf + m.pk.ncol * k;
     if (j > m.j1)
         return false;
@@ -181,10 +185,15 @@ __device__ __forceinline__ bool packed_cell(const MomArgs& m, int c, int& i, int
 
 // |R| per cell in the residual range, R = (nb + pterm) - AP x (src/momentum_u.cpp:48-50), into a
 // compact buffer (one thread per cell; the per-phase entry point and sweepless iterations).
-__global__ void __launch_bounds__(BX* BY) k_mom_residual(MomArgs m, double* cell, const int* stop) {
-    if (stopped(stop))
-        return;
-    const int j = m.j0 + blockIdx.x * BX + threadIdx.x, i = m.i0 + blockIdx.y * BY + threadIdx.y;
+struct ResArgs {
+    MomArgs m;
+    double* cell;
+};
+
+__device__ __forceinline__ void k_mom_residual(const ResArgs& ra, const Blk& bk) {
+    const MomArgs& m = ra.m;
+    double* cell = ra.cell;
+    const int j = m.j0 + bk.x * BX + threadIdx.x, i = m.i0 + bk.y * BY + threadIdx.y;
     if (i > m.i1 || j > m.j1)
         return;
     const long ld = m.ld;
@@ -201,17 +210,25 @@ constexpr int ST = 1024; // sweep-tile workgroup size
 
 // One multicolour sweep (src/momentum_u.cpp:123-143) on a TI x TJ tile, in LDS: src -> dst.
 // first: also |R| of the tile's cells from the loaded field, into cell.
-template <int NCOL, int TI, int TJ>
-__global__ void __launch_bounds__(ST)
-    k_mom_sweep(MomArgs m, const double* src, double* dst, double* cell, int nr, int nc,
-                int first, const int* stop) {
-    if (stopped(stop))
-        return;
+struct SweepArgs {
+    MomArgs m;
+    const double* src;
+    double* dst;
+    double* cell;
+    int nr, nc, first;
+};
+
+template <int NCOL, int TI, int TJ> __device__ __forceinline__ void k_mom_sweep(const SweepArgs& sa, const Blk& bk) {
+    const MomArgs& m = sa.m;
+    const double* src = sa.src;
+    double* dst = sa.dst;
+    double* cell = sa.cell;
+    const int nr = sa.nr, nc = sa.nc, first = sa.first;
     constexpr int R = NCOL == 2 ? 1 : 2; // stencil reach
     constexpr int HL = R * NCOL;          // halo read by the first pass
     constexpr int LW = TJ + 2 * HL, LH = TI + 2 * HL;
     __shared__ double su[LH * LW];
-    const int ti0 = blockIdx.y * TI, tj0 = blockIdx.x * TJ;
+    const int ti0 = bk.y * TI, tj0 = bk.x * TJ;
     const int gi0 = ti0 - HL, gj0 = tj0 - HL; // array cell of su[0]
     const long ld = m.ld;
     // Batched loads: every load of a chunk is in flight before the first LDS store.
@@ -287,14 +304,20 @@ __global__ void __launch_bounds__(ST)
 #undef SU
 }
 
-// One Jacobi pass, dst from src (src/momentum_u.cpp:105-113); colour = blockIdx.z.
-__global__ void __launch_bounds__(BX* BY)
-    k_mom_jacobi(MomArgs m, const double* src, double* dst, const int* stop) {
-    if (stopped(stop))
-        return;
+// One Jacobi pass, dst from src (src/momentum_u.cpp:105-113); colour = block index z.
+struct JacArgs {
+    MomArgs m;
+    const double* src;
+    double* dst;
+};
+
+__device__ __forceinline__ void k_mom_jacobi(const JacArgs& ja, const Blk& bk) {
+    const MomArgs& m = ja.m;
+    const double* src = ja.src;
+    double* dst = ja.dst;
     int i, j;
     size_t o;
-    if (!packed_cell(m, blockIdx.z, i, j, o))
+    if (!packed_cell(m, bk, bk.z, i, j, o))
         return;
     const long ld = m.ld;
     const size_t q = static_cast<size_t>(i) * ld + j, as = m.pk.astride;
@@ -303,12 +326,21 @@ __global__ void __launch_bounds__(BX* BY)
              m.one_minus_alpha * src[q];
 }
 
-__global__ void k_copy(const double* a, double* b, size_t n, const int* stop) {
-    if (stopped(stop))
-        return;
-    for (size_t k = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; k < n;
-         k += static_cast<size_t>(gridDim.x) * blockDim.x)
-        b[k] = a[k];
+struct CopyArgs {
+    const double* a;
+    double* b;
+    size_t n;
+};
+constexpr int CP_T = 256;
+
+__device__ __forceinline__ void k_copy(const CopyArgs& c, const Blk& bk) {
+    for (size_t k = bk.x * static_cast<size_t>(CP_T) + threadIdx.x; k < c.n;
+         k += static_cast<size_t>(bk.nx) * CP_T)
+        c.b[k] = c.a[k];
+}
+
+void launch_copy(Solver& s, const double* a, double* b, size_t n, const int* stop) {
+    emit<CopyArgs, k_copy, CP_T>(s, dim3(1024), dim3(CP_T), 0, CopyArgs{a, b, n}, stop);
 }
 
 // Row-major <-> packed copies of coefficient array a (0..8) over the assembly range.
@@ -399,21 +431,19 @@ void launch_assemble(Solver& s, bool is_u, const int* stop) {
     a.d4 = s.diff4;
     const dim3 grid(cdiv(a.pk.aj1 - a.pk.aj0 + 1, BX), cdiv(a.pk.rows, BY));
     if (is_u)
-        hipLaunchKernelGGL(k_assemble<true>, grid, dim3(BX, BY), 0, s.stream, a, stop);
+        emit<AsmArgs, k_assemble<true>, BX * BY>(s, grid, dim3(BX, BY), 0, a, stop);
     else
-        hipLaunchKernelGGL(k_assemble<false>, grid, dim3(BX, BY), 0, s.stream, a, stop);
-    SG_HIP(hipGetLastError());
+        emit<AsmArgs, k_assemble<false>, BX * BY>(s, grid, dim3(BX, BY), 0, a, stop);
 }
 
 void launch_bc(Solver& s, bool is_u, const int* stop) {
     const sg_params& p = s.p;
     if (is_u)
-        hipLaunchKernelGGL(k_bc_u, dim3(cdiv(p.NY, 256)), dim3(256), 0, s.stream, s.F(SG_F_U),
-                           s.ld, p.H, p.NX, p.NY, stop);
+        emit<BcArgs, k_bc_u, BC_T>(s, dim3(cdiv(p.NY, BC_T)), dim3(BC_T), 0,
+                                   BcArgs{s.F(SG_F_U), s.ld, p.H, p.NX, p.NY}, stop);
     else
-        hipLaunchKernelGGL(k_bc_v, dim3(cdiv(p.NX, 256)), dim3(256), 0, s.stream, s.F(SG_F_V),
-                           s.ld, p.H, p.NX, p.NY, stop);
-    SG_HIP(hipGetLastError());
+        emit<BcArgs, k_bc_v, BC_T>(s, dim3(cdiv(p.NX, BC_T)), dim3(BC_T), 0,
+                                   BcArgs{s.F(SG_F_V), s.ld, p.H, p.NX, p.NY}, stop);
 }
 
 // residual_u / residual_v (src/momentum_u.cpp:43-59): |R| per cell, then the ordered sum into
@@ -422,11 +452,9 @@ void launch_mom_residual(Solver& s, bool is_u, double* slot, const int* stop, bo
     const MomArgs m = mom_args(s, is_u);
     const int nrow = m.i1 - m.i0 + 1, w = m.j1 - m.j0 + 1;
     double* cell = s.rcell[is_u ? 0 : 1];
-    if (nrow > 0 && w > 0) {
-        hipLaunchKernelGGL(k_mom_residual, dim3(cdiv(w, BX), cdiv(nrow, BY)), dim3(BX, BY), 0,
-                           s.stream, m, cell, stop);
-        SG_HIP(hipGetLastError());
-    }
+    if (nrow > 0 && w > 0)
+        emit<ResArgs, k_mom_residual, BX * BY>(s, dim3(cdiv(w, BX), cdiv(nrow, BY)), dim3(BX, BY),
+                                               0, ResArgs{m, cell}, stop);
     if (side)
         side_sum(s, is_u ? 1 : 2, cell, nrow, w, slot, stop);
     else if (nrow <= 0 || w <= 0)
@@ -440,24 +468,24 @@ template <int NCOL>
 void sweep_tile(Solver& s, const MomArgs& m, const double* src, double* dst, double* cell,
                 int first, const int* stop) {
     const dim3 b(ST);
+    const SweepArgs a{m, src, dst, cell, s.nr, s.nc, first};
     switch (s.mom_tile) {
     case 0:
-        hipLaunchKernelGGL((k_mom_sweep<NCOL, 32, 64>), dim3(cdiv(s.nc, 64), cdiv(s.nr, 32)), b, 0,
-                           s.stream, m, src, dst, cell, s.nr, s.nc, first, stop);
+        emit<SweepArgs, k_mom_sweep<NCOL, 32, 64>, ST>(s, dim3(cdiv(s.nc, 64), cdiv(s.nr, 32)), b,
+                                                       0, a, stop);
         break;
     case 1:
-        hipLaunchKernelGGL((k_mom_sweep<NCOL, 64, 64>), dim3(cdiv(s.nc, 64), cdiv(s.nr, 64)), b, 0,
-                           s.stream, m, src, dst, cell, s.nr, s.nc, first, stop);
+        emit<SweepArgs, k_mom_sweep<NCOL, 64, 64>, ST>(s, dim3(cdiv(s.nc, 64), cdiv(s.nr, 64)), b,
+                                                       0, a, stop);
         break;
     case 2:
-        hipLaunchKernelGGL((k_mom_sweep<NCOL, 32, 128>), dim3(cdiv(s.nc, 128), cdiv(s.nr, 32)), b,
-                           0, s.stream, m, src, dst, cell, s.nr, s.nc, first, stop);
+        emit<SweepArgs, k_mom_sweep<NCOL, 32, 128>, ST>(s, dim3(cdiv(s.nc, 128), cdiv(s.nr, 32)),
+                                                        b, 0, a, stop);
         break;
     default:
-        hipLaunchKernelGGL((k_mom_sweep<NCOL, 16, 128>), dim3(cdiv(s.nc, 128), cdiv(s.nr, 16)), b,
-                           0, s.stream, m, src, dst, cell, s.nr, s.nc, first, stop);
+        emit<SweepArgs, k_mom_sweep<NCOL, 16, 128>, ST>(s, dim3(cdiv(s.nc, 128), cdiv(s.nr, 16)),
+                                                        b, 0, a, stop);
     }
-    SG_HIP(hipGetLastError());
 }
 } // namespace
 
@@ -492,11 +520,8 @@ void launch_mom_sweeps(Solver& s, bool is_u, int nsweeps, const int* stop, doubl
             launch_mom_sweep_once(s, is_u, even ? m.x : s.scratch, even ? s.scratch : m.x,
                                   fuse && k == 0, stop);
         }
-        if (nsweeps % 2) {
-            hipLaunchKernelGGL(k_copy, dim3(1024), dim3(256), 0, s.stream, s.scratch, m.x, s.n,
-                               stop);
-            SG_HIP(hipGetLastError());
-        }
+        if (nsweeps % 2)
+            launch_copy(s, s.scratch, m.x, s.n, stop);
         if (fuse) {
             const int nrow = m.i1 - m.i0 + 1, w = m.j1 - m.j0 + 1;
             if (side)
@@ -508,17 +533,15 @@ void launch_mom_sweeps(Solver& s, bool is_u, int nsweeps, const int* stop, doubl
         return;
     }
     // Jacobi batch: full copy, then alternate so that the last pass writes the field.
-    hipLaunchKernelGGL(k_copy, dim3(1024), dim3(256), 0, s.stream, m.x, s.scratch, s.n, stop);
-    SG_HIP(hipGetLastError());
+    launch_copy(s, m.x, s.scratch, s.n, stop);
     if (m.i1 < m.i0 || m.j1 < m.j0)
         return;
     for (int k = 0; k < nsweeps; ++k) {
         const bool to_x = (nsweeps - k) % 2 == 1;
         const double* src = to_x ? s.scratch : m.x;
         double* dst = to_x ? m.x : s.scratch;
-        hipLaunchKernelGGL(k_mom_jacobi, pass_grid(m, m.pk.ncol), dim3(BX, BY), 0, s.stream, m,
-                           src, dst, stop);
-        SG_HIP(hipGetLastError());
+        emit<JacArgs, k_mom_jacobi, BX * BY>(s, pass_grid(m, m.pk.ncol), dim3(BX, BY), 0,
+                                             JacArgs{m, src, dst}, stop);
     }
 }
 

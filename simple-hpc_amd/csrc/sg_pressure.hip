@@ -24,14 +24,21 @@ PArgs p_args(Solver& s) {
 
 // assemble_p (src/pressure.cpp:14-54) for interior cell (i, j): links dy^2/APU, dx^2/APV, the
 // wall links zeroed, APP = AE + AW + AN + AS, mass source b, |b| for res_p.
-__global__ void __launch_bounds__(BX* BY)
-    k_assemble_p(PArgs a, const double* APU, const double* APV, const double* u, const double* v,
-                 double dx, double dy, double* cell, const int* stop) {
-    if (stopped(stop))
-        return;
+struct ApArgs {
+    PArgs a;
+    const double *APU, *APV, *u, *v;
+    double dx, dy;
+    double* cell;
+};
+
+__device__ __forceinline__ void k_assemble_p(const ApArgs& g, const Blk& bk) {
+    const PArgs& a = g.a;
+    const double *APU = g.APU, *APV = g.APV, *u = g.u, *v = g.v;
+    const double dx = g.dx, dy = g.dy;
+    double* cell = g.cell;
     const int H = a.H;
-    const int j = H + 1 + blockIdx.x * BX + threadIdx.x;
-    const int i = H + blockIdx.y * BY + threadIdx.y;
+    const int j = H + 1 + bk.x * BX + threadIdx.x;
+    const int i = H + bk.y * BY + threadIdx.y;
     if (i > a.NX + H - 1 || j > a.NY + H)
         return;
     const long ld = a.ld;
@@ -55,10 +62,16 @@ __global__ void __launch_bounds__(BX* BY)
 }
 
 // The wall-link zeroing of src/pressure.cpp:30-41 also covers halo rows and columns.
-__global__ void k_p_walls(PArgs a, int nr, int nc, const int* stop) {
-    if (stopped(stop))
-        return;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+struct WallArgs {
+    PArgs a;
+    int nr, nc;
+};
+constexpr int WT = 256;
+
+__device__ __forceinline__ void k_p_walls(const WallArgs& g, const Blk& bk) {
+    const PArgs& a = g.a;
+    const int nr = g.nr, nc = g.nc;
+    const int k = bk.x * WT + threadIdx.x;
     const long ld = a.ld;
     if (k < nr) {
         a.AW[k * ld + a.H + 1] = 0.0;
@@ -81,27 +94,38 @@ __device__ __forceinline__ void pc_update(const PArgs& a, double* dst, const dou
 }
 
 // Red-black pass, colour (i + j) & 1 (src/pressure.cpp:114-134).
-__global__ void __launch_bounds__(BX* BY) k_pc_rb(PArgs a, int col, const int* stop) {
-    if (stopped(stop))
-        return;
+struct RbArgs {
+    PArgs a;
+    int col;
+};
+
+__device__ __forceinline__ void k_pc_rb(const RbArgs& g, const Blk& bk) {
+    const PArgs& a = g.a;
+    const int col = g.col;
     const int H = a.H;
-    const int i = H + blockIdx.y * BY + threadIdx.y;
+    const int i = H + bk.y * BY + threadIdx.y;
     if (i > a.NX + H - 1)
         return;
-    const int j = H + 1 + ((i + H + 1 + col) & 1) + 2 * (blockIdx.x * BX + threadIdx.x);
+    const int j = H + 1 + ((i + H + 1 + col) & 1) + 2 * (bk.x * BX + threadIdx.x);
     if (j > a.NY + H)
         return;
     pc_update(a, a.pc, a.pc, static_cast<size_t>(i) * a.ld + j);
 }
 
 // Damped Jacobi pass, weight 0.8 (src/pressure.cpp:98-104).
-__global__ void __launch_bounds__(BX* BY)
-    k_pc_jacobi(PArgs a, const double* src, double* dst, const int* stop) {
-    if (stopped(stop))
-        return;
+struct PJacArgs {
+    PArgs a;
+    const double* src;
+    double* dst;
+};
+
+__device__ __forceinline__ void k_pc_jacobi(const PJacArgs& g, const Blk& bk) {
+    const PArgs& a = g.a;
+    const double* src = g.src;
+    double* dst = g.dst;
     const int H = a.H;
-    const int j = H + 1 + blockIdx.x * BX + threadIdx.x;
-    const int i = H + blockIdx.y * BY + threadIdx.y;
+    const int j = H + 1 + bk.x * BX + threadIdx.x;
+    const int i = H + bk.y * BY + threadIdx.y;
     if (i > a.NX + H - 1 || j > a.NY + H)
         return;
     const size_t q = static_cast<size_t>(i) * a.ld + j;
@@ -111,14 +135,21 @@ __global__ void __launch_bounds__(BX* BY)
 }
 
 // correct (src/pressure.cpp:136-157).
-__global__ void __launch_bounds__(BX* BY)
-    k_correct(PArgs a, const double* APU, const double* APV, double* u, double* v, double* pf,
-              double dx, double dy, double ap, const int* stop) {
-    if (stopped(stop))
-        return;
+struct CorrArgs {
+    PArgs a;
+    const double *APU, *APV;
+    double *u, *v, *pf;
+    double dx, dy, ap;
+};
+
+__device__ __forceinline__ void k_correct(const CorrArgs& g, const Blk& bk) {
+    const PArgs& a = g.a;
+    const double *APU = g.APU, *APV = g.APV;
+    double *u = g.u, *v = g.v, *pf = g.pf;
+    const double dx = g.dx, dy = g.dy, ap = g.ap;
     const int H = a.H;
-    const int j = H + 1 + blockIdx.x * BX + threadIdx.x;
-    const int i = H + blockIdx.y * BY + threadIdx.y;
+    const int j = H + 1 + bk.x * BX + threadIdx.x;
+    const int i = H + bk.y * BY + threadIdx.y;
     if (i > a.NX + H - 1 || j > a.NY + H)
         return;
     const long ld = a.ld;
@@ -131,12 +162,17 @@ __global__ void __launch_bounds__(BX* BY)
     pf[q] += ap * pc[q];
 }
 
-__global__ void k_fill(double* a, size_t n, double v, const int* stop) {
-    if (stopped(stop))
-        return;
-    for (size_t k = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; k < n;
-         k += static_cast<size_t>(gridDim.x) * blockDim.x)
-        a[k] = v;
+struct FillArgs {
+    double* a;
+    size_t n;
+    double v;
+};
+constexpr int FT = 256;
+
+__device__ __forceinline__ void k_fill(const FillArgs& g, const Blk& bk) {
+    for (size_t k = bk.x * static_cast<size_t>(FT) + threadIdx.x; k < g.n;
+         k += static_cast<size_t>(bk.nx) * FT)
+        g.a[k] = g.v;
 }
 
 // ---- CG (src/pressure.cpp:208-317); per-cell work on the device, the scalar recurrence on the
@@ -222,17 +258,22 @@ __global__ void __launch_bounds__(BX* BY) k_cg_update(CgArgs c, double alpha) {
 }
 
 // x -= sx / n.
-__global__ void __launch_bounds__(BX* BY) k_sub_mean(PArgs a, double* x, const double* sum, double n,
-                                                     const int* stop) {
-    if (stopped(stop))
-        return;
+struct SubArgs {
+    PArgs a;
+    double* x;
+    const double* sum;
+    double n;
+};
+
+__device__ __forceinline__ void k_sub_mean(const SubArgs& g, const Blk& bk) {
+    const PArgs& a = g.a;
     const int H = a.H;
-    const int j = H + 1 + blockIdx.x * BX + threadIdx.x;
-    const int i = H + blockIdx.y * BY + threadIdx.y;
+    const int j = H + 1 + bk.x * BX + threadIdx.x;
+    const int i = H + bk.y * BY + threadIdx.y;
     if (i > a.NX + H - 1 || j > a.NY + H)
         return;
-    const double m = *sum / n;
-    x[static_cast<size_t>(i) * a.ld + j] -= m;
+    const double m = *g.sum / g.n;
+    g.x[static_cast<size_t>(i) * a.ld + j] -= m;
 }
 
 dim3 pgrid(const sg_params& p) { return dim3(cdiv(p.NY, BX), cdiv(p.NX, BY)); }
@@ -301,28 +342,27 @@ void solve_cg(Solver& s) {
         SG_HIP(hipGetLastError());
     }
     launch_ordered_sum(s, a.pc, s.ld, p.H, p.NX + p.H - 1, p.H + 1, p.NY + p.H, slot, never);
-    hipLaunchKernelGGL(k_sub_mean, pgrid(p), dim3(BX, BY), 0, s.stream, a, a.pc, slot, n, never);
-    SG_HIP(hipGetLastError());
+    emit<SubArgs, k_sub_mean, BX * BY>(s, pgrid(p), dim3(BX, BY), 0, SubArgs{a, a.pc, slot, n},
+                                       never);
     s.cg_last = k;
 }
 
 } // namespace
 
 void launch_fill(Solver& s, double* a, size_t n, double v, const int* stop) {
-    hipLaunchKernelGGL(k_fill, dim3(std::min<size_t>(2048, (n + 255) / 256 + 1)), dim3(256), 0,
-                       s.stream, a, n, v, stop);
-    SG_HIP(hipGetLastError());
+    emit<FillArgs, k_fill, FT>(s, dim3(std::min<size_t>(2048, (n + FT - 1) / FT + 1)), dim3(FT), 0,
+                               FillArgs{a, n, v}, stop);
 }
 
 void launch_assemble_p(Solver& s, double* slot, const int* stop, bool side) {
     const sg_params& p = s.p;
     const PArgs a = p_args(s);
-    hipLaunchKernelGGL(k_p_walls, dim3(cdiv(std::max(s.nr, s.nc), 256)), dim3(256), 0, s.stream,
-                       a, s.nr, s.nc, stop);
-    SG_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_assemble_p, pgrid(p), dim3(BX, BY), 0, s.stream, a, s.F(SG_F_CU_AP),
-                       s.F(SG_F_CV_AP), s.F(SG_F_U), s.F(SG_F_V), p.dx, p.dy, s.rcell[2], stop);
-    SG_HIP(hipGetLastError());
+    emit<WallArgs, k_p_walls, WT>(s, dim3(cdiv(std::max(s.nr, s.nc), WT)), dim3(WT), 0,
+                                  WallArgs{a, s.nr, s.nc}, stop);
+    emit<ApArgs, k_assemble_p, BX * BY>(s, pgrid(p), dim3(BX, BY), 0,
+                                        ApArgs{a, s.F(SG_F_CU_AP), s.F(SG_F_CV_AP), s.F(SG_F_U),
+                                               s.F(SG_F_V), p.dx, p.dy, s.rcell[2]},
+                                        stop);
     if (side)
         side_sum(s, 3, s.rcell[2], p.NX, p.NY, slot, stop);
     else
@@ -331,19 +371,19 @@ void launch_assemble_p(Solver& s, double* slot, const int* stop, bool side) {
 
 void launch_correct(Solver& s, const int* stop) {
     const sg_params& p = s.p;
-    hipLaunchKernelGGL(k_correct, pgrid(p), dim3(BX, BY), 0, s.stream, p_args(s),
-                       s.F(SG_F_CU_AP), s.F(SG_F_CV_AP), s.F(SG_F_U), s.F(SG_F_V), s.F(SG_F_P),
-                       p.dx, p.dy, p.relaxation_factor_p, stop);
-    SG_HIP(hipGetLastError());
+    const CorrArgs c{p_args(s),   s.F(SG_F_CU_AP), s.F(SG_F_CV_AP), s.F(SG_F_U),
+                     s.F(SG_F_V), s.F(SG_F_P),     p.dx,            p.dy,
+                     p.relaxation_factor_p};
+    emit<CorrArgs, k_correct, BX * BY>(s, pgrid(p), dim3(BX, BY), 0, c, stop);
 }
 
 void launch_remove_mean0(Solver& s, double* x, const int* stop) {
     const sg_params& p = s.p;
     double* slot = s.scal + S_SUM_D;
     launch_ordered_sum(s, x, s.ld, p.H, p.NX + p.H - 1, p.H + 1, p.NY + p.H, slot, stop);
-    hipLaunchKernelGGL(k_sub_mean, pgrid(p), dim3(BX, BY), 0, s.stream, p_args(s), x, slot,
-                       static_cast<double>(p.NX) * p.NY, stop);
-    SG_HIP(hipGetLastError());
+    emit<SubArgs, k_sub_mean, BX * BY>(s, pgrid(p), dim3(BX, BY), 0,
+                                       SubArgs{p_args(s), x, slot, static_cast<double>(p.NX) * p.NY},
+                                       stop);
 }
 
 // solve_p for every GPU pressure solver. CG needs the host between iterations (eager only).
@@ -355,9 +395,8 @@ void launch_solve_p(Solver& s, const int* stop, bool eager) {
         launch_fill(s, a.pc, s.n, 0.0, stop);
         for (int k = 0; k < p.maxit_inner_p; ++k)
             for (int c = 0; c < 2; ++c) {
-                hipLaunchKernelGGL(k_pc_rb, dim3(cdiv(cdiv(p.NY, 2), BX), cdiv(p.NX, BY)),
-                                   dim3(BX, BY), 0, s.stream, a, c, stop);
-                SG_HIP(hipGetLastError());
+                emit<RbArgs, k_pc_rb, BX * BY>(s, dim3(cdiv(cdiv(p.NY, 2), BX), cdiv(p.NX, BY)),
+                                               dim3(BX, BY), 0, RbArgs{a, c}, stop);
             }
         return;
     case SG_P_JACOBI: {
@@ -368,9 +407,8 @@ void launch_solve_p(Solver& s, const int* stop, bool eager) {
             const bool to_pc = (n - k) % 2 == 1;
             const double* src = to_pc ? s.scratch : a.pc;
             double* dst = to_pc ? a.pc : s.scratch;
-            hipLaunchKernelGGL(k_pc_jacobi, pgrid(p), dim3(BX, BY), 0, s.stream, a, src, dst,
-                               stop);
-            SG_HIP(hipGetLastError());
+            emit<PJacArgs, k_pc_jacobi, BX * BY>(s, pgrid(p), dim3(BX, BY), 0,
+                                                 PJacArgs{a, src, dst}, stop);
         }
         return;
     }

@@ -51,14 +51,25 @@ __device__ __forceinline__ double add_in_order(double s, const double* v, int n)
     return s;
 }
 
-__global__ void __launch_bounds__(NT)
-    k_ordered_sum(const double* a, long pitch, int i0, int nrow, int j0, int w, double* rows,
-                  unsigned* ticket, double* slot, const int* stop) {
-    if (stopped(stop))
-        return;
+struct SumArgs {
+    const double* a;
+    long pitch;
+    int i0, nrow, j0, w;
+    double* rows;
+    unsigned* ticket;
+    double* slot;
+};
+
+__device__ __forceinline__ void k_ordered_sum(const SumArgs& sa, const Blk& bk) {
+    const double* a = sa.a;
+    const long pitch = sa.pitch;
+    const int i0 = sa.i0, nrow = sa.nrow, j0 = sa.j0, w = sa.w;
+    double* rows = sa.rows;
+    unsigned* ticket = sa.ticket;
+    double* slot = sa.slot;
     __shared__ double tile[RW][CW + PAD];
     __shared__ bool last;
-    const int r0 = blockIdx.x * RW;
+    const int r0 = bk.x * RW;
     const int t = threadIdx.x;
     double s = 0.0;
     for (int c0 = 0; c0 < w; c0 += CW) {
@@ -86,7 +97,7 @@ __global__ void __launch_bounds__(NT)
     __threadfence();
     __syncthreads();
     if (t == 0)
-        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+        last = atomicAdd(ticket, 1u) == static_cast<unsigned>(bk.nx - 1);
     __syncthreads();
     if (!last)
         return;
@@ -110,9 +121,13 @@ __global__ void __launch_bounds__(NT)
     }
 }
 
-__global__ void k_zero_slot(double* slot, const int* stop) {
-    if (!stopped(stop) && threadIdx.x == 0)
-        *slot = 0.0;
+struct SlotArgs {
+    double* slot;
+};
+
+__device__ __forceinline__ void k_zero_slot(const SlotArgs& a, const Blk&) {
+    if (threadIdx.x == 0)
+        *a.slot = 0.0;
 }
 
 } // namespace
@@ -122,18 +137,20 @@ void launch_ordered_sum(Solver& s, const double* a, int pitch, int i0, int i1, i
     const int nrow = i1 - i0 + 1, w = j1 - j0 + 1;
     hipStream_t st = stream ? stream : s.stream;
     if (nrow <= 0 || w <= 0) {
-        hipLaunchKernelGGL(k_zero_slot, dim3(1), dim3(64), 0, st, slot, stop);
+        emit<SlotArgs, k_zero_slot, 64>(s, dim3(1), dim3(64), 0, SlotArgs{slot}, stop, st);
     } else {
-        hipLaunchKernelGGL(k_ordered_sum, dim3(cdiv(nrow, RW)), dim3(NT), 0, st, a,
-                           static_cast<long>(pitch), i0, nrow, j0, w,
-                           s.rowbuf + static_cast<size_t>(lane) * s.nr, s.ticket + lane, slot,
-                           stop);
+        const SumArgs sa{a, static_cast<long>(pitch), i0, nrow, j0, w,
+                         s.rowbuf + static_cast<size_t>(lane) * s.nr, s.ticket + lane, slot};
+        emit<SumArgs, k_ordered_sum, NT>(s, dim3(cdiv(nrow, RW)), dim3(NT), 0, sa, stop, st);
     }
-    SG_HIP(hipGetLastError());
 }
 
 void side_sum(Solver& s, int lane, const double* cell, int nrow, int w, double* slot,
               const int* stop) {
+    if (s.rec) { // recorded as a side-stream launch; the batch executor forks and joins
+        launch_ordered_sum(s, cell, w > 0 ? w : 1, 0, nrow - 1, 0, w - 1, slot, stop, lane, s.side);
+        return;
+    }
     SG_HIP(hipEventRecord(s.fork[lane - 1], s.stream));
     SG_HIP(hipStreamWaitEvent(s.side, s.fork[lane - 1], 0));
     launch_ordered_sum(s, cell, w > 0 ? w : 1, 0, nrow - 1, 0, w - 1, slot, stop, lane, s.side);
@@ -141,6 +158,14 @@ void side_sum(Solver& s, int lane, const double* cell, int nrow, int w, double* 
 }
 
 void join_side(Solver& s) {
+    if (s.rec) {
+        Rec r; // join marker
+        r.tag = s.tag;
+        for (const Rec& q : *s.rec)
+            r.occ += (q.vt == nullptr && q.tag == r.tag) ? 1 : 0;
+        s.rec->push_back(std::move(r));
+        return;
+    }
     for (int k = 0; k < 3; ++k)
         SG_HIP(hipStreamWaitEvent(s.stream, s.join[k], 0));
 }

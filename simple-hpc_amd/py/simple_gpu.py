@@ -137,6 +137,16 @@ def gpu():
         g.sg_set_tuning.argtypes = [vp, C.c_int, C.c_long]
         g.sg_get_tuning.argtypes = [vp, C.c_int, C.POINTER(C.c_long)]
         g.sg_autotune.argtypes = [vp, C.c_int, C.c_char_p, C.c_size_t]
+        ip = C.POINTER(C.c_int)
+        g.sg_batch_create.argtypes = [P, C.c_int, C.POINTER(vp)]
+        g.sg_batch_destroy.argtypes = [vp]
+        g.sg_batch_size.argtypes = [vp, ip]
+        g.sg_batch_solver.argtypes = [vp, C.c_int, C.POINTER(vp)]
+        g.sg_batch_advance.argtypes = [vp, C.c_int, ip, ip, dp]
+        g.sg_batch_launches.argtypes = [vp, ip]
+        g.sg_batch_set_tuning.argtypes = [vp, C.c_int, C.c_long]
+        g.sg_batch_get_tuning.argtypes = [vp, C.c_int, C.POINTER(C.c_long)]
+        g.sg_batch_autotune.argtypes = [vp, C.c_int, C.c_char_p, C.c_size_t]
         _gpu = g
     return _gpu
 
@@ -261,6 +271,102 @@ class Solver:
         n = C.c_int()
         check(gpu().sg_mg_levels(self.h, C.byref(n)))
         return n.value
+
+
+class _Borrowed(Solver):
+    """A cavity of a Batch: the Solver interface without ownership."""
+
+    def __init__(self, h, p):
+        self.p = p.copy()
+        self.h = h
+
+    def close(self):
+        self.h = None
+
+
+class Batch:
+    """Owner of an sg_batch handle: cavities advanced together, one launch per phase."""
+
+    def __init__(self, plist):
+        g = gpu()
+        self.n = len(plist)
+        arr = (SgParams * self.n)(*[q.copy() for q in plist])
+        self.plist = [q.copy() for q in plist]
+        h = C.c_void_p()
+        check(g.sg_batch_create(arr, self.n, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            gpu().sg_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def cavity(self, k):
+        h = C.c_void_p()
+        check(gpu().sg_batch_solver(self.h, k, C.byref(h)))
+        return _Borrowed(h, self.plist[k])
+
+    def advance(self, n):
+        """Up to n outer iterations; returns (iterations, status, last residuals) per cavity."""
+        import numpy as np
+        its = (C.c_int * self.n)()
+        st = (C.c_int * self.n)()
+        last = np.zeros((self.n, 3))
+        check(gpu().sg_batch_advance(self.h, n, its, st, _dptr(last)))
+        return list(its), list(st), last
+
+    def launches(self):
+        k = C.c_int()
+        check(gpu().sg_batch_launches(self.h, C.byref(k)))
+        return k.value
+
+    def set_tuning(self, key, value):
+        check(gpu().sg_batch_set_tuning(self.h, key, value))
+
+    def get_tuning(self, key):
+        v = C.c_long()
+        check(gpu().sg_batch_get_tuning(self.h, key, C.byref(v)))
+        return v.value
+
+    def autotune(self, reps=5):
+        buf = C.create_string_buffer(16384)
+        check(gpu().sg_batch_autotune(self.h, reps, buf, 16384))
+        return buf.value.decode()
+
+
+WORKLOADS = os.path.join(ROOT, "simple-hpc_amd", "workloads")
+
+
+def cavities(path=None):
+    """The uneven-cavity workload as (N, Re) pairs: 64 square cavities, N uniform over
+    {64, 96, 128, 192, 256}, then Re log-uniform in [100, 3200], drawn in that order from
+    std::mt19937_64(0) by workloads/cavities.cpp (its output is committed)."""
+    path = path or os.path.join(WORKLOADS, "cavities_seed0.txt")
+    out = []
+    with open(path) as f:
+        for line in f:
+            if line.strip():
+                n, re = line.split()
+                out.append((int(n), float(re)))
+    return out
+
+
+def batch_config(n, re, precision="mixed", tol=1e-5, maxit=10000000, check_every=200):
+    """Config text of one cavity of the batched workload: QUICK, 2nd-order diffusion,
+    multicolour momentum, multigrid p' (mixed or fp64), tolerance tol."""
+    text = (f"N_cells_x = {n}\nN_cells_y = {n}\nRe = {re!r}\nconvection_scheme = QUICK\n"
+            f"diffusion_order = 2\nmomentum_solver = gs_multicolour\npressure_solver = mg\n"
+            f"error_tolerance = {tol!r}\nmaxit = {maxit}\nbackend = gpu\nwrite_fields = False\n"
+            f"check_every = {check_every}\n")
+    if precision == "mixed":
+        text += "mg_precision = mixed\n"
+    return text
 
 
 def device_count():
