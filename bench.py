@@ -14,7 +14,17 @@ u-momentum multicolour sweep) from hipEvent timings on the solver's stream, and 
 the oracle (a restatement of the reference's OpenMP CPU path) timed on this host's cores on a
 bounded sample of the same workload.
 
+Other workloads (--workload):
+  batch  the 64 uneven cavities of simple-hpc_amd/workloads/cavities_seed0.txt (N in
+         {64..256}, Re log-uniform in [100, 3200], QUICK, 2nd-order diffusion), advanced together
+         (one persistent launch per phase, one hipGraph per batched iteration) to 1e-5 with
+         mixed-precision multigrid, and again with FP64 multigrid; every cavity's u, v, p compared
+         with its own FP64 solve;
+  large  one 8192 x 8192 cavity, Re 3200, QUICK, 4th-order diffusion: fixed-work outer
+         iterations/s, plain FP64 and mixed.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode fixed|tol] [--forms plain,mixed]
+                       [--workload cavity|batch|large]
 """
 import argparse
 import json
@@ -110,6 +120,132 @@ def cpu_baseline(sg, n, seconds=12.0):
                       f"(oracle restating the reference's OpenMP path, {cores} threads), {t:.1f} s"}
 
 
+def run_batch(sg, steps_cap, knobs=None, cpu=True):
+    """The batched uneven-cavity workload to tolerance, mixed and FP64 forms."""
+    import numpy as np
+    cases = sg.cavities()
+    out = {}
+    batches = {}
+    tune = None
+    for form in ("mixed", "fp64"):
+        plist = [sg.params_from_config(sg.batch_config(n, re, precision=form))[0]
+                 for n, re in cases]
+        b = sg.Batch(plist)
+        if tune is None:
+            if knobs:
+                t, th, mid, lds = (int(x) for x in knobs.split(","))
+                tune = (t, th, mid, lds)
+            else:
+                b.advance(3)
+                rep = b.autotune(reps=5)
+                log("batch autotune -> " + rep.strip().splitlines()[-1])
+                f = [l.split() for l in rep.strip().splitlines()]
+                t = int([l for l in f if l[:2] == ["best", "momentum"]][0][-1].strip("()"))
+                last = f[-1]
+                tune = (t, int(last[2]), int(last[4]), int(last[6]))
+                b.close()
+                b = sg.Batch(plist)
+        for key, v in zip((sg.TUNE_MOM_TILE, sg.TUNE_TAIL_THREADS, sg.TUNE_MID_CELLS,
+                           sg.TUNE_LDS_CELLS), tune):
+            b.set_tuning(key, v)
+        log(f"batch {form}: {len(cases)} cavities to tolerance "
+            f"({b.launches()} launches per batched iteration)")
+        b.advance(1)  # graph capture outside the timed region
+        t0 = time.perf_counter()
+        total_its, st = None, None
+        next_report = t0 + 20
+        done = 1
+        while True:
+            its, st, last = b.advance(2000)
+            if all(x != 0 for x in st) or max(its) >= steps_cap:
+                total_its = its
+                break
+            if time.perf_counter() > next_report:
+                log(f"  {form}: max iterations {max(its)}, running {st.count(0)}")
+                next_report += 20
+        t = time.perf_counter() - t0
+        out[form] = {"seconds": t, "iterations": total_its, "status": st,
+                     "cavity_iterations": int(sum(total_its) - len(cases)),
+                     "cavity_iterations_per_s": (sum(total_its) - len(cases)) / t,
+                     "converged": sum(1 for x in st if x == 1)}
+        batches[form] = b
+    errs = []
+    for k in range(len(cases)):
+        cm, c6 = batches["mixed"].cavity(k), batches["fp64"].cavity(k)
+        errs.append([float(np.max(np.abs(cm.get(f) - c6.get(f)))) for f in ("u", "v", "p")])
+    for b in batches.values():
+        b.close()
+    e = np.array(errs)
+    res = {"cases": [[n, re] for n, re in cases], "forms": out,
+           "max_abs_err_mixed_vs_fp64": {"u": float(e[:, 0].max()), "v": float(e[:, 1].max()),
+                                         "p": float(e[:, 2].max())},
+           "per_cavity_err": errs, "tuning": list(tune)}
+    if cpu:
+        res["cpu_baseline"] = cpu_batch_baseline(sg, cases, out["fp64"]["iterations"])
+    return res
+
+
+def cpu_batch_baseline(sg, cases, gpu_iters, seconds=15.0):
+    """Oracle s/iteration per distinct cavity size (all host cores), projected onto the FP64
+    iteration counts (bit-identical paths, so the CPU would need the same counts)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    cores = os.cpu_count() or 1
+    pyoracle.set_threads(cores)
+    per = {}
+    budget = seconds / len(set(n for n, _ in cases))
+    for n in sorted(set(n for n, _ in cases)):
+        re = [r for m, r in cases if m == n][0]
+        p, _ = sg.params_from_config(sg.batch_config(n, re, precision="fp64"))
+        o = pyoracle.Oracle(p)
+        k, t = 0, 0.0
+        while t < budget and k < 200:
+            dt, _ = o.run_fixed(1)
+            t += dt
+            k += 1
+        per[n] = t / k
+    total = sum(per[n] * (its - 1) for (n, _), its in zip(cases, gpu_iters))
+    work = sum(its - 1 for its in gpu_iters)
+    return {"value": work / total, "unit": "cavity outer iterations/s", "cores": cores,
+            "kind": "port", "projected_time_to_tolerance_s": total,
+            "s_per_iteration": {str(k): v for k, v in per.items()},
+            "sample": f"oracle outer iterations per cavity size ({budget:.1f} s each, "
+                      f"{cores} threads), projected onto the FP64 iteration counts"}
+
+
+def large_config(form):
+    text = (f"N_cells_x = 8192\nN_cells_y = 8192\nRe = 3200\nconvection_scheme = QUICK\n"
+            f"diffusion_order = 4\nmomentum_solver = gs_multicolour\npressure_solver = mg\n"
+            f"mg_pre = 2\nmg_post = 2\nerror_tolerance = 1e-5\nmaxit = 10000000\n"
+            f"backend = gpu\nwrite_fields = False\n")
+    if form == "mixed":
+        text += "mg_precision = mixed\n"
+    return text
+
+
+def run_large(sg, steps, warmup, knobs=None):
+    res = []
+    for form in ("plain", "mixed"):
+        p, _ = sg.params_from_config(large_config(form))
+        s = sg.Solver(p)
+        if knobs:
+            t, th, mid, lds = (int(x) for x in knobs.split(","))
+            for key, v in zip((sg.TUNE_MOM_TILE, sg.TUNE_TAIL_THREADS, sg.TUNE_MID_CELLS,
+                               sg.TUNE_LDS_CELLS), (t, th, mid, lds)):
+                s.set_tuning(key, v)
+        s.advance(warmup)
+        log(f"8192^2 {form}: timing {steps} outer iterations")
+        t0 = time.perf_counter()
+        done, lg, conv = s.advance(steps)
+        t = time.perf_counter() - t0
+        sec, bytes_, per = s.probe(sg.K_SWEEP_U, reps=20)
+        res.append({"form": form, "iterations": done, "seconds": t, "ms_per_iter": 1e3 * t / done,
+                    "last_residuals": [float(x) for x in lg[-1]],
+                    "sweep_u_us": sec * 1e6, "sweep_u_GBs": bytes_ / sec / 1e9})
+        s.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -119,6 +255,7 @@ def main():
     ap.add_argument("--forms", default="plain")
     ap.add_argument("--n", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="cavity", choices=["cavity", "batch", "large"])
     ap.add_argument("--knobs", default=None,
                     help="launch shapes 'tile,tail_threads,mid_cells,lds_cells' instead of "
                          "autotuning (profiling runs)")
@@ -128,6 +265,34 @@ def main():
               "(DESIGN.md describes how it would shard).", file=sys.stderr)
         sys.exit(2)
     import simple_gpu as sg
+    if a.workload == "batch":
+        r = run_batch(sg, 10000000, a.knobs, cpu=not a.no_cpu)
+        m = r["forms"]["mixed"]
+        out = {"metric": "time to SIMPLE residual 1e-5, 64 uneven cavities (mixed, batched)",
+               "value": m["seconds"], "unit": "s", "n_gpus": 1, "steps": max(m["iterations"]),
+               "warmup": 1, "ms_per_step": 1e3 * m["seconds"] / max(m["iterations"]),
+               "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+               "dtype": "f64+f32", "data": "synthetic (lid-driven cavities from zero fields)",
+               "config": {"workload": "64 square cavities, N uniform in {64,96,128,192,256}, Re "
+                                      "log-uniform [100,3200], mt19937_64 seed 0, QUICK, "
+                                      "2nd-order diffusion, gs_multicolour, mg V(2,2), tol 1e-5",
+                          **r}}
+        if "cpu_baseline" in r:
+            out["cpu_baseline"] = r.pop("cpu_baseline")
+        print(json.dumps(out))
+        return
+    if a.workload == "large":
+        r = run_large(sg, a.steps, a.warmup, a.knobs)
+        out = {"metric": "outer iterations/s, 8192^2 Re3200 (plain)",
+               "value": r[0]["iterations"] / r[0]["seconds"], "unit": "outer iterations/s",
+               "n_gpus": 1, "steps": r[0]["iterations"], "warmup": a.warmup,
+               "ms_per_step": r[0]["ms_per_iter"], "higher_is_better": True, "scaling": "weak",
+               "vs_baseline": None, "dtype": "f64",
+               "data": "synthetic (lid-driven cavity from zero fields, lid velocity 1)",
+               "config": {"workload": "lid-driven cavity 8192x8192, Re 3200, QUICK, 4th-order "
+                                      "diffusion, gs_multicolour, mg V(2,2)", "forms": r}}
+        print(json.dumps(out))
+        return
     results = []
     probe = None
     for form in a.forms.split(","):
