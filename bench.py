@@ -36,6 +36,19 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "simple-hpc_amd", "py"))
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the dominant kernel from rocprofv3 FETCH_SIZE / WRITE_SIZE runs,
+# calibrated on a copy kernel (recorded with the commands that made them).
+PROFILE = os.path.join(ROOT, "profiles", "k_mom_sweep_1024sq_re1000.md")
+
+
+def recorded_traffic(n):
+    if n != 1024 or not os.path.exists(PROFILE):
+        return None
+    with open(PROFILE) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch ="):
+                return float(line.split("=")[1])
+    return None
 
 
 def config_text(n, form, re=1000, maxit=10000000):
@@ -301,6 +314,8 @@ def main():
         if probe is None:
             sec, bytes_, per = s.probe(sg.K_SWEEP_U, reps=200)
             probe = (sec, bytes_, per)
+            csec, cbytes, _ = s.probe(sg.K_COPY, reps=200)  # device copy, for calibration
+            copy = {"bytes_per_launch": cbytes, "us": csec * 1e6, "GBs": cbytes / csec / 1e9}
         s.close()
     head = results[0]
     sec, bytes_, per = probe
@@ -321,10 +336,10 @@ def main():
                                f"diffusion, gs_multicolour momentum, mg V(2,2) p', tol 1e-5",
                    "forms": results},
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": gbs / HBM_PEAK_GBS, "traffic": None,
+                     "frac": gbs / HBM_PEAK_GBS, "traffic": recorded_traffic(a.n),
                      "kernel": "k_mom_sweep<5> (one u multicolour sweep, all colours)",
                      "kernel_us": sec * 1e6, "bytes_per_launch": bytes_,
-                     "launches_per_iteration": per},
+                     "launches_per_iteration": per, "device_copy": copy},
     }
     if not a.no_cpu:
         log("cpu baseline")

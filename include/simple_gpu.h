@@ -161,7 +161,7 @@ int sg_iterations(sg_solver* s, int* iters);
 enum {
     SG_K_SWEEP_U = 0,  /* one u multicolour sweep (every colour, one launch) */
     SG_K_SWEEP_V,
-    SG_K_MG_SMOOTH0,   /* one colour pass of the level-0 multigrid smoother */
+    SG_K_COPY,         /* plain device copy of one field array (nrows x ld doubles): calibration */
     SG_K_ASSEMBLE_U,
     SG_K_PSOLVE,       /* one complete p' solve (pressure_solver of the params) */
     SG_K_COUNT

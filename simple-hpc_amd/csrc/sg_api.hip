@@ -644,6 +644,12 @@ int sg_kernel_probe(sg_solver* h, int kid, int reps, double* mean_s, double* byt
             per = 1;
             one = [&s](int) { launch_solve_p(s, s.never, false); };
             break;
+        case SG_K_COPY:
+            cells = static_cast<double>(s.n);
+            bpc = 16.0; // 8 B read + 8 B written per element
+            per = 0;
+            one = [&s](int) { launch_copy(s, s.F(SG_F_U), s.scratch, s.n, s.never); };
+            break;
         case SG_K_ASSEMBLE_U:
             cells = static_cast<double>(p.NX) * (p.NY + 1);
             bpc = 88.0; // u, v read; 9 coefficients written

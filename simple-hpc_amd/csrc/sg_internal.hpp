@@ -169,6 +169,7 @@ void launch_mom_sweeps(Solver& s, bool is_u, int nsweeps, const int* stop, doubl
 void launch_mom_sweep_once(Solver& s, bool is_u, const double* src, double* dst, bool first,
                            const int* stop);
 int ncolours(const sg_params& p);
+void launch_copy(Solver& s, const double* a, double* b, size_t n, const int* stop);
 Pack make_pack(const sg_params& p, bool is_u);
 void sync_packed(Solver& s, int field_id, bool to_packed);
 // pressure (sg_pressure.hip)

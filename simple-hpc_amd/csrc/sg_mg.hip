@@ -265,7 +265,9 @@ template <class T> __device__ __forceinline__ void k_mg_down(const DU<T>& a, con
     extern __shared__ __align__(16) unsigned char smem[];
     T* sx = reinterpret_cast<T*>(smem);
     const int H = 2 * pre + 1, LW = MT_J + 2 * H, LH = MT_I + 2 * H;
-    const int ti0 = bk.y * MT_I, tj0 = bk.x * MT_J;
+    int tx, ty;
+    xcd_tile(bk, tx, ty);
+    const int ti0 = ty * MT_I, tj0 = tx * MT_J;
     const int gi0 = ti0 - H, gj0 = tj0 - H;
     tile_load<T>(sx, F, xin, gi0, gj0, LH, LW, nullptr, T(0));
     for (int p = 0; p < 2 * pre; ++p) {
@@ -312,7 +314,9 @@ template <class T> __device__ __forceinline__ void k_mg_up(const DU<T>& a, const
     extern __shared__ __align__(16) unsigned char smem[];
     T* sx = reinterpret_cast<T*>(smem);
     const int H = 2 * post, LW = MT_J + 2 * H, LH = MT_I + 2 * H;
-    const int ti0 = bk.y * MT_I, tj0 = bk.x * MT_J;
+    int tx, ty;
+    xcd_tile(bk, tx, ty);
+    const int ti0 = ty * MT_I, tj0 = tx * MT_J;
     const int gi0 = ti0 - H, gj0 = tj0 - H;
     tile_load<T>(sx, F, xpre, gi0, gj0, LH, LW, &C, omega);
     for (int p = 0; p < 2 * post; ++p) {

@@ -228,7 +228,9 @@ template <int NCOL, int TI, int TJ> __device__ __forceinline__ void k_mom_sweep(
     constexpr int HL = R * NCOL;          // halo read by the first pass
     constexpr int LW = TJ + 2 * HL, LH = TI + 2 * HL;
     __shared__ double su[LH * LW];
-    const int ti0 = bk.y * TI, tj0 = bk.x * TJ;
+    int tx, ty;
+    xcd_tile(bk, tx, ty);
+    const int ti0 = ty * TI, tj0 = tx * TJ;
     const int gi0 = ti0 - HL, gj0 = tj0 - HL; // array cell of su[0]
     const long ld = m.ld;
     // Batched loads: every load of a chunk is in flight before the first LDS store.
@@ -339,9 +341,6 @@ __device__ __forceinline__ void k_copy(const CopyArgs& c, const Blk& bk) {
         c.b[k] = c.a[k];
 }
 
-void launch_copy(Solver& s, const double* a, double* b, size_t n, const int* stop) {
-    emit<CopyArgs, k_copy, CP_T>(s, dim3(1024), dim3(CP_T), 0, CopyArgs{a, b, n}, stop);
-}
 
 // Row-major <-> packed copies of coefficient array a (0..8) over the assembly range.
 __global__ void __launch_bounds__(BX* BY) k_pack(Pack pk, double* nat, int ld, int a, int to_packed) {
@@ -393,6 +392,10 @@ dim3 pass_grid(const MomArgs& m, int nz) {
 }
 
 } // namespace
+
+void launch_copy(Solver& s, const double* a, double* b, size_t n, const int* stop) {
+    emit<CopyArgs, k_copy, CP_T>(s, dim3(1024), dim3(CP_T), 0, CopyArgs{a, b, n}, stop);
+}
 
 int ncolours(const sg_params& p) {
     return (p.convection_scheme == SG_UDS && p.diffusion_order == 2) ? 2 : 5;

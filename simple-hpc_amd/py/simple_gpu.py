@@ -26,7 +26,7 @@ FIELD_ID = {n: k for k, n in enumerate(FIELDS)}
 NPHASES = 11
 PHASES = ["assemble_u", "bc_u", "residual_u", "sweep_u", "assemble_v", "bc_v", "residual_v",
           "sweep_v", "assemble_p", "solve_p", "correct"]
-K_SWEEP_U, K_SWEEP_V, K_MG_SMOOTH0, K_ASSEMBLE_U, K_PSOLVE = 0, 1, 2, 3, 4
+K_SWEEP_U, K_SWEEP_V, K_COPY, K_ASSEMBLE_U, K_PSOLVE = 0, 1, 2, 3, 4
 TUNE_TAIL_THREADS, TUNE_MID_CELLS, TUNE_LDS_CELLS, TUNE_MOM_TILE = 0, 1, 2, 3
 
 
