@@ -221,6 +221,42 @@ int sg_batch_get_tuning(sg_batch* b, int key, long* value);
  * fastest; report format as sg_autotune. */
 int sg_batch_autotune(sg_batch* b, int reps, char* report, size_t len);
 
+/* ---- Learned warm start of the p' solve ----
+ * No counterpart in the reference (it has no learned component). A U-Net (4 levels of 16, 32, 64,
+ * 128 channels, 3x3 convolutions, bfloat16 weights and activations, FP32 accumulation) maps the
+ * coarse-grid p' system (block sums of b, block means of a_P, log10(Re/1000)) to the block means
+ * of the converged p'. Its parameters are one FP32 vector (sg_unet_nparams entries; layout:
+ * per convolution W[c_out][c_in][3][3] then b[c_out], then the 1x1 head's 16 weights and bias).
+ * Images are NHWC float32 with 8 input channels (3 used), sides multiples of 8. */
+typedef struct sg_unet sg_unet;
+int sg_unet_nparams(int* n);
+int sg_unet_create(sg_unet** out);
+int sg_unet_destroy(sg_unet* net);
+/* Set / get the FP32 parameters (set also resets the optimiser state). */
+int sg_unet_set_params(sg_unet* net, const float* theta, int n);
+int sg_unet_get_params(sg_unet* net, float* theta, int n);
+/* Forward pass: x [nimg][h][w][8] -> y [nimg][h][w]. */
+int sg_unet_forward(sg_unet* net, int nimg, int h, int w, const float* x, float* y);
+/* Mean squared error against t [nimg][h][w] and its gradient (n params, may be NULL). */
+int sg_unet_loss_grad(sg_unet* net, int nimg, int h, int w, const float* x, const float* t,
+                      float* loss, float* grad);
+/* One Adam step (beta 0.9, 0.999) on the loss of one batch; the loss before the step. */
+int sg_unet_train_step(sg_unet* net, int nimg, int h, int w, const float* x, const float* t,
+                       float lr, float* loss);
+/* Mean seconds of one forward pass at this shape (hipEvents, reps) and its MFMA flops. */
+int sg_unet_mfma_probe(sg_unet* net, int nimg, int h, int w, int reps, double* seconds,
+                       double* flops);
+/* Use net (not owned) ahead of every p' solve of s when s's params say pressure_warm_start =
+ * learned (pressure_solver = mg, mg_precision = mixed), on multigrid level `level` (2^level
+ * blocks; N_cells_x and N_cells_y multiples of 8 * 2^level). A guess whose FP64 defect is not
+ * below ||b|| is discarded (p' starts from zero). Do not train net while s runs. */
+int sg_attach_warm_start(sg_solver* s, sg_unet* net, int level);
+/* The network input of s's current p' system at `level` (x: (NX/2^level)*(NY/2^level)*8 floats)
+ * and the scale that turns a prediction into p'. */
+int sg_warm_features(sg_solver* s, int level, float* x, double* scale);
+/* Guesses kept and discarded so far. */
+int sg_warm_stats(sg_solver* s, int* accepted, int* rejected);
+
 #ifdef __cplusplus
 }
 #endif

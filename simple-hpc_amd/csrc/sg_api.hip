@@ -245,6 +245,10 @@ void validate(const sg_params& p) {
         throw ArgError("mg_dc_maxit must be >= 1");
     if (p.check_every < 1)
         throw ArgError("check_every must be >= 1");
+    if (p.pressure_warm_start == SG_WARM_LEARNED &&
+        (p.pressure_solver != SG_P_MG || p.mg_precision != SG_MG_MIXED))
+        throw ArgError("pressure_warm_start = learned needs pressure_solver = mg and "
+                       "mg_precision = mixed");
 }
 
 } // namespace
@@ -413,6 +417,7 @@ Solver* create_solver(const sg_params& p, const double* u, const double* v, cons
 void destroy_solver(Solver* s) {
     if (s->stream)
         (void)hipStreamSynchronize(s->stream);
+    warm_free(*s);
     if (s->graph)
         (void)hipGraphExecDestroy(s->graph);
     for (auto e : s->ev)

@@ -75,7 +75,7 @@ struct Ctl {
     int diverged;   // a residual was not finite
     int log_pos;    // entries written into the log buffer in the current sg_advance call
     int coarse_sweeps;
-    int pad[2];
+    int warm_ok, warm_rejected; // learned warm start: guesses kept / discarded
     double mu, mv, mp; // running maxima (residual_normalisation = running_max)
 };
 
@@ -83,6 +83,8 @@ struct Ctl {
 enum {
     S_RES_U = 0, S_RES_V, S_RES_P, S_SUM_A, S_SUM_B, S_SUM_C, S_SUM_D, S_COUNT = 32
 };
+
+struct Warm;
 
 struct Solver {
     sg_params p{};
@@ -128,6 +130,7 @@ struct Solver {
     std::vector<Rec>* rec = nullptr;
     int tag = 0;
     int vcount = 0; // V-cycles enqueued in the current p' solve
+    Warm* warm = nullptr; // learned warm start of the p' solve (sg_unet.hip)
 
     const double* F(int id) const { return f[id]; }
     double* F(int id) { return f[id]; }
@@ -192,6 +195,9 @@ void join_side(Solver& s);
 void mg_setup(Solver& s);
 void launch_mg_solve(Solver& s, const int* stop, bool has_guess);
 void launch_remove_mean0(Solver& s, double* a, const int* stop);
+// learned warm start (sg_unet.hip): p' guess into the level-0 x, kept only if it lowers the defect
+void launch_warm_guess(Solver& s, const int* stop);
+void warm_free(Solver& s);
 // outer loop control (sg_api.hip)
 
 // One outer iteration (src/solver.cpp:27-93), enqueued or recorded. ev: optional phase events.

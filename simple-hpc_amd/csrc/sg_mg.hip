@@ -802,9 +802,16 @@ void launch_mg_solve(Solver& s, const int* stop, bool has_guess) {
     const MgLevel& L0 = s.mg[0];
     const Mix m{L0.nx, L0.ny, s.ld, L0.AE, L0.AW, L0.AN, L0.AS, L0.AP, L0.b, L0.x, t.G[0]};
     const dim3 g = lgrid(m.nx, m.ny);
-    if (!has_guess)
+    // Learned warm start: the network's guess of p' (or zero, if the guess does not lower the
+    // defect) replaces the zero start.
+    const bool warm = p.pressure_warm_start == SG_WARM_LEARNED && s.warm;
+    if (!has_guess && !warm)
         launch_fill(s, s.F(SG_F_PC), s.n, 0.0, stop);
     launch_remove_mean0(s, s.F(SG_F_B), stop);
+    if (warm) {
+        launch_warm_guess(s, stop);
+        has_guess = true;
+    }
     emit<Mix, k_mx_init, BX * BY>(s, g, dim3(BX, BY), 0, m, stop);
     launch_builds(s, t, stop);
     for (int k = 0; k < p.mg_dc_maxit; ++k) {

@@ -1,0 +1,1220 @@
+// sg_unet.hip — the learned warm start of the p' solve: a small U-Net run and trained on gfx950
+// with bfloat16 MFMA kernels (FP32 accumulation), and its use ahead of the multigrid solve.
+//
+// Network (no counterpart in the reference): 4 levels of 16, 32, 64, 128 channels, two 3x3
+// convolutions (zero padding, bias, ReLU) per level, 2x2 average pooling down, nearest 2x
+// upsampling and channel concatenation with the encoder features up, a 1x1 head to one channel.
+// Input: 3 channels (mass imbalance, a_P, Re) zero-padded to 8. Activations and weights are
+// bfloat16 (round to nearest even), NHWC; accumulators, biases, the head, gradients of weights,
+// the loss and Adam's state are FP32.
+//
+// Convolution as implicit GEMM on v_mfma_f32_16x16x32_bf16: rows = output pixels, columns =
+// output channels, K = 9 * C_in ordered (tap, channel), so the 8 K-values a lane feeds are 8
+// consecutive channels of one input pixel (one 16-byte load). A wave computes 16 pixels x 64
+// channels (4 accumulators). The input gradient is the same kernel on the output gradient with the
+// flipped, transposed weights; the weight gradient is a GEMM over pixels (rows = output channels,
+// columns = (tap, input channel)), split over pixel ranges whose partial sums are added in a
+// fixed order, so training is deterministic.
+//
+// Warm start: the p' system is reduced to a coarse grid (level L: 2^L x 2^L blocks): block sums of
+// b and block means of a_P, normalised by rms(block b) and mean(block a_P) (deterministic
+// fixed-order sums). The network predicts the block means of the converged p' in those units;
+// the guess p'(i, j) = prediction(block) * scale is kept only if its FP64 defect ||b - A p'|| is
+// below ||b|| (else p' starts from zero, the plain solve), and the mixed-precision multigrid solve
+// then starts from it. The stopping rule of the outer loop is untouched: a run ends only when the
+// SIMPLE residuals meet the tolerance.
+#include "sg_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+
+namespace sg {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16;
+
+constexpr int IN_CH = 3, IN_PAD = 8;
+constexpr int LV[4] = {16, 32, 64, 128};
+constexpr int NCONV = 14;
+
+struct Spec {
+    int cin, cout, ci_real;
+};
+// The 14 convolutions in execution order (input channels as stored).
+constexpr Spec SPEC[NCONV] = {
+    {IN_PAD, 16, IN_CH}, {16, 16, 16},   {16, 32, 16},   {32, 32, 32},   {32, 64, 32},
+    {64, 64, 64},        {64, 128, 64},  {128, 128, 128}, {192, 64, 192}, {64, 64, 64},
+    {96, 32, 96},        {32, 32, 32},   {48, 16, 48},   {16, 16, 16}};
+
+int pad32(int k) { return (k + 31) / 32 * 32; }
+
+__device__ __forceinline__ u16 f2bf(float f) {
+    unsigned u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return static_cast<u16>(u >> 16);
+}
+__device__ __forceinline__ float bf2f(u16 h) { return __uint_as_float(static_cast<unsigned>(h) << 16); }
+
+__device__ __forceinline__ bf16x8 ld8(const u16* p) {
+    const uint4 v = *reinterpret_cast<const uint4*>(p);
+    return __builtin_bit_cast(bf16x8, v);
+}
+__device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, make_uint4(0, 0, 0, 0)); }
+__device__ __forceinline__ bf16x8 pack8(const u16* v) {
+    uint4 r;
+    r.x = v[0] | (static_cast<unsigned>(v[1]) << 16);
+    r.y = v[2] | (static_cast<unsigned>(v[3]) << 16);
+    r.z = v[4] | (static_cast<unsigned>(v[5]) << 16);
+    r.w = v[6] | (static_cast<unsigned>(v[7]) << 16);
+    return __builtin_bit_cast(bf16x8, r);
+}
+
+// ---- weights ----
+
+// FP32 master (co, ci_real, ky, kx) -> forward weights [co][kpad] (k = tap * cin + ci) and input-
+// gradient weights [ci][kpadt] (k = tap' * cout + co, tap' = 8 - tap), zero elsewhere.
+__global__ void k_wprep(const float* master, int cin, int cout, int ci_real, int kpad, int kpadt,
+                        u16* wf, u16* wt) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nf = cout * kpad;
+    if (e < nf) {
+        const int co = e / kpad, k = e - co * kpad;
+        float v = 0.0f;
+        if (k < 9 * cin) {
+            const int tap = k / cin, ci = k - tap * cin;
+            if (ci < ci_real)
+                v = master[(static_cast<size_t>(co) * ci_real + ci) * 9 + tap];
+        }
+        wf[e] = f2bf(v);
+    }
+    const int nt = cin * kpadt;
+    if (wt && e < nt) {
+        const int ci = e / kpadt, k = e - ci * kpadt;
+        float v = 0.0f;
+        if (k < 9 * cout && ci < ci_real) {
+            const int tp = k / cout, co = k - tp * cout;
+            v = master[(static_cast<size_t>(co) * ci_real + ci) * 9 + (8 - tp)];
+        }
+        wt[e] = f2bf(v);
+    }
+}
+
+// ---- convolution (forward and input gradient) ----
+
+struct ConvArgs {
+    const u16* x;
+    int ldx;         // input view: pixel stride (channels of the buffer)
+    const u16* w;    // [cout][kpad]
+    int kpad;
+    const float* bias; // may be null
+    u16* y;
+    int ldy;         // output view
+    int n, h, wd, cin, cout;
+    int relu;
+};
+
+constexpr int CV_T = 256; // 4 waves x 16 pixels; 64 output channels per workgroup
+
+__global__ void __launch_bounds__(CV_T) k_conv(ConvArgs a) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int m = lane & 15, oct = lane >> 4;
+    const int npix = a.n * a.h * a.wd;
+    const int pix0 = (blockIdx.x * 4 + wave) * 16;
+    const int co0 = blockIdx.y * 64;
+    const int p = pix0 + m;
+    const bool pv = p < npix;
+    int pn = 0, pi = 0, pj = 0;
+    if (pv) {
+        pj = p % a.wd;
+        const int r = p / a.wd;
+        pi = r % a.h;
+        pn = r / a.h;
+    }
+    f4 acc[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+        acc[c] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+    const int K = 9 * a.cin;
+    for (int k0 = 0; k0 < a.kpad; k0 += 32) {
+        const int kk = k0 + oct * 8;
+        bf16x8 A = zero8();
+        if (pv && kk < K) {
+            const int tap = kk / a.cin, ci = kk - tap * a.cin;
+            const int ii = pi + tap / 3 - 1, jj = pj + tap % 3 - 1;
+            if (ii >= 0 && ii < a.h && jj >= 0 && jj < a.wd)
+                A = ld8(a.x + (static_cast<size_t>(pn * a.h + ii) * a.wd + jj) * a.ldx + ci);
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int co = co0 + c * 16 + m;
+            const bf16x8 B = co < a.cout ? ld8(a.w + static_cast<size_t>(co) * a.kpad + kk) : zero8();
+            acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[c], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int co = co0 + c * 16 + m;
+        if (co >= a.cout)
+            continue;
+        const float b = a.bias ? a.bias[co] : 0.0f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int q = pix0 + 4 * oct + i;
+            if (q >= npix)
+                continue;
+            float z = acc[c][i] + b;
+            if (a.relu)
+                z = fmaxf(z, 0.0f);
+            a.y[static_cast<size_t>(q) * a.ldy + co] = f2bf(z);
+        }
+    }
+}
+
+// ---- weight gradient ----
+
+struct WgArgs {
+    const u16* x;
+    int ldx, cin;
+    const u16* dz;
+    int ldz, cout;
+    float* part; // [splits][cout][9*cin]
+    int n, h, wd, chunk;
+};
+
+// grid: x = cdiv(9*cin, 64), y = cdiv(cout, 16), z = pixel splits.
+__global__ void __launch_bounds__(256) k_wgrad(WgArgs a) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int m = lane & 15, oct = lane >> 4;
+    const int K = 9 * a.cin;
+    const int co = blockIdx.y * 16 + m;
+    const int kc = blockIdx.x * 64 + wave * 16 + m;
+    int tap = 0, ci = 0, dy = 0, dx = 0;
+    if (kc < K) {
+        tap = kc / a.cin;
+        ci = kc - tap * a.cin;
+        dy = tap / 3 - 1;
+        dx = tap % 3 - 1;
+    }
+    const int npix = a.n * a.h * a.wd;
+    const int q0b = blockIdx.z * a.chunk, q1 = min(npix, q0b + a.chunk);
+    f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int q0 = q0b; q0 < q1; q0 += 32) {
+        u16 av[8], bv[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int q = q0 + oct * 8 + t;
+            av[t] = 0;
+            bv[t] = 0;
+            if (q < q1) {
+                if (co < a.cout)
+                    av[t] = a.dz[static_cast<size_t>(q) * a.ldz + co];
+                if (kc < K) {
+                    const int j = q % a.wd, r = q / a.wd, i = r % a.h, nn = r / a.h;
+                    const int ii = i + dy, jj = j + dx;
+                    if (ii >= 0 && ii < a.h && jj >= 0 && jj < a.wd)
+                        bv[t] = a.x[(static_cast<size_t>(nn * a.h + ii) * a.wd + jj) * a.ldx + ci];
+                }
+            }
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(pack8(av), pack8(bv), acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int cr = blockIdx.y * 16 + 4 * oct + i;
+        const int kr = blockIdx.x * 64 + wave * 16 + m;
+        if (cr < a.cout && kr < K)
+            a.part[(static_cast<size_t>(blockIdx.z) * a.cout + cr) * K + kr] = acc[i];
+    }
+}
+
+// Sum of the splits in order, into the master-layout gradient (co, ci_real, tap).
+__global__ void k_wreduce(const float* part, int splits, int cout, int cin, int ci_real, float* g) {
+    const int K = 9 * cin;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= cout * K)
+        return;
+    float s = 0.0f;
+    for (int z = 0; z < splits; ++z)
+        s += part[static_cast<size_t>(z) * cout * K + e];
+    const int co = e / K, k = e - co * K, tap = k / cin, ci = k - tap * cin;
+    if (ci < ci_real)
+        g[(static_cast<size_t>(co) * ci_real + ci) * 9 + tap] = s;
+}
+
+// Bias gradient: sum over pixels, one workgroup per channel, fixed-order tree.
+__global__ void __launch_bounds__(256) k_bgrad(const u16* dz, int ldz, int npix, float* g) {
+    __shared__ float sh[256];
+    const int co = blockIdx.x;
+    float s = 0.0f;
+    for (int q = threadIdx.x; q < npix; q += 256)
+        s += bf2f(dz[static_cast<size_t>(q) * ldz + co]);
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        g[co] = sh[0];
+}
+
+// ---- elementwise pieces (views: base pointer, pixel stride) ----
+
+// d *= (out > 0), c channels.
+__global__ void k_mask(u16* d, int ldd, const u16* out, int ldo, int npix, int c) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= npix * c)
+        return;
+    const int q = e / c, ch = e - q * c;
+    if (!(bf2f(out[static_cast<size_t>(q) * ldo + ch]) > 0.0f))
+        d[static_cast<size_t>(q) * ldd + ch] = 0;
+}
+
+// 2x2 average pooling: x (n, h, w) -> y (n, h/2, w/2).
+__global__ void k_pool(const u16* x, int ldx, u16* y, int ldy, int n, int h, int w, int c) {
+    const int ho = h / 2, wo = w / 2;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * ho * wo * c)
+        return;
+    const int ch = e % c, q = e / c, j = q % wo, r = q / wo, i = r % ho, nn = r / ho;
+    const size_t b = (static_cast<size_t>(nn * h + 2 * i) * w + 2 * j) * ldx + ch;
+    const float s = (bf2f(x[b]) + bf2f(x[b + ldx])) + (bf2f(x[b + static_cast<size_t>(w) * ldx]) +
+                                                      bf2f(x[b + static_cast<size_t>(w + 1) * ldx]));
+    y[static_cast<size_t>(q) * ldy + ch] = f2bf(s * 0.25f);
+}
+
+// Nearest 2x upsampling: x (n, h, w) -> y (n, 2h, 2w).
+__global__ void k_up(const u16* x, int ldx, u16* y, int ldy, int n, int h, int w, int c) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int H = 2 * h, W = 2 * w;
+    if (e >= n * H * W * c)
+        return;
+    const int ch = e % c, q = e / c, j = q % W, r = q / W, i = r % H, nn = r / H;
+    y[static_cast<size_t>(q) * ldy + ch] =
+        x[(static_cast<size_t>(nn * h + i / 2) * w + j / 2) * ldx + ch];
+}
+
+// Upsampling's gradient: sum of the 4 children, (n, 2h, 2w) -> (n, h, w).
+__global__ void k_up_back(const u16* d, int ldd, u16* y, int ldy, int n, int h, int w, int c) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * h * w * c)
+        return;
+    const int W = 2 * w;
+    const int ch = e % c, q = e / c, j = q % w, r = q / w, i = r % h, nn = r / h;
+    const size_t b = (static_cast<size_t>(nn * 2 * h + 2 * i) * W + 2 * j) * ldd + ch;
+    const float s = (bf2f(d[b]) + bf2f(d[b + ldd])) + (bf2f(d[b + static_cast<size_t>(W) * ldd]) +
+                                                      bf2f(d[b + static_cast<size_t>(W + 1) * ldd]));
+    y[static_cast<size_t>(q) * ldy + ch] = f2bf(s);
+}
+
+// Pooling's gradient plus the skip gradient: y(n, 2h, 2w) = d(parent) / 4 + skip.
+__global__ void k_pool_back_add(const u16* d, int ldd, const u16* skip, int lds, u16* y, int ldy,
+                                int n, int h, int w, int c) {
+    const int H = 2 * h, W = 2 * w;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * H * W * c)
+        return;
+    const int ch = e % c, q = e / c, j = q % W, r = q / W, i = r % H, nn = r / H;
+    const float v = bf2f(d[(static_cast<size_t>(nn * h + i / 2) * w + j / 2) * ldd + ch]) * 0.25f +
+                    bf2f(skip[static_cast<size_t>(q) * lds + ch]);
+    y[static_cast<size_t>(q) * ldy + ch] = f2bf(v);
+}
+
+// Head: y = d0 . hw + hb (FP32 output).
+__global__ void k_head(const u16* d0, int ld, const float* hw, const float* hb, float* y, int npix) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= npix)
+        return;
+    float s = 0.0f;
+    for (int c = 0; c < LV[0]; ++c)
+        s += bf2f(d0[static_cast<size_t>(q) * ld + c]) * hw[c];
+    y[q] = s + *hb;
+}
+
+// Loss pieces: r = y - t, dy = 2 r / npix (per pixel), per-workgroup partial sums of r^2.
+__global__ void __launch_bounds__(256) k_loss(const float* y, const float* t, float* dy, int npix,
+                                              float* part) {
+    __shared__ float sh[256];
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    float r2 = 0.0f;
+    if (q < npix) {
+        const float r = y[q] - t[q];
+        dy[q] = 2.0f * r / static_cast<float>(npix);
+        r2 = r * r;
+    }
+    sh[threadIdx.x] = r2;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        part[blockIdx.x] = sh[0];
+}
+
+// Head backward: dd0 = bf16(dy * hw); per-workgroup partials of d0 * dy (16) and dy (1).
+__global__ void __launch_bounds__(256) k_head_back(const u16* d0, int ld, const float* dy,
+                                                   const float* hw, u16* dd0, int ldd, int npix,
+                                                   float* part) {
+    __shared__ float sh[256][LV[0] + 1];
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    float g = 0.0f;
+    if (q < npix)
+        g = dy[q];
+    for (int c = 0; c < LV[0]; ++c) {
+        float v = 0.0f;
+        if (q < npix) {
+            dd0[static_cast<size_t>(q) * ldd + c] = f2bf(g * hw[c]);
+            v = bf2f(d0[static_cast<size_t>(q) * ld + c]) * g;
+        }
+        sh[threadIdx.x][c] = v;
+    }
+    sh[threadIdx.x][LV[0]] = g;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            for (int c = 0; c <= LV[0]; ++c)
+                sh[threadIdx.x][c] += sh[threadIdx.x + w][c];
+        __syncthreads();
+    }
+    if (threadIdx.x <= LV[0])
+        part[blockIdx.x * (LV[0] + 1) + threadIdx.x] = sh[0][threadIdx.x];
+}
+
+// Sum of nb partial rows of width w in order (one thread per column).
+__global__ void k_sum_rows(const float* part, int nb, int w, float* out) {
+    const int c = threadIdx.x;
+    if (c >= w)
+        return;
+    float s = 0.0f;
+    for (int b = 0; b < nb; ++b)
+        s += part[static_cast<size_t>(b) * w + c];
+    out[c] = s;
+}
+
+// Adam (FP32 master, bias-corrected).
+__global__ void k_adam(float* w, const float* g, float* m1, float* m2, int n, float lr, float b1,
+                       float b2, float eps, float c1, float c2) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n)
+        return;
+    const float gi = g[e];
+    const float a = b1 * m1[e] + (1.0f - b1) * gi;
+    const float b = b2 * m2[e] + (1.0f - b2) * gi * gi;
+    m1[e] = a;
+    m2[e] = b;
+    w[e] -= lr * (a / c1) / (sqrtf(b / c2) + eps);
+}
+
+// f32 NHWC (8 channels) -> bf16 input view.
+__global__ void k_to_bf(const float* x, u16* y, int n) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n)
+        y[e] = f2bf(x[e]);
+}
+
+unsigned g1(long n, int t = 256) { return static_cast<unsigned>((n + t - 1) / t); }
+
+} // namespace
+
+// ---- the network object ----
+
+struct UNet {
+    int nparams = 0;
+    int woff[NCONV] = {}, boff[NCONV] = {}, hwoff = 0, hboff = 0;
+    int kpad[NCONV] = {}, kpadt[NCONV] = {};
+    float *master = nullptr, *grad = nullptr, *m1 = nullptr, *m2 = nullptr;
+    u16 *wf[NCONV] = {}, *wt[NCONV] = {};
+    int step = 0;
+    hipStream_t stream = nullptr;
+    std::vector<void*> owned;
+    // activation workspaces by (n, h, w)
+    struct Ctx {
+        int n = 0, h = 0, w = 0;
+        u16* x_in = nullptr;
+        u16 *t0, *cat0, *d0a, *d0, *p0, *t1, *cat1, *d1a, *d1, *p1, *t2, *cat2, *d2a, *d2, *p2,
+            *t3, *bt;
+        float *y = nullptr, *t = nullptr, *dy = nullptr, *xf = nullptr, *part = nullptr;
+        float* lsum = nullptr; // loss: sum of squared errors
+        // gradients
+        u16 *g_d0, *g_d0a, *g_cat0, *g_d1, *g_d1a, *g_cat1, *g_d2, *g_d2a, *g_cat2, *g_bt, *g_t3,
+            *g_p2, *g_e2, *g_t2, *g_p1, *g_e1, *g_t1, *g_p0, *g_e0, *g_t0;
+        float* wpart = nullptr; // weight-gradient splits
+        size_t wpart_elems = 0;
+        std::vector<void*> owned;
+        ~Ctx() {
+            for (void* q : owned)
+                (void)hipFree(q);
+        }
+    };
+    std::map<std::tuple<int, int, int>, std::unique_ptr<Ctx>> ctx;
+
+    template <class T> T* alloc(std::vector<void*>& list, size_t n) {
+        void* q = nullptr;
+        SG_HIP(hipMalloc(&q, sizeof(T) * std::max<size_t>(n, 1)));
+        SG_HIP(hipMemset(q, 0, sizeof(T) * std::max<size_t>(n, 1)));
+        list.push_back(q);
+        return static_cast<T*>(q);
+    }
+
+    UNet() {
+        int o = 0;
+        for (int k = 0; k < NCONV; ++k) {
+            woff[k] = o;
+            o += SPEC[k].cout * SPEC[k].ci_real * 9;
+            boff[k] = o;
+            o += SPEC[k].cout;
+            kpad[k] = pad32(9 * SPEC[k].cin);
+            kpadt[k] = pad32(9 * SPEC[k].cout);
+        }
+        hwoff = o;
+        o += LV[0];
+        hboff = o;
+        o += 1;
+        nparams = o;
+        SG_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        master = alloc<float>(owned, nparams);
+        grad = alloc<float>(owned, nparams);
+        m1 = alloc<float>(owned, nparams);
+        m2 = alloc<float>(owned, nparams);
+        for (int k = 0; k < NCONV; ++k) {
+            wf[k] = alloc<u16>(owned, static_cast<size_t>(SPEC[k].cout) * kpad[k]);
+            wt[k] = alloc<u16>(owned, static_cast<size_t>(SPEC[k].cin) * kpadt[k]);
+        }
+    }
+    ~UNet() {
+        if (stream)
+            (void)hipStreamSynchronize(stream);
+        ctx.clear();
+        for (void* q : owned)
+            (void)hipFree(q);
+        if (stream)
+            (void)hipStreamDestroy(stream);
+    }
+
+    void prep_weights(hipStream_t st) {
+        for (int k = 0; k < NCONV; ++k) {
+            const Spec& s = SPEC[k];
+            const long n = std::max<long>(static_cast<long>(s.cout) * kpad[k],
+                                          static_cast<long>(s.cin) * kpadt[k]);
+            hipLaunchKernelGGL(k_wprep, dim3(g1(n)), dim3(256), 0, st, master + woff[k], s.cin,
+                               s.cout, s.ci_real, kpad[k], kpadt[k], wf[k], wt[k]);
+            SG_HIP(hipGetLastError());
+        }
+    }
+
+    Ctx& get_ctx(int n, int h, int w) {
+        auto key = std::make_tuple(n, h, w);
+        auto it = ctx.find(key);
+        if (it != ctx.end())
+            return *it->second;
+        if (h % 8 || w % 8 || h < 8 || w < 8 || n < 1)
+            throw ArgError("U-Net input sides must be positive multiples of 8");
+        auto c = std::make_unique<Ctx>();
+        Ctx& x = *c;
+        x.n = n;
+        x.h = h;
+        x.w = w;
+        auto& L = x.owned;
+        const size_t P0 = static_cast<size_t>(n) * h * w, P1 = P0 / 4, P2 = P0 / 16, P3 = P0 / 64;
+        x.x_in = alloc<u16>(L, P0 * IN_PAD);
+        x.xf = alloc<float>(L, P0 * IN_PAD);
+        x.t0 = alloc<u16>(L, P0 * 16);
+        x.cat0 = alloc<u16>(L, P0 * 48);
+        x.d0a = alloc<u16>(L, P0 * 16);
+        x.d0 = alloc<u16>(L, P0 * 16);
+        x.p0 = alloc<u16>(L, P1 * 16);
+        x.t1 = alloc<u16>(L, P1 * 32);
+        x.cat1 = alloc<u16>(L, P1 * 96);
+        x.d1a = alloc<u16>(L, P1 * 32);
+        x.d1 = alloc<u16>(L, P1 * 32);
+        x.p1 = alloc<u16>(L, P2 * 32);
+        x.t2 = alloc<u16>(L, P2 * 64);
+        x.cat2 = alloc<u16>(L, P2 * 192);
+        x.d2a = alloc<u16>(L, P2 * 64);
+        x.d2 = alloc<u16>(L, P2 * 64);
+        x.p2 = alloc<u16>(L, P3 * 64);
+        x.t3 = alloc<u16>(L, P3 * 128);
+        x.bt = alloc<u16>(L, P3 * 128);
+        x.y = alloc<float>(L, P0);
+        x.t = alloc<float>(L, P0);
+        x.dy = alloc<float>(L, P0);
+        x.part = alloc<float>(L, (P0 / 256 + 2) * (LV[0] + 1));
+        x.lsum = alloc<float>(L, 1);
+        x.g_d0 = alloc<u16>(L, P0 * 16);
+        x.g_d0a = alloc<u16>(L, P0 * 16);
+        x.g_cat0 = alloc<u16>(L, P0 * 48);
+        x.g_d1 = alloc<u16>(L, P1 * 32);
+        x.g_d1a = alloc<u16>(L, P1 * 32);
+        x.g_cat1 = alloc<u16>(L, P1 * 96);
+        x.g_d2 = alloc<u16>(L, P2 * 64);
+        x.g_d2a = alloc<u16>(L, P2 * 64);
+        x.g_cat2 = alloc<u16>(L, P2 * 192);
+        x.g_bt = alloc<u16>(L, P3 * 128);
+        x.g_t3 = alloc<u16>(L, P3 * 128);
+        x.g_p2 = alloc<u16>(L, P3 * 64);
+        x.g_e2 = alloc<u16>(L, P2 * 64);
+        x.g_t2 = alloc<u16>(L, P2 * 64);
+        x.g_p1 = alloc<u16>(L, P2 * 32);
+        x.g_e1 = alloc<u16>(L, P1 * 32);
+        x.g_t1 = alloc<u16>(L, P1 * 32);
+        x.g_p0 = alloc<u16>(L, P1 * 16);
+        x.g_e0 = alloc<u16>(L, P0 * 16);
+        x.g_t0 = alloc<u16>(L, P0 * 16);
+        // weight-gradient splits: largest cout * 9 * cin over the layers, times the split count
+        size_t mx = 0;
+        for (int k = 0; k < NCONV; ++k)
+            mx = std::max<size_t>(mx, static_cast<size_t>(SPEC[k].cout) * 9 * SPEC[k].cin);
+        x.wpart_elems = mx * wsplits(P0);
+        x.wpart = alloc<float>(L, x.wpart_elems);
+        Ctx& ref = *c;
+        ctx.emplace(key, std::move(c));
+        return ref;
+    }
+
+    // Pixel splits of the weight gradient at npix pixels (each split a multiple of 32 pixels).
+    static int wsplits(size_t npix) { return static_cast<int>(std::min<size_t>(64, std::max<size_t>(1, npix / 1024))); }
+
+    void conv(hipStream_t st, int k, const u16* x, int ldx, u16* y, int ldy, int n, int h, int w,
+              bool dgrad) {
+        const Spec& s = SPEC[k];
+        ConvArgs a{};
+        a.x = x;
+        a.ldx = ldx;
+        a.y = y;
+        a.ldy = ldy;
+        a.n = n;
+        a.h = h;
+        a.wd = w;
+        if (!dgrad) {
+            a.w = wf[k];
+            a.kpad = kpad[k];
+            a.bias = master + boff[k];
+            a.cin = s.cin;
+            a.cout = s.cout;
+            a.relu = 1;
+        } else {
+            a.w = wt[k];
+            a.kpad = kpadt[k];
+            a.bias = nullptr;
+            a.cin = s.cout;
+            a.cout = s.cin;
+            a.relu = 0;
+        }
+        const long npix = static_cast<long>(n) * h * w;
+        hipLaunchKernelGGL(k_conv, dim3(g1(npix, 64), cdiv(a.cout, 64)), dim3(CV_T), 0, st, a);
+        SG_HIP(hipGetLastError());
+    }
+
+    void forward(hipStream_t st, Ctx& c) {
+        const int n = c.n, h = c.h, w = c.w;
+        const int h1 = h / 2, w1 = w / 2, h2 = h / 4, w2 = w / 4, h3 = h / 8, w3 = w / 8;
+        auto ew = [&](long cnt) { return dim3(g1(cnt)); };
+        conv(st, 0, c.x_in, IN_PAD, c.t0, 16, n, h, w, false);
+        conv(st, 1, c.t0, 16, c.cat0 + 32, 48, n, h, w, false); // e0
+        hipLaunchKernelGGL(k_pool, ew(static_cast<long>(n) * h1 * w1 * 16), dim3(256), 0, st,
+                           c.cat0 + 32, 48, c.p0, 16, n, h, w, 16);
+        conv(st, 2, c.p0, 16, c.t1, 32, n, h1, w1, false);
+        conv(st, 3, c.t1, 32, c.cat1 + 64, 96, n, h1, w1, false); // e1
+        hipLaunchKernelGGL(k_pool, ew(static_cast<long>(n) * h2 * w2 * 32), dim3(256), 0, st,
+                           c.cat1 + 64, 96, c.p1, 32, n, h1, w1, 32);
+        conv(st, 4, c.p1, 32, c.t2, 64, n, h2, w2, false);
+        conv(st, 5, c.t2, 64, c.cat2 + 128, 192, n, h2, w2, false); // e2
+        hipLaunchKernelGGL(k_pool, ew(static_cast<long>(n) * h3 * w3 * 64), dim3(256), 0, st,
+                           c.cat2 + 128, 192, c.p2, 64, n, h2, w2, 64);
+        conv(st, 6, c.p2, 64, c.t3, 128, n, h3, w3, false);
+        conv(st, 7, c.t3, 128, c.bt, 128, n, h3, w3, false);
+        hipLaunchKernelGGL(k_up, ew(static_cast<long>(n) * h2 * w2 * 128), dim3(256), 0, st, c.bt,
+                           128, c.cat2, 192, n, h3, w3, 128);
+        conv(st, 8, c.cat2, 192, c.d2a, 64, n, h2, w2, false);
+        conv(st, 9, c.d2a, 64, c.d2, 64, n, h2, w2, false);
+        hipLaunchKernelGGL(k_up, ew(static_cast<long>(n) * h1 * w1 * 64), dim3(256), 0, st, c.d2,
+                           64, c.cat1, 96, n, h2, w2, 64);
+        conv(st, 10, c.cat1, 96, c.d1a, 32, n, h1, w1, false);
+        conv(st, 11, c.d1a, 32, c.d1, 32, n, h1, w1, false);
+        hipLaunchKernelGGL(k_up, ew(static_cast<long>(n) * h * w * 32), dim3(256), 0, st, c.d1, 32,
+                           c.cat0, 48, n, h1, w1, 32);
+        conv(st, 12, c.cat0, 48, c.d0a, 16, n, h, w, false);
+        conv(st, 13, c.d0a, 16, c.d0, 16, n, h, w, false);
+        const long P0 = static_cast<long>(n) * h * w;
+        hipLaunchKernelGGL(k_head, dim3(g1(P0)), dim3(256), 0, st, c.d0, 16, master + hwoff,
+                           master + hboff, c.y, static_cast<int>(P0));
+        SG_HIP(hipGetLastError());
+    }
+
+    // Weight and bias gradients of conv k (input view x, masked output gradient dz).
+    void wgrad(hipStream_t st, Ctx& c, int k, const u16* x, int ldx, const u16* dz, int ldz, int n,
+               int h, int w) {
+        const Spec& s = SPEC[k];
+        const long npix = static_cast<long>(n) * h * w;
+        const int splits = wsplits(npix);
+        int chunk = static_cast<int>((npix + splits - 1) / splits);
+        chunk = (chunk + 31) / 32 * 32;
+        WgArgs a{x, ldx, s.cin, dz, ldz, s.cout, c.wpart, n, h, w, chunk};
+        hipLaunchKernelGGL(k_wgrad, dim3(cdiv(9 * s.cin, 64), cdiv(s.cout, 16), splits),
+                           dim3(256), 0, st, a);
+        SG_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_wreduce, dim3(g1(static_cast<long>(s.cout) * 9 * s.cin)), dim3(256),
+                           0, st, c.wpart, splits, s.cout, s.cin, s.ci_real, grad + woff[k]);
+        SG_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_bgrad, dim3(s.cout), dim3(256), 0, st, dz, ldz,
+                           static_cast<int>(npix), grad + boff[k]);
+        SG_HIP(hipGetLastError());
+    }
+
+    void mask(hipStream_t st, u16* d, int ldd, const u16* out, int ldo, long npix, int ch) {
+        hipLaunchKernelGGL(k_mask, dim3(g1(npix * ch)), dim3(256), 0, st, d, ldd, out, ldo,
+                           static_cast<int>(npix), ch);
+        SG_HIP(hipGetLastError());
+    }
+
+    // Layer k backward: mask its output gradient in place, weight gradients, input gradient.
+    void layer_back(hipStream_t st, Ctx& c, int k, u16* gout, int ldg, const u16* out, int ldo,
+                    const u16* x, int ldx, u16* gin, int ldgin, int n, int h, int w) {
+        const long npix = static_cast<long>(n) * h * w;
+        mask(st, gout, ldg, out, ldo, npix, SPEC[k].cout);
+        wgrad(st, c, k, x, ldx, gout, ldg, n, h, w);
+        if (gin)
+            conv(st, k, gout, ldg, gin, ldgin, n, h, w, true);
+    }
+
+    // Loss (mean squared error over all pixels) and every gradient into grad.
+    void backward(hipStream_t st, Ctx& c) {
+        const int n = c.n, h = c.h, w = c.w;
+        const int h1 = h / 2, w1 = w / 2, h2 = h / 4, w2 = w / 4, h3 = h / 8, w3 = w / 8;
+        const long P0 = static_cast<long>(n) * h * w;
+        const int nb = static_cast<int>(g1(P0));
+        hipLaunchKernelGGL(k_loss, dim3(nb), dim3(256), 0, st, c.y, c.t, c.dy,
+                           static_cast<int>(P0), c.part);
+        SG_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(64), 0, st, c.part, nb, 1, c.lsum);
+        SG_HIP(hipGetLastError());
+        auto ew = [&](long cnt) { return dim3(g1(cnt)); };
+        hipLaunchKernelGGL(k_head_back, dim3(nb), dim3(256), 0, st, c.d0, 16, c.dy,
+                           master + hwoff, c.g_d0, 16, static_cast<int>(P0), c.part);
+        SG_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(64), 0, st, c.part, nb, LV[0] + 1,
+                           grad + hwoff);
+        SG_HIP(hipGetLastError());
+        layer_back(st, c, 13, c.g_d0, 16, c.d0, 16, c.d0a, 16, c.g_d0a, 16, n, h, w);
+        layer_back(st, c, 12, c.g_d0a, 16, c.d0a, 16, c.cat0, 48, c.g_cat0, 48, n, h, w);
+        hipLaunchKernelGGL(k_up_back, ew(static_cast<long>(n) * h1 * w1 * 32), dim3(256), 0, st,
+                           c.g_cat0, 48, c.g_d1, 32, n, h1, w1, 32);
+        layer_back(st, c, 11, c.g_d1, 32, c.d1, 32, c.d1a, 32, c.g_d1a, 32, n, h1, w1);
+        layer_back(st, c, 10, c.g_d1a, 32, c.d1a, 32, c.cat1, 96, c.g_cat1, 96, n, h1, w1);
+        hipLaunchKernelGGL(k_up_back, ew(static_cast<long>(n) * h2 * w2 * 64), dim3(256), 0, st,
+                           c.g_cat1, 96, c.g_d2, 64, n, h2, w2, 64);
+        layer_back(st, c, 9, c.g_d2, 64, c.d2, 64, c.d2a, 64, c.g_d2a, 64, n, h2, w2);
+        layer_back(st, c, 8, c.g_d2a, 64, c.d2a, 64, c.cat2, 192, c.g_cat2, 192, n, h2, w2);
+        hipLaunchKernelGGL(k_up_back, ew(static_cast<long>(n) * h3 * w3 * 128), dim3(256), 0, st,
+                           c.g_cat2, 192, c.g_bt, 128, n, h3, w3, 128);
+        layer_back(st, c, 7, c.g_bt, 128, c.bt, 128, c.t3, 128, c.g_t3, 128, n, h3, w3);
+        layer_back(st, c, 6, c.g_t3, 128, c.t3, 128, c.p2, 64, c.g_p2, 64, n, h3, w3);
+        hipLaunchKernelGGL(k_pool_back_add, ew(static_cast<long>(n) * h2 * w2 * 64), dim3(256), 0,
+                           st, c.g_p2, 64, c.g_cat2 + 128, 192, c.g_e2, 64, n, h3, w3, 64);
+        layer_back(st, c, 5, c.g_e2, 64, c.cat2 + 128, 192, c.t2, 64, c.g_t2, 64, n, h2, w2);
+        layer_back(st, c, 4, c.g_t2, 64, c.t2, 64, c.p1, 32, c.g_p1, 32, n, h2, w2);
+        hipLaunchKernelGGL(k_pool_back_add, ew(static_cast<long>(n) * h1 * w1 * 32), dim3(256), 0,
+                           st, c.g_p1, 32, c.g_cat1 + 64, 96, c.g_e1, 32, n, h2, w2, 32);
+        layer_back(st, c, 3, c.g_e1, 32, c.cat1 + 64, 96, c.t1, 32, c.g_t1, 32, n, h1, w1);
+        layer_back(st, c, 2, c.g_t1, 32, c.t1, 32, c.p0, 16, c.g_p0, 16, n, h1, w1);
+        hipLaunchKernelGGL(k_pool_back_add, ew(static_cast<long>(n) * h * w * 16), dim3(256), 0,
+                           st, c.g_p0, 16, c.g_cat0 + 32, 48, c.g_e0, 16, n, h1, w1, 16);
+        layer_back(st, c, 1, c.g_e0, 16, c.cat0 + 32, 48, c.t0, 16, c.g_t0, 16, n, h, w);
+        layer_back(st, c, 0, c.g_t0, 16, c.t0, 16, c.x_in, IN_PAD, nullptr, 0, n, h, w);
+    }
+
+    float loss_of(Ctx& c) {
+        const long P0 = static_cast<long>(c.n) * c.h * c.w;
+        float s = 0.0f;
+        SG_HIP(hipMemcpyAsync(&s, c.lsum, sizeof(float), hipMemcpyDeviceToHost, stream));
+        SG_HIP(hipStreamSynchronize(stream));
+        return s / static_cast<float>(P0);
+    }
+
+    void upload_x(Ctx& c, const float* x) {
+        const size_t n = static_cast<size_t>(c.n) * c.h * c.w * IN_PAD;
+        SG_HIP(hipMemcpyAsync(c.xf, x, sizeof(float) * n, hipMemcpyHostToDevice, stream));
+        hipLaunchKernelGGL(k_to_bf, dim3(g1(static_cast<long>(n))), dim3(256), 0, stream, c.xf,
+                           c.x_in, static_cast<int>(n));
+        SG_HIP(hipGetLastError());
+    }
+};
+
+// ---- warm start inside the p' solve ----
+
+namespace {
+
+struct FeatArgs {
+    const double* b;    // level-0 b (interior cell (0,0) at b[0], stride ld)
+    const double* ap;   // level-0 APP, same layout
+    int ld, nx, ny, bs; // fine interior size, block side
+    float* cb;          // coarse block sums of b [nxc*nyc]
+    float* ca;          // coarse block means of a_P
+};
+
+// Coarse block sums / means (one thread per coarse cell, rows then columns of the block).
+__global__ void k_feat_blocks(FeatArgs a) {
+    const int nyc = a.ny / a.bs, nxc = a.nx / a.bs;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nxc * nyc)
+        return;
+    const int I = e / nyc, J = e - I * nyc;
+    double sb = 0.0, sa = 0.0;
+    for (int i = 0; i < a.bs; ++i)
+        for (int j = 0; j < a.bs; ++j) {
+            const size_t q = static_cast<size_t>(I * a.bs + i) * a.ld + J * a.bs + j;
+            sb += a.b[q];
+            sa += a.ap[q];
+        }
+    a.cb[e] = static_cast<float>(sb);
+    a.ca[e] = static_cast<float>(sa / (a.bs * a.bs));
+}
+
+// Deterministic sums of cb^2 and ca over n cells: one workgroup, fixed-order tree.
+__global__ void __launch_bounds__(1024) k_feat_stats(const float* cb, const float* ca, int n,
+                                                     double* out) {
+    __shared__ double s1[1024], s2[1024];
+    double a = 0.0, b = 0.0;
+    for (int e = threadIdx.x; e < n; e += 1024) {
+        a += static_cast<double>(cb[e]) * cb[e];
+        b += ca[e];
+    }
+    s1[threadIdx.x] = a;
+    s2[threadIdx.x] = b;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            s1[threadIdx.x] += s1[threadIdx.x + w];
+            s2[threadIdx.x] += s2[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double sb = sqrt(s1[0] / n), ma = s2[0] / n;
+        out[0] = sb > 0.0 ? sb : 1.0;
+        out[1] = ma > 0.0 ? ma : 1.0;
+        out[2] = out[0] / out[1]; // scale of the prediction
+    }
+}
+
+// Network input (bf16 NHWC, 8 channels) from the coarse fields and the statistics.
+__global__ void k_feat_input(const float* cb, const float* ca, int n, const double* st, float lre,
+                             u16* x) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n)
+        return;
+    const float v[IN_PAD] = {static_cast<float>(cb[e] / st[0]),
+                             static_cast<float>(ca[e] / st[1] - 1.0), lre, 0, 0, 0, 0, 0};
+    for (int c = 0; c < IN_PAD; ++c)
+        x[static_cast<size_t>(e) * IN_PAD + c] = f2bf(v[c]);
+}
+
+// Same input as FP32 (for the host: training data).
+__global__ void k_feat_input_f32(const float* cb, const float* ca, int n, const double* st,
+                                 float lre, float* x) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n)
+        return;
+    const float v[IN_PAD] = {static_cast<float>(cb[e] / st[0]),
+                             static_cast<float>(ca[e] / st[1] - 1.0), lre, 0, 0, 0, 0, 0};
+    for (int c = 0; c < IN_PAD; ++c)
+        x[static_cast<size_t>(e) * IN_PAD + c] = v[c];
+}
+
+struct GuessArgs {
+    const float* y;     // prediction on the coarse grid
+    const double* st;   // statistics (st[2] = scale)
+    double* pc;         // level-0 p' (interior cell (0,0) at pc[0], stride ld)
+    const double *AE, *AW, *AN, *AS, *AP, *b;
+    int ld, nx, ny, bs;
+    double* part;       // per-row partials: [nx][2] (defect^2, b^2)
+    int* flag;          // 1: keep the guess
+    Ctl* ctl;
+};
+
+// p' = prediction * scale on every fine cell of its block (halo stays zero).
+__device__ __forceinline__ void k_guess_write(const GuessArgs& a, const Blk& bk) {
+    const int j = bk.x * 64 + threadIdx.x, i = bk.y * 4 + threadIdx.y;
+    if (i >= a.nx || j >= a.ny)
+        return;
+    const int nyc = a.ny / a.bs;
+    a.pc[static_cast<size_t>(i) * a.ld + j] =
+        static_cast<double>(a.y[(i / a.bs) * nyc + j / a.bs]) * a.st[2];
+}
+
+// Row partial sums of (b - A p')^2 and b^2, each row in order (one thread per row).
+__device__ __forceinline__ void k_guess_rows(const GuessArgs& a, const Blk& bk) {
+    const int i = bk.x * 256 + threadIdx.x;
+    if (i >= a.nx)
+        return;
+    double sd = 0.0, sb = 0.0;
+    const long s = a.ld;
+    for (int j = 0; j < a.ny; ++j) {
+        const size_t q = static_cast<size_t>(i) * s + j;
+        const double* x = a.pc;
+        const double r = a.b[q] - (a.AP[q] * x[q] - (a.AE[q] * x[q + 1] + a.AW[q] * x[q - 1] +
+                                                     a.AS[q] * x[q + s] + a.AN[q] * x[q - s]));
+        sd += r * r;
+        sb += a.b[q] * a.b[q];
+    }
+    a.part[2 * i] = sd;
+    a.part[2 * i + 1] = sb;
+}
+
+// Accept the guess if its defect is below ||b|| (rows added in order, one thread).
+__device__ __forceinline__ void k_guess_decide(const GuessArgs& a, const Blk&) {
+    if (threadIdx.x != 0)
+        return;
+    double sd = 0.0, sb = 0.0;
+    for (int i = 0; i < a.nx; ++i) {
+        sd += a.part[2 * i];
+        sb += a.part[2 * i + 1];
+    }
+    const int keep = (sd < sb) ? 1 : 0;
+    *a.flag = keep;
+    if (keep)
+        ++a.ctl->warm_ok;
+    else
+        ++a.ctl->warm_rejected;
+}
+
+// A rejected guess is cleared (p' = 0 over the interior).
+__device__ __forceinline__ void k_guess_clear(const GuessArgs& a, const Blk& bk) {
+    if (*a.flag)
+        return;
+    const int j = bk.x * 64 + threadIdx.x, i = bk.y * 4 + threadIdx.y;
+    if (i < a.nx && j < a.ny)
+        a.pc[static_cast<size_t>(i) * a.ld + j] = 0.0;
+}
+
+} // namespace
+
+// Per-solver state of the warm start.
+struct Warm {
+    UNet* net = nullptr;
+    int level = 2, bs = 4, nxc = 0, nyc = 0;
+    float *cb = nullptr, *ca = nullptr;
+    double* st = nullptr;
+    double* part = nullptr;
+    int* flag = nullptr;
+    float* xf = nullptr;
+    std::vector<void*> owned;
+    ~Warm() {
+        for (void* q : owned)
+            (void)hipFree(q);
+    }
+};
+
+void warm_free(Solver& s) {
+    delete s.warm;
+    s.warm = nullptr;
+}
+
+static void feat_launch(Solver& s, hipStream_t st) {
+    Warm& w = *s.warm;
+    const MgLevel& L0 = s.mg[0];
+    FeatArgs f{L0.b, L0.AP, s.ld, s.p.NX, s.p.NY, w.bs, w.cb, w.ca};
+    const int nc = w.nxc * w.nyc;
+    hipLaunchKernelGGL(k_feat_blocks, dim3(g1(nc)), dim3(256), 0, st, f);
+    SG_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_feat_stats, dim3(1), dim3(1024), 0, st, w.cb, w.ca, nc, w.st);
+    SG_HIP(hipGetLastError());
+}
+
+void warm_attach(Solver& s, UNet* net, int level) {
+    if (s.p.pressure_solver != SG_P_MG)
+        throw ArgError("the learned warm start needs pressure_solver = mg");
+    if (level < 0 || level > 8)
+        throw ArgError("warm-start level must be in 0..8");
+    const int bs = 1 << level;
+    if (s.p.NX % (8 * bs) || s.p.NY % (8 * bs))
+        throw ArgError("warm start: N_cells_x and N_cells_y must be multiples of 8 * 2^level");
+    warm_free(s);
+    s.warm = new Warm;
+    Warm& w = *s.warm;
+    w.net = net;
+    w.level = level;
+    w.bs = bs;
+    w.nxc = s.p.NX / bs;
+    w.nyc = s.p.NY / bs;
+    const size_t nc = static_cast<size_t>(w.nxc) * w.nyc;
+    auto al = [&](size_t bytes) {
+        void* q = nullptr;
+        SG_HIP(hipMalloc(&q, std::max<size_t>(bytes, 16)));
+        SG_HIP(hipMemset(q, 0, std::max<size_t>(bytes, 16)));
+        w.owned.push_back(q);
+        return q;
+    };
+    w.cb = static_cast<float*>(al(nc * sizeof(float)));
+    w.ca = static_cast<float*>(al(nc * sizeof(float)));
+    w.st = static_cast<double*>(al(4 * sizeof(double)));
+    w.part = static_cast<double*>(al(2 * static_cast<size_t>(s.p.NX) * sizeof(double)));
+    w.flag = static_cast<int*>(al(sizeof(int)));
+    w.xf = static_cast<float*>(al(nc * IN_PAD * sizeof(float)));
+    if (net)
+        net->get_ctx(1, w.nxc, w.nyc); // workspace exists before any capture
+    if (s.graph) {
+        SG_HIP(hipGraphExecDestroy(s.graph));
+        s.graph = nullptr;
+    }
+}
+
+// The guess: features, network, p' = prediction * scale, acceptance test (enqueued on s.stream;
+// every kernel is stream-ordered, so the sequence can be captured).
+void launch_warm_guess(Solver& s, const int* stop) {
+    Warm& w = *s.warm;
+    if (!w.net)
+        throw ArgError("pressure_warm_start = learned but no network is attached");
+    if (s.rec)
+        throw ArgError("the learned warm start does not run in a batch");
+    UNet& net = *w.net;
+    UNet::Ctx& c = net.get_ctx(1, w.nxc, w.nyc);
+    const int nc = w.nxc * w.nyc;
+    feat_launch(s, s.stream);
+    const float lre = static_cast<float>(std::log10(s.p.Re / 1000.0));
+    hipLaunchKernelGGL(k_feat_input, dim3(g1(nc)), dim3(256), 0, s.stream, w.cb, w.ca, nc, w.st,
+                       lre, c.x_in);
+    SG_HIP(hipGetLastError());
+    net.forward(s.stream, c);
+    const MgLevel& L0 = s.mg[0];
+    GuessArgs g{c.y,  w.st, L0.x,    L0.AE,   L0.AW, L0.AN, L0.AS, L0.AP, L0.b,
+                s.ld, s.p.NX, s.p.NY, w.bs, w.part, w.flag, s.ctl};
+    const dim3 grid(cdiv(s.p.NY, 64), cdiv(s.p.NX, 4));
+    emit<GuessArgs, k_guess_write, 256>(s, grid, dim3(64, 4), 0, g, stop);
+    emit<GuessArgs, k_guess_rows, 256>(s, dim3(cdiv(s.p.NX, 256)), dim3(256), 0, g, stop);
+    emit<GuessArgs, k_guess_decide, 64>(s, dim3(1), dim3(64), 0, g, stop);
+    emit<GuessArgs, k_guess_clear, 256>(s, grid, dim3(64, 4), 0, g, stop);
+}
+
+} // namespace sg
+
+using namespace sg;
+
+namespace {
+template <class F> int uguard(F f) {
+    try {
+        f();
+        return SG_OK;
+    } catch (const HipError& e) {
+        set_error(e.what());
+        return SG_E_HIP;
+    } catch (const ArgError& e) {
+        set_error(e.what());
+        return SG_E_ARG;
+    } catch (const std::bad_alloc&) {
+        set_error("out of host or device memory");
+        return SG_E_NOMEM;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return SG_E_STATE;
+    }
+}
+UNet& N(sg_unet* h) {
+    if (!h)
+        throw ArgError("null network");
+    return *reinterpret_cast<UNet*>(h);
+}
+} // namespace
+
+extern "C" {
+
+int sg_unet_nparams(int* n) {
+    return uguard([&] {
+        if (!n)
+            throw ArgError("null pointer");
+        UNet tmp;
+        *n = tmp.nparams;
+    });
+}
+
+int sg_unet_create(sg_unet** out) {
+    return uguard([&] {
+        if (!out)
+            throw ArgError("null pointer");
+        *out = reinterpret_cast<sg_unet*>(new UNet);
+    });
+}
+
+int sg_unet_destroy(sg_unet* h) {
+    delete reinterpret_cast<UNet*>(h);
+    return SG_OK;
+}
+
+int sg_unet_set_params(sg_unet* h, const float* theta, int n) {
+    return uguard([&] {
+        UNet& u = N(h);
+        if (!theta || n != u.nparams)
+            throw ArgError("parameter vector of the wrong size");
+        SG_HIP(hipMemcpyAsync(u.master, theta, sizeof(float) * n, hipMemcpyHostToDevice, u.stream));
+        SG_HIP(hipMemsetAsync(u.m1, 0, sizeof(float) * n, u.stream));
+        SG_HIP(hipMemsetAsync(u.m2, 0, sizeof(float) * n, u.stream));
+        u.step = 0;
+        u.prep_weights(u.stream);
+        SG_HIP(hipStreamSynchronize(u.stream));
+    });
+}
+
+int sg_unet_get_params(sg_unet* h, float* theta, int n) {
+    return uguard([&] {
+        UNet& u = N(h);
+        if (!theta || n != u.nparams)
+            throw ArgError("parameter vector of the wrong size");
+        SG_HIP(hipMemcpyAsync(theta, u.master, sizeof(float) * n, hipMemcpyDeviceToHost, u.stream));
+        SG_HIP(hipStreamSynchronize(u.stream));
+    });
+}
+
+int sg_unet_forward(sg_unet* h, int nimg, int ht, int wd, const float* x, float* y) {
+    return uguard([&] {
+        UNet& u = N(h);
+        if (!x || !y)
+            throw ArgError("null pointer");
+        UNet::Ctx& c = u.get_ctx(nimg, ht, wd);
+        u.upload_x(c, x);
+        u.forward(u.stream, c);
+        SG_HIP(hipMemcpyAsync(y, c.y, sizeof(float) * nimg * ht * wd, hipMemcpyDeviceToHost,
+                              u.stream));
+        SG_HIP(hipStreamSynchronize(u.stream));
+    });
+}
+
+int sg_unet_loss_grad(sg_unet* h, int nimg, int ht, int wd, const float* x, const float* t,
+                      float* loss, float* grad) {
+    return uguard([&] {
+        UNet& u = N(h);
+        if (!x || !t)
+            throw ArgError("null pointer");
+        UNet::Ctx& c = u.get_ctx(nimg, ht, wd);
+        u.upload_x(c, x);
+        SG_HIP(hipMemcpyAsync(c.t, t, sizeof(float) * nimg * ht * wd, hipMemcpyHostToDevice,
+                              u.stream));
+        u.forward(u.stream, c);
+        u.backward(u.stream, c);
+        if (loss)
+            *loss = u.loss_of(c);
+        if (grad) {
+            SG_HIP(hipMemcpyAsync(grad, u.grad, sizeof(float) * u.nparams, hipMemcpyDeviceToHost,
+                                  u.stream));
+            SG_HIP(hipStreamSynchronize(u.stream));
+        }
+    });
+}
+
+int sg_unet_train_step(sg_unet* h, int nimg, int ht, int wd, const float* x, const float* t,
+                       float lr, float* loss) {
+    return uguard([&] {
+        UNet& u = N(h);
+        if (!x || !t || !(lr > 0.0f))
+            throw ArgError("null pointer or non-positive learning rate");
+        UNet::Ctx& c = u.get_ctx(nimg, ht, wd);
+        u.upload_x(c, x);
+        SG_HIP(hipMemcpyAsync(c.t, t, sizeof(float) * nimg * ht * wd, hipMemcpyHostToDevice,
+                              u.stream));
+        u.forward(u.stream, c);
+        u.backward(u.stream, c);
+        ++u.step;
+        const float b1 = 0.9f, b2 = 0.999f;
+        const float c1 = 1.0f - std::pow(b1, static_cast<float>(u.step));
+        const float c2 = 1.0f - std::pow(b2, static_cast<float>(u.step));
+        hipLaunchKernelGGL(k_adam, dim3(g1(u.nparams)), dim3(256), 0, u.stream, u.master, u.grad,
+                           u.m1, u.m2, u.nparams, lr, b1, b2, 1e-8f, c1, c2);
+        SG_HIP(hipGetLastError());
+        u.prep_weights(u.stream);
+        if (loss)
+            *loss = u.loss_of(c);
+        else
+            SG_HIP(hipStreamSynchronize(u.stream));
+    });
+}
+
+int sg_unet_mfma_probe(sg_unet* h, int nimg, int ht, int wd, int reps, double* seconds,
+                       double* flops) {
+    return uguard([&] {
+        UNet& u = N(h);
+        UNet::Ctx& c = u.get_ctx(nimg, ht, wd);
+        hipEvent_t e0, e1;
+        SG_HIP(hipEventCreate(&e0));
+        SG_HIP(hipEventCreate(&e1));
+        u.forward(u.stream, c);
+        SG_HIP(hipEventRecord(e0, u.stream));
+        for (int r = 0; r < std::max(reps, 1); ++r)
+            u.forward(u.stream, c);
+        SG_HIP(hipEventRecord(e1, u.stream));
+        SG_HIP(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        SG_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (seconds)
+            *seconds = 1e-3 * ms / std::max(reps, 1);
+        if (flops) {
+            double f = 0.0;
+            const double P0 = static_cast<double>(nimg) * ht * wd;
+            const double lvl[NCONV] = {1, 1, 4, 4, 16, 16, 64, 64, 16, 16, 4, 4, 1, 1};
+            for (int k = 0; k < NCONV; ++k)
+                f += 2.0 * 9 * SPEC[k].ci_real * SPEC[k].cout * P0 / lvl[k];
+            *flops = f;
+        }
+    });
+}
+
+int sg_attach_warm_start(sg_solver* sh, sg_unet* net, int level) {
+    return uguard([&] {
+        if (!sh)
+            throw ArgError("null solver");
+        Solver& s = *reinterpret_cast<Solver*>(sh);
+        warm_attach(s, reinterpret_cast<UNet*>(net), level);
+    });
+}
+
+int sg_warm_features(sg_solver* sh, int level, float* x, double* scale) {
+    return uguard([&] {
+        if (!sh || !x)
+            throw ArgError("null pointer");
+        Solver& s = *reinterpret_cast<Solver*>(sh);
+        if (!s.warm || s.warm->level != level) {
+            UNet* keep = s.warm ? s.warm->net : nullptr;
+            warm_attach(s, keep, level);
+        }
+        Warm& w = *s.warm;
+        const int nc = w.nxc * w.nyc;
+        feat_launch(s, s.stream);
+        const float lre = static_cast<float>(std::log10(s.p.Re / 1000.0));
+        hipLaunchKernelGGL(k_feat_input_f32, dim3(g1(nc)), dim3(256), 0, s.stream, w.cb, w.ca, nc,
+                           w.st, lre, w.xf);
+        SG_HIP(hipGetLastError());
+        double st[3];
+        SG_HIP(hipMemcpyAsync(x, w.xf, sizeof(float) * nc * IN_PAD, hipMemcpyDeviceToHost,
+                              s.stream));
+        SG_HIP(hipMemcpyAsync(st, w.st, sizeof st, hipMemcpyDeviceToHost, s.stream));
+        SG_HIP(hipStreamSynchronize(s.stream));
+        if (scale)
+            *scale = st[2];
+    });
+}
+
+int sg_warm_stats(sg_solver* sh, int* accepted, int* rejected) {
+    return uguard([&] {
+        if (!sh)
+            throw ArgError("null solver");
+        Solver& s = *reinterpret_cast<Solver*>(sh);
+        Ctl c;
+        SG_HIP(hipMemcpyAsync(&c, s.ctl, sizeof c, hipMemcpyDeviceToHost, s.stream));
+        SG_HIP(hipStreamSynchronize(s.stream));
+        if (accepted)
+            *accepted = c.warm_ok;
+        if (rejected)
+            *rejected = c.warm_rejected;
+    });
+}
+
+} // extern "C"

@@ -147,6 +147,19 @@ def gpu():
         g.sg_batch_set_tuning.argtypes = [vp, C.c_int, C.c_long]
         g.sg_batch_get_tuning.argtypes = [vp, C.c_int, C.POINTER(C.c_long)]
         g.sg_batch_autotune.argtypes = [vp, C.c_int, C.c_char_p, C.c_size_t]
+        fp = C.POINTER(C.c_float)
+        g.sg_unet_nparams.argtypes = [ip]
+        g.sg_unet_create.argtypes = [C.POINTER(vp)]
+        g.sg_unet_destroy.argtypes = [vp]
+        g.sg_unet_set_params.argtypes = [vp, fp, C.c_int]
+        g.sg_unet_get_params.argtypes = [vp, fp, C.c_int]
+        g.sg_unet_forward.argtypes = [vp, C.c_int, C.c_int, C.c_int, fp, fp]
+        g.sg_unet_loss_grad.argtypes = [vp, C.c_int, C.c_int, C.c_int, fp, fp, fp, fp]
+        g.sg_unet_train_step.argtypes = [vp, C.c_int, C.c_int, C.c_int, fp, fp, C.c_float, fp]
+        g.sg_unet_mfma_probe.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp]
+        g.sg_attach_warm_start.argtypes = [vp, vp, C.c_int]
+        g.sg_warm_features.argtypes = [vp, C.c_int, fp, dp]
+        g.sg_warm_stats.argtypes = [vp, ip, ip]
         _gpu = g
     return _gpu
 
@@ -271,6 +284,100 @@ class Solver:
         n = C.c_int()
         check(gpu().sg_mg_levels(self.h, C.byref(n)))
         return n.value
+
+
+def _fptr(a):
+    import numpy as np
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class UNet:
+    """Owner of an sg_unet handle (the learned warm start's network)."""
+
+    def __init__(self, theta=None):
+        h = C.c_void_p()
+        check(gpu().sg_unet_create(C.byref(h)))
+        self.h = h
+        n = C.c_int()
+        check(gpu().sg_unet_nparams(C.byref(n)))
+        self.nparams = n.value
+        if theta is not None:
+            self.set_params(theta)
+
+    def close(self):
+        if self.h:
+            gpu().sg_unet_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, theta):
+        import numpy as np
+        t = np.ascontiguousarray(theta, dtype=np.float32)
+        check(gpu().sg_unet_set_params(self.h, _fptr(t), t.size))
+
+    def get_params(self):
+        import numpy as np
+        t = np.empty(self.nparams, np.float32)
+        check(gpu().sg_unet_get_params(self.h, _fptr(t), t.size))
+        return t
+
+    def forward(self, x):
+        import numpy as np
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        n, h, w, _ = x.shape
+        y = np.empty((n, h, w), np.float32)
+        check(gpu().sg_unet_forward(self.h, n, h, w, _fptr(x), _fptr(y)))
+        return y
+
+    def loss_grad(self, x, t):
+        import numpy as np
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        n, h, w, _ = x.shape
+        loss = C.c_float()
+        g = np.empty(self.nparams, np.float32)
+        check(gpu().sg_unet_loss_grad(self.h, n, h, w, _fptr(x), _fptr(t), C.byref(loss), _fptr(g)))
+        return loss.value, g
+
+    def train_step(self, x, t, lr):
+        import numpy as np
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        n, h, w, _ = x.shape
+        loss = C.c_float()
+        check(gpu().sg_unet_train_step(self.h, n, h, w, _fptr(x), _fptr(t), lr, C.byref(loss)))
+        return loss.value
+
+    def probe(self, n, h, w, reps=20):
+        sec, fl = C.c_double(), C.c_double()
+        check(gpu().sg_unet_mfma_probe(self.h, n, h, w, reps, C.byref(sec), C.byref(fl)))
+        return sec.value, fl.value
+
+
+def attach_warm_start(solver, net, level=2):
+    check(gpu().sg_attach_warm_start(solver.h, net.h if net is not None else None, level))
+
+
+def warm_features(solver, level=2):
+    import numpy as np
+    p = solver.p
+    b = 1 << level
+    x = np.empty((p.NX // b, p.NY // b, 8), np.float32)
+    sc = C.c_double()
+    check(gpu().sg_warm_features(solver.h, level, _fptr(x), C.byref(sc)))
+    return x, sc.value
+
+
+def warm_stats(solver):
+    a, r = C.c_int(), C.c_int()
+    check(gpu().sg_warm_stats(solver.h, C.byref(a), C.byref(r)))
+    return a.value, r.value
 
 
 class _Borrowed(Solver):

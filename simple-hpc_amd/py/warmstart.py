@@ -1,0 +1,139 @@
+"""Training data, training and weights files of the learned p' warm start.
+
+Data: batched cavity runs (one persistent launch per phase, simple_gpu.Batch) are advanced through
+their convergence history; at chosen outer iterations every cavity contributes one pair:
+  input  = the network features of its current p' system on multigrid level L (sg_warm_features:
+           block sums of b, block means of a_P, log10(Re/1000), normalised), and
+  target = the block means of the converged p' of the same system, divided by the same scale.
+The converged p' comes from a teacher solver of the same size (FP64 multigrid, 30 V-cycles) into
+which the system's arrays (AE, AW, AN, AS, AP of p' and b) are uploaded.
+
+Training: Adam on mean squared error, minibatches of equal-sized coarse grids, run on the GPU by
+the library's own bfloat16 MFMA forward and backward kernels (sg_unet_train_step).
+
+Weights file: the ASCII line "SGUNET1 <n>\\n" then n little-endian float32 parameters.
+"""
+import math
+import os
+import struct
+import time
+
+import numpy as np
+
+import simple_gpu as sg
+
+P_SYSTEM = ("AEP", "AWP", "ANP", "ASP", "APP", "b")
+
+
+def teacher_params(n, re):
+    text = (f"N_cells_x = {n}\nN_cells_y = {n}\nRe = {re!r}\nconvection_scheme = QUICK\n"
+            f"diffusion_order = 2\nmomentum_solver = gs_multicolour\npressure_solver = mg\n"
+            f"mg_cycles = 30\nbackend = gpu\nwrite_fields = False\n")
+    return sg.params_from_config(text)[0]
+
+
+def block_mean(a, bs):
+    nx, ny = a.shape
+    return a.reshape(nx // bs, bs, ny // bs, bs).mean(axis=(1, 3))
+
+
+def training_cavities(n=32, seed=1, sizes=(128, 192, 256, 384, 512), re_lo=100.0, re_hi=3200.0):
+    """(N, Re) pairs for training data: N cycling over sizes, Re log-uniform, numpy PCG64(seed)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        u = rng.random()
+        out.append((sizes[k % len(sizes)],
+                    10 ** (math.log10(re_lo) + u * (math.log10(re_hi) - math.log10(re_lo)))))
+    return out
+
+
+def generate(cases, snapshots, level=2, precision="mixed", log=print):
+    """Pairs from batched runs of `cases` sampled after each outer iteration count in `snapshots`
+    (ascending). Returns {coarse shape: (x [m,h,w,8], t [m,h,w])}."""
+    plist = [sg.params_from_config(sg.batch_config(n, re, precision=precision, tol=0.0))[0]
+             for n, re in cases]
+    b = sg.Batch(plist)
+    teachers = {}
+    data = {}
+    bs = 1 << level
+    done = 0
+    t0 = time.perf_counter()
+    for snap in snapshots:
+        b.advance(snap - done)
+        done = snap
+        for k, (n, re) in enumerate(cases):
+            c = b.cavity(k)
+            x, scale = sg.warm_features(c, level)
+            if n not in teachers:
+                teachers[n] = sg.Solver(teacher_params(n, re))
+            tch = teachers[n]
+            for name in P_SYSTEM:
+                tch.set(name, c.get(name))
+            tch.solve_p()
+            H = c.p.H
+            pc = tch.get("pc")[H:H + c.p.NX, H + 1:H + 1 + c.p.NY]
+            t = (block_mean(pc, bs) / scale).astype(np.float32)
+            key = x.shape[:2]
+            xs, ts = data.setdefault(key, ([], []))
+            xs.append(x)
+            ts.append(t)
+        log(f"  data: iteration {snap}, {sum(len(v[0]) for v in data.values())} pairs, "
+            f"{time.perf_counter() - t0:.1f} s")
+    b.close()
+    for t in teachers.values():
+        t.close()
+    return {k: (np.stack(v[0]), np.stack(v[1])) for k, v in data.items()}
+
+
+def train(net, data, steps=2000, batch=8, lr=1e-3, seed=0, log=print, every=200):
+    """Adam steps over minibatches drawn from the size groups in proportion to their pairs."""
+    rng = np.random.default_rng(seed)
+    keys = sorted(data)
+    weights = np.array([len(data[k][0]) for k in keys], np.float64)
+    weights /= weights.sum()
+    hist = []
+    t0 = time.perf_counter()
+    for s in range(steps):
+        k = keys[rng.choice(len(keys), p=weights)]
+        x, t = data[k]
+        idx = rng.choice(len(x), size=min(batch, len(x)), replace=False)
+        # cosine decay of the step size
+        a = lr * 0.5 * (1 + math.cos(math.pi * s / steps))
+        loss = net.train_step(x[idx], t[idx], max(a, lr * 0.02))
+        hist.append(loss)
+        if log and (s % every == 0 or s == steps - 1):
+            log(f"  train step {s}: loss {np.mean(hist[-every:]):.4f} "
+                f"({time.perf_counter() - t0:.1f} s)")
+    return hist
+
+
+def evaluate(net, data):
+    """Mean squared error of the network and of the zero prediction on data."""
+    num, den, cnt = 0.0, 0.0, 0
+    for k, (x, t) in data.items():
+        for i in range(0, len(x), 16):
+            y = net.forward(x[i:i + 16])
+            num += float(((y - t[i:i + 16]) ** 2).sum())
+            den += float((t[i:i + 16] ** 2).sum())
+            cnt += t[i:i + 16].size
+    return num / cnt, den / cnt
+
+
+def save(path, theta):
+    theta = np.ascontiguousarray(theta, dtype="<f4")
+    with open(path, "wb") as f:
+        f.write(f"SGUNET1 {theta.size}\n".encode())
+        f.write(theta.tobytes())
+
+
+def load(path):
+    with open(path, "rb") as f:
+        head = f.readline().decode().split()
+        if len(head) != 2 or head[0] != "SGUNET1":
+            raise ValueError(f"{path}: not a warm-start weights file")
+        n = int(head[1])
+        theta = np.frombuffer(f.read(4 * n), dtype="<f4")
+    if theta.size != n:
+        raise ValueError(f"{path}: truncated")
+    return theta.astype(np.float32)

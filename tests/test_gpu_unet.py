@@ -1,0 +1,139 @@
+"""The warm-start U-Net's bfloat16 MFMA kernels against the CPU oracle (oracle/unet_ref.py, whose
+own gradients are pinned by finite differences), on the same inputs and weights.
+
+The oracle in bf16 mode rounds where the GPU stores bfloat16, so the two differ only by float32
+summation order (and the rare bfloat16 rounding flip that order causes): outputs are compared to
+a few bfloat16 ulps, gradients by relative L2 error and cosine per parameter block. Also: training
+lowers the loss; an all-zero network's guess is discarded and the run is bit-identical to the
+plain mixed-precision run (the fallback is exact).
+"""
+import numpy as np
+import pytest
+
+from conftest import bits_equal
+
+pytestmark = pytest.mark.gpu
+
+import simple_gpu as sg  # noqa: E402
+import unet_ref as U  # noqa: E402
+
+
+def _inputs(seed, n, h, w):
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal((n, h, w, U.IN_PAD)).astype(np.float32)
+    x[..., U.IN_CH:] = 0
+    t = rng.standard_normal((n, h, w)).astype(np.float32)
+    return x, t
+
+
+def _theta(seed):
+    rng = np.random.default_rng(seed + 100)
+    th = U.init_params(seed)
+    return (th + rng.standard_normal(th.size).astype(np.float32) * 0.01).astype(np.float32)
+
+
+def test_param_count_matches_oracle():
+    net = sg.UNet()
+    assert net.nparams == U.n_params()
+    th = _theta(0)
+    net.set_params(th)
+    assert np.array_equal(net.get_params(), th)
+    net.close()
+
+
+@pytest.mark.parametrize("n,h,w", [(1, 8, 8), (2, 16, 24), (1, 32, 32)])
+def test_forward_matches_oracle(n, h, w):
+    x, _ = _inputs(1, n, h, w)
+    th = _theta(1)
+    net = sg.UNet(th)
+    y = net.forward(x)
+    ref = U.forward(th, x, bf=True)
+    err = np.abs(y - ref).max() / np.abs(ref).max()
+    print(f"forward {n}x{h}x{w}: max rel err {err:.2e}", flush=True)
+    assert err < 3e-2
+    assert np.corrcoef(y.ravel(), ref.ravel())[0, 1] > 0.9995
+    net.close()
+
+
+@pytest.mark.parametrize("n,h,w", [(2, 8, 8), (1, 16, 16)])
+def test_gradients_match_oracle(n, h, w):
+    x, t = _inputs(2, n, h, w)
+    th = _theta(2)
+    net = sg.UNet(th)
+    loss, g = net.loss_grad(x, t)
+    lref, gref = U.loss_and_grad(th, x, t, bf=True)
+    print(f"loss gpu {loss:.6f} cpu {lref:.6f}", flush=True)
+    assert abs(loss - lref) <= 2e-2 * lref
+    o = 0
+    for k, (ci, co) in enumerate(U.conv_specs()):
+        ci_real = U.IN_CH if k == 0 else ci
+        for name, sz in (("W", co * ci_real * 9), ("b", co)):
+            a, b = g[o:o + sz].astype(np.float64), gref[o:o + sz].astype(np.float64)
+            rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+            cos = np.dot(a, b) / max(np.linalg.norm(a) * np.linalg.norm(b), 1e-30)
+            print(f"conv {k:2d} {name}: rel L2 {rel:.2e} cos {cos:.6f}", flush=True)
+            assert rel < 6e-2 and cos > 0.998, (k, name, rel, cos)
+            o += sz
+    a, b = g[o:], gref[o:]
+    assert np.linalg.norm(a - b) / np.linalg.norm(b) < 3e-2
+    net.close()
+
+
+def test_training_reduces_loss():
+    # A learnable target: a smooth function of the inputs (channel 0 blurred).
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((8, 32, 32, U.IN_PAD)).astype(np.float32)
+    x[..., U.IN_CH:] = 0
+    k = np.ones((5, 5)) / 25.0
+    t = np.zeros((8, 32, 32), np.float32)
+    for i in range(8):
+        xp = np.pad(x[i, :, :, 0], 2)
+        for a in range(5):
+            for b in range(5):
+                t[i] += k[a, b] * xp[a:a + 32, b:b + 32]
+    net = sg.UNet(U.init_params(4))
+    first = None
+    for step in range(200):
+        loss = net.train_step(x, t, 2e-3)
+        first = loss if first is None else first
+    final, _ = net.loss_grad(x, t)
+    print(f"loss {first:.4f} -> {final:.4f}", flush=True)
+    assert final < 0.2 * first
+    net.close()
+
+
+def _mixed_params(n, re, learned):
+    text = (f"N_cells_x = {n}\nN_cells_y = {n}\nRe = {re}\nconvection_scheme = QUICK\n"
+            f"diffusion_order = 2\nmomentum_solver = gs_multicolour\npressure_solver = mg\n"
+            f"mg_precision = mixed\nerror_tolerance = 0\nmaxit = 100000\nbackend = gpu\n"
+            f"write_fields = False\n")
+    if learned:
+        text += "pressure_warm_start = learned\n"
+    return sg.params_from_config(text)[0]
+
+
+def test_rejected_guess_is_exactly_the_plain_solve():
+    net = sg.UNet(np.zeros(U.n_params(), np.float32))  # predicts 0: never lowers the defect
+    plain = sg.Solver(_mixed_params(64, 400.0, False))
+    warm = sg.Solver(_mixed_params(64, 400.0, True))
+    sg.attach_warm_start(warm, net, level=2)
+    plain.advance(25)
+    warm.advance(25)
+    for f in ("u", "v", "p"):
+        assert bits_equal(warm.get(f), plain.get(f)), f
+    acc, rej = sg.warm_stats(warm)
+    assert acc == 0 and rej == 25, (acc, rej)
+    plain.close()
+    warm.close()
+    net.close()
+
+
+def test_features_shape_and_scale():
+    s = sg.Solver(_mixed_params(64, 400.0, False))
+    s.advance(5)
+    x, scale = sg.warm_features(s, level=2)
+    assert x.shape == (16, 16, 8) and scale > 0
+    assert abs(np.sqrt(np.mean(x[..., 0] ** 2)) - 1.0) < 1e-3  # b channel normalised to rms 1
+    assert abs(np.mean(x[..., 1])) < 1e-3                       # a_P channel: mean 0
+    assert np.allclose(x[..., 2], np.log10(0.4)) and np.all(x[..., 3:] == 0)
+    s.close()
