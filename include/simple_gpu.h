@@ -248,8 +248,9 @@ int sg_unet_mfma_probe(sg_unet* net, int nimg, int h, int w, int reps, double* s
                        double* flops);
 /* Use net (not owned) ahead of every p' solve of s when s's params say pressure_warm_start =
  * learned (pressure_solver = mg, mg_precision = mixed), on multigrid level `level` (2^level
- * blocks; N_cells_x and N_cells_y multiples of 8 * 2^level). A guess whose FP64 defect is not
- * below ||b|| is discarded (p' starts from zero). Do not train net while s runs. */
+ * blocks; N_cells_x and N_cells_y multiples of 8 * 2^level). The guess g enters scaled by
+ * b.g / g.A.g (the energy minimiser along g: never farther from the solution than zero in the
+ * A-norm); a degenerate guess is discarded (p' starts from zero). Do not train net while s runs. */
 int sg_attach_warm_start(sg_solver* s, sg_unet* net, int level);
 /* The network input of s's current p' system at `level` (x: (NX/2^level)*(NY/2^level)*8 floats)
  * and the scale that turns a prediction into p'. */

@@ -19,9 +19,10 @@
 // Warm start: the p' system is reduced to a coarse grid (level L: 2^L x 2^L blocks): block sums of
 // b and block means of a_P, normalised by rms(block b) and mean(block a_P) (deterministic
 // fixed-order sums). The network predicts the block means of the converged p' in those units;
-// the guess p'(i, j) = prediction(block) * scale is kept only if its FP64 defect ||b - A p'|| is
-// below ||b|| (else p' starts from zero, the plain solve), and the mixed-precision multigrid solve
-// then starts from it. The stopping rule of the outer loop is untouched: a run ends only when the
+// the guess g (the prediction times the scale, interpolated bilinearly to the fine cells) enters
+// scaled by the energy-minimising factor b.g / g.A.g, which is never farther from the solution
+// than zero in the A-norm (a degenerate guess is discarded: p' starts from zero, the plain solve),
+// and the mixed-precision multigrid solve then starts from it. The stopping rule of the outer loop is untouched: a run ends only when the
 // SIMPLE residuals meet the tolerance.
 #include "sg_internal.hpp"
 
@@ -119,14 +120,20 @@ struct ConvArgs {
     int relu;
 };
 
-constexpr int CV_T = 256; // 4 waves x 16 pixels; 64 output channels per workgroup
+constexpr int CV_T = 256; // 4 waves; a workgroup covers 64 output channels
 
-__global__ void __launch_bounds__(CV_T) k_conv(ConvArgs a) {
+// CPW = column blocks (16 output channels each) per wave. CPW = 4: a wave computes 16 pixels x 64
+// channels and a workgroup 64 pixels (large images: the A fragment is loaded once for 4 MFMAs).
+// CPW = 1: a wave computes 16 pixels x 16 channels and a workgroup 16 pixels x 64 channels (small
+// images: 4x more waves, 4x shorter dependent MFMA chains).
+template <int CPW> __global__ void __launch_bounds__(CV_T) k_conv(ConvArgs a) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int m = lane & 15, oct = lane >> 4;
     const int npix = a.n * a.h * a.wd;
-    const int pix0 = (blockIdx.x * 4 + wave) * 16;
-    const int co0 = blockIdx.y * 64;
+    const int pix0 = CPW == 4 ? (blockIdx.x * 4 + wave) * 16 : blockIdx.x * 16;
+    const int co0 = blockIdx.y * 64 + (CPW == 4 ? 0 : wave * 16);
+    if (co0 >= a.cout)
+        return;
     const int p = pix0 + m;
     const bool pv = p < npix;
     int pn = 0, pi = 0, pj = 0;
@@ -136,9 +143,9 @@ __global__ void __launch_bounds__(CV_T) k_conv(ConvArgs a) {
         pi = r % a.h;
         pn = r / a.h;
     }
-    f4 acc[4];
+    f4 acc[CPW];
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
+    for (int c = 0; c < CPW; ++c)
         acc[c] = f4{0.0f, 0.0f, 0.0f, 0.0f};
     const int K = 9 * a.cin;
     for (int k0 = 0; k0 < a.kpad; k0 += 32) {
@@ -151,14 +158,14 @@ __global__ void __launch_bounds__(CV_T) k_conv(ConvArgs a) {
                 A = ld8(a.x + (static_cast<size_t>(pn * a.h + ii) * a.wd + jj) * a.ldx + ci);
         }
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
+        for (int c = 0; c < CPW; ++c) {
             const int co = co0 + c * 16 + m;
             const bf16x8 B = co < a.cout ? ld8(a.w + static_cast<size_t>(co) * a.kpad + kk) : zero8();
             acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[c], 0, 0, 0);
         }
     }
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
+    for (int c = 0; c < CPW; ++c) {
         const int co = co0 + c * 16 + m;
         if (co >= a.cout)
             continue;
@@ -610,7 +617,13 @@ struct UNet {
             a.relu = 0;
         }
         const long npix = static_cast<long>(n) * h * w;
-        hipLaunchKernelGGL(k_conv, dim3(g1(npix, 64), cdiv(a.cout, 64)), dim3(CV_T), 0, st, a);
+        // 64 pixels x 64 channels per workgroup when that fills the GPU, else 16 x 64.
+        if (static_cast<long>(g1(npix, 64)) * cdiv(a.cout, 64) >= 1024)
+            hipLaunchKernelGGL(k_conv<4>, dim3(g1(npix, 64), cdiv(a.cout, 64)), dim3(CV_T), 0, st,
+                               a);
+        else
+            hipLaunchKernelGGL(k_conv<1>, dim3(g1(npix, 16), cdiv(a.cout, 64)), dim3(CV_T), 0, st,
+                               a);
         SG_HIP(hipGetLastError());
     }
 
@@ -802,7 +815,9 @@ __global__ void __launch_bounds__(1024) k_feat_stats(const float* cb, const floa
         const double sb = sqrt(s1[0] / n), ma = s2[0] / n;
         out[0] = sb > 0.0 ? sb : 1.0;
         out[1] = ma > 0.0 ? ma : 1.0;
-        out[2] = out[0] / out[1]; // scale of the prediction
+        // Scale of the prediction: b / a_P, times the coarse cell count (the inverse operator's
+        // growth with the grid), over 1024.
+        out[2] = out[0] / out[1] * (static_cast<double>(n) / 1024.0);
     }
 }
 
@@ -841,59 +856,85 @@ struct GuessArgs {
     Ctl* ctl;
 };
 
-// p' = prediction * scale on every fine cell of its block (halo stays zero).
+// p' = scale * the prediction interpolated bilinearly from the coarse cell centres (clamped at
+// the walls) to the fine cell centres; the halo stays zero.
 __device__ __forceinline__ void k_guess_write(const GuessArgs& a, const Blk& bk) {
     const int j = bk.x * 64 + threadIdx.x, i = bk.y * 4 + threadIdx.y;
     if (i >= a.nx || j >= a.ny)
         return;
-    const int nyc = a.ny / a.bs;
-    a.pc[static_cast<size_t>(i) * a.ld + j] =
-        static_cast<double>(a.y[(i / a.bs) * nyc + j / a.bs]) * a.st[2];
+    const int nxc = a.nx / a.bs, nyc = a.ny / a.bs;
+    const float fi = (i + 0.5f) / a.bs - 0.5f, fj = (j + 0.5f) / a.bs - 0.5f;
+    const float ci = fminf(fmaxf(fi, 0.0f), static_cast<float>(nxc - 1));
+    const float cj = fminf(fmaxf(fj, 0.0f), static_cast<float>(nyc - 1));
+    const int i0 = min(static_cast<int>(ci), nxc - 1), j0 = min(static_cast<int>(cj), nyc - 1);
+    const int i1 = min(i0 + 1, nxc - 1), j1 = min(j0 + 1, nyc - 1);
+    const float wi = ci - i0, wj = cj - j0;
+    const float* y = a.y;
+    const float v = (1.0f - wi) * ((1.0f - wj) * y[i0 * nyc + j0] + wj * y[i0 * nyc + j1]) +
+                    wi * ((1.0f - wj) * y[i1 * nyc + j0] + wj * y[i1 * nyc + j1]);
+    a.pc[static_cast<size_t>(i) * a.ld + j] = static_cast<double>(v) * a.st[2];
 }
 
-// Row partial sums of (b - A p')^2 and b^2, each row in order (one thread per row).
+// Per row: sums of g (A g) and b g for the guess g (one workgroup per row, fixed-order tree).
 __device__ __forceinline__ void k_guess_rows(const GuessArgs& a, const Blk& bk) {
-    const int i = bk.x * 256 + threadIdx.x;
-    if (i >= a.nx)
-        return;
-    double sd = 0.0, sb = 0.0;
+    __shared__ double sd[256], sb[256];
+    const int i = bk.x;
     const long s = a.ld;
-    for (int j = 0; j < a.ny; ++j) {
+    double d2 = 0.0, b2 = 0.0;
+    for (int j = threadIdx.x; j < a.ny; j += 256) {
         const size_t q = static_cast<size_t>(i) * s + j;
         const double* x = a.pc;
-        const double r = a.b[q] - (a.AP[q] * x[q] - (a.AE[q] * x[q + 1] + a.AW[q] * x[q - 1] +
-                                                     a.AS[q] * x[q + s] + a.AN[q] * x[q - s]));
-        sd += r * r;
-        sb += a.b[q] * a.b[q];
+        const double ag = a.AP[q] * x[q] - (a.AE[q] * x[q + 1] + a.AW[q] * x[q - 1] +
+                                            a.AS[q] * x[q + s] + a.AN[q] * x[q - s]);
+        d2 += x[q] * ag;
+        b2 += a.b[q] * x[q];
     }
-    a.part[2 * i] = sd;
-    a.part[2 * i + 1] = sb;
+    sd[threadIdx.x] = d2;
+    sb[threadIdx.x] = b2;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            sd[threadIdx.x] += sd[threadIdx.x + w];
+            sb[threadIdx.x] += sb[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        a.part[2 * i] = sd[0];
+        a.part[2 * i + 1] = sb[0];
+    }
 }
 
-// Accept the guess if its defect is below ||b|| (rows added in order, one thread).
+// The guess enters scaled by alpha = b.g / g.A.g, the minimiser of the energy
+// E(alpha g) = alpha^2 g.A.g / 2 - alpha b.g, whose minimum -(b.g)^2 / (2 g.A.g) is never above
+// E(0) = 0: since E(x) - E(p') = |x - p'|_A^2 / 2, the scaled guess is never farther from the
+// solution than zero in the A-norm. It is discarded (p' = 0) when g.A.g is not positive or alpha is
+// not finite. Rows added in order, one thread.
 __device__ __forceinline__ void k_guess_decide(const GuessArgs& a, const Blk&) {
     if (threadIdx.x != 0)
         return;
-    double sd = 0.0, sb = 0.0;
+    double gag = 0.0, bg = 0.0;
     for (int i = 0; i < a.nx; ++i) {
-        sd += a.part[2 * i];
-        sb += a.part[2 * i + 1];
+        gag += a.part[2 * i];
+        bg += a.part[2 * i + 1];
     }
-    const int keep = (sd < sb) ? 1 : 0;
+    const double alpha = gag > 0.0 ? bg / gag : 0.0;
+    const int keep = (gag > 0.0 && isfinite(alpha) && alpha != 0.0) ? 1 : 0;
     *a.flag = keep;
+    a.part[2 * a.nx] = keep ? alpha : 0.0;
     if (keep)
         ++a.ctl->warm_ok;
     else
         ++a.ctl->warm_rejected;
 }
 
-// A rejected guess is cleared (p' = 0 over the interior).
-__device__ __forceinline__ void k_guess_clear(const GuessArgs& a, const Blk& bk) {
-    if (*a.flag)
-        return;
+// p' = alpha g (0 if the guess was discarded).
+__device__ __forceinline__ void k_guess_scale(const GuessArgs& a, const Blk& bk) {
     const int j = bk.x * 64 + threadIdx.x, i = bk.y * 4 + threadIdx.y;
-    if (i < a.nx && j < a.ny)
-        a.pc[static_cast<size_t>(i) * a.ld + j] = 0.0;
+    if (i >= a.nx || j >= a.ny)
+        return;
+    double& x = a.pc[static_cast<size_t>(i) * a.ld + j];
+    x = *a.flag ? x * a.part[2 * a.nx] : 0.0; // a discarded guess may hold non-finite values
 }
 
 } // namespace
@@ -957,7 +998,7 @@ void warm_attach(Solver& s, UNet* net, int level) {
     w.cb = static_cast<float*>(al(nc * sizeof(float)));
     w.ca = static_cast<float*>(al(nc * sizeof(float)));
     w.st = static_cast<double*>(al(4 * sizeof(double)));
-    w.part = static_cast<double*>(al(2 * static_cast<size_t>(s.p.NX) * sizeof(double)));
+    w.part = static_cast<double*>(al((2 * static_cast<size_t>(s.p.NX) + 1) * sizeof(double)));
     w.flag = static_cast<int*>(al(sizeof(int)));
     w.xf = static_cast<float*>(al(nc * IN_PAD * sizeof(float)));
     if (net)
@@ -990,9 +1031,9 @@ void launch_warm_guess(Solver& s, const int* stop) {
                 s.ld, s.p.NX, s.p.NY, w.bs, w.part, w.flag, s.ctl};
     const dim3 grid(cdiv(s.p.NY, 64), cdiv(s.p.NX, 4));
     emit<GuessArgs, k_guess_write, 256>(s, grid, dim3(64, 4), 0, g, stop);
-    emit<GuessArgs, k_guess_rows, 256>(s, dim3(cdiv(s.p.NX, 256)), dim3(256), 0, g, stop);
+    emit<GuessArgs, k_guess_rows, 256>(s, dim3(s.p.NX), dim3(256), 0, g, stop);
     emit<GuessArgs, k_guess_decide, 64>(s, dim3(1), dim3(64), 0, g, stop);
-    emit<GuessArgs, k_guess_clear, 256>(s, grid, dim3(64, 4), 0, g, stop);
+    emit<GuessArgs, k_guess_scale, 256>(s, grid, dim3(64, 4), 0, g, stop);
 }
 
 } // namespace sg

@@ -27,6 +27,7 @@ struct Params {
     std::string mpi_grid = "auto";      // accepted for config compatibility; one GPU runs 1x1
     bool halo_overlap = true;
     std::string warm_start_weights;     // pressure_warm_start = learned: network weights file
+    int warm_start_level = 2;           // multigrid level the network runs on (2^level blocks)
 };
 
 // Known keys only, typed parsing, validation and derived sizes (src/params.cpp:50-180), plus this
@@ -51,6 +52,7 @@ struct RunResult {
     std::array<double, kPhases> phase_s{};
     long long cg_iterations = 0;
     std::string device;
+    int warm_kept = 0, warm_discarded = 0; // learned warm start: guesses used / discarded
 };
 
 // C ABI status -> std::runtime_error with sg_last_error() (the "GPU error: ..." path of

@@ -140,6 +140,11 @@ void write_outputs(const Params& P, const std::vector<double>& u, const std::vec
     std::fprintf(fp, "  \"collective_seconds_mean\": %.6f,\n  \"collective_seconds_max\": %.6f,\n",
                  0.0, 0.0);
     std::fprintf(fp, "  \"diverged\": %s,\n", r.diverged ? "true" : "false");
+    if (p.pressure_warm_start == SG_WARM_LEARNED)
+        std::fprintf(fp, "  \"warm_start\": {\"weights\": \"%s\", \"level\": %d, \"kept\": %d, "
+                         "\"discarded\": %d},\n",
+                     json_escape(P.warm_start_weights).c_str(), P.warm_start_level, r.warm_kept,
+                     r.warm_discarded);
     std::fprintf(fp,
                  "  \"iterations\": %d,\n  \"converged\": %s,\n  \"seconds\": %.6f,\n"
                  "  \"seconds_per_iteration\": %.9f\n}\n",

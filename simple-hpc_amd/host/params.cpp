@@ -62,7 +62,7 @@ const std::set<std::string>& known_keys() {
         "mpi_grid", "halo_overlap", "write_fields",
         // additions
         "mg_precision", "mg_coarsest", "mg_dc_maxit", "pressure_warm_start",
-        "warm_start_weights", "check_every"};
+        "warm_start_weights", "warm_start_level", "check_every"};
     return k;
 }
 } // namespace
@@ -155,6 +155,7 @@ Params make_params(const ConfigMap& cfg) {
             p.pressure_warm_start =
                 as_enum(k, v, {{"none", SG_WARM_NONE}, {"learned", SG_WARM_LEARNED}});
         else if (k == "warm_start_weights") P.warm_start_weights = v;
+        else if (k == "warm_start_level") P.warm_start_level = as_int(k, v);
         else if (k == "check_every") p.check_every = as_int(k, v);
     }
     check(sg_params_finalize(&p));

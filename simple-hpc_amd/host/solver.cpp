@@ -7,17 +7,40 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
 
 namespace sh {
 
 namespace {
 struct Handle {
     sg_solver* s = nullptr;
+    sg_unet* net = nullptr;
     ~Handle() {
         if (s)
             sg_destroy(s);
+        if (net)
+            sg_unet_destroy(net);
     }
 };
+
+// Weights file of the warm-start network: the line "SGUNET1 <n>" then n little-endian float32.
+std::vector<float> read_weights(const std::string& path) {
+    std::FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f)
+        throw std::runtime_error("cannot open warm_start_weights file " + path);
+    char head[64] = {0};
+    int n = 0;
+    const bool ok = std::fgets(head, sizeof head, f) && std::sscanf(head, "SGUNET1 %d", &n) == 1 &&
+                    n > 0;
+    std::vector<float> w(ok ? static_cast<size_t>(n) : 0);
+    const bool full = ok && std::fread(w.data(), sizeof(float), w.size(), f) == w.size();
+    std::fclose(f);
+    if (!full)
+        throw std::runtime_error(path + ": not a warm-start weights file (SGUNET1)");
+    return w;
+}
 } // namespace
 
 RunResult run_simple_gpu(const Params& P, std::vector<double>& u, std::vector<double>& v,
@@ -29,6 +52,14 @@ RunResult run_simple_gpu(const Params& P, std::vector<double>& u, std::vector<do
         out.device = name;
     Handle h;
     check(sg_create(&p, u.data(), v.data(), pf.data(), &h.s));
+    if (p.pressure_warm_start == SG_WARM_LEARNED) {
+        if (P.warm_start_weights.empty())
+            throw std::runtime_error("pressure_warm_start = learned needs warm_start_weights");
+        const std::vector<float> w = read_weights(P.warm_start_weights);
+        check(sg_unet_create(&h.net));
+        check(sg_unet_set_params(h.net, w.data(), static_cast<int>(w.size())));
+        check(sg_attach_warm_start(h.s, h.net, P.warm_start_level));
+    }
     const auto t0 = std::chrono::steady_clock::now();
     std::array<double, SG_NPHASES> ph1{};
     double eager_s = 0.0;
@@ -70,6 +101,8 @@ RunResult run_simple_gpu(const Params& P, std::vector<double>& u, std::vector<do
             break;
     }
     out.iterations = done;
+    if (h.net)
+        check(sg_warm_stats(h.s, &out.warm_kept, &out.warm_discarded));
     check(sg_download(h.s, SG_F_U, u.data()));
     check(sg_download(h.s, SG_F_V, v.data()));
     check(sg_download(h.s, SG_F_P, pf.data()));

@@ -14,8 +14,6 @@ the library's own bfloat16 MFMA forward and backward kernels (sg_unet_train_step
 Weights file: the ASCII line "SGUNET1 <n>\\n" then n little-endian float32 parameters.
 """
 import math
-import os
-import struct
 import time
 
 import numpy as np
@@ -23,6 +21,24 @@ import numpy as np
 import simple_gpu as sg
 
 P_SYSTEM = ("AEP", "AWP", "ANP", "ASP", "APP", "b")
+
+# (C_in as stored, C_out, C_in used) of the network's 14 3x3 convolutions, as in sg_unet.hip.
+CONVS = [(8, 16, 3), (16, 16, 16), (16, 32, 16), (32, 32, 32), (32, 64, 32), (64, 64, 64),
+         (64, 128, 64), (128, 128, 128), (192, 64, 192), (64, 64, 64), (96, 32, 96), (32, 32, 32),
+         (48, 16, 48), (16, 16, 16)]
+
+
+def init_params(seed=0):
+    """He-normal convolution weights, zero biases, head weights N(0, 1/16): the flat FP32 layout of
+    sg_unet_set_params, from numpy's PCG64(seed)."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for _, co, ci in CONVS:
+        parts.append((rng.standard_normal(co * ci * 9) * np.sqrt(2.0 / (ci * 9))).astype(np.float32))
+        parts.append(np.zeros(co, np.float32))
+    parts.append((rng.standard_normal(16) * 0.25).astype(np.float32))
+    parts.append(np.zeros(1, np.float32))
+    return np.concatenate(parts)
 
 
 def teacher_params(n, re):
@@ -137,3 +153,80 @@ def load(path):
     if theta.size != n:
         raise ValueError(f"{path}: truncated")
     return theta.astype(np.float32)
+
+
+def _mixed_params(n, re, learned, cycles=1, tol=1e-5, check_every=200):
+    text = (f"N_cells_x = {n}\nN_cells_y = {n}\nRe = {re!r}\nconvection_scheme = QUICK\n"
+            f"diffusion_order = 4\nmomentum_solver = gs_multicolour\npressure_solver = mg\n"
+            f"mg_pre = 2\nmg_post = 2\nmg_cycles = {cycles}\nmg_precision = mixed\n"
+            f"error_tolerance = {tol!r}\nmaxit = 10000000\nbackend = gpu\nwrite_fields = False\n"
+            f"check_every = {check_every}\n")
+    if learned:
+        text += "pressure_warm_start = learned\n"
+    return sg.params_from_config(text)[0]
+
+
+def ab_per_solve(net, n, re, snapshots, level=2):
+    """Error of p' after one mixed-precision p' solve, started from zero and from the network's
+    guess, against the converged p' of the same system, on states of an n x n run at Re.
+    Returns a list of (iteration, error zero start, error warm start, guess kept)."""
+    run = sg.Solver(_mixed_params(n, re, False))
+    zero = sg.Solver(_mixed_params(n, re, False))
+    warm = sg.Solver(_mixed_params(n, re, True))
+    sg.attach_warm_start(warm, net, level)
+    teacher = sg.Solver(teacher_params(n, re))
+    out, done = [], 0
+    H = run.p.H
+    for snap in snapshots:
+        run.advance(snap - done)
+        done = snap
+        sysa = {name: run.get(name) for name in P_SYSTEM}
+        res = []
+        for s in (teacher, zero, warm):
+            for name in P_SYSTEM:
+                s.set(name, sysa[name])
+            s.solve_p()
+            res.append(s.get("pc")[H:H + n, H + 1:H + 1 + n])
+        conv = res[0] - res[0].mean()
+        e = [float(np.linalg.norm(r - r.mean() - conv) / np.linalg.norm(conv)) for r in res[1:]]
+        out.append((snap, e[0], e[1]))
+    acc, rej = sg.warm_stats(warm)
+    for s in (run, zero, warm, teacher):
+        s.close()
+    return out, (acc, rej)
+
+
+def ab_to_tolerance(net, n, re, level=2, cap=10000000, log=print):
+    """Outer iterations and seconds to the tolerance, mixed vs learned + mixed."""
+    res = {}
+    for form in ("mixed", "learned"):
+        s = sg.Solver(_mixed_params(n, re, form == "learned", check_every=500))
+        if form == "learned":
+            sg.attach_warm_start(s, net, level)
+        s.advance(3)
+        s.autotune(reps=5)  # launch shapes only: results are unchanged
+        keys = (sg.TUNE_TAIL_THREADS, sg.TUNE_MID_CELLS, sg.TUNE_LDS_CELLS, sg.TUNE_MOM_TILE)
+        knobs = [s.get_tuning(k) for k in keys]
+        s.close()
+        s = sg.Solver(_mixed_params(n, re, form == "learned", check_every=500))
+        for k, v in zip(keys, knobs):
+            s.set_tuning(k, v)
+        if form == "learned":
+            sg.attach_warm_start(s, net, level)
+        s.advance(1)
+        t0 = time.perf_counter()
+        total, conv = 1, False
+        while not conv and total < cap:
+            k, lg, conv = s.advance(5000)
+            total += k
+            if k == 0:
+                break
+        t = time.perf_counter() - t0
+        r = {"iterations": total, "seconds": t, "converged": bool(conv),
+             "ms_per_iter": 1e3 * t / max(total - 1, 1)}
+        if form == "learned":
+            r["guesses_kept"], r["guesses_discarded"] = sg.warm_stats(s)
+        res[form] = r
+        log(f"  {n}^2 Re {re:g} {form}: {r}")
+        s.close()
+    return res

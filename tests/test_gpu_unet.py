@@ -41,7 +41,7 @@ def test_param_count_matches_oracle():
     net.close()
 
 
-@pytest.mark.parametrize("n,h,w", [(1, 8, 8), (2, 16, 24), (1, 32, 32)])
+@pytest.mark.parametrize("n,h,w", [(1, 8, 8), (2, 16, 24), (1, 32, 32), (1, 256, 256)])
 def test_forward_matches_oracle(n, h, w):
     x, _ = _inputs(1, n, h, w)
     th = _theta(1)
@@ -113,7 +113,7 @@ def _mixed_params(n, re, learned):
 
 
 def test_rejected_guess_is_exactly_the_plain_solve():
-    net = sg.UNet(np.zeros(U.n_params(), np.float32))  # predicts 0: never lowers the defect
+    net = sg.UNet(np.zeros(U.n_params(), np.float32))  # predicts 0: a degenerate guess
     plain = sg.Solver(_mixed_params(64, 400.0, False))
     warm = sg.Solver(_mixed_params(64, 400.0, True))
     sg.attach_warm_start(warm, net, level=2)
@@ -137,3 +137,32 @@ def test_features_shape_and_scale():
     assert abs(np.mean(x[..., 1])) < 1e-3                       # a_P channel: mean 0
     assert np.allclose(x[..., 2], np.log10(0.4)) and np.all(x[..., 3:] == 0)
     s.close()
+
+
+def test_cli_learned_warm_start(tmp_path):
+    import json
+    import os
+    import subprocess
+    import warmstart as ws
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    wpath = tmp_path / "w.bin"
+    ws.save(str(wpath), ws.init_params(0))
+    assert np.array_equal(ws.load(str(wpath)), ws.init_params(0))
+    out = tmp_path / "run"
+    cfg = tmp_path / "c.cfg"
+    cfg.write_text(f"N_cells_x = 64\nN_cells_y = 64\nRe = 400\nconvection_scheme = QUICK\n"
+                   f"diffusion_order = 2\nmomentum_solver = gs_multicolour\npressure_solver = mg\n"
+                   f"mg_precision = mixed\npressure_warm_start = learned\n"
+                   f"warm_start_weights = {wpath}\nerror_tolerance = 0\nmaxit = 60\n"
+                   f"backend = gpu\noutput_dir = {out}\n")
+    r = subprocess.run([os.path.join(root, "build", "simple"), str(cfg)], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    info = json.loads((out / "run_info.json").read_text())
+    w = info["warm_start"]
+    # one guess per p' solve; iteration 1 runs eagerly too
+    assert w["kept"] + w["discarded"] == 60 and w["level"] == 2, w
+    cfg.write_text(cfg.read_text().replace(f"warm_start_weights = {wpath}\n", ""))
+    r = subprocess.run([os.path.join(root, "build", "simple"), str(cfg)], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 1 and "needs warm_start_weights" in r.stderr

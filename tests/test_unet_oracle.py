@@ -3,7 +3,7 @@ its analytic gradients (every parameter block, a tiny input), the bfloat16 round
 known values, and the convolution against a direct loop."""
 import numpy as np
 
-import unet_ref as U
+import unet_ref as U  # noqa: E402
 
 
 def test_bf16_rounding_known_values():
@@ -80,3 +80,9 @@ def test_bf16_mode_stays_close_to_fp64():
     assert abs(l16 - l64) < 2e-2 * l64
     cos = float(np.dot(g16, g64) / np.linalg.norm(g16) / np.linalg.norm(g64))
     assert cos > 0.99, cos
+
+
+def test_product_initialisation_matches_layout():
+    import warmstart as ws
+    assert [(ci, co) for ci, co, _ in ws.CONVS] == U.conv_specs()
+    assert np.array_equal(ws.init_params(0), U.init_params(0))
