@@ -299,6 +299,29 @@ def run_learned(sg, quick=False, full=False):
     return out
 
 
+def run_unet(sg, reps=50):
+    """The warm-start network's bfloat16 MFMA kernels: one forward pass at the 1024^2 cavity's
+    level-2 shape (256 x 256) and one training step (8 x 64 x 64), timed with hipEvents."""
+    import numpy as np
+    import warmstart as ws
+    net = sg.UNet(ws.init_params(0))
+    sec, fl = net.probe(1, 256, 256, reps=reps)
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal((8, 64, 64, 8)).astype(np.float32)
+    x[..., 3:] = 0
+    t = rng.standard_normal((8, 64, 64)).astype(np.float32)
+    net.train_step(x, t, 1e-4)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        net.train_step(x, t, 1e-4)
+    tstep = (time.perf_counter() - t0) / reps
+    _, fl64 = net.probe(8, 64, 64, reps=1)
+    net.close()
+    return {"forward_us": sec * 1e6, "forward_flops": fl, "forward_tflops": fl / sec / 1e12,
+            "train_step_ms": tstep * 1e3, "train_step_flops": 3 * fl64,
+            "train_tflops": 3 * fl64 / tstep / 1e12}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -309,7 +332,7 @@ def main():
     ap.add_argument("--n", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="cavity",
-                    choices=["cavity", "batch", "large", "learned"])
+                    choices=["cavity", "batch", "large", "learned", "unet"])
     ap.add_argument("--quick", action="store_true", help="learned: a small, fast version")
     ap.add_argument("--full", action="store_true", help="learned: also 1024^2 to tolerance")
     ap.add_argument("--knobs", default=None,
@@ -335,6 +358,21 @@ def main():
                           **r}}
         if "cpu_baseline" in r:
             out["cpu_baseline"] = r.pop("cpu_baseline")
+        print(json.dumps(out))
+        return
+    if a.workload == "unet":
+        r = run_unet(sg)
+        peak = 2500.0  # MI355X dense bf16 MFMA, TFLOP/s
+        out = {"metric": "U-Net forward pass, 256x256 (bf16 MFMA, FP32 accumulation)",
+               "value": r["forward_tflops"], "unit": "TFLOP/s", "n_gpus": 1, "steps": 50,
+               "warmup": 1, "ms_per_step": r["forward_us"] / 1e3, "higher_is_better": True,
+               "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
+               "data": "synthetic (random weights and inputs)",
+               "config": {"workload": "warm-start U-Net 16-32-64-128, 3x3 convs, one 256x256 "
+                                      "image; training step 8x64x64", **r},
+               "roofline": {"bound": "mfma", "achieved": r["forward_tflops"], "peak": peak,
+                            "unit": "TFLOP/s", "frac": r["forward_tflops"] / peak,
+                            "traffic": None}}
         print(json.dumps(out))
         return
     if a.workload == "learned":

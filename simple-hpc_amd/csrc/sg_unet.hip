@@ -159,6 +159,8 @@ template <int CPW> __global__ void __launch_bounds__(CV_T) k_conv(ConvArgs a) {
         }
 #pragma unroll
         for (int c = 0; c < CPW; ++c) {
+            if (co0 + c * 16 >= a.cout) // wave-uniform: no MFMA for absent channel blocks
+                break;
             const int co = co0 + c * 16 + m;
             const bf16x8 B = co < a.cout ? ld8(a.w + static_cast<size_t>(co) * a.kpad + kk) : zero8();
             acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[c], 0, 0, 0);
