@@ -375,7 +375,6 @@ Solver* create_solver(const sg_params& p, const double* u, const double* v, cons
         s.cell = dalloc<double>(s, 4 * static_cast<size_t>(std::max(p.NX + 2, 1)) * (p.NY + 2));
         s.rowbuf = dalloc<double>(s, 4 * static_cast<size_t>(s.nr));
         s.scal = dalloc<double>(s, S_COUNT);
-        s.ticket = dalloc<unsigned>(s, 4);
         for (auto& c : s.rcell)
             c = dalloc<double>(s, static_cast<size_t>(p.NX + 2) * (p.NY + 2));
         SG_HIP(hipStreamCreateWithFlags(&s.side, hipStreamNonBlocking));

@@ -96,7 +96,6 @@ struct Solver {
     double* scratch = nullptr;         // Jacobi previous iterate
     double* cell = nullptr;            // per-cell values for ordered sums (4 planes of n)
     double* rowbuf = nullptr;          // row sums, one nr-slice per ordered-sum lane (4 lanes)
-    unsigned* ticket = nullptr;        // per lane: last-workgroup ticket of the ordered sums (kept 0)
     double* rcell[3] = {};             // |R| cells of res_u, res_v, res_p (summed on the side stream)
     hipStream_t side = nullptr;        // side stream: the residual sums, off the critical path
     hipEvent_t fork[3] = {}, join[3] = {};
@@ -182,7 +181,7 @@ void launch_correct(Solver& s, const int* stop);
 void launch_fill(Solver& s, double* a, size_t n, double v, const int* stop);
 // ordered sums (sg_reduce.hip): rows [i0, i1], columns [j0, j1] of a pitched array, row sums j
 // ascending from 0.0 into rowbuf, then rows ascending from 0.0 into *slot.
-// lane: which row-sum slice and ticket (0 on the main stream; 1..3 on the side stream); the sum
+// lane: which row-sum slice (0 on the main stream; 1..3 on the side stream); the sum
 // is enqueued on `stream` (the main stream if null).
 void launch_ordered_sum(Solver& s, const double* a, int pitch, int i0, int i1, int j0, int j1,
                         double* slot, const int* stop, int lane = 0, hipStream_t stream = nullptr);

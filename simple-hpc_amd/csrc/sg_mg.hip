@@ -441,6 +441,101 @@ template <class T> __device__ int wg_coarse_solve(const Lv<T>& L, T* rows) {
     return sweeps;
 }
 
+// The same coarse solve by one wave for a level of at most 64 cells (lane = cell I * ny + J):
+// operator, b and x in registers, neighbours by lane shuffles, every sum by a wave-uniform chain in
+// det_sum's order (each row J ascending from 0.0, then the rows ascending from 0.0). Same
+// expressions in the same order as smooth_cell / resid_cell / wg_det_sum, so the same bits; no
+// workgroup barrier inside the sweep loop. Ends with a barrier (other waves then see x and b).
+template <class T> __device__ __forceinline__ T lane_val(T v, int l);
+template <> __device__ __forceinline__ double lane_val<double>(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), l);
+    const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), l);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
+template <> __device__ __forceinline__ float lane_val<float>(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+template <class T> __device__ int wave_coarse_solve(const Lv<T>& L) {
+    __shared__ int nsweeps;
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x, nx = L.nx, ny = L.ny;
+        const bool act = lane < nx * ny;
+        const int I = act ? lane / ny : 0, J = act ? lane - (lane / ny) * ny : 0;
+        const size_t q = static_cast<size_t>(I) * L.stride + J;
+        T ae = T(0), aw = T(0), an = T(0), as = T(0), ap = T(1), b = T(0), x = T(0);
+        if (act) {
+            ae = L.AE[q];
+            aw = L.AW[q];
+            an = L.AN[q];
+            as = L.AS[q];
+            ap = L.AP[q];
+            b = L.b[q];
+            x = L.x[q];
+        }
+        const bool hasE = act && J + 1 < ny, hasW = act && J > 0;
+        const bool hasS = act && I + 1 < nx, hasN = act && I > 0;
+        const int col = (I + J) & 1;
+        auto dsum = [&](T v) {
+            T tot = T(0);
+            for (int i = 0; i < nx; ++i) {
+                T r = T(0);
+                for (int j = 0; j < ny; ++j)
+                    r += lane_val<T>(v, i * ny + j);
+                tot += r;
+            }
+            return tot;
+        };
+        // neighbour values (zero halo)
+        auto nbs = [&](T v, T& xe, T& xw, T& xs, T& xn) {
+            const T e = __shfl(v, lane + 1, 64), w = __shfl(v, lane - 1, 64);
+            const T sv = __shfl(v, lane + ny, 64), nv = __shfl(v, lane - ny, 64);
+            xe = hasE ? e : T(0);
+            xw = hasW ? w : T(0);
+            xs = hasS ? sv : T(0);
+            xn = hasN ? nv : T(0);
+        };
+        const T ncell = static_cast<T>(static_cast<double>(nx) * ny);
+        const T mb = dsum(b) / ncell;
+        if (act)
+            b -= mb;
+        const T bb = dsum(b * b);
+        int sweeps = 0;
+        if (bb > T(0)) {
+            const T bn = sqrt(bb);
+            while (sweeps < 200) {
+                for (int c = 0; c < 2; ++c) {
+                    T xe, xw, xs, xn;
+                    nbs(x, xe, xw, xs, xn);
+                    if (act && col == c) {
+                        T t = b / ap;
+                        t += (ae * xe + aw * xw + as * xs + an * xn) / ap;
+                        x = t;
+                    }
+                }
+                ++sweeps;
+                T xe, xw, xs, xn;
+                nbs(x, xe, xw, xs, xn);
+                const T r = act ? b - (ap * x - (ae * xe + aw * xw + as * xs + an * xn)) : T(0);
+                const T rr = dsum(r * r);
+                if (sqrt(rr) < coarse_tol<T>() * bn)
+                    break;
+            }
+        }
+        const T mx = dsum(x) / ncell;
+        if (act) {
+            x -= mx;
+            L.x[q] = x;
+            L.b[q] = b;
+        }
+        if (lane == 0)
+            nsweeps = sweeps;
+    }
+    __syncthreads();
+    return nsweeps;
+}
+
 template <class T> __device__ void wg_build(const Lv<T>& F, const Lv<T>& C) {
     for (int e = threadIdx.x; e < C.nx * C.ny; e += blockDim.x)
         build_cell(F, C, e / C.ny, e % C.ny);
@@ -499,7 +594,8 @@ template <class T> __device__ __forceinline__ void k_mg_tail(const Tail<T>& t, c
         }
         __syncthreads();
     }
-    const int sweeps = wg_coarse_solve(V(t.nl - 1), rows);
+    const Lv<T> Lc = V(t.nl - 1);
+    const int sweeps = Lc.nx * Lc.ny <= 64 ? wave_coarse_solve(Lc) : wg_coarse_solve(Lc, rows);
     for (int l = t.nl - 2; l >= t.tm; --l) {
         const Lv<T> F = V(l), C = V(l + 1);
         for (int e = threadIdx.x; e < F.nx * F.ny; e += blockDim.x) {

@@ -4,11 +4,13 @@
 // associative, so this order is what makes every sum that feeds back into a field (mean
 // removals, CG alpha/beta) and every residual bit-identical to the CPU.
 //
-// One launch. A workgroup of 256 threads owns RW rows: all threads load a RW x CW chunk with
-// coalesced, independent loads into LDS, then RW threads add their rows' chunk in order. The
-// additions of a row are inherently sequential (~1 FP64 add per few cycles); spreading the rows
-// over many workgroups keeps every CU loading while they run. The last workgroup to finish (a
-// device-scope atomic ticket) adds the row sums in order and re-arms the ticket.
+// Two launches, no fence inside either. Rows: a workgroup of 256 threads owns 4 rows; every
+// thread loads its share of a 4 x 1024 chunk at once (all loads in flight) into LDS, then one lane
+// per row walks its row in order from LDS (reads in batches of 8 ahead of the adds), so each
+// chain waits on FP64 adds rather than memory. Total: one workgroup loads the row sums into LDS
+// the same way and one lane adds them in row order. At 1024 x 1024 each launch takes about 14 us.
+// (A single launch that hands the row sums to the last workgroup through a ticket needs a
+// device-scope fence in every workgroup; measured at the same size it took 34-48 us.)
 #include "sg_internal.hpp"
 
 namespace sg {
@@ -16,37 +18,25 @@ namespace sg {
 namespace {
 
 constexpr int NT = 256;  // threads per workgroup
-constexpr int RW = 4;    // rows per workgroup
+constexpr int RW = 4;    // rows per workgroup (one lane of wave 0 each)
 constexpr int CW = 1024; // columns per LDS chunk
 constexpr int PAD = 2;   // row padding (doubles) against bank conflicts
 constexpr int LOADS = RW * CW / NT;
+constexpr int TCH = 4096; // row sums per chunk of the total
 
-// s += v[0] + v[1] + ... in order, from LDS. The reads of the next batch of 8 are issued before
-// the 8 dependent additions of the current one, so LDS latency overlaps the add chain.
-__device__ __forceinline__ double add_in_order(double s, const double* v, int n) {
-    const int nb = n / 8;
-    if (nb > 0) {
+// s + v[0] + ... + v[n-1] in order, from LDS, reads in batches of 8.
+__device__ __forceinline__ double lds_chain(double s, const double* v, int n) {
+    int c = 0;
+    for (; c + 8 <= n; c += 8) {
         double a[8];
 #pragma unroll
         for (int q = 0; q < 8; ++q)
-            a[q] = v[q];
-        for (int b = 1; b < nb; ++b) {
-            double c[8];
-#pragma unroll
-            for (int q = 0; q < 8; ++q)
-                c[q] = v[8 * b + q];
-#pragma unroll
-            for (int q = 0; q < 8; ++q)
-                s += a[q];
-#pragma unroll
-            for (int q = 0; q < 8; ++q)
-                a[q] = c[q];
-        }
+            a[q] = v[c + q];
 #pragma unroll
         for (int q = 0; q < 8; ++q)
             s += a[q];
     }
-    for (int c = 8 * nb; c < n; ++c)
+    for (; c < n; ++c)
         s += v[c];
     return s;
 }
@@ -56,30 +46,21 @@ struct SumArgs {
     long pitch;
     int i0, nrow, j0, w;
     double* rows;
-    unsigned* ticket;
     double* slot;
 };
 
-__device__ __forceinline__ void k_ordered_sum(const SumArgs& sa, const Blk& bk) {
-    const double* a = sa.a;
-    const long pitch = sa.pitch;
-    const int i0 = sa.i0, nrow = sa.nrow, j0 = sa.j0, w = sa.w;
-    double* rows = sa.rows;
-    unsigned* ticket = sa.ticket;
-    double* slot = sa.slot;
+__device__ __forceinline__ void k_row_sums(const SumArgs& sa, const Blk& bk) {
     __shared__ double tile[RW][CW + PAD];
-    __shared__ bool last;
-    const int r0 = bk.x * RW;
-    const int t = threadIdx.x;
+    const int r0 = bk.x * RW, t = threadIdx.x;
     double s = 0.0;
-    for (int c0 = 0; c0 < w; c0 += CW) {
-        const int cw = min(CW, w - c0);
+    for (int c0 = 0; c0 < sa.w; c0 += CW) {
+        const int cw = min(CW, sa.w - c0);
         double v[LOADS];
 #pragma unroll
         for (int k = 0; k < LOADS; ++k) {
             const int e = k * NT + t, r = e / CW, c = e - r * CW;
-            v[k] = (c < cw && r0 + r < nrow)
-                       ? a[static_cast<size_t>(i0 + r0 + r) * pitch + j0 + c0 + c]
+            v[k] = (c < cw && r0 + r < sa.nrow)
+                       ? sa.a[static_cast<size_t>(sa.i0 + r0 + r) * sa.pitch + sa.j0 + c0 + c]
                        : 0.0;
         }
 #pragma unroll
@@ -88,37 +69,29 @@ __device__ __forceinline__ void k_ordered_sum(const SumArgs& sa, const Blk& bk) 
             tile[r][c] = v[k];
         }
         __syncthreads();
-        if (t < RW && r0 + t < nrow)
-            s = add_in_order(s, tile[t], cw);
+        if (t < RW && r0 + t < sa.nrow)
+            s = lds_chain(s, tile[t], cw);
         __syncthreads();
     }
-    if (t < RW && r0 + t < nrow)
-        rows[r0 + t] = s;
-    __threadfence();
-    __syncthreads();
-    if (t == 0)
-        last = atomicAdd(ticket, 1u) == static_cast<unsigned>(bk.nx - 1);
-    __syncthreads();
-    if (!last)
-        return;
-    __threadfence();
-    // Rows ascending from 0.0, staged through LDS, added by thread 0.
-    double* buf = &tile[0][0];
-    const int chunk = RW * (CW + PAD);
-    double total = 0.0;
-    for (int b0 = 0; b0 < nrow; b0 += chunk) {
-        const int n = min(chunk, nrow - b0);
+    if (t < RW && r0 + t < sa.nrow)
+        sa.rows[r0 + t] = s;
+}
+
+__device__ __forceinline__ void k_rows_total(const SumArgs& sa, const Blk&) {
+    __shared__ double buf[TCH];
+    const int t = threadIdx.x;
+    double s = 0.0;
+    for (int b0 = 0; b0 < sa.nrow; b0 += TCH) {
+        const int n = min(TCH, sa.nrow - b0);
         for (int k = t; k < n; k += NT)
-            buf[k] = __builtin_nontemporal_load(&rows[b0 + k]);
+            buf[k] = sa.rows[b0 + k];
         __syncthreads();
         if (t == 0)
-            total = add_in_order(total, buf, n);
+            s = lds_chain(s, buf, n);
         __syncthreads();
     }
-    if (t == 0) {
-        *slot = total;
-        *ticket = 0u;
-    }
+    if (t == 0)
+        *sa.slot = s;
 }
 
 struct SlotArgs {
@@ -140,8 +113,9 @@ void launch_ordered_sum(Solver& s, const double* a, int pitch, int i0, int i1, i
         emit<SlotArgs, k_zero_slot, 64>(s, dim3(1), dim3(64), 0, SlotArgs{slot}, stop, st);
     } else {
         const SumArgs sa{a, static_cast<long>(pitch), i0, nrow, j0, w,
-                         s.rowbuf + static_cast<size_t>(lane) * s.nr, s.ticket + lane, slot};
-        emit<SumArgs, k_ordered_sum, NT>(s, dim3(cdiv(nrow, RW)), dim3(NT), 0, sa, stop, st);
+                         s.rowbuf + static_cast<size_t>(lane) * s.nr, slot};
+        emit<SumArgs, k_row_sums, NT>(s, dim3(cdiv(nrow, RW)), dim3(NT), 0, sa, stop, st);
+        emit<SumArgs, k_rows_total, NT>(s, dim3(1), dim3(NT), 0, sa, stop, st);
     }
 }
 
