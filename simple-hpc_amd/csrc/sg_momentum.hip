@@ -218,16 +218,21 @@ struct SweepArgs {
     int nr, nc, first;
 };
 
-template <int NCOL, int TI, int TJ> __device__ __forceinline__ void k_mom_sweep(const SweepArgs& sa, const Blk& bk) {
+// FIRST: a second LDS copy keeps the loaded (pre-sweep) tile, and each core cell's residual
+// |R| = |(nb + pterm) - AP x| (src/momentum_u.cpp:48-50) is evaluated from that copy during the
+// cell's own colour pass, with the coefficients the update loads anyway.
+template <int NCOL, int TI, int TJ, bool FIRST>
+__device__ __forceinline__ void k_mom_sweep(const SweepArgs& sa, const Blk& bk) {
     const MomArgs& m = sa.m;
     const double* src = sa.src;
     double* dst = sa.dst;
     double* cell = sa.cell;
-    const int nr = sa.nr, nc = sa.nc, first = sa.first;
+    const int nr = sa.nr, nc = sa.nc;
     constexpr int R = NCOL == 2 ? 1 : 2; // stencil reach
     constexpr int HL = R * NCOL;          // halo read by the first pass
     constexpr int LW = TJ + 2 * HL, LH = TI + 2 * HL;
     __shared__ double su[LH * LW];
+    __shared__ double su0[FIRST ? LH * LW : 1];
     int tx, ty;
     xcd_tile(bk, tx, ty);
     const int ti0 = ty * TI, tj0 = tx * TJ;
@@ -247,32 +252,19 @@ template <int NCOL, int TI, int TJ> __device__ __forceinline__ void k_mom_sweep(
 #pragma unroll
         for (int k = 0; k < NL; ++k) {
             const int e = k * ST + threadIdx.x;
-            if (e < LH * LW)
+            if (e < LH * LW) {
                 su[e] = v[k];
+                if (FIRST)
+                    su0[e] = v[k];
+            }
         }
     }
     __syncthreads();
     const Pack& pk = m.pk;
     const size_t as = pk.astride;
+    const int rw = m.j1 - m.j0 + 1;
 #define SU(i, j) su[((i) - gi0) * LW + ((j) - gj0)]
-    if (first) {
-        const int ia = max(m.i0, ti0), ib = min(m.i1, ti0 + TI - 1);
-        const int ja = max(m.j0, tj0), jb = min(m.j1, tj0 + TJ - 1);
-        const int w = jb - ja + 1, n = (ib - ia + 1) * w;
-        for (int e = threadIdx.x; e < n; e += ST) {
-            const int i = ia + e / w, j = ja + e % w;
-            const int c = colour_of(NCOL, i, j);
-            const double* co = pk.base + pidx(pk, c, i, (j - jfirst(pk, c, i)) / NCOL);
-            const double nb =
-                (co[0] * SU(i, j + 1) + co[as] * SU(i, j - 1) + co[2 * as] * SU(i - 1, j) +
-                 co[3 * as] * SU(i + 1, j)) +
-                (co[4 * as] * SU(i, j + 2) + co[5 * as] * SU(i, j - 2) +
-                 co[6 * as] * SU(i - 2, j) + co[7 * as] * SU(i + 2, j));
-            const double Rv = nb + co[9 * as] - co[8 * as] * SU(i, j);
-            cell[static_cast<size_t>(i - m.i0) * (m.j1 - m.j0 + 1) + (j - m.j0)] = fabs(Rv);
-        }
-        __syncthreads(); // the passes below overwrite values the residual reads
-    }
+#define S0(i, j) su0[((i) - gi0) * LW + ((j) - gj0)]
     for (int k = 0; k < NCOL; ++k) {
         const int d = R * (NCOL - 1 - k);
         const int ia = max(m.i0, ti0 - d), ib = min(m.i1, ti0 + TI - 1 + d);
@@ -287,12 +279,30 @@ template <int NCOL, int TI, int TJ> __device__ __forceinline__ void k_mom_sweep(
                 if (j > jb)
                     continue;
                 const double* co = pk.base + pidx(pk, k, i, (j - jfirst(pk, k, i)) / NCOL);
-                double nb = co[0] * SU(i, j + 1) + co[as] * SU(i, j - 1) +
-                            co[2 * as] * SU(i - 1, j) + co[3 * as] * SU(i + 1, j);
+                const double ce = co[0], cw = co[as], cn = co[2 * as], cs = co[3 * as];
+                const double cp = co[8 * as], ct = co[9 * as];
+                double cee = 0.0, cww = 0.0, cnn = 0.0, css = 0.0;
+                const bool core = FIRST && i >= ti0 && i < ti0 + TI && j >= tj0 && j < tj0 + TJ;
+                if (NCOL != 2 || core) {
+                    cee = co[4 * as];
+                    cww = co[5 * as];
+                    cnn = co[6 * as];
+                    css = co[7 * as];
+                }
+                if (core) {
+                    const double nb0 = (ce * S0(i, j + 1) + cw * S0(i, j - 1) + cn * S0(i - 1, j) +
+                                        cs * S0(i + 1, j)) +
+                                       (cee * S0(i, j + 2) + cww * S0(i, j - 2) +
+                                        cnn * S0(i - 2, j) + css * S0(i + 2, j));
+                    cell[static_cast<size_t>(i - m.i0) * rw + (j - m.j0)] =
+                        fabs(nb0 + ct - cp * S0(i, j));
+                }
+                double nb = ce * SU(i, j + 1) + cw * SU(i, j - 1) + cn * SU(i - 1, j) +
+                            cs * SU(i + 1, j);
                 if (NCOL != 2)
-                    nb = nb + (co[4 * as] * SU(i, j + 2) + co[5 * as] * SU(i, j - 2) +
-                               co[6 * as] * SU(i - 2, j) + co[7 * as] * SU(i + 2, j));
-                SU(i, j) = m.alpha * (nb + co[9 * as]) / co[8 * as] + m.one_minus_alpha * SU(i, j);
+                    nb = nb + (cee * SU(i, j + 2) + cww * SU(i, j - 2) + cnn * SU(i - 2, j) +
+                               css * SU(i + 2, j));
+                SU(i, j) = m.alpha * (nb + ct) / cp + m.one_minus_alpha * SU(i, j);
             }
         }
         __syncthreads();
@@ -303,6 +313,7 @@ template <int NCOL, int TI, int TJ> __device__ __forceinline__ void k_mom_sweep(
         if (e < TI * TJ && i < nr && j < nc)
             dst[static_cast<size_t>(i) * ld + j] = SU(i, j);
     }
+#undef S0
 #undef SU
 }
 
@@ -467,28 +478,36 @@ void launch_mom_residual(Solver& s, bool is_u, double* slot, const int* stop, bo
 }
 
 namespace {
+template <int NCOL, bool FIRST>
+void sweep_tile_f(Solver& s, const SweepArgs& a, const int* stop) {
+    const dim3 b(ST);
+    switch (s.mom_tile) {
+    case 0:
+        emit<SweepArgs, k_mom_sweep<NCOL, 32, 64, FIRST>, ST>(
+            s, dim3(cdiv(s.nc, 64), cdiv(s.nr, 32)), b, 0, a, stop);
+        break;
+    case 1:
+        emit<SweepArgs, k_mom_sweep<NCOL, 64, 64, FIRST>, ST>(
+            s, dim3(cdiv(s.nc, 64), cdiv(s.nr, 64)), b, 0, a, stop);
+        break;
+    case 2:
+        emit<SweepArgs, k_mom_sweep<NCOL, 32, 128, FIRST>, ST>(
+            s, dim3(cdiv(s.nc, 128), cdiv(s.nr, 32)), b, 0, a, stop);
+        break;
+    default:
+        emit<SweepArgs, k_mom_sweep<NCOL, 16, 128, FIRST>, ST>(
+            s, dim3(cdiv(s.nc, 128), cdiv(s.nr, 16)), b, 0, a, stop);
+    }
+}
+
 template <int NCOL>
 void sweep_tile(Solver& s, const MomArgs& m, const double* src, double* dst, double* cell,
                 int first, const int* stop) {
-    const dim3 b(ST);
     const SweepArgs a{m, src, dst, cell, s.nr, s.nc, first};
-    switch (s.mom_tile) {
-    case 0:
-        emit<SweepArgs, k_mom_sweep<NCOL, 32, 64>, ST>(s, dim3(cdiv(s.nc, 64), cdiv(s.nr, 32)), b,
-                                                       0, a, stop);
-        break;
-    case 1:
-        emit<SweepArgs, k_mom_sweep<NCOL, 64, 64>, ST>(s, dim3(cdiv(s.nc, 64), cdiv(s.nr, 64)), b,
-                                                       0, a, stop);
-        break;
-    case 2:
-        emit<SweepArgs, k_mom_sweep<NCOL, 32, 128>, ST>(s, dim3(cdiv(s.nc, 128), cdiv(s.nr, 32)),
-                                                        b, 0, a, stop);
-        break;
-    default:
-        emit<SweepArgs, k_mom_sweep<NCOL, 16, 128>, ST>(s, dim3(cdiv(s.nc, 128), cdiv(s.nr, 16)),
-                                                        b, 0, a, stop);
-    }
+    if (first)
+        sweep_tile_f<NCOL, true>(s, a, stop);
+    else
+        sweep_tile_f<NCOL, false>(s, a, stop);
 }
 } // namespace
 
