@@ -120,10 +120,10 @@ void ensure_log(Solver& s, int cap) {
 }
 
 // One outer iteration (src/solver.cpp:27-93), enqueued. ev: optional phase events (12).
-// The residual sums go to the side stream (they feed only the stop rule, which joins them).
+// The residual sums (they feed only the stop rule) are deferred and done together before it.
 void enqueue_iteration(Solver& s, const int* stop, bool eager, hipEvent_t* ev) {
     const sg_params& p = s.p;
-    const bool side = !ev; // per-phase timing keeps every phase on the main stream
+    const bool defer = !ev; // per-phase timing sums each residual in its own phase
     auto mark = [&](int k) {
         if (ev)
             SG_HIP(hipEventRecord(ev[k], s.stream));
@@ -134,23 +134,23 @@ void enqueue_iteration(Solver& s, const int* stop, bool eager, hipEvent_t* ev) {
     launch_bc(s, true, stop);
     mark(2);
     mark(3);
-    launch_mom_sweeps(s, true, p.maxit_inner_u, stop, s.scal + S_RES_U, side);
+    launch_mom_sweeps(s, true, p.maxit_inner_u, stop, s.scal + S_RES_U, defer);
     mark(4);
     launch_assemble(s, false, stop);
     mark(5);
     launch_bc(s, false, stop);
     mark(6);
     mark(7);
-    launch_mom_sweeps(s, false, p.maxit_inner_v, stop, s.scal + S_RES_V, side);
+    launch_mom_sweeps(s, false, p.maxit_inner_v, stop, s.scal + S_RES_V, defer);
     mark(8);
-    launch_assemble_p(s, s.scal + S_RES_P, stop, side);
+    launch_assemble_p(s, s.scal + S_RES_P, stop, defer);
     mark(9);
     launch_solve_p(s, stop, eager);
     mark(10);
     launch_correct(s, stop);
     mark(11);
-    if (side)
-        join_side(s);
+    if (defer)
+        flush_sums(s, stop);
     emit<StopArgs, k_stop_rule, 64>(s, dim3(1), dim3(64), 0,
                                     StopArgs{s.ctl, s.scal, s.dlog, s.log_cap,
                                              p.residual_normalisation, p.error_tolerance},
@@ -377,11 +377,6 @@ Solver* create_solver(const sg_params& p, const double* u, const double* v, cons
         s.scal = dalloc<double>(s, S_COUNT);
         for (auto& c : s.rcell)
             c = dalloc<double>(s, static_cast<size_t>(p.NX + 2) * (p.NY + 2));
-        SG_HIP(hipStreamCreateWithFlags(&s.side, hipStreamNonBlocking));
-        for (int k = 0; k < 3; ++k) {
-            SG_HIP(hipEventCreateWithFlags(&s.fork[k], hipEventDisableTiming));
-            SG_HIP(hipEventCreateWithFlags(&s.join[k], hipEventDisableTiming));
-        }
         s.ctl = dalloc<Ctl>(s, 1);
         s.never = dalloc<int>(s, 1);
         if (p.pressure_solver == SG_P_CG)
@@ -426,18 +421,8 @@ void destroy_solver(Solver* s) {
         (void)hipEventDestroy(s->kt0);
     if (s->kt1)
         (void)hipEventDestroy(s->kt1);
-    if (s->side)
-        (void)hipStreamSynchronize(s->side);
-    for (int k = 0; k < 3; ++k) {
-        if (s->fork[k])
-            (void)hipEventDestroy(s->fork[k]);
-        if (s->join[k])
-            (void)hipEventDestroy(s->join[k]);
-    }
     for (void* q : s->owned)
         (void)hipFree(q);
-    if (s->side)
-        (void)hipStreamDestroy(s->side);
     if (s->stream)
         (void)hipStreamDestroy(s->stream);
     delete s;

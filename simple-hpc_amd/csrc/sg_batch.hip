@@ -5,7 +5,7 @@
 // Each cavity is an ordinary Solver (its own fields, coefficients, multigrid hierarchy, control
 // block and stopping rule). To build the batch's iteration, every cavity's outer iteration is
 // enqueued in recording mode (emit() appends instead of launching). A record carries its kernel
-// instantiation, its argument struct and its grid, plus a label: (kernel, stream, structural tag,
+// instantiation, its argument struct and its grid, plus a label: (kernel, structural tag,
 // occurrence). The tag names the multigrid level and V-cycle, so cavities with different numbers of
 // levels still put "level 1 of V-cycle 0, going down" in the same launch. The labels of all
 // cavities form a DAG (each cavity's records in order); a topological order of it gives the
@@ -32,14 +32,14 @@ __global__ void k_batch_begin(Ctl* ctl) {
     }
 }
 
-using Label = std::tuple<const void*, bool, int, int>;
+using Label = std::tuple<const void*, int, int>;
 
-Label label_of(const Rec& r) { return Label{r.vt, r.side, r.tag, r.occ}; }
+Label label_of(const Rec& r) { return Label{r.vt, r.tag, r.occ}; }
 
 struct Batch {
     std::vector<Solver*> cav;
-    hipStream_t stream = nullptr, side = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr, t0 = nullptr, t1 = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
     std::vector<std::vector<Rec>> recs;
     std::vector<Group> plan;
     std::vector<void*> tables; // device tables of the plan
@@ -127,10 +127,9 @@ void build_plan(Batch& b) {
     for (int v : order) {
         Group g;
         g.vt = static_cast<const OpVT*>(std::get<0>(labels[v]));
-        g.side = std::get<1>(labels[v]);
         for (const auto& m : members[v])
             g.m.push_back(m.second);
-        if (g.vt) {
+        {
             for (const Rec* r : g.m)
                 if (r->block.x != g.m[0]->block.x || r->block.y != g.m[0]->block.y ||
                     r->block.z != g.m[0]->block.z)
@@ -143,29 +142,11 @@ void build_plan(Batch& b) {
     // Capture the batched iteration.
     hipGraph_t graph = nullptr;
     SG_HIP(hipStreamBeginCapture(b.stream, hipStreamCaptureModeRelaxed));
-    bool pending = false;
     hipError_t err = hipSuccess;
     for (const Group& g : b.plan) {
-        if (!g.vt) {
-            if (pending)
-                err = err ? err : hipStreamWaitEvent(b.stream, b.join, 0);
-            pending = false;
-            continue;
-        }
-        if (g.side) {
-            err = err ? err : hipEventRecord(b.fork, b.stream);
-            err = err ? err : hipStreamWaitEvent(b.side, b.fork, 0);
-            g.vt->launch(g, b.side);
-            err = err ? err : hipGetLastError();
-            err = err ? err : hipEventRecord(b.join, b.side);
-            pending = true;
-        } else {
-            g.vt->launch(g, b.stream);
-            err = err ? err : hipGetLastError();
-        }
+        g.vt->launch(g, b.stream);
+        err = err ? err : hipGetLastError();
     }
-    if (pending)
-        err = err ? err : hipStreamWaitEvent(b.stream, b.join, 0);
     const hipError_t e2 = hipStreamEndCapture(b.stream, &graph);
     SG_HIP(err);
     SG_HIP(e2);
@@ -353,11 +334,9 @@ void destroy_batch(Batch* b) {
         (void)hipFree(q);
     for (Solver* s : b->cav)
         destroy_solver(s);
-    for (hipEvent_t e : {b->fork, b->join, b->t0, b->t1})
+    for (hipEvent_t e : {b->t0, b->t1})
         if (e)
             (void)hipEventDestroy(e);
-    if (b->side)
-        (void)hipStreamDestroy(b->side);
     if (b->stream)
         (void)hipStreamDestroy(b->stream);
     delete b;
@@ -402,9 +381,6 @@ int sg_batch_create(const sg_params* ps, int n, sg_batch** out) {
         SG_HIP(hipGetDevice(&dev));
         SG_HIP(hipDeviceGetAttribute(&b.cus, hipDeviceAttributeMultiprocessorCount, dev));
         SG_HIP(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
-        SG_HIP(hipStreamCreateWithFlags(&b.side, hipStreamNonBlocking));
-        SG_HIP(hipEventCreateWithFlags(&b.fork, hipEventDisableTiming));
-        SG_HIP(hipEventCreateWithFlags(&b.join, hipEventDisableTiming));
         SG_HIP(hipEventCreate(&b.t0));
         SG_HIP(hipEventCreate(&b.t1));
         b.done.assign(n, 0);

@@ -86,6 +86,13 @@ enum {
 
 struct Warm;
 
+// An ordered sum of a compact nrow x w cell buffer into *slot, enqueued later (flush_sums).
+struct DeferredSum {
+    const double* cell; // null: empty range (the sum is 0.0)
+    int nrow, w, lane;
+    double* slot;
+};
+
 struct Solver {
     sg_params p{};
     int nr = 0, nc = 0, ld = 0;
@@ -96,9 +103,7 @@ struct Solver {
     double* scratch = nullptr;         // Jacobi previous iterate
     double* cell = nullptr;            // per-cell values for ordered sums (4 planes of n)
     double* rowbuf = nullptr;          // row sums, one nr-slice per ordered-sum lane (4 lanes)
-    double* rcell[3] = {};             // |R| cells of res_u, res_v, res_p (summed on the side stream)
-    hipStream_t side = nullptr;        // side stream: the residual sums, off the critical path
-    hipEvent_t fork[3] = {}, join[3] = {};
+    double* rcell[3] = {};             // |R| cells of res_u, res_v, res_p (deferred sums)
     double* scal = nullptr;            // S_COUNT device scalars
     double* cg[4] = {};                // CG r, z, d, q
     Ctl* ctl = nullptr;                // device control block
@@ -130,15 +135,15 @@ struct Solver {
     int tag = 0;
     int vcount = 0; // V-cycles enqueued in the current p' solve
     Warm* warm = nullptr; // learned warm start of the p' solve (sg_unet.hip)
+    std::vector<DeferredSum> deferred; // residual sums waiting for flush_sums
 
     const double* F(int id) const { return f[id]; }
     double* F(int id) { return f[id]; }
 };
 
-// Launch body B over `grid` for this cavity on `st` (the main stream if null), or record it.
+// Launch body B over `grid` for this cavity on its stream, or record it.
 template <class A, void (*B)(const A&, const Blk&), int NT>
-void emit(Solver& s, dim3 grid, dim3 block, size_t lds, const A& a, const int* stop,
-          hipStream_t st = nullptr) {
+void emit(Solver& s, dim3 grid, dim3 block, size_t lds, const A& a, const int* stop) {
     if (grid.x == 0 || grid.y == 0 || grid.z == 0)
         return;
     if (s.rec) {
@@ -150,14 +155,13 @@ void emit(Solver& s, dim3 grid, dim3 block, size_t lds, const A& a, const int* s
         r.block = block;
         r.lds = lds;
         r.stop = stop;
-        r.side = st != nullptr && st != s.stream;
         r.tag = s.tag;
         for (const Rec& q : *s.rec)
-            r.occ += (q.vt == r.vt && q.side == r.side && q.tag == r.tag) ? 1 : 0;
+            r.occ += (q.vt == r.vt && q.tag == r.tag) ? 1 : 0;
         s.rec->push_back(std::move(r));
         return;
     }
-    hipLaunchKernelGGL((k_one<A, B, NT>), grid, block, lds, st ? st : s.stream, a, stop);
+    hipLaunchKernelGGL((k_one<A, B, NT>), grid, block, lds, s.stream, a, stop);
     SG_HIP(hipGetLastError());
 }
 
@@ -165,9 +169,9 @@ void emit(Solver& s, dim3 grid, dim3 block, size_t lds, const A& a, const int* s
 // momentum (sg_momentum.hip)
 void launch_assemble(Solver& s, bool is_u, const int* stop);
 void launch_bc(Solver& s, bool is_u, const int* stop);
-void launch_mom_residual(Solver& s, bool is_u, double* slot, const int* stop, bool side);
+void launch_mom_residual(Solver& s, bool is_u, double* slot, const int* stop, bool defer);
 void launch_mom_sweeps(Solver& s, bool is_u, int nsweeps, const int* stop, double* res_slot,
-                       bool side);
+                       bool defer);
 void launch_mom_sweep_once(Solver& s, bool is_u, const double* src, double* dst, bool first,
                            const int* stop);
 int ncolours(const sg_params& p);
@@ -175,21 +179,19 @@ void launch_copy(Solver& s, const double* a, double* b, size_t n, const int* sto
 Pack make_pack(const sg_params& p, bool is_u);
 void sync_packed(Solver& s, int field_id, bool to_packed);
 // pressure (sg_pressure.hip)
-void launch_assemble_p(Solver& s, double* slot, const int* stop, bool side);
+void launch_assemble_p(Solver& s, double* slot, const int* stop, bool defer);
 void launch_solve_p(Solver& s, const int* stop, bool eager);
 void launch_correct(Solver& s, const int* stop);
 void launch_fill(Solver& s, double* a, size_t n, double v, const int* stop);
 // ordered sums (sg_reduce.hip): rows [i0, i1], columns [j0, j1] of a pitched array, row sums j
 // ascending from 0.0 into rowbuf, then rows ascending from 0.0 into *slot.
-// lane: which row-sum slice (0 on the main stream; 1..3 on the side stream); the sum
-// is enqueued on `stream` (the main stream if null).
+// lane: which row-sum slice (0 for immediate sums; 1..3 for the deferred residual sums).
 void launch_ordered_sum(Solver& s, const double* a, int pitch, int i0, int i1, int j0, int j1,
-                        double* slot, const int* stop, int lane = 0, hipStream_t stream = nullptr);
-// Enqueue an ordered sum of a compact nrow x w cell buffer on the side stream (lane 1..3), forked
-// from and later joined (join_side) into the main stream.
-void side_sum(Solver& s, int lane, const double* cell, int nrow, int w, double* slot,
-              const int* stop);
-void join_side(Solver& s);
+                        double* slot, const int* stop, int lane = 0);
+// Record an ordered sum of a compact nrow x w cell buffer (row-sum lane 1..3) for flush_sums.
+void defer_sum(Solver& s, int lane, const double* cell, int nrow, int w, double* slot);
+// Enqueue every deferred sum: one launch for all their rows, one for their totals.
+void flush_sums(Solver& s, const int* stop);
 // multigrid (sg_mg.hip)
 void mg_setup(Solver& s);
 void launch_mg_solve(Solver& s, const int* stop, bool has_guess);

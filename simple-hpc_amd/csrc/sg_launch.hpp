@@ -104,22 +104,20 @@ struct OpVT {
     void (*launch)(const Group& g, hipStream_t st);
 };
 
-// One recorded launch. vt == nullptr: a join marker (the main stream waits for the side stream).
+// One recorded launch.
 struct Rec {
     const OpVT* vt = nullptr;
     std::vector<unsigned char> args;
     dim3 grid, block;
     size_t lds = 0;
     const int* stop = nullptr;
-    bool side = false;
     int tag = 0; // structural label set by the enqueuing code (multigrid level and cycle)
-    int occ = 0; // occurrence of (vt, side, tag) in the cavity's record list
+    int occ = 0; // occurrence of (vt, tag) in the cavity's record list
 };
 
 // A batched launch: the records of several cavities with one label.
 struct Group {
     const OpVT* vt = nullptr;
-    bool side = false;
     std::vector<const Rec*> m;
     void* tab = nullptr;
     Work* work = nullptr;

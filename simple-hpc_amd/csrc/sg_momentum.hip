@@ -461,16 +461,16 @@ void launch_bc(Solver& s, bool is_u, const int* stop) {
 }
 
 // residual_u / residual_v (src/momentum_u.cpp:43-59): |R| per cell, then the ordered sum into
-// *slot (on the side stream if side).
-void launch_mom_residual(Solver& s, bool is_u, double* slot, const int* stop, bool side) {
+// *slot (deferred to flush_sums if defer).
+void launch_mom_residual(Solver& s, bool is_u, double* slot, const int* stop, bool defer) {
     const MomArgs m = mom_args(s, is_u);
     const int nrow = m.i1 - m.i0 + 1, w = m.j1 - m.j0 + 1;
     double* cell = s.rcell[is_u ? 0 : 1];
     if (nrow > 0 && w > 0)
         emit<ResArgs, k_mom_residual, BX * BY>(s, dim3(cdiv(w, BX), cdiv(nrow, BY)), dim3(BX, BY),
                                                0, ResArgs{m, cell}, stop);
-    if (side)
-        side_sum(s, is_u ? 1 : 2, cell, nrow, w, slot, stop);
+    if (defer)
+        defer_sum(s, is_u ? 1 : 2, cell, nrow, w, slot);
     else if (nrow <= 0 || w <= 0)
         launch_ordered_sum(s, cell, 1, 0, -1, 0, -1, slot, stop); // empty range: 0.0
     else
@@ -525,14 +525,14 @@ void launch_mom_sweep_once(Solver& s, bool is_u, const double* src, double* dst,
 
 // sweep_u_batch / sweep_v_batch (src/momentum_u.cpp:164-171) for the two parallel solvers. With
 // res_slot (multicolour only), the residual is evaluated inside the first sweep and summed (on
-// the side stream if side) into *res_slot.
+// flush_sums if defer) into *res_slot.
 void launch_mom_sweeps(Solver& s, bool is_u, int nsweeps, const int* stop, double* res_slot,
-                       bool side) {
+                       bool defer) {
     const sg_params& p = s.p;
     const MomArgs m = mom_args(s, is_u);
     const bool fuse = res_slot && p.momentum_solver == SG_MOM_GS_MULTICOLOUR && nsweeps > 0;
     if (res_slot && !fuse)
-        launch_mom_residual(s, is_u, res_slot, stop, side);
+        launch_mom_residual(s, is_u, res_slot, stop, defer);
     if (nsweeps <= 0)
         return;
     if (p.momentum_solver == SG_MOM_GS_MULTICOLOUR) {
@@ -546,8 +546,8 @@ void launch_mom_sweeps(Solver& s, bool is_u, int nsweeps, const int* stop, doubl
             launch_copy(s, s.scratch, m.x, s.n, stop);
         if (fuse) {
             const int nrow = m.i1 - m.i0 + 1, w = m.j1 - m.j0 + 1;
-            if (side)
-                side_sum(s, is_u ? 1 : 2, s.rcell[is_u ? 0 : 1], nrow, w, res_slot, stop);
+            if (defer)
+                defer_sum(s, is_u ? 1 : 2, s.rcell[is_u ? 0 : 1], nrow, w, res_slot);
             else
                 launch_ordered_sum(s, s.rcell[is_u ? 0 : 1], w, 0, nrow - 1, 0, w - 1, res_slot,
                                    stop);

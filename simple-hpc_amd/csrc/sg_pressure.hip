@@ -354,7 +354,7 @@ void launch_fill(Solver& s, double* a, size_t n, double v, const int* stop) {
                                FillArgs{a, n, v}, stop);
 }
 
-void launch_assemble_p(Solver& s, double* slot, const int* stop, bool side) {
+void launch_assemble_p(Solver& s, double* slot, const int* stop, bool defer) {
     const sg_params& p = s.p;
     const PArgs a = p_args(s);
     emit<WallArgs, k_p_walls, WT>(s, dim3(cdiv(std::max(s.nr, s.nc), WT)), dim3(WT), 0,
@@ -363,8 +363,8 @@ void launch_assemble_p(Solver& s, double* slot, const int* stop, bool side) {
                                         ApArgs{a, s.F(SG_F_CU_AP), s.F(SG_F_CV_AP), s.F(SG_F_U),
                                                s.F(SG_F_V), p.dx, p.dy, s.rcell[2]},
                                         stop);
-    if (side)
-        side_sum(s, 3, s.rcell[2], p.NX, p.NY, slot, stop);
+    if (defer)
+        defer_sum(s, 3, s.rcell[2], p.NX, p.NY, slot);
     else
         launch_ordered_sum(s, s.rcell[2], p.NY, 0, p.NX - 1, 0, p.NY - 1, slot, stop);
 }

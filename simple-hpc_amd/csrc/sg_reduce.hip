@@ -9,6 +9,8 @@
 // per row walks its row in order from LDS (reads in batches of 8 ahead of the adds), so each
 // chain waits on FP64 adds rather than memory. Total: one workgroup loads the row sums into LDS
 // the same way and one lane adds them in row order. At 1024 x 1024 each launch takes about 14 us.
+// The residual sums, which feed only the stopping rule, are deferred and done together (one launch
+// for the rows of all three, one for the totals) just before it.
 // (A single launch that hands the row sums to the last workgroup through a ticket needs a
 // device-scope fence in every workgroup; measured at the same size it took 34-48 us.)
 #include "sg_internal.hpp"
@@ -49,9 +51,10 @@ struct SumArgs {
     double* slot;
 };
 
-__device__ __forceinline__ void k_row_sums(const SumArgs& sa, const Blk& bk) {
+// Row sums of rows [RW*blk, RW*blk + RW) of one sum.
+__device__ __forceinline__ void row_sums_block(const SumArgs& sa, int blk) {
     __shared__ double tile[RW][CW + PAD];
-    const int r0 = bk.x * RW, t = threadIdx.x;
+    const int r0 = blk * RW, t = threadIdx.x;
     double s = 0.0;
     for (int c0 = 0; c0 < sa.w; c0 += CW) {
         const int cw = min(CW, sa.w - c0);
@@ -77,7 +80,8 @@ __device__ __forceinline__ void k_row_sums(const SumArgs& sa, const Blk& bk) {
         sa.rows[r0 + t] = s;
 }
 
-__device__ __forceinline__ void k_rows_total(const SumArgs& sa, const Blk&) {
+// The row sums of one sum added in row order into its slot.
+__device__ __forceinline__ void rows_total(const SumArgs& sa) {
     __shared__ double buf[TCH];
     const int t = threadIdx.x;
     double s = 0.0;
@@ -94,6 +98,31 @@ __device__ __forceinline__ void k_rows_total(const SumArgs& sa, const Blk&) {
         *sa.slot = s;
 }
 
+__device__ __forceinline__ void k_row_sums(const SumArgs& sa, const Blk& bk) {
+    row_sums_block(sa, bk.x);
+}
+
+__device__ __forceinline__ void k_rows_total(const SumArgs& sa, const Blk&) { rows_total(sa); }
+
+// Up to 3 independent sums in one launch each for the rows and the totals.
+constexpr int MAXJ = 3;
+struct MultiArgs {
+    SumArgs j[MAXJ];
+    int first[MAXJ + 1]; // first row block of each sum
+    int n;
+};
+
+__device__ __forceinline__ void k_row_sums_multi(const MultiArgs& ma, const Blk& bk) {
+    int t = 0;
+    while (t + 1 < ma.n && bk.x >= ma.first[t + 1])
+        ++t;
+    row_sums_block(ma.j[t], bk.x - ma.first[t]);
+}
+
+__device__ __forceinline__ void k_rows_total_multi(const MultiArgs& ma, const Blk& bk) {
+    rows_total(ma.j[bk.x]);
+}
+
 struct SlotArgs {
     double* slot;
 };
@@ -106,42 +135,49 @@ __device__ __forceinline__ void k_zero_slot(const SlotArgs& a, const Blk&) {
 } // namespace
 
 void launch_ordered_sum(Solver& s, const double* a, int pitch, int i0, int i1, int j0, int j1,
-                        double* slot, const int* stop, int lane, hipStream_t stream) {
+                        double* slot, const int* stop, int lane) {
     const int nrow = i1 - i0 + 1, w = j1 - j0 + 1;
-    hipStream_t st = stream ? stream : s.stream;
     if (nrow <= 0 || w <= 0) {
-        emit<SlotArgs, k_zero_slot, 64>(s, dim3(1), dim3(64), 0, SlotArgs{slot}, stop, st);
+        emit<SlotArgs, k_zero_slot, 64>(s, dim3(1), dim3(64), 0, SlotArgs{slot}, stop);
     } else {
         const SumArgs sa{a, static_cast<long>(pitch), i0, nrow, j0, w,
                          s.rowbuf + static_cast<size_t>(lane) * s.nr, slot};
-        emit<SumArgs, k_row_sums, NT>(s, dim3(cdiv(nrow, RW)), dim3(NT), 0, sa, stop, st);
-        emit<SumArgs, k_rows_total, NT>(s, dim3(1), dim3(NT), 0, sa, stop, st);
+        emit<SumArgs, k_row_sums, NT>(s, dim3(cdiv(nrow, RW)), dim3(NT), 0, sa, stop);
+        emit<SumArgs, k_rows_total, NT>(s, dim3(1), dim3(NT), 0, sa, stop);
     }
 }
 
-void side_sum(Solver& s, int lane, const double* cell, int nrow, int w, double* slot,
-              const int* stop) {
-    if (s.rec) { // recorded as a side-stream launch; the batch executor forks and joins
-        launch_ordered_sum(s, cell, w > 0 ? w : 1, 0, nrow - 1, 0, w - 1, slot, stop, lane, s.side);
+void defer_sum(Solver& s, int lane, const double* cell, int nrow, int w, double* slot) {
+    if (nrow <= 0 || w <= 0) {
+        s.deferred.push_back(DeferredSum{nullptr, 0, 0, 0, slot});
         return;
     }
-    SG_HIP(hipEventRecord(s.fork[lane - 1], s.stream));
-    SG_HIP(hipStreamWaitEvent(s.side, s.fork[lane - 1], 0));
-    launch_ordered_sum(s, cell, w > 0 ? w : 1, 0, nrow - 1, 0, w - 1, slot, stop, lane, s.side);
-    SG_HIP(hipEventRecord(s.join[lane - 1], s.side));
+    s.deferred.push_back(DeferredSum{cell, nrow, w, lane, slot});
 }
 
-void join_side(Solver& s) {
-    if (s.rec) {
-        Rec r; // join marker
-        r.tag = s.tag;
-        for (const Rec& q : *s.rec)
-            r.occ += (q.vt == nullptr && q.tag == r.tag) ? 1 : 0;
-        s.rec->push_back(std::move(r));
-        return;
+void flush_sums(Solver& s, const int* stop) {
+    std::vector<DeferredSum> d;
+    d.swap(s.deferred);
+    MultiArgs ma{};
+    int blocks = 0;
+    for (const DeferredSum& x : d) {
+        if (!x.cell) { // empty range: the sum is 0.0
+            emit<SlotArgs, k_zero_slot, 64>(s, dim3(1), dim3(64), 0, SlotArgs{x.slot}, stop);
+            continue;
+        }
+        if (ma.n == MAXJ)
+            throw ArgError("too many deferred sums");
+        ma.j[ma.n] = SumArgs{x.cell, static_cast<long>(x.w), 0, x.nrow, 0, x.w,
+                             s.rowbuf + static_cast<size_t>(x.lane) * s.nr, x.slot};
+        ma.first[ma.n] = blocks;
+        blocks += static_cast<int>(cdiv(x.nrow, RW));
+        ++ma.n;
     }
-    for (int k = 0; k < 3; ++k)
-        SG_HIP(hipStreamWaitEvent(s.stream, s.join[k], 0));
+    if (ma.n == 0)
+        return;
+    ma.first[ma.n] = blocks;
+    emit<MultiArgs, k_row_sums_multi, NT>(s, dim3(blocks), dim3(NT), 0, ma, stop);
+    emit<MultiArgs, k_rows_total_multi, NT>(s, dim3(ma.n), dim3(NT), 0, ma, stop);
 }
 
 } // namespace sg
