@@ -31,9 +31,19 @@ constexpr int BX = 64, BY = 4;
 constexpr int MAXLEV = 24;
 
 template <class T> struct Lv {
+    using value_type = T;
     int nx, ny, stride;
     T *AE, *AW, *AN, *AS, *AP, *b, *x, *r;
     T* xt; // pre-smoothed iterate of the tiled V-cycle
+};
+
+// A level held in LDS, addressed through LDS pointers, so its accesses compile to LDS
+// instructions (through a generic pointer they would be flat accesses).
+template <class T> using Lp = __attribute__((address_space(3))) T*;
+template <class T> struct LLv {
+    using value_type = T;
+    int nx, ny, stride;
+    Lp<T> AE, AW, AN, AS, AP, b, x, r;
 };
 
 Lv<double> lv64(const MgLevel& L) {
@@ -53,8 +63,9 @@ template <> __device__ constexpr float coarse_tol<float>() { return 1e-6f; }
 
 // Coarse links: sums of the two fine links crossing each coarse face; AP = ae + aw + an + as
 // (mg_build, :232-261).
-template <class T>
-__device__ __forceinline__ void build_cell(const Lv<T>& F, const Lv<T>& C, int I, int J) {
+template <class LF, class LC>
+__device__ __forceinline__ void build_cell(const LF& F, const LC& C, int I, int J) {
+    using T = typename LF::value_type;
     const long fs = F.stride;
     const size_t f = 2 * I * fs + 2 * J, c = static_cast<size_t>(I) * C.stride + J;
     const T ae = F.AE[f + 1] + F.AE[f + fs + 1];
@@ -69,9 +80,10 @@ __device__ __forceinline__ void build_cell(const Lv<T>& F, const Lv<T>& C, int I
 }
 
 // Two-statement smoother update (smooth, :62-64).
-template <class T> __device__ __forceinline__ void smooth_cell(const Lv<T>& L, size_t q) {
+template <class LV> __device__ __forceinline__ void smooth_cell(const LV& L, size_t q) {
+    using T = typename LV::value_type;
     const long s = L.stride;
-    const T* x = L.x;
+    const auto x = L.x;
     T t = L.b[q] / L.AP[q];
     t += (L.AE[q] * x[q + 1] + L.AW[q] * x[q - 1] + L.AS[q] * x[q + s] + L.AN[q] * x[q - s]) /
          L.AP[q];
@@ -79,9 +91,10 @@ template <class T> __device__ __forceinline__ void smooth_cell(const Lv<T>& L, s
 }
 
 // b - (AP x - (E + W + S + N)) (residual, :83-84).
-template <class T> __device__ __forceinline__ T resid_cell(const Lv<T>& L, size_t q) {
+template <class LV>
+__device__ __forceinline__ typename LV::value_type resid_cell(const LV& L, size_t q) {
     const long s = L.stride;
-    const T* x = L.x;
+    const auto x = L.x;
     return L.b[q] - (L.AP[q] * x[q] - (L.AE[q] * x[q + 1] + L.AW[q] * x[q - 1] +
                                        L.AS[q] * x[q + s] + L.AN[q] * x[q - s]));
 }
@@ -352,15 +365,14 @@ template <class T> struct Tail {
     Ctl* ctl;        // receives the coarse-solve sweep count
 };
 
-template <class T> __device__ __forceinline__ Lv<T> lds_view(const Tail<T>& t, T* sm, int l) {
+template <class T> __device__ __forceinline__ LLv<T> lds_view(const Tail<T>& t, Lp<T> sm, int l) {
     const int nx = t.G[l].nx, ny = t.G[l].ny, st = ny + 2;
     const size_t n = static_cast<size_t>(nx + 2) * st;
-    T* b = sm + t.off[l] + st + 1;
-    return {nx, ny, st, b, b + n, b + 2 * n, b + 3 * n, b + 4 * n, b + 5 * n, b + 6 * n, b + 7 * n,
-            nullptr};
+    Lp<T> b = sm + t.off[l] + st + 1;
+    return {nx, ny, st, b, b + n, b + 2 * n, b + 3 * n, b + 4 * n, b + 5 * n, b + 6 * n, b + 7 * n};
 }
 
-template <class T> __device__ void wg_smooth(const Lv<T>& L, int sweeps) {
+template <class LV> __device__ void wg_smooth(const LV& L, int sweeps) {
     const int half = (L.ny + 1) / 2, n = L.nx * half;
     for (int k = 0; k < sweeps; ++k)
         for (int c = 0; c < 2; ++c) {
@@ -374,7 +386,7 @@ template <class T> __device__ void wg_smooth(const Lv<T>& L, int sweeps) {
         }
 }
 
-template <class T> __device__ void wg_resid(const Lv<T>& L) {
+template <class LV> __device__ void wg_resid(const LV& L) {
     for (int e = threadIdx.x; e < L.nx * L.ny; e += blockDim.x) {
         const int I = e / L.ny, J = e - I * L.ny;
         const size_t q = static_cast<size_t>(I) * L.stride + J;
@@ -385,7 +397,8 @@ template <class T> __device__ void wg_resid(const Lv<T>& L) {
 
 // det_sum (:18-34): each row by one thread (J ascending from 0.0), then the rows by thread 0
 // (I ascending from 0.0). which: 0 b, 1 b*b, 2 r*r, 3 x.
-template <class T> __device__ T wg_det_sum(const Lv<T>& L, T* rows, int which) {
+template <class LV, class T = typename LV::value_type>
+__device__ T wg_det_sum(const LV& L, T* rows, int which) {
     __shared__ T total;
     for (int I = threadIdx.x; I < L.nx; I += blockDim.x) {
         T s = T(0);
@@ -411,7 +424,8 @@ template <class T> __device__ T wg_det_sum(const Lv<T>& L, T* rows, int which) {
     return v;
 }
 
-template <class T> __device__ void wg_sub(const Lv<T>& L, T* x, T m) {
+template <class LV, class P, class T = typename LV::value_type>
+__device__ void wg_sub(const LV& L, P x, T m) {
     for (int e = threadIdx.x; e < L.nx * L.ny; e += blockDim.x) {
         const int I = e / L.ny, J = e - I * L.ny;
         x[static_cast<size_t>(I) * L.stride + J] -= m;
@@ -421,7 +435,8 @@ template <class T> __device__ void wg_sub(const Lv<T>& L, T* x, T m) {
 
 // coarse_solve (:120-140): mean-free b, red-black sweeps until ||r|| < tol ||b|| or 200 sweeps,
 // mean-free x.
-template <class T> __device__ int wg_coarse_solve(const Lv<T>& L, T* rows) {
+template <class LV, class T = typename LV::value_type>
+__device__ int wg_coarse_solve(const LV& L, T* rows) {
     const T ncell = static_cast<T>(static_cast<double>(L.nx) * L.ny);
     wg_sub(L, L.b, wg_det_sum(L, rows, 0) / ncell);
     const T bb = wg_det_sum(L, rows, 1);
@@ -457,7 +472,8 @@ template <> __device__ __forceinline__ float lane_val<float>(float v, int l) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 
-template <class T> __device__ int wave_coarse_solve(const Lv<T>& L) {
+template <class LV> __device__ int wave_coarse_solve(const LV& L) {
+    using T = typename LV::value_type;
     __shared__ int nsweeps;
     if (threadIdx.x < 64) {
         const int lane = threadIdx.x, nx = L.nx, ny = L.ny;
@@ -536,13 +552,113 @@ template <class T> __device__ int wave_coarse_solve(const Lv<T>& L) {
     return nsweeps;
 }
 
-template <class T> __device__ void wg_build(const Lv<T>& F, const Lv<T>& C) {
+// The coarse solve for a level of at most 16 cells with NY columns (one 16-lane DPP row): the
+// neighbours by DPP row shifts (row_shl / row_shr by 1 and by NY; lanes whose source lies outside
+// the row read 0, and wall neighbours are zero anyway), and every ordered sum from the values read
+// out of their lanes first, then added in det_sum's order. Same expressions in the same order as
+// wave_coarse_solve.
+template <int CTRL> __device__ __forceinline__ double dpp_val(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(b), CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(b >> 32), CTRL, 0xF, 0xF, true);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
+template <int CTRL> __device__ __forceinline__ float dpp_val(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+
+template <int NY, class LV> __device__ int wave_coarse_solve16(const LV& L) {
+    using T = typename LV::value_type;
+    __shared__ int nsweeps;
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x, nx = L.nx;
+        const int nc = nx * NY;
+        const bool act = lane < nc;
+        const int I = act ? lane / NY : 0, J = act ? lane % NY : 0;
+        const size_t q = static_cast<size_t>(I) * L.stride + J;
+        T ae = T(0), aw = T(0), an = T(0), as = T(0), ap = T(1), b = T(0), x = T(0);
+        if (act) {
+            ae = L.AE[q];
+            aw = L.AW[q];
+            an = L.AN[q];
+            as = L.AS[q];
+            ap = L.AP[q];
+            b = L.b[q];
+            x = L.x[q];
+        }
+        const bool hasE = act && J + 1 < NY, hasW = act && J > 0;
+        const bool hasS = act && I + 1 < nx, hasN = act && I > 0;
+        const int col = (I + J) & 1;
+        auto dsum = [&](T v) {
+            T vals[16];
+#pragma unroll
+            for (int k = 0; k < 16; ++k)
+                vals[k] = lane_val<T>(v, k);
+            T tot = T(0);
+            for (int i = 0; i < nx; ++i) {
+                T r = T(0);
+#pragma unroll
+                for (int j = 0; j < NY; ++j)
+                    r += vals[(i * NY + j) & 15];
+                tot += r;
+            }
+            return tot;
+        };
+        auto nbs = [&](T v, T& xe, T& xw, T& xs, T& xn) {
+            const T e = dpp_val<0x101>(v), w = dpp_val<0x111>(v);   // row_shl:1, row_shr:1
+            const T sv = dpp_val<0x100 + NY>(v), nv = dpp_val<0x110 + NY>(v); // by NY
+            xe = hasE ? e : T(0);
+            xw = hasW ? w : T(0);
+            xs = hasS ? sv : T(0);
+            xn = hasN ? nv : T(0);
+        };
+        const T ncell = static_cast<T>(static_cast<double>(nx) * NY);
+        const T mb = dsum(b) / ncell;
+        if (act)
+            b -= mb;
+        const T bb = dsum(b * b);
+        int sweeps = 0;
+        if (bb > T(0)) {
+            const T bn = sqrt(bb);
+            while (sweeps < 200) {
+                for (int c = 0; c < 2; ++c) {
+                    T xe, xw, xs, xn;
+                    nbs(x, xe, xw, xs, xn);
+                    if (act && col == c) {
+                        T t = b / ap;
+                        t += (ae * xe + aw * xw + as * xs + an * xn) / ap;
+                        x = t;
+                    }
+                }
+                ++sweeps;
+                T xe, xw, xs, xn;
+                nbs(x, xe, xw, xs, xn);
+                const T r = act ? b - (ap * x - (ae * xe + aw * xw + as * xs + an * xn)) : T(0);
+                const T rr = dsum(r * r);
+                if (sqrt(rr) < coarse_tol<T>() * bn)
+                    break;
+            }
+        }
+        const T mx = dsum(x) / ncell;
+        if (act) {
+            x -= mx;
+            L.x[q] = x;
+            L.b[q] = b;
+        }
+        if (lane == 0)
+            nsweeps = sweeps;
+    }
+    __syncthreads();
+    return nsweeps;
+}
+
+template <class LF, class LC> __device__ void wg_build(const LF& F, const LC& C) {
     for (int e = threadIdx.x; e < C.nx * C.ny; e += blockDim.x)
         build_cell(F, C, e / C.ny, e % C.ny);
     __syncthreads();
 }
 
-template <class T> __device__ void wg_copy_level(const Lv<T>& D, const Lv<T>& S, bool coefs) {
+template <class LD, class LS> __device__ void wg_copy_level(const LD& D, const LS& S, bool coefs) {
     for (int e = threadIdx.x; e < D.nx * D.ny; e += blockDim.x) {
         const int I = e / D.ny, J = e - I * D.ny;
         const size_t d = static_cast<size_t>(I) * D.stride + J, g = static_cast<size_t>(I) * S.stride + J;
@@ -566,49 +682,75 @@ template <class T> __device__ __forceinline__ void k_mg_tail(const Tail<T>& t, c
         sm[e] = T(0);
     __syncthreads();
     T* rows = t.t0 < t.nl ? sm + t.lds_elems : t.grows;
-    // Level views are rebuilt where needed (a local array indexed by the level would live in
-    // scratch memory).
-    auto V = [&](int l) { return l < t.t0 ? t.G[l] : lds_view(t, sm, l); };
+    const Lp<T> lsm = (Lp<T>)sm;
+    // Run f on level l's view: global levels through generic pointers, LDS levels through LDS
+    // pointers (the combinations are separate instantiations).
+    auto with = [&](int l, auto&& f) {
+        if (l < t.t0)
+            f(t.G[l]);
+        else
+            f(lds_view(t, lsm, l));
+    };
     // Operators: global levels from their (global) parents, LDS levels from theirs; level 0 in
     // LDS is loaded.
     for (int l = t.tm; l < t.nl; ++l) {
         if (l == 0) {
             if (t.t0 == 0)
-                wg_copy_level(V(0), t.G[0], true);
+                with(0, [&](const auto& D) { wg_copy_level(D, t.G[0], true); });
             continue;
         }
-        wg_build(l == t.tm ? t.G[l - 1] : V(l - 1), V(l));
+        with(l, [&](const auto& C) {
+            if (l == t.tm)
+                wg_build(t.G[l - 1], C);
+            else
+                with(l - 1, [&](const auto& F) { wg_build(F, C); });
+        });
     }
-    if (t.tm == t.t0 && t.t0 > 0)
-        wg_copy_level(V(t.tm), t.G[t.tm], false); // b and x of the first level
+    if (t.tm == t.t0 && t.t0 > 0) // b and x of the first level
+        with(t.tm, [&](const auto& D) { wg_copy_level(D, t.G[t.tm], false); });
     for (int l = t.tm; l + 1 < t.nl; ++l) {
-        const Lv<T> F = V(l), C = V(l + 1);
-        wg_smooth(F, t.pre);
-        wg_resid(F);
-        for (int e = threadIdx.x; e < C.nx * C.ny; e += blockDim.x) {
-            const int I = e / C.ny, J = e - I * C.ny;
-            const long fs = F.stride;
-            const size_t f = 2 * I * fs + 2 * J, c = static_cast<size_t>(I) * C.stride + J;
-            C.b[c] = F.r[f] + F.r[f + 1] + F.r[f + fs] + F.r[f + fs + 1];
-            C.x[c] = T(0);
-        }
-        __syncthreads();
+        with(l, [&](const auto& F) {
+            with(l + 1, [&](const auto& C) {
+                wg_smooth(F, t.pre);
+                wg_resid(F);
+                for (int e = threadIdx.x; e < C.nx * C.ny; e += blockDim.x) {
+                    const int I = e / C.ny, J = e - I * C.ny;
+                    const long fs = F.stride;
+                    const size_t f = 2 * I * fs + 2 * J, c = static_cast<size_t>(I) * C.stride + J;
+                    C.b[c] = F.r[f] + F.r[f + 1] + F.r[f + fs] + F.r[f + fs + 1];
+                    C.x[c] = T(0);
+                }
+                __syncthreads();
+            });
+        });
     }
-    const Lv<T> Lc = V(t.nl - 1);
-    const int sweeps = Lc.nx * Lc.ny <= 64 ? wave_coarse_solve(Lc) : wg_coarse_solve(Lc, rows);
+    int sweeps = 0;
+    with(t.nl - 1, [&](const auto& Lc) {
+        const int n = Lc.nx * Lc.ny;
+        if (n <= 16 && Lc.ny == 4) // the coarsest level of every power-of-two grid: 4 x 4
+            sweeps = wave_coarse_solve16<4>(Lc);
+        else if (n <= 64)
+            sweeps = wave_coarse_solve(Lc);
+        else
+            sweeps = wg_coarse_solve(Lc, rows);
+    });
     for (int l = t.nl - 2; l >= t.tm; --l) {
-        const Lv<T> F = V(l), C = V(l + 1);
-        for (int e = threadIdx.x; e < F.nx * F.ny; e += blockDim.x) {
-            const int i = e / F.ny, j = e - i * F.ny;
-            F.x[static_cast<size_t>(i) * F.stride + j] +=
-                t.omega * C.x[static_cast<size_t>(i / 2) * C.stride + j / 2];
-        }
-        __syncthreads();
-        wg_smooth(F, t.post);
+        with(l, [&](const auto& F) {
+            with(l + 1, [&](const auto& C) {
+                for (int e = threadIdx.x; e < F.nx * F.ny; e += blockDim.x) {
+                    const int i = e / F.ny, j = e - i * F.ny;
+                    F.x[static_cast<size_t>(i) * F.stride + j] +=
+                        t.omega * C.x[static_cast<size_t>(i / 2) * C.stride + j / 2];
+                }
+                __syncthreads();
+                wg_smooth(F, t.post);
+            });
+        });
     }
     if (t.tm == t.t0 && t.t0 < t.nl) {
         // The first level was in LDS: its x (and, if it is the coarsest, its mean-free b) back.
-        const Lv<T> D = V(t.tm), G0 = t.G[t.tm];
+        const LLv<T> D = lds_view(t, lsm, t.tm);
+        const Lv<T> G0 = t.G[t.tm];
         for (int e = threadIdx.x; e < D.nx * D.ny; e += blockDim.x) {
             const int I = e / D.ny, J = e - I * D.ny;
             const size_t g = static_cast<size_t>(I) * G0.stride + J,
