@@ -67,25 +67,18 @@ def log(msg):
 
 def run_form(sg, n, form, mode, steps, warmup, knobs=None):
     p, _ = sg.params_from_config(config_text(n, form))
-    if knobs:  # given launch shapes: "tile,threads,mid,lds" (skips the autotuner)
-        t, th, mid, lds = (int(x) for x in knobs.split(","))
-        tune = (f"best momentum tile: given ({t})\n"
-                f"best: tail_threads {th} mid_cells {mid} lds_cells {lds} : given\n")
+    if knobs:  # given launch shapes: "tile,threads,mid,lds[,staged]" (skips the autotuner)
+        tune = tuple(int(x) for x in knobs.split(","))
     else:
         s = sg.Solver(p)
         s.advance(3)
-        tune = s.autotune(reps=10)
-        log(f"{n}^2 {form}: autotune -> {tune.strip().splitlines()[-1]}")
+        rep = s.autotune(reps=10)
+        for line in rep.strip().splitlines():
+            log(f"{n}^2 {form}: autotune: {line}")
+        tune = sg.tuning_of(s)
         s.close()
     s = sg.Solver(p)  # fresh fields; the tuned launch shapes
-    for line in tune.strip().splitlines():
-        f = line.split()
-        if line.startswith("best momentum tile"):
-            s.set_tuning(sg.TUNE_MOM_TILE, int(f[-1].strip("()")))
-        elif line.startswith("best:"):
-            s.set_tuning(sg.TUNE_TAIL_THREADS, int(f[2]))
-            s.set_tuning(sg.TUNE_MID_CELLS, int(f[4]))
-            s.set_tuning(sg.TUNE_LDS_CELLS, int(f[6]))
+    sg.set_tunings(s, tune)
     if mode == "fixed":
         s.advance(warmup)
         log(f"{n}^2 {form}: timing {steps} outer iterations")
@@ -95,7 +88,7 @@ def run_form(sg, n, form, mode, steps, warmup, knobs=None):
         return s, {"form": form, "iterations": done, "seconds": t, "ms_per_iter": 1e3 * t / done,
                    "last_residuals": [float(x) for x in lg[-1]],
                    "coarse_sweeps_last": s.coarse_sweeps(),
-                   "autotune": [l for l in tune.strip().splitlines() if l.startswith("best")]}
+                   "knobs": list(tune)}
     log(f"{n}^2 {form}: to tolerance")
     t0 = time.perf_counter()
     total, conv, last = 0, False, None
@@ -112,7 +105,7 @@ def run_form(sg, n, form, mode, steps, warmup, knobs=None):
     t = time.perf_counter() - t0
     return s, {"form": form, "iterations": total, "seconds": t, "ms_per_iter": 1e3 * t / total,
                "converged": bool(conv), "last_residuals": [float(x) for x in last],
-               "autotune": [l for l in tune.strip().splitlines() if l.startswith("best")]}
+               "knobs": list(tune)}
 
 
 def cpu_baseline(sg, n, seconds=12.0):
@@ -146,21 +139,15 @@ def run_batch(sg, steps_cap, knobs=None, cpu=True):
         b = sg.Batch(plist)
         if tune is None:
             if knobs:
-                t, th, mid, lds = (int(x) for x in knobs.split(","))
-                tune = (t, th, mid, lds)
+                tune = tuple(int(x) for x in knobs.split(","))
             else:
                 b.advance(3)
                 rep = b.autotune(reps=5)
                 log("batch autotune -> " + rep.strip().splitlines()[-1])
-                f = [l.split() for l in rep.strip().splitlines()]
-                t = int([l for l in f if l[:2] == ["best", "momentum"]][0][-1].strip("()"))
-                last = f[-1]
-                tune = (t, int(last[2]), int(last[4]), int(last[6]))
+                tune = sg.tuning_of(b)
                 b.close()
                 b = sg.Batch(plist)
-        for key, v in zip((sg.TUNE_MOM_TILE, sg.TUNE_TAIL_THREADS, sg.TUNE_MID_CELLS,
-                           sg.TUNE_LDS_CELLS), tune):
-            b.set_tuning(key, v)
+        sg.set_tunings(b, tune)
         log(f"batch {form}: {len(cases)} cavities to tolerance "
             f"({b.launches()} launches per batched iteration)")
         b.advance(1)  # graph capture outside the timed region
@@ -242,10 +229,7 @@ def run_large(sg, steps, warmup, knobs=None):
         p, _ = sg.params_from_config(large_config(form))
         s = sg.Solver(p)
         if knobs:
-            t, th, mid, lds = (int(x) for x in knobs.split(","))
-            for key, v in zip((sg.TUNE_MOM_TILE, sg.TUNE_TAIL_THREADS, sg.TUNE_MID_CELLS,
-                               sg.TUNE_LDS_CELLS), (t, th, mid, lds)):
-                s.set_tuning(key, v)
+            sg.set_tunings(s, [int(x) for x in knobs.split(",")])
         s.advance(warmup)
         log(f"8192^2 {form}: timing {steps} outer iterations")
         t0 = time.perf_counter()

@@ -680,6 +680,7 @@ static long* knob(Solver& s, int key) {
     case SG_TUNE_LDS_CELLS: return &s.lds_cells;
     case SG_TUNE_TAIL_THREADS: tmp = s.tail_threads; return &tmp;
     case SG_TUNE_MOM_TILE: tmp = s.mom_tile; return &tmp;
+    case SG_TUNE_MG_STAGED: tmp = s.mg_staged; return &tmp;
     default: throw ArgError("unknown tuning key");
     }
 }
@@ -695,6 +696,10 @@ int sg_set_tuning(sg_solver* h, int key, long value) {
             if (value < 0 || value > 3)
                 throw ArgError("momentum tile shape must be 0..3");
             s.mom_tile = static_cast<int>(value);
+        } else if (key == SG_TUNE_MG_STAGED) {
+            if (value < 0 || value > 64)
+                throw ArgError("staged level must be 0..64");
+            s.mg_staged = static_cast<int>(value);
         } else {
             if (value < 0)
                 throw ArgError("tuning value must be >= 0");
@@ -806,6 +811,21 @@ int sg_autotune(sg_solver* h, int reps, char* report, size_t len) {
         std::snprintf(line, sizeof line,
                       "best: tail_threads %d mid_cells %ld lds_cells %ld : %.2f us\n", bt, bm, bl,
                       1e6 * best);
+        rep += line;
+        // Staged tiles from level 0, 1, 2 or none.
+        int bs = s.mg_staged;
+        for (int st : {0, 1, 2, 64}) {
+            s.mg_staged = st;
+            const double t = time_psolve(s, reps);
+            std::snprintf(line, sizeof line, "staged from level %2d : %9.2f us\n", st, 1e6 * t);
+            rep += line;
+            if (t < best) {
+                best = t;
+                bs = st;
+            }
+        }
+        s.mg_staged = bs;
+        std::snprintf(line, sizeof line, "best: staged from level %d : %.2f us\n", bs, 1e6 * best);
         rep += line;
         SG_HIP(hipMemcpyAsync(s.F(SG_F_B), kb, sizeof(double) * s.n, hipMemcpyDeviceToDevice, s.stream));
         SG_HIP(hipMemcpyAsync(s.F(SG_F_PC), kp, sizeof(double) * s.n, hipMemcpyDeviceToDevice, s.stream));

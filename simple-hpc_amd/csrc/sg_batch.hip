@@ -269,6 +269,11 @@ void set_knob(Batch& b, int key, long v) {
                 throw ArgError("momentum tile must be 0..3");
             s->mom_tile = static_cast<int>(v);
             break;
+        case SG_TUNE_MG_STAGED:
+            if (v < 0 || v > 64)
+                throw ArgError("staged level must be 0..64");
+            s->mg_staged = static_cast<int>(v);
+            break;
         case SG_TUNE_MID_CELLS:
         case SG_TUNE_LDS_CELLS:
             if (v < 0)
@@ -289,6 +294,8 @@ long get_knob(Batch& b, int key) {
         return s.tail_threads;
     case SG_TUNE_MOM_TILE:
         return s.mom_tile;
+    case SG_TUNE_MG_STAGED:
+        return s.mg_staged;
     case SG_TUNE_MID_CELLS:
         return s.mid_cells;
     case SG_TUNE_LDS_CELLS:
@@ -521,6 +528,20 @@ int sg_batch_autotune(sg_batch* h, int reps, char* report, size_t len) {
         std::snprintf(line, sizeof line,
                       "best: tail_threads %ld mid_cells %ld lds_cells %ld : %.2f us\n", bth, bmid,
                       blds, best);
+        out += line;
+        long bst = get_knob(b, SG_TUNE_MG_STAGED);
+        for (long st : {0L, 1L, 64L}) {
+            set_knob(b, SG_TUNE_MG_STAGED, st);
+            const double us = 1e6 * time_iteration(b, reps);
+            std::snprintf(line, sizeof line, "staged from level %2ld : %10.2f us\n", st, us);
+            out += line;
+            if (us < best) {
+                best = us;
+                bst = st;
+            }
+        }
+        set_knob(b, SG_TUNE_MG_STAGED, bst);
+        std::snprintf(line, sizeof line, "best: staged from level %ld : %.2f us\n", bst, best);
         out += line;
         snapshot(b, false);
         build_plan(b);

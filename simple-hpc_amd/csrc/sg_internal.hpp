@@ -117,6 +117,7 @@ struct Solver {
     long mid_cells = 16384, lds_cells = 4096;
     int tail_threads = 256;
     bool mg_tiled = true;              // multigrid levels above the tail: down/up tile kernels
+    int mg_staged = 0;                 // first level whose tiles stage the operator in LDS too
     size_t lds_budget = 65536 - 1024;  // dynamic LDS of the multigrid tail (set from the device)
     std::vector<void*> owned;          // allocations to free
     Pack pku{}, pkv{};                 // packed momentum coefficients

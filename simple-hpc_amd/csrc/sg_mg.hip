@@ -345,6 +345,172 @@ template <class T> __device__ __forceinline__ void k_mg_up(const DU<T>& a, const
     }
 }
 
+// ---- staged tiles for the levels between level 0 and the tail ----
+//
+// The same steps as k_mg_down / k_mg_up, but x and the six operator arrays (AE, AW, AN, AS, AP,
+// b) of the tile and its halo are loaded into LDS at once, so the smoothing passes, the residual
+// and the Galerkin build read LDS only: on these small levels a pass is bounded by load latency,
+// not bandwidth. 32 x 32 tiles (the seven arrays of a tile with its halo fill up to 99 KB).
+
+constexpr int SG_I = 32, SG_J = 32;
+
+template <class T> struct Stage {
+    Lp<T> x, ae, aw, an, as, ap, b;
+    int gi0, gj0, LW;
+    __device__ __forceinline__ int at(int I, int J) const { return (I - gi0) * LW + (J - gj0); }
+};
+
+template <class T> __device__ __forceinline__ Stage<T> stage_view(int gi0, int gj0, int LH, int LW) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const Lp<T> base = (Lp<T>)reinterpret_cast<T*>(smem);
+    const int n = LH * LW;
+    return {base, base + n, base + 2 * n, base + 3 * n, base + 4 * n, base + 5 * n, base + 6 * n,
+            gi0, gj0, LW};
+}
+
+// x of the tile and its halo (zero outside the level's padded range; prolonged if C), and the
+// operator inside the level (zero elsewhere).
+template <class T>
+__device__ __forceinline__ void stage_load(const Stage<T>& st, const Lv<T>& L, const T* x, int LH,
+                                           const Lv<T>* C, T omega) {
+    constexpr int B = 4;
+    const int n = LH * st.LW;
+    for (int base = 0; base < n; base += B * MT_N) {
+        T vx[B], v0[B], v1[B], v2[B], v3[B], v4[B], v5[B];
+#pragma unroll
+        for (int k = 0; k < B; ++k) {
+            const int e = base + k * MT_N + threadIdx.x, a = e / st.LW, bb = e - a * st.LW;
+            const int I = st.gi0 + a, J = st.gj0 + bb;
+            vx[k] = v0[k] = v1[k] = v2[k] = v3[k] = v4[k] = v5[k] = T(0);
+            if (e < n && I >= -1 && I <= L.nx && J >= -1 && J <= L.ny) {
+                const long q = static_cast<long>(I) * L.stride + J;
+                vx[k] = x[q];
+                if (I >= 0 && I < L.nx && J >= 0 && J < L.ny) {
+                    if (C)
+                        vx[k] = vx[k] + omega * C->x[static_cast<size_t>(I / 2) * C->stride + J / 2];
+                    v0[k] = L.AE[q];
+                    v1[k] = L.AW[q];
+                    v2[k] = L.AN[q];
+                    v3[k] = L.AS[q];
+                    v4[k] = L.AP[q];
+                    v5[k] = L.b[q];
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < B; ++k) {
+            const int e = base + k * MT_N + threadIdx.x;
+            if (e < n) {
+                st.x[e] = vx[k];
+                st.ae[e] = v0[k];
+                st.aw[e] = v1[k];
+                st.an[e] = v2[k];
+                st.as[e] = v3[k];
+                st.ap[e] = v4[k];
+                st.b[e] = v5[k];
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// Colour pass `col` over rows [ia, ib], columns [ja, jb], everything in LDS (smooth, :62-64).
+template <class T>
+__device__ __forceinline__ void stage_pass(const Stage<T>& st, int col, int ia, int ib, int ja,
+                                           int jb) {
+    if (ia <= ib && ja <= jb) {
+        const int cpr = (jb - ja + 2) / 2, n = (ib - ia + 1) * cpr;
+        const int LW = st.LW;
+        for (int e = threadIdx.x; e < n; e += MT_N) {
+            const int r = e / cpr, I = ia + r;
+            const int J = ja + ((I + ja + col) & 1) + 2 * (e - r * cpr);
+            if (J > jb)
+                continue;
+            const int q = st.at(I, J);
+            const auto x = st.x;
+            T t = st.b[q] / st.ap[q];
+            t += (st.ae[q] * x[q + 1] + st.aw[q] * x[q - 1] + st.as[q] * x[q + LW] +
+                  st.an[q] * x[q - LW]) /
+                 st.ap[q];
+            st.x[q] = t;
+        }
+    }
+    __syncthreads();
+}
+
+template <class T> __device__ __forceinline__ void k_mg_down_s(const DU<T>& a, const Blk& bk) {
+    const Lv<T>& F = a.F;
+    const Lv<T>& C = a.C;
+    const int pre = a.sweeps, build = a.build;
+    const int H = 2 * pre + 1, LW = SG_J + 2 * H, LH = SG_I + 2 * H;
+    int tx, ty;
+    xcd_tile(bk, tx, ty);
+    const int ti0 = ty * SG_I, tj0 = tx * SG_J;
+    const Stage<T> st = stage_view<T>(ti0 - H, tj0 - H, LH, LW);
+    stage_load<T>(st, F, a.xin, LH, nullptr, T(0));
+    for (int p = 0; p < 2 * pre; ++p) {
+        const int d = 2 * pre - p;
+        stage_pass(st, p & 1, max(0, ti0 - d), min(F.nx - 1, ti0 + SG_I - 1 + d), max(0, tj0 - d),
+                   min(F.ny - 1, tj0 + SG_J - 1 + d));
+    }
+    // Residual and restriction of the tile's coarse cells (residual :83-84, restrict_residual
+    // :90-103); their operator on the first cycle (mg_build, :232-261).
+    const int ci0 = ti0 / 2, cj0 = tj0 / 2;
+    const int cn = min(SG_I / 2, C.nx - ci0), cm = min(SG_J / 2, C.ny - cj0);
+    for (int e = threadIdx.x; e < cn * cm; e += MT_N) {
+        const int I = ci0 + e / cm, J = cj0 + e % cm;
+        const size_t c = static_cast<size_t>(I) * C.stride + J;
+        const int fq[4] = {st.at(2 * I, 2 * J), st.at(2 * I, 2 * J + 1), st.at(2 * I + 1, 2 * J),
+                           st.at(2 * I + 1, 2 * J + 1)};
+        T r4[4];
+        for (int k = 0; k < 4; ++k) {
+            const int q = fq[k];
+            const auto x = st.x;
+            r4[k] = st.b[q] - (st.ap[q] * x[q] - (st.ae[q] * x[q + 1] + st.aw[q] * x[q - 1] +
+                                                  st.as[q] * x[q + LW] + st.an[q] * x[q - LW]));
+        }
+        C.b[c] = r4[0] + r4[1] + r4[2] + r4[3];
+        C.x[c] = T(0);
+        if (build) {
+            const T ae = st.ae[fq[1]] + st.ae[fq[3]];
+            const T aw = st.aw[fq[0]] + st.aw[fq[2]];
+            const T an = st.an[fq[0]] + st.an[fq[1]];
+            const T as = st.as[fq[2]] + st.as[fq[3]];
+            C.AE[c] = ae;
+            C.AW[c] = aw;
+            C.AN[c] = an;
+            C.AS[c] = as;
+            C.AP[c] = ae + aw + an + as;
+        }
+    }
+    for (int e = threadIdx.x; e < SG_I * SG_J; e += MT_N) {
+        const int I = ti0 + e / SG_J, J = tj0 + e % SG_J;
+        if (I < F.nx && J < F.ny)
+            a.xout[static_cast<size_t>(I) * F.stride + J] = st.x[st.at(I, J)];
+    }
+}
+
+template <class T> __device__ __forceinline__ void k_mg_up_s(const DU<T>& a, const Blk& bk) {
+    const Lv<T>& F = a.F;
+    const int post = a.sweeps;
+    const int H = 2 * post, LW = SG_J + 2 * H, LH = SG_I + 2 * H;
+    int tx, ty;
+    xcd_tile(bk, tx, ty);
+    const int ti0 = ty * SG_I, tj0 = tx * SG_J;
+    const Stage<T> st = stage_view<T>(ti0 - H, tj0 - H, LH, LW);
+    stage_load<T>(st, F, a.xin, LH, &a.C, a.omega);
+    for (int p = 0; p < 2 * post; ++p) {
+        const int d = 2 * post - 1 - p;
+        stage_pass(st, p & 1, max(0, ti0 - d), min(F.nx - 1, ti0 + SG_I - 1 + d), max(0, tj0 - d),
+                   min(F.ny - 1, tj0 + SG_J - 1 + d));
+    }
+    for (int e = threadIdx.x; e < SG_I * SG_J; e += MT_N) {
+        const int I = ti0 + e / SG_J, J = tj0 + e % SG_J;
+        if (I < F.nx && J < F.ny)
+            a.xout[static_cast<size_t>(I) * F.stride + J] = st.x[st.at(I, J)];
+    }
+}
+
 // ---- the tail: one workgroup for the small levels ----
 //
 // Levels tm..nl-1 run in one workgroup, one launch per V-cycle, with barriers between steps:
@@ -878,6 +1044,16 @@ template <class T> void launch_builds(Solver& s, const Tail<T>& t, const int* st
     s.tag = 0;
 }
 
+// The staged tile kernels need more than the default 64 KB of dynamic LDS when launched alone:
+// allow them the device's whole per-workgroup LDS.
+template <class T> void staged_lds(const Solver& s) {
+    const int max = static_cast<int>(s.lds_budget + 1024);
+    SG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_one<DU<T>, k_mg_down_s<T>, MT_N>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, max));
+    SG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_one<DU<T>, k_mg_up_s<T>, MT_N>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, max));
+}
+
 // mg_cycles V-cycles (mg_vcycle, :299-323) on level 0 of t.
 template <class T> void launch_vcycles(Solver& s, const Tail<T>& t, const int* stop) {
     const sg_params& p = s.p;
@@ -885,6 +1061,11 @@ template <class T> void launch_vcycles(Solver& s, const Tail<T>& t, const int* s
     const bool tiled = tiled_ok(s);
     const size_t dn = sizeof(T) * (MT_I + 2 * (2 * p.mg_pre + 1)) * (MT_J + 2 * (2 * p.mg_pre + 1));
     const size_t up = sizeof(T) * (MT_I + 4 * p.mg_post) * (MT_J + 4 * p.mg_post);
+    const size_t sdn = 7 * sizeof(T) * (SG_I + 2 * (2 * p.mg_pre + 1)) * (SG_J + 2 * (2 * p.mg_pre + 1));
+    const size_t sup = 7 * sizeof(T) * (SG_I + 4 * p.mg_post) * (SG_J + 4 * p.mg_post);
+    const bool staged = tiled && s.mg_staged < t.tm && std::max(sdn, sup) <= s.lds_budget + 1024;
+    if (staged)
+        staged_lds<T>(s);
     for (int cyc = 0; cyc < p.mg_cycles; ++cyc) {
         const int v = s.vcount++;
         for (int l = 0; l < t.tm; ++l) {
@@ -893,6 +1074,12 @@ template <class T> void launch_vcycles(Solver& s, const Tail<T>& t, const int* s
             if (tiled) {
                 // The first cycle builds level l+1 here unless the tail builds it (l+1 = tm).
                 const int build = (cyc == 0 && l + 1 < t.tm) ? 1 : 0;
+                if (staged && l >= s.mg_staged)
+                    emit<DU<T>, k_mg_down_s<T>, MT_N>(s, dim3(cdiv(F.ny, SG_J), cdiv(F.nx, SG_I)),
+                                                      dim3(MT_N), sdn,
+                                                      DU<T>{F, C, F.x, F.xt, p.mg_pre, build, T(0)},
+                                                      stop);
+                else
                 emit<DU<T>, k_mg_down<T>, MT_N>(s, dim3(cdiv(F.ny, MT_J), cdiv(F.nx, MT_I)),
                                                 dim3(MT_N), dn,
                                                 DU<T>{F, C, F.x, F.xt, p.mg_pre, build, T(0)},
@@ -908,7 +1095,11 @@ template <class T> void launch_vcycles(Solver& s, const Tail<T>& t, const int* s
         for (int l = t.tm - 1; l >= 0; --l) {
             const Lv<T> F = t.G[l], C = t.G[l + 1];
             s.tag = mg_tag(v, 1, l);
-            if (tiled) {
+            if (staged && l >= s.mg_staged) {
+                emit<DU<T>, k_mg_up_s<T>, MT_N>(s, dim3(cdiv(F.ny, SG_J), cdiv(F.nx, SG_I)),
+                                                dim3(MT_N), sup,
+                                                DU<T>{F, C, F.xt, F.x, p.mg_post, 0, t.omega}, stop);
+            } else if (tiled) {
                 emit<DU<T>, k_mg_up<T>, MT_N>(s, dim3(cdiv(F.ny, MT_J), cdiv(F.nx, MT_I)),
                                               dim3(MT_N), up,
                                               DU<T>{F, C, F.xt, F.x, p.mg_post, 0, t.omega}, stop);

@@ -6,8 +6,8 @@
 //
 // Two launches, no fence inside either. Rows: a workgroup of 256 threads owns 4 rows; every
 // thread loads its share of a 4 x 1024 chunk at once (all loads in flight) into LDS, then one lane
-// per row walks its row in order from LDS (reads in batches of 8 ahead of the adds), so each
-// chain waits on FP64 adds rather than memory. Total: one workgroup loads the row sums into LDS
+// per row walks its row in order from LDS (reads a batch ahead of the adds), so each chain waits
+// on FP64 adds rather than memory. Total: one workgroup loads the row sums into LDS
 // the same way and one lane adds them in row order. At 1024 x 1024 each launch takes about 14 us.
 // The residual sums, which feed only the stopping rule, are deferred and done together (one launch
 // for the rows of all three, one for the totals) just before it.
@@ -26,19 +26,40 @@ constexpr int PAD = 2;   // row padding (doubles) against bank conflicts
 constexpr int LOADS = RW * CW / NT;
 constexpr int TCH = 4096; // row sums per chunk of the total
 
-// s + v[0] + ... + v[n-1] in order, from LDS, reads in batches of 8.
+// s + v[0] + ... + v[n-1] in order, from LDS: batches of 16 in two register buffers, the next
+// batch's reads in flight while the current one is added, so the chain waits on the adds.
 __device__ __forceinline__ double lds_chain(double s, const double* v, int n) {
-    int c = 0;
-    for (; c + 8 <= n; c += 8) {
-        double a[8];
+    constexpr int B = 16;
+    const int nb = n / B;
+    double a[B], b[B];
+    if (nb > 0) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q)
-            a[q] = v[c + q];
+        for (int q = 0; q < B; ++q)
+            a[q] = v[q];
+    }
+    int k = 0;
+    for (; k + 1 < nb; k += 2) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q)
+        for (int q = 0; q < B; ++q)
+            b[q] = v[(k + 1) * B + q];
+#pragma unroll
+        for (int q = 0; q < B; ++q)
+            s += a[q];
+        if (k + 2 < nb) {
+#pragma unroll
+            for (int q = 0; q < B; ++q)
+                a[q] = v[(k + 2) * B + q];
+        }
+#pragma unroll
+        for (int q = 0; q < B; ++q)
+            s += b[q];
+    }
+    if (k < nb) {
+#pragma unroll
+        for (int q = 0; q < B; ++q)
             s += a[q];
     }
-    for (; c < n; ++c)
+    for (int c = nb * B; c < n; ++c)
         s += v[c];
     return s;
 }

@@ -27,7 +27,20 @@ NPHASES = 11
 PHASES = ["assemble_u", "bc_u", "residual_u", "sweep_u", "assemble_v", "bc_v", "residual_v",
           "sweep_v", "assemble_p", "solve_p", "correct"]
 K_SWEEP_U, K_SWEEP_V, K_COPY, K_ASSEMBLE_U, K_PSOLVE = 0, 1, 2, 3, 4
-TUNE_TAIL_THREADS, TUNE_MID_CELLS, TUNE_LDS_CELLS, TUNE_MOM_TILE = 0, 1, 2, 3
+TUNE_TAIL_THREADS, TUNE_MID_CELLS, TUNE_LDS_CELLS, TUNE_MOM_TILE, TUNE_MG_STAGED = 0, 1, 2, 3, 4
+# Launch-shape knobs in the order of bench.py's --knobs "tile,threads,mid,lds[,staged]".
+TUNE_KEYS = (TUNE_MOM_TILE, TUNE_TAIL_THREADS, TUNE_MID_CELLS, TUNE_LDS_CELLS, TUNE_MG_STAGED)
+
+
+def tuning_of(obj):
+    """The launch-shape knobs of a Solver or Batch, in TUNE_KEYS order."""
+    return tuple(obj.get_tuning(k) for k in TUNE_KEYS)
+
+
+def set_tunings(obj, values):
+    """Set the first len(values) knobs of TUNE_KEYS on a Solver or Batch."""
+    for k, v in zip(TUNE_KEYS, values):
+        obj.set_tuning(k, int(v))
 
 
 class SgParams(C.Structure):

@@ -205,12 +205,10 @@ def ab_to_tolerance(net, n, re, level=2, cap=10000000, log=print):
             sg.attach_warm_start(s, net, level)
         s.advance(3)
         s.autotune(reps=5)  # launch shapes only: results are unchanged
-        keys = (sg.TUNE_TAIL_THREADS, sg.TUNE_MID_CELLS, sg.TUNE_LDS_CELLS, sg.TUNE_MOM_TILE)
-        knobs = [s.get_tuning(k) for k in keys]
+        knobs = sg.tuning_of(s)
         s.close()
         s = sg.Solver(_mixed_params(n, re, form == "learned", check_every=500))
-        for k, v in zip(keys, knobs):
-            s.set_tuning(k, v)
+        sg.set_tunings(s, knobs)
         if form == "learned":
             sg.attach_warm_start(s, net, level)
         s.advance(1)
