@@ -17,57 +17,9 @@ namespace sg {
 namespace {
 thread_local std::string g_err;
 
-// Stopping rule of run_outer_loop (src/solver.cpp:112-128), one thread: log the residuals,
-// update the running maxima, test max(...) <= tol && iter > 1 with std::max's comparison order.
-// A residual that is not finite stops the loop (diverged), which the reference does not detect.
-struct StopArgs {
-    Ctl* ctl;
-    const double* scal;
-    double* log;
-    int log_cap, norm;
-    double tol;
-};
-
 __device__ __forceinline__ void k_stop_rule(const StopArgs& a, const Blk&) {
-    Ctl* ctl = a.ctl;
-    const double* scal = a.scal;
-    double* log = a.log;
-    const int log_cap = a.log_cap, norm = a.norm;
-    const double tol = a.tol;
-    if (threadIdx.x != 0 || ctl->stop)
-        return;
-    const int iter = ++ctl->iter;
-    const double ru = scal[S_RES_U], rv = scal[S_RES_V], rp = scal[S_RES_P];
-    if (ctl->log_pos < log_cap) {
-        double* e = log + 3 * ctl->log_pos;
-        e[0] = ru;
-        e[1] = rv;
-        e[2] = rp;
-    }
-    ++ctl->log_pos;
-    if (!isfinite(ru) || !isfinite(rv) || !isfinite(rp)) {
-        ctl->diverged = 1;
-        ctl->stop = 1;
-        return;
-    }
-    double tu = ru, tv = rv, tp = rp;
-    if (norm == SG_NORM_RUNNING_MAX) {
-        ctl->mu = ctl->mu < ru ? ru : ctl->mu;
-        ctl->mv = ctl->mv < rv ? rv : ctl->mv;
-        ctl->mp = ctl->mp < rp ? rp : ctl->mp;
-        tu = ctl->mu > 0.0 ? ru / ctl->mu : 0.0;
-        tv = ctl->mv > 0.0 ? rv / ctl->mv : 0.0;
-        tp = ctl->mp > 0.0 ? rp / ctl->mp : 0.0;
-    }
-    double m = tu;
-    if (m < tv)
-        m = tv;
-    if (m < tp)
-        m = tp;
-    if (m <= tol && iter > 1) {
-        ctl->converged = 1;
-        ctl->stop = 1;
-    }
+    if (threadIdx.x == 0)
+        stop_rule(a);
 }
 
 __global__ void k_ctl_begin(Ctl* ctl) {
@@ -129,9 +81,12 @@ void enqueue_iteration(Solver& s, const int* stop, bool eager, hipEvent_t* ev) {
             SG_HIP(hipEventRecord(ev[k], s.stream));
     };
     mark(0);
-    launch_assemble(s, true, stop);
+    // Without per-phase events, u's ghost rows are set by the assembly launch itself.
+    const bool fuse_bc = defer && s.pku.rows >= 2;
+    launch_assemble(s, true, stop, fuse_bc);
     mark(1);
-    launch_bc(s, true, stop);
+    if (!fuse_bc)
+        launch_bc(s, true, stop);
     mark(2);
     mark(3);
     launch_mom_sweeps(s, true, p.maxit_inner_u, stop, s.scal + S_RES_U, defer);
@@ -149,12 +104,17 @@ void enqueue_iteration(Solver& s, const int* stop, bool eager, hipEvent_t* ev) {
     mark(10);
     launch_correct(s, stop);
     mark(11);
+    const StopArgs rule{s.ctl, s.scal, s.dlog, s.log_cap, p.residual_normalisation,
+                        p.error_tolerance};
+    // Captured iterations (every launch skips once the rule has fired): the residual totals and
+    // the rule in one launch.
+    if (defer && stop == &s.ctl->stop) {
+        flush_sums(s, stop, &rule);
+        return;
+    }
     if (defer)
         flush_sums(s, stop);
-    emit<StopArgs, k_stop_rule, 64>(s, dim3(1), dim3(64), 0,
-                                    StopArgs{s.ctl, s.scal, s.dlog, s.log_cap,
-                                             p.residual_normalisation, p.error_tolerance},
-                                    &s.ctl->stop);
+    emit<StopArgs, k_stop_rule, 64>(s, dim3(1), dim3(64), 0, rule, &s.ctl->stop);
 }
 
 namespace {

@@ -168,7 +168,8 @@ void emit(Solver& s, dim3 grid, dim3 block, size_t lds, const A& a, const int* s
 
 // ---- launchers (each enqueues on s.stream; `stop` may be s.never) ----
 // momentum (sg_momentum.hip)
-void launch_assemble(Solver& s, bool is_u, const int* stop);
+// with_bc (u only): the same launch also sets u's ghost rows (launch_bc).
+void launch_assemble(Solver& s, bool is_u, const int* stop, bool with_bc = false);
 void launch_bc(Solver& s, bool is_u, const int* stop);
 void launch_mom_residual(Solver& s, bool is_u, double* slot, const int* stop, bool defer);
 void launch_mom_sweeps(Solver& s, bool is_u, int nsweeps, const int* stop, double* res_slot,
@@ -189,14 +190,71 @@ void launch_fill(Solver& s, double* a, size_t n, double v, const int* stop);
 // lane: which row-sum slice (0 for immediate sums; 1..3 for the deferred residual sums).
 void launch_ordered_sum(Solver& s, const double* a, int pitch, int i0, int i1, int j0, int j1,
                         double* slot, const int* stop, int lane = 0);
+// The ordered sum of x over rows [i0, i1], columns [j0, j1] into *slot, then x -= sum / n over
+// that range.
+void launch_ordered_mean_sub(Solver& s, double* x, int pitch, int i0, int i1, int j0, int j1,
+                             double n, double* slot, const int* stop, double* zero = nullptr,
+                             size_t nzero = 0);
 // Record an ordered sum of a compact nrow x w cell buffer (row-sum lane 1..3) for flush_sums.
 void defer_sum(Solver& s, int lane, const double* cell, int nrow, int w, double* slot);
-// Enqueue every deferred sum: one launch for all their rows, one for their totals.
-void flush_sums(Solver& s, const int* stop);
+// Stopping rule of run_outer_loop (src/solver.cpp:112-128), one thread: log the residuals,
+// update the running maxima, test max(...) <= tol && iter > 1 with std::max's comparison order.
+// A residual that is not finite stops the loop (diverged), which the reference does not detect.
+struct StopArgs {
+    Ctl* ctl;
+    const double* scal;
+    double* log;
+    int log_cap, norm;
+    double tol;
+};
+
+__device__ __forceinline__ void stop_rule(const StopArgs& a) {
+    Ctl* ctl = a.ctl;
+    const double* scal = a.scal;
+    if (ctl->stop)
+        return;
+    const int iter = ++ctl->iter;
+    const double ru = scal[S_RES_U], rv = scal[S_RES_V], rp = scal[S_RES_P];
+    if (ctl->log_pos < a.log_cap) {
+        double* e = a.log + 3 * ctl->log_pos;
+        e[0] = ru;
+        e[1] = rv;
+        e[2] = rp;
+    }
+    ++ctl->log_pos;
+    if (!isfinite(ru) || !isfinite(rv) || !isfinite(rp)) {
+        ctl->diverged = 1;
+        ctl->stop = 1;
+        return;
+    }
+    double tu = ru, tv = rv, tp = rp;
+    if (a.norm == SG_NORM_RUNNING_MAX) {
+        ctl->mu = ctl->mu < ru ? ru : ctl->mu;
+        ctl->mv = ctl->mv < rv ? rv : ctl->mv;
+        ctl->mp = ctl->mp < rp ? rp : ctl->mp;
+        tu = ctl->mu > 0.0 ? ru / ctl->mu : 0.0;
+        tv = ctl->mv > 0.0 ? rv / ctl->mv : 0.0;
+        tp = ctl->mp > 0.0 ? rp / ctl->mp : 0.0;
+    }
+    double m = tu;
+    if (m < tv)
+        m = tv;
+    if (m < tp)
+        m = tp;
+    if (m <= a.tol && iter > 1) {
+        ctl->converged = 1;
+        ctl->stop = 1;
+    }
+}
+
+// Enqueue every deferred sum: one launch for all their rows, one for their totals (with rule:
+// the totals and then the stopping rule in that second launch).
+void flush_sums(Solver& s, const int* stop, const StopArgs* rule = nullptr);
 // multigrid (sg_mg.hip)
 void mg_setup(Solver& s);
 void launch_mg_solve(Solver& s, const int* stop, bool has_guess);
-void launch_remove_mean0(Solver& s, double* a, const int* stop);
+// With zero, the same launches also fill zero[0..s.n) with 0.0.
+void launch_remove_mean0(Solver& s, double* a, const int* stop, double* zero = nullptr);
 // learned warm start (sg_unet.hip): p' guess into the level-0 x, kept only if it lowers the defect
 void launch_warm_guess(Solver& s, const int* stop);
 void warm_free(Solver& s);

@@ -1221,8 +1221,7 @@ void launch_mg_solve(Solver& s, const int* stop, bool has_guess) {
     if (p.mg_precision != SG_MG_MIXED) {
         const Tail<double> t = make_tail<double>(s, lv64);
         launch_builds(s, t, stop);
-        launch_fill(s, s.F(SG_F_PC), s.n, 0.0, stop);
-        launch_remove_mean0(s, s.F(SG_F_B), stop);
+        launch_remove_mean0(s, s.F(SG_F_B), stop, s.F(SG_F_PC)); // and p' = 0
         launch_vcycles(s, t, stop);
         launch_remove_mean0(s, s.F(SG_F_PC), stop);
         return;
@@ -1234,9 +1233,8 @@ void launch_mg_solve(Solver& s, const int* stop, bool has_guess) {
     // Learned warm start: the network's guess of p' (or zero, if the guess does not lower the
     // defect) replaces the zero start.
     const bool warm = p.pressure_warm_start == SG_WARM_LEARNED && s.warm;
-    if (!has_guess && !warm)
-        launch_fill(s, s.F(SG_F_PC), s.n, 0.0, stop);
-    launch_remove_mean0(s, s.F(SG_F_B), stop);
+    launch_remove_mean0(s, s.F(SG_F_B), stop,
+                        !has_guess && !warm ? s.F(SG_F_PC) : nullptr); // and p' = 0
     if (warm) {
         launch_warm_guess(s, stop);
         has_guess = true;

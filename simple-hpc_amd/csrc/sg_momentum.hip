@@ -57,33 +57,18 @@ struct AsmArgs {
     double dx, dy;
     double h;    // face length of the pressure term (dy for u, dx for v)
     long poff;   // second pressure cell (+1 for u, -ld for v)
+    double* ubc; // u only: also set u's ghost rows (bc_u) in this launch, or null
     int uds, order;
     C9 d2, d4;
 };
 
 // One control volume: convection (src/coefficients.cpp:15-39) plus diffusion (:41-63,
 // precomputed on the host as d2 / d4), summed element-wise (:5-8); face velocities of
-// src/momentum_u.cpp:13-16 or src/momentum_v.cpp:14-17.
-template <bool IS_U> __device__ __forceinline__ void k_assemble(const AsmArgs& a, const Blk& bk) {
-    const Pack& pk = a.pk;
-    const int j = pk.aj0 + bk.x * BX + threadIdx.x;
-    const int i = pk.ai0 + bk.y * BY + threadIdx.y;
-    if (i > pk.ai1 || j > pk.aj1)
-        return;
-    const long ld = a.ld;
-    const size_t q = static_cast<size_t>(i) * ld + j;
-    double ue, uw, vn, vs;
-    if (IS_U) {
-        ue = (a.u[q] + a.u[q + 1]) / 2;
-        uw = (a.u[q] + a.u[q - 1]) / 2;
-        vn = (a.v[q] + a.v[q + 1]) / 2;
-        vs = (a.v[q + ld] + a.v[q + ld + 1]) / 2;
-    } else {
-        ue = (a.u[q - ld] + a.u[q]) / 2;
-        uw = (a.u[q - 1] + a.u[q - ld - 1]) / 2;
-        vn = (a.v[q] + a.v[q - ld]) / 2;
-        vs = (a.v[q] + a.v[q + ld]) / 2;
-    }
+// src/momentum_u.cpp:13-16 or src/momentum_v.cpp:14-17. Written at packed entry b (and AP
+// row-major at q).
+__device__ __forceinline__ void assemble_cell(const AsmArgs& a, int i, int j, size_t q, double* b,
+                                              double ue, double uw, double vn, double vs,
+                                              double pt) {
     const bool band = j == a.H + 1 || j == a.NY + a.H || i == a.NX + a.H - 1 || i == a.H;
     const double dx = a.dx, dy = a.dy;
     double e, w, n, s, ee, ww, nn, ss;
@@ -105,10 +90,7 @@ template <bool IS_U> __device__ __forceinline__ void k_assemble(const AsmArgs& a
     }
     const double ap = s + w + n + e + ss + ww + nn + ee;
     const C9& d = (a.order == 2 || band) ? a.d2 : a.d4;
-    const int c = colour_of(pk.ncol, i, j);
-    const size_t o = pidx(pk, c, i, (j - jfirst(pk, c, i)) / pk.ncol);
-    double* b = pk.base + o;
-    const size_t as = pk.astride;
+    const size_t as = a.pk.astride;
     b[0 * as] = e + d.v[0];
     b[1 * as] = w + d.v[1];
     b[2 * as] = n + d.v[2];
@@ -120,7 +102,67 @@ template <bool IS_U> __device__ __forceinline__ void k_assemble(const AsmArgs& a
     const double app = ap + d.v[8];
     b[8 * as] = app;
     a.APn[q] = app;
-    b[9 * as] = (a.pf[q] - a.pf[q + a.poff]) * a.h;
+    b[9 * as] = pt;
+}
+
+// Colour-major: thread (x, y) takes the kk-th cell of every colour in row i, so each store to a
+// packed array is one contiguous run per wave. All loads of the thread's cells come first.
+template <bool IS_U> __device__ __forceinline__ void k_assemble(const AsmArgs& a, const Blk& bk) {
+    const Pack& pk = a.pk;
+    const int kk = bk.x * BX + threadIdx.x;
+    const int i = pk.ai0 + bk.y * BY + threadIdx.y;
+    if (i > pk.ai1)
+        return;
+    const long ld = a.ld;
+    constexpr int MC = 5;
+    double fue[MC], fuw[MC], fvn[MC], fvs[MC], fpt[MC];
+    int jj[MC];
+#pragma unroll
+    for (int c = 0; c < MC; ++c) {
+        const int j = jfirst(pk, c < pk.ncol ? c : 0, i) + pk.ncol * kk;
+        jj[c] = (c < pk.ncol && j <= pk.aj1) ? j : -1;
+        if (jj[c] < 0)
+            continue;
+        const size_t q = static_cast<size_t>(i) * ld + j;
+        if (IS_U) {
+            fue[c] = (a.u[q] + a.u[q + 1]) / 2;
+            fuw[c] = (a.u[q] + a.u[q - 1]) / 2;
+            fvn[c] = (a.v[q] + a.v[q + 1]) / 2;
+            fvs[c] = (a.v[q + ld] + a.v[q + ld + 1]) / 2;
+        } else {
+            fue[c] = (a.u[q - ld] + a.u[q]) / 2;
+            fuw[c] = (a.u[q - 1] + a.u[q - ld - 1]) / 2;
+            fvn[c] = (a.v[q] + a.v[q - ld]) / 2;
+            fvs[c] = (a.v[q] + a.v[q + ld]) / 2;
+        }
+        fpt[c] = (a.pf[q] - a.pf[q + a.poff]) * a.h;
+    }
+#pragma unroll
+    for (int c = 0; c < MC; ++c) {
+        if (jj[c] < 0)
+            continue;
+        const size_t q = static_cast<size_t>(i) * ld + jj[c];
+        assemble_cell(a, i, jj[c], q, pk.base + pidx(pk, c, i, kk), fue[c], fuw[c], fvn[c],
+                      fvs[c], fpt[c]);
+    }
+    // Ghost rows of u (src/momentum_u.cpp:24-34), which the assembly above does not read: the
+    // threads of the first row take the lid, those of the second the south wall, for the
+    // columns of their cells.
+    if (IS_U && a.ubc && i - pk.ai0 < 2) {
+        double* u = a.ubc;
+        const int H = a.H;
+#pragma unroll
+        for (int c = 0; c < MC; ++c) {
+            const int j = jj[c];
+            if (j < H + 1 || j > a.NY + H)
+                continue;
+            if (i == pk.ai0)
+                u[static_cast<size_t>(H - 1) * ld + j] = 2 - u[static_cast<size_t>(H) * ld + j];
+            else
+                u[static_cast<size_t>(a.NX + H) * ld + j] =
+                    -u[static_cast<size_t>(a.NX + H - 1) * ld + j];
+        }
+    }
 }
 
 // Ghost cells: u lid 2 - u and south -u (src/momentum_u.cpp:24-34); v west/east -v
@@ -218,9 +260,11 @@ struct SweepArgs {
     int nr, nc, first;
 };
 
-// FIRST: a second LDS copy keeps the loaded (pre-sweep) tile, and each core cell's residual
-// |R| = |(nb + pterm) - AP x| (src/momentum_u.cpp:48-50) is evaluated from that copy during the
-// cell's own colour pass, with the coefficients the update loads anyway.
+// FIRST: a second LDS copy keeps the loaded (pre-sweep) values of the tile and the ring of R
+// cells its stencils reach, and each core cell's residual |R| = |(nb + pterm) - AP x|
+// (src/momentum_u.cpp:48-50) is evaluated from that copy during the cell's own colour pass, with
+// the coefficients the update loads anyway. (Only that ring, not the whole halo: the two copies
+// then fit two workgroups per CU.)
 template <int NCOL, int TI, int TJ, bool FIRST>
 __device__ __forceinline__ void k_mom_sweep(const SweepArgs& sa, const Blk& bk) {
     const MomArgs& m = sa.m;
@@ -231,8 +275,10 @@ __device__ __forceinline__ void k_mom_sweep(const SweepArgs& sa, const Blk& bk) 
     constexpr int R = NCOL == 2 ? 1 : 2; // stencil reach
     constexpr int HL = R * NCOL;          // halo read by the first pass
     constexpr int LW = TJ + 2 * HL, LH = TI + 2 * HL;
+    constexpr int W0 = TJ + 2 * R, H0 = TI + 2 * R; // pre-sweep copy: tile and stencil ring
     __shared__ double su[LH * LW];
-    __shared__ double su0[FIRST ? LH * LW : 1];
+    __shared__ double su0[FIRST ? H0 * W0 : 1];
+    __shared__ double res[FIRST ? TI * TJ : 1]; // |R| of the core, written out row by row
     int tx, ty;
     xcd_tile(bk, tx, ty);
     const int ti0 = ty * TI, tj0 = tx * TJ;
@@ -254,8 +300,11 @@ __device__ __forceinline__ void k_mom_sweep(const SweepArgs& sa, const Blk& bk) 
             const int e = k * ST + threadIdx.x;
             if (e < LH * LW) {
                 su[e] = v[k];
-                if (FIRST)
-                    su0[e] = v[k];
+                if (FIRST) {
+                    const int a = e / LW - (HL - R), b = e % LW - (HL - R);
+                    if (a >= 0 && a < H0 && b >= 0 && b < W0)
+                        su0[a * W0 + b] = v[k];
+                }
             }
         }
     }
@@ -264,7 +313,7 @@ __device__ __forceinline__ void k_mom_sweep(const SweepArgs& sa, const Blk& bk) 
     const size_t as = pk.astride;
     const int rw = m.j1 - m.j0 + 1;
 #define SU(i, j) su[((i) - gi0) * LW + ((j) - gj0)]
-#define S0(i, j) su0[((i) - gi0) * LW + ((j) - gj0)]
+#define S0(i, j) su0[((i) - (ti0 - R)) * W0 + ((j) - (tj0 - R))]
     for (int k = 0; k < NCOL; ++k) {
         const int d = R * (NCOL - 1 - k);
         const int ia = max(m.i0, ti0 - d), ib = min(m.i1, ti0 + TI - 1 + d);
@@ -294,8 +343,7 @@ __device__ __forceinline__ void k_mom_sweep(const SweepArgs& sa, const Blk& bk) 
                                         cs * S0(i + 1, j)) +
                                        (cee * S0(i, j + 2) + cww * S0(i, j - 2) +
                                         cnn * S0(i - 2, j) + css * S0(i + 2, j));
-                    cell[static_cast<size_t>(i - m.i0) * rw + (j - m.j0)] =
-                        fabs(nb0 + ct - cp * S0(i, j));
+                    res[(i - ti0) * TJ + (j - tj0)] = fabs(nb0 + ct - cp * S0(i, j));
                 }
                 double nb = ce * SU(i, j + 1) + cw * SU(i, j - 1) + cn * SU(i - 1, j) +
                             cs * SU(i + 1, j);
@@ -312,6 +360,8 @@ __device__ __forceinline__ void k_mom_sweep(const SweepArgs& sa, const Blk& bk) 
         const int e = k * ST + threadIdx.x, a = e / TJ, b = e - a * TJ, i = ti0 + a, j = tj0 + b;
         if (e < TI * TJ && i < nr && j < nc)
             dst[static_cast<size_t>(i) * ld + j] = SU(i, j);
+        if (FIRST && e < TI * TJ && i >= m.i0 && i <= m.i1 && j >= m.j0 && j <= m.j1)
+            cell[static_cast<size_t>(i - m.i0) * rw + (j - m.j0)] = res[e];
     }
 #undef S0
 #undef SU
@@ -427,7 +477,7 @@ Pack make_pack(const sg_params& p, bool is_u) {
     return k;
 }
 
-void launch_assemble(Solver& s, bool is_u, const int* stop) {
+void launch_assemble(Solver& s, bool is_u, const int* stop, bool with_bc) {
     const sg_params& p = s.p;
     AsmArgs a{};
     a.u = s.F(SG_F_U);
@@ -439,11 +489,12 @@ void launch_assemble(Solver& s, bool is_u, const int* stop) {
     a.dx = p.dx, a.dy = p.dy;
     a.h = is_u ? p.dy : p.dx;
     a.poff = is_u ? 1 : -static_cast<long>(s.ld);
+    a.ubc = is_u && with_bc ? s.F(SG_F_U) : nullptr;
     a.uds = p.convection_scheme == SG_UDS;
     a.order = p.diffusion_order;
     a.d2 = s.diff2;
     a.d4 = s.diff4;
-    const dim3 grid(cdiv(a.pk.aj1 - a.pk.aj0 + 1, BX), cdiv(a.pk.rows, BY));
+    const dim3 grid(cdiv(a.pk.kp, BX), cdiv(a.pk.rows, BY));
     if (is_u)
         emit<AsmArgs, k_assemble<true>, BX * BY>(s, grid, dim3(BX, BY), 0, a, stop);
     else

@@ -29,6 +29,7 @@ struct ApArgs {
     const double *APU, *APV, *u, *v;
     double dx, dy;
     double* cell;
+    int nr, nc; // array rows and columns (the wall links extend over the halo)
 };
 
 __device__ __forceinline__ void k_assemble_p(const ApArgs& g, const Blk& bk) {
@@ -39,9 +40,24 @@ __device__ __forceinline__ void k_assemble_p(const ApArgs& g, const Blk& bk) {
     const int H = a.H;
     const int j = H + 1 + bk.x * BX + threadIdx.x;
     const int i = H + bk.y * BY + threadIdx.y;
+    const long ld = a.ld;
+    // The wall-link zeroing of src/pressure.cpp:30-41, which also covers halo rows and columns,
+    // by the first row of workgroups (the cells below write the same zeros inside the range).
+    if (bk.y == 0) {
+        for (int k = bk.x * (BX * BY) + threadIdx.y * BX + threadIdx.x; k < max(g.nr, g.nc);
+             k += bk.nx * (BX * BY)) {
+            if (k < g.nr) {
+                a.AW[k * ld + H + 1] = 0.0;
+                a.AE[k * ld + a.NY + H] = 0.0;
+            }
+            if (k < g.nc) {
+                a.AS[(a.NX + H - 1) * ld + k] = 0.0;
+                a.AN[H * ld + k] = 0.0;
+            }
+        }
+    }
     if (i > a.NX + H - 1 || j > a.NY + H)
         return;
-    const long ld = a.ld;
     const size_t q = static_cast<size_t>(i) * ld + j;
     double ae = dy * dy / APU[q];
     double aw = dy * dy / APU[q - 1];
@@ -59,28 +75,6 @@ __device__ __forceinline__ void k_assemble_p(const ApArgs& g, const Blk& bk) {
     const double bq = -dx * (-v[q + ld] + v[q]) - dy * (-u[q - 1] + u[q]);
     a.b[q] = bq;
     cell[static_cast<size_t>(i - H) * a.NY + (j - H - 1)] = fabs(bq);
-}
-
-// The wall-link zeroing of src/pressure.cpp:30-41 also covers halo rows and columns.
-struct WallArgs {
-    PArgs a;
-    int nr, nc;
-};
-constexpr int WT = 256;
-
-__device__ __forceinline__ void k_p_walls(const WallArgs& g, const Blk& bk) {
-    const PArgs& a = g.a;
-    const int nr = g.nr, nc = g.nc;
-    const int k = bk.x * WT + threadIdx.x;
-    const long ld = a.ld;
-    if (k < nr) {
-        a.AW[k * ld + a.H + 1] = 0.0;
-        a.AE[k * ld + a.NY + a.H] = 0.0;
-    }
-    if (k < nc) {
-        a.AS[(a.NX + a.H - 1) * ld + k] = 0.0;
-        a.AN[a.H * ld + k] = 0.0;
-    }
 }
 
 // pc_update (src/pressure.cpp:70-75): p' = b/APP; p' += (E + W + S + N)/APP, neighbours from src.
@@ -257,25 +251,6 @@ __global__ void __launch_bounds__(BX* BY) k_cg_update(CgArgs c, double alpha) {
     c.cell[k + 3 * c.plane] = ri;
 }
 
-// x -= sx / n.
-struct SubArgs {
-    PArgs a;
-    double* x;
-    const double* sum;
-    double n;
-};
-
-__device__ __forceinline__ void k_sub_mean(const SubArgs& g, const Blk& bk) {
-    const PArgs& a = g.a;
-    const int H = a.H;
-    const int j = H + 1 + bk.x * BX + threadIdx.x;
-    const int i = H + bk.y * BY + threadIdx.y;
-    if (i > a.NX + H - 1 || j > a.NY + H)
-        return;
-    const double m = *g.sum / g.n;
-    g.x[static_cast<size_t>(i) * a.ld + j] -= m;
-}
-
 dim3 pgrid(const sg_params& p) { return dim3(cdiv(p.NY, BX), cdiv(p.NX, BY)); }
 
 void solve_cg(Solver& s) {
@@ -341,9 +316,8 @@ void solve_cg(Solver& s) {
         hipLaunchKernelGGL(k_cg_dir, pgrid(p), dim3(BX, BY), 0, s.stream, c, mz, beta, 0);
         SG_HIP(hipGetLastError());
     }
-    launch_ordered_sum(s, a.pc, s.ld, p.H, p.NX + p.H - 1, p.H + 1, p.NY + p.H, slot, never);
-    emit<SubArgs, k_sub_mean, BX * BY>(s, pgrid(p), dim3(BX, BY), 0, SubArgs{a, a.pc, slot, n},
-                                       never);
+    launch_ordered_mean_sub(s, a.pc, s.ld, p.H, p.NX + p.H - 1, p.H + 1, p.NY + p.H, n, slot,
+                            never);
     s.cg_last = k;
 }
 
@@ -357,11 +331,9 @@ void launch_fill(Solver& s, double* a, size_t n, double v, const int* stop) {
 void launch_assemble_p(Solver& s, double* slot, const int* stop, bool defer) {
     const sg_params& p = s.p;
     const PArgs a = p_args(s);
-    emit<WallArgs, k_p_walls, WT>(s, dim3(cdiv(std::max(s.nr, s.nc), WT)), dim3(WT), 0,
-                                  WallArgs{a, s.nr, s.nc}, stop);
     emit<ApArgs, k_assemble_p, BX * BY>(s, pgrid(p), dim3(BX, BY), 0,
                                         ApArgs{a, s.F(SG_F_CU_AP), s.F(SG_F_CV_AP), s.F(SG_F_U),
-                                               s.F(SG_F_V), p.dx, p.dy, s.rcell[2]},
+                                               s.F(SG_F_V), p.dx, p.dy, s.rcell[2], s.nr, s.nc},
                                         stop);
     if (defer)
         defer_sum(s, 3, s.rcell[2], p.NX, p.NY, slot);
@@ -377,13 +349,11 @@ void launch_correct(Solver& s, const int* stop) {
     emit<CorrArgs, k_correct, BX * BY>(s, pgrid(p), dim3(BX, BY), 0, c, stop);
 }
 
-void launch_remove_mean0(Solver& s, double* x, const int* stop) {
+void launch_remove_mean0(Solver& s, double* x, const int* stop, double* zero) {
     const sg_params& p = s.p;
-    double* slot = s.scal + S_SUM_D;
-    launch_ordered_sum(s, x, s.ld, p.H, p.NX + p.H - 1, p.H + 1, p.NY + p.H, slot, stop);
-    emit<SubArgs, k_sub_mean, BX * BY>(s, pgrid(p), dim3(BX, BY), 0,
-                                       SubArgs{p_args(s), x, slot, static_cast<double>(p.NX) * p.NY},
-                                       stop);
+    launch_ordered_mean_sub(s, x, s.ld, p.H, p.NX + p.H - 1, p.H + 1, p.NY + p.H,
+                            static_cast<double>(p.NX) * p.NY, s.scal + S_SUM_D, stop, zero,
+                            zero ? s.n : 0);
 }
 
 // solve_p for every GPU pressure solver. CG needs the host between iterations (eager only).

@@ -144,16 +144,155 @@ __device__ __forceinline__ void k_rows_total_multi(const MultiArgs& ma, const Bl
     rows_total(ma.j[bk.x]);
 }
 
+// The totals of up to MAXJ sums (wave w adds sum w's row sums in row order, in parallel), then
+// the stopping rule, which reads them.
+constexpr int TS_CH = 1024; // row sums per LDS chunk and sum
+struct TotalsStop {
+    MultiArgs ma;
+    StopArgs rule;
+};
+
+__device__ __forceinline__ void k_totals_stop(const TotalsStop& a, const Blk&) {
+    __shared__ double buf[MAXJ][TS_CH];
+    __shared__ double tot[MAXJ];
+    const MultiArgs& ma = a.ma;
+    const int t = threadIdx.x, w = t / 64, lane = t % 64;
+    int most = 0;
+#pragma unroll
+    for (int x = 0; x < MAXJ; ++x)
+        if (x < ma.n)
+            most = max(most, ma.j[x].nrow);
+    double s = 0.0;
+    for (int b0 = 0; b0 < most; b0 += TS_CH) {
+#pragma unroll
+        for (int x = 0; x < MAXJ; ++x) {
+            if (x < ma.n) {
+                const int n = min(TS_CH, ma.j[x].nrow - b0);
+                for (int k = t; k < n; k += NT)
+                    buf[x][k] = ma.j[x].rows[b0 + k];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int x = 0; x < MAXJ; ++x) {
+            if (x < ma.n && w == x && lane == 0) {
+                const int n = min(TS_CH, ma.j[x].nrow - b0);
+                if (n > 0)
+                    s = lds_chain(s, buf[x], n);
+            }
+        }
+        __syncthreads();
+    }
+    if (lane == 0 && w < MAXJ)
+        tot[w] = s;
+    __syncthreads();
+    if (t == 0) {
+#pragma unroll
+        for (int x = 0; x < MAXJ; ++x)
+            if (x < ma.n)
+                *ma.j[x].slot = tot[x];
+        stop_rule(a.rule);
+    }
+}
+
 struct SlotArgs {
     double* slot;
 };
+
+__device__ __forceinline__ void k_stop_only(const StopArgs& a, const Blk&) {
+    if (threadIdx.x == 0)
+        stop_rule(a);
+}
 
 __device__ __forceinline__ void k_zero_slot(const SlotArgs& a, const Blk&) {
     if (threadIdx.x == 0)
         *a.slot = 0.0;
 }
 
+// The total of one sum's row sums, added in row order by every workgroup (no handoff between
+// workgroups), then x -= total / n over the same range (mean removal, the reference's
+// x(i,j) - mean); workgroup 0 also stores the total in the slot.
+struct MeanArgs {
+    SumArgs sa;
+    double* x;
+    double n;
+    double* zero; // also filled with 0.0 (nzero elements), or null
+    size_t nzero;
+};
+constexpr int MEAN_WG = 256; // workgroups of the fused total and subtraction
+constexpr int PF = 16;       // cells per thread per batch of the subtraction
+
+__device__ __forceinline__ void k_total_sub(const MeanArgs& ma, const Blk& bk) {
+    __shared__ double buf[TCH];
+    __shared__ double tot;
+    const SumArgs& sa = ma.sa;
+    const int t = threadIdx.x;
+    // This workgroup's cells (rows bk.x, bk.x + nx, ...; columns t, t + NT, ...) in batches of
+    // PF: a batch's loads are all in flight before its stores, and the first batch's before the
+    // total's chain.
+    const int cpr = (sa.w + NT - 1) / NT;
+    const int items = ((sa.nrow - bk.x + bk.nx - 1) / bk.nx) * cpr;
+    double v[PF];
+    double* ptr[PF];
+    bool ok[PF];
+    auto load = [&](int l0) {
+#pragma unroll
+        for (int k = 0; k < PF; ++k) {
+            const int l = l0 + k, q = l / cpr, c = t + (l - q * cpr) * NT;
+            ok[k] = l < items && c < sa.w;
+            ptr[k] = ma.x + static_cast<size_t>(sa.i0 + bk.x + q * bk.nx) * sa.pitch + sa.j0 + c;
+            v[k] = ok[k] ? *ptr[k] : 0.0;
+        }
+    };
+    load(0);
+    if (ma.zero)
+        for (size_t k = static_cast<size_t>(bk.x) * NT + t; k < ma.nzero;
+             k += static_cast<size_t>(bk.nx) * NT)
+            ma.zero[k] = 0.0;
+    double s = 0.0;
+    for (int b0 = 0; b0 < sa.nrow; b0 += TCH) {
+        const int n = min(TCH, sa.nrow - b0);
+        for (int k = t; k < n; k += NT)
+            buf[k] = sa.rows[b0 + k];
+        __syncthreads();
+        if (t == 0)
+            s = lds_chain(s, buf, n);
+        __syncthreads();
+    }
+    if (t == 0) {
+        tot = s;
+        if (bk.x == 0)
+            *sa.slot = s;
+    }
+    __syncthreads();
+    const double m = tot / ma.n;
+    for (int l0 = 0; l0 < items; l0 += PF) {
+        if (l0 > 0)
+            load(l0);
+#pragma unroll
+        for (int k = 0; k < PF; ++k)
+            if (ok[k])
+                *ptr[k] = v[k] - m;
+    }
+}
+
 } // namespace
+
+void launch_ordered_mean_sub(Solver& s, double* x, int pitch, int i0, int i1, int j0, int j1,
+                             double n, double* slot, const int* stop, double* zero,
+                             size_t nzero) {
+    const int nrow = i1 - i0 + 1, w = j1 - j0 + 1;
+    if (nrow <= 0 || w <= 0) {
+        emit<SlotArgs, k_zero_slot, 64>(s, dim3(1), dim3(64), 0, SlotArgs{slot}, stop);
+        if (zero)
+            launch_fill(s, zero, nzero, 0.0, stop);
+        return;
+    }
+    const SumArgs sa{x, static_cast<long>(pitch), i0, nrow, j0, w, s.rowbuf, slot};
+    emit<SumArgs, k_row_sums, NT>(s, dim3(cdiv(nrow, RW)), dim3(NT), 0, sa, stop);
+    emit<MeanArgs, k_total_sub, NT>(s, dim3(std::min(nrow, MEAN_WG)), dim3(NT), 0,
+                                    MeanArgs{sa, x, n, zero, nzero}, stop);
+}
 
 void launch_ordered_sum(Solver& s, const double* a, int pitch, int i0, int i1, int j0, int j1,
                         double* slot, const int* stop, int lane) {
@@ -176,7 +315,7 @@ void defer_sum(Solver& s, int lane, const double* cell, int nrow, int w, double*
     s.deferred.push_back(DeferredSum{cell, nrow, w, lane, slot});
 }
 
-void flush_sums(Solver& s, const int* stop) {
+void flush_sums(Solver& s, const int* stop, const StopArgs* rule) {
     std::vector<DeferredSum> d;
     d.swap(s.deferred);
     MultiArgs ma{};
@@ -194,11 +333,18 @@ void flush_sums(Solver& s, const int* stop) {
         blocks += static_cast<int>(cdiv(x.nrow, RW));
         ++ma.n;
     }
-    if (ma.n == 0)
-        return;
-    ma.first[ma.n] = blocks;
-    emit<MultiArgs, k_row_sums_multi, NT>(s, dim3(blocks), dim3(NT), 0, ma, stop);
-    emit<MultiArgs, k_rows_total_multi, NT>(s, dim3(ma.n), dim3(NT), 0, ma, stop);
+    if (ma.n > 0) {
+        ma.first[ma.n] = blocks;
+        emit<MultiArgs, k_row_sums_multi, NT>(s, dim3(blocks), dim3(NT), 0, ma, stop);
+        if (rule) {
+            emit<TotalsStop, k_totals_stop, NT>(s, dim3(1), dim3(NT), 0, TotalsStop{ma, *rule},
+                                                stop);
+            return;
+        }
+        emit<MultiArgs, k_rows_total_multi, NT>(s, dim3(ma.n), dim3(NT), 0, ma, stop);
+    }
+    if (rule)
+        emit<StopArgs, k_stop_only, 64>(s, dim3(1), dim3(64), 0, *rule, stop);
 }
 
 } // namespace sg
