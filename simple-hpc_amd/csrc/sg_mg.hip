@@ -686,24 +686,34 @@ template <class LV> __device__ int wave_coarse_solve(const LV& L) {
         int sweeps = 0;
         if (bb > T(0)) {
             const T bn = sqrt(bb);
-            while (sweeps < 200) {
+            const T bap = b / ap; // the update's first term does not change between sweeps
+            auto sweep = [&](T v) {
+#pragma unroll
                 for (int c = 0; c < 2; ++c) {
                     T xe, xw, xs, xn;
-                    nbs(x, xe, xw, xs, xn);
-                    if (act && col == c) {
-                        T t = b / ap;
-                        t += (ae * xe + aw * xw + as * xs + an * xn) / ap;
-                        x = t;
-                    }
+                    nbs(v, xe, xw, xs, xn);
+                    T t = bap;
+                    t += (ae * xe + aw * xw + as * xs + an * xn) / ap;
+                    v = act && col == c ? t : v; // no branch: the next sweep can interleave
                 }
-                ++sweeps;
+                return v;
+            };
+            // Sweep k + 1 runs alongside sweep k's residual test (they are independent) and is
+            // dropped when the test ends the loop: the same sweeps and iterate as testing first.
+            T xc = sweep(x);
+            sweeps = 1;
+            while (true) {
+                const T xnext = sweep(xc);
                 T xe, xw, xs, xn;
-                nbs(x, xe, xw, xs, xn);
-                const T r = act ? b - (ap * x - (ae * xe + aw * xw + as * xs + an * xn)) : T(0);
+                nbs(xc, xe, xw, xs, xn);
+                const T r = act ? b - (ap * xc - (ae * xe + aw * xw + as * xs + an * xn)) : T(0);
                 const T rr = dsum(r * r);
-                if (sqrt(rr) < coarse_tol<T>() * bn)
+                if (sqrt(rr) < coarse_tol<T>() * bn || sweeps >= 200)
                     break;
+                xc = xnext;
+                ++sweeps;
             }
+            x = xc;
         }
         const T mx = dsum(x) / ncell;
         if (act) {
@@ -755,18 +765,23 @@ template <int NY, class LV> __device__ int wave_coarse_solve16(const LV& L) {
         const bool hasE = act && J + 1 < NY, hasW = act && J > 0;
         const bool hasS = act && I + 1 < nx, hasN = act && I > 0;
         const int col = (I + J) & 1;
+        // Ordered sum (rows in order, each from 0.0): fully unrolled, so the lane values stay in
+        // registers.
         auto dsum = [&](T v) {
             T vals[16];
 #pragma unroll
             for (int k = 0; k < 16; ++k)
                 vals[k] = lane_val<T>(v, k);
             T tot = T(0);
-            for (int i = 0; i < nx; ++i) {
-                T r = T(0);
 #pragma unroll
-                for (int j = 0; j < NY; ++j)
-                    r += vals[(i * NY + j) & 15];
-                tot += r;
+            for (int i = 0; i < 16 / NY; ++i) {
+                if (i < nx) {
+                    T r = T(0);
+#pragma unroll
+                    for (int j = 0; j < NY; ++j)
+                        r += vals[i * NY + j];
+                    tot += r;
+                }
             }
             return tot;
         };
@@ -786,24 +801,34 @@ template <int NY, class LV> __device__ int wave_coarse_solve16(const LV& L) {
         int sweeps = 0;
         if (bb > T(0)) {
             const T bn = sqrt(bb);
-            while (sweeps < 200) {
+            const T bap = b / ap; // the update's first term does not change between sweeps
+            auto sweep = [&](T v) {
+#pragma unroll
                 for (int c = 0; c < 2; ++c) {
                     T xe, xw, xs, xn;
-                    nbs(x, xe, xw, xs, xn);
-                    if (act && col == c) {
-                        T t = b / ap;
-                        t += (ae * xe + aw * xw + as * xs + an * xn) / ap;
-                        x = t;
-                    }
+                    nbs(v, xe, xw, xs, xn);
+                    T t = bap;
+                    t += (ae * xe + aw * xw + as * xs + an * xn) / ap;
+                    v = act && col == c ? t : v; // no branch: the next sweep can interleave
                 }
-                ++sweeps;
+                return v;
+            };
+            // Sweep k + 1 runs alongside sweep k's residual test (they are independent) and is
+            // dropped when the test ends the loop: the same sweeps and iterate as testing first.
+            T xc = sweep(x);
+            sweeps = 1;
+            while (true) {
+                const T xnext = sweep(xc);
                 T xe, xw, xs, xn;
-                nbs(x, xe, xw, xs, xn);
-                const T r = act ? b - (ap * x - (ae * xe + aw * xw + as * xs + an * xn)) : T(0);
+                nbs(xc, xe, xw, xs, xn);
+                const T r = act ? b - (ap * xc - (ae * xe + aw * xw + as * xs + an * xn)) : T(0);
                 const T rr = dsum(r * r);
-                if (sqrt(rr) < coarse_tol<T>() * bn)
+                if (sqrt(rr) < coarse_tol<T>() * bn || sweeps >= 200)
                     break;
+                xc = xnext;
+                ++sweeps;
             }
+            x = xc;
         }
         const T mx = dsum(x) / ncell;
         if (act) {
