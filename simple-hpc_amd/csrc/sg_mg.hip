@@ -352,7 +352,10 @@ template <class T> __device__ __forceinline__ void k_mg_up(const DU<T>& a, const
 // and the Galerkin build read LDS only: on these small levels a pass is bounded by load latency,
 // not bandwidth. 32 x 32 tiles (the seven arrays of a tile with its halo fill up to 99 KB).
 
-constexpr int SG_I = 32, SG_J = 32;
+constexpr int SG_I = 32, SG_J = 32; // tile rows, columns
+// Threads per staged tile: the workgroup is alone on its CU (the FP64 tile and halo fill
+// ~99 KB of LDS), so it brings its own latency hiding: 1024 (FP64) and 512 (FP32) measured best.
+template <class T> constexpr int sg_n() { return sizeof(T) == 8 ? 1024 : 512; }
 
 template <class T> struct Stage {
     Lp<T> x, ae, aw, an, as, ap, b;
@@ -375,11 +378,11 @@ __device__ __forceinline__ void stage_load(const Stage<T>& st, const Lv<T>& L, c
                                            const Lv<T>* C, T omega) {
     constexpr int B = 4;
     const int n = LH * st.LW;
-    for (int base = 0; base < n; base += B * MT_N) {
+    for (int base = 0; base < n; base += B * sg_n<T>()) {
         T vx[B], v0[B], v1[B], v2[B], v3[B], v4[B], v5[B];
 #pragma unroll
         for (int k = 0; k < B; ++k) {
-            const int e = base + k * MT_N + threadIdx.x, a = e / st.LW, bb = e - a * st.LW;
+            const int e = base + k * sg_n<T>() + threadIdx.x, a = e / st.LW, bb = e - a * st.LW;
             const int I = st.gi0 + a, J = st.gj0 + bb;
             vx[k] = v0[k] = v1[k] = v2[k] = v3[k] = v4[k] = v5[k] = T(0);
             if (e < n && I >= -1 && I <= L.nx && J >= -1 && J <= L.ny) {
@@ -399,7 +402,7 @@ __device__ __forceinline__ void stage_load(const Stage<T>& st, const Lv<T>& L, c
         }
 #pragma unroll
         for (int k = 0; k < B; ++k) {
-            const int e = base + k * MT_N + threadIdx.x;
+            const int e = base + k * sg_n<T>() + threadIdx.x;
             if (e < n) {
                 st.x[e] = vx[k];
                 st.ae[e] = v0[k];
@@ -421,7 +424,7 @@ __device__ __forceinline__ void stage_pass(const Stage<T>& st, int col, int ia, 
     if (ia <= ib && ja <= jb) {
         const int cpr = (jb - ja + 2) / 2, n = (ib - ia + 1) * cpr;
         const int LW = st.LW;
-        for (int e = threadIdx.x; e < n; e += MT_N) {
+        for (int e = threadIdx.x; e < n; e += sg_n<T>()) {
             const int r = e / cpr, I = ia + r;
             const int J = ja + ((I + ja + col) & 1) + 2 * (e - r * cpr);
             if (J > jb)
@@ -457,7 +460,7 @@ template <class T> __device__ __forceinline__ void k_mg_down_s(const DU<T>& a, c
     // :90-103); their operator on the first cycle (mg_build, :232-261).
     const int ci0 = ti0 / 2, cj0 = tj0 / 2;
     const int cn = min(SG_I / 2, C.nx - ci0), cm = min(SG_J / 2, C.ny - cj0);
-    for (int e = threadIdx.x; e < cn * cm; e += MT_N) {
+    for (int e = threadIdx.x; e < cn * cm; e += sg_n<T>()) {
         const int I = ci0 + e / cm, J = cj0 + e % cm;
         const size_t c = static_cast<size_t>(I) * C.stride + J;
         const int fq[4] = {st.at(2 * I, 2 * J), st.at(2 * I, 2 * J + 1), st.at(2 * I + 1, 2 * J),
@@ -483,7 +486,7 @@ template <class T> __device__ __forceinline__ void k_mg_down_s(const DU<T>& a, c
             C.AP[c] = ae + aw + an + as;
         }
     }
-    for (int e = threadIdx.x; e < SG_I * SG_J; e += MT_N) {
+    for (int e = threadIdx.x; e < SG_I * SG_J; e += sg_n<T>()) {
         const int I = ti0 + e / SG_J, J = tj0 + e % SG_J;
         if (I < F.nx && J < F.ny)
             a.xout[static_cast<size_t>(I) * F.stride + J] = st.x[st.at(I, J)];
@@ -504,7 +507,7 @@ template <class T> __device__ __forceinline__ void k_mg_up_s(const DU<T>& a, con
         stage_pass(st, p & 1, max(0, ti0 - d), min(F.nx - 1, ti0 + SG_I - 1 + d), max(0, tj0 - d),
                    min(F.ny - 1, tj0 + SG_J - 1 + d));
     }
-    for (int e = threadIdx.x; e < SG_I * SG_J; e += MT_N) {
+    for (int e = threadIdx.x; e < SG_I * SG_J; e += sg_n<T>()) {
         const int I = ti0 + e / SG_J, J = tj0 + e % SG_J;
         if (I < F.nx && J < F.ny)
             a.xout[static_cast<size_t>(I) * F.stride + J] = st.x[st.at(I, J)];
@@ -1073,9 +1076,9 @@ template <class T> void launch_builds(Solver& s, const Tail<T>& t, const int* st
 // allow them the device's whole per-workgroup LDS.
 template <class T> void staged_lds(const Solver& s) {
     const int max = static_cast<int>(s.lds_budget + 1024);
-    SG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_one<DU<T>, k_mg_down_s<T>, MT_N>),
+    SG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_one<DU<T>, k_mg_down_s<T>, sg_n<T>()>),
                                hipFuncAttributeMaxDynamicSharedMemorySize, max));
-    SG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_one<DU<T>, k_mg_up_s<T>, MT_N>),
+    SG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_one<DU<T>, k_mg_up_s<T>, sg_n<T>()>),
                                hipFuncAttributeMaxDynamicSharedMemorySize, max));
 }
 
@@ -1100,8 +1103,8 @@ template <class T> void launch_vcycles(Solver& s, const Tail<T>& t, const int* s
                 // The first cycle builds level l+1 here unless the tail builds it (l+1 = tm).
                 const int build = (cyc == 0 && l + 1 < t.tm) ? 1 : 0;
                 if (staged && l >= s.mg_staged)
-                    emit<DU<T>, k_mg_down_s<T>, MT_N>(s, dim3(cdiv(F.ny, SG_J), cdiv(F.nx, SG_I)),
-                                                      dim3(MT_N), sdn,
+                    emit<DU<T>, k_mg_down_s<T>, sg_n<T>()>(s, dim3(cdiv(F.ny, SG_J), cdiv(F.nx, SG_I)),
+                                                      dim3(sg_n<T>()), sdn,
                                                       DU<T>{F, C, F.x, F.xt, p.mg_pre, build, T(0)},
                                                       stop);
                 else
@@ -1121,8 +1124,8 @@ template <class T> void launch_vcycles(Solver& s, const Tail<T>& t, const int* s
             const Lv<T> F = t.G[l], C = t.G[l + 1];
             s.tag = mg_tag(v, 1, l);
             if (staged && l >= s.mg_staged) {
-                emit<DU<T>, k_mg_up_s<T>, MT_N>(s, dim3(cdiv(F.ny, SG_J), cdiv(F.nx, SG_I)),
-                                                dim3(MT_N), sup,
+                emit<DU<T>, k_mg_up_s<T>, sg_n<T>()>(s, dim3(cdiv(F.ny, SG_J), cdiv(F.nx, SG_I)),
+                                                dim3(sg_n<T>()), sup,
                                                 DU<T>{F, C, F.xt, F.x, p.mg_post, 0, t.omega}, stop);
             } else if (tiled) {
                 emit<DU<T>, k_mg_up<T>, MT_N>(s, dim3(cdiv(F.ny, MT_J), cdiv(F.nx, MT_I)),
