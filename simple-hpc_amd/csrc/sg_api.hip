@@ -615,6 +615,12 @@ int sg_kernel_probe(sg_solver* h, int kid, int reps, double* mean_s, double* byt
             per = 1;
             one = [&s](int) { launch_assemble(s, true, s.never); };
             break;
+        case SG_K_ASSEMBLE_V:
+            cells = static_cast<double>(p.NX + 2) * (p.NY + 2);
+            bpc = 88.0;
+            per = 1;
+            one = [&s](int) { launch_assemble(s, false, s.never); };
+            break;
         default:
             throw ArgError("unknown kernel id");
         }

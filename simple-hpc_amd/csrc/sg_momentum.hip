@@ -106,11 +106,14 @@ __device__ __forceinline__ void assemble_cell(const AsmArgs& a, int i, int j, si
 }
 
 // Colour-major: thread (x, y) takes the kk-th cell of every colour in row i, so each store to a
-// packed array is one contiguous run per wave. All loads of the thread's cells come first.
+// packed array is one contiguous run per wave. All loads of the thread's cells come first. AY
+// rows per workgroup: the rows above and below that a cell's faces read are mostly its
+// neighbours' own rows, already in L1.
+constexpr int AY = 16;
 template <bool IS_U> __device__ __forceinline__ void k_assemble(const AsmArgs& a, const Blk& bk) {
     const Pack& pk = a.pk;
     const int kk = bk.x * BX + threadIdx.x;
-    const int i = pk.ai0 + bk.y * BY + threadIdx.y;
+    const int i = pk.ai0 + bk.y * AY + threadIdx.y;
     if (i > pk.ai1)
         return;
     const long ld = a.ld;
@@ -494,11 +497,11 @@ void launch_assemble(Solver& s, bool is_u, const int* stop, bool with_bc) {
     a.order = p.diffusion_order;
     a.d2 = s.diff2;
     a.d4 = s.diff4;
-    const dim3 grid(cdiv(a.pk.kp, BX), cdiv(a.pk.rows, BY));
+    const dim3 grid(cdiv(a.pk.kp, BX), cdiv(a.pk.rows, AY));
     if (is_u)
-        emit<AsmArgs, k_assemble<true>, BX * BY>(s, grid, dim3(BX, BY), 0, a, stop);
+        emit<AsmArgs, k_assemble<true>, BX * AY>(s, grid, dim3(BX, AY), 0, a, stop);
     else
-        emit<AsmArgs, k_assemble<false>, BX * BY>(s, grid, dim3(BX, BY), 0, a, stop);
+        emit<AsmArgs, k_assemble<false>, BX * AY>(s, grid, dim3(BX, AY), 0, a, stop);
 }
 
 void launch_bc(Solver& s, bool is_u, const int* stop) {

@@ -9,7 +9,7 @@ namespace sg {
 
 namespace {
 
-constexpr int BX = 64, BY = 4;
+constexpr int BX = 64, BY = 16; // 16 rows: neighbouring rows a cell reads are mostly in L1
 
 struct PArgs {
     double *AE, *AW, *AN, *AS, *AP, *b, *pc;
