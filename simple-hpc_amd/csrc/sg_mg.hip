@@ -350,12 +350,16 @@ template <class T> __device__ __forceinline__ void k_mg_up(const DU<T>& a, const
 // The same steps as k_mg_down / k_mg_up, but x and the six operator arrays (AE, AW, AN, AS, AP,
 // b) of the tile and its halo are loaded into LDS at once, so the smoothing passes, the residual
 // and the Galerkin build read LDS only: on these small levels a pass is bounded by load latency,
-// not bandwidth. 32 x 32 tiles (the seven arrays of a tile with its halo fill up to 99 KB).
+// not bandwidth. TS x TS tiles: 32 x 32 on large levels (the seven arrays of a tile with its halo
+// fill up to 99 KB), 16 x 16 on small ones, which then still spread over many CUs.
 
-constexpr int SG_I = 32, SG_J = 32; // tile rows, columns
-// Threads per staged tile: the workgroup is alone on its CU (the FP64 tile and halo fill
+// Threads per staged tile. A 32 x 32 workgroup is alone on its CU (the FP64 tile and halo fill
 // ~99 KB of LDS), so it brings its own latency hiding: 1024 (FP64) and 512 (FP32) measured best.
-template <class T> constexpr int sg_n() { return sizeof(T) == 8 ? 1024 : 512; }
+template <class T, int TS> constexpr int sg_n() {
+    return TS == 32 ? (sizeof(T) == 8 ? 1024 : 512) : 256;
+}
+// Levels with at least this many cells take 32 x 32 tiles.
+constexpr long SG_BIG = 512L * 512;
 
 template <class T> struct Stage {
     Lp<T> x, ae, aw, an, as, ap, b;
@@ -373,16 +377,16 @@ template <class T> __device__ __forceinline__ Stage<T> stage_view(int gi0, int g
 
 // x of the tile and its halo (zero outside the level's padded range; prolonged if C), and the
 // operator inside the level (zero elsewhere).
-template <class T>
+template <class T, int NTH>
 __device__ __forceinline__ void stage_load(const Stage<T>& st, const Lv<T>& L, const T* x, int LH,
                                            const Lv<T>* C, T omega) {
     constexpr int B = 4;
     const int n = LH * st.LW;
-    for (int base = 0; base < n; base += B * sg_n<T>()) {
+    for (int base = 0; base < n; base += B * NTH) {
         T vx[B], v0[B], v1[B], v2[B], v3[B], v4[B], v5[B];
 #pragma unroll
         for (int k = 0; k < B; ++k) {
-            const int e = base + k * sg_n<T>() + threadIdx.x, a = e / st.LW, bb = e - a * st.LW;
+            const int e = base + k * NTH + threadIdx.x, a = e / st.LW, bb = e - a * st.LW;
             const int I = st.gi0 + a, J = st.gj0 + bb;
             vx[k] = v0[k] = v1[k] = v2[k] = v3[k] = v4[k] = v5[k] = T(0);
             if (e < n && I >= -1 && I <= L.nx && J >= -1 && J <= L.ny) {
@@ -402,7 +406,7 @@ __device__ __forceinline__ void stage_load(const Stage<T>& st, const Lv<T>& L, c
         }
 #pragma unroll
         for (int k = 0; k < B; ++k) {
-            const int e = base + k * sg_n<T>() + threadIdx.x;
+            const int e = base + k * NTH + threadIdx.x;
             if (e < n) {
                 st.x[e] = vx[k];
                 st.ae[e] = v0[k];
@@ -418,13 +422,13 @@ __device__ __forceinline__ void stage_load(const Stage<T>& st, const Lv<T>& L, c
 }
 
 // Colour pass `col` over rows [ia, ib], columns [ja, jb], everything in LDS (smooth, :62-64).
-template <class T>
+template <class T, int NTH>
 __device__ __forceinline__ void stage_pass(const Stage<T>& st, int col, int ia, int ib, int ja,
                                            int jb) {
     if (ia <= ib && ja <= jb) {
         const int cpr = (jb - ja + 2) / 2, n = (ib - ia + 1) * cpr;
         const int LW = st.LW;
-        for (int e = threadIdx.x; e < n; e += sg_n<T>()) {
+        for (int e = threadIdx.x; e < n; e += NTH) {
             const int r = e / cpr, I = ia + r;
             const int J = ja + ((I + ja + col) & 1) + 2 * (e - r * cpr);
             if (J > jb)
@@ -441,26 +445,27 @@ __device__ __forceinline__ void stage_pass(const Stage<T>& st, int col, int ia, 
     __syncthreads();
 }
 
-template <class T> __device__ __forceinline__ void k_mg_down_s(const DU<T>& a, const Blk& bk) {
+template <class T, int TS> __device__ __forceinline__ void k_mg_down_s(const DU<T>& a, const Blk& bk) {
+    constexpr int NTH = sg_n<T, TS>();
     const Lv<T>& F = a.F;
     const Lv<T>& C = a.C;
     const int pre = a.sweeps, build = a.build;
-    const int H = 2 * pre + 1, LW = SG_J + 2 * H, LH = SG_I + 2 * H;
+    const int H = 2 * pre + 1, LW = TS + 2 * H, LH = TS + 2 * H;
     int tx, ty;
     xcd_tile(bk, tx, ty);
-    const int ti0 = ty * SG_I, tj0 = tx * SG_J;
+    const int ti0 = ty * TS, tj0 = tx * TS;
     const Stage<T> st = stage_view<T>(ti0 - H, tj0 - H, LH, LW);
-    stage_load<T>(st, F, a.xin, LH, nullptr, T(0));
+    stage_load<T, NTH>(st, F, a.xin, LH, nullptr, T(0));
     for (int p = 0; p < 2 * pre; ++p) {
         const int d = 2 * pre - p;
-        stage_pass(st, p & 1, max(0, ti0 - d), min(F.nx - 1, ti0 + SG_I - 1 + d), max(0, tj0 - d),
-                   min(F.ny - 1, tj0 + SG_J - 1 + d));
+        stage_pass<T, NTH>(st, p & 1, max(0, ti0 - d), min(F.nx - 1, ti0 + TS - 1 + d), max(0, tj0 - d),
+                   min(F.ny - 1, tj0 + TS - 1 + d));
     }
     // Residual and restriction of the tile's coarse cells (residual :83-84, restrict_residual
     // :90-103); their operator on the first cycle (mg_build, :232-261).
     const int ci0 = ti0 / 2, cj0 = tj0 / 2;
-    const int cn = min(SG_I / 2, C.nx - ci0), cm = min(SG_J / 2, C.ny - cj0);
-    for (int e = threadIdx.x; e < cn * cm; e += sg_n<T>()) {
+    const int cn = min(TS / 2, C.nx - ci0), cm = min(TS / 2, C.ny - cj0);
+    for (int e = threadIdx.x; e < cn * cm; e += NTH) {
         const int I = ci0 + e / cm, J = cj0 + e % cm;
         const size_t c = static_cast<size_t>(I) * C.stride + J;
         const int fq[4] = {st.at(2 * I, 2 * J), st.at(2 * I, 2 * J + 1), st.at(2 * I + 1, 2 * J),
@@ -486,29 +491,30 @@ template <class T> __device__ __forceinline__ void k_mg_down_s(const DU<T>& a, c
             C.AP[c] = ae + aw + an + as;
         }
     }
-    for (int e = threadIdx.x; e < SG_I * SG_J; e += sg_n<T>()) {
-        const int I = ti0 + e / SG_J, J = tj0 + e % SG_J;
+    for (int e = threadIdx.x; e < TS * TS; e += NTH) {
+        const int I = ti0 + e / TS, J = tj0 + e % TS;
         if (I < F.nx && J < F.ny)
             a.xout[static_cast<size_t>(I) * F.stride + J] = st.x[st.at(I, J)];
     }
 }
 
-template <class T> __device__ __forceinline__ void k_mg_up_s(const DU<T>& a, const Blk& bk) {
+template <class T, int TS> __device__ __forceinline__ void k_mg_up_s(const DU<T>& a, const Blk& bk) {
+    constexpr int NTH = sg_n<T, TS>();
     const Lv<T>& F = a.F;
     const int post = a.sweeps;
-    const int H = 2 * post, LW = SG_J + 2 * H, LH = SG_I + 2 * H;
+    const int H = 2 * post, LW = TS + 2 * H, LH = TS + 2 * H;
     int tx, ty;
     xcd_tile(bk, tx, ty);
-    const int ti0 = ty * SG_I, tj0 = tx * SG_J;
+    const int ti0 = ty * TS, tj0 = tx * TS;
     const Stage<T> st = stage_view<T>(ti0 - H, tj0 - H, LH, LW);
-    stage_load<T>(st, F, a.xin, LH, &a.C, a.omega);
+    stage_load<T, NTH>(st, F, a.xin, LH, &a.C, a.omega);
     for (int p = 0; p < 2 * post; ++p) {
         const int d = 2 * post - 1 - p;
-        stage_pass(st, p & 1, max(0, ti0 - d), min(F.nx - 1, ti0 + SG_I - 1 + d), max(0, tj0 - d),
-                   min(F.ny - 1, tj0 + SG_J - 1 + d));
+        stage_pass<T, NTH>(st, p & 1, max(0, ti0 - d), min(F.nx - 1, ti0 + TS - 1 + d), max(0, tj0 - d),
+                   min(F.ny - 1, tj0 + TS - 1 + d));
     }
-    for (int e = threadIdx.x; e < SG_I * SG_J; e += sg_n<T>()) {
-        const int I = ti0 + e / SG_J, J = tj0 + e % SG_J;
+    for (int e = threadIdx.x; e < TS * TS; e += NTH) {
+        const int I = ti0 + e / TS, J = tj0 + e % TS;
         if (I < F.nx && J < F.ny)
             a.xout[static_cast<size_t>(I) * F.stride + J] = st.x[st.at(I, J)];
     }
@@ -1076,10 +1082,23 @@ template <class T> void launch_builds(Solver& s, const Tail<T>& t, const int* st
 // allow them the device's whole per-workgroup LDS.
 template <class T> void staged_lds(const Solver& s) {
     const int max = static_cast<int>(s.lds_budget + 1024);
-    SG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_one<DU<T>, k_mg_down_s<T>, sg_n<T>()>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, max));
-    SG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_one<DU<T>, k_mg_up_s<T>, sg_n<T>()>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, max));
+    SG_HIP(hipFuncSetAttribute(
+        reinterpret_cast<const void*>(&k_one<DU<T>, k_mg_down_s<T, 32>, sg_n<T, 32>()>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, max));
+    SG_HIP(hipFuncSetAttribute(
+        reinterpret_cast<const void*>(&k_one<DU<T>, k_mg_up_s<T, 32>, sg_n<T, 32>()>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, max));
+    SG_HIP(hipFuncSetAttribute(
+        reinterpret_cast<const void*>(&k_one<DU<T>, k_mg_down_s<T, 16>, sg_n<T, 16>()>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, max));
+    SG_HIP(hipFuncSetAttribute(
+        reinterpret_cast<const void*>(&k_one<DU<T>, k_mg_up_s<T, 16>, sg_n<T, 16>()>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, max));
+}
+
+// LDS bytes of a staged TS x TS tile and its halo of h cells.
+template <class T> size_t staged_bytes(int ts, int h) {
+    return 7 * sizeof(T) * static_cast<size_t>(ts + 2 * h) * (ts + 2 * h);
 }
 
 // mg_cycles V-cycles (mg_vcycle, :299-323) on level 0 of t.
@@ -1089,8 +1108,9 @@ template <class T> void launch_vcycles(Solver& s, const Tail<T>& t, const int* s
     const bool tiled = tiled_ok(s);
     const size_t dn = sizeof(T) * (MT_I + 2 * (2 * p.mg_pre + 1)) * (MT_J + 2 * (2 * p.mg_pre + 1));
     const size_t up = sizeof(T) * (MT_I + 4 * p.mg_post) * (MT_J + 4 * p.mg_post);
-    const size_t sdn = 7 * sizeof(T) * (SG_I + 2 * (2 * p.mg_pre + 1)) * (SG_J + 2 * (2 * p.mg_pre + 1));
-    const size_t sup = 7 * sizeof(T) * (SG_I + 4 * p.mg_post) * (SG_J + 4 * p.mg_post);
+    const int hdn = 2 * p.mg_pre + 1, hup = 2 * p.mg_post;
+    auto big = [](const Lv<T>& L) { return static_cast<long>(L.nx) * L.ny >= SG_BIG; };
+    const size_t sdn = staged_bytes<T>(32, hdn), sup = staged_bytes<T>(32, hup);
     const bool staged = tiled && s.mg_staged < t.tm && std::max(sdn, sup) <= s.lds_budget + 1024;
     if (staged)
         staged_lds<T>(s);
@@ -1102,11 +1122,15 @@ template <class T> void launch_vcycles(Solver& s, const Tail<T>& t, const int* s
             if (tiled) {
                 // The first cycle builds level l+1 here unless the tail builds it (l+1 = tm).
                 const int build = (cyc == 0 && l + 1 < t.tm) ? 1 : 0;
-                if (staged && l >= s.mg_staged)
-                    emit<DU<T>, k_mg_down_s<T>, sg_n<T>()>(s, dim3(cdiv(F.ny, SG_J), cdiv(F.nx, SG_I)),
-                                                      dim3(sg_n<T>()), sdn,
-                                                      DU<T>{F, C, F.x, F.xt, p.mg_pre, build, T(0)},
-                                                      stop);
+                const DU<T> du{F, C, F.x, F.xt, p.mg_pre, build, T(0)};
+                if (staged && l >= s.mg_staged && big(F))
+                    emit<DU<T>, k_mg_down_s<T, 32>, sg_n<T, 32>()>(
+                        s, dim3(cdiv(F.ny, 32), cdiv(F.nx, 32)), dim3(sg_n<T, 32>()), sdn, du,
+                        stop);
+                else if (staged && l >= s.mg_staged)
+                    emit<DU<T>, k_mg_down_s<T, 16>, sg_n<T, 16>()>(
+                        s, dim3(cdiv(F.ny, 16), cdiv(F.nx, 16)), dim3(sg_n<T, 16>()),
+                        staged_bytes<T>(16, hdn), du, stop);
                 else
                 emit<DU<T>, k_mg_down<T>, MT_N>(s, dim3(cdiv(F.ny, MT_J), cdiv(F.nx, MT_I)),
                                                 dim3(MT_N), dn,
@@ -1123,10 +1147,14 @@ template <class T> void launch_vcycles(Solver& s, const Tail<T>& t, const int* s
         for (int l = t.tm - 1; l >= 0; --l) {
             const Lv<T> F = t.G[l], C = t.G[l + 1];
             s.tag = mg_tag(v, 1, l);
-            if (staged && l >= s.mg_staged) {
-                emit<DU<T>, k_mg_up_s<T>, sg_n<T>()>(s, dim3(cdiv(F.ny, SG_J), cdiv(F.nx, SG_I)),
-                                                dim3(sg_n<T>()), sup,
-                                                DU<T>{F, C, F.xt, F.x, p.mg_post, 0, t.omega}, stop);
+            const DU<T> du{F, C, F.xt, F.x, p.mg_post, 0, t.omega};
+            if (staged && l >= s.mg_staged && big(F)) {
+                emit<DU<T>, k_mg_up_s<T, 32>, sg_n<T, 32>()>(
+                    s, dim3(cdiv(F.ny, 32), cdiv(F.nx, 32)), dim3(sg_n<T, 32>()), sup, du, stop);
+            } else if (staged && l >= s.mg_staged) {
+                emit<DU<T>, k_mg_up_s<T, 16>, sg_n<T, 16>()>(
+                    s, dim3(cdiv(F.ny, 16), cdiv(F.nx, 16)), dim3(sg_n<T, 16>()),
+                    staged_bytes<T>(16, hup), du, stop);
             } else if (tiled) {
                 emit<DU<T>, k_mg_up<T>, MT_N>(s, dim3(cdiv(F.ny, MT_J), cdiv(F.nx, MT_I)),
                                               dim3(MT_N), up,

@@ -201,11 +201,13 @@ def test_tuning_knobs_do_not_change_results(threads, mid, lds):
 
 @pytest.mark.parametrize("staged", [0, 2, 64])
 @pytest.mark.parametrize("pre,post", [(2, 2), (1, 3), (3, 0)])
-def test_staged_multigrid_tiles_bit_identical(staged, pre, post):
+@pytest.mark.parametrize("nx,ny", [(256, 192), (640, 528)])
+def test_staged_multigrid_tiles_bit_identical(staged, pre, post, nx, ny):
     """Multigrid tiles that stage the operator in LDS from level `staged` on (64: none), with the
-    one-workgroup tail cut to the coarsest level so every other level runs tiled; 256 x 192 makes
-    partial tiles on every level."""
-    cfg = ("N_cells_x = 256\nN_cells_y = 192\nRe = 1000\nmomentum_solver = gs_multicolour\n"
+    one-workgroup tail cut to the coarsest level so every other level runs tiled. 256 x 192 makes
+    partial 16 x 16 tiles on every level; 640 x 528 puts level 0 on 32 x 32 tiles (partial at the
+    edges) and the rest on 16 x 16."""
+    cfg = (f"N_cells_x = {nx}\nN_cells_y = {ny}\nRe = 1000\nmomentum_solver = gs_multicolour\n"
            f"pressure_solver = mg\nmg_pre = {pre}\nmg_post = {post}\nmaxit = 6\n"
            "error_tolerance = 0\nbackend = gpu\n")
     p, _ = sg.params_from_config(cfg)
@@ -215,7 +217,7 @@ def test_staged_multigrid_tiles_bit_identical(staged, pre, post):
     gpu.set_tuning(sg.TUNE_MID_CELLS, 0)
     gpu.set_tuning(sg.TUNE_LDS_CELLS, 0)
     gpu.set_tuning(sg.TUNE_MG_STAGED, staged)
-    print(f"staged from {staged}, pre {pre} post {post}", flush=True)
+    print(f"{nx}x{ny} staged from {staged}, pre {pre} post {post}", flush=True)
     done, logg, _ = gpu.advance(6)
     assert bits_equal(logg, logc[:, 1:]), first_diff(logg, logc[:, 1:])
     for f in ("p", "pc"):
@@ -225,8 +227,9 @@ def test_staged_multigrid_tiles_bit_identical(staged, pre, post):
 
 @pytest.mark.parametrize("staged", [0, 1])
 def test_staged_multigrid_tiles_mixed_bit_identical(staged):
-    """The FP32 V-cycle of the mixed form gives the same bits with and without staged tiles."""
-    cfg = ("N_cells_x = 320\nN_cells_y = 256\nRe = 400\nmomentum_solver = gs_multicolour\n"
+    """The FP32 V-cycle of the mixed form gives the same bits with and without staged tiles
+    (level 0 on 32 x 32 tiles, the others on 16 x 16)."""
+    cfg = ("N_cells_x = 640\nN_cells_y = 512\nRe = 400\nmomentum_solver = gs_multicolour\n"
            "pressure_solver = mg\nmg_precision = mixed\nmaxit = 8\nerror_tolerance = 0\n"
            "backend = gpu\n")
     p, _ = sg.params_from_config(cfg)
