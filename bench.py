@@ -67,7 +67,7 @@ def log(msg):
 
 def run_form(sg, n, form, mode, steps, warmup, knobs=None):
     p, _ = sg.params_from_config(config_text(n, form))
-    if knobs:  # given launch shapes: "tile,threads,mid,lds[,staged]" (skips the autotuner)
+    if knobs:  # given launch shapes: "tile,threads,mid,lds[,staged[,first]]" (no autotuner)
         tune = tuple(int(x) for x in knobs.split(","))
     else:
         s = sg.Solver(p)

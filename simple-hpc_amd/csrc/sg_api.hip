@@ -656,6 +656,7 @@ static long* knob(Solver& s, int key) {
     case SG_TUNE_LDS_CELLS: return &s.lds_cells;
     case SG_TUNE_TAIL_THREADS: tmp = s.tail_threads; return &tmp;
     case SG_TUNE_MOM_TILE: tmp = s.mom_tile; return &tmp;
+    case SG_TUNE_MOM_TILE_FIRST: tmp = s.mom_tile_first >= 0 ? s.mom_tile_first : s.mom_tile; return &tmp;
     case SG_TUNE_MG_STAGED: tmp = s.mg_staged; return &tmp;
     default: throw ArgError("unknown tuning key");
     }
@@ -672,6 +673,10 @@ int sg_set_tuning(sg_solver* h, int key, long value) {
             if (value < 0 || value > 3)
                 throw ArgError("momentum tile shape must be 0..3");
             s.mom_tile = static_cast<int>(value);
+        } else if (key == SG_TUNE_MOM_TILE_FIRST) {
+            if (value < 0 || value > 3)
+                throw ArgError("momentum tile shape must be 0..3");
+            s.mom_tile_first = static_cast<int>(value);
         } else if (key == SG_TUNE_MG_STAGED) {
             if (value < 0 || value > 64)
                 throw ArgError("staged level must be 0..64");
@@ -745,6 +750,30 @@ int sg_autotune(sg_solver* h, int reps, char* report, size_t len) {
             s.mom_tile = best_k;
             char line[160];
             std::snprintf(line, sizeof line, "best momentum tile: %s (%d)\n", shape[best_k], best_k);
+            rep += line;
+            // The first sweep (fused residual, more LDS) on its own.
+            best_t = 1e30;
+            int best_f = best_k;
+            for (int k = 0; k < 4; ++k) {
+                s.mom_tile_first = k;
+                launch_mom_sweep_once(s, true, s.F(SG_F_U), s.scratch, true, s.never);
+                SG_HIP(hipEventRecord(s.kt0, s.stream));
+                for (int r = 0; r < reps; ++r)
+                    launch_mom_sweep_once(s, true, s.F(SG_F_U), s.scratch, true, s.never);
+                SG_HIP(hipEventRecord(s.kt1, s.stream));
+                sync(s);
+                float ms = 0.f;
+                SG_HIP(hipEventElapsedTime(&ms, s.kt0, s.kt1));
+                const double t = 1e-3 * ms / reps;
+                std::snprintf(line, sizeof line,
+                              "first-sweep tile %-6s : %9.2f us per sweep\n", shape[k], 1e6 * t);
+                rep += line;
+                if (t < best_t)
+                    best_t = t, best_f = k;
+            }
+            s.mom_tile_first = best_f;
+            std::snprintf(line, sizeof line, "best first-sweep tile: %s (%d)\n", shape[best_f],
+                          best_f);
             rep += line;
         }
         if (s.p.pressure_solver != SG_P_MG) {

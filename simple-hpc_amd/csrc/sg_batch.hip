@@ -269,6 +269,11 @@ void set_knob(Batch& b, int key, long v) {
                 throw ArgError("momentum tile must be 0..3");
             s->mom_tile = static_cast<int>(v);
             break;
+        case SG_TUNE_MOM_TILE_FIRST:
+            if (v < 0 || v > 3)
+                throw ArgError("momentum tile must be 0..3");
+            s->mom_tile_first = static_cast<int>(v);
+            break;
         case SG_TUNE_MG_STAGED:
             if (v < 0 || v > 64)
                 throw ArgError("staged level must be 0..64");
@@ -296,6 +301,8 @@ long get_knob(Batch& b, int key) {
         return s.mom_tile;
     case SG_TUNE_MG_STAGED:
         return s.mg_staged;
+    case SG_TUNE_MOM_TILE_FIRST:
+        return s.mom_tile_first >= 0 ? s.mom_tile_first : s.mom_tile;
     case SG_TUNE_MID_CELLS:
         return s.mid_cells;
     case SG_TUNE_LDS_CELLS:
@@ -500,6 +507,21 @@ int sg_batch_autotune(sg_batch* h, int reps, char* report, size_t len) {
         }
         set_knob(b, SG_TUNE_MOM_TILE, bt);
         std::snprintf(line, sizeof line, "best momentum tile: %s (%d)\n", tiles[bt], bt);
+        out += line;
+        int bf = bt;
+        for (int t = 0; t < 4; ++t) {
+            set_knob(b, SG_TUNE_MOM_TILE_FIRST, t);
+            const double us = 1e6 * time_iteration(b, reps);
+            std::snprintf(line, sizeof line,
+                          "first-sweep tile %-7s: %10.2f us per batch iteration\n", tiles[t], us);
+            out += line;
+            if (us < best) {
+                best = us;
+                bf = t;
+            }
+        }
+        set_knob(b, SG_TUNE_MOM_TILE_FIRST, bf);
+        std::snprintf(line, sizeof line, "best first-sweep tile: %s (%d)\n", tiles[bf], bf);
         out += line;
         long bth = get_knob(b, SG_TUNE_TAIL_THREADS), bmid = get_knob(b, SG_TUNE_MID_CELLS),
              blds = get_knob(b, SG_TUNE_LDS_CELLS);

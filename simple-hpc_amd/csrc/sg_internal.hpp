@@ -122,6 +122,7 @@ struct Solver {
     std::vector<void*> owned;          // allocations to free
     Pack pku{}, pkv{};                 // packed momentum coefficients
     int mom_tile = 0;                  // momentum sweep tile shape (tunable; sg_momentum.hip)
+    int mom_tile_first = -1;           // the first sweep's tile shape (-1: mom_tile)
     C9 diff2{}, diff4{};               // diffusive coefficients: order 2 (and band), order 4
     hipGraphExec_t graph = nullptr;
     int iters = 0;                     // host copy of completed iterations

@@ -535,7 +535,7 @@ namespace {
 template <int NCOL, bool FIRST>
 void sweep_tile_f(Solver& s, const SweepArgs& a, const int* stop) {
     const dim3 b(ST);
-    switch (s.mom_tile) {
+    switch (FIRST && s.mom_tile_first >= 0 ? s.mom_tile_first : s.mom_tile) {
     case 0:
         emit<SweepArgs, k_mom_sweep<NCOL, 32, 64, FIRST>, ST>(
             s, dim3(cdiv(s.nc, 64), cdiv(s.nr, 32)), b, 0, a, stop);

@@ -28,8 +28,10 @@ PHASES = ["assemble_u", "bc_u", "residual_u", "sweep_u", "assemble_v", "bc_v", "
           "sweep_v", "assemble_p", "solve_p", "correct"]
 K_SWEEP_U, K_SWEEP_V, K_COPY, K_ASSEMBLE_U, K_PSOLVE, K_SWEEP_U_FIRST, K_ASSEMBLE_V = range(7)
 TUNE_TAIL_THREADS, TUNE_MID_CELLS, TUNE_LDS_CELLS, TUNE_MOM_TILE, TUNE_MG_STAGED = 0, 1, 2, 3, 4
-# Launch-shape knobs in the order of bench.py's --knobs "tile,threads,mid,lds[,staged]".
-TUNE_KEYS = (TUNE_MOM_TILE, TUNE_TAIL_THREADS, TUNE_MID_CELLS, TUNE_LDS_CELLS, TUNE_MG_STAGED)
+TUNE_MOM_TILE_FIRST = 5
+# Launch-shape knobs in the order of bench.py's --knobs "tile,threads,mid,lds[,staged[,first]]".
+TUNE_KEYS = (TUNE_MOM_TILE, TUNE_TAIL_THREADS, TUNE_MID_CELLS, TUNE_LDS_CELLS, TUNE_MG_STAGED,
+             TUNE_MOM_TILE_FIRST)
 
 
 def tuning_of(obj):
