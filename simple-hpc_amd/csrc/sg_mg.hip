@@ -55,6 +55,19 @@ Lv<float> lv32(const MgLevel& L) {
 
 // Relative tolerance of the coarsest-level solve: the reference's 1e-10 in FP64; in FP32 a
 // tolerance the arithmetic can reach.
+// c ? a : b by bit masking. The empty asm hides the mask's origin, so the compiler cannot fold
+// the blend back into a select and then into a branch around a's computation.
+__device__ __forceinline__ double blend(bool c, double a, double b) {
+    long long m = -static_cast<long long>(c);
+    asm volatile("" : "+v"(m));
+    return __longlong_as_double((__double_as_longlong(a) & m) | (__double_as_longlong(b) & ~m));
+}
+__device__ __forceinline__ float blend(bool c, float a, float b) {
+    int m = -static_cast<int>(c);
+    asm volatile("" : "+v"(m));
+    return __int_as_float((__float_as_int(a) & m) | (__float_as_int(b) & ~m));
+}
+
 template <class T> __device__ constexpr T coarse_tol();
 template <> __device__ constexpr double coarse_tol<double>() { return 1e-10; }
 template <> __device__ constexpr float coarse_tol<float>() { return 1e-6f; }
@@ -703,7 +716,9 @@ template <class LV> __device__ int wave_coarse_solve(const LV& L) {
                     nbs(v, xe, xw, xs, xn);
                     T t = bap;
                     t += (ae * xe + aw * xw + as * xs + an * xn) / ap;
-                    v = act && col == c ? t : v; // no branch: the next sweep can interleave
+                    // A bit blend, not a select: the compiler would turn a select of a division
+                    // back into a branch, and the next sweep could not interleave with the test.
+                    v = blend(act && col == c, t, v);
                 }
                 return v;
             };
@@ -715,7 +730,7 @@ template <class LV> __device__ int wave_coarse_solve(const LV& L) {
                 const T xnext = sweep(xc);
                 T xe, xw, xs, xn;
                 nbs(xc, xe, xw, xs, xn);
-                const T r = act ? b - (ap * xc - (ae * xe + aw * xw + as * xs + an * xn)) : T(0);
+                const T r = blend(act, b - (ap * xc - (ae * xe + aw * xw + as * xs + an * xn)), T(0));
                 const T rr = dsum(r * r);
                 if (sqrt(rr) < coarse_tol<T>() * bn || sweeps >= 200)
                     break;
@@ -774,25 +789,23 @@ template <int NY, class LV> __device__ int wave_coarse_solve16(const LV& L) {
         const bool hasE = act && J + 1 < NY, hasW = act && J > 0;
         const bool hasS = act && I + 1 < nx, hasN = act && I > 0;
         const int col = (I + J) & 1;
-        // Ordered sum (rows in order, each from 0.0): fully unrolled, so the lane values stay in
-        // registers.
+        // Ordered sum (rows in order, each from 0.0) by DPP row shifts: lane 4i adds its row's
+        // values in column order, then lane 0 adds the row sums in row order; the same additions
+        // in the same order as reading every lane out first, with a shorter dependency chain.
+        static_assert(NY == 4, "the DPP sums assume rows of 4 lanes");
         auto dsum = [&](T v) {
-            T vals[16];
-#pragma unroll
-            for (int k = 0; k < 16; ++k)
-                vals[k] = lane_val<T>(v, k);
-            T tot = T(0);
-#pragma unroll
-            for (int i = 0; i < 16 / NY; ++i) {
-                if (i < nx) {
-                    T r = T(0);
-#pragma unroll
-                    for (int j = 0; j < NY; ++j)
-                        r += vals[i * NY + j];
-                    tot += r;
-                }
-            }
-            return tot;
+            T r = T(0) + v;
+            r = r + dpp_val<0x101>(v); // row_shl:1
+            r = r + dpp_val<0x102>(v);
+            r = r + dpp_val<0x103>(v);
+            T tot = T(0) + r;
+            if (nx > 1)
+                tot = tot + dpp_val<0x104>(r); // row 1's sum, from lane 4
+            if (nx > 2)
+                tot = tot + dpp_val<0x108>(r);
+            if (nx > 3)
+                tot = tot + dpp_val<0x10C>(r);
+            return lane_val<T>(tot, 0);
         };
         auto nbs = [&](T v, T& xe, T& xw, T& xs, T& xn) {
             const T e = dpp_val<0x101>(v), w = dpp_val<0x111>(v);   // row_shl:1, row_shr:1
@@ -818,7 +831,9 @@ template <int NY, class LV> __device__ int wave_coarse_solve16(const LV& L) {
                     nbs(v, xe, xw, xs, xn);
                     T t = bap;
                     t += (ae * xe + aw * xw + as * xs + an * xn) / ap;
-                    v = act && col == c ? t : v; // no branch: the next sweep can interleave
+                    // A bit blend, not a select: the compiler would turn a select of a division
+                    // back into a branch, and the next sweep could not interleave with the test.
+                    v = blend(act && col == c, t, v);
                 }
                 return v;
             };
@@ -830,7 +845,7 @@ template <int NY, class LV> __device__ int wave_coarse_solve16(const LV& L) {
                 const T xnext = sweep(xc);
                 T xe, xw, xs, xn;
                 nbs(xc, xe, xw, xs, xn);
-                const T r = act ? b - (ap * xc - (ae * xe + aw * xw + as * xs + an * xn)) : T(0);
+                const T r = blend(act, b - (ap * xc - (ae * xe + aw * xw + as * xs + an * xn)), T(0));
                 const T rr = dsum(r * r);
                 if (sqrt(rr) < coarse_tol<T>() * bn || sweeps >= 200)
                     break;
