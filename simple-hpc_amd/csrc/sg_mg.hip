@@ -363,14 +363,14 @@ template <class T> __device__ __forceinline__ void k_mg_up(const DU<T>& a, const
 // The same steps as k_mg_down / k_mg_up, but x and the six operator arrays (AE, AW, AN, AS, AP,
 // b) of the tile and its halo are loaded into LDS at once, so the smoothing passes, the residual
 // and the Galerkin build read LDS only: on these small levels a pass is bounded by load latency,
-// not bandwidth. TS x TS tiles: 32 x 32 on large levels (the seven arrays of a tile with its halo
-// fill up to 99 KB), 16 x 16 on small ones, which then still spread over many CUs.
-
-// Threads per staged tile. A 32 x 32 workgroup is alone on its CU (the FP64 tile and halo fill
-// ~99 KB of LDS), so it brings its own latency hiding: 1024 (FP64) and 512 (FP32) measured best.
-template <class T, int TS> constexpr int sg_n() {
-    return TS == 32 ? (sizeof(T) == 8 ? 1024 : 512) : 256;
+// not bandwidth. Tiles of TI rows x TS columns: on large levels 24 x 32 in FP64 (the seven arrays
+// of the tile and its halo fill 80 KB, so two workgroups of 512 threads share a CU, one loading
+// while the other smooths) and 32 x 32 in FP32 (512 threads); 16 x 16 on small levels (256
+// threads), which then still spread over many CUs. Each choice was the fastest measured.
+template <class T, int TS, int TI = TS> constexpr int sg_n() {
+    return TI != TS ? 512 : TS == 32 ? (sizeof(T) == 8 ? 1024 : 512) : 256;
 }
+template <class T> constexpr int BR = sizeof(T) == 8 ? 24 : 32; // rows of a large-level tile
 // Levels with at least this many cells take 32 x 32 tiles.
 constexpr long SG_BIG = 512L * 512;
 
@@ -458,26 +458,27 @@ __device__ __forceinline__ void stage_pass(const Stage<T>& st, int col, int ia, 
     __syncthreads();
 }
 
-template <class T, int TS> __device__ __forceinline__ void k_mg_down_s(const DU<T>& a, const Blk& bk) {
-    constexpr int NTH = sg_n<T, TS>();
+// TS columns, TI rows.
+template <class T, int TS, int TI = TS> __device__ __forceinline__ void k_mg_down_s(const DU<T>& a, const Blk& bk) {
+    constexpr int NTH = sg_n<T, TS, TI>();
     const Lv<T>& F = a.F;
     const Lv<T>& C = a.C;
     const int pre = a.sweeps, build = a.build;
-    const int H = 2 * pre + 1, LW = TS + 2 * H, LH = TS + 2 * H;
+    const int H = 2 * pre + 1, LW = TS + 2 * H, LH = TI + 2 * H;
     int tx, ty;
     xcd_tile(bk, tx, ty);
-    const int ti0 = ty * TS, tj0 = tx * TS;
+    const int ti0 = ty * TI, tj0 = tx * TS;
     const Stage<T> st = stage_view<T>(ti0 - H, tj0 - H, LH, LW);
     stage_load<T, NTH>(st, F, a.xin, LH, nullptr, T(0));
     for (int p = 0; p < 2 * pre; ++p) {
         const int d = 2 * pre - p;
-        stage_pass<T, NTH>(st, p & 1, max(0, ti0 - d), min(F.nx - 1, ti0 + TS - 1 + d), max(0, tj0 - d),
+        stage_pass<T, NTH>(st, p & 1, max(0, ti0 - d), min(F.nx - 1, ti0 + TI - 1 + d), max(0, tj0 - d),
                    min(F.ny - 1, tj0 + TS - 1 + d));
     }
     // Residual and restriction of the tile's coarse cells (residual :83-84, restrict_residual
     // :90-103); their operator on the first cycle (mg_build, :232-261).
     const int ci0 = ti0 / 2, cj0 = tj0 / 2;
-    const int cn = min(TS / 2, C.nx - ci0), cm = min(TS / 2, C.ny - cj0);
+    const int cn = min(TI / 2, C.nx - ci0), cm = min(TS / 2, C.ny - cj0);
     for (int e = threadIdx.x; e < cn * cm; e += NTH) {
         const int I = ci0 + e / cm, J = cj0 + e % cm;
         const size_t c = static_cast<size_t>(I) * C.stride + J;
@@ -504,29 +505,29 @@ template <class T, int TS> __device__ __forceinline__ void k_mg_down_s(const DU<
             C.AP[c] = ae + aw + an + as;
         }
     }
-    for (int e = threadIdx.x; e < TS * TS; e += NTH) {
+    for (int e = threadIdx.x; e < TI * TS; e += NTH) {
         const int I = ti0 + e / TS, J = tj0 + e % TS;
         if (I < F.nx && J < F.ny)
             a.xout[static_cast<size_t>(I) * F.stride + J] = st.x[st.at(I, J)];
     }
 }
 
-template <class T, int TS> __device__ __forceinline__ void k_mg_up_s(const DU<T>& a, const Blk& bk) {
-    constexpr int NTH = sg_n<T, TS>();
+template <class T, int TS, int TI = TS> __device__ __forceinline__ void k_mg_up_s(const DU<T>& a, const Blk& bk) {
+    constexpr int NTH = sg_n<T, TS, TI>();
     const Lv<T>& F = a.F;
     const int post = a.sweeps;
-    const int H = 2 * post, LW = TS + 2 * H, LH = TS + 2 * H;
+    const int H = 2 * post, LW = TS + 2 * H, LH = TI + 2 * H;
     int tx, ty;
     xcd_tile(bk, tx, ty);
-    const int ti0 = ty * TS, tj0 = tx * TS;
+    const int ti0 = ty * TI, tj0 = tx * TS;
     const Stage<T> st = stage_view<T>(ti0 - H, tj0 - H, LH, LW);
     stage_load<T, NTH>(st, F, a.xin, LH, &a.C, a.omega);
     for (int p = 0; p < 2 * post; ++p) {
         const int d = 2 * post - 1 - p;
-        stage_pass<T, NTH>(st, p & 1, max(0, ti0 - d), min(F.nx - 1, ti0 + TS - 1 + d), max(0, tj0 - d),
+        stage_pass<T, NTH>(st, p & 1, max(0, ti0 - d), min(F.nx - 1, ti0 + TI - 1 + d), max(0, tj0 - d),
                    min(F.ny - 1, tj0 + TS - 1 + d));
     }
-    for (int e = threadIdx.x; e < TS * TS; e += NTH) {
+    for (int e = threadIdx.x; e < TI * TS; e += NTH) {
         const int I = ti0 + e / TS, J = tj0 + e % TS;
         if (I < F.nx && J < F.ny)
             a.xout[static_cast<size_t>(I) * F.stride + J] = st.x[st.at(I, J)];
@@ -1098,10 +1099,10 @@ template <class T> void launch_builds(Solver& s, const Tail<T>& t, const int* st
 template <class T> void staged_lds(const Solver& s) {
     const int max = static_cast<int>(s.lds_budget + 1024);
     SG_HIP(hipFuncSetAttribute(
-        reinterpret_cast<const void*>(&k_one<DU<T>, k_mg_down_s<T, 32>, sg_n<T, 32>()>),
+        reinterpret_cast<const void*>(&k_one<DU<T>, k_mg_down_s<T, 32, BR<T>>, sg_n<T, 32, BR<T>>()>),
         hipFuncAttributeMaxDynamicSharedMemorySize, max));
     SG_HIP(hipFuncSetAttribute(
-        reinterpret_cast<const void*>(&k_one<DU<T>, k_mg_up_s<T, 32>, sg_n<T, 32>()>),
+        reinterpret_cast<const void*>(&k_one<DU<T>, k_mg_up_s<T, 32, BR<T>>, sg_n<T, 32, BR<T>>()>),
         hipFuncAttributeMaxDynamicSharedMemorySize, max));
     SG_HIP(hipFuncSetAttribute(
         reinterpret_cast<const void*>(&k_one<DU<T>, k_mg_down_s<T, 16>, sg_n<T, 16>()>),
@@ -1112,8 +1113,8 @@ template <class T> void staged_lds(const Solver& s) {
 }
 
 // LDS bytes of a staged TS x TS tile and its halo of h cells.
-template <class T> size_t staged_bytes(int ts, int h) {
-    return 7 * sizeof(T) * static_cast<size_t>(ts + 2 * h) * (ts + 2 * h);
+template <class T> size_t staged_bytes(int ts, int h, int ti = 0) {
+    return 7 * sizeof(T) * static_cast<size_t>((ti ? ti : ts) + 2 * h) * (ts + 2 * h);
 }
 
 // mg_cycles V-cycles (mg_vcycle, :299-323) on level 0 of t.
@@ -1139,9 +1140,9 @@ template <class T> void launch_vcycles(Solver& s, const Tail<T>& t, const int* s
                 const int build = (cyc == 0 && l + 1 < t.tm) ? 1 : 0;
                 const DU<T> du{F, C, F.x, F.xt, p.mg_pre, build, T(0)};
                 if (staged && l >= s.mg_staged && big(F))
-                    emit<DU<T>, k_mg_down_s<T, 32>, sg_n<T, 32>()>(
-                        s, dim3(cdiv(F.ny, 32), cdiv(F.nx, 32)), dim3(sg_n<T, 32>()), sdn, du,
-                        stop);
+                    emit<DU<T>, k_mg_down_s<T, 32, BR<T>>, sg_n<T, 32, BR<T>>()>(
+                        s, dim3(cdiv(F.ny, 32), cdiv(F.nx, BR<T>)), dim3(sg_n<T, 32, BR<T>>()),
+                        staged_bytes<T>(32, hdn, BR<T>), du, stop);
                 else if (staged && l >= s.mg_staged)
                     emit<DU<T>, k_mg_down_s<T, 16>, sg_n<T, 16>()>(
                         s, dim3(cdiv(F.ny, 16), cdiv(F.nx, 16)), dim3(sg_n<T, 16>()),
@@ -1164,8 +1165,9 @@ template <class T> void launch_vcycles(Solver& s, const Tail<T>& t, const int* s
             s.tag = mg_tag(v, 1, l);
             const DU<T> du{F, C, F.xt, F.x, p.mg_post, 0, t.omega};
             if (staged && l >= s.mg_staged && big(F)) {
-                emit<DU<T>, k_mg_up_s<T, 32>, sg_n<T, 32>()>(
-                    s, dim3(cdiv(F.ny, 32), cdiv(F.nx, 32)), dim3(sg_n<T, 32>()), sup, du, stop);
+                emit<DU<T>, k_mg_up_s<T, 32, BR<T>>, sg_n<T, 32, BR<T>>()>(
+                    s, dim3(cdiv(F.ny, 32), cdiv(F.nx, BR<T>)), dim3(sg_n<T, 32, BR<T>>()),
+                    staged_bytes<T>(32, hup, BR<T>), du, stop);
             } else if (staged && l >= s.mg_staged) {
                 emit<DU<T>, k_mg_up_s<T, 16>, sg_n<T, 16>()>(
                     s, dim3(cdiv(F.ny, 16), cdiv(F.nx, 16)), dim3(sg_n<T, 16>()),

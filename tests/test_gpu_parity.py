@@ -201,12 +201,12 @@ def test_tuning_knobs_do_not_change_results(threads, mid, lds):
 
 @pytest.mark.parametrize("staged", [0, 2, 64])
 @pytest.mark.parametrize("pre,post", [(2, 2), (1, 3), (3, 0)])
-@pytest.mark.parametrize("nx,ny", [(256, 192), (640, 528)])
+@pytest.mark.parametrize("nx,ny", [(256, 192), (640, 528), (612, 520)])
 def test_staged_multigrid_tiles_bit_identical(staged, pre, post, nx, ny):
     """Multigrid tiles that stage the operator in LDS from level `staged` on (64: none), with the
     one-workgroup tail cut to the coarsest level so every other level runs tiled. 256 x 192 makes
-    partial 16 x 16 tiles on every level; 640 x 528 puts level 0 on 32 x 32 tiles (partial at the
-    edges) and the rest on 16 x 16."""
+    partial 16 x 16 tiles on every level; 640 x 528 and 612 x 520 put level 0 on the large tiles
+    (24 x 32 in FP64; partial at the edges) and the rest on 16 x 16."""
     cfg = (f"N_cells_x = {nx}\nN_cells_y = {ny}\nRe = 1000\nmomentum_solver = gs_multicolour\n"
            f"pressure_solver = mg\nmg_pre = {pre}\nmg_post = {post}\nmaxit = 6\n"
            "error_tolerance = 0\nbackend = gpu\n")
