@@ -166,6 +166,7 @@ enum {
     SG_K_PSOLVE,       /* one complete p' solve (pressure_solver of the params) */
     SG_K_SWEEP_U_FIRST, /* the first u sweep of an outer iteration (|R| cells fused in) */
     SG_K_ASSEMBLE_V,
+    SG_K_SWEEP_V_FIRST, /* the first v sweep of an outer iteration */
     SG_K_COUNT
 };
 int sg_kernel_probe(sg_solver* s, int kernel_id, int reps, double* mean_seconds,

@@ -588,15 +588,20 @@ int sg_kernel_probe(sg_solver* h, int kid, int reps, double* mean_s, double* byt
             break;
         }
         case SG_K_SWEEP_U_FIRST:
+        case SG_K_SWEEP_V_FIRST: {
+            const bool is_u = kid == SG_K_SWEEP_U_FIRST;
             if (p.momentum_solver != SG_MOM_GS_MULTICOLOUR)
                 throw ArgError("sweep probe needs momentum_solver = gs_multicolour");
-            cells = static_cast<double>(p.NX) * (p.NY - 1);
+            cells = is_u ? static_cast<double>(p.NX) * (p.NY - 1)
+                         : static_cast<double>(p.NX - 1) * p.NY;
             bpc = 104.0; // as a sweep, plus the |R| cell written
             per = 1;
-            one = [&s](int) {
-                launch_mom_sweep_once(s, true, s.F(SG_F_U), s.scratch, true, s.never);
+            one = [&s, is_u](int) {
+                launch_mom_sweep_once(s, is_u, s.F(is_u ? SG_F_U : SG_F_V), s.scratch, true,
+                                      s.never);
             };
             break;
+        }
         case SG_K_PSOLVE:
             cells = static_cast<double>(p.NX) * p.NY;
             bpc = 0.0;

@@ -268,7 +268,7 @@ struct SweepArgs {
 // (src/momentum_u.cpp:48-50) is evaluated from that copy during the cell's own colour pass, with
 // the coefficients the update loads anyway. (Only that ring, not the whole halo: the two copies
 // then fit two workgroups per CU.)
-template <int NCOL, int TI, int TJ, bool FIRST>
+template <int NCOL, int TI, int TJ, bool FIRST, int NT = ST>
 __device__ __forceinline__ void k_mom_sweep(const SweepArgs& sa, const Blk& bk) {
     const MomArgs& m = sa.m;
     const double* src = sa.src;
@@ -288,19 +288,19 @@ __device__ __forceinline__ void k_mom_sweep(const SweepArgs& sa, const Blk& bk) 
     const int gi0 = ti0 - HL, gj0 = tj0 - HL; // array cell of su[0]
     const long ld = m.ld;
     // Batched loads: every load of a chunk is in flight before the first LDS store.
-    constexpr int NL = (LH * LW + ST - 1) / ST;
+    constexpr int NL = (LH * LW + NT - 1) / NT;
     {
         double v[NL];
 #pragma unroll
         for (int k = 0; k < NL; ++k) {
-            const int e = k * ST + threadIdx.x, a = e / LW, b = e - a * LW, i = gi0 + a, j = gj0 + b;
+            const int e = k * NT + threadIdx.x, a = e / LW, b = e - a * LW, i = gi0 + a, j = gj0 + b;
             v[k] = (e < LH * LW && i >= 0 && i < nr && j >= 0 && j < nc)
                        ? src[static_cast<size_t>(i) * ld + j]
                        : 0.0;
         }
 #pragma unroll
         for (int k = 0; k < NL; ++k) {
-            const int e = k * ST + threadIdx.x;
+            const int e = k * NT + threadIdx.x;
             if (e < LH * LW) {
                 su[e] = v[k];
                 if (FIRST) {
@@ -324,7 +324,7 @@ __device__ __forceinline__ void k_mom_sweep(const SweepArgs& sa, const Blk& bk) 
         if (ia <= ib && ja <= jb) {
             const int cpr = (jb - ja + NCOL) / NCOL;
             const int n = (ib - ia + 1) * cpr;
-            for (int e = threadIdx.x; e < n; e += ST) {
+            for (int e = threadIdx.x; e < n; e += NT) {
                 const int r = e / cpr, t = e - r * cpr, i = ia + r;
                 const int c0 = colour_of(NCOL, i, ja);
                 const int j = ja + (NCOL == 2 ? ((k - c0) & 1) : (2 * (k - c0) + 10) % 5) + NCOL * t;
@@ -359,8 +359,8 @@ __device__ __forceinline__ void k_mom_sweep(const SweepArgs& sa, const Blk& bk) 
         __syncthreads();
     }
 #pragma unroll
-    for (int k = 0; k < (TI * TJ + ST - 1) / ST; ++k) {
-        const int e = k * ST + threadIdx.x, a = e / TJ, b = e - a * TJ, i = ti0 + a, j = tj0 + b;
+    for (int k = 0; k < (TI * TJ + NT - 1) / NT; ++k) {
+        const int e = k * NT + threadIdx.x, a = e / TJ, b = e - a * TJ, i = ti0 + a, j = tj0 + b;
         if (e < TI * TJ && i < nr && j < nc)
             dst[static_cast<size_t>(i) * ld + j] = SU(i, j);
         if (FIRST && e < TI * TJ && i >= m.i0 && i <= m.i1 && j >= m.j0 && j <= m.j1)
@@ -532,24 +532,28 @@ void launch_mom_residual(Solver& s, bool is_u, double* slot, const int* stop, bo
 }
 
 namespace {
+// Threads of a regular sweep's workgroup (the first sweep, with its extra LDS, uses ST).
+constexpr int SW_NT = 512;
+
 template <int NCOL, bool FIRST>
 void sweep_tile_f(Solver& s, const SweepArgs& a, const int* stop) {
-    const dim3 b(ST);
+    constexpr int NT = FIRST ? ST : SW_NT;
+    const dim3 b(NT);
     switch (FIRST && s.mom_tile_first >= 0 ? s.mom_tile_first : s.mom_tile) {
     case 0:
-        emit<SweepArgs, k_mom_sweep<NCOL, 32, 64, FIRST>, ST>(
+        emit<SweepArgs, k_mom_sweep<NCOL, 32, 64, FIRST, NT>, NT>(
             s, dim3(cdiv(s.nc, 64), cdiv(s.nr, 32)), b, 0, a, stop);
         break;
     case 1:
-        emit<SweepArgs, k_mom_sweep<NCOL, 64, 64, FIRST>, ST>(
+        emit<SweepArgs, k_mom_sweep<NCOL, 64, 64, FIRST, NT>, NT>(
             s, dim3(cdiv(s.nc, 64), cdiv(s.nr, 64)), b, 0, a, stop);
         break;
     case 2:
-        emit<SweepArgs, k_mom_sweep<NCOL, 32, 128, FIRST>, ST>(
+        emit<SweepArgs, k_mom_sweep<NCOL, 32, 128, FIRST, NT>, NT>(
             s, dim3(cdiv(s.nc, 128), cdiv(s.nr, 32)), b, 0, a, stop);
         break;
     default:
-        emit<SweepArgs, k_mom_sweep<NCOL, 16, 128, FIRST>, ST>(
+        emit<SweepArgs, k_mom_sweep<NCOL, 16, 128, FIRST, NT>, NT>(
             s, dim3(cdiv(s.nc, 128), cdiv(s.nr, 16)), b, 0, a, stop);
     }
 }

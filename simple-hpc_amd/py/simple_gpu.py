@@ -26,7 +26,8 @@ FIELD_ID = {n: k for k, n in enumerate(FIELDS)}
 NPHASES = 11
 PHASES = ["assemble_u", "bc_u", "residual_u", "sweep_u", "assemble_v", "bc_v", "residual_v",
           "sweep_v", "assemble_p", "solve_p", "correct"]
-K_SWEEP_U, K_SWEEP_V, K_COPY, K_ASSEMBLE_U, K_PSOLVE, K_SWEEP_U_FIRST, K_ASSEMBLE_V = range(7)
+(K_SWEEP_U, K_SWEEP_V, K_COPY, K_ASSEMBLE_U, K_PSOLVE, K_SWEEP_U_FIRST, K_ASSEMBLE_V,
+ K_SWEEP_V_FIRST) = range(8)
 TUNE_TAIL_THREADS, TUNE_MID_CELLS, TUNE_LDS_CELLS, TUNE_MOM_TILE, TUNE_MG_STAGED = 0, 1, 2, 3, 4
 TUNE_MOM_TILE_FIRST = 5
 # Launch-shape knobs in the order of bench.py's --knobs "tile,threads,mid,lds[,staged[,first]]".
