@@ -674,14 +674,10 @@ int sg_set_tuning(sg_solver* h, int key, long value) {
             if (value < 64 || value > 1024 || value % 64)
                 throw ArgError("tail threads must be a multiple of 64 in [64, 1024]");
             s.tail_threads = static_cast<int>(value);
-        } else if (key == SG_TUNE_MOM_TILE) {
-            if (value < 0 || value > 3)
-                throw ArgError("momentum tile shape must be 0..3");
-            s.mom_tile = static_cast<int>(value);
-        } else if (key == SG_TUNE_MOM_TILE_FIRST) {
-            if (value < 0 || value > 3)
-                throw ArgError("momentum tile shape must be 0..3");
-            s.mom_tile_first = static_cast<int>(value);
+        } else if (key == SG_TUNE_MOM_TILE || key == SG_TUNE_MOM_TILE_FIRST) {
+            if (value < 0 || value >= N_MOM_TILES)
+                throw ArgError("momentum tile shape must be 0.." + std::to_string(N_MOM_TILES - 1));
+            (key == SG_TUNE_MOM_TILE ? s.mom_tile : s.mom_tile_first) = static_cast<int>(value);
         } else if (key == SG_TUNE_MG_STAGED) {
             if (value < 0 || value > 64)
                 throw ArgError("staged level must be 0..64");
@@ -731,10 +727,9 @@ int sg_autotune(sg_solver* h, int reps, char* report, size_t len) {
         std::string rep;
         if (s.p.momentum_solver == SG_MOM_GS_MULTICOLOUR) {
             // Momentum sweep tile: one u sweep (field -> scratch) per candidate shape.
-            static const char* shape[4] = {"32x64", "64x64", "32x128", "16x128"};
             double best_t = 1e30;
             int best_k = s.mom_tile;
-            for (int k = 0; k < 4; ++k) {
+            for (int k = 0; k < N_MOM_TILES; ++k) {
                 s.mom_tile = k;
                 launch_mom_sweep_once(s, true, s.F(SG_F_U), s.scratch, false, s.never);
                 SG_HIP(hipEventRecord(s.kt0, s.stream));
@@ -747,19 +742,20 @@ int sg_autotune(sg_solver* h, int reps, char* report, size_t len) {
                 const double t = 1e-3 * ms / reps;
                 char line[160];
                 std::snprintf(line, sizeof line, "momentum tile %-6s : %9.2f us per sweep\n",
-                              shape[k], 1e6 * t);
+                              tile_name(k).c_str(), 1e6 * t);
                 rep += line;
                 if (t < best_t)
                     best_t = t, best_k = k;
             }
             s.mom_tile = best_k;
             char line[160];
-            std::snprintf(line, sizeof line, "best momentum tile: %s (%d)\n", shape[best_k], best_k);
+            std::snprintf(line, sizeof line, "best momentum tile: %s (%d)\n",
+                          tile_name(best_k).c_str(), best_k);
             rep += line;
             // The first sweep (fused residual, more LDS) on its own.
             best_t = 1e30;
             int best_f = best_k;
-            for (int k = 0; k < 4; ++k) {
+            for (int k = 0; k < N_MOM_TILES; ++k) {
                 s.mom_tile_first = k;
                 launch_mom_sweep_once(s, true, s.F(SG_F_U), s.scratch, true, s.never);
                 SG_HIP(hipEventRecord(s.kt0, s.stream));
@@ -771,14 +767,15 @@ int sg_autotune(sg_solver* h, int reps, char* report, size_t len) {
                 SG_HIP(hipEventElapsedTime(&ms, s.kt0, s.kt1));
                 const double t = 1e-3 * ms / reps;
                 std::snprintf(line, sizeof line,
-                              "first-sweep tile %-6s : %9.2f us per sweep\n", shape[k], 1e6 * t);
+                              "first-sweep tile %-6s : %9.2f us per sweep\n",
+                              tile_name(k).c_str(), 1e6 * t);
                 rep += line;
                 if (t < best_t)
                     best_t = t, best_f = k;
             }
             s.mom_tile_first = best_f;
-            std::snprintf(line, sizeof line, "best first-sweep tile: %s (%d)\n", shape[best_f],
-                          best_f);
+            std::snprintf(line, sizeof line, "best first-sweep tile: %s (%d)\n",
+                          tile_name(best_f).c_str(), best_f);
             rep += line;
         }
         if (s.p.pressure_solver != SG_P_MG) {

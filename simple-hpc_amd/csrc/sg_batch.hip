@@ -265,14 +265,10 @@ void set_knob(Batch& b, int key, long v) {
             s->tail_threads = static_cast<int>(v);
             break;
         case SG_TUNE_MOM_TILE:
-            if (v < 0 || v > 3)
-                throw ArgError("momentum tile must be 0..3");
-            s->mom_tile = static_cast<int>(v);
-            break;
         case SG_TUNE_MOM_TILE_FIRST:
-            if (v < 0 || v > 3)
-                throw ArgError("momentum tile must be 0..3");
-            s->mom_tile_first = static_cast<int>(v);
+            if (v < 0 || v >= N_MOM_TILES)
+                throw ArgError("momentum tile must be 0.." + std::to_string(N_MOM_TILES - 1));
+            (key == SG_TUNE_MOM_TILE ? s->mom_tile : s->mom_tile_first) = static_cast<int>(v);
             break;
         case SG_TUNE_MG_STAGED:
             if (v < 0 || v > 64)
@@ -491,14 +487,13 @@ int sg_batch_autotune(sg_batch* h, int reps, char* report, size_t len) {
         std::string out;
         char line[256];
         snapshot(b, true);
-        static const char* tiles[4] = {"32x64", "64x64", "32x128", "16x128"};
         int bt = 0;
         double best = 1e30;
-        for (int t = 0; t < 4; ++t) {
+        for (int t = 0; t < N_MOM_TILES; ++t) {
             set_knob(b, SG_TUNE_MOM_TILE, t);
             const double us = 1e6 * time_iteration(b, reps);
             std::snprintf(line, sizeof line, "momentum tile %-7s: %10.2f us per batch iteration\n",
-                          tiles[t], us);
+                          tile_name(t).c_str(), us);
             out += line;
             if (us < best) {
                 best = us;
@@ -506,14 +501,16 @@ int sg_batch_autotune(sg_batch* h, int reps, char* report, size_t len) {
             }
         }
         set_knob(b, SG_TUNE_MOM_TILE, bt);
-        std::snprintf(line, sizeof line, "best momentum tile: %s (%d)\n", tiles[bt], bt);
+        std::snprintf(line, sizeof line, "best momentum tile: %s (%d)\n", tile_name(bt).c_str(),
+                      bt);
         out += line;
         int bf = bt;
-        for (int t = 0; t < 4; ++t) {
+        for (int t = 0; t < N_MOM_TILES; ++t) {
             set_knob(b, SG_TUNE_MOM_TILE_FIRST, t);
             const double us = 1e6 * time_iteration(b, reps);
             std::snprintf(line, sizeof line,
-                          "first-sweep tile %-7s: %10.2f us per batch iteration\n", tiles[t], us);
+                          "first-sweep tile %-7s: %10.2f us per batch iteration\n",
+                          tile_name(t).c_str(), us);
             out += line;
             if (us < best) {
                 best = us;
@@ -521,7 +518,8 @@ int sg_batch_autotune(sg_batch* h, int reps, char* report, size_t len) {
             }
         }
         set_knob(b, SG_TUNE_MOM_TILE_FIRST, bf);
-        std::snprintf(line, sizeof line, "best first-sweep tile: %s (%d)\n", tiles[bf], bf);
+        std::snprintf(line, sizeof line, "best first-sweep tile: %s (%d)\n",
+                      tile_name(bf).c_str(), bf);
         out += line;
         long bth = get_knob(b, SG_TUNE_TAIL_THREADS), bmid = get_knob(b, SG_TUNE_MID_CELLS),
              blds = get_knob(b, SG_TUNE_LDS_CELLS);

@@ -67,6 +67,19 @@ struct Pack {
     double* base;
 };
 
+// Momentum sweep tile shapes (rows x columns of cells), indexed by the momentum tile tuning knobs.
+// The autotuner times them all: which one wins depends on how many workgroups a shape makes (one
+// resident wave over the 256 CUs without a partial second one) against its halo overhead.
+struct TileShape {
+    int ti, tj;
+};
+constexpr TileShape MOM_TILES[] = {{32, 64}, {64, 64}, {32, 128}, {16, 128},
+                                   {24, 96}, {48, 96}, {24, 64},  {40, 128}};
+constexpr int N_MOM_TILES = sizeof(MOM_TILES) / sizeof(MOM_TILES[0]);
+inline std::string tile_name(int k) {
+    return std::to_string(MOM_TILES[k].ti) + "x" + std::to_string(MOM_TILES[k].tj);
+}
+
 // Device control block of the outer loop.
 struct Ctl {
     int stop;       // nonzero: skip all work (set by the stop rule)
@@ -121,7 +134,7 @@ struct Solver {
     size_t lds_budget = 65536 - 1024;  // dynamic LDS of the multigrid tail (set from the device)
     std::vector<void*> owned;          // allocations to free
     Pack pku{}, pkv{};                 // packed momentum coefficients
-    int mom_tile = 0;                  // momentum sweep tile shape (tunable; sg_momentum.hip)
+    int mom_tile = 0;                  // momentum sweep tile shape, an index of MOM_TILES (tunable)
     int mom_tile_first = -1;           // the first sweep's tile shape (-1: mom_tile)
     C9 diff2{}, diff4{};               // diffusive coefficients: order 2 (and band), order 4
     hipGraphExec_t graph = nullptr;

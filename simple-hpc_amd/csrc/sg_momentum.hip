@@ -24,6 +24,8 @@
 // pressure term (p(i,j) - p(i,j+1)) * dy is evaluated once, at assembly, with the same operands.
 #include "sg_internal.hpp"
 
+#include <utility>
+
 namespace sg {
 
 namespace {
@@ -535,37 +537,30 @@ namespace {
 // Threads of a regular sweep's workgroup (the first sweep, with its extra LDS, uses ST).
 constexpr int SW_NT = 512;
 
-template <int NCOL, bool FIRST>
-void sweep_tile_f(Solver& s, const SweepArgs& a, const int* stop) {
-    constexpr int NT = FIRST ? ST : SW_NT;
-    const dim3 b(NT);
-    switch (FIRST && s.mom_tile_first >= 0 ? s.mom_tile_first : s.mom_tile) {
-    case 0:
-        emit<SweepArgs, k_mom_sweep<NCOL, 32, 64, FIRST, NT>, NT>(
-            s, dim3(cdiv(s.nc, 64), cdiv(s.nr, 32)), b, 0, a, stop);
-        break;
-    case 1:
-        emit<SweepArgs, k_mom_sweep<NCOL, 64, 64, FIRST, NT>, NT>(
-            s, dim3(cdiv(s.nc, 64), cdiv(s.nr, 64)), b, 0, a, stop);
-        break;
-    case 2:
-        emit<SweepArgs, k_mom_sweep<NCOL, 32, 128, FIRST, NT>, NT>(
-            s, dim3(cdiv(s.nc, 128), cdiv(s.nr, 32)), b, 0, a, stop);
-        break;
-    default:
-        emit<SweepArgs, k_mom_sweep<NCOL, 16, 128, FIRST, NT>, NT>(
-            s, dim3(cdiv(s.nc, 128), cdiv(s.nr, 16)), b, 0, a, stop);
-    }
+template <int NCOL, bool FIRST, int K>
+void sweep_shape(Solver& s, const SweepArgs& a, const int* stop) {
+    constexpr int NT = FIRST ? ST : SW_NT, TI = MOM_TILES[K].ti, TJ = MOM_TILES[K].tj;
+    emit<SweepArgs, k_mom_sweep<NCOL, TI, TJ, FIRST, NT>, NT>(
+        s, dim3(cdiv(s.nc, TJ), cdiv(s.nr, TI)), dim3(NT), 0, a, stop);
+}
+
+template <int NCOL, bool FIRST, int... K>
+void sweep_tile_f(Solver& s, const SweepArgs& a, const int* stop,
+                  std::integer_sequence<int, K...>) {
+    static_assert(sizeof...(K) == N_MOM_TILES);
+    const int t = FIRST && s.mom_tile_first >= 0 ? s.mom_tile_first : s.mom_tile;
+    ((t == K ? sweep_shape<NCOL, FIRST, K>(s, a, stop) : void()), ...);
 }
 
 template <int NCOL>
 void sweep_tile(Solver& s, const MomArgs& m, const double* src, double* dst, double* cell,
                 int first, const int* stop) {
     const SweepArgs a{m, src, dst, cell, s.nr, s.nc, first};
+    constexpr auto shapes = std::make_integer_sequence<int, N_MOM_TILES>{};
     if (first)
-        sweep_tile_f<NCOL, true>(s, a, stop);
+        sweep_tile_f<NCOL, true>(s, a, stop, shapes);
     else
-        sweep_tile_f<NCOL, false>(s, a, stop);
+        sweep_tile_f<NCOL, false>(s, a, stop, shapes);
 }
 } // namespace
 

@@ -264,7 +264,7 @@ def test_autotune_restores_state_and_keeps_parity():
     _same("u", gpu.get("u"), cpu.get("u"))
 
 
-@pytest.mark.parametrize("tile", [0, 1, 2, 3])
+@pytest.mark.parametrize("tile", range(len(sg.MOM_TILES)))
 @pytest.mark.parametrize("scheme,order,inner", [("QUICK", 4, 4), ("UDS", 2, 3), ("QUICK", 2, 1)])
 def test_sweep_tile_shapes_bit_identical(tile, scheme, order, inner):
     """Every momentum sweep tile shape, odd and even sweep counts, 5 and 2 colours."""
@@ -277,7 +277,7 @@ def test_sweep_tile_shapes_bit_identical(tile, scheme, order, inner):
     _, logc, _ = cpu.run()
     gpu = sg.Solver(p)
     gpu.set_tuning(sg.TUNE_MOM_TILE, tile)
-    gpu.set_tuning(sg.TUNE_MOM_TILE_FIRST, (tile + 2) % 4)  # the first sweep's own shape
+    gpu.set_tuning(sg.TUNE_MOM_TILE_FIRST, (tile + 3) % len(sg.MOM_TILES))  # its own shape
     done, logg, _ = gpu.advance(6)
     assert bits_equal(logg, logc[:, 1:]), first_diff(logg, logc[:, 1:])
     for f in ("u", "v", "p"):
