@@ -360,13 +360,14 @@ template <class T> __device__ __forceinline__ void k_mg_up(const DU<T>& a, const
 
 // ---- staged tiles for the levels between level 0 and the tail ----
 //
-// The same steps as k_mg_down / k_mg_up, but x and the six operator arrays (AE, AW, AN, AS, AP,
-// b) of the tile and its halo are loaded into LDS at once, so the smoothing passes, the residual
-// and the Galerkin build read LDS only: on these small levels a pass is bounded by load latency,
-// not bandwidth. Tiles of TI rows x TS columns: on large levels 24 x 32 in FP64 (the seven arrays
-// of the tile and its halo fill 80 KB, so two workgroups of 512 threads share a CU, one loading
-// while the other smooths) and 32 x 32 in FP32 (512 threads); 16 x 16 on small levels (256
-// threads), which then still spread over many CUs. Each choice was the fastest measured.
+// The same steps as k_mg_down / k_mg_up, but x, b and the operator of the tile and its halo are
+// loaded into LDS at once (AE and AN, plus AP in FP32: the operator is symmetric, below), so the
+// smoothing passes, the residual and the Galerkin build read LDS only: on these small levels a
+// pass is bounded by load latency, not bandwidth. Tiles of TI rows x TS columns: on large levels
+// 24 x 32 in FP64 (the four arrays of the tile and its halo fill 46 KB, so several workgroups of
+// 512 threads share a CU, some loading while others smooth; 16 x 32 and 32 x 32 measure the same)
+// and 32 x 32 in FP32 (512 threads); 16 x 16 on small levels (256 threads), which then still
+// spread over many CUs. Each choice was the fastest measured.
 template <class T, int TS, int TI = TS> constexpr int sg_n() {
     return TI != TS ? 512 : TS == 32 ? (sizeof(T) == 8 ? 1024 : 512) : 256;
 }
@@ -374,46 +375,65 @@ template <class T> constexpr int BR = sizeof(T) == 8 ? 24 : 32; // rows of a lar
 // Levels with at least this many cells take 32 x 32 tiles.
 constexpr long SG_BIG = 512L * 512;
 
+// The p' operator is symmetric bit for bit on every level: AW(I, J) = AE(I, J - 1) and
+// AS(I, J) = AN(I + 1, J). Level 0 divides the same APU / APV entry for both links of a face
+// (src/pressure.cpp:21-24) and zeroes the wall links, as the zero halo is; a coarse level sums the
+// same fine links for both sides of a coarse face (mg_build, src/multigrid.cpp:246-253), and levels
+// exist only while both sizes are even (mg_level_dims, :143-151), so every coarse face has fine
+// faces on both sides. A tile therefore stages x, b, AE and AN and reads AW and AS at the
+// neighbouring cell. In FP64 it also recomputes AP = AE + AW + AN + AS, in the order of
+// src/pressure.cpp:45 and src/multigrid.cpp:254 (so the same bits); the FP32 level 0 holds the
+// rounded FP64 AP, so FP32 tiles stage AP as well.
+template <class T> constexpr bool SG_DERIVE_AP = sizeof(T) == 8;
+template <class T> constexpr int SG_NSTAGE = SG_DERIVE_AP<T> ? 4 : 5; // arrays staged per cell
+
 template <class T> struct Stage {
-    Lp<T> x, ae, aw, an, as, ap, b;
+    Lp<T> x, ae, an, b, ap; // ap: FP32 only
     int gi0, gj0, LW;
     __device__ __forceinline__ int at(int I, int J) const { return (I - gi0) * LW + (J - gj0); }
+    __device__ __forceinline__ T aw(int q) const { return ae[q - 1]; }
+    __device__ __forceinline__ T as(int q) const { return an[q + LW]; }
+    __device__ __forceinline__ T app(int q) const {
+        if constexpr (SG_DERIVE_AP<T>)
+            return ae[q] + ae[q - 1] + an[q] + an[q + LW];
+        else
+            return ap[q];
+    }
 };
 
 template <class T> __device__ __forceinline__ Stage<T> stage_view(int gi0, int gj0, int LH, int LW) {
     extern __shared__ __align__(16) unsigned char smem[];
     const Lp<T> base = (Lp<T>)reinterpret_cast<T*>(smem);
     const int n = LH * LW;
-    return {base, base + n, base + 2 * n, base + 3 * n, base + 4 * n, base + 5 * n, base + 6 * n,
+    return {base, base + n, base + 2 * n, base + 3 * n, SG_DERIVE_AP<T> ? nullptr : base + 4 * n,
             gi0, gj0, LW};
 }
 
 // x of the tile and its halo (zero outside the level's padded range; prolonged if C), and the
-// operator inside the level (zero elsewhere).
+// staged operator inside the level (zero elsewhere: the walls' AW and AS read there).
 template <class T, int NTH>
 __device__ __forceinline__ void stage_load(const Stage<T>& st, const Lv<T>& L, const T* x, int LH,
                                            const Lv<T>* C, T omega) {
     constexpr int B = 4;
     const int n = LH * st.LW;
     for (int base = 0; base < n; base += B * NTH) {
-        T vx[B], v0[B], v1[B], v2[B], v3[B], v4[B], v5[B];
+        T vx[B], ve[B], vn[B], vb[B], vp[B];
 #pragma unroll
         for (int k = 0; k < B; ++k) {
             const int e = base + k * NTH + threadIdx.x, a = e / st.LW, bb = e - a * st.LW;
             const int I = st.gi0 + a, J = st.gj0 + bb;
-            vx[k] = v0[k] = v1[k] = v2[k] = v3[k] = v4[k] = v5[k] = T(0);
+            vx[k] = ve[k] = vn[k] = vb[k] = vp[k] = T(0);
             if (e < n && I >= -1 && I <= L.nx && J >= -1 && J <= L.ny) {
                 const long q = static_cast<long>(I) * L.stride + J;
                 vx[k] = x[q];
                 if (I >= 0 && I < L.nx && J >= 0 && J < L.ny) {
                     if (C)
                         vx[k] = vx[k] + omega * C->x[static_cast<size_t>(I / 2) * C->stride + J / 2];
-                    v0[k] = L.AE[q];
-                    v1[k] = L.AW[q];
-                    v2[k] = L.AN[q];
-                    v3[k] = L.AS[q];
-                    v4[k] = L.AP[q];
-                    v5[k] = L.b[q];
+                    ve[k] = L.AE[q];
+                    vn[k] = L.AN[q];
+                    vb[k] = L.b[q];
+                    if constexpr (!SG_DERIVE_AP<T>)
+                        vp[k] = L.AP[q];
                 }
             }
         }
@@ -422,12 +442,11 @@ __device__ __forceinline__ void stage_load(const Stage<T>& st, const Lv<T>& L, c
             const int e = base + k * NTH + threadIdx.x;
             if (e < n) {
                 st.x[e] = vx[k];
-                st.ae[e] = v0[k];
-                st.aw[e] = v1[k];
-                st.an[e] = v2[k];
-                st.as[e] = v3[k];
-                st.ap[e] = v4[k];
-                st.b[e] = v5[k];
+                st.ae[e] = ve[k];
+                st.an[e] = vn[k];
+                st.b[e] = vb[k];
+                if constexpr (!SG_DERIVE_AP<T>)
+                    st.ap[e] = vp[k];
             }
         }
     }
@@ -448,10 +467,11 @@ __device__ __forceinline__ void stage_pass(const Stage<T>& st, int col, int ia, 
                 continue;
             const int q = st.at(I, J);
             const auto x = st.x;
-            T t = st.b[q] / st.ap[q];
-            t += (st.ae[q] * x[q + 1] + st.aw[q] * x[q - 1] + st.as[q] * x[q + LW] +
+            const T ap = st.app(q);
+            T t = st.b[q] / ap;
+            t += (st.ae[q] * x[q + 1] + st.aw(q) * x[q - 1] + st.as(q) * x[q + LW] +
                   st.an[q] * x[q - LW]) /
-                 st.ap[q];
+                 ap;
             st.x[q] = t;
         }
     }
@@ -488,16 +508,16 @@ template <class T, int TS, int TI = TS> __device__ __forceinline__ void k_mg_dow
         for (int k = 0; k < 4; ++k) {
             const int q = fq[k];
             const auto x = st.x;
-            r4[k] = st.b[q] - (st.ap[q] * x[q] - (st.ae[q] * x[q + 1] + st.aw[q] * x[q - 1] +
-                                                  st.as[q] * x[q + LW] + st.an[q] * x[q - LW]));
+            r4[k] = st.b[q] - (st.app(q) * x[q] - (st.ae[q] * x[q + 1] + st.aw(q) * x[q - 1] +
+                                                   st.as(q) * x[q + LW] + st.an[q] * x[q - LW]));
         }
         C.b[c] = r4[0] + r4[1] + r4[2] + r4[3];
         C.x[c] = T(0);
         if (build) {
             const T ae = st.ae[fq[1]] + st.ae[fq[3]];
-            const T aw = st.aw[fq[0]] + st.aw[fq[2]];
+            const T aw = st.aw(fq[0]) + st.aw(fq[2]);
             const T an = st.an[fq[0]] + st.an[fq[1]];
-            const T as = st.as[fq[2]] + st.as[fq[3]];
+            const T as = st.as(fq[2]) + st.as(fq[3]);
             C.AE[c] = ae;
             C.AW[c] = aw;
             C.AN[c] = an;
@@ -1114,7 +1134,7 @@ template <class T> void staged_lds(const Solver& s) {
 
 // LDS bytes of a staged TS x TS tile and its halo of h cells.
 template <class T> size_t staged_bytes(int ts, int h, int ti = 0) {
-    return 7 * sizeof(T) * static_cast<size_t>((ti ? ti : ts) + 2 * h) * (ts + 2 * h);
+    return SG_NSTAGE<T> * sizeof(T) * static_cast<size_t>((ti ? ti : ts) + 2 * h) * (ts + 2 * h);
 }
 
 // mg_cycles V-cycles (mg_vcycle, :299-323) on level 0 of t.
