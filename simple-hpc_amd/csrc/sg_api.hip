@@ -100,19 +100,25 @@ void enqueue_iteration(Solver& s, const int* stop, bool eager, hipEvent_t* ev) {
     mark(8);
     launch_assemble_p(s, s.scal + S_RES_P, stop, defer);
     mark(9);
+    // The residual sums, all deferred by now, may ride along with the p' solve's first mean
+    // removal (multigrid), which sums its right-hand side anyway.
+    s.ride = defer && !s.rec;
     launch_solve_p(s, stop, eager);
+    s.ride = false;
+    const bool rode = s.rode;
+    s.rode = false;
     mark(10);
     launch_correct(s, stop);
     mark(11);
     const StopArgs rule{s.ctl, s.scal, s.dlog, s.log_cap, p.residual_normalisation,
                         p.error_tolerance};
     // Captured iterations (every launch skips once the rule has fired): the residual totals and
-    // the rule in one launch.
-    if (defer && stop == &s.ctl->stop) {
+    // the rule in one launch, unless the totals are done already.
+    if (defer && stop == &s.ctl->stop && !rode) {
         flush_sums(s, stop, &rule);
         return;
     }
-    if (defer)
+    if (defer && !rode)
         flush_sums(s, stop);
     emit<StopArgs, k_stop_rule, 64>(s, dim3(1), dim3(64), 0, rule, &s.ctl->stop);
 }
@@ -127,6 +133,8 @@ void capture_graph(Solver& s) {
     try {
         enqueue_iteration(s, &s.ctl->stop, false, nullptr);
     } catch (...) {
+        s.ride = s.rode = false;
+        s.deferred.clear();
         hipGraph_t tmp = nullptr;
         (void)hipStreamEndCapture(s.stream, &tmp);
         if (tmp)

@@ -151,6 +151,10 @@ struct Solver {
     int vcount = 0; // V-cycles enqueued in the current p' solve
     Warm* warm = nullptr; // learned warm start of the p' solve (sg_unet.hip)
     std::vector<DeferredSum> deferred; // residual sums waiting for flush_sums
+    // ride: the next ordered mean removal also takes the deferred residual sums (their rows in its
+    // row-sum launch, their totals in its total launch); rode: it did, so the iteration's end has
+    // only the stopping rule left.
+    bool ride = false, rode = false;
 
     const double* F(int id) const { return f[id]; }
     double* F(int id) { return f[id]; }

@@ -125,8 +125,8 @@ __device__ __forceinline__ void k_row_sums(const SumArgs& sa, const Blk& bk) {
 
 __device__ __forceinline__ void k_rows_total(const SumArgs& sa, const Blk&) { rows_total(sa); }
 
-// Up to 3 independent sums in one launch each for the rows and the totals.
-constexpr int MAXJ = 3;
+// Up to 4 independent sums in one launch each for the rows and the totals (one wave each).
+constexpr int MAXJ = 4;
 struct MultiArgs {
     SumArgs j[MAXJ];
     int first[MAXJ + 1]; // first row block of each sum
@@ -211,19 +211,24 @@ __device__ __forceinline__ void k_zero_slot(const SlotArgs& a, const Blk&) {
 
 // The total of one sum's row sums, added in row order by every workgroup (no handoff between
 // workgroups), then x -= total / n over the same range (mean removal, the reference's
-// x(i,j) - mean); workgroup 0 also stores the total in the slot.
+// x(i,j) - mean); workgroup 0 also stores the total in the slot, and adds the totals of up to
+// MAXJ - 1 other sums (res: residual sums whose rows were summed in the same row launch) into
+// their slots, each in its own wave at the same time as its own chain.
 struct MeanArgs {
     SumArgs sa;
     double* x;
     double n;
     double* zero; // also filled with 0.0 (nzero elements), or null
     size_t nzero;
+    MultiArgs res;
 };
 constexpr int MEAN_WG = 256; // workgroups of the fused total and subtraction
 constexpr int PF = 16;       // cells per thread per batch of the subtraction
+constexpr int MCH = 2048;    // row sums per LDS chunk and sum
+static_assert(MAXJ * 64 <= NT, "one wave per chained sum");
 
 __device__ __forceinline__ void k_total_sub(const MeanArgs& ma, const Blk& bk) {
-    __shared__ double buf[TCH];
+    __shared__ double buf[MAXJ][MCH];
     __shared__ double tot;
     const SumArgs& sa = ma.sa;
     const int t = threadIdx.x;
@@ -249,20 +254,37 @@ __device__ __forceinline__ void k_total_sub(const MeanArgs& ma, const Blk& bk) {
         for (size_t k = static_cast<size_t>(bk.x) * NT + t; k < ma.nzero;
              k += static_cast<size_t>(bk.nx) * NT)
             ma.zero[k] = 0.0;
+    // Wave 0 chains this sum; in workgroup 0, wave 1 + x chains residual sum x.
+    const int w = t / 64, lane = t % 64;
+    const int nres = bk.x == 0 ? ma.res.n : 0;
+    int most = sa.nrow;
+    for (int x = 0; x < nres; ++x)
+        most = max(most, ma.res.j[x].nrow);
     double s = 0.0;
-    for (int b0 = 0; b0 < sa.nrow; b0 += TCH) {
-        const int n = min(TCH, sa.nrow - b0);
-        for (int k = t; k < n; k += NT)
-            buf[k] = sa.rows[b0 + k];
+    for (int b0 = 0; b0 < most; b0 += MCH) {
+        for (int x = 0; x <= nres; ++x) {
+            const SumArgs& q = x == 0 ? sa : ma.res.j[x - 1];
+            const int n = min(MCH, q.nrow - b0);
+            for (int k = t; k < n; k += NT)
+                buf[x][k] = q.rows[b0 + k];
+        }
         __syncthreads();
-        if (t == 0)
-            s = lds_chain(s, buf, n);
+        if (lane == 0 && w <= nres) {
+            const SumArgs& q = w == 0 ? sa : ma.res.j[w - 1];
+            const int n = min(MCH, q.nrow - b0);
+            if (n > 0)
+                s = lds_chain(s, buf[w], n);
+        }
         __syncthreads();
     }
-    if (t == 0) {
-        tot = s;
-        if (bk.x == 0)
-            *sa.slot = s;
+    if (lane == 0) {
+        if (w == 0) {
+            tot = s;
+            if (bk.x == 0)
+                *sa.slot = s;
+        } else if (w <= nres) {
+            *ma.res.j[w - 1].slot = s;
+        }
     }
     __syncthreads();
     const double m = tot / ma.n;
@@ -289,9 +311,37 @@ void launch_ordered_mean_sub(Solver& s, double* x, int pitch, int i0, int i1, in
         return;
     }
     const SumArgs sa{x, static_cast<long>(pitch), i0, nrow, j0, w, s.rowbuf, slot};
-    emit<SumArgs, k_row_sums, NT>(s, dim3(cdiv(nrow, RW)), dim3(NT), 0, sa, stop);
+    MultiArgs res{};
+    if (s.ride && !s.rec && !s.deferred.empty() && s.deferred.size() < MAXJ) {
+        // The deferred residual sums ride along: their rows and this sum's in one launch.
+        s.ride = false;
+        s.rode = true;
+        MultiArgs rows{};
+        int blocks = 0;
+        for (const DeferredSum& d : s.deferred) {
+            if (!d.cell) { // empty range: the sum is 0.0
+                emit<SlotArgs, k_zero_slot, 64>(s, dim3(1), dim3(64), 0, SlotArgs{d.slot}, stop);
+                continue;
+            }
+            rows.j[rows.n] = SumArgs{d.cell, static_cast<long>(d.w), 0, d.nrow, 0, d.w,
+                                     s.rowbuf + static_cast<size_t>(d.lane) * s.nr, d.slot};
+            rows.first[rows.n] = blocks;
+            blocks += static_cast<int>(cdiv(d.nrow, RW));
+            ++rows.n;
+        }
+        s.deferred.clear();
+        res = rows;
+        rows.j[rows.n] = sa;
+        rows.first[rows.n] = blocks;
+        blocks += static_cast<int>(cdiv(nrow, RW));
+        ++rows.n;
+        rows.first[rows.n] = blocks;
+        emit<MultiArgs, k_row_sums_multi, NT>(s, dim3(blocks), dim3(NT), 0, rows, stop);
+    } else {
+        emit<SumArgs, k_row_sums, NT>(s, dim3(cdiv(nrow, RW)), dim3(NT), 0, sa, stop);
+    }
     emit<MeanArgs, k_total_sub, NT>(s, dim3(std::min(nrow, MEAN_WG)), dim3(NT), 0,
-                                    MeanArgs{sa, x, n, zero, nzero}, stop);
+                                    MeanArgs{sa, x, n, zero, nzero, res}, stop);
 }
 
 void launch_ordered_sum(Solver& s, const double* a, int pitch, int i0, int i1, int j0, int j1,
