@@ -83,7 +83,7 @@ void enqueue_iteration(Solver& s, const int* stop, bool eager, hipEvent_t* ev) {
     mark(0);
     // Without per-phase events, u's ghost rows are set by the assembly launch itself.
     const bool fuse_bc = defer && s.pku.rows >= 2;
-    launch_assemble(s, true, stop, fuse_bc);
+    launch_assemble(s, true, stop, fuse_bc, face_ok(s, true));
     mark(1);
     if (!fuse_bc)
         launch_bc(s, true, stop);
@@ -91,7 +91,7 @@ void enqueue_iteration(Solver& s, const int* stop, bool eager, hipEvent_t* ev) {
     mark(3);
     launch_mom_sweeps(s, true, p.maxit_inner_u, stop, s.scal + S_RES_U, defer);
     mark(4);
-    launch_assemble(s, false, stop);
+    launch_assemble(s, false, stop, false, face_ok(s, false));
     mark(5);
     launch_bc(s, false, stop);
     mark(6);
@@ -125,9 +125,17 @@ void enqueue_iteration(Solver& s, const int* stop, bool eager, hipEvent_t* ev) {
 
 namespace {
 
+// Capture one outer iteration. The coefficient form flags are left as they were: the graph's
+// launches set them (see face_after).
 void capture_graph(Solver& s) {
     if (s.graph)
         return;
+    const bool f0 = s.face[0], f1 = s.face[1];
+    struct Restore {
+        Solver& s;
+        bool f0, f1;
+        ~Restore() { s.face[0] = f0, s.face[1] = f1; }
+    } restore{s, f0, f1};
     hipGraph_t g = nullptr;
     SG_HIP(hipStreamBeginCapture(s.stream, hipStreamCaptureModeRelaxed));
     try {
@@ -354,6 +362,10 @@ Solver* create_solver(const sg_params& p, const double* u, const double* v, cons
         s.pkv = make_pack(p, false);
         s.pku.base = dalloc<double>(s, 10 * s.pku.astride);
         s.pkv.base = dalloc<double>(s, 10 * s.pkv.astride);
+        s.fku = make_face_pack(p, true);
+        s.fkv = make_face_pack(p, false);
+        s.fku.base = dalloc<double>(s, 3 * s.fku.astride);
+        s.fkv.base = dalloc<double>(s, 3 * s.fkv.astride);
         ensure_log(s, 256);
         diffusive(p, s.diff2, s.diff4);
         if (p.pressure_solver == SG_P_MG)
@@ -544,6 +556,8 @@ int sg_advance(sg_solver* h, int max_iters, int* iters_done, double* log, int* c
             capture_graph(s);
             for (int k = 0; k < max_iters; ++k)
                 SG_HIP(hipGraphLaunch(s.graph, s.stream));
+            if (max_iters > 0) // the iterations' assemblies leave their coefficient form
+                s.face[0] = face_ok(s, true), s.face[1] = face_ok(s, false);
         }
         const Ctl c = read_ctl(s);
         const int done = std::min(c.log_pos, max_iters);
@@ -585,8 +599,10 @@ int sg_kernel_probe(sg_solver* h, int kid, int reps, double* mean_s, double* byt
                 throw ArgError("sweep probe needs momentum_solver = gs_multicolour");
             cells = is_u ? static_cast<double>(p.NX) * (p.NY - 1)
                          : static_cast<double>(p.NX - 1) * p.NY;
-            // one launch is a whole sweep (every colour)
-            bpc = 96.0; // 9 coefficients + pressure term + u read + u write, 8 B each, per cell
+            // one launch is a whole sweep (every colour): the field read and written, and the
+            // face pack's ue, vn and pressure term (face form) or the 9 coefficients and the
+            // pressure term, 8 B each, per cell
+            bpc = s.face[is_u ? 0 : 1] ? 40.0 : 96.0;
             per = is_u ? p.maxit_inner_u : p.maxit_inner_v;
             // field -> scratch: the field itself is left unchanged
             one = [&s, is_u](int) {
@@ -602,7 +618,7 @@ int sg_kernel_probe(sg_solver* h, int kid, int reps, double* mean_s, double* byt
                 throw ArgError("sweep probe needs momentum_solver = gs_multicolour");
             cells = is_u ? static_cast<double>(p.NX) * (p.NY - 1)
                          : static_cast<double>(p.NX - 1) * p.NY;
-            bpc = 104.0; // as a sweep, plus the |R| cell written
+            bpc = s.face[is_u ? 0 : 1] ? 48.0 : 104.0; // as a sweep, plus the |R| cell written
             per = 1;
             one = [&s, is_u](int) {
                 launch_mom_sweep_once(s, is_u, s.F(is_u ? SG_F_U : SG_F_V), s.scratch, true,
@@ -623,17 +639,18 @@ int sg_kernel_probe(sg_solver* h, int kid, int reps, double* mean_s, double* byt
             one = [&s](int) { launch_copy(s, s.F(SG_F_U), s.scratch, s.n, s.never); };
             break;
         case SG_K_ASSEMBLE_U:
-            cells = static_cast<double>(p.NX) * (p.NY + 1);
-            bpc = 88.0; // u, v read; 9 coefficients written
+        case SG_K_ASSEMBLE_V: {
+            // In the form the outer iterations use: u, v and p read, and the face pack and AP
+            // (face form) or the 9 coefficients and the pressure term written, 8 B each.
+            const bool is_u = kid == SG_K_ASSEMBLE_U;
+            const bool face = face_ok(s, is_u);
+            cells = is_u ? static_cast<double>(p.NX) * (p.NY + 1)
+                         : static_cast<double>(p.NX + 2) * (p.NY + 2);
+            bpc = face ? 56.0 : 104.0;
             per = 1;
-            one = [&s](int) { launch_assemble(s, true, s.never); };
+            one = [&s, is_u, face](int) { launch_assemble(s, is_u, s.never, false, face); };
             break;
-        case SG_K_ASSEMBLE_V:
-            cells = static_cast<double>(p.NX + 2) * (p.NY + 2);
-            bpc = 88.0;
-            per = 1;
-            one = [&s](int) { launch_assemble(s, false, s.never); };
-            break;
+        }
         default:
             throw ArgError("unknown kernel id");
         }

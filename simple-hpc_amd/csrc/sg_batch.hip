@@ -73,15 +73,18 @@ void build_plan(Batch& b) {
     b.recs.assign(n, {});
     for (int c = 0; c < n; ++c) {
         Solver& s = *b.cav[c];
+        const bool f0 = s.face[0], f1 = s.face[1]; // recording runs nothing
         s.rec = &b.recs[c];
         try {
             enqueue_iteration(s, &s.ctl->stop, false, nullptr);
         } catch (...) {
             s.rec = nullptr;
+            s.face[0] = f0, s.face[1] = f1;
             throw;
         }
         s.rec = nullptr;
         s.tag = 0;
+        s.face[0] = f0, s.face[1] = f1;
     }
     // Label DAG.
     std::map<Label, int> id;
@@ -441,6 +444,12 @@ int sg_batch_advance(sg_batch* h, int max_iters, int* iters, int* status, double
             if (std::all_of(b.done.begin(), b.done.end(), [](int d) { return d != 0; }))
                 break;
             const int k = std::min(left, chunk);
+            // Cavities still running assemble in this chunk, in their iterations' form.
+            for (int c = 0; c < n; ++c)
+                if (!b.done[c]) {
+                    Solver& s = *b.cav[c];
+                    s.face[0] = face_ok(s, true), s.face[1] = face_ok(s, false);
+                }
             begin_chunk(b);
             replay(b, k);
             read_state(b);

@@ -134,6 +134,9 @@ struct Solver {
     size_t lds_budget = 65536 - 1024;  // dynamic LDS of the multigrid tail (set from the device)
     std::vector<void*> owned;          // allocations to free
     Pack pku{}, pkv{};                 // packed momentum coefficients
+    Pack fku{}, fkv{};                 // face packs: ue, vn, pressure term (face form)
+    bool face[2] = {};                 // u, v: the face pack holds the current coefficients (the
+                                       // coefficient pack is stale until ensure_packed)
     int mom_tile = 0;                  // momentum sweep tile shape, an index of MOM_TILES (tunable)
     int mom_tile_first = -1;           // the first sweep's tile shape (-1: mom_tile)
     C9 diff2{}, diff4{};               // diffusive coefficients: order 2 (and band), order 4
@@ -187,7 +190,14 @@ void emit(Solver& s, dim3 grid, dim3 block, size_t lds, const A& a, const int* s
 // ---- launchers (each enqueues on s.stream; `stop` may be s.never) ----
 // momentum (sg_momentum.hip)
 // with_bc (u only): the same launch also sets u's ghost rows (launch_bc).
-void launch_assemble(Solver& s, bool is_u, const int* stop, bool with_bc = false);
+// face: the face form (face pack and AP only; the sweeps then recompute the coefficients).
+void launch_assemble(Solver& s, bool is_u, const int* stop, bool with_bc = false,
+                     bool face = false);
+// Whether outer iterations use the face form for this component (multicolour sweeps, at least one).
+bool face_ok(const Solver& s, bool is_u);
+// Bring the coefficient pack up to date if the face pack holds the current coefficients.
+void ensure_packed(Solver& s, bool is_u);
+Pack make_face_pack(const sg_params& p, bool is_u);
 void launch_bc(Solver& s, bool is_u, const int* stop);
 void launch_mom_residual(Solver& s, bool is_u, double* slot, const int* stop, bool defer);
 void launch_mom_sweeps(Solver& s, bool is_u, int nsweeps, const int* stop, double* res_slot,

@@ -49,28 +49,70 @@ __device__ __forceinline__ size_t pidx(const Pack& k, int c, int i, int kk) {
     return static_cast<size_t>(c * k.rows + (i - k.ai0)) * k.kp + kk;
 }
 
-struct AsmArgs {
-    const double* u;
-    const double* v;
-    const double* pf;
-    double* APn; // row-major AP
-    Pack pk;
-    int ld, H, NX, NY;
+// What a control volume's coefficients depend on besides its face velocities.
+struct MomConv {
+    int H, NX, NY, uds, order;
     double dx, dy;
-    double h;    // face length of the pressure term (dy for u, dx for v)
-    long poff;   // second pressure cell (+1 for u, -ld for v)
-    double* ubc; // u only: also set u's ghost rows (bc_u) in this launch, or null
-    int uds, order;
-    C9 d2, d4;
+    C9 d2, d4; // diffusive coefficients (src/coefficients.cpp:41-63, evaluated on the host)
 };
 
-// One control volume: convection (src/coefficients.cpp:15-39) plus diffusion (:41-63,
-// precomputed on the host as d2 / d4), summed element-wise (:5-8); face velocities of
-// src/momentum_u.cpp:13-16 or src/momentum_v.cpp:14-17. Written at packed entry b (and AP
-// row-major at q).
-__device__ __forceinline__ void assemble_cell(const AsmArgs& a, int i, int j, size_t q, double* b,
-                                              double ue, double uw, double vn, double vs,
-                                              double pt) {
+// The coefficients of control volume (i, j) from its face velocities, in the order AE, AW, AN,
+// AS, AEE, AWW, ANN, ASS, AP: convection (src/coefficients.cpp:15-39) plus diffusion, summed
+// element-wise (:5-8). Every coefficient any kernel uses comes from here, so the assembled
+// arrays and the coefficients a sweep recomputes from stored face velocities are the same bits.
+// The two schemes of src/coefficients.cpp:15-39 and the sum with diffusion, for one cell.
+__device__ __forceinline__ void conv_uds(const MomConv& a, double ue, double uw, double vn,
+                                         double vs, double* c) {
+    c[0] = a.dy * pos(-ue), c[1] = a.dy * pos(uw), c[2] = a.dx * pos(-vn), c[3] = a.dx * pos(vs);
+    c[4] = c[5] = c[6] = c[7] = 0.0;
+}
+__device__ __forceinline__ void conv_quick(const MomConv& a, double ue, double uw, double vn,
+                                           double vs, double* c) {
+    const double dx = a.dx, dy = a.dy;
+    c[0] = -dy * (-0.75 * pos(-ue) + 0.375 * pos(ue) - 0.125 * pos(-uw));
+    c[1] = -dy * (-0.125 * pos(ue) + 0.375 * pos(-uw) - 0.75 * pos(uw));
+    c[2] = -dx * (-0.75 * pos(-vn) + 0.375 * pos(vn) - 0.125 * pos(-vs));
+    c[3] = -dx * (-0.125 * pos(vn) + 0.375 * pos(-vn) - 0.75 * pos(vs));
+    c[4] = -0.125 * dy * pos(-ue);
+    c[5] = -0.125 * dy * pos(uw);
+    c[6] = -0.125 * dx * pos(-vn);
+    c[7] = -0.125 * dx * pos(vs);
+}
+__device__ __forceinline__ C9 conv_plus_diff(const double* c, const C9& d) {
+    // ap = s + w + n + e + ss + ww + nn + ee (src/coefficients.cpp:36-37)
+    const double ap = c[3] + c[1] + c[2] + c[0] + c[7] + c[5] + c[6] + c[4];
+    C9 r;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        r.v[k] = c[k] + d.v[k];
+    r.v[8] = ap + d.v[8];
+    return r;
+}
+
+// mom_coefs with the scheme chosen per wave where it can be: UDS runs take the UDS expressions
+// only, and a QUICK wave with no cell in the boundary band the QUICK ones only (the same
+// expressions as mom_coefs, so the same bits, without evaluating both schemes per lane).
+__device__ __forceinline__ C9 mom_coefs_w(const MomConv& a, int i, int j, double ue, double uw,
+                                          double vn, double vs) {
+    const bool band = j == a.H + 1 || j == a.NY + a.H || i == a.NX + a.H - 1 || i == a.H;
+    double c[8];
+    if (a.uds) {
+        conv_uds(a, ue, uw, vn, vs, c);
+        return conv_plus_diff(c, (a.order == 2 || band) ? a.d2 : a.d4);
+    }
+    if (!__any(band)) {
+        conv_quick(a, ue, uw, vn, vs, c);
+        return conv_plus_diff(c, a.order == 2 ? a.d2 : a.d4);
+    }
+    if (band)
+        conv_uds(a, ue, uw, vn, vs, c);
+    else
+        conv_quick(a, ue, uw, vn, vs, c);
+    return conv_plus_diff(c, (a.order == 2 || band) ? a.d2 : a.d4);
+}
+
+__device__ __forceinline__ C9 mom_coefs(const MomConv& a, int i, int j, double ue, double uw,
+                                        double vn, double vs) {
     const bool band = j == a.H + 1 || j == a.NY + a.H || i == a.NX + a.H - 1 || i == a.H;
     const double dx = a.dx, dy = a.dy;
     double e, w, n, s, ee, ww, nn, ss;
@@ -92,18 +134,94 @@ __device__ __forceinline__ void assemble_cell(const AsmArgs& a, int i, int j, si
     }
     const double ap = s + w + n + e + ss + ww + nn + ee;
     const C9& d = (a.order == 2 || band) ? a.d2 : a.d4;
+    C9 r;
+    r.v[0] = e + d.v[0];
+    r.v[1] = w + d.v[1];
+    r.v[2] = n + d.v[2];
+    r.v[3] = s + d.v[3];
+    r.v[4] = ee + d.v[4];
+    r.v[5] = ww + d.v[5];
+    r.v[6] = nn + d.v[6];
+    r.v[7] = ss + d.v[7];
+    r.v[8] = ap + d.v[8];
+    return r;
+}
+
+// Face form. The face velocities of src/momentum_u.cpp:13-16 and src/momentum_v.cpp:14-17 are
+// shared bit for bit by neighbouring control volumes: the west face of (i, j) is the east face of
+// (i, j - 1) and the south face of (i, j) the north face of (i + 1, j) (the same two velocities
+// added, in the other order, which gives the same sum). So a component's coefficients are fixed by
+// three colour-packed arrays, ue, vn and the pressure term, over the assembly range grown by a
+// column to the west and a row to the south (F_UE, F_VN, F_PT of the face pack), and a sweep
+// recomputes them with mom_coefs instead of streaming the nine assembled arrays.
+enum { F_UE = 0, F_VN = 1, F_PT = 2, F_NARR = 3 };
+
+// Face velocities (ue, vn) of the face-pack cell (i, j) for u or v.
+template <bool IS_U>
+__device__ __forceinline__ double face_ue(const double* u, size_t q, long ld) {
+    return IS_U ? (u[q] + u[q + 1]) / 2 : (u[q - ld] + u[q]) / 2;
+}
+template <bool IS_U>
+__device__ __forceinline__ double face_vn(const double* v, size_t q, long ld) {
+    return IS_U ? (v[q] + v[q + 1]) / 2 : (v[q] + v[q - ld]) / 2;
+}
+
+// The face values of cell (i, j) of colour c from the face pack fk: ue, uw, vn, vs and the
+// pressure term.
+// Packed index of cell (i, j) of colour c: the cells of one colour in a row are NCOL columns
+// apart starting less than NCOL columns after aj0, so the k-th of them is (j - aj0) / NCOL. (An
+// array of a face pack has fewer than 2^31 entries: ncol x rows x kp.)
+template <int NCOL> __device__ __forceinline__ int fidx(const Pack& k, int c, int i, int j) {
+    return (c * k.rows + (i - k.ai0)) * k.kp + (j - k.aj0) / NCOL;
+}
+template <int NCOL>
+__device__ __forceinline__ void face_load(const Pack& fk, int c, int i, int j, double* f) {
+    const double* fue = fk.base + F_UE * fk.astride;
+    const double* fvn = fk.base + F_VN * fk.astride;
+    const double* fpt = fk.base + F_PT * fk.astride;
+    const int cw = NCOL == 2 ? (c + 1) & 1 : (c + 2) % 5; // colour of (i, j - 1)
+    const int cs = NCOL == 2 ? (c + 1) & 1 : (c + 1) % 5; // colour of (i + 1, j)
+    const int o = fidx<NCOL>(fk, c, i, j);
+    f[0] = fue[o];
+    f[1] = fue[fidx<NCOL>(fk, cw, i, j - 1)];
+    f[2] = fvn[o];
+    f[3] = fvn[fidx<NCOL>(fk, cs, i + 1, j)];
+    f[4] = fpt[o];
+}
+
+// The coefficients of cell (i, j) of colour c from the face pack fk.
+template <int NCOL>
+__device__ __forceinline__ C9 face_coefs(const Pack& fk, const MomConv& cv, int c, int i, int j,
+                                         double& pt) {
+    double f[5];
+    face_load<NCOL>(fk, c, i, j, f);
+    pt = f[4];
+    return mom_coefs(cv, i, j, f[0], f[1], f[2], f[3]);
+}
+
+struct AsmArgs {
+    const double* u;
+    const double* v;
+    const double* pf;
+    double* APn; // row-major AP
+    Pack pk;     // the coefficient pack, or the face pack (face form)
+    int ld;
+    double h;    // face length of the pressure term (dy for u, dx for v)
+    long poff;   // second pressure cell (+1 for u, -ld for v)
+    double* ubc; // u only: also set u's ghost rows (bc_u) in this launch, or null
+    MomConv cv;
+};
+
+// One control volume, written at packed entry b (and AP row-major at q).
+__device__ __forceinline__ void assemble_cell(const AsmArgs& a, int i, int j, size_t q, double* b,
+                                              double ue, double uw, double vn, double vs,
+                                              double pt) {
+    const C9 c = mom_coefs(a.cv, i, j, ue, uw, vn, vs);
     const size_t as = a.pk.astride;
-    b[0 * as] = e + d.v[0];
-    b[1 * as] = w + d.v[1];
-    b[2 * as] = n + d.v[2];
-    b[3 * as] = s + d.v[3];
-    b[4 * as] = ee + d.v[4];
-    b[5 * as] = ww + d.v[5];
-    b[6 * as] = nn + d.v[6];
-    b[7 * as] = ss + d.v[7];
-    const double app = ap + d.v[8];
-    b[8 * as] = app;
-    a.APn[q] = app;
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+        b[k * as] = c.v[k];
+    a.APn[q] = c.v[8];
     b[9 * as] = pt;
 }
 
@@ -112,6 +230,27 @@ __device__ __forceinline__ void assemble_cell(const AsmArgs& a, int i, int j, si
 // rows per workgroup: the rows above and below that a cell's faces read are mostly its
 // neighbours' own rows, already in L1.
 constexpr int AY = 16;
+
+// Ghost rows of u (src/momentum_u.cpp:24-34), which the assembly does not read, set by the
+// threads of the first two rows of an assembly launch for the columns of their cells: row 0
+// the lid, row 1 the south wall.
+__device__ __forceinline__ void ghost_rows_u(const AsmArgs& a, int row, const int* jj) {
+    double* u = a.ubc;
+    const long ld = a.ld;
+    const int H = a.cv.H;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+        const int j = jj[c];
+        if (j < H + 1 || j > a.cv.NY + H)
+            continue;
+        if (row == 0)
+            u[static_cast<size_t>(H - 1) * ld + j] = 2 - u[static_cast<size_t>(H) * ld + j];
+        else
+            u[static_cast<size_t>(a.cv.NX + H) * ld + j] =
+                -u[static_cast<size_t>(a.cv.NX + H - 1) * ld + j];
+    }
+}
+
 template <bool IS_U> __device__ __forceinline__ void k_assemble(const AsmArgs& a, const Blk& bk) {
     const Pack& pk = a.pk;
     const int kk = bk.x * BX + threadIdx.x;
@@ -150,24 +289,94 @@ template <bool IS_U> __device__ __forceinline__ void k_assemble(const AsmArgs& a
         assemble_cell(a, i, jj[c], q, pk.base + pidx(pk, c, i, kk), fue[c], fuw[c], fvn[c],
                       fvs[c], fpt[c]);
     }
-    // Ghost rows of u (src/momentum_u.cpp:24-34), which the assembly above does not read: the
-    // threads of the first row take the lid, those of the second the south wall, for the
-    // columns of their cells.
-    if (IS_U && a.ubc && i - pk.ai0 < 2) {
-        double* u = a.ubc;
-        const int H = a.H;
+    if (IS_U && a.ubc && i - pk.ai0 < 2)
+        ghost_rows_u(a, i - pk.ai0, jj);
+}
+
+// Face form of the assembly: ue, vn and the pressure term of every face-pack cell (the extra
+// column needs only ue and the extra row only vn: they are the west faces of the first column and
+// the south faces of the last row), and AP of every assembly-range cell, row-major (assemble_p and
+// correct read it). With ubc, u's ghost rows as in k_assemble.
+template <bool IS_U>
+__device__ __forceinline__ void k_assemble_face(const AsmArgs& a, const Blk& bk) {
+    const Pack& fk = a.pk;
+    const int kk = bk.x * BX + threadIdx.x;
+    const int i = fk.ai0 + bk.y * AY + threadIdx.y;
+    if (i > fk.ai1)
+        return;
+    const long ld = a.ld;
+    const size_t fs = fk.astride;
+    const bool xrow = i == fk.ai1; // the extra row
+    constexpr int MC = 5;
+    double fue[MC], fuw[MC], fvn[MC], fvs[MC], fpt[MC];
+    int jj[MC];
 #pragma unroll
-        for (int c = 0; c < MC; ++c) {
-            const int j = jj[c];
-            if (j < H + 1 || j > a.NY + H)
-                continue;
-            if (i == pk.ai0)
-                u[static_cast<size_t>(H - 1) * ld + j] = 2 - u[static_cast<size_t>(H) * ld + j];
-            else
-                u[static_cast<size_t>(a.NX + H) * ld + j] =
-                    -u[static_cast<size_t>(a.NX + H - 1) * ld + j];
+    for (int c = 0; c < MC; ++c) {
+        const int j = jfirst(fk, c < fk.ncol ? c : 0, i) + fk.ncol * kk;
+        jj[c] = (c < fk.ncol && j <= fk.aj1) ? j : -1;
+        fue[c] = fuw[c] = fvn[c] = fvs[c] = fpt[c] = 0.0;
+        if (jj[c] < 0)
+            continue;
+        const size_t q = static_cast<size_t>(i) * ld + j;
+        const bool xcol = j == fk.aj0; // the extra column
+        if (!xcol)
+            fvn[c] = face_vn<IS_U>(a.v, q, ld);
+        if (!xrow)
+            fue[c] = face_ue<IS_U>(a.u, q, ld);
+        if (!xrow && !xcol) {
+            if (IS_U) {
+                fuw[c] = (a.u[q] + a.u[q - 1]) / 2;
+                fvs[c] = (a.v[q + ld] + a.v[q + ld + 1]) / 2;
+            } else {
+                fuw[c] = (a.u[q - 1] + a.u[q - ld - 1]) / 2;
+                fvs[c] = (a.v[q] + a.v[q + ld]) / 2;
+            }
+            fpt[c] = (a.pf[q] - a.pf[q + a.poff]) * a.h;
         }
     }
+#pragma unroll
+    for (int c = 0; c < MC; ++c) {
+        const int j = jj[c];
+        if (j < 0)
+            continue;
+        const bool xcol = j == fk.aj0;
+        double* b = fk.base + pidx(fk, c, i, kk);
+        if (!xrow)
+            b[F_UE * fs] = fue[c];
+        if (!xcol)
+            b[F_VN * fs] = fvn[c];
+        if (!xrow && !xcol) {
+            b[F_PT * fs] = fpt[c];
+            a.APn[static_cast<size_t>(i) * ld + j] =
+                mom_coefs(a.cv, i, j, fue[c], fuw[c], fvn[c], fvs[c]).v[8];
+        }
+    }
+    if (IS_U && a.ubc && i - fk.ai0 < 2)
+        ghost_rows_u(a, i - fk.ai0, jj);
+}
+
+// Face form -> the coefficient pack (and AP row-major) over the assembly range.
+struct ExpArgs {
+    Pack fk, pk;
+    MomConv cv;
+    double* APn;
+    int ld;
+};
+template <int NCOL> __global__ void __launch_bounds__(BX* BY) k_expand(ExpArgs e) {
+    const Pack& pk = e.pk;
+    const int j = pk.aj0 + blockIdx.x * BX + threadIdx.x;
+    const int i = pk.ai0 + blockIdx.y * BY + threadIdx.y;
+    if (i > pk.ai1 || j > pk.aj1)
+        return;
+    const int c = colour_of(NCOL, i, j);
+    double pt;
+    const C9 co = face_coefs<NCOL>(e.fk, e.cv, c, i, j, pt);
+    double* b = pk.base + pidx(pk, c, i, (j - jfirst(pk, c, i)) / NCOL);
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+        b[k * pk.astride] = co.v[k];
+    b[9 * pk.astride] = pt;
+    e.APn[static_cast<size_t>(i) * e.ld + j] = co.v[8];
 }
 
 // Ghost cells: u lid 2 - u and south -u (src/momentum_u.cpp:24-34); v west/east -v
@@ -201,7 +410,9 @@ __device__ __forceinline__ void k_bc_v(const BcArgs& a, const Blk& bk) {
 
 struct MomArgs {
     double* x;
-    Pack pk;
+    Pack pk; // the coefficient pack
+    Pack fk; // the face pack
+    MomConv cv;
     int ld;
     int i0, i1, j0, j1; // sweep / residual range
     double alpha, one_minus_alpha;
@@ -270,7 +481,9 @@ struct SweepArgs {
 // (src/momentum_u.cpp:48-50) is evaluated from that copy during the cell's own colour pass, with
 // the coefficients the update loads anyway. (Only that ring, not the whole halo: the two copies
 // then fit two workgroups per CU.)
-template <int NCOL, int TI, int TJ, bool FIRST, int NT = ST>
+// FACE: the coefficients are recomputed from the face pack (face form) instead of read from the
+// coefficient pack.
+template <int NCOL, int TI, int TJ, bool FIRST, int NT, bool FACE>
 __device__ __forceinline__ void k_mom_sweep(const SweepArgs& sa, const Blk& bk) {
     const MomArgs& m = sa.m;
     const double* src = sa.src;
@@ -319,7 +532,76 @@ __device__ __forceinline__ void k_mom_sweep(const SweepArgs& sa, const Blk& bk) 
     const int rw = m.j1 - m.j0 + 1;
 #define SU(i, j) su[((i) - gi0) * LW + ((j) - gj0)]
 #define S0(i, j) su0[((i) - (ti0 - R)) * W0 + ((j) - (tj0 - R))]
-    for (int k = 0; k < NCOL; ++k) {
+    // One cell's update (and, in the core of the first sweep, its residual) with its coefficients.
+    auto update = [&](int i, int j, const C9& co, double ct) {
+        const double ce = co.v[0], cw = co.v[1], cn = co.v[2], cs = co.v[3];
+        const double cee = co.v[4], cww = co.v[5], cnn = co.v[6], css = co.v[7], cp = co.v[8];
+        if (FIRST && i >= ti0 && i < ti0 + TI && j >= tj0 && j < tj0 + TJ) {
+            const double nb0 = (ce * S0(i, j + 1) + cw * S0(i, j - 1) + cn * S0(i - 1, j) +
+                                cs * S0(i + 1, j)) +
+                               (cee * S0(i, j + 2) + cww * S0(i, j - 2) + cnn * S0(i - 2, j) +
+                                css * S0(i + 2, j));
+            res[(i - ti0) * TJ + (j - tj0)] = fabs(nb0 + ct - cp * S0(i, j));
+        }
+        double nb = ce * SU(i, j + 1) + cw * SU(i, j - 1) + cn * SU(i - 1, j) + cs * SU(i + 1, j);
+        if (NCOL != 2)
+            nb = nb + (cee * SU(i, j + 2) + cww * SU(i, j - 2) + cnn * SU(i - 2, j) +
+                       css * SU(i + 2, j));
+        SU(i, j) = m.alpha * (nb + ct) / cp + m.one_minus_alpha * SU(i, j);
+    };
+    if constexpr (FACE) {
+        // The face values do not depend on the field, so each thread loads those of its cells of
+        // pass k + 1 before it updates its cells of pass k: their latency hides behind pass k's
+        // arithmetic and barrier. Cell u of pass k is the thread's (u * NT + tid)-th of the pass.
+        constexpr int DM = R * (NCOL - 1);
+        constexpr int MU = ((TI + 2 * DM) * ((TJ + 2 * DM + NCOL) / NCOL) + NT - 1) / NT;
+        double fv[2][MU][5];
+        int fi[2][MU], fj[2][MU];
+        auto fetch = [&](int k, double (&f)[MU][5], int (&ci)[MU], int (&cj)[MU]) {
+            const int d = R * (NCOL - 1 - k);
+            const int ia = max(m.i0, ti0 - d), ib = min(m.i1, ti0 + TI - 1 + d);
+            const int ja = max(m.j0, tj0 - d), jb = min(m.j1, tj0 + TJ - 1 + d);
+            const int cpr = (jb - ja + NCOL) / NCOL;
+            const int n = ia <= ib && ja <= jb ? (ib - ia + 1) * cpr : 0;
+            // e / cpr in FP32: e < 2^16 and cpr < 2^12, so (e + 0.5) / cpr is at least 1 / (2 cpr)
+            // away from an integer, far more than the rounding error.
+            const float rcp = 1.0f / static_cast<float>(max(cpr, 1));
+#pragma unroll
+            for (int u = 0; u < MU; ++u) {
+                const int e = u * NT + threadIdx.x;
+                cj[u] = -1;
+                if (e >= n)
+                    continue;
+                const int r = static_cast<int>((static_cast<float>(e) + 0.5f) * rcp);
+                const int t = e - r * cpr, i = ia + r;
+                const int c0 = colour_of(NCOL, i, ja);
+                const int j =
+                    ja + (NCOL == 2 ? ((k - c0) & 1) : (2 * (k - c0) + 10) % 5) + NCOL * t;
+                if (j > jb)
+                    continue;
+                ci[u] = i;
+                cj[u] = j;
+                face_load<NCOL>(m.fk, k, i, j, f[u]);
+            }
+        };
+        fetch(0, fv[0], fi[0], fj[0]);
+#pragma unroll
+        for (int k = 0; k < NCOL; ++k) {
+            if (k + 1 < NCOL)
+                fetch(k + 1, fv[(k + 1) & 1], fi[(k + 1) & 1], fj[(k + 1) & 1]);
+#pragma unroll
+            for (int u = 0; u < MU; ++u) {
+                const int j = fj[k & 1][u];
+                if (j < 0)
+                    continue;
+                const int i = fi[k & 1][u];
+                const double* f = fv[k & 1][u];
+                update(i, j, mom_coefs_w(m.cv, i, j, f[0], f[1], f[2], f[3]), f[4]);
+            }
+            __syncthreads();
+        }
+    }
+    for (int k = 0; k < (FACE ? 0 : NCOL); ++k) {
         const int d = R * (NCOL - 1 - k);
         const int ia = max(m.i0, ti0 - d), ib = min(m.i1, ti0 + TI - 1 + d);
         const int ja = max(m.j0, tj0 - d), jb = min(m.j1, tj0 + TJ - 1 + d);
@@ -332,30 +614,19 @@ __device__ __forceinline__ void k_mom_sweep(const SweepArgs& sa, const Blk& bk) 
                 const int j = ja + (NCOL == 2 ? ((k - c0) & 1) : (2 * (k - c0) + 10) % 5) + NCOL * t;
                 if (j > jb)
                     continue;
-                const double* co = pk.base + pidx(pk, k, i, (j - jfirst(pk, k, i)) / NCOL);
-                const double ce = co[0], cw = co[as], cn = co[2 * as], cs = co[3 * as];
-                const double cp = co[8 * as], ct = co[9 * as];
-                double cee = 0.0, cww = 0.0, cnn = 0.0, css = 0.0;
                 const bool core = FIRST && i >= ti0 && i < ti0 + TI && j >= tj0 && j < tj0 + TJ;
+                const double* co = pk.base + pidx(pk, k, i, (j - jfirst(pk, k, i)) / NCOL);
+                C9 c;
+                c.v[0] = co[0], c.v[1] = co[as], c.v[2] = co[2 * as], c.v[3] = co[3 * as];
+                c.v[4] = c.v[5] = c.v[6] = c.v[7] = 0.0;
+                c.v[8] = co[8 * as];
                 if (NCOL != 2 || core) {
-                    cee = co[4 * as];
-                    cww = co[5 * as];
-                    cnn = co[6 * as];
-                    css = co[7 * as];
+                    c.v[4] = co[4 * as];
+                    c.v[5] = co[5 * as];
+                    c.v[6] = co[6 * as];
+                    c.v[7] = co[7 * as];
                 }
-                if (core) {
-                    const double nb0 = (ce * S0(i, j + 1) + cw * S0(i, j - 1) + cn * S0(i - 1, j) +
-                                        cs * S0(i + 1, j)) +
-                                       (cee * S0(i, j + 2) + cww * S0(i, j - 2) +
-                                        cnn * S0(i - 2, j) + css * S0(i + 2, j));
-                    res[(i - ti0) * TJ + (j - tj0)] = fabs(nb0 + ct - cp * S0(i, j));
-                }
-                double nb = ce * SU(i, j + 1) + cw * SU(i, j - 1) + cn * SU(i - 1, j) +
-                            cs * SU(i + 1, j);
-                if (NCOL != 2)
-                    nb = nb + (cee * SU(i, j + 2) + cww * SU(i, j - 2) + cnn * SU(i - 2, j) +
-                               css * SU(i + 2, j));
-                SU(i, j) = m.alpha * (nb + ct) / cp + m.one_minus_alpha * SU(i, j);
+                update(i, j, c, co[9 * as]);
             }
         }
         __syncthreads();
@@ -435,11 +706,25 @@ __global__ void __launch_bounds__(BX* BY) k_pterm(Pack pk, const double* pf, int
         (pf[q] - pf[q + poff]) * h;
 }
 
+MomConv mom_conv(const Solver& s) {
+    const sg_params& p = s.p;
+    MomConv c{};
+    c.H = p.H, c.NX = p.NX, c.NY = p.NY;
+    c.uds = p.convection_scheme == SG_UDS;
+    c.order = p.diffusion_order;
+    c.dx = p.dx, c.dy = p.dy;
+    c.d2 = s.diff2;
+    c.d4 = s.diff4;
+    return c;
+}
+
 MomArgs mom_args(Solver& s, bool is_u) {
     const sg_params& p = s.p;
     MomArgs m{};
     m.x = s.F(is_u ? SG_F_U : SG_F_V);
     m.pk = is_u ? s.pku : s.pkv;
+    m.fk = is_u ? s.fku : s.fkv;
+    m.cv = mom_conv(s);
     m.ld = s.ld;
     const int H = p.H;
     if (is_u) { // rows [H, NX+H-1], columns [H+1, NY+H-1]
@@ -467,6 +752,18 @@ int ncolours(const sg_params& p) {
     return (p.convection_scheme == SG_UDS && p.diffusion_order == 2) ? 2 : 5;
 }
 
+// Face-pack geometry: the assembly range grown by a column to the west and a row to the south.
+Pack make_face_pack(const sg_params& p, bool is_u) {
+    Pack k = make_pack(p, is_u);
+    k.aj0 -= 1;
+    k.ai1 += 1;
+    k.rows = k.ai1 - k.ai0 + 1;
+    const int w = k.aj1 - k.aj0 + 1;
+    k.kp = ((w + k.ncol - 1) / k.ncol + 15) / 16 * 16;
+    k.astride = static_cast<size_t>(k.ncol) * k.rows * k.kp;
+    return k;
+}
+
 // Packed-layout geometry over the assembly range (src/momentum_u.cpp:11-12, momentum_v.cpp:12-13).
 Pack make_pack(const sg_params& p, bool is_u) {
     Pack k{};
@@ -482,28 +779,52 @@ Pack make_pack(const sg_params& p, bool is_u) {
     return k;
 }
 
-void launch_assemble(Solver& s, bool is_u, const int* stop, bool with_bc) {
+void launch_assemble(Solver& s, bool is_u, const int* stop, bool with_bc, bool face) {
     const sg_params& p = s.p;
     AsmArgs a{};
     a.u = s.F(SG_F_U);
     a.v = s.F(SG_F_V);
     a.pf = s.F(SG_F_P);
     a.APn = s.F(is_u ? SG_F_CU_AP : SG_F_CV_AP);
-    a.pk = is_u ? s.pku : s.pkv;
-    a.ld = s.ld, a.H = p.H, a.NX = p.NX, a.NY = p.NY;
-    a.dx = p.dx, a.dy = p.dy;
+    a.pk = face ? (is_u ? s.fku : s.fkv) : (is_u ? s.pku : s.pkv);
+    a.ld = s.ld;
     a.h = is_u ? p.dy : p.dx;
     a.poff = is_u ? 1 : -static_cast<long>(s.ld);
     a.ubc = is_u && with_bc ? s.F(SG_F_U) : nullptr;
-    a.uds = p.convection_scheme == SG_UDS;
-    a.order = p.diffusion_order;
-    a.d2 = s.diff2;
-    a.d4 = s.diff4;
+    a.cv = mom_conv(s);
     const dim3 grid(cdiv(a.pk.kp, BX), cdiv(a.pk.rows, AY));
-    if (is_u)
+    if (face) {
+        if (is_u)
+            emit<AsmArgs, k_assemble_face<true>, BX * AY>(s, grid, dim3(BX, AY), 0, a, stop);
+        else
+            emit<AsmArgs, k_assemble_face<false>, BX * AY>(s, grid, dim3(BX, AY), 0, a, stop);
+    } else if (is_u) {
         emit<AsmArgs, k_assemble<true>, BX * AY>(s, grid, dim3(BX, AY), 0, a, stop);
-    else
+    } else {
         emit<AsmArgs, k_assemble<false>, BX * AY>(s, grid, dim3(BX, AY), 0, a, stop);
+    }
+    s.face[is_u ? 0 : 1] = face;
+}
+
+bool face_ok(const Solver& s, bool is_u) {
+    const sg_params& p = s.p;
+    return p.momentum_solver == SG_MOM_GS_MULTICOLOUR &&
+           (is_u ? p.maxit_inner_u : p.maxit_inner_v) > 0;
+}
+
+void ensure_packed(Solver& s, bool is_u) {
+    if (!s.face[is_u ? 0 : 1])
+        return;
+    s.face[is_u ? 0 : 1] = false;
+    const Pack& pk = is_u ? s.pku : s.pkv;
+    const ExpArgs e{is_u ? s.fku : s.fkv, pk, mom_conv(s),
+                    s.F(is_u ? SG_F_CU_AP : SG_F_CV_AP), s.ld};
+    const dim3 grid(cdiv(pk.aj1 - pk.aj0 + 1, BX), cdiv(pk.rows, BY)), block(BX, BY);
+    if (pk.ncol == 2)
+        hipLaunchKernelGGL(k_expand<2>, grid, block, 0, s.stream, e);
+    else
+        hipLaunchKernelGGL(k_expand<5>, grid, block, 0, s.stream, e);
+    SG_HIP(hipGetLastError());
 }
 
 void launch_bc(Solver& s, bool is_u, const int* stop) {
@@ -519,6 +840,7 @@ void launch_bc(Solver& s, bool is_u, const int* stop) {
 // residual_u / residual_v (src/momentum_u.cpp:43-59): |R| per cell, then the ordered sum into
 // *slot (deferred to flush_sums if defer).
 void launch_mom_residual(Solver& s, bool is_u, double* slot, const int* stop, bool defer) {
+    ensure_packed(s, is_u);
     const MomArgs m = mom_args(s, is_u);
     const int nrow = m.i1 - m.i0 + 1, w = m.j1 - m.j0 + 1;
     double* cell = s.rcell[is_u ? 0 : 1];
@@ -537,30 +859,37 @@ namespace {
 // Threads of a regular sweep's workgroup (the first sweep, with its extra LDS, uses ST).
 constexpr int SW_NT = 512;
 
-template <int NCOL, bool FIRST, int K>
+template <int NCOL, bool FIRST, bool FACE, int K>
 void sweep_shape(Solver& s, const SweepArgs& a, const int* stop) {
     constexpr int NT = FIRST ? ST : SW_NT, TI = MOM_TILES[K].ti, TJ = MOM_TILES[K].tj;
-    emit<SweepArgs, k_mom_sweep<NCOL, TI, TJ, FIRST, NT>, NT>(
+    emit<SweepArgs, k_mom_sweep<NCOL, TI, TJ, FIRST, NT, FACE>, NT>(
         s, dim3(cdiv(s.nc, TJ), cdiv(s.nr, TI)), dim3(NT), 0, a, stop);
 }
 
+// Face form: every tile shape (tunable). Coefficient pack (the per-phase entry points after an
+// explicit assembly or upload): one shape.
+constexpr int PACK_TILE = 4;
 template <int NCOL, bool FIRST, int... K>
-void sweep_tile_f(Solver& s, const SweepArgs& a, const int* stop,
+void sweep_tile_f(Solver& s, const SweepArgs& a, bool face, const int* stop,
                   std::integer_sequence<int, K...>) {
     static_assert(sizeof...(K) == N_MOM_TILES);
+    if (!face) {
+        sweep_shape<NCOL, FIRST, false, PACK_TILE>(s, a, stop);
+        return;
+    }
     const int t = FIRST && s.mom_tile_first >= 0 ? s.mom_tile_first : s.mom_tile;
-    ((t == K ? sweep_shape<NCOL, FIRST, K>(s, a, stop) : void()), ...);
+    ((t == K ? sweep_shape<NCOL, FIRST, true, K>(s, a, stop) : void()), ...);
 }
 
 template <int NCOL>
 void sweep_tile(Solver& s, const MomArgs& m, const double* src, double* dst, double* cell,
-                int first, const int* stop) {
+                int first, bool face, const int* stop) {
     const SweepArgs a{m, src, dst, cell, s.nr, s.nc, first};
     constexpr auto shapes = std::make_integer_sequence<int, N_MOM_TILES>{};
     if (first)
-        sweep_tile_f<NCOL, true>(s, a, stop, shapes);
+        sweep_tile_f<NCOL, true>(s, a, face, stop, shapes);
     else
-        sweep_tile_f<NCOL, false>(s, a, stop, shapes);
+        sweep_tile_f<NCOL, false>(s, a, face, stop, shapes);
 }
 } // namespace
 
@@ -570,10 +899,11 @@ void launch_mom_sweep_once(Solver& s, bool is_u, const double* src, double* dst,
                            const int* stop) {
     const MomArgs m = mom_args(s, is_u);
     double* cell = s.rcell[is_u ? 0 : 1];
+    const bool face = s.face[is_u ? 0 : 1];
     if (m.pk.ncol == 2)
-        sweep_tile<2>(s, m, src, dst, cell, first ? 1 : 0, stop);
+        sweep_tile<2>(s, m, src, dst, cell, first ? 1 : 0, face, stop);
     else
-        sweep_tile<5>(s, m, src, dst, cell, first ? 1 : 0, stop);
+        sweep_tile<5>(s, m, src, dst, cell, first ? 1 : 0, face, stop);
 }
 
 // sweep_u_batch / sweep_v_batch (src/momentum_u.cpp:164-171) for the two parallel solvers. With
@@ -608,6 +938,7 @@ void launch_mom_sweeps(Solver& s, bool is_u, int nsweeps, const int* stop, doubl
         return;
     }
     // Jacobi batch: full copy, then alternate so that the last pass writes the field.
+    ensure_packed(s, is_u);
     launch_copy(s, m.x, s.scratch, s.n, stop);
     if (m.i1 < m.i0 || m.j1 < m.j0)
         return;
@@ -625,6 +956,12 @@ void launch_mom_sweeps(Solver& s, bool is_u, int nsweeps, const int* stop, doubl
 void sync_packed(Solver& s, int id, bool to_packed) {
     const bool cu = id >= SG_F_CU_AE && id <= SG_F_CU_AP;
     const bool cv = id >= SG_F_CV_AE && id <= SG_F_CV_AP;
+    if (cu || cv)
+        ensure_packed(s, cu);
+    else if (id == SG_F_P && to_packed) {
+        ensure_packed(s, true);
+        ensure_packed(s, false);
+    }
     if (cu || cv) {
         const Pack& pk = cu ? s.pku : s.pkv;
         const int a = id - (cu ? SG_F_CU_AE : SG_F_CV_AE);

@@ -9,8 +9,10 @@
 // per row walks its row in order from LDS (reads a batch ahead of the adds), so each chain waits
 // on FP64 adds rather than memory. Total: one workgroup loads the row sums into LDS
 // the same way and one lane adds them in row order. At 1024 x 1024 each launch takes about 14 us.
-// The residual sums, which feed only the stopping rule, are deferred and done together (one launch
-// for the rows of all three, one for the totals) just before it.
+// The residual sums, which feed only the stopping rule, are deferred and done together: with the
+// p' solve's first mean removal when it has one (their rows in its row launch, their totals in
+// its total launch, each in its own wave), else in one launch for the rows of all three and one
+// for the totals and the rule.
 // (A single launch that hands the row sums to the last workgroup through a ticket needs a
 // device-scope fence in every workgroup; measured at the same size it took 34-48 us.)
 #include "sg_internal.hpp"
