@@ -130,12 +130,11 @@ namespace {
 void capture_graph(Solver& s) {
     if (s.graph)
         return;
-    const bool f0 = s.face[0], f1 = s.face[1];
     struct Restore {
         Solver& s;
-        bool f0, f1;
-        ~Restore() { s.face[0] = f0, s.face[1] = f1; }
-    } restore{s, f0, f1};
+        bool f0, f1, p0, p1;
+        ~Restore() { s.face[0] = f0, s.face[1] = f1, s.pt_stale[0] = p0, s.pt_stale[1] = p1; }
+    } restore{s, s.face[0], s.face[1], s.pt_stale[0], s.pt_stale[1]};
     hipGraph_t g = nullptr;
     SG_HIP(hipStreamBeginCapture(s.stream, hipStreamCaptureModeRelaxed));
     try {
@@ -556,8 +555,10 @@ int sg_advance(sg_solver* h, int max_iters, int* iters_done, double* log, int* c
             capture_graph(s);
             for (int k = 0; k < max_iters; ++k)
                 SG_HIP(hipGraphLaunch(s.graph, s.stream));
-            if (max_iters > 0) // the iterations' assemblies leave their coefficient form
+            if (max_iters > 0) { // the iterations' assemblies and corrections
                 s.face[0] = face_ok(s, true), s.face[1] = face_ok(s, false);
+                s.pt_stale[0] = s.pt_stale[1] = true;
+            }
         }
         const Ctl c = read_ctl(s);
         const int done = std::min(c.log_pos, max_iters);

@@ -73,18 +73,22 @@ void build_plan(Batch& b) {
     b.recs.assign(n, {});
     for (int c = 0; c < n; ++c) {
         Solver& s = *b.cav[c];
-        const bool f0 = s.face[0], f1 = s.face[1]; // recording runs nothing
+        // Recording runs nothing: the coefficient form and pressure-term flags stay.
+        const bool f0 = s.face[0], f1 = s.face[1], p0 = s.pt_stale[0], p1 = s.pt_stale[1];
+        auto restore = [&] {
+            s.face[0] = f0, s.face[1] = f1, s.pt_stale[0] = p0, s.pt_stale[1] = p1;
+        };
         s.rec = &b.recs[c];
         try {
             enqueue_iteration(s, &s.ctl->stop, false, nullptr);
         } catch (...) {
             s.rec = nullptr;
-            s.face[0] = f0, s.face[1] = f1;
+            restore();
             throw;
         }
         s.rec = nullptr;
         s.tag = 0;
-        s.face[0] = f0, s.face[1] = f1;
+        restore();
     }
     // Label DAG.
     std::map<Label, int> id;
@@ -449,6 +453,7 @@ int sg_batch_advance(sg_batch* h, int max_iters, int* iters, int* status, double
                 if (!b.done[c]) {
                     Solver& s = *b.cav[c];
                     s.face[0] = face_ok(s, true), s.face[1] = face_ok(s, false);
+                    s.pt_stale[0] = s.pt_stale[1] = true;
                 }
             begin_chunk(b);
             replay(b, k);

@@ -137,6 +137,7 @@ struct Solver {
     Pack fku{}, fkv{};                 // face packs: ue, vn, pressure term (face form)
     bool face[2] = {};                 // u, v: the face pack holds the current coefficients (the
                                        // coefficient pack is stale until ensure_packed)
+    bool pt_stale[2] = {};             // u, v: p changed since the pressure terms were evaluated
     int mom_tile = 0;                  // momentum sweep tile shape, an index of MOM_TILES (tunable)
     int mom_tile_first = -1;           // the first sweep's tile shape (-1: mom_tile)
     C9 diff2{}, diff4{};               // diffusive coefficients: order 2 (and band), order 4
@@ -197,6 +198,9 @@ void launch_assemble(Solver& s, bool is_u, const int* stop, bool with_bc = false
 bool face_ok(const Solver& s, bool is_u);
 // Bring the coefficient pack up to date if the face pack holds the current coefficients.
 void ensure_packed(Solver& s, bool is_u);
+// Re-evaluate the component's pressure terms from the current p if it changed since (correct,
+// an upload of p), in the form that holds the coefficients.
+void refresh_pterm(Solver& s, bool is_u);
 Pack make_face_pack(const sg_params& p, bool is_u);
 void launch_bc(Solver& s, bool is_u, const int* stop);
 void launch_mom_residual(Solver& s, bool is_u, double* slot, const int* stop, bool defer);

@@ -60,23 +60,32 @@ struct MomConv {
 // AS, AEE, AWW, ANN, ASS, AP: convection (src/coefficients.cpp:15-39) plus diffusion, summed
 // element-wise (:5-8). Every coefficient any kernel uses comes from here, so the assembled
 // arrays and the coefficients a sweep recomputes from stored face velocities are the same bits.
-// The two schemes of src/coefficients.cpp:15-39 and the sum with diffusion, for one cell.
+// The two schemes of src/coefficients.cpp:15-39 and the sum with diffusion, for one cell. Here
+// max(0, a) is one v_max_f64 (fmax), where pos's select returns +0 for a = -0: the two differ
+// only in the sign of a zero, and no coefficient can see it. A nonzero product or sum does not
+// depend on a zero operand's sign, so a convective coefficient differs at most in the sign of a
+// zero result; AE, AW, AN, AS and AP then add a nonzero diffusive coefficient, and AEE..ASS add a
+// negative one or +0 (and -0 + +0 = +0 + +0 = +0).
+__device__ __forceinline__ double posm(double a) { return fmax(a, 0.0); }
 __device__ __forceinline__ void conv_uds(const MomConv& a, double ue, double uw, double vn,
                                          double vs, double* c) {
-    c[0] = a.dy * pos(-ue), c[1] = a.dy * pos(uw), c[2] = a.dx * pos(-vn), c[3] = a.dx * pos(vs);
+    c[0] = a.dy * posm(-ue), c[1] = a.dy * posm(uw), c[2] = a.dx * posm(-vn);
+    c[3] = a.dx * posm(vs);
     c[4] = c[5] = c[6] = c[7] = 0.0;
 }
 __device__ __forceinline__ void conv_quick(const MomConv& a, double ue, double uw, double vn,
                                            double vs, double* c) {
     const double dx = a.dx, dy = a.dy;
-    c[0] = -dy * (-0.75 * pos(-ue) + 0.375 * pos(ue) - 0.125 * pos(-uw));
-    c[1] = -dy * (-0.125 * pos(ue) + 0.375 * pos(-uw) - 0.75 * pos(uw));
-    c[2] = -dx * (-0.75 * pos(-vn) + 0.375 * pos(vn) - 0.125 * pos(-vs));
-    c[3] = -dx * (-0.125 * pos(vn) + 0.375 * pos(-vn) - 0.75 * pos(vs));
-    c[4] = -0.125 * dy * pos(-ue);
-    c[5] = -0.125 * dy * pos(uw);
-    c[6] = -0.125 * dx * pos(-vn);
-    c[7] = -0.125 * dx * pos(vs);
+    const double pue = posm(ue), nue = posm(-ue), puw = posm(uw), nuw = posm(-uw);
+    const double pvn = posm(vn), nvn = posm(-vn), pvs = posm(vs), nvs = posm(-vs);
+    c[0] = -dy * (-0.75 * nue + 0.375 * pue - 0.125 * nuw);
+    c[1] = -dy * (-0.125 * pue + 0.375 * nuw - 0.75 * puw);
+    c[2] = -dx * (-0.75 * nvn + 0.375 * pvn - 0.125 * nvs);
+    c[3] = -dx * (-0.125 * pvn + 0.375 * nvn - 0.75 * pvs);
+    c[4] = -0.125 * dy * nue;
+    c[5] = -0.125 * dy * puw;
+    c[6] = -0.125 * dx * nvn;
+    c[7] = -0.125 * dx * pvs;
 }
 __device__ __forceinline__ C9 conv_plus_diff(const double* c, const C9& d) {
     // ap = s + w + n + e + ss + ww + nn + ee (src/coefficients.cpp:36-37)
@@ -174,6 +183,9 @@ __device__ __forceinline__ double face_vn(const double* v, size_t q, long ld) {
 template <int NCOL> __device__ __forceinline__ int fidx(const Pack& k, int c, int i, int j) {
     return (c * k.rows + (i - k.ai0)) * k.kp + (j - k.aj0) / NCOL;
 }
+// The west and south indices follow from the cell's own: the same row (one colour plane over),
+// one entry back when the cell is its colour's first of the row's NCOL-column group; the next
+// row of the next colour's plane.
 template <int NCOL>
 __device__ __forceinline__ void face_load(const Pack& fk, int c, int i, int j, double* f) {
     const double* fue = fk.base + F_UE * fk.astride;
@@ -181,11 +193,13 @@ __device__ __forceinline__ void face_load(const Pack& fk, int c, int i, int j, d
     const double* fpt = fk.base + F_PT * fk.astride;
     const int cw = NCOL == 2 ? (c + 1) & 1 : (c + 2) % 5; // colour of (i, j - 1)
     const int cs = NCOL == 2 ? (c + 1) & 1 : (c + 1) % 5; // colour of (i + 1, j)
-    const int o = fidx<NCOL>(fk, c, i, j);
+    const int dj = j - fk.aj0, kk = dj / NCOL;
+    const int plane = fk.rows * fk.kp;
+    const int o = (c * fk.rows + (i - fk.ai0)) * fk.kp + kk;
     f[0] = fue[o];
-    f[1] = fue[fidx<NCOL>(fk, cw, i, j - 1)];
+    f[1] = fue[o + (cw - c) * plane - (dj == kk * NCOL ? 1 : 0)];
     f[2] = fvn[o];
-    f[3] = fvn[fidx<NCOL>(fk, cs, i + 1, j)];
+    f[3] = fvn[o + (cs - c) * plane + fk.kp];
     f[4] = fpt[o];
 }
 
@@ -694,15 +708,17 @@ __global__ void __launch_bounds__(BX* BY) k_pack(Pack pk, double* nat, int ld, i
         *n = *p;
 }
 
-// Pressure term of every cell of the assembly range from the current p.
-__global__ void __launch_bounds__(BX* BY) k_pterm(Pack pk, const double* pf, int ld, long poff, double h) {
-    const int j = pk.aj0 + blockIdx.x * BX + threadIdx.x;
+// Pressure term of every cell of the assembly range [ai0, ai1] x [aj0, aj1] from the current p,
+// into array `a` of pack pk (the coefficient pack's 9, or the face pack's F_PT).
+__global__ void __launch_bounds__(BX* BY) k_pterm(Pack pk, int a, int ai1, int aj0,
+                                                  const double* pf, int ld, long poff, double h) {
+    const int j = aj0 + blockIdx.x * BX + threadIdx.x;
     const int i = pk.ai0 + blockIdx.y * BY + threadIdx.y;
-    if (i > pk.ai1 || j > pk.aj1)
+    if (i > ai1 || j > pk.aj1)
         return;
     const int c = colour_of(pk.ncol, i, j);
     const size_t q = static_cast<size_t>(i) * ld + j;
-    pk.base[9 * pk.astride + pidx(pk, c, i, (j - jfirst(pk, c, i)) / pk.ncol)] =
+    pk.base[a * pk.astride + pidx(pk, c, i, (j - jfirst(pk, c, i)) / pk.ncol)] =
         (pf[q] - pf[q + poff]) * h;
 }
 
@@ -804,6 +820,7 @@ void launch_assemble(Solver& s, bool is_u, const int* stop, bool with_bc, bool f
         emit<AsmArgs, k_assemble<false>, BX * AY>(s, grid, dim3(BX, AY), 0, a, stop);
     }
     s.face[is_u ? 0 : 1] = face;
+    s.pt_stale[is_u ? 0 : 1] = false;
 }
 
 bool face_ok(const Solver& s, bool is_u) {
@@ -840,6 +857,7 @@ void launch_bc(Solver& s, bool is_u, const int* stop) {
 // residual_u / residual_v (src/momentum_u.cpp:43-59): |R| per cell, then the ordered sum into
 // *slot (deferred to flush_sums if defer).
 void launch_mom_residual(Solver& s, bool is_u, double* slot, const int* stop, bool defer) {
+    refresh_pterm(s, is_u);
     ensure_packed(s, is_u);
     const MomArgs m = mom_args(s, is_u);
     const int nrow = m.i1 - m.i0 + 1, w = m.j1 - m.j0 + 1;
@@ -912,6 +930,7 @@ void launch_mom_sweep_once(Solver& s, bool is_u, const double* src, double* dst,
 void launch_mom_sweeps(Solver& s, bool is_u, int nsweeps, const int* stop, double* res_slot,
                        bool defer) {
     const sg_params& p = s.p;
+    refresh_pterm(s, is_u);
     const MomArgs m = mom_args(s, is_u);
     const bool fuse = res_slot && p.momentum_solver == SG_MOM_GS_MULTICOLOUR && nsweeps > 0;
     if (res_slot && !fuse)
@@ -958,10 +977,6 @@ void sync_packed(Solver& s, int id, bool to_packed) {
     const bool cv = id >= SG_F_CV_AE && id <= SG_F_CV_AP;
     if (cu || cv)
         ensure_packed(s, cu);
-    else if (id == SG_F_P && to_packed) {
-        ensure_packed(s, true);
-        ensure_packed(s, false);
-    }
     if (cu || cv) {
         const Pack& pk = cu ? s.pku : s.pkv;
         const int a = id - (cu ? SG_F_CU_AE : SG_F_CV_AE);
@@ -969,14 +984,23 @@ void sync_packed(Solver& s, int id, bool to_packed) {
                            dim3(BX, BY), 0, s.stream, pk, s.F(id), s.ld, a, to_packed ? 1 : 0);
         SG_HIP(hipGetLastError());
     } else if (id == SG_F_P && to_packed) {
-        for (int k = 0; k < 2; ++k) {
-            const Pack& pk = k == 0 ? s.pku : s.pkv;
-            hipLaunchKernelGGL(k_pterm, dim3(cdiv(pk.aj1 - pk.aj0 + 1, BX), cdiv(pk.rows, BY)),
-                               dim3(BX, BY), 0, s.stream, pk, s.F(SG_F_P), s.ld,
-                               k == 0 ? 1L : -static_cast<long>(s.ld), k == 0 ? s.p.dy : s.p.dx);
-            SG_HIP(hipGetLastError());
-        }
+        s.pt_stale[0] = s.pt_stale[1] = true;
     }
+}
+
+void refresh_pterm(Solver& s, bool is_u) {
+    const int k = is_u ? 0 : 1;
+    if (!s.pt_stale[k])
+        return;
+    s.pt_stale[k] = false;
+    const Pack& pk = is_u ? s.pku : s.pkv;
+    // The face pack covers the assembly range plus a column west and a row south.
+    const Pack& tgt = s.face[k] ? (is_u ? s.fku : s.fkv) : pk;
+    hipLaunchKernelGGL(k_pterm, dim3(cdiv(pk.aj1 - pk.aj0 + 1, BX), cdiv(pk.rows, BY)),
+                       dim3(BX, BY), 0, s.stream, tgt, s.face[k] ? int(F_PT) : 9, pk.ai1, pk.aj0,
+                       s.F(SG_F_P), s.ld, is_u ? 1L : -static_cast<long>(s.ld),
+                       is_u ? s.p.dy : s.p.dx);
+    SG_HIP(hipGetLastError());
 }
 
 } // namespace sg

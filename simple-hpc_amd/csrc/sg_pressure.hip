@@ -343,6 +343,7 @@ void launch_assemble_p(Solver& s, double* slot, const int* stop, bool defer) {
 
 void launch_correct(Solver& s, const int* stop) {
     const sg_params& p = s.p;
+    s.pt_stale[0] = s.pt_stale[1] = true; // p changes
     const CorrArgs c{p_args(s),   s.F(SG_F_CU_AP), s.F(SG_F_CV_AP), s.F(SG_F_U),
                      s.F(SG_F_V), s.F(SG_F_P),     p.dx,            p.dy,
                      p.relaxation_factor_p};

@@ -282,3 +282,40 @@ def test_sweep_tile_shapes_bit_identical(tile, scheme, order, inner):
     assert bits_equal(logg, logc[:, 1:]), first_diff(logg, logc[:, 1:])
     for f in ("u", "v", "p"):
         _same(f, gpu.get(f), cpu.get(f))
+
+
+@pytest.mark.parametrize("scheme,order", [("QUICK", 4), ("UDS", 2), ("UDS", 4)])
+def test_face_form_after_advance(scheme, order):
+    """Outer iterations keep the momentum coefficients as face velocities (multicolour sweeps):
+    downloads expand them, a per-phase sweep uses them as they are, a per-phase residual and an
+    uploaded coefficient switch to the explicit arrays; every result bit for bit."""
+    cfg = (f"N_cells_x = 130\nN_cells_y = 200\nRe = 1000\nconvection_scheme = {scheme}\n"
+           f"diffusion_order = {order}\nmomentum_solver = gs_multicolour\n"
+           "maxit_inner_u = 3\nmaxit_inner_v = 4\n"
+           "pressure_solver = mg\nmaxit = 3\nerror_tolerance = 0\nbackend = gpu\n")
+    p, _ = sg.params_from_config(cfg)
+    cpu = po.Oracle(p)
+    for _ in range(3):
+        cpu.outer_iteration()
+    # Downloads after the iterations: every coefficient of both components.
+    g1 = sg.Solver(p)
+    g1.advance(3)
+    for comp in ("cu_", "cv_"):
+        for c in COEFS:
+            _same("advance " + comp + c, g1.get(comp + c), cpu.get(comp + c))
+    # A per-phase sweep on the face form, then a residual (explicit arrays).
+    g2 = sg.Solver(p)
+    g2.advance(3)
+    cpu.sweep_u(2); g2.sweep_u(2)
+    _same("sweep_u after advance", g2.get("u"), cpu.get("u"))
+    rc, rg = cpu.residual_v(), g2.residual_v()
+    assert _rel_close(rg, rc), (rg, rc)
+    cpu.sweep_v(1); g2.sweep_v(1)
+    _same("sweep_v after residual_v", g2.get("v"), cpu.get("v"))
+    # An uploaded coefficient replaces the face form.
+    ae = cpu.get("cu_AE") * 1.001
+    cpu.set("cu_AE", ae); g2.set("cu_AE", ae)
+    cpu.sweep_u(1); g2.sweep_u(1)
+    _same("sweep_u after upload", g2.get("u"), cpu.get("u"))
+    for c in COEFS:
+        _same("after upload cu_" + c, g2.get("cu_" + c), cpu.get("cu_" + c))
