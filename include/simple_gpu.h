@@ -186,7 +186,8 @@ enum {
     SG_TUNE_MID_CELLS = 1,
     SG_TUNE_LDS_CELLS = 2,
     SG_TUNE_MOM_TILE = 3, /* momentum sweep tile (rows x columns): 0 32x64, 1 64x64, 2 32x128,
-                           * 3 16x128, 4 24x96, 5 48x96, 6 24x64, 7 40x128 */
+                           * 3 16x128, 4 24x96, 5 48x96, 6 24x64, 7 40x128, 8 32x72, 9 28x80,
+                           * 10 36x64, 11 40x56 */
     SG_TUNE_MG_STAGED = 4, /* first multigrid level whose tiles also stage the operator in LDS */
     SG_TUNE_MOM_TILE_FIRST = 5, /* tile of the first sweep (fused residual) of each component;
                                  * unset: SG_TUNE_MOM_TILE */

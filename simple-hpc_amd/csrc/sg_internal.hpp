@@ -73,8 +73,8 @@ struct Pack {
 struct TileShape {
     int ti, tj;
 };
-constexpr TileShape MOM_TILES[] = {{32, 64}, {64, 64}, {32, 128}, {16, 128},
-                                   {24, 96}, {48, 96}, {24, 64},  {40, 128}};
+constexpr TileShape MOM_TILES[] = {{32, 64}, {64, 64}, {32, 128}, {16, 128}, {24, 96}, {48, 96},
+                                   {24, 64}, {40, 128}, {32, 72}, {28, 80}, {36, 64}, {40, 56}};
 constexpr int N_MOM_TILES = sizeof(MOM_TILES) / sizeof(MOM_TILES[0]);
 inline std::string tile_name(int k) {
     return std::to_string(MOM_TILES[k].ti) + "x" + std::to_string(MOM_TILES[k].tj);

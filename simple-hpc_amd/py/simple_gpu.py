@@ -31,7 +31,8 @@ PHASES = ["assemble_u", "bc_u", "residual_u", "sweep_u", "assemble_v", "bc_v", "
 TUNE_TAIL_THREADS, TUNE_MID_CELLS, TUNE_LDS_CELLS, TUNE_MOM_TILE, TUNE_MG_STAGED = 0, 1, 2, 3, 4
 TUNE_MOM_TILE_FIRST = 5
 # Values of TUNE_MOM_TILE / TUNE_MOM_TILE_FIRST: sweep tile shapes as (rows, columns).
-MOM_TILES = ((32, 64), (64, 64), (32, 128), (16, 128), (24, 96), (48, 96), (24, 64), (40, 128))
+MOM_TILES = ((32, 64), (64, 64), (32, 128), (16, 128), (24, 96), (48, 96), (24, 64), (40, 128),
+             (32, 72), (28, 80), (36, 64), (40, 56))
 # Launch-shape knobs in the order of bench.py's --knobs "tile,threads,mid,lds[,staged[,first]]".
 TUNE_KEYS = (TUNE_MOM_TILE, TUNE_TAIL_THREADS, TUNE_MID_CELLS, TUNE_LDS_CELLS, TUNE_MG_STAGED,
              TUNE_MOM_TILE_FIRST)
