@@ -9,7 +9,9 @@ namespace sg {
 
 namespace {
 
-constexpr int BX = 64, BY = 16; // 16 rows: neighbouring rows a cell reads are mostly in L1
+// Workgroups of 256 cells x 4 rows (assemble_p and correct read each cell's row and the one
+// above or below: 256 x 4 measured 3.4 us per outer iteration faster at 1024^2 than 64 x 16).
+constexpr int BX = 256, BY = 4;
 
 struct PArgs {
     double *AE, *AW, *AN, *AS, *AP, *b, *pc;
