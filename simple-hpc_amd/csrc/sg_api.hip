@@ -55,6 +55,14 @@ template <class T> T* dalloc(Solver& s, size_t n) {
 
 } // namespace
 
+void drop_graphs(Solver& s) {
+    for (hipGraphExec_t* g : {&s.graph, &s.graph_n})
+        if (*g) {
+            SG_HIP(hipGraphExecDestroy(*g));
+            *g = nullptr;
+        }
+}
+
 void ensure_log(Solver& s, int cap) {
     if (cap <= s.log_cap)
         return;
@@ -65,10 +73,7 @@ void ensure_log(Solver& s, int cap) {
     }
     s.log_cap = std::max(cap, 256);
     s.dlog = dalloc<double>(s, 3 * static_cast<size_t>(s.log_cap));
-    if (s.graph) { // the stop-rule node holds the old log pointer
-        SG_HIP(hipGraphExecDestroy(s.graph));
-        s.graph = nullptr;
-    }
+    drop_graphs(s); // the stop-rule node holds the old log pointer
 }
 
 // One outer iteration (src/solver.cpp:27-93), enqueued. ev: optional phase events (12).
@@ -125,10 +130,10 @@ void enqueue_iteration(Solver& s, const int* stop, bool eager, hipEvent_t* ev) {
 
 namespace {
 
-// Capture one outer iteration. The coefficient form flags are left as they were: the graph's
-// launches set them (see face_after).
-void capture_graph(Solver& s) {
-    if (s.graph)
+// Capture `reps` outer iterations into *out. The coefficient form flags are left as they were:
+// sg_advance sets them after replaying the graphs.
+void capture_graph(Solver& s, int reps, hipGraphExec_t* out) {
+    if (*out)
         return;
     struct Restore {
         Solver& s;
@@ -138,7 +143,8 @@ void capture_graph(Solver& s) {
     hipGraph_t g = nullptr;
     SG_HIP(hipStreamBeginCapture(s.stream, hipStreamCaptureModeRelaxed));
     try {
-        enqueue_iteration(s, &s.ctl->stop, false, nullptr);
+        for (int r = 0; r < reps; ++r)
+            enqueue_iteration(s, &s.ctl->stop, false, nullptr);
     } catch (...) {
         s.ride = s.rode = false;
         s.deferred.clear();
@@ -149,7 +155,7 @@ void capture_graph(Solver& s) {
         throw;
     }
     SG_HIP(hipStreamEndCapture(s.stream, &g));
-    hipError_t e = hipGraphInstantiate(&s.graph, g, nullptr, nullptr, 0);
+    hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     SG_HIP(e);
 }
@@ -393,6 +399,8 @@ void destroy_solver(Solver* s) {
     warm_free(*s);
     if (s->graph)
         (void)hipGraphExecDestroy(s->graph);
+    if (s->graph_n)
+        (void)hipGraphExecDestroy(s->graph_n);
     for (auto e : s->ev)
         if (e)
             (void)hipEventDestroy(e);
@@ -552,8 +560,17 @@ int sg_advance(sg_solver* h, int max_iters, int* iters_done, double* log, int* c
                     break;
             }
         } else {
-            capture_graph(s);
-            for (int k = 0; k < max_iters; ++k)
+            // Every kernel of an iteration returns at once after the stop rule fires, so replaying
+            // GRAPH_ITERS iterations per launch stops at the same iteration.
+            int k = 0;
+            if (max_iters >= GRAPH_ITERS) {
+                capture_graph(s, GRAPH_ITERS, &s.graph_n);
+                for (; k + GRAPH_ITERS <= max_iters; k += GRAPH_ITERS)
+                    SG_HIP(hipGraphLaunch(s.graph_n, s.stream));
+            }
+            if (k < max_iters)
+                capture_graph(s, 1, &s.graph);
+            for (; k < max_iters; ++k)
                 SG_HIP(hipGraphLaunch(s.graph, s.stream));
             if (max_iters > 0) { // the iterations' assemblies and corrections
                 s.face[0] = face_ok(s, true), s.face[1] = face_ok(s, false);
@@ -713,10 +730,7 @@ int sg_set_tuning(sg_solver* h, int key, long value) {
                 throw ArgError("tuning value must be >= 0");
             *knob(s, key) = value;
         }
-        if (s.graph) { // launch shapes are baked into the captured graph
-            SG_HIP(hipGraphExecDestroy(s.graph));
-            s.graph = nullptr;
-        }
+        drop_graphs(s); // launch shapes are baked into the captured graphs
     });
 }
 
@@ -865,10 +879,7 @@ int sg_autotune(sg_solver* h, int reps, char* report, size_t len) {
         sync(s);
         (void)hipFree(kb);
         (void)hipFree(kp);
-        if (s.graph) {
-            SG_HIP(hipGraphExecDestroy(s.graph));
-            s.graph = nullptr;
-        }
+        drop_graphs(s);
         if (report && len) {
             std::strncpy(report, rep.c_str(), len - 1);
             report[len - 1] = 0;

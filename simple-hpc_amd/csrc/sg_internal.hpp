@@ -141,7 +141,8 @@ struct Solver {
     int mom_tile = 0;                  // momentum sweep tile shape, an index of MOM_TILES (tunable)
     int mom_tile_first = -1;           // the first sweep's tile shape (-1: mom_tile)
     C9 diff2{}, diff4{};               // diffusive coefficients: order 2 (and band), order 4
-    hipGraphExec_t graph = nullptr;
+    hipGraphExec_t graph = nullptr;    // one captured outer iteration
+    hipGraphExec_t graph_n = nullptr;  // GRAPH_ITERS of them (fewer launch gaps)
     int iters = 0;                     // host copy of completed iterations
     long long cg_total = 0;
     int cg_last = 0;
@@ -278,6 +279,11 @@ __device__ __forceinline__ void stop_rule(const StopArgs& a) {
         ctl->stop = 1;
     }
 }
+
+// Outer iterations per replay of Solver::graph_n, and dropping both captured graphs (their launch
+// shapes and pointers are baked in).
+constexpr int GRAPH_ITERS = 4;
+void drop_graphs(Solver& s);
 
 // Enqueue every deferred sum: one launch for all their rows, one for their totals (with rule:
 // the totals and then the stopping rule in that second launch).

@@ -1005,10 +1005,7 @@ void warm_attach(Solver& s, UNet* net, int level) {
     w.xf = static_cast<float*>(al(nc * IN_PAD * sizeof(float)));
     if (net)
         net->get_ctx(1, w.nxc, w.nyc); // workspace exists before any capture
-    if (s.graph) {
-        SG_HIP(hipGraphExecDestroy(s.graph));
-        s.graph = nullptr;
-    }
+    drop_graphs(s);
 }
 
 // The guess: features, network, p' = prediction * scale, acceptance test (enqueued on s.stream;
