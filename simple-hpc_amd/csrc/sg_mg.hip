@@ -27,7 +27,9 @@ namespace sg {
 
 namespace {
 
-constexpr int BX = 64, BY = 4;
+// Elementwise level kernels (the mixed form's FP64 <-> FP32 conversions, the unstaged V-cycle):
+// 256 x 1 workgroups (3 us per mixed outer iteration faster at 1024^2 than 64 x 4).
+constexpr int BX = 256, BY = 1;
 constexpr int MAXLEV = 24;
 
 template <class T> struct Lv {
