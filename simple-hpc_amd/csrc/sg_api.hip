@@ -103,7 +103,7 @@ void enqueue_iteration(Solver& s, const int* stop, bool eager, hipEvent_t* ev) {
     mark(7);
     launch_mom_sweeps(s, false, p.maxit_inner_v, stop, s.scal + S_RES_V, defer);
     mark(8);
-    launch_assemble_p(s, s.scal + S_RES_P, stop, defer);
+    launch_assemble_p(s, s.scal + S_RES_P, stop, defer, mg_lean_ok(s));
     mark(9);
     // The residual sums, all deferred by now, may ride along with the p' solve's first mean
     // removal (multigrid), which sums its right-hand side anyway.
@@ -137,9 +137,12 @@ void capture_graph(Solver& s, int reps, hipGraphExec_t* out) {
         return;
     struct Restore {
         Solver& s;
-        bool f0, f1, p0, p1;
-        ~Restore() { s.face[0] = f0, s.face[1] = f1, s.pt_stale[0] = p0, s.pt_stale[1] = p1; }
-    } restore{s, s.face[0], s.face[1], s.pt_stale[0], s.pt_stale[1]};
+        bool f0, f1, p0, p1, lean;
+        ~Restore() {
+            s.face[0] = f0, s.face[1] = f1, s.pt_stale[0] = p0, s.pt_stale[1] = p1;
+            s.p_lean = lean;
+        }
+    } restore{s, s.face[0], s.face[1], s.pt_stale[0], s.pt_stale[1], s.p_lean};
     hipGraph_t g = nullptr;
     SG_HIP(hipStreamBeginCapture(s.stream, hipStreamCaptureModeRelaxed));
     try {
@@ -439,6 +442,8 @@ int sg_upload(sg_solver* h, int id, const double* host) {
         Solver& s = S(h);
         if (id < 0 || id >= SG_F_COUNT || !host)
             throw ArgError("bad field id or null pointer");
+        if (id >= SG_F_AEP && id <= SG_F_APP)
+            ensure_p_full(s);
         SG_HIP(hipMemcpy2DAsync(s.f[id], sizeof(double) * s.ld, host, sizeof(double) * s.nc,
                                 sizeof(double) * s.nc, s.nr, hipMemcpyHostToDevice, s.stream));
         sync_packed(s, id, true);
@@ -451,6 +456,8 @@ int sg_download(sg_solver* h, int id, double* host) {
         Solver& s = S(h);
         if (id < 0 || id >= SG_F_COUNT || !host)
             throw ArgError("bad field id or null pointer");
+        if (id >= SG_F_AEP && id <= SG_F_APP)
+            ensure_p_full(s);
         sync_packed(s, id, false);
         SG_HIP(hipMemcpy2DAsync(host, sizeof(double) * s.nc, s.f[id], sizeof(double) * s.ld,
                                 sizeof(double) * s.nc, s.nr, hipMemcpyDeviceToHost, s.stream));
@@ -517,7 +524,10 @@ int sg_assemble_p(sg_solver* h, double* res) {
                      [](Solver& s) { launch_assemble_p(s, s.scal + S_RES_P, s.never, false); });
 }
 int sg_solve_p(sg_solver* h) {
-    return phase(h, [](Solver& s) { launch_solve_p(s, s.never, true); });
+    return phase(h, [](Solver& s) {
+        ensure_p_full(s);
+        launch_solve_p(s, s.never, true);
+    });
 }
 int sg_correct(sg_solver* h) {
     return phase(h, [](Solver& s) { launch_correct(s, s.never); });
@@ -575,6 +585,7 @@ int sg_advance(sg_solver* h, int max_iters, int* iters_done, double* log, int* c
             if (max_iters > 0) { // the iterations' assemblies and corrections
                 s.face[0] = face_ok(s, true), s.face[1] = face_ok(s, false);
                 s.pt_stale[0] = s.pt_stale[1] = true;
+                s.p_lean = mg_lean_ok(s);
             }
         }
         const Ctl c = read_ctl(s);
@@ -603,6 +614,7 @@ int sg_kernel_probe(sg_solver* h, int kid, int reps, double* mean_s, double* byt
                     int* per_iter) {
     return guard([&] {
         Solver& s = S(h);
+        ensure_p_full(s);
         const sg_params& p = s.p;
         if (reps < 1)
             throw ArgError("reps must be >= 1");
@@ -762,6 +774,7 @@ static double time_psolve(Solver& s, int reps) {
 int sg_autotune(sg_solver* h, int reps, char* report, size_t len) {
     return guard([&] {
         Solver& s = S(h);
+        ensure_p_full(s); // the candidates include solves that read every p' array
         if (reps < 1)
             throw ArgError("reps must be >= 1");
         std::string rep;

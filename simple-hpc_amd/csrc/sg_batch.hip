@@ -75,8 +75,10 @@ void build_plan(Batch& b) {
         Solver& s = *b.cav[c];
         // Recording runs nothing: the coefficient form and pressure-term flags stay.
         const bool f0 = s.face[0], f1 = s.face[1], p0 = s.pt_stale[0], p1 = s.pt_stale[1];
+        const bool lean = s.p_lean;
         auto restore = [&] {
             s.face[0] = f0, s.face[1] = f1, s.pt_stale[0] = p0, s.pt_stale[1] = p1;
+            s.p_lean = lean;
         };
         s.rec = &b.recs[c];
         try {
@@ -454,6 +456,7 @@ int sg_batch_advance(sg_batch* h, int max_iters, int* iters, int* status, double
                     Solver& s = *b.cav[c];
                     s.face[0] = face_ok(s, true), s.face[1] = face_ok(s, false);
                     s.pt_stale[0] = s.pt_stale[1] = true;
+                    s.p_lean = mg_lean_ok(s);
                 }
             begin_chunk(b);
             replay(b, k);

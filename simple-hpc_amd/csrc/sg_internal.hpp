@@ -138,6 +138,8 @@ struct Solver {
     bool face[2] = {};                 // u, v: the face pack holds the current coefficients (the
                                        // coefficient pack is stale until ensure_packed)
     bool pt_stale[2] = {};             // u, v: p changed since the pressure terms were evaluated
+    bool p_lean = false;               // AWP, ASP, APP not written by the last assemble_p (stale
+                                       // until ensure_p_full)
     int mom_tile = 0;                  // momentum sweep tile shape, an index of MOM_TILES (tunable)
     int mom_tile_first = -1;           // the first sweep's tile shape (-1: mom_tile)
     C9 diff2{}, diff4{};               // diffusive coefficients: order 2 (and band), order 4
@@ -214,7 +216,14 @@ void launch_copy(Solver& s, const double* a, double* b, size_t n, const int* sto
 Pack make_pack(const sg_params& p, bool is_u);
 void sync_packed(Solver& s, int field_id, bool to_packed);
 // pressure (sg_pressure.hip)
-void launch_assemble_p(Solver& s, double* slot, const int* stop, bool defer);
+// lean: write AE, AN, b only (and |b|): AW, AS and AP follow from them (the operator's symmetry)
+// and are derived on demand (ensure_p_full).
+void launch_assemble_p(Solver& s, double* slot, const int* stop, bool defer, bool lean = false);
+// Derive AWP, ASP and APP from AEP and ANP if the last assemble_p was lean.
+void ensure_p_full(Solver& s);
+// Whether the FP64 multigrid p' solve reads only AE and AN of level 0 (every level from 0 to the
+// tail runs staged tiles, and the tail starts at level 2 or below).
+bool mg_lean_ok(Solver& s);
 void launch_solve_p(Solver& s, const int* stop, bool eager);
 void launch_correct(Solver& s, const int* stop);
 void launch_fill(Solver& s, double* a, size_t n, double v, const int* stop);

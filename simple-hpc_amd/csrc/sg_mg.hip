@@ -1346,4 +1346,17 @@ void launch_mg_solve(Solver& s, const int* stop, bool has_guess) {
     launch_remove_mean0(s, s.F(SG_F_PC), stop);
 }
 
+// The same conditions as launch_vcycles' use of staged tiles on level 0, with the tail (whose first
+// level is built from the level above it, in global memory) starting at level 2 or below.
+bool mg_lean_ok(Solver& s) {
+    const sg_params& p = s.p;
+    if (p.pressure_solver != SG_P_MG || p.mg_precision == SG_MG_MIXED || s.mg.empty() ||
+        !tiled_ok(s) || s.mg_staged != 0)
+        return false;
+    const int t0 = lds_start<double>(s), tm = mid_start(s, t0);
+    const size_t sdn = staged_bytes<double>(32, 2 * p.mg_pre + 1);
+    const size_t sup = staged_bytes<double>(32, 2 * p.mg_post);
+    return tm >= 2 && std::max(sdn, sup) <= s.lds_budget + 1024;
+}
+
 } // namespace sg

@@ -32,6 +32,7 @@ struct ApArgs {
     double dx, dy;
     double* cell;
     int nr, nc; // array rows and columns (the wall links extend over the halo)
+    int lean;   // AE, AN, b only
 };
 
 __device__ __forceinline__ void k_assemble_p(const ApArgs& g, const Blk& bk) {
@@ -49,11 +50,13 @@ __device__ __forceinline__ void k_assemble_p(const ApArgs& g, const Blk& bk) {
         for (int k = bk.x * (BX * BY) + threadIdx.y * BX + threadIdx.x; k < max(g.nr, g.nc);
              k += bk.nx * (BX * BY)) {
             if (k < g.nr) {
-                a.AW[k * ld + H + 1] = 0.0;
+                if (!g.lean)
+                    a.AW[k * ld + H + 1] = 0.0;
                 a.AE[k * ld + a.NY + H] = 0.0;
             }
             if (k < g.nc) {
-                a.AS[(a.NX + H - 1) * ld + k] = 0.0;
+                if (!g.lean)
+                    a.AS[(a.NX + H - 1) * ld + k] = 0.0;
                 a.AN[H * ld + k] = 0.0;
             }
         }
@@ -70,10 +73,12 @@ __device__ __forceinline__ void k_assemble_p(const ApArgs& g, const Blk& bk) {
     if (i == a.NX + H - 1) as = 0.0;
     if (i == H) an = 0.0;
     a.AE[q] = ae;
-    a.AW[q] = aw;
     a.AN[q] = an;
-    a.AS[q] = as;
-    a.AP[q] = ae + aw + an + as;
+    if (!g.lean) {
+        a.AW[q] = aw;
+        a.AS[q] = as;
+        a.AP[q] = ae + aw + an + as;
+    }
     const double bq = -dx * (-v[q + ld] + v[q]) - dy * (-u[q - 1] + u[q]);
     a.b[q] = bq;
     cell[static_cast<size_t>(i - H) * a.NY + (j - H - 1)] = fabs(bq);
@@ -131,6 +136,33 @@ __device__ __forceinline__ void k_pc_jacobi(const PJacArgs& g, const Blk& bk) {
 }
 
 // correct (src/pressure.cpp:136-157).
+// AW, AS and AP from AE and AN after a lean assemble_p: AW(i, j) = AE(i, j - 1) and AS(i, j) =
+// AN(i + 1, j) (the same divisions, src/pressure.cpp:21-24), the wall links zeroed as there
+// (:28-39), AP = AE + AW + AN + AS in that order (:45): the arrays a full assemble_p writes.
+__global__ void __launch_bounds__(BX* BY) k_p_derive(PArgs a, int nr, int nc) {
+    const int H = a.H;
+    const long ld = a.ld;
+    if (blockIdx.y == 0)
+        for (int k = blockIdx.x * (BX * BY) + threadIdx.y * BX + threadIdx.x; k < max(nr, nc);
+             k += gridDim.x * (BX * BY)) {
+            if (k < nr)
+                a.AW[k * ld + H + 1] = 0.0;
+            if (k < nc)
+                a.AS[(a.NX + H - 1) * ld + k] = 0.0;
+        }
+    const int j = H + 1 + blockIdx.x * BX + threadIdx.x;
+    const int i = H + blockIdx.y * BY + threadIdx.y;
+    if (i > a.NX + H - 1 || j > a.NY + H)
+        return;
+    const size_t q = static_cast<size_t>(i) * ld + j;
+    const double ae = a.AE[q], an = a.AN[q];
+    const double aw = j == H + 1 ? 0.0 : a.AE[q - 1];
+    const double as = i == a.NX + H - 1 ? 0.0 : a.AN[q + ld];
+    a.AW[q] = aw;
+    a.AS[q] = as;
+    a.AP[q] = ae + aw + an + as;
+}
+
 struct CorrArgs {
     PArgs a;
     const double *APU, *APV;
@@ -330,17 +362,27 @@ void launch_fill(Solver& s, double* a, size_t n, double v, const int* stop) {
                                FillArgs{a, n, v}, stop);
 }
 
-void launch_assemble_p(Solver& s, double* slot, const int* stop, bool defer) {
+void launch_assemble_p(Solver& s, double* slot, const int* stop, bool defer, bool lean) {
     const sg_params& p = s.p;
     const PArgs a = p_args(s);
     emit<ApArgs, k_assemble_p, BX * BY>(s, pgrid(p), dim3(BX, BY), 0,
                                         ApArgs{a, s.F(SG_F_CU_AP), s.F(SG_F_CV_AP), s.F(SG_F_U),
-                                               s.F(SG_F_V), p.dx, p.dy, s.rcell[2], s.nr, s.nc},
+                                               s.F(SG_F_V), p.dx, p.dy, s.rcell[2], s.nr, s.nc,
+                                               lean ? 1 : 0},
                                         stop);
+    s.p_lean = lean;
     if (defer)
         defer_sum(s, 3, s.rcell[2], p.NX, p.NY, slot);
     else
         launch_ordered_sum(s, s.rcell[2], p.NY, 0, p.NX - 1, 0, p.NY - 1, slot, stop);
+}
+
+void ensure_p_full(Solver& s) {
+    if (!s.p_lean)
+        return;
+    s.p_lean = false;
+    hipLaunchKernelGGL(k_p_derive, pgrid(s.p), dim3(BX, BY), 0, s.stream, p_args(s), s.nr, s.nc);
+    SG_HIP(hipGetLastError());
 }
 
 void launch_correct(Solver& s, const int* stop) {

@@ -178,7 +178,8 @@ def test_fixed_work_sizes_bit_identical(nx, ny, pres, scheme, order, iters):
     done, logg, _ = gpu.advance(iters)
     assert done == iters
     assert bits_equal(logg, logc[:, 1:]), first_diff(logg, logc[:, 1:])
-    for f in ["u", "v", "p", "pc", "b", "cu_AP", "cv_AP", "APP"]:
+    # (After outer iterations AWP, ASP and APP may be derived from AEP and ANP on download.)
+    for f in ["u", "v", "p", "pc", "b", "cu_AP", "cv_AP", "AEP", "AWP", "ANP", "ASP", "APP"]:
         _same(f, gpu.get(f), cpu.get(f))
     gpu.close()
 
@@ -319,3 +320,31 @@ def test_face_form_after_advance(scheme, order):
     _same("sweep_u after upload", g2.get("u"), cpu.get("u"))
     for c in COEFS:
         _same("after upload cu_" + c, g2.get("cu_" + c), cpu.get("cu_" + c))
+
+
+def test_lean_pressure_assembly_after_advance():
+    """With the FP64 multigrid staged from level 0 and the tail below level 1, outer iterations
+    write only AE, AN and b of the p' operator; AW, AS and AP are derived on demand, bit for bit,
+    for downloads and for a per-phase p' solve."""
+    cfg = ("N_cells_x = 256\nN_cells_y = 256\nRe = 1000\nmomentum_solver = gs_multicolour\n"
+           "pressure_solver = mg\nmaxit = 4\nerror_tolerance = 0\nbackend = gpu\n")
+    p, _ = sg.params_from_config(cfg)
+    cpu = po.Oracle(p)
+    for _ in range(4):
+        cpu.outer_iteration()
+    for order in ("download", "solve"):
+        gpu = sg.Solver(p)
+        gpu.set_tuning(sg.TUNE_MID_CELLS, 0)  # the tail from 64x64: level 0 and 1 staged
+        gpu.set_tuning(sg.TUNE_MG_STAGED, 0)
+        done, _, _ = gpu.advance(4)
+        assert done == 4
+        if order == "download":
+            for f in ["AEP", "AWP", "ANP", "ASP", "APP", "b", "pc", "u", "v", "p"]:
+                _same("lean " + f, gpu.get(f), cpu.get(f))
+        else:
+            ref = po.Oracle(p)
+            for _ in range(4):
+                ref.outer_iteration()
+            ref.solve_p(); gpu.solve_p()
+            _same("solve_p after lean iterations", gpu.get("pc"), ref.get("pc"))
+        gpu.close()
