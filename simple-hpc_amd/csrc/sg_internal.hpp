@@ -221,8 +221,8 @@ void sync_packed(Solver& s, int field_id, bool to_packed);
 void launch_assemble_p(Solver& s, double* slot, const int* stop, bool defer, bool lean = false);
 // Derive AWP, ASP and APP from AEP and ANP if the last assemble_p was lean.
 void ensure_p_full(Solver& s);
-// Whether the FP64 multigrid p' solve reads only AE and AN of level 0 (every level from 0 to the
-// tail runs staged tiles, and the tail starts at level 2 or below).
+// Whether the multigrid p' solve reads only AE and AN of level 0 (every level from 0 to the tail
+// runs staged tiles, and the tail starts at level 2 or below; no learned warm start).
 bool mg_lean_ok(Solver& s);
 void launch_solve_p(Solver& s, const int* stop, bool eager);
 void launch_correct(Solver& s, const int* stop);

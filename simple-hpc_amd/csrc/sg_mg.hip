@@ -1009,7 +1009,23 @@ struct Mix {
     const double *AE, *AW, *AN, *AS, *AP, *b; // FP64 level 0 (at interior cell 0, 0)
     double* x;               // FP64 p' (at interior cell 0, 0)
     Lv<float> f;             // FP32 level 0
+    int lean;                // AW, AS, AP not assembled: derived from AE and AN (ensure_p_full)
 };
+
+// Level-0 AW, AS and AP of interior cell (I, J) at q: stored, or after a lean assemble_p the
+// values ensure_p_full would derive.
+__device__ __forceinline__ void mix_op(const Mix& m, int I, int J, size_t q, double& aw,
+                                       double& as, double& ap) {
+    if (m.lean) {
+        aw = J == 0 ? 0.0 : m.AE[q - 1];
+        as = I == m.nx - 1 ? 0.0 : m.AN[q + m.ld];
+        ap = m.AE[q] + aw + m.AN[q] + as;
+    } else {
+        aw = m.AW[q];
+        as = m.AS[q];
+        ap = m.AP[q];
+    }
+}
 
 // FP32 copies of the level-0 operator; rhs = b (first step, p' = 0), x = 0.
 __device__ __forceinline__ void k_mx_init(const Mix& m, const Blk& bk) {
@@ -1017,11 +1033,13 @@ __device__ __forceinline__ void k_mx_init(const Mix& m, const Blk& bk) {
     if (I >= m.nx || J >= m.ny)
         return;
     const size_t q = static_cast<size_t>(I) * m.ld + J, g = static_cast<size_t>(I) * m.f.stride + J;
+    double aw, as, ap;
+    mix_op(m, I, J, q, aw, as, ap);
     m.f.AE[g] = static_cast<float>(m.AE[q]);
-    m.f.AW[g] = static_cast<float>(m.AW[q]);
+    m.f.AW[g] = static_cast<float>(aw);
     m.f.AN[g] = static_cast<float>(m.AN[q]);
-    m.f.AS[g] = static_cast<float>(m.AS[q]);
-    m.f.AP[g] = static_cast<float>(m.AP[q]);
+    m.f.AS[g] = static_cast<float>(as);
+    m.f.AP[g] = static_cast<float>(ap);
     m.f.b[g] = static_cast<float>(m.b[q]);
     m.f.x[g] = 0.0f;
 }
@@ -1034,8 +1052,10 @@ __device__ __forceinline__ void k_mx_defect(const Mix& m, const Blk& bk) {
     const long s = m.ld;
     const size_t q = static_cast<size_t>(I) * s + J, g = static_cast<size_t>(I) * m.f.stride + J;
     const double* x = m.x;
-    const double r = m.b[q] - (m.AP[q] * x[q] - (m.AE[q] * x[q + 1] + m.AW[q] * x[q - 1] +
-                                                 m.AS[q] * x[q + s] + m.AN[q] * x[q - s]));
+    double aw, as, ap;
+    mix_op(m, I, J, q, aw, as, ap);
+    const double r = m.b[q] - (ap * x[q] - (m.AE[q] * x[q + 1] + aw * x[q - 1] +
+                                            as * x[q + s] + m.AN[q] * x[q - s]));
     m.f.b[g] = static_cast<float>(r);
     m.f.x[g] = 0.0f;
 }
@@ -1323,7 +1343,8 @@ void launch_mg_solve(Solver& s, const int* stop, bool has_guess) {
     }
     const Tail<float> t = make_tail<float>(s, lv32);
     const MgLevel& L0 = s.mg[0];
-    const Mix m{L0.nx, L0.ny, s.ld, L0.AE, L0.AW, L0.AN, L0.AS, L0.AP, L0.b, L0.x, t.G[0]};
+    const Mix m{L0.nx, L0.ny, s.ld, L0.AE, L0.AW, L0.AN, L0.AS, L0.AP, L0.b, L0.x, t.G[0],
+                s.p_lean ? 1 : 0};
     const dim3 g = lgrid(m.nx, m.ny);
     // Learned warm start: the network's guess of p' (or zero, if the guess does not lower the
     // defect) replaces the zero start.
@@ -1348,14 +1369,18 @@ void launch_mg_solve(Solver& s, const int* stop, bool has_guess) {
 
 // The same conditions as launch_vcycles' use of staged tiles on level 0, with the tail (whose first
 // level is built from the level above it, in global memory) starting at level 2 or below.
+// The mixed form's FP64 conversion and defect kernels derive the three themselves (mix_op); the
+// learned warm start reads them.
 bool mg_lean_ok(Solver& s) {
     const sg_params& p = s.p;
-    if (p.pressure_solver != SG_P_MG || p.mg_precision == SG_MG_MIXED || s.mg.empty() ||
-        !tiled_ok(s) || s.mg_staged != 0)
+    if (p.pressure_solver != SG_P_MG || s.mg.empty() || !tiled_ok(s) || s.mg_staged != 0 ||
+        p.pressure_warm_start == SG_WARM_LEARNED)
         return false;
-    const int t0 = lds_start<double>(s), tm = mid_start(s, t0);
-    const size_t sdn = staged_bytes<double>(32, 2 * p.mg_pre + 1);
-    const size_t sup = staged_bytes<double>(32, 2 * p.mg_post);
+    const bool mixed = p.mg_precision == SG_MG_MIXED;
+    const int hdn = 2 * p.mg_pre + 1, hup = 2 * p.mg_post;
+    const int t0 = mixed ? lds_start<float>(s) : lds_start<double>(s), tm = mid_start(s, t0);
+    const size_t sdn = mixed ? staged_bytes<float>(32, hdn) : staged_bytes<double>(32, hdn);
+    const size_t sup = mixed ? staged_bytes<float>(32, hup) : staged_bytes<double>(32, hup);
     return tm >= 2 && std::max(sdn, sup) <= s.lds_budget + 1024;
 }
 

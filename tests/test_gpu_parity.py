@@ -348,3 +348,23 @@ def test_lean_pressure_assembly_after_advance():
             ref.solve_p(); gpu.solve_p()
             _same("solve_p after lean iterations", gpu.get("pc"), ref.get("pc"))
         gpu.close()
+
+
+def test_lean_pressure_assembly_mixed():
+    """The mixed form with a lean p' assembly (its FP64 conversion and defect kernels derive AW,
+    AS and AP) gives the same bits as with the full assembly (unstaged tiles, not lean)."""
+    cfg = ("N_cells_x = 256\nN_cells_y = 256\nRe = 1000\nmomentum_solver = gs_multicolour\n"
+           "pressure_solver = mg\nmg_precision = mixed\nmg_dc_maxit = 2\nmaxit = 5\n"
+           "error_tolerance = 0\nbackend = gpu\n")
+    p, _ = sg.params_from_config(cfg)
+    runs = []
+    for staged in (0, 64):  # 0: staged from level 0, lean; 64: unstaged, full assembly
+        gpu = sg.Solver(p)
+        gpu.set_tuning(sg.TUNE_MID_CELLS, 0)
+        gpu.set_tuning(sg.TUNE_MG_STAGED, staged)
+        done, log, _ = gpu.advance(5)
+        runs.append((log, [gpu.get(f) for f in ("u", "v", "p", "pc", "AWP", "ASP", "APP")]))
+        gpu.close()
+    assert bits_equal(runs[0][0], runs[1][0]), first_diff(runs[0][0], runs[1][0])
+    for a, b in zip(runs[0][1], runs[1][1]):
+        assert bits_equal(a, b), first_diff(a, b)
