@@ -106,10 +106,13 @@ void enqueue_iteration(Solver& s, const int* stop, bool eager, hipEvent_t* ev) {
     launch_assemble_p(s, s.scal + S_RES_P, stop, defer, mg_lean_ok(s));
     mark(9);
     // The residual sums, all deferred by now, may ride along with the p' solve's first mean
-    // removal (multigrid), which sums its right-hand side anyway.
+    // removal (multigrid), which sums its right-hand side anyway; the multigrid solve's last mean
+    // removal is carried by correct.
     s.ride = defer && !s.rec;
+    s.fuse_pc_mean = defer && p.pressure_solver == SG_P_MG;
     launch_solve_p(s, stop, eager);
     s.ride = false;
+    s.fuse_pc_mean = false;
     const bool rode = s.rode;
     s.rode = false;
     mark(10);
@@ -137,12 +140,13 @@ void capture_graph(Solver& s, int reps, hipGraphExec_t* out) {
         return;
     struct Restore {
         Solver& s;
-        bool f0, f1, p0, p1, lean;
+        bool f0, f1, p0, p1, lean, pcs;
         ~Restore() {
             s.face[0] = f0, s.face[1] = f1, s.pt_stale[0] = p0, s.pt_stale[1] = p1;
-            s.p_lean = lean;
+            s.p_lean = lean, s.pc_in_scratch = pcs;
+            s.fuse_pc_mean = s.pc_mean_pending = false;
         }
-    } restore{s, s.face[0], s.face[1], s.pt_stale[0], s.pt_stale[1], s.p_lean};
+    } restore{s, s.face[0], s.face[1], s.pt_stale[0], s.pt_stale[1], s.p_lean, s.pc_in_scratch};
     hipGraph_t g = nullptr;
     SG_HIP(hipStreamBeginCapture(s.stream, hipStreamCaptureModeRelaxed));
     try {
@@ -192,10 +196,14 @@ template <class F> int guard(F f) {
     }
 }
 
+// Every entry point reaches its solver here, so p' is back in place (materialize_pc) before any
+// of them reads or replaces it.
 Solver& S(sg_solver* h) {
     if (!h)
         throw ArgError("null solver");
-    return *reinterpret_cast<Solver*>(h);
+    Solver& s = *reinterpret_cast<Solver*>(h);
+    materialize_pc(s);
+    return s;
 }
 
 void validate(const sg_params& p) {
@@ -586,6 +594,7 @@ int sg_advance(sg_solver* h, int max_iters, int* iters_done, double* log, int* c
                 s.face[0] = face_ok(s, true), s.face[1] = face_ok(s, false);
                 s.pt_stale[0] = s.pt_stale[1] = true;
                 s.p_lean = mg_lean_ok(s);
+                s.pc_in_scratch = s.p.pressure_solver == SG_P_MG;
             }
         }
         const Ctl c = read_ctl(s);

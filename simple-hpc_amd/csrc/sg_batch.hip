@@ -75,10 +75,11 @@ void build_plan(Batch& b) {
         Solver& s = *b.cav[c];
         // Recording runs nothing: the coefficient form and pressure-term flags stay.
         const bool f0 = s.face[0], f1 = s.face[1], p0 = s.pt_stale[0], p1 = s.pt_stale[1];
-        const bool lean = s.p_lean;
+        const bool lean = s.p_lean, pcs = s.pc_in_scratch;
         auto restore = [&] {
             s.face[0] = f0, s.face[1] = f1, s.pt_stale[0] = p0, s.pt_stale[1] = p1;
-            s.p_lean = lean;
+            s.p_lean = lean, s.pc_in_scratch = pcs;
+            s.fuse_pc_mean = s.pc_mean_pending = false;
         };
         s.rec = &b.recs[c];
         try {
@@ -164,10 +165,15 @@ void build_plan(Batch& b) {
     SG_HIP(e3);
 }
 
+// Every batch entry point reaches its batch here: each cavity's p' goes back in place first
+// (materialize_pc), before snapshots, tuning or replays touch the cavities' arrays.
 Batch& B(sg_batch* h) {
     if (!h)
         throw ArgError("null batch");
-    return *reinterpret_cast<Batch*>(h);
+    Batch& b = *reinterpret_cast<Batch*>(h);
+    for (Solver* c : b.cav)
+        materialize_pc(*c);
+    return b;
 }
 
 // Replay k batched iterations (every cavity not yet stopped advances by one per replay).
@@ -457,6 +463,7 @@ int sg_batch_advance(sg_batch* h, int max_iters, int* iters, int* status, double
                     s.face[0] = face_ok(s, true), s.face[1] = face_ok(s, false);
                     s.pt_stale[0] = s.pt_stale[1] = true;
                     s.p_lean = mg_lean_ok(s);
+                    s.pc_in_scratch = s.p.pressure_solver == SG_P_MG;
                 }
             begin_chunk(b);
             replay(b, k);

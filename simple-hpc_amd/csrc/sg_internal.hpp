@@ -140,6 +140,11 @@ struct Solver {
     bool pt_stale[2] = {};             // u, v: p changed since the pressure terms were evaluated
     bool p_lean = false;               // AWP, ASP, APP not written by the last assemble_p (stale
                                        // until ensure_p_full)
+    // p' mean removal carried by correct (outer iterations with the multigrid p' solve): fuse_pc_mean
+    // arms it for the next solve, which then only sums p' into S_SUM_D (pc_mean_pending); correct
+    // reads p' - mean and writes it to scratch (pc_in_scratch), from where materialize_pc copies it
+    // back before any other use of the solver.
+    bool fuse_pc_mean = false, pc_mean_pending = false, pc_in_scratch = false;
     int mom_tile = 0;                  // momentum sweep tile shape, an index of MOM_TILES (tunable)
     int mom_tile_first = -1;           // the first sweep's tile shape (-1: mom_tile)
     C9 diff2{}, diff4{};               // diffusive coefficients: order 2 (and band), order 4
@@ -226,6 +231,10 @@ void ensure_p_full(Solver& s);
 bool mg_lean_ok(Solver& s);
 void launch_solve_p(Solver& s, const int* stop, bool eager);
 void launch_correct(Solver& s, const int* stop);
+// The multigrid solve's final p' mean removal, or only its sum when fuse_pc_mean is armed.
+void finish_pc_mean(Solver& s, const int* stop);
+// Copy the mean-free p' back from scratch if correct left it there.
+void materialize_pc(Solver& s);
 void launch_fill(Solver& s, double* a, size_t n, double v, const int* stop);
 // ordered sums (sg_reduce.hip): rows [i0, i1], columns [j0, j1] of a pitched array, row sums j
 // ascending from 0.0 into rowbuf, then rows ascending from 0.0 into *slot.

@@ -1338,7 +1338,7 @@ void launch_mg_solve(Solver& s, const int* stop, bool has_guess) {
         launch_builds(s, t, stop);
         launch_remove_mean0(s, s.F(SG_F_B), stop, s.F(SG_F_PC)); // and p' = 0
         launch_vcycles(s, t, stop);
-        launch_remove_mean0(s, s.F(SG_F_PC), stop);
+        finish_pc_mean(s, stop);
         return;
     }
     const Tail<float> t = make_tail<float>(s, lv32);
@@ -1364,7 +1364,7 @@ void launch_mg_solve(Solver& s, const int* stop, bool has_guess) {
         launch_vcycles(s, t, stop);
         emit<Mix, k_mx_update, BX * BY>(s, g, dim3(BX, BY), 0, m, stop);
     }
-    launch_remove_mean0(s, s.F(SG_F_PC), stop);
+    finish_pc_mean(s, stop);
 }
 
 // The same conditions as launch_vcycles' use of staged tiles on level 0, with the tail (whose first

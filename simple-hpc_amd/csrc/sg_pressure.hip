@@ -168,6 +168,9 @@ struct CorrArgs {
     const double *APU, *APV;
     double *u, *v, *pf;
     double dx, dy, ap;
+    const double* mslot; // non-null: p' still holds its mean, total at *mslot over n cells
+    double n;
+    double* pcout;       // with mslot: p' - mean of every interior cell, written here
 };
 
 __device__ __forceinline__ void k_correct(const CorrArgs& g, const Blk& bk) {
@@ -183,11 +186,19 @@ __device__ __forceinline__ void k_correct(const CorrArgs& g, const Blk& bk) {
     const long ld = a.ld;
     const size_t q = static_cast<size_t>(i) * ld + j;
     const double* pc = a.pc;
+    // With mslot, every p' read is p' - mean, the value the mean removal (k_total_sub) would
+    // have stored (the same subtraction), so the corrections have the same bits.
+    const bool sub = g.mslot != nullptr;
+    const double m = sub ? *g.mslot / g.n : 0.0;
+    auto pm = [&](size_t k) { return sub ? pc[k] - m : pc[k]; };
+    const double pq = pm(q);
     if (j <= a.NY + H - 1)
-        u[q] += dy * (pc[q] - pc[q + 1]) / APU[q];
+        u[q] += dy * (pq - pm(q + 1)) / APU[q];
     if (i >= H + 1)
-        v[q] += dx * (pc[q] - pc[q - ld]) / APV[q];
-    pf[q] += ap * pc[q];
+        v[q] += dx * (pq - pm(q - ld)) / APV[q];
+    pf[q] += ap * pq;
+    if (sub)
+        g.pcout[q] = pq;
 }
 
 struct FillArgs {
@@ -388,10 +399,45 @@ void ensure_p_full(Solver& s) {
 void launch_correct(Solver& s, const int* stop) {
     const sg_params& p = s.p;
     s.pt_stale[0] = s.pt_stale[1] = true; // p changes
-    const CorrArgs c{p_args(s),   s.F(SG_F_CU_AP), s.F(SG_F_CV_AP), s.F(SG_F_U),
-                     s.F(SG_F_V), s.F(SG_F_P),     p.dx,            p.dy,
-                     p.relaxation_factor_p};
+    const bool fused = s.pc_mean_pending;
+    s.pc_mean_pending = false;
+    const CorrArgs c{p_args(s),
+                     s.F(SG_F_CU_AP),
+                     s.F(SG_F_CV_AP),
+                     s.F(SG_F_U),
+                     s.F(SG_F_V),
+                     s.F(SG_F_P),
+                     p.dx,
+                     p.dy,
+                     p.relaxation_factor_p,
+                     fused ? s.scal + S_SUM_D : nullptr,
+                     static_cast<double>(p.NX) * p.NY,
+                     fused ? s.scratch : nullptr};
     emit<CorrArgs, k_correct, BX * BY>(s, pgrid(p), dim3(BX, BY), 0, c, stop);
+    if (fused)
+        s.pc_in_scratch = true;
+}
+
+void finish_pc_mean(Solver& s, const int* stop) {
+    if (!s.fuse_pc_mean) {
+        launch_remove_mean0(s, s.F(SG_F_PC), stop);
+        return;
+    }
+    const sg_params& p = s.p;
+    launch_ordered_sum(s, s.F(SG_F_PC), s.ld, p.H, p.NX + p.H - 1, p.H + 1, p.NY + p.H,
+                       s.scal + S_SUM_D, stop, 0);
+    s.pc_mean_pending = true;
+}
+
+void materialize_pc(Solver& s) {
+    if (!s.pc_in_scratch)
+        return;
+    s.pc_in_scratch = false;
+    const sg_params& p = s.p;
+    const size_t o = static_cast<size_t>(p.H) * s.ld + p.H + 1;
+    SG_HIP(hipMemcpy2DAsync(s.F(SG_F_PC) + o, sizeof(double) * s.ld, s.scratch + o,
+                            sizeof(double) * s.ld, sizeof(double) * p.NY, p.NX,
+                            hipMemcpyDeviceToDevice, s.stream));
 }
 
 void launch_remove_mean0(Solver& s, double* x, const int* stop, double* zero) {
