@@ -110,9 +110,10 @@ void enqueue_iteration(Solver& s, const int* stop, bool eager, hipEvent_t* ev) {
     // removal is carried by correct.
     s.ride = defer && !s.rec;
     s.fuse_pc_mean = defer && p.pressure_solver == SG_P_MG;
+    s.fuse_b_mean = defer && mg_fuse_b_ok(s);
     launch_solve_p(s, stop, eager);
     s.ride = false;
-    s.fuse_pc_mean = false;
+    s.fuse_pc_mean = s.fuse_b_mean = false;
     const bool rode = s.rode;
     s.rode = false;
     mark(10);
@@ -140,13 +141,14 @@ void capture_graph(Solver& s, int reps, hipGraphExec_t* out) {
         return;
     struct Restore {
         Solver& s;
-        bool f0, f1, p0, p1, lean, pcs;
+        bool f0, f1, p0, p1, lean, pcs, braw;
         ~Restore() {
             s.face[0] = f0, s.face[1] = f1, s.pt_stale[0] = p0, s.pt_stale[1] = p1;
-            s.p_lean = lean, s.pc_in_scratch = pcs;
-            s.fuse_pc_mean = s.pc_mean_pending = false;
+            s.p_lean = lean, s.pc_in_scratch = pcs, s.b_raw = braw;
+            s.fuse_pc_mean = s.pc_mean_pending = s.fuse_b_mean = s.b_mean_pending = false;
         }
-    } restore{s, s.face[0], s.face[1], s.pt_stale[0], s.pt_stale[1], s.p_lean, s.pc_in_scratch};
+    } restore{s, s.face[0], s.face[1], s.pt_stale[0], s.pt_stale[1], s.p_lean, s.pc_in_scratch,
+              s.b_raw};
     hipGraph_t g = nullptr;
     SG_HIP(hipStreamBeginCapture(s.stream, hipStreamCaptureModeRelaxed));
     try {
@@ -203,6 +205,7 @@ Solver& S(sg_solver* h) {
         throw ArgError("null solver");
     Solver& s = *reinterpret_cast<Solver*>(h);
     materialize_pc(s);
+    materialize_b(s);
     return s;
 }
 
@@ -595,6 +598,7 @@ int sg_advance(sg_solver* h, int max_iters, int* iters_done, double* log, int* c
                 s.pt_stale[0] = s.pt_stale[1] = true;
                 s.p_lean = mg_lean_ok(s);
                 s.pc_in_scratch = s.p.pressure_solver == SG_P_MG;
+                s.b_raw = mg_fuse_b_ok(s);
             }
         }
         const Ctl c = read_ctl(s);

@@ -75,11 +75,11 @@ void build_plan(Batch& b) {
         Solver& s = *b.cav[c];
         // Recording runs nothing: the coefficient form and pressure-term flags stay.
         const bool f0 = s.face[0], f1 = s.face[1], p0 = s.pt_stale[0], p1 = s.pt_stale[1];
-        const bool lean = s.p_lean, pcs = s.pc_in_scratch;
+        const bool lean = s.p_lean, pcs = s.pc_in_scratch, braw = s.b_raw;
         auto restore = [&] {
             s.face[0] = f0, s.face[1] = f1, s.pt_stale[0] = p0, s.pt_stale[1] = p1;
-            s.p_lean = lean, s.pc_in_scratch = pcs;
-            s.fuse_pc_mean = s.pc_mean_pending = false;
+            s.p_lean = lean, s.pc_in_scratch = pcs, s.b_raw = braw;
+            s.fuse_pc_mean = s.pc_mean_pending = s.fuse_b_mean = s.b_mean_pending = false;
         };
         s.rec = &b.recs[c];
         try {
@@ -171,8 +171,10 @@ Batch& B(sg_batch* h) {
     if (!h)
         throw ArgError("null batch");
     Batch& b = *reinterpret_cast<Batch*>(h);
-    for (Solver* c : b.cav)
+    for (Solver* c : b.cav) {
         materialize_pc(*c);
+        materialize_b(*c);
+    }
     return b;
 }
 
@@ -464,6 +466,7 @@ int sg_batch_advance(sg_batch* h, int max_iters, int* iters, int* status, double
                     s.pt_stale[0] = s.pt_stale[1] = true;
                     s.p_lean = mg_lean_ok(s);
                     s.pc_in_scratch = s.p.pressure_solver == SG_P_MG;
+                    s.b_raw = mg_fuse_b_ok(s);
                 }
             begin_chunk(b);
             replay(b, k);

@@ -94,7 +94,7 @@ struct Ctl {
 
 // Device scalar slots.
 enum {
-    S_RES_U = 0, S_RES_V, S_RES_P, S_SUM_A, S_SUM_B, S_SUM_C, S_SUM_D, S_COUNT = 32
+    S_RES_U = 0, S_RES_V, S_RES_P, S_SUM_A, S_SUM_B, S_SUM_C, S_SUM_D, S_BSUM, S_COUNT = 32
 };
 
 struct Warm;
@@ -145,6 +145,11 @@ struct Solver {
     // reads p' - mean and writes it to scratch (pc_in_scratch), from where materialize_pc copies it
     // back before any other use of the solver.
     bool fuse_pc_mean = false, pc_mean_pending = false, pc_in_scratch = false;
+    // b's mean removal carried by the V-cycle (the same outer iterations): fuse_b_mean arms it for
+    // the next multigrid solve, which then only sums b into S_BSUM and zeroes p' outside the
+    // interior; b_mean_pending: level 0's kernels read b - mean and start from p' = 0 unread;
+    // b_raw: b still holds its mean, until materialize_b subtracts it.
+    bool fuse_b_mean = false, b_mean_pending = false, b_raw = false;
     int mom_tile = 0;                  // momentum sweep tile shape, an index of MOM_TILES (tunable)
     int mom_tile_first = -1;           // the first sweep's tile shape (-1: mom_tile)
     C9 diff2{}, diff4{};               // diffusive coefficients: order 2 (and band), order 4
@@ -235,6 +240,14 @@ void launch_correct(Solver& s, const int* stop);
 void finish_pc_mean(Solver& s, const int* stop);
 // Copy the mean-free p' back from scratch if correct left it there.
 void materialize_pc(Solver& s);
+// Subtract b's mean from b if the multigrid solve left it in (b_raw).
+void materialize_b(Solver& s);
+// b's sums for a fused mean removal (rows, then totals into S_BSUM with the deferred residual
+// totals, and p' zeroed outside the interior).
+void launch_b_mean_fused(Solver& s, const int* stop);
+// Whether b's mean removal can be carried by the V-cycle: multigrid staged from level 0, level 0
+// above the tail, no learned warm start.
+bool mg_fuse_b_ok(Solver& s);
 void launch_fill(Solver& s, double* a, size_t n, double v, const int* stop);
 // ordered sums (sg_reduce.hip): rows [i0, i1], columns [j0, j1] of a pitched array, row sums j
 // ascending from 0.0 into rowbuf, then rows ascending from 0.0 into *slot.

@@ -282,6 +282,9 @@ template <class T> struct DU {
     T* xout;
     int sweeps, build;
     T omega;
+    const double* bmean = nullptr; // level 0, b's mean not removed: b - *bmean / bn is read
+    double bn = 0.0;
+    int xzero = 0;                 // the iterate is zero (first V-cycle from p' = 0): not read
 };
 
 template <class T> __device__ __forceinline__ void k_mg_down(const DU<T>& a, const Blk& bk) {
@@ -413,11 +416,15 @@ template <class T> __device__ __forceinline__ Stage<T> stage_view(int gi0, int g
 
 // x of the tile and its halo (zero outside the level's padded range; prolonged if C), and the
 // staged operator inside the level (zero elsewhere: the walls' AW and AS read there).
+// bmean: b - *bmean / bn is staged, the value b's mean removal (k_total_sub) would store, by the
+// same subtraction; xzero: the iterate is zero everywhere, so it is not read.
 template <class T, int NTH>
 __device__ __forceinline__ void stage_load(const Stage<T>& st, const Lv<T>& L, const T* x, int LH,
-                                           const Lv<T>* C, T omega) {
+                                           const Lv<T>* C, T omega, const double* bmean = nullptr,
+                                           double bn = 0.0, int xzero = 0) {
     constexpr int B = 4;
     const int n = LH * st.LW;
+    const T bm = bmean ? static_cast<T>(*bmean / bn) : T(0);
     for (int base = 0; base < n; base += B * NTH) {
         T vx[B], ve[B], vn[B], vb[B], vp[B];
 #pragma unroll
@@ -427,13 +434,13 @@ __device__ __forceinline__ void stage_load(const Stage<T>& st, const Lv<T>& L, c
             vx[k] = ve[k] = vn[k] = vb[k] = vp[k] = T(0);
             if (e < n && I >= -1 && I <= L.nx && J >= -1 && J <= L.ny) {
                 const long q = static_cast<long>(I) * L.stride + J;
-                vx[k] = x[q];
+                vx[k] = xzero ? T(0) : x[q];
                 if (I >= 0 && I < L.nx && J >= 0 && J < L.ny) {
                     if (C)
                         vx[k] = vx[k] + omega * C->x[static_cast<size_t>(I / 2) * C->stride + J / 2];
                     ve[k] = L.AE[q];
                     vn[k] = L.AN[q];
-                    vb[k] = L.b[q];
+                    vb[k] = bmean ? L.b[q] - bm : L.b[q];
                     if constexpr (!SG_DERIVE_AP<T>)
                         vp[k] = L.AP[q];
                 }
@@ -491,7 +498,7 @@ template <class T, int TS, int TI = TS> __device__ __forceinline__ void k_mg_dow
     xcd_tile(bk, tx, ty);
     const int ti0 = ty * TI, tj0 = tx * TS;
     const Stage<T> st = stage_view<T>(ti0 - H, tj0 - H, LH, LW);
-    stage_load<T, NTH>(st, F, a.xin, LH, nullptr, T(0));
+    stage_load<T, NTH>(st, F, a.xin, LH, nullptr, T(0), a.bmean, a.bn, a.xzero);
     for (int p = 0; p < 2 * pre; ++p) {
         const int d = 2 * pre - p;
         stage_pass<T, NTH>(st, p & 1, max(0, ti0 - d), min(F.nx - 1, ti0 + TI - 1 + d), max(0, tj0 - d),
@@ -543,7 +550,7 @@ template <class T, int TS, int TI = TS> __device__ __forceinline__ void k_mg_up_
     xcd_tile(bk, tx, ty);
     const int ti0 = ty * TI, tj0 = tx * TS;
     const Stage<T> st = stage_view<T>(ti0 - H, tj0 - H, LH, LW);
-    stage_load<T, NTH>(st, F, a.xin, LH, &a.C, a.omega);
+    stage_load<T, NTH>(st, F, a.xin, LH, &a.C, a.omega, a.bmean, a.bn);
     for (int p = 0; p < 2 * post; ++p) {
         const int d = 2 * post - 1 - p;
         stage_pass<T, NTH>(st, p & 1, max(0, ti0 - d), min(F.nx - 1, ti0 + TI - 1 + d), max(0, tj0 - d),
@@ -1010,7 +1017,13 @@ struct Mix {
     double* x;               // FP64 p' (at interior cell 0, 0)
     Lv<float> f;             // FP32 level 0
     int lean;                // AW, AS, AP not assembled: derived from AE and AN (ensure_p_full)
+    const double* bmean;     // b's mean not removed: b - *bmean / bn is read (as k_total_sub)
+    double bn;
+    int first;               // p' not zero-filled: the first update sets it (0.0 + e)
 };
+__device__ __forceinline__ double mix_b(const Mix& m, size_t q) {
+    return m.bmean ? m.b[q] - *m.bmean / m.bn : m.b[q];
+}
 
 // Level-0 AW, AS and AP of interior cell (I, J) at q: stored, or after a lean assemble_p the
 // values ensure_p_full would derive.
@@ -1040,7 +1053,7 @@ __device__ __forceinline__ void k_mx_init(const Mix& m, const Blk& bk) {
     m.f.AN[g] = static_cast<float>(m.AN[q]);
     m.f.AS[g] = static_cast<float>(as);
     m.f.AP[g] = static_cast<float>(ap);
-    m.f.b[g] = static_cast<float>(m.b[q]);
+    m.f.b[g] = static_cast<float>(mix_b(m, q));
     m.f.x[g] = 0.0f;
 }
 
@@ -1054,8 +1067,8 @@ __device__ __forceinline__ void k_mx_defect(const Mix& m, const Blk& bk) {
     const double* x = m.x;
     double aw, as, ap;
     mix_op(m, I, J, q, aw, as, ap);
-    const double r = m.b[q] - (ap * x[q] - (m.AE[q] * x[q + 1] + aw * x[q - 1] +
-                                            as * x[q + s] + m.AN[q] * x[q - s]));
+    const double r = mix_b(m, q) - (ap * x[q] - (m.AE[q] * x[q + 1] + aw * x[q - 1] +
+                                                 as * x[q + s] + m.AN[q] * x[q - s]));
     m.f.b[g] = static_cast<float>(r);
     m.f.x[g] = 0.0f;
 }
@@ -1065,8 +1078,9 @@ __device__ __forceinline__ void k_mx_update(const Mix& m, const Blk& bk) {
     const int J = bk.x * BX + threadIdx.x, I = bk.y * BY + threadIdx.y;
     if (I >= m.nx || J >= m.ny)
         return;
-    m.x[static_cast<size_t>(I) * m.ld + J] +=
-        static_cast<double>(m.f.x[static_cast<size_t>(I) * m.f.stride + J]);
+    double& x = m.x[static_cast<size_t>(I) * m.ld + J];
+    // From p' = 0 without the zero fill (fused mean removal): 0.0 + e, as the fill then += gives.
+    x = (m.first ? 0.0 : x) + static_cast<double>(m.f.x[static_cast<size_t>(I) * m.f.stride + J]);
 }
 
 dim3 lgrid(int nx, int ny) { return dim3(cdiv(ny, BX), cdiv(nx, BY)); }
@@ -1172,6 +1186,12 @@ template <class T> void launch_vcycles(Solver& s, const Tail<T>& t, const int* s
     const bool staged = tiled && s.mg_staged < t.tm && std::max(sdn, sup) <= s.lds_budget + 1024;
     if (staged)
         staged_lds<T>(s);
+    // b's mean removal carried by level 0's staged kernels (FP64 only: the mixed form's FP32 b
+    // is converted with it).
+    const bool bfused = sizeof(T) == 8 && s.b_mean_pending;
+    const double bn = static_cast<double>(p.NX) * p.NY;
+    if (bfused && !(staged && s.mg_staged == 0 && t.tm >= 1))
+        throw HipError("fused mean removal without staged level-0 tiles");
     for (int cyc = 0; cyc < p.mg_cycles; ++cyc) {
         const int v = s.vcount++;
         for (int l = 0; l < t.tm; ++l) {
@@ -1180,7 +1200,11 @@ template <class T> void launch_vcycles(Solver& s, const Tail<T>& t, const int* s
             if (tiled) {
                 // The first cycle builds level l+1 here unless the tail builds it (l+1 = tm).
                 const int build = (cyc == 0 && l + 1 < t.tm) ? 1 : 0;
-                const DU<T> du{F, C, F.x, F.xt, p.mg_pre, build, T(0)};
+                DU<T> du{F, C, F.x, F.xt, p.mg_pre, build, T(0)};
+                if (bfused && l == 0) {
+                    du.bmean = s.scal + S_BSUM, du.bn = bn;
+                    du.xzero = cyc == 0 ? 1 : 0;
+                }
                 if (staged && l >= s.mg_staged && big(F))
                     emit<DU<T>, k_mg_down_s<T, 32, BR<T>>, sg_n<T, 32, BR<T>>()>(
                         s, dim3(cdiv(F.ny, 32), cdiv(F.nx, BR<T>)), dim3(sg_n<T, 32, BR<T>>()),
@@ -1205,7 +1229,9 @@ template <class T> void launch_vcycles(Solver& s, const Tail<T>& t, const int* s
         for (int l = t.tm - 1; l >= 0; --l) {
             const Lv<T> F = t.G[l], C = t.G[l + 1];
             s.tag = mg_tag(v, 1, l);
-            const DU<T> du{F, C, F.xt, F.x, p.mg_post, 0, t.omega};
+            DU<T> du{F, C, F.xt, F.x, p.mg_post, 0, t.omega};
+            if (bfused && l == 0)
+                du.bmean = s.scal + S_BSUM, du.bn = bn;
             if (staged && l >= s.mg_staged && big(F)) {
                 emit<DU<T>, k_mg_up_s<T, 32, BR<T>>, sg_n<T, 32, BR<T>>()>(
                     s, dim3(cdiv(F.ny, 32), cdiv(F.nx, BR<T>)), dim3(sg_n<T, 32, BR<T>>()),
@@ -1336,21 +1362,30 @@ void launch_mg_solve(Solver& s, const int* stop, bool has_guess) {
     if (p.mg_precision != SG_MG_MIXED) {
         const Tail<double> t = make_tail<double>(s, lv64);
         launch_builds(s, t, stop);
-        launch_remove_mean0(s, s.F(SG_F_B), stop, s.F(SG_F_PC)); // and p' = 0
+        if (s.fuse_b_mean)
+            launch_b_mean_fused(s, stop); // level 0 reads b - mean, p' = 0 unread
+        else
+            launch_remove_mean0(s, s.F(SG_F_B), stop, s.F(SG_F_PC)); // and p' = 0
         launch_vcycles(s, t, stop);
+        s.b_mean_pending = false;
         finish_pc_mean(s, stop);
         return;
     }
     const Tail<float> t = make_tail<float>(s, lv32);
     const MgLevel& L0 = s.mg[0];
-    const Mix m{L0.nx, L0.ny, s.ld, L0.AE, L0.AW, L0.AN, L0.AS, L0.AP, L0.b, L0.x, t.G[0],
-                s.p_lean ? 1 : 0};
+    const bool bfused = s.fuse_b_mean;
+    Mix m{L0.nx, L0.ny, s.ld, L0.AE, L0.AW, L0.AN, L0.AS, L0.AP, L0.b, L0.x, t.G[0],
+          s.p_lean ? 1 : 0, bfused ? s.scal + S_BSUM : nullptr,
+          static_cast<double>(p.NX) * p.NY, 0};
     const dim3 g = lgrid(m.nx, m.ny);
     // Learned warm start: the network's guess of p' (or zero, if the guess does not lower the
     // defect) replaces the zero start.
     const bool warm = p.pressure_warm_start == SG_WARM_LEARNED && s.warm;
-    launch_remove_mean0(s, s.F(SG_F_B), stop,
-                        !has_guess && !warm ? s.F(SG_F_PC) : nullptr); // and p' = 0
+    if (bfused)
+        launch_b_mean_fused(s, stop); // the conversions read b - mean; p' set by the first update
+    else
+        launch_remove_mean0(s, s.F(SG_F_B), stop,
+                            !has_guess && !warm ? s.F(SG_F_PC) : nullptr); // and p' = 0
     if (warm) {
         launch_warm_guess(s, stop);
         has_guess = true;
@@ -1362,16 +1397,16 @@ void launch_mg_solve(Solver& s, const int* stop, bool has_guess) {
             emit<Mix, k_mx_defect, BX * BY>(s, g, dim3(BX, BY), 0, m, stop);
         }
         launch_vcycles(s, t, stop);
+        m.first = bfused && k == 0 && !has_guess ? 1 : 0;
         emit<Mix, k_mx_update, BX * BY>(s, g, dim3(BX, BY), 0, m, stop);
     }
+    s.b_mean_pending = false;
     finish_pc_mean(s, stop);
 }
 
-// The same conditions as launch_vcycles' use of staged tiles on level 0, with the tail (whose first
-// level is built from the level above it, in global memory) starting at level 2 or below.
-// The mixed form's FP64 conversion and defect kernels derive the three themselves (mix_op); the
-// learned warm start reads them.
-bool mg_lean_ok(Solver& s) {
+// The same conditions as launch_vcycles' use of staged tiles on level 0, the tail starting at
+// level min_tm or below; no learned warm start (it reads level 0's operator and b).
+static bool staged_from0(Solver& s, int min_tm) {
     const sg_params& p = s.p;
     if (p.pressure_solver != SG_P_MG || s.mg.empty() || !tiled_ok(s) || s.mg_staged != 0 ||
         p.pressure_warm_start == SG_WARM_LEARNED)
@@ -1381,7 +1416,16 @@ bool mg_lean_ok(Solver& s) {
     const int t0 = mixed ? lds_start<float>(s) : lds_start<double>(s), tm = mid_start(s, t0);
     const size_t sdn = mixed ? staged_bytes<float>(32, hdn) : staged_bytes<double>(32, hdn);
     const size_t sup = mixed ? staged_bytes<float>(32, hup) : staged_bytes<double>(32, hup);
-    return tm >= 2 && std::max(sdn, sup) <= s.lds_budget + 1024;
+    return tm >= min_tm && std::max(sdn, sup) <= s.lds_budget + 1024;
 }
+
+// The tail builds its first level from the level above it in global memory, so level 0's AW, AS
+// and AP go unread only if the tail starts at level 2 or below. The mixed form's FP64 conversion
+// and defect kernels derive the three themselves (mix_op).
+bool mg_lean_ok(Solver& s) { return staged_from0(s, 2); }
+
+// Level 0 above the tail: only level 0's staged kernels (or the mixed form's conversions) read b
+// and the initial p'.
+bool mg_fuse_b_ok(Solver& s) { return staged_from0(s, 1); }
 
 } // namespace sg

@@ -382,6 +382,7 @@ void launch_assemble_p(Solver& s, double* slot, const int* stop, bool defer, boo
                                                lean ? 1 : 0},
                                         stop);
     s.p_lean = lean;
+    s.b_raw = false; // a new b, as the reference's assemble_p leaves it
     if (defer)
         defer_sum(s, 3, s.rcell[2], p.NX, p.NY, slot);
     else

@@ -146,6 +146,54 @@ __device__ __forceinline__ void k_rows_total_multi(const MultiArgs& ma, const Bl
     rows_total(ma.j[bk.x]);
 }
 
+// The totals of ma's sums (one workgroup each), and the remaining workgroups zero an array
+// outside its interior rows [H, NX+H-1] x columns [H+1, NY+H] (p' before a solve from zero).
+struct TotRing {
+    MultiArgs ma;
+    double* x;
+    long ld;
+    int nr, nc, H, NX, NY;
+};
+__device__ __forceinline__ void k_totals_ring(const TotRing& a, const Blk& bk) {
+    if (bk.x < a.ma.n) {
+        rows_total(a.ma.j[bk.x]);
+        return;
+    }
+    const int H = a.H, nc = a.nc, w = nc - a.NY; // columns outside the interior per row
+    const int top = H * nc, bot = (a.nr - a.NX - H) * nc, total = top + bot + a.NX * w;
+    const int nw = bk.nx - a.ma.n;
+    for (int k = (bk.x - a.ma.n) * NT + threadIdx.x; k < total; k += nw * NT) {
+        int i, j;
+        if (k < top) {
+            i = k / nc, j = k % nc;
+        } else if (k < top + bot) {
+            const int kk = k - top;
+            i = a.NX + H + kk / nc, j = kk % nc;
+        } else {
+            const int kk = k - top - bot, c = kk % w;
+            i = H + kk / w, j = c <= H ? c : a.NY + c;
+        }
+        a.x[static_cast<size_t>(i) * a.ld + j] = 0.0;
+    }
+}
+
+// b -= total / n over the interior (what k_total_sub stores).
+struct BSubArgs {
+    double* b;
+    long ld;
+    int H, NX, NY;
+    const double* tot;
+    double n;
+};
+__global__ void __launch_bounds__(NT) k_bsub(BSubArgs a) {
+    const int j = a.H + 1 + blockIdx.x * NT + threadIdx.x, i = a.H + blockIdx.y;
+    if (j > a.NY + a.H)
+        return;
+    const double m = *a.tot / a.n;
+    double* q = a.b + static_cast<size_t>(i) * a.ld + j;
+    *q = *q - m;
+}
+
 // The totals of up to MAXJ sums (wave w adds sum w's row sums in row order, in parallel), then
 // the stopping rule, which reads them.
 constexpr int TS_CH = 1024; // row sums per LDS chunk and sum
@@ -344,6 +392,60 @@ void launch_ordered_mean_sub(Solver& s, double* x, int pitch, int i0, int i1, in
     }
     emit<MeanArgs, k_total_sub, NT>(s, dim3(std::min(nrow, MEAN_WG)), dim3(NT), 0,
                                     MeanArgs{sa, x, n, zero, nzero, res}, stop);
+}
+
+void launch_b_mean_fused(Solver& s, const int* stop) {
+    const sg_params& p = s.p;
+    const SumArgs sb{s.F(SG_F_B), static_cast<long>(s.ld), p.H, p.NX, p.H + 1, p.NY, s.rowbuf,
+                     s.scal + S_BSUM};
+    // The same launch structure as launch_ordered_mean_sub (one row launch, k_row_sums_multi only
+    // when residual sums ride along), so batched cavities with and without the fusion still merge.
+    MultiArgs rows{}, tots{};
+    int blocks = 0;
+    const bool riding = s.ride && !s.rec && !s.deferred.empty() && s.deferred.size() < MAXJ;
+    if (riding) {
+        s.ride = false;
+        s.rode = true;
+        for (const DeferredSum& d : s.deferred) {
+            if (!d.cell) {
+                emit<SlotArgs, k_zero_slot, 64>(s, dim3(1), dim3(64), 0, SlotArgs{d.slot}, stop);
+                continue;
+            }
+            rows.j[rows.n] = SumArgs{d.cell, static_cast<long>(d.w), 0, d.nrow, 0, d.w,
+                                     s.rowbuf + static_cast<size_t>(d.lane) * s.nr, d.slot};
+            rows.first[rows.n] = blocks;
+            blocks += static_cast<int>(cdiv(d.nrow, RW));
+            ++rows.n;
+        }
+        s.deferred.clear();
+    }
+    rows.j[rows.n] = sb;
+    rows.first[rows.n] = blocks;
+    blocks += static_cast<int>(cdiv(p.NX, RW));
+    ++rows.n;
+    rows.first[rows.n] = blocks;
+    if (riding)
+        emit<MultiArgs, k_row_sums_multi, NT>(s, dim3(blocks), dim3(NT), 0, rows, stop);
+    else
+        emit<SumArgs, k_row_sums, NT>(s, dim3(cdiv(p.NX, RW)), dim3(NT), 0, sb, stop);
+    tots = rows;
+    constexpr int RING_WG = 8;
+    emit<TotRing, k_totals_ring, NT>(
+        s, dim3(tots.n + RING_WG), dim3(NT), 0,
+        TotRing{tots, s.F(SG_F_PC), static_cast<long>(s.ld), s.nr, s.nc, p.H, p.NX, p.NY}, stop);
+    s.b_mean_pending = true;
+    s.b_raw = true;
+}
+
+void materialize_b(Solver& s) {
+    if (!s.b_raw)
+        return;
+    s.b_raw = false;
+    const sg_params& p = s.p;
+    hipLaunchKernelGGL(k_bsub, dim3(cdiv(p.NY, NT), p.NX), dim3(NT), 0, s.stream,
+                       BSubArgs{s.F(SG_F_B), static_cast<long>(s.ld), p.H, p.NX, p.NY,
+                                s.scal + S_BSUM, static_cast<double>(p.NX) * p.NY});
+    SG_HIP(hipGetLastError());
 }
 
 void launch_ordered_sum(Solver& s, const double* a, int pitch, int i0, int i1, int j0, int j1,

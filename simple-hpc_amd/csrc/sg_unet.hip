@@ -1212,6 +1212,8 @@ int sg_attach_warm_start(sg_solver* sh, sg_unet* net, int level) {
         if (!sh)
             throw ArgError("null solver");
         Solver& s = *reinterpret_cast<Solver*>(sh);
+        materialize_pc(s);
+        materialize_b(s);
         warm_attach(s, reinterpret_cast<UNet*>(net), level);
     });
 }
@@ -1221,6 +1223,8 @@ int sg_warm_features(sg_solver* sh, int level, float* x, double* scale) {
         if (!sh || !x)
             throw ArgError("null pointer");
         Solver& s = *reinterpret_cast<Solver*>(sh);
+        materialize_pc(s);
+        materialize_b(s);
         if (!s.warm || s.warm->level != level) {
             UNet* keep = s.warm ? s.warm->net : nullptr;
             warm_attach(s, keep, level);
@@ -1247,6 +1251,8 @@ int sg_warm_stats(sg_solver* sh, int* accepted, int* rejected) {
         if (!sh)
             throw ArgError("null solver");
         Solver& s = *reinterpret_cast<Solver*>(sh);
+        materialize_pc(s);
+        materialize_b(s);
         Ctl c;
         SG_HIP(hipMemcpyAsync(&c, s.ctl, sizeof c, hipMemcpyDeviceToHost, s.stream));
         SG_HIP(hipStreamSynchronize(s.stream));
