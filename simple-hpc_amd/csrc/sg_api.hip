@@ -98,10 +98,13 @@ void enqueue_iteration(Solver& s, const int* stop, bool eager, hipEvent_t* ev) {
     mark(4);
     launch_assemble(s, false, stop, false, face_ok(s, false));
     mark(5);
-    launch_bc(s, false, stop);
+    // Without per-phase events, v's ghost columns are set by its first sweep.
+    const bool bcv = defer && bcv_in_sweep(s);
+    if (!bcv)
+        launch_bc(s, false, stop);
     mark(6);
     mark(7);
-    launch_mom_sweeps(s, false, p.maxit_inner_v, stop, s.scal + S_RES_V, defer);
+    launch_mom_sweeps(s, false, p.maxit_inner_v, stop, s.scal + S_RES_V, defer, bcv);
     mark(8);
     launch_assemble_p(s, s.scal + S_RES_P, stop, defer, mg_lean_ok(s));
     mark(9);
@@ -367,6 +370,8 @@ Solver* create_solver(const sg_params& p, const double* u, const double* v, cons
         for (int k = 0; k < SG_F_COUNT; ++k)
             s.f[k] = dalloc<double>(s, s.n);
         s.scratch = dalloc<double>(s, s.n);
+        if (p.momentum_solver == SG_MOM_GS_MULTICOLOUR)
+            s.scratch2 = dalloc<double>(s, s.n);
         s.cell = dalloc<double>(s, 4 * static_cast<size_t>(std::max(p.NX + 2, 1)) * (p.NY + 2));
         s.rowbuf = dalloc<double>(s, 4 * static_cast<size_t>(s.nr));
         s.scal = dalloc<double>(s, S_COUNT);

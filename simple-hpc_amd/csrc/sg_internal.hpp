@@ -113,7 +113,8 @@ struct Solver {
     hipStream_t stream = nullptr;
     int device = 0;
     double* f[SG_F_COUNT] = {};
-    double* scratch = nullptr;         // Jacobi previous iterate
+    double* scratch = nullptr;         // Jacobi previous iterate, multicolour sweep ping-pong
+    double* scratch2 = nullptr;        // multicolour: third buffer of odd sweep counts
     double* cell = nullptr;            // per-cell values for ordered sums (4 planes of n)
     double* rowbuf = nullptr;          // row sums, one nr-slice per ordered-sum lane (4 lanes)
     double* rcell[3] = {};             // |R| cells of res_u, res_v, res_p (deferred sums)
@@ -217,10 +218,13 @@ void refresh_pterm(Solver& s, bool is_u);
 Pack make_face_pack(const sg_params& p, bool is_u);
 void launch_bc(Solver& s, bool is_u, const int* stop);
 void launch_mom_residual(Solver& s, bool is_u, double* slot, const int* stop, bool defer);
+// bcv (v, multicolour, at least one sweep): v's ghost columns are set by the first sweep, which
+// loads them as the boundary condition gives them, instead of a launch_bc before the sweeps.
+bool bcv_in_sweep(const Solver& s);
 void launch_mom_sweeps(Solver& s, bool is_u, int nsweeps, const int* stop, double* res_slot,
-                       bool defer);
+                       bool defer, bool bcv = false);
 void launch_mom_sweep_once(Solver& s, bool is_u, const double* src, double* dst, bool first,
-                           const int* stop);
+                           const int* stop, bool bcv = false);
 int ncolours(const sg_params& p);
 void launch_copy(Solver& s, const double* a, double* b, size_t n, const int* stop);
 Pack make_pack(const sg_params& p, bool is_u);

@@ -493,6 +493,9 @@ struct SweepArgs {
     double* dst;
     double* cell;
     int nr, nc, first;
+    // FIRST only: v's ghost columns (src/momentum_v.cpp:24-34). Cells (i, bjw) and (i, bje) of
+    // rows bi0..bi1 are loaded as minus their inner neighbour; none when bi1 < bi0.
+    int bi0, bi1, bjw, bje;
 };
 
 // FIRST: a second LDS copy keeps the loaded (pre-sweep) values of the tile and the ring of R
@@ -528,9 +531,16 @@ __device__ __forceinline__ void k_mom_sweep(const SweepArgs& sa, const Blk& bk) 
 #pragma unroll
         for (int k = 0; k < NL; ++k) {
             const int e = k * NT + threadIdx.x, a = e / LW, b = e - a * LW, i = gi0 + a, j = gj0 + b;
-            v[k] = (e < LH * LW && i >= 0 && i < nr && j >= 0 && j < nc)
-                       ? src[static_cast<size_t>(i) * ld + j]
-                       : 0.0;
+            int jl = j;
+            bool neg = false;
+            if (FIRST && i >= sa.bi0 && i <= sa.bi1 && (j == sa.bjw || j == sa.bje)) {
+                neg = true;
+                jl = j == sa.bjw ? j + 1 : j - 1;
+            }
+            const double x = (e < LH * LW && i >= 0 && i < nr && j >= 0 && j < nc)
+                                 ? src[static_cast<size_t>(i) * ld + jl]
+                                 : 0.0;
+            v[k] = neg ? -x : x;
         }
 #pragma unroll
         for (int k = 0; k < NL; ++k) {
@@ -907,8 +917,11 @@ void sweep_tile_f(Solver& s, const SweepArgs& a, bool face, const int* stop,
 
 template <int NCOL>
 void sweep_tile(Solver& s, const MomArgs& m, const double* src, double* dst, double* cell,
-                int first, bool face, const int* stop) {
-    const SweepArgs a{m, src, dst, cell, s.nr, s.nc, first};
+                int first, bool face, const int* stop, bool bcv) {
+    const sg_params& p = s.p;
+    SweepArgs a{m, src, dst, cell, s.nr, s.nc, first, 0, -1, p.H, p.NY + p.H + 1};
+    if (bcv) // the rows of launch_bc's v ghost columns
+        a.bi0 = p.H, a.bi1 = p.NX + p.H - 1;
     constexpr auto shapes = std::make_integer_sequence<int, N_MOM_TILES>{};
     if (first)
         sweep_tile_f<NCOL, true>(s, a, face, stop, shapes);
@@ -920,37 +933,52 @@ void sweep_tile(Solver& s, const MomArgs& m, const double* src, double* dst, dou
 // One multicolour sweep of the field into dst (tile shape s.mom_tile); first: also the residual
 // cells.
 void launch_mom_sweep_once(Solver& s, bool is_u, const double* src, double* dst, bool first,
-                           const int* stop) {
+                           const int* stop, bool bcv) {
     const MomArgs m = mom_args(s, is_u);
     double* cell = s.rcell[is_u ? 0 : 1];
     const bool face = s.face[is_u ? 0 : 1];
+    if (bcv && (is_u || !first))
+        throw HipError("v's boundary condition rides only on v's first sweep");
     if (m.pk.ncol == 2)
-        sweep_tile<2>(s, m, src, dst, cell, first ? 1 : 0, face, stop);
+        sweep_tile<2>(s, m, src, dst, cell, first ? 1 : 0, face, stop, bcv);
     else
-        sweep_tile<5>(s, m, src, dst, cell, first ? 1 : 0, face, stop);
+        sweep_tile<5>(s, m, src, dst, cell, first ? 1 : 0, face, stop, bcv);
+}
+
+bool bcv_in_sweep(const Solver& s) {
+    return s.p.momentum_solver == SG_MOM_GS_MULTICOLOUR && s.p.maxit_inner_v > 0;
 }
 
 // sweep_u_batch / sweep_v_batch (src/momentum_u.cpp:164-171) for the two parallel solvers. With
 // res_slot (multicolour only), the residual is evaluated inside the first sweep and summed (on
 // flush_sums if defer) into *res_slot.
 void launch_mom_sweeps(Solver& s, bool is_u, int nsweeps, const int* stop, double* res_slot,
-                       bool defer) {
+                       bool defer, bool bcv) {
     const sg_params& p = s.p;
     refresh_pterm(s, is_u);
     const MomArgs m = mom_args(s, is_u);
     const bool fuse = res_slot && p.momentum_solver == SG_MOM_GS_MULTICOLOUR && nsweeps > 0;
+    if (bcv && !fuse)
+        throw HipError("v's boundary condition needs a first multicolour sweep");
     if (res_slot && !fuse)
         launch_mom_residual(s, is_u, res_slot, stop, defer);
     if (nsweeps <= 0)
         return;
     if (p.momentum_solver == SG_MOM_GS_MULTICOLOUR) {
-        // Alternate field -> scratch -> field ...; an odd count ends with a copy back.
-        for (int k = 0; k < nsweeps; ++k) {
-            const bool even = (k % 2) == 0;
-            launch_mom_sweep_once(s, is_u, even ? m.x : s.scratch, even ? s.scratch : m.x,
-                                  fuse && k == 0, stop);
-        }
-        if (nsweeps % 2)
+        // Every sweep writes its whole destination array. An even count alternates field ->
+        // scratch -> field ...; an odd count above one goes field -> scratch -> scratch2 ->
+        // scratch ... -> scratch2 -> field, so that the last sweep writes the field; a single
+        // sweep ends with a copy back.
+        const bool three = nsweeps % 2 == 1 && nsweeps > 1;
+        auto out = [&](int k) -> double* {
+            if (k == nsweeps - 1 && three)
+                return m.x;
+            return k % 2 == 0 ? s.scratch : (three ? s.scratch2 : m.x);
+        };
+        for (int k = 0; k < nsweeps; ++k)
+            launch_mom_sweep_once(s, is_u, k == 0 ? m.x : out(k - 1), out(k), fuse && k == 0,
+                                  stop, bcv && k == 0);
+        if (nsweeps == 1)
             launch_copy(s, s.scratch, m.x, s.n, stop);
         if (fuse) {
             const int nrow = m.i1 - m.i0 + 1, w = m.j1 - m.j0 + 1;
