@@ -55,8 +55,6 @@ Lv<float> lv32(const MgLevel& L) {
     return {L.nx, L.ny, L.ny + 2, L.fAE, L.fAW, L.fAN, L.fAS, L.fAP, L.fb, L.fx, L.fr, L.fxt};
 }
 
-// Relative tolerance of the coarsest-level solve: the reference's 1e-10 in FP64; in FP32 a
-// tolerance the arithmetic can reach.
 // c ? a : b by bit masking. The empty asm hides the mask's origin, so the compiler cannot fold
 // the blend back into a select and then into a branch around a's computation.
 __device__ __forceinline__ double blend(bool c, double a, double b) {
@@ -70,9 +68,37 @@ __device__ __forceinline__ float blend(bool c, float a, float b) {
     return __int_as_float((__float_as_int(a) & m) | (__float_as_int(b) & ~m));
 }
 
+// Relative tolerance of the coarsest-level solve: the reference's 1e-10 in FP64; in FP32 a
+// tolerance the arithmetic can reach.
 template <class T> __device__ constexpr T coarse_tol();
 template <> __device__ constexpr double coarse_tol<double>() { return 1e-10; }
 template <> __device__ constexpr float coarse_tol<float>() { return 1e-6f; }
+
+// sqrt(rr) < c for a fixed c > 0, decided as the correctly rounded root decides it, mostly
+// without taking the root. Outside [t (1 - g), t (1 + g)], t = c * c rounded, the root lies at
+// least about g / 2 relatively below or above c, far more than the few roundings in t, in the
+// bounds and in the root (g = 1e-12 in FP64, 1e-5 in FP32), so the comparison of rr decides;
+// inside, or when t is outside the normal range, the root is taken. NaN reaches the root.
+template <class T> struct RootBelow {
+    T c, lo, hi;
+    bool direct;
+    __device__ explicit RootBelow(T c_) : c(c_) {
+        constexpr bool d = sizeof(T) == 8;
+        const T t = c * c, g = d ? T(1e-12) : T(1e-5);
+        direct = !(t > (d ? T(1e-280) : T(1e-30)) && t < (d ? T(1e280) : T(1e30)));
+        lo = t * (T(1) - g);
+        hi = t * (T(1) + g);
+    }
+    __device__ bool operator()(T rr) const {
+        if (!direct) {
+            if (rr < lo)
+                return true;
+            if (rr > hi)
+                return false;
+        }
+        return sqrt(rr) < c;
+    }
+};
 
 // ---- per-cell pieces (src/multigrid.cpp) ----
 
@@ -660,13 +686,12 @@ __device__ int wg_coarse_solve(const LV& L, T* rows) {
     const T bb = wg_det_sum(L, rows, 1);
     int sweeps = 0;
     if (bb > T(0)) {
-        const T bn = sqrt(bb);
+        const RootBelow<T> converged(coarse_tol<T>() * sqrt(bb)); // ||r|| < tol ||b||
         while (sweeps < 200) {
             wg_smooth(L, 1);
             ++sweeps;
             wg_resid(L);
-            const T rr = wg_det_sum(L, rows, 2);
-            if (sqrt(rr) < coarse_tol<T>() * bn)
+            if (converged(wg_det_sum(L, rows, 2)))
                 break;
         }
     }
@@ -737,7 +762,7 @@ template <class LV> __device__ int wave_coarse_solve(const LV& L) {
         const T bb = dsum(b * b);
         int sweeps = 0;
         if (bb > T(0)) {
-            const T bn = sqrt(bb);
+            const RootBelow<T> converged(coarse_tol<T>() * sqrt(bb)); // ||r|| < tol ||b||
             const T bap = b / ap; // the update's first term does not change between sweeps
             auto sweep = [&](T v) {
 #pragma unroll
@@ -762,7 +787,7 @@ template <class LV> __device__ int wave_coarse_solve(const LV& L) {
                 nbs(xc, xe, xw, xs, xn);
                 const T r = blend(act, b - (ap * xc - (ae * xe + aw * xw + as * xs + an * xn)), T(0));
                 const T rr = dsum(r * r);
-                if (sqrt(rr) < coarse_tol<T>() * bn || sweeps >= 200)
+                if (converged(rr) || sweeps >= 200)
                     break;
                 xc = xnext;
                 ++sweeps;
@@ -797,11 +822,12 @@ template <int CTRL> __device__ __forceinline__ float dpp_val(float v) {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
 }
 
-template <int NY, class LV> __device__ int wave_coarse_solve16(const LV& L) {
+// NX: the row count when known at compile time (the sums then have no branches), else 0.
+template <int NY, int NX, class LV> __device__ int wave_coarse_solve16(const LV& L) {
     using T = typename LV::value_type;
     __shared__ int nsweeps;
     if (threadIdx.x < 64) {
-        const int lane = threadIdx.x, nx = L.nx;
+        const int lane = threadIdx.x, nx = NX > 0 ? NX : L.nx;
         const int nc = nx * NY;
         const bool act = lane < nc;
         const int I = act ? lane / NY : 0, J = act ? lane % NY : 0;
@@ -852,7 +878,7 @@ template <int NY, class LV> __device__ int wave_coarse_solve16(const LV& L) {
         const T bb = dsum(b * b);
         int sweeps = 0;
         if (bb > T(0)) {
-            const T bn = sqrt(bb);
+            const RootBelow<T> converged(coarse_tol<T>() * sqrt(bb)); // ||r|| < tol ||b||
             const T bap = b / ap; // the update's first term does not change between sweeps
             auto sweep = [&](T v) {
 #pragma unroll
@@ -877,7 +903,7 @@ template <int NY, class LV> __device__ int wave_coarse_solve16(const LV& L) {
                 nbs(xc, xe, xw, xs, xn);
                 const T r = blend(act, b - (ap * xc - (ae * xe + aw * xw + as * xs + an * xn)), T(0));
                 const T rr = dsum(r * r);
-                if (sqrt(rr) < coarse_tol<T>() * bn || sweeps >= 200)
+                if (converged(rr) || sweeps >= 200)
                     break;
                 xc = xnext;
                 ++sweeps;
@@ -972,8 +998,10 @@ template <class T> __device__ __forceinline__ void k_mg_tail(const Tail<T>& t, c
     int sweeps = 0;
     with(t.nl - 1, [&](const auto& Lc) {
         const int n = Lc.nx * Lc.ny;
-        if (n <= 16 && Lc.ny == 4) // the coarsest level of every power-of-two grid: 4 x 4
-            sweeps = wave_coarse_solve16<4>(Lc);
+        if (n == 16 && Lc.ny == 4) // the coarsest level of every power-of-two grid: 4 x 4
+            sweeps = wave_coarse_solve16<4, 4>(Lc);
+        else if (n <= 16 && Lc.ny == 4)
+            sweeps = wave_coarse_solve16<4, 0>(Lc);
         else if (n <= 64)
             sweeps = wave_coarse_solve(Lc);
         else

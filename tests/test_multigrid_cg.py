@@ -203,3 +203,42 @@ def test_cg_zero_rhs_gives_zero():
     o.solve_p_kind(sg.P_CG)
     assert o.cg_last_iterations() == 0
     assert np.abs(interior(p, o.get("pc"))).max() == 0.0
+
+
+@pytest.mark.parametrize("dt,g,rng_lo,rng_hi", [(np.float64, 1e-12, 1e-280, 1e280),
+                                                (np.float32, 1e-5, 1e-30, 1e30)])
+def test_coarse_stop_test_bounds(dt, g, rng_lo, rng_hi):
+    """The coarse solve's stop test sqrt(rr) < c decides by comparing rr with t (1 -/+ g),
+    t = c * c rounded, and takes the root only between those bounds (or when t leaves the normal
+    range). Wherever the comparison decides, it must agree with the correctly rounded root: rr
+    right at and around t, the bounds and their neighbours, and random values, over c spread
+    across the exponent range."""
+    rs = np.random.default_rng(3)
+    one = dt(1)
+    emax = 300 if dt is np.float64 else 36
+    cs = (10.0 ** rs.uniform(-emax / 2, emax / 2, 400)).astype(dt)
+    cs = np.concatenate([cs, np.array([1, 2, 0.5, 1e-10, 3.0], dt)])
+    decided = 0
+    for c in cs:
+        t = dt(c * c)
+        if not (rng_lo < t < rng_hi):
+            continue
+        lo, hi = dt(t * (one - dt(g))), dt(t * (one + dt(g)))
+        probes = [t, lo, hi]
+        for b in (t, lo, hi):
+            x = b
+            y = b
+            for _ in range(40):
+                x, y = np.nextafter(x, dt(np.inf)), np.nextafter(y, dt(0))
+                probes += [x, y]
+        probes += list((t * rs.uniform(0.5, 2.0, 50)).astype(dt))
+        probes += list((t * (1 + rs.uniform(-4, 4, 50) * g)).astype(dt))
+        for rr in np.array(probes, dt):
+            root = bool(np.sqrt(rr) < c)
+            if rr < lo:
+                assert root, (c, rr)
+                decided += 1
+            elif rr > hi:
+                assert not root, (c, rr)
+                decided += 1
+    assert decided > 10000
