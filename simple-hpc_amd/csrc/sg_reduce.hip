@@ -23,7 +23,7 @@ namespace {
 
 constexpr int NT = 256;  // threads per workgroup
 constexpr int RW = 4;    // rows per workgroup (one lane of wave 0 each)
-constexpr int CW = 1024; // columns per LDS chunk
+constexpr int CW = 256;  // columns per LDS chunk
 constexpr int PAD = 2;   // row padding (doubles) against bank conflicts
 constexpr int LOADS = RW * CW / NT;
 constexpr int TCH = 4096; // row sums per chunk of the total
@@ -74,14 +74,14 @@ struct SumArgs {
     double* slot;
 };
 
-// Row sums of rows [RW*blk, RW*blk + RW) of one sum.
+// Row sums of rows [RW*blk, RW*blk + RW) of one sum, CW columns at a time through two LDS
+// buffers: the next chunk's loads are in flight while the lanes of wave 0 add the current one.
 __device__ __forceinline__ void row_sums_block(const SumArgs& sa, int blk) {
-    __shared__ double tile[RW][CW + PAD];
+    __shared__ double tile[2][RW][CW + PAD];
     const int r0 = blk * RW, t = threadIdx.x;
-    double s = 0.0;
-    for (int c0 = 0; c0 < sa.w; c0 += CW) {
+    double v[LOADS];
+    auto load = [&](int c0) {
         const int cw = min(CW, sa.w - c0);
-        double v[LOADS];
 #pragma unroll
         for (int k = 0; k < LOADS; ++k) {
             const int e = k * NT + t, r = e / CW, c = e - r * CW;
@@ -89,14 +89,29 @@ __device__ __forceinline__ void row_sums_block(const SumArgs& sa, int blk) {
                        ? sa.a[static_cast<size_t>(sa.i0 + r0 + r) * sa.pitch + sa.j0 + c0 + c]
                        : 0.0;
         }
+    };
+    auto store = [&](int q) {
 #pragma unroll
         for (int k = 0; k < LOADS; ++k) {
             const int e = k * NT + t, r = e / CW, c = e - r * CW;
-            tile[r][c] = v[k];
+            tile[q][r][c] = v[k];
         }
-        __syncthreads();
+    };
+    double s = 0.0;
+    const int nch = (sa.w + CW - 1) / CW;
+    if (nch > 0) {
+        load(0);
+        store(0);
+    }
+    __syncthreads();
+    for (int ch = 0; ch < nch; ++ch) {
+        const int c0 = ch * CW;
+        if (ch + 1 < nch)
+            load(c0 + CW);
         if (t < RW && r0 + t < sa.nrow)
-            s = lds_chain(s, tile[t], cw);
+            s = lds_chain(s, tile[ch & 1][t], min(CW, sa.w - c0));
+        if (ch + 1 < nch)
+            store((ch + 1) & 1);
         __syncthreads();
     }
     if (t < RW && r0 + t < sa.nrow)
