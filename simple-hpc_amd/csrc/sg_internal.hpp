@@ -317,7 +317,7 @@ __device__ __forceinline__ void stop_rule(const StopArgs& a) {
 
 // Outer iterations per replay of Solver::graph_n, and dropping both captured graphs (their launch
 // shapes and pointers are baked in).
-constexpr int GRAPH_ITERS = 4;
+constexpr int GRAPH_ITERS = 8;
 void drop_graphs(Solver& s);
 
 // Enqueue every deferred sum: one launch for all their rows, one for their totals (with rule:
