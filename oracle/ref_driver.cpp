@@ -1,7 +1,8 @@
 // ref_driver.cpp — C entry points over the reference's own CPU code, compiled from
 // /root/reference/src by oracle/ref.mk into oracle/_ref/libsimple_ref.so. TEST INFRASTRUCTURE
 // ONLY: it lets tests check the oracle (oracle.cpp) against the reference itself on the same
-// inputs. It exists only in the build container (the reference sources never travel).
+// inputs. It is built only where the reference sources are (this build container); the built
+// library travels with the tree, and on the GPU box only oracle/_ref/adapter_check links it.
 #include "simple/config.hpp"
 #include "simple/momentum_u.hpp"
 #include "simple/momentum_v.hpp"

@@ -768,15 +768,44 @@ int sg_get_tuning(sg_solver* h, int key, long* value) {
     return guard([&] { *value = *knob(S(h), key); });
 }
 
-// Device time of one p' solve with the current knobs: the solve captured once and replayed.
-static double time_psolve(Solver& s, int reps) {
+// Graph executables of the autotuner's candidates, destroyed together once the stream is idle
+// (after the last timing, or when an error unwinds the autotuner).
+struct CandidateGraphs {
+    hipStream_t stream;
+    std::vector<hipGraphExec_t> g;
+    ~CandidateGraphs() {
+        (void)hipStreamSynchronize(stream);
+        for (hipGraphExec_t e : g)
+            (void)hipGraphExecDestroy(e);
+    }
+};
+
+// One p' solve with the current knobs captured into a new executable of `keep`. A failure while
+// capturing ends the capture, so the stream never stays in capture mode.
+static hipGraphExec_t capture_psolve(Solver& s, CandidateGraphs& keep) {
     hipGraph_t g = nullptr;
-    hipGraphExec_t ge = nullptr;
     SG_HIP(hipStreamBeginCapture(s.stream, hipStreamCaptureModeRelaxed));
-    launch_solve_p(s, s.never, false);
+    try {
+        launch_solve_p(s, s.never, false);
+    } catch (...) {
+        hipGraph_t tmp = nullptr;
+        (void)hipStreamEndCapture(s.stream, &tmp);
+        if (tmp)
+            (void)hipGraphDestroy(tmp);
+        throw;
+    }
     SG_HIP(hipStreamEndCapture(s.stream, &g));
-    SG_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+    hipGraphExec_t ge = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
+    SG_HIP(e);
+    keep.g.push_back(ge);
+    return ge;
+}
+
+// Device time of one p' solve with the current knobs: the solve captured once and replayed.
+static double time_psolve(Solver& s, int reps, CandidateGraphs& keep) {
+    const hipGraphExec_t ge = capture_psolve(s, keep);
     SG_HIP(hipGraphLaunch(ge, s.stream)); // warm
     SG_HIP(hipEventRecord(s.kt0, s.stream));
     for (int k = 0; k < reps; ++k)
@@ -785,9 +814,44 @@ static double time_psolve(Solver& s, int reps) {
     SG_HIP(hipStreamSynchronize(s.stream));
     float ms = 0.f;
     SG_HIP(hipEventElapsedTime(&ms, s.kt0, s.kt1));
-    (void)hipGraphExecDestroy(ge);
     return 1e-3 * ms / reps;
 }
+
+// b and p' (every p' solve rewrites both) kept in device copies while the autotuner times solves,
+// and the multigrid knobs as they were: the destructor puts all of them back, also when an error
+// unwinds the autotuner, and frees the copies.
+struct PsolveState {
+    Solver& s;
+    double* kb = nullptr;
+    double* kp = nullptr;
+    int threads, staged;
+    long mid, lds;
+    bool keep_knobs = false; // set once the autotuner has chosen its knobs
+    explicit PsolveState(Solver& sv)
+        : s(sv), threads(sv.tail_threads), staged(sv.mg_staged), mid(sv.mid_cells),
+          lds(sv.lds_cells) {
+        SG_HIP(hipMalloc(reinterpret_cast<void**>(&kb), sizeof(double) * s.n));
+        SG_HIP(hipMalloc(reinterpret_cast<void**>(&kp), sizeof(double) * s.n));
+        SG_HIP(hipMemcpyAsync(kb, s.F(SG_F_B), sizeof(double) * s.n, hipMemcpyDeviceToDevice,
+                              s.stream));
+        SG_HIP(hipMemcpyAsync(kp, s.F(SG_F_PC), sizeof(double) * s.n, hipMemcpyDeviceToDevice,
+                              s.stream));
+    }
+    ~PsolveState() {
+        (void)hipStreamSynchronize(s.stream);
+        (void)hipMemcpyAsync(s.F(SG_F_B), kb, sizeof(double) * s.n, hipMemcpyDeviceToDevice,
+                             s.stream);
+        (void)hipMemcpyAsync(s.F(SG_F_PC), kp, sizeof(double) * s.n, hipMemcpyDeviceToDevice,
+                             s.stream);
+        (void)hipStreamSynchronize(s.stream);
+        (void)hipFree(kb);
+        (void)hipFree(kp);
+        if (!keep_knobs) {
+            s.tail_threads = threads, s.mg_staged = staged;
+            s.mid_cells = mid, s.lds_cells = lds;
+        }
+    }
+};
 
 int sg_autotune(sg_solver* h, int reps, char* report, size_t len) {
     return guard([&] {
@@ -850,18 +914,17 @@ int sg_autotune(sg_solver* h, int reps, char* report, size_t len) {
             rep += line;
         }
         if (s.p.pressure_solver != SG_P_MG) {
+            drop_graphs(s); // the captured iterations hold the previous tile shapes
             if (report && len) {
                 std::strncpy(report, rep.c_str(), len - 1);
                 report[len - 1] = 0;
             }
             return;
         }
-        // Keep b and p' (the solve rewrites both) to restore them afterwards.
-        double *kb = nullptr, *kp = nullptr;
-        SG_HIP(hipMalloc(reinterpret_cast<void**>(&kb), sizeof(double) * s.n));
-        SG_HIP(hipMalloc(reinterpret_cast<void**>(&kp), sizeof(double) * s.n));
-        SG_HIP(hipMemcpyAsync(kb, s.F(SG_F_B), sizeof(double) * s.n, hipMemcpyDeviceToDevice, s.stream));
-        SG_HIP(hipMemcpyAsync(kp, s.F(SG_F_PC), sizeof(double) * s.n, hipMemcpyDeviceToDevice, s.stream));
+        // b and p' are restored (and the candidate graphs destroyed) when these go out of
+        // scope, on success or error; a p' solve does not change the solver's other arrays.
+        PsolveState keep_state(s);
+        CandidateGraphs graphs{s.stream, {}};
         double best = 1e30;
         int bt = s.tail_threads;
         long bm = s.mid_cells, bl = s.lds_cells;
@@ -871,7 +934,7 @@ int sg_autotune(sg_solver* h, int reps, char* report, size_t len) {
                     s.tail_threads = threads;
                     s.mid_cells = mid;
                     s.lds_cells = lds;
-                    const double t = time_psolve(s, reps);
+                    const double t = time_psolve(s, reps, graphs);
                     char line[160];
                     std::snprintf(line, sizeof line,
                                   "tail_threads %4d mid_cells %6ld lds_cells %5ld : %9.2f us\n",
@@ -894,7 +957,7 @@ int sg_autotune(sg_solver* h, int reps, char* report, size_t len) {
         int bs = s.mg_staged;
         for (int st : {0, 1, 2, 64}) {
             s.mg_staged = st;
-            const double t = time_psolve(s, reps);
+            const double t = time_psolve(s, reps, graphs);
             std::snprintf(line, sizeof line, "staged from level %2d : %9.2f us\n", st, 1e6 * t);
             rep += line;
             if (t < best) {
@@ -903,13 +966,9 @@ int sg_autotune(sg_solver* h, int reps, char* report, size_t len) {
             }
         }
         s.mg_staged = bs;
+        keep_state.keep_knobs = true;
         std::snprintf(line, sizeof line, "best: staged from level %d : %.2f us\n", bs, 1e6 * best);
         rep += line;
-        SG_HIP(hipMemcpyAsync(s.F(SG_F_B), kb, sizeof(double) * s.n, hipMemcpyDeviceToDevice, s.stream));
-        SG_HIP(hipMemcpyAsync(s.F(SG_F_PC), kp, sizeof(double) * s.n, hipMemcpyDeviceToDevice, s.stream));
-        sync(s);
-        (void)hipFree(kb);
-        (void)hipFree(kp);
         drop_graphs(s);
         if (report && len) {
             std::strncpy(report, rep.c_str(), len - 1);

@@ -21,24 +21,7 @@ namespace sg {
 
 void set_error(const std::string& msg);
 
-#define SG_HIP(call)                                                                           \
-    do {                                                                                       \
-        hipError_t e_ = (call);                                                                \
-        if (e_ != hipSuccess)                                                                  \
-            throw sg::HipError(std::string("GPU error: ") + #call + ": " +                     \
-                               hipGetErrorString(e_));                                         \
-    } while (0)
-
-struct HipError : std::exception {
-    std::string m;
-    explicit HipError(std::string s) : m(std::move(s)) {}
-    const char* what() const noexcept override { return m.c_str(); }
-};
-struct ArgError : std::exception {
-    std::string m;
-    explicit ArgError(std::string s) : m(std::move(s)) {}
-    const char* what() const noexcept override { return m.c_str(); }
-};
+// SG_HIP, HipError and ArgError are defined in sg_launch.hpp.
 
 // Nine coefficients in the order AE, AW, AN, AS, AEE, AWW, ANN, ASS, AP.
 struct C9 {

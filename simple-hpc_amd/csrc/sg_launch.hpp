@@ -19,9 +19,32 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <exception>
+#include <string>
 #include <vector>
 
 namespace sg {
+
+// A failed HIP call ("GPU error: <call>: <message>") and an invalid argument; the C ABI turns
+// them into SG_E_HIP and SG_E_ARG.
+struct HipError : std::exception {
+    std::string m;
+    explicit HipError(std::string s) : m(std::move(s)) {}
+    const char* what() const noexcept override { return m.c_str(); }
+};
+struct ArgError : std::exception {
+    std::string m;
+    explicit ArgError(std::string s) : m(std::move(s)) {}
+    const char* what() const noexcept override { return m.c_str(); }
+};
+
+#define SG_HIP(call)                                                                           \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            throw sg::HipError(std::string("GPU error: ") + #call + ": " +                     \
+                               hipGetErrorString(e_));                                         \
+    } while (0)
 
 struct Blk {
     int x, y, z;    // block index in the cavity's grid
@@ -145,19 +168,18 @@ template <class A, void (*B)(const A&, const Blk&), int NT> struct Op {
         g.total = start;
         void* dt = nullptr;
         void* dw = nullptr;
-        if (hipMalloc(&dt, sizeof(A) * n) != hipSuccess ||
-            hipMalloc(&dw, sizeof(Work) * n) != hipSuccess)
-            throw std::bad_alloc();
+        SG_HIP(hipMalloc(&dt, sizeof(A) * n));
         owned.push_back(dt);
+        SG_HIP(hipMalloc(&dw, sizeof(Work) * n));
         owned.push_back(dw);
-        (void)hipMemcpy(dt, tab.data(), sizeof(A) * n, hipMemcpyHostToDevice);
-        (void)hipMemcpy(dw, work.data(), sizeof(Work) * n, hipMemcpyHostToDevice);
+        SG_HIP(hipMemcpy(dt, tab.data(), sizeof(A) * n, hipMemcpyHostToDevice));
+        SG_HIP(hipMemcpy(dw, work.data(), sizeof(Work) * n, hipMemcpyHostToDevice));
         g.tab = dt;
         g.work = static_cast<Work*>(dw);
         const void* fn = reinterpret_cast<const void*>(&k_many<A, B, NT>);
         if (g.lds > 65536)
-            (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      static_cast<int>(g.lds));
+            SG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       static_cast<int>(g.lds)));
         int per_cu = 0;
         const int threads = static_cast<int>(g.block.x * g.block.y * g.block.z);
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, g.lds) !=

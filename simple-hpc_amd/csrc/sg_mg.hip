@@ -1178,22 +1178,24 @@ template <class T> void launch_builds(Solver& s, const Tail<T>& t, const int* st
     s.tag = 0;
 }
 
-// The staged tile kernels need more than the default 64 KB of dynamic LDS when launched alone:
-// allow them the device's whole per-workgroup LDS.
-template <class T> void staged_lds(const Solver& s) {
+// The staged tile kernels and the one-workgroup tail need more than the default 64 KB of dynamic
+// LDS: allow them the device's whole per-workgroup LDS. Set once per solver at set-up, never
+// while the stream is being captured.
+template <class T> void dyn_lds_limits(const Solver& s) {
     const int max = static_cast<int>(s.lds_budget + 1024);
-    SG_HIP(hipFuncSetAttribute(
+    const void* fns[] = {
         reinterpret_cast<const void*>(&k_one<DU<T>, k_mg_down_s<T, 32, BR<T>>, sg_n<T, 32, BR<T>>()>),
-        hipFuncAttributeMaxDynamicSharedMemorySize, max));
-    SG_HIP(hipFuncSetAttribute(
         reinterpret_cast<const void*>(&k_one<DU<T>, k_mg_up_s<T, 32, BR<T>>, sg_n<T, 32, BR<T>>()>),
-        hipFuncAttributeMaxDynamicSharedMemorySize, max));
-    SG_HIP(hipFuncSetAttribute(
         reinterpret_cast<const void*>(&k_one<DU<T>, k_mg_down_s<T, 16>, sg_n<T, 16>()>),
-        hipFuncAttributeMaxDynamicSharedMemorySize, max));
-    SG_HIP(hipFuncSetAttribute(
         reinterpret_cast<const void*>(&k_one<DU<T>, k_mg_up_s<T, 16>, sg_n<T, 16>()>),
-        hipFuncAttributeMaxDynamicSharedMemorySize, max));
+        reinterpret_cast<const void*>(&k_one<Tail<T>, k_mg_tail<T>, 1024>)};
+    for (const void* f : fns) {
+        // The limit is on static plus dynamic LDS: the tail has some static LDS of its own.
+        hipFuncAttributes fa{};
+        SG_HIP(hipFuncGetAttributes(&fa, f));
+        SG_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   max - static_cast<int>(fa.sharedSizeBytes)));
+    }
 }
 
 // LDS bytes of a staged TS x TS tile and its halo of h cells.
@@ -1212,8 +1214,6 @@ template <class T> void launch_vcycles(Solver& s, const Tail<T>& t, const int* s
     auto big = [](const Lv<T>& L) { return static_cast<long>(L.nx) * L.ny >= SG_BIG; };
     const size_t sdn = staged_bytes<T>(32, hdn), sup = staged_bytes<T>(32, hup);
     const bool staged = tiled && s.mg_staged < t.tm && std::max(sdn, sup) <= s.lds_budget + 1024;
-    if (staged)
-        staged_lds<T>(s);
     // b's mean removal carried by level 0's staged kernels (FP64 only: the mixed form's FP32 b
     // is converted with it).
     const bool bfused = sizeof(T) == 8 && s.b_mean_pending;
@@ -1305,11 +1305,6 @@ template <class T> Tail<T> make_tail(Solver& s, Lv<T> (*view)(const MgLevel&)) {
         e += 8 * (s.mg[l].nx + 2) * (s.mg[l].ny + 2);
     }
     t.lds_elems = e;
-    const size_t bytes = lds_bytes<T>(s, t.t0);
-    if (bytes > 65536)
-        SG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_one<Tail<T>, k_mg_tail<T>, 1024>),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   static_cast<int>(bytes)));
     return t;
 }
 
@@ -1378,6 +1373,10 @@ void mg_setup(Solver& s) {
         for (int k = 0; k < 9; ++k)
             *ptr[k] = blk + k * n + L.stride + 1;
     }
+    if (mixed)
+        dyn_lds_limits<float>(s);
+    else
+        dyn_lds_limits<double>(s);
 }
 
 // solve_pc_mg (src/multigrid.cpp:347-357): build, p' = 0, mean-free b, mg_cycles V-cycles,

@@ -3,7 +3,8 @@ own gradients are pinned by finite differences), on the same inputs and weights.
 
 The oracle in bf16 mode rounds where the GPU stores bfloat16, so the two differ only by float32
 summation order (and the rare bfloat16 rounding flip that order causes): outputs are compared to
-a few bfloat16 ulps, gradients by relative L2 error and cosine per parameter block. Also: training
+within about 10x the measured error per image size, gradients to 1e-6 relative L2 per parameter
+block (measured: at most 8.9e-8). Also: training
 lowers the loss; an all-zero network's guess is discarded and the run is bit-identical to the
 plain mixed-precision run (the fallback is exact).
 """
@@ -41,8 +42,12 @@ def test_param_count_matches_oracle():
     net.close()
 
 
-@pytest.mark.parametrize("n,h,w", [(1, 8, 8), (2, 16, 24), (1, 32, 32), (1, 256, 256)])
-def test_forward_matches_oracle(n, h, w):
+# Bounds about 10x the measured errors (7.7e-8, 4.7e-4, 1.9e-3, 3.9e-3): beyond the smallest
+# image a bfloat16 rounding flip in an intermediate activation (summation order) moves outputs by
+# a bfloat16 ulp of that activation.
+@pytest.mark.parametrize("n,h,w,bound", [(1, 8, 8, 1e-6), (2, 16, 24, 5e-3), (1, 32, 32, 2e-2),
+                                         (1, 256, 256, 4e-2)])
+def test_forward_matches_oracle(n, h, w, bound):
     x, _ = _inputs(1, n, h, w)
     th = _theta(1)
     net = sg.UNet(th)
@@ -50,7 +55,7 @@ def test_forward_matches_oracle(n, h, w):
     ref = U.forward(th, x, bf=True)
     err = np.abs(y - ref).max() / np.abs(ref).max()
     print(f"forward {n}x{h}x{w}: max rel err {err:.2e}", flush=True)
-    assert err < 3e-2
+    assert err < bound
     assert np.corrcoef(y.ravel(), ref.ravel())[0, 1] > 0.9995
     net.close()
 
@@ -63,7 +68,7 @@ def test_gradients_match_oracle(n, h, w):
     loss, g = net.loss_grad(x, t)
     lref, gref = U.loss_and_grad(th, x, t, bf=True)
     print(f"loss gpu {loss:.6f} cpu {lref:.6f}", flush=True)
-    assert abs(loss - lref) <= 2e-2 * lref
+    assert abs(loss - lref) <= 1e-5 * lref  # equal to 7 digits as measured
     o = 0
     for k, (ci, co) in enumerate(U.conv_specs()):
         ci_real = U.IN_CH if k == 0 else ci
@@ -72,10 +77,11 @@ def test_gradients_match_oracle(n, h, w):
             rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
             cos = np.dot(a, b) / max(np.linalg.norm(a) * np.linalg.norm(b), 1e-30)
             print(f"conv {k:2d} {name}: rel L2 {rel:.2e} cos {cos:.6f}", flush=True)
-            assert rel < 6e-2 and cos > 0.998, (k, name, rel, cos)
+            # measured: rel L2 at most 8.9e-8 in any block, cosine 1.000000
+            assert rel < 1e-6 and cos > 0.999999, (k, name, rel, cos)
             o += sz
     a, b = g[o:], gref[o:]
-    assert np.linalg.norm(a - b) / np.linalg.norm(b) < 3e-2
+    assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-6
     net.close()
 
 

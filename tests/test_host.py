@@ -189,3 +189,19 @@ def test_abi_errors_without_solver():
 
 def sg_E_ARG():
     return 1
+
+
+@pytest.mark.skipif(not os.path.isdir("/root/reference/include"),
+                    reason="the reference's headers are only in the build container")
+def test_integration_adapter_compiles_against_reference_headers(tmp_path):
+    """The reference-side adapter (the reference's HipSolver over this C ABI) compiles against
+    the reference's own headers, warning-free, and INTEGRATION.md shows that same file."""
+    src = os.path.join(ROOT, "simple-hpc_amd", "integration", "solver_sg.cpp")
+    r = subprocess.run(["g++", "-std=c++17", "-fopenmp", "-fsyntax-only", "-Wall", "-Wextra", "-Werror",
+                        "-I/root/reference/include", "-I" + os.path.join(ROOT, "include"), src],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    doc = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    body = open(src).read()
+    block = doc.split("## The adapter", 1)[1].split("```cpp\n", 1)[1].split("```", 1)[0]
+    assert block == body, "INTEGRATION.md's adapter listing differs from integration/solver_sg.cpp"
