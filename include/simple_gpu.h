@@ -245,6 +245,17 @@ int sg_unet_set_params(sg_unet* net, const float* theta, int n);
 int sg_unet_get_params(sg_unet* net, float* theta, int n);
 /* Forward pass: x [nimg][h][w][8] -> y [nimg][h][w]. */
 int sg_unet_forward(sg_unet* net, int nimg, int h, int w, const float* x, float* y);
+/* sg_unet_forward without allocation. No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc). x [nimg][h][w][8] and y [nimg][h][w] are DEVICE float arrays of the
+ * network's device (current when it was created, and current at the call); the pass is enqueued
+ * on `stream` (a hipStream_t; NULL: the null stream) and not waited for. Every intermediate
+ * activation lives in one fixed scratch area per GPU (7 MiB, held until the process ends), so
+ * nimg * h * w is limited to 16384 pixels (larger: SG_E_ARG). Calls that overlap in time on one
+ * GPU share that area and race: order them on one stream. The network's parameters must not
+ * change while a call runs. Same kernels, same results as sg_unet_forward. */
+int sg_unet_forward_noalloc(sg_unet* net, int nimg, int h, int w, const float* x, float* y,
+                            void* stream);
 /* Mean squared error against t [nimg][h][w] and its gradient (n params, may be NULL). */
 int sg_unet_loss_grad(sg_unet* net, int nimg, int h, int w, const float* x, const float* t,
                       float* loss, float* grad);

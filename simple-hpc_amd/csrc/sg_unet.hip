@@ -31,6 +31,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 
 namespace sg {
 
@@ -443,6 +444,7 @@ struct UNet {
     float *master = nullptr, *grad = nullptr, *m1 = nullptr, *m2 = nullptr;
     u16 *wf[NCONV] = {}, *wt[NCONV] = {};
     int step = 0;
+    int device = 0; // the device the network's arrays live on (current at creation)
     hipStream_t stream = nullptr;
     std::vector<void*> owned;
     // activation workspaces by (n, h, w)
@@ -489,6 +491,7 @@ struct UNet {
         hboff = o;
         o += 1;
         nparams = o;
+        SG_HIP(hipGetDevice(&device));
         SG_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         master = alloc<float>(owned, nparams);
         grad = alloc<float>(owned, nparams);
@@ -587,6 +590,38 @@ struct UNet {
         Ctx& ref = *c;
         ctx.emplace(key, std::move(c));
         return ref;
+    }
+
+    // The forward pass's activations of an (n, h, w) input, carved from the block at base
+    // (256-byte aligned pieces, nothing allocated). Returns the bytes used; with base null only
+    // counts them. The context's input x_in is set; its y is the caller's output.
+    static size_t carve_forward(Ctx& x, unsigned char* base, int n, int h, int w) {
+        size_t off = 0;
+        auto take = [&](u16*& q, size_t elems) {
+            q = base ? reinterpret_cast<u16*>(base + off) : nullptr;
+            off += (elems * sizeof(u16) + 255) / 256 * 256;
+        };
+        const size_t P0 = static_cast<size_t>(n) * h * w, P1 = P0 / 4, P2 = P0 / 16, P3 = P0 / 64;
+        x.n = n, x.h = h, x.w = w;
+        take(x.x_in, P0 * IN_PAD);
+        take(x.t0, P0 * 16);
+        take(x.cat0, P0 * 48);
+        take(x.d0a, P0 * 16);
+        take(x.d0, P0 * 16);
+        take(x.p0, P1 * 16);
+        take(x.t1, P1 * 32);
+        take(x.cat1, P1 * 96);
+        take(x.d1a, P1 * 32);
+        take(x.d1, P1 * 32);
+        take(x.p1, P2 * 32);
+        take(x.t2, P2 * 64);
+        take(x.cat2, P2 * 192);
+        take(x.d2a, P2 * 64);
+        take(x.d2, P2 * 64);
+        take(x.p2, P3 * 64);
+        take(x.t3, P3 * 128);
+        take(x.bt, P3 * 128);
+        return off;
     }
 
     // Pixel splits of the weight gradient at npix pixels (each split a multiple of 32 pixels).
@@ -763,6 +798,38 @@ struct UNet {
         SG_HIP(hipGetLastError());
     }
 };
+
+// ---- forward pass without allocation (sg_unet_forward_noalloc) ----
+
+// Every intermediate activation of one forward pass of at most NOALLOC_PIXELS pixels (nimg * h *
+// w), bfloat16: 374 bytes per pixel and 256-byte alignment of each of the 18 pieces, 6.2 MB at
+// the maximum. One copy per GPU, held until the process ends.
+constexpr long NOALLOC_PIXELS = 16384;
+constexpr size_t NOALLOC_BYTES = 7u << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch[MAX_DEVICES] = {}; // device address of `scratch` on each GPU
+
+// The device address of `scratch` on the current device, looked up once per device with that
+// device current (each GPU holds its own copy at its own address). The array's name is only a
+// host-side stand-in: kernels get this address.
+unsigned char* scratch_here() {
+    int dev = 0;
+    SG_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAX_DEVICES)
+        throw ArgError("device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* q = nullptr;
+        SG_HIP(hipGetSymbolAddress(&q, HIP_SYMBOL(scratch)));
+        g_scratch[dev] = static_cast<unsigned char*>(q);
+    }
+    return g_scratch[dev];
+}
+} // namespace
 
 // ---- warm start inside the p' solve ----
 
@@ -1080,7 +1147,14 @@ int sg_unet_create(sg_unet** out) {
     return uguard([&] {
         if (!out)
             throw ArgError("null pointer");
-        *out = reinterpret_cast<sg_unet*>(new UNet);
+        auto* net = new UNet;
+        try {
+            scratch_here(); // the no-allocation forward pass's scratch on this device
+        } catch (...) {
+            delete net;
+            throw;
+        }
+        *out = reinterpret_cast<sg_unet*>(net);
     });
 }
 
@@ -1124,6 +1198,37 @@ int sg_unet_forward(sg_unet* h, int nimg, int ht, int wd, const float* x, float*
         SG_HIP(hipMemcpyAsync(y, c.y, sizeof(float) * nimg * ht * wd, hipMemcpyDeviceToHost,
                               u.stream));
         SG_HIP(hipStreamSynchronize(u.stream));
+    });
+}
+
+int sg_unet_forward_noalloc(sg_unet* h, int nimg, int ht, int wd, const float* x, float* y,
+                            void* stream) {
+    return uguard([&] {
+        UNet& u = N(h);
+        if (!x || !y)
+            throw ArgError("null pointer");
+        if (ht % 8 || wd % 8 || ht < 8 || wd < 8 || nimg < 1)
+            throw ArgError("U-Net input sides must be positive multiples of 8");
+        const long npix = static_cast<long>(nimg) * ht * wd;
+        if (npix > NOALLOC_PIXELS)
+            throw ArgError("sg_unet_forward_noalloc: nimg * h * w = " + std::to_string(npix) +
+                           " pixels exceeds the scratch maximum of " +
+                           std::to_string(NOALLOC_PIXELS));
+        int dev = 0;
+        SG_HIP(hipGetDevice(&dev));
+        if (dev != u.device)
+            throw ArgError("sg_unet_forward_noalloc: the current device is not the network's");
+        UNet::Ctx c;
+        if (UNet::carve_forward(c, nullptr, nimg, ht, wd) > NOALLOC_BYTES)
+            throw ArgError("sg_unet_forward_noalloc: activations exceed the scratch area");
+        UNet::carve_forward(c, scratch_here(), nimg, ht, wd);
+        c.y = y;
+        const hipStream_t st = static_cast<hipStream_t>(stream);
+        const long nin = npix * IN_PAD;
+        hipLaunchKernelGGL(k_to_bf, dim3(g1(nin)), dim3(256), 0, st, x, c.x_in,
+                           static_cast<int>(nin));
+        SG_HIP(hipGetLastError());
+        u.forward(st, c);
     });
 }
 

@@ -177,6 +177,7 @@ def gpu():
         g.sg_unet_train_step.argtypes = [vp, C.c_int, C.c_int, C.c_int, fp, fp, C.c_float, fp]
         g.sg_unet_mfma_probe.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp]
         g.sg_attach_warm_start.argtypes = [vp, vp, C.c_int]
+        g.sg_unet_forward_noalloc.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp]
         g.sg_warm_features.argtypes = [vp, C.c_int, fp, dp]
         g.sg_warm_stats.argtypes = [vp, ip, ip]
         _gpu = g
@@ -353,6 +354,11 @@ class UNet:
         y = np.empty((n, h, w), np.float32)
         check(gpu().sg_unet_forward(self.h, n, h, w, _fptr(x), _fptr(y)))
         return y
+
+    def forward_noalloc(self, nimg, h, w, x_dev, y_dev, stream=None):
+        """sg_unet_forward_noalloc on device pointers (ints); enqueued on `stream`, not waited
+        for."""
+        check(gpu().sg_unet_forward_noalloc(self.h, nimg, h, w, x_dev, y_dev, stream))
 
     def loss_grad(self, x, t):
         import numpy as np
