@@ -28,6 +28,16 @@ int sh_params_from_config(const char* text, sg_params* out, char* output_dir, in
     }
 }
 
+// One array as a .npy file (the field_format = npy writer); 0 on success.
+int sh_write_npy(const char* path, const double* f, int rows, int cols) {
+    try {
+        sh::write_npy(path, f, rows, cols);
+        return 0;
+    } catch (const std::exception&) {
+        return 1;
+    }
+}
+
 // Writes the output files of a finished run (u, v, pf: nrows x ncols; log: n rows of
 // [iter, res_u, res_v, res_p]) to output_dir with the host's writers; 0 on success.
 int sh_write_outputs(const sg_params* p, const char* output_dir, const double* u, const double* v,

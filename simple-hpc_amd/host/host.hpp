@@ -28,6 +28,7 @@ struct Params {
     bool halo_overlap = true;
     std::string warm_start_weights;     // pressure_warm_start = learned: network weights file
     int warm_start_level = 2;           // multigrid level the network runs on (2^level blocks)
+    std::string field_format = "csv";   // field files: csv (the reference's), or npy (binary)
 };
 
 // Known keys only, typed parsing, validation and derived sizes (src/params.cpp:50-180), plus this
@@ -65,9 +66,13 @@ void check(int rc);
 RunResult run_simple_gpu(const Params& p, std::vector<double>& u, std::vector<double>& v,
                          std::vector<double>& pf);
 
-// Output files of src/io.cpp:43-151: u/v/p.csv, ux/uy_centre.csv (%.18e), alog.csv, timing.csv,
-// run_info.json.
+// Output files of src/io.cpp:43-151: u/v/p.csv, ux/uy_centre.csv (%.18e; .npy with
+// field_format = npy), alog.csv, timing.csv, run_info.json.
 void write_csv(const std::string& path, const double* f, int rows, int cols);
+// The same array as a NumPy .npy file (format 1.0, little-endian float64, C order): 8 bytes per
+// value instead of ~25 characters, and exact. The reference's run loader (postprocess/io.py:11-19)
+// reads NAME.npy when NAME.csv is absent.
+void write_npy(const std::string& path, const double* f, int rows, int cols);
 void write_outputs(const Params& p, const std::vector<double>& u, const std::vector<double>& v,
                    const std::vector<double>& pf, const RunResult& r, const std::string& cfg_path);
 

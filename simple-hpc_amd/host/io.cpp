@@ -49,6 +49,29 @@ void write_csv(const std::string& path, const double* f, int rows, int cols) {
     std::fclose(fp);
 }
 
+void write_npy(const std::string& path, const double* f, int rows, int cols) {
+    std::FILE* fp = std::fopen(path.c_str(), "wb");
+    if (!fp)
+        throw std::runtime_error("cannot open for writing: " + path);
+    // Magic, version 1.0, header length, then the header dict padded with spaces to a multiple of
+    // 64 bytes in all and ended by a newline.
+    std::string hdr = "{'descr': '<f8', 'fortran_order': False, 'shape': (" +
+                      std::to_string(rows) + ", " + std::to_string(cols) + "), }";
+    const size_t total = (10 + hdr.size() + 1 + 63) / 64 * 64;
+    hdr.append(total - 10 - hdr.size() - 1, ' ');
+    hdr += '\n';
+    const unsigned char pre[10] = {0x93, 'N', 'U', 'M', 'P', 'Y', 1, 0,
+                                   static_cast<unsigned char>(hdr.size() & 0xff),
+                                   static_cast<unsigned char>(hdr.size() >> 8)};
+    const size_t n = static_cast<size_t>(rows) * cols;
+    const bool ok = std::fwrite(pre, 1, 10, fp) == 10 &&
+                    std::fwrite(hdr.data(), 1, hdr.size(), fp) == hdr.size() &&
+                    std::fwrite(f, sizeof(double), n, fp) == n; // x86-64: little-endian
+    std::fclose(fp);
+    if (!ok)
+        throw std::runtime_error("cannot write " + path);
+}
+
 void write_outputs(const Params& P, const std::vector<double>& u, const std::vector<double>& v,
                    const std::vector<double>& pf, const RunResult& r, const std::string& cfg_path) {
     namespace fs = std::filesystem;
@@ -59,9 +82,16 @@ void write_outputs(const Params& P, const std::vector<double>& u, const std::vec
     fs::create_directories(d);
     if (p.write_fields) {
         const int nr = p.nrows, nc = p.ncols, H = p.H;
-        write_csv((d / "u.csv").string(), u.data(), nr, nc);
-        write_csv((d / "v.csv").string(), v.data(), nr, nc);
-        write_csv((d / "p.csv").string(), pf.data(), nr, nc);
+        const bool npy = P.field_format == "npy";
+        auto field = [&](const char* name, const double* a, int rows, int cols) {
+            if (npy)
+                write_npy((d / (std::string(name) + ".npy")).string(), a, rows, cols);
+            else
+                write_csv((d / (std::string(name) + ".csv")).string(), a, rows, cols);
+        };
+        field("u", u.data(), nr, nc);
+        field("v", v.data(), nr, nc);
+        field("p", pf.data(), nr, nc);
         // Cell-centred velocities (src/io.cpp:22-31, docs/SPEC.md section 6).
         std::vector<double> ux(static_cast<size_t>(p.NX) * p.NY), uy(ux.size());
         for (int rr = 0; rr < p.NX; ++rr)
@@ -72,8 +102,8 @@ void write_outputs(const Params& P, const std::vector<double>& u, const std::vec
                 uy[q] = (v[static_cast<size_t>(H + rr) * nc + H + 1 + c] +
                          v[static_cast<size_t>(H + rr + 1) * nc + H + 1 + c]) / 2;
             }
-        write_csv((d / "ux_centre.csv").string(), ux.data(), p.NX, p.NY);
-        write_csv((d / "uy_centre.csv").string(), uy.data(), p.NX, p.NY);
+        field("ux_centre", ux.data(), p.NX, p.NY);
+        field("uy_centre", uy.data(), p.NX, p.NY);
     }
     {
         File al((d / "alog.csv").string(), "alog.csv");

@@ -62,7 +62,7 @@ const std::set<std::string>& known_keys() {
         "mpi_grid", "halo_overlap", "write_fields",
         // additions
         "mg_precision", "mg_coarsest", "mg_dc_maxit", "pressure_warm_start",
-        "warm_start_weights", "warm_start_level", "check_every"};
+        "warm_start_weights", "warm_start_level", "check_every", "field_format"};
     return k;
 }
 } // namespace
@@ -157,6 +157,11 @@ Params make_params(const ConfigMap& cfg) {
         else if (k == "warm_start_weights") P.warm_start_weights = v;
         else if (k == "warm_start_level") P.warm_start_level = as_int(k, v);
         else if (k == "check_every") p.check_every = as_int(k, v);
+        else if (k == "field_format") {
+            if (v != "csv" && v != "npy")
+                throw std::runtime_error("field_format must be csv or npy: " + v);
+            P.field_format = v;
+        }
     }
     check(sg_params_finalize(&p));
     return P;
@@ -187,6 +192,7 @@ void print_params(std::ostream& os, const Params& P) {
        << "\nmg_pre = " << p.mg_pre << "\nmg_post = " << p.mg_post << "\nmg_omega = " << p.mg_omega
        << "\nhalo_overlap = " << (P.halo_overlap ? "True" : "False")
        << "\nwrite_fields = " << (p.write_fields ? "True" : "False")
+       << "\nfield_format = " << P.field_format
        << "\nmg_precision = " << (p.mg_precision == SG_MG_MIXED ? "mixed" : "fp64")
        << "\nmg_coarsest = " << p.mg_coarsest
        << "\npressure_warm_start = " << (p.pressure_warm_start == SG_WARM_LEARNED ? "learned" : "none")

@@ -1,0 +1,117 @@
+"""Output formats and the Ghia validation, on the CPU.
+
+ - The .npy field writer (field_format = npy) round-trips exactly through numpy's own loader
+   (no pickled objects), header aligned as the format specifies.
+ - The Ghia data as the reference holds them (postprocess/ghia.py:74-88 self-checks): 17 points a
+   centreline, wall values, the Re 1000 minimum.
+ - Centreline extraction on a field whose answer is known (postprocess/fields.py:37-47) and the
+   error measure on the Ghia data itself (zero) and on a constant offset (the offset).
+ - The whole validation pinned against the reference's published numbers: the oracle (bit-exact
+   with the reference) runs the 80 x 80, Re 1000 cavity (multicolour + red-black, tolerance 1e-5)
+   to convergence in the reference's 3718 iterations, and its centrelines give the published
+   rms_u 0.01698, max_u 0.03268, rms_v 0.00825 and max_v 0.01386 (docs/RESULTS.md:10, 329).
+"""
+import ctypes as C
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+import pyoracle as po
+import simple_gpu as sg
+import validate_ghia as vg
+
+
+def _write_npy(path, a):
+    h = sg.host()
+    h.sh_write_npy.argtypes = [C.c_char_p, C.POINTER(C.c_double), C.c_int, C.c_int]
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    assert h.sh_write_npy(str(path).encode(), a.ctypes.data_as(C.POINTER(C.c_double)),
+                          a.shape[0], a.shape[1]) == 0
+
+
+def test_npy_writer_round_trip(tmp_path):
+    rng = np.random.default_rng(0)
+    for shape in [(1, 1), (37, 45), (1029, 17)]:
+        a = rng.standard_normal(shape) * 10.0 ** rng.integers(-300, 300, shape)
+        a.flat[0] = -0.0
+        f = tmp_path / "a.npy"
+        _write_npy(f, a)
+        b = np.load(f, allow_pickle=False)
+        assert b.dtype == np.float64 and b.shape == shape and b.flags["C_CONTIGUOUS"]
+        assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+        raw = f.read_bytes()
+        assert raw[:8] == b"\x93NUMPY\x01\x00"
+        hlen = raw[8] | (raw[9] << 8)
+        assert (10 + hlen) % 64 == 0 and raw[10 + hlen - 1:10 + hlen] == b"\n"
+        assert len(raw) == 10 + hlen + 8 * a.size
+
+
+def test_ghia_data():
+    for re in (400, 1000):
+        y, u = vg.ghia_u(re)
+        x, v = vg.ghia_v(re)
+        assert len(y) == len(u) == len(x) == len(v) == 17
+        assert y[0] == 0.0 and u[0] == 0.0 and y[-1] == 1.0 and u[-1] == 1.0
+        assert x[0] == 0.0 and v[0] == 0.0 and x[-1] == 1.0 and v[-1] == 0.0
+        assert np.all(np.diff(y) > 0) and np.all(np.diff(x) > 0)
+    assert vg.ghia_u(1000)[1].min() == -0.38289
+    assert vg.suspect(400, "v") == [0.90625] and vg.suspect(1000, "v") == []
+    with pytest.raises(ValueError):
+        vg.ghia_u(3200)
+
+
+def test_centrelines_and_errors():
+    ux = np.tile(np.arange(4.0), (4, 1))  # value = column index
+    c = vg.centrelines(ux, ux.T.copy())
+    assert np.allclose(c["u"][1:-1], 1.5) and np.allclose(c["v"][1:-1], 1.5)
+    assert c["u"][0] == 0 and c["u"][-1] == 1 and c["v"][0] == 0 and c["v"][-1] == 0
+    assert np.allclose(c["y"][1:-1], [0.125, 0.375, 0.625, 0.875])
+    ux3 = np.arange(9.0).reshape(3, 3)  # odd count: the middle column, flipped (lid row first)
+    assert np.array_equal(vg.centrelines(ux3, ux3)["u"][1:-1], [7.0, 4.0, 1.0])
+    gy, gu = vg.ghia_u(1000)
+    assert vg.errors(gy, gu, gy, gu) == (0.0, 0.0)
+    rms, mx = vg.errors(gy, gu + 0.01, gy, gu)
+    assert abs(rms - 0.01) < 1e-15 and abs(mx - 0.01) < 1e-15
+    gx, gv = vg.ghia_v(400)
+    bumped = gv.copy()
+    bumped[np.argmin(abs(gx - 0.90625))] += 1.0
+    assert vg.errors(gx, bumped, gx, gv, vg.suspect(400, "v")) == (0.0, 0.0)
+
+
+def _fmt5(xs):
+    return [f"{x:.5f}" for x in xs]
+
+
+def _centred(p, u, v):
+    H = p.H
+    ux = (u[H:H + p.NX, H:H + p.NY] + u[H:H + p.NX, H + 1:H + 1 + p.NY]) / 2
+    uy = (v[H:H + p.NX, H + 1:H + 1 + p.NY] + v[H + 1:H + 1 + p.NX, H + 1:H + 1 + p.NY]) / 2
+    return ux, uy
+
+
+def test_validation_matches_published_numbers(tmp_path):
+    cfg = open(os.path.join(ROOT, "tests", "golden", "ghia", "re1000_80.cfg")).read()
+    cfg = cfg.replace("backend = gpu", "backend = serial")
+    p, _ = sg.params_from_config(cfg)
+    po.set_threads(max(1, min(8, os.cpu_count() or 1)))
+    cpu = po.Oracle(p)
+    n, log, conv = cpu.run()
+    assert conv and n == 3718  # docs/RESULTS.md:79, 329
+    ux, uy = _centred(p, cpu.get("u"), cpu.get("v"))
+    # through the files the CLI writes (one as .npy) and the run loader
+    d = tmp_path / "re1000_80"
+    d.mkdir()
+    np.savetxt(d / "ux_centre.csv", ux, delimiter=",", fmt="%.18e")
+    _write_npy(d / "uy_centre.npy", uy)
+    (d / "run_info.json").write_text(json.dumps({"Re": 1000, "N_cells_x": 80, "iterations": n,
+                                                 "converged": True}))
+    row = vg.evaluate(str(d))
+    print(row, flush=True)
+    # rms_u, max_u, rms_v, max_v as published (docs/RESULTS.md:10, 329)
+    assert _fmt5(row[5:]) == ["0.01698", "0.03268", "0.00825", "0.01386"]
+    assert vg.main([str(d), "--check", "--max-rms-u", "0.025", "--max-rms-v", "0.015"]) == 0
+    assert vg.main([str(d), "--check", "--max-rms-u", "0.015", "--max-rms-v", "0.015"]) == 1
