@@ -47,6 +47,7 @@ enum { SG_P_GS_LEX = 0, SG_P_JACOBI = 1, SG_P_GS_REDBLACK = 2, SG_P_CG = 3, SG_P
 /* Additions with no reference counterpart. */
 enum { SG_MG_FP64 = 0, SG_MG_MIXED = 1 };      /* mg_precision */
 enum { SG_WARM_NONE = 0, SG_WARM_LEARNED = 1 }; /* pressure_warm_start */
+enum { SG_MOMP_FP64 = 0, SG_MOMP_MIXED = 1 };  /* momentum_precision */
 
 /* POD mirror of Params (include/simple/params.hpp:15-58): every numeric key of the config
  * file, the derived sizes, and the additions of this backend. output_dir and the MPI fields
@@ -80,6 +81,9 @@ typedef struct sg_params {
     int mg_dc_maxit;            /* mixed: FP64 defect-correction steps per solve */
     int pressure_warm_start;    /* SG_WARM_* */
     int check_every;            /* device-side stop test: host looks every this many iterations */
+    int momentum_precision;     /* SG_MOMP_FP64 (reference arithmetic) or SG_MOMP_MIXED: outer
+                                 * iterations sweep an FP32 correction of u and v inside FP64
+                                 * defect correction (gs_multicolour, at least one sweep) */
 } sg_params;
 
 /* Array identifiers. Host layout of every id is nrows x ncols doubles. */
@@ -167,6 +171,8 @@ enum {
     SG_K_SWEEP_U_FIRST, /* the first u sweep of an outer iteration (|R| cells fused in) */
     SG_K_ASSEMBLE_V,
     SG_K_SWEEP_V_FIRST, /* the first v sweep of an outer iteration */
+    SG_K_SWEEP_U_DC,   /* momentum_precision = mixed: one FP32 u correction sweep (every colour) */
+    SG_K_ASSEMBLE_U_DC, /* momentum_precision = mixed: the u assembly with the FP64 residual */
     SG_K_COUNT
 };
 int sg_kernel_probe(sg_solver* s, int kernel_id, int reps, double* mean_seconds,
@@ -191,6 +197,9 @@ enum {
     SG_TUNE_MG_STAGED = 4, /* first multigrid level whose tiles also stage the operator in LDS */
     SG_TUNE_MOM_TILE_FIRST = 5, /* tile of the first sweep (fused residual) of each component;
                                  * unset: SG_TUNE_MOM_TILE */
+    SG_TUNE_DC_TILE = 6, /* momentum_precision = mixed: FP32 correction sweep tile (rows x
+                          * columns / threads): 0 32x64/256, 1 32x64/512, 2 64x64/512,
+                          * 3 32x128/512, 4 48x64/512, 5 16x128/256, 6 24x96/512, 7 40x56/512 */
     SG_TUNE_COUNT
 };
 int sg_set_tuning(sg_solver* s, int key, long value);

@@ -295,6 +295,64 @@ void mom_multicolour(or_state* s, const Mom& m, int nsweeps) {
         }
 }
 
+// momentum_precision = mixed: FP32 sweeps of the correction d inside FP64 defect correction, as
+// the GPU runs them (no reference counterpart). r0 = nb(x0) + pterm - AP x0 in FP64 (mom_residual's
+// expression), then per sweep and colour d = (a / AP) (r0 + nb(d)) + (1 - a) d in float with the
+// coefficients, a / AP and r0 rounded to float once and d = 0 outside the sweep range, then
+// x = x0 + d. Same operations in the same order as the GPU kernel, so the same bits
+// (-ffp-contract=off on both sides).
+void mom_dc(or_state* s, const Mom& m, int nsweeps) {
+    if (nsweeps <= 0)
+        return;
+    const int nc = s->nc, ncol = ncolours(s->p);
+    const double* c[9];
+    for (int k = 0; k < 9; ++k)
+        c[k] = s->a(m.id0 + k);
+    double* x = s->a(m.field);
+    const double* pf = s->a(SG_F_P);
+    const size_t n = s->f[m.field].size();
+    std::vector<float> d(n, 0.0f), r0(n, 0.0f), w(n, 0.0f);
+    std::vector<float> cf[8];
+    for (auto& v : cf)
+        v.assign(n, 0.0f);
+#pragma omp parallel for schedule(static)
+    for (int i = m.i0; i <= m.i1; ++i)
+        for (int j = m.j0; j <= m.j1; ++j) {
+            const size_t q = static_cast<size_t>(i) * nc + j;
+            r0[q] = static_cast<float>(nb9(c, x, q, nc) + (pf[q] - pf[q + m.poff]) * m.h -
+                                       c[8][q] * x[q]);
+            w[q] = static_cast<float>(m.alpha / c[8][q]);
+            for (int k = 0; k < 8; ++k)
+                cf[k][q] = static_cast<float>(c[k][q]);
+        }
+    const float oma = static_cast<float>(1 - m.alpha);
+    const long L = nc;
+    for (int sw = 0; sw < nsweeps; ++sw)
+        for (int col = 0; col < ncol; ++col) {
+#pragma omp parallel for schedule(static)
+            for (int i = m.i0; i <= m.i1; ++i) {
+                int j = m.j0;
+                while (colour_of(ncol, i, j) != col)
+                    ++j;
+                for (; j <= m.j1; j += ncol) {
+                    const size_t q = static_cast<size_t>(i) * nc + j;
+                    float nb = cf[0][q] * d[q + 1] + cf[1][q] * d[q - 1] + cf[2][q] * d[q - L] +
+                               cf[3][q] * d[q + L];
+                    if (ncol != 2)
+                        nb += cf[4][q] * d[q + 2] + cf[5][q] * d[q - 2] + cf[6][q] * d[q - 2 * L] +
+                              cf[7][q] * d[q + 2 * L];
+                    d[q] = w[q] * (r0[q] + nb) + oma * d[q];
+                }
+            }
+        }
+#pragma omp parallel for schedule(static)
+    for (int i = m.i0; i <= m.i1; ++i)
+        for (int j = m.j0; j <= m.j1; ++j) {
+            const size_t q = static_cast<size_t>(i) * nc + j;
+            x[q] = x[q] + static_cast<double>(d[q]);
+        }
+}
+
 void mom_sweep(or_state* s, const Mom& m, int kind, int nsweeps) {
     if (kind == SG_MOM_JACOBI)
         mom_jacobi(s, m, nsweeps);
@@ -781,6 +839,7 @@ void or_bc_u(or_state* s) {
 
 double or_residual_u(or_state* s) { return mom_residual(s, mom_u(s)); }
 void or_sweep_u(or_state* s, int n) { mom_sweep(s, mom_u(s), s->p.momentum_solver, n); }
+void or_sweep_u_dc(or_state* s, int n) { mom_dc(s, mom_u(s), n); }
 void or_sweep_u_kind(or_state* s, int kind, int n) { mom_sweep(s, mom_u(s), kind, n); }
 
 // assemble_v, src/momentum_v.cpp:9-22: rows [H, NX+H+1], columns [H, NY+H+1].
@@ -813,6 +872,7 @@ void or_bc_v(or_state* s) {
 
 double or_residual_v(or_state* s) { return mom_residual(s, mom_v(s)); }
 void or_sweep_v(or_state* s, int n) { mom_sweep(s, mom_v(s), s->p.momentum_solver, n); }
+void or_sweep_v_dc(or_state* s, int n) { mom_dc(s, mom_v(s), n); }
 void or_sweep_v_kind(or_state* s, int kind, int n) { mom_sweep(s, mom_v(s), kind, n); }
 
 // assemble_p, src/pressure.cpp:14-54: links, zeroed wall links, APP, mass source b, res_p.
@@ -888,14 +948,21 @@ void or_correct(or_state* s) {
 
 // outer_iteration, src/solver.cpp:27-93.
 void or_outer_iteration(or_state* s, double res[3]) {
+    const bool dc = s->p.momentum_precision == SG_MOMP_MIXED;
     or_assemble_u(s);
     or_bc_u(s);
     res[0] = or_residual_u(s);
-    or_sweep_u(s, s->p.maxit_inner_u);
+    if (dc)
+        or_sweep_u_dc(s, s->p.maxit_inner_u);
+    else
+        or_sweep_u(s, s->p.maxit_inner_u);
     or_assemble_v(s);
     or_bc_v(s);
     res[1] = or_residual_v(s);
-    or_sweep_v(s, s->p.maxit_inner_v);
+    if (dc)
+        or_sweep_v_dc(s, s->p.maxit_inner_v);
+    else
+        or_sweep_v(s, s->p.maxit_inner_v);
     res[2] = or_assemble_p(s);
     or_solve_p(s);
     or_correct(s);

@@ -58,6 +58,11 @@ double or_run_fixed(or_state* s, int n, double* last_res);
 /* Individual inner solvers (for brute-force parity tests on random systems). */
 void or_sweep_u_kind(or_state* s, int kind, int nsweeps); /* SG_MOM_* */
 void or_sweep_v_kind(or_state* s, int kind, int nsweeps);
+/* momentum_precision = mixed: nsweeps FP32 multicolour sweeps of the correction inside FP64
+ * defect correction (the GPU's form, no reference counterpart); or_outer_iteration uses them when
+ * the params say so. */
+void or_sweep_u_dc(or_state* s, int nsweeps);
+void or_sweep_v_dc(or_state* s, int nsweeps);
 void or_solve_p_kind(or_state* s, int kind);              /* SG_P_* */
 int or_cg_last_iterations(or_state* s);
 double or_cg_last_rel(or_state* s);

@@ -40,6 +40,9 @@ def lib():
         for n in ["or_sweep_u", "or_sweep_v"]:
             getattr(L, n).argtypes = [vp, C.c_int]
             getattr(L, n).restype = None
+        for n in ["or_sweep_u_dc", "or_sweep_v_dc"]:
+            getattr(L, n).argtypes = [vp, C.c_int]
+            getattr(L, n).restype = None
         for n in ["or_sweep_u_kind", "or_sweep_v_kind"]:
             getattr(L, n).argtypes = [vp, C.c_int, C.c_int]
             getattr(L, n).restype = None
@@ -139,6 +142,8 @@ class Oracle:
     def assemble_p(self): return self.L.or_assemble_p(self.h)
     def solve_p(self): self.L.or_solve_p(self.h)
     def correct(self): self.L.or_correct(self.h)
+    def sweep_u_dc(self, n): self.L.or_sweep_u_dc(self.h, n)
+    def sweep_v_dc(self, n): self.L.or_sweep_v_dc(self.h, n)
     def sweep_u_kind(self, kind, n): self.L.or_sweep_u_kind(self.h, kind, n)
     def sweep_v_kind(self, kind, n): self.L.or_sweep_v_kind(self.h, kind, n)
     def solve_p_kind(self, kind): self.L.or_solve_p_kind(self.h, kind)

@@ -55,6 +55,19 @@ template <class T> T* dalloc(Solver& s, size_t n) {
 
 } // namespace
 
+// The arrays' state after replayed outer iterations (their assemblies, solves and corrections
+// ran on the device): which form holds the momentum coefficients, the pressure terms stale, the
+// lean p' operator, p' in scratch and b with its mean (see Solver).
+void mark_iterated(Solver& s) {
+    const bool dc = mom_dc_ok(s.p);
+    s.face[0] = !dc && face_ok(s, true), s.face[1] = !dc && face_ok(s, false);
+    s.dc_stale[0] = s.dc_stale[1] = dc;
+    s.pt_stale[0] = s.pt_stale[1] = true;
+    s.p_lean = mg_lean_ok(s);
+    s.pc_in_scratch = s.p.pressure_solver == SG_P_MG;
+    s.b_raw = mg_fuse_b_ok(s);
+}
+
 void drop_graphs(Solver& s) {
     for (hipGraphExec_t* g : {&s.graph, &s.graph_n})
         if (*g) {
@@ -86,26 +99,57 @@ void enqueue_iteration(Solver& s, const int* stop, bool eager, hipEvent_t* ev) {
             SG_HIP(hipEventRecord(ev[k], s.stream));
     };
     mark(0);
-    // Without per-phase events, u's ghost rows are set by the assembly launch itself.
-    const bool fuse_bc = defer && s.pku.rows >= 2;
-    launch_assemble(s, true, stop, fuse_bc, face_ok(s, true));
-    mark(1);
-    if (!fuse_bc)
-        launch_bc(s, true, stop);
-    mark(2);
-    mark(3);
-    launch_mom_sweeps(s, true, p.maxit_inner_u, stop, s.scal + S_RES_U, defer);
-    mark(4);
-    launch_assemble(s, false, stop, false, face_ok(s, false));
-    mark(5);
-    // Without per-phase events, v's ghost columns are set by its first sweep.
-    const bool bcv = defer && bcv_in_sweep(s);
-    if (!bcv)
+    if (mom_dc_ok(p)) {
+        // FP32 sweeps of the corrections inside FP64 defect correction: each assembly also
+        // evaluates the FP64 residual (its |r| cells give res_u / res_v) and the FP32 pack; u's
+        // ghost rows are set by its assembly launch, v's by bc_v after v's assembly (which reads
+        // the old ones) and before anything reads them.
+        auto res_sum = [&](bool is_u) {
+            const int nrow = is_u ? p.NX : p.NX - 1, w = is_u ? p.NY - 1 : p.NY;
+            double* slot = s.scal + (is_u ? S_RES_U : S_RES_V);
+            if (defer)
+                defer_sum(s, is_u ? 1 : 2, s.rcell[is_u ? 0 : 1], nrow, w, slot);
+            else
+                launch_ordered_sum(s, s.rcell[is_u ? 0 : 1], w, 0, nrow - 1, 0, w - 1, slot,
+                                   stop);
+        };
+        launch_asm_dc(s, true, stop, true);
+        mark(1);
+        mark(2);
+        res_sum(true);
+        mark(3);
+        launch_dc_sweeps(s, true, p.maxit_inner_u, stop);
+        mark(4);
+        launch_asm_dc(s, false, stop, false);
+        mark(5);
         launch_bc(s, false, stop);
-    mark(6);
-    mark(7);
-    launch_mom_sweeps(s, false, p.maxit_inner_v, stop, s.scal + S_RES_V, defer, bcv);
-    mark(8);
+        mark(6);
+        res_sum(false);
+        mark(7);
+        launch_dc_sweeps(s, false, p.maxit_inner_v, stop);
+        mark(8);
+    } else {
+        // Without per-phase events, u's ghost rows are set by the assembly launch itself.
+        const bool fuse_bc = defer && s.pku.rows >= 2;
+        launch_assemble(s, true, stop, fuse_bc, face_ok(s, true));
+        mark(1);
+        if (!fuse_bc)
+            launch_bc(s, true, stop);
+        mark(2);
+        mark(3);
+        launch_mom_sweeps(s, true, p.maxit_inner_u, stop, s.scal + S_RES_U, defer);
+        mark(4);
+        launch_assemble(s, false, stop, false, face_ok(s, false));
+        mark(5);
+        // Without per-phase events, v's ghost columns are set by its first sweep.
+        const bool bcv = defer && bcv_in_sweep(s);
+        if (!bcv)
+            launch_bc(s, false, stop);
+        mark(6);
+        mark(7);
+        launch_mom_sweeps(s, false, p.maxit_inner_v, stop, s.scal + S_RES_V, defer, bcv);
+        mark(8);
+    }
     launch_assemble_p(s, s.scal + S_RES_P, stop, defer, mg_lean_ok(s));
     mark(9);
     // The residual sums, all deferred by now, may ride along with the p' solve's first mean
@@ -144,14 +188,15 @@ void capture_graph(Solver& s, int reps, hipGraphExec_t* out) {
         return;
     struct Restore {
         Solver& s;
-        bool f0, f1, p0, p1, lean, pcs, braw;
+        bool f0, f1, p0, p1, d0, d1, lean, pcs, braw;
         ~Restore() {
             s.face[0] = f0, s.face[1] = f1, s.pt_stale[0] = p0, s.pt_stale[1] = p1;
+            s.dc_stale[0] = d0, s.dc_stale[1] = d1;
             s.p_lean = lean, s.pc_in_scratch = pcs, s.b_raw = braw;
             s.fuse_pc_mean = s.pc_mean_pending = s.fuse_b_mean = s.b_mean_pending = false;
         }
-    } restore{s, s.face[0], s.face[1], s.pt_stale[0], s.pt_stale[1], s.p_lean, s.pc_in_scratch,
-              s.b_raw};
+    } restore{s,           s.face[0],   s.face[1], s.pt_stale[0],    s.pt_stale[1],
+              s.dc_stale[0], s.dc_stale[1], s.p_lean, s.pc_in_scratch, s.b_raw};
     hipGraph_t g = nullptr;
     SG_HIP(hipStreamBeginCapture(s.stream, hipStreamCaptureModeRelaxed));
     try {
@@ -243,6 +288,13 @@ void validate(const sg_params& p) {
         throw ArgError("mg_dc_maxit must be >= 1");
     if (p.check_every < 1)
         throw ArgError("check_every must be >= 1");
+    if (p.momentum_precision != SG_MOMP_FP64 && p.momentum_precision != SG_MOMP_MIXED)
+        throw ArgError("momentum_precision must be fp64 or mixed");
+    if (p.momentum_precision == SG_MOMP_MIXED &&
+        (p.momentum_solver != SG_MOM_GS_MULTICOLOUR || p.maxit_inner_u < 1 ||
+         p.maxit_inner_v < 1 || p.NX < 2 || p.NY < 2))
+        throw ArgError("momentum_precision = mixed needs momentum_solver = gs_multicolour, "
+                       "maxit_inner_u and maxit_inner_v >= 1, and at least 2 x 2 cells");
     if (p.pressure_warm_start == SG_WARM_LEARNED &&
         (p.pressure_solver != SG_P_MG || p.mg_precision != SG_MG_MIXED))
         throw ArgError("pressure_warm_start = learned needs pressure_solver = mg and "
@@ -318,6 +370,7 @@ void sg_params_default(sg_params* p) {
     p->mg_dc_maxit = 1;
     p->pressure_warm_start = SG_WARM_NONE;
     p->check_every = 200;
+    p->momentum_precision = SG_MOMP_FP64;
 }
 
 int sg_params_finalize(sg_params* p) {
@@ -390,6 +443,12 @@ Solver* create_solver(const sg_params& p, const double* u, const double* v, cons
         s.fkv = make_face_pack(p, false);
         s.fku.base = dalloc<double>(s, 3 * s.fku.astride);
         s.fkv.base = dalloc<double>(s, 3 * s.fkv.astride);
+        if (mom_dc_ok(p)) {
+            s.dcf[0] = dalloc<float>(s, 10 * s.pku.astride);
+            s.dcf[1] = dalloc<float>(s, 10 * s.pkv.astride);
+            for (auto& d : s.dd)
+                d = dalloc<float>(s, s.n);
+        }
         ensure_log(s, 256);
         diffusive(p, s.diff2, s.diff4);
         if (p.pressure_solver == SG_P_MG)
@@ -598,13 +657,8 @@ int sg_advance(sg_solver* h, int max_iters, int* iters_done, double* log, int* c
                 capture_graph(s, 1, &s.graph);
             for (; k < max_iters; ++k)
                 SG_HIP(hipGraphLaunch(s.graph, s.stream));
-            if (max_iters > 0) { // the iterations' assemblies and corrections
-                s.face[0] = face_ok(s, true), s.face[1] = face_ok(s, false);
-                s.pt_stale[0] = s.pt_stale[1] = true;
-                s.p_lean = mg_lean_ok(s);
-                s.pc_in_scratch = s.p.pressure_solver == SG_P_MG;
-                s.b_raw = mg_fuse_b_ok(s);
-            }
+            if (max_iters > 0)
+                mark_iterated(s);
         }
         const Ctl c = read_ctl(s);
         const int done = std::min(c.log_pos, max_iters);
@@ -674,6 +728,28 @@ int sg_kernel_probe(sg_solver* h, int kid, int reps, double* mean_s, double* byt
             };
             break;
         }
+        case SG_K_SWEEP_U_DC: {
+            if (!mom_dc_ok(p))
+                throw ArgError("the correction sweep probe needs momentum_precision = mixed");
+            cells = static_cast<double>(p.NX) * (p.NY - 1);
+            // FP32 arrays: the 10 of the pack read (8 coefficients, a / AP, r0), the correction
+            // read and written, 4 B each, per cell
+            bpc = 48.0;
+            per = p.maxit_inner_u;
+            one = [&s](int) { launch_dc_sweep_probe(s, true, s.never); };
+            break;
+        }
+        case SG_K_ASSEMBLE_U_DC: {
+            if (!mom_dc_ok(p))
+                throw ArgError("the assembly probe needs momentum_precision = mixed");
+            // u, v, p read (8 B each); AP written (8 B) over the assembly range; the pack (40 B)
+            // and the |r| cell (8 B) written over the sweep range
+            cells = static_cast<double>(p.NX) * (p.NY + 1);
+            bpc = 32.0 + 48.0 * (p.NY - 1) / (p.NY + 1);
+            per = 1;
+            one = [&s](int) { launch_asm_dc(s, true, s.never, false); };
+            break;
+        }
         case SG_K_PSOLVE:
             cells = static_cast<double>(p.NX) * p.NY;
             bpc = 0.0;
@@ -736,6 +812,7 @@ static long* knob(Solver& s, int key) {
     case SG_TUNE_MOM_TILE: tmp = s.mom_tile; return &tmp;
     case SG_TUNE_MOM_TILE_FIRST: tmp = s.mom_tile_first >= 0 ? s.mom_tile_first : s.mom_tile; return &tmp;
     case SG_TUNE_MG_STAGED: tmp = s.mg_staged; return &tmp;
+    case SG_TUNE_DC_TILE: tmp = s.dc_tile; return &tmp;
     default: throw ArgError("unknown tuning key");
     }
 }
@@ -751,6 +828,11 @@ int sg_set_tuning(sg_solver* h, int key, long value) {
             if (value < 0 || value >= N_MOM_TILES)
                 throw ArgError("momentum tile shape must be 0.." + std::to_string(N_MOM_TILES - 1));
             (key == SG_TUNE_MOM_TILE ? s.mom_tile : s.mom_tile_first) = static_cast<int>(value);
+        } else if (key == SG_TUNE_DC_TILE) {
+            if (value < 0 || value >= N_DC_TILES)
+                throw ArgError("correction sweep tile shape must be 0.." +
+                               std::to_string(N_DC_TILES - 1));
+            s.dc_tile = static_cast<int>(value);
         } else if (key == SG_TUNE_MG_STAGED) {
             if (value < 0 || value > 64)
                 throw ArgError("staged level must be 0..64");
@@ -911,6 +993,34 @@ int sg_autotune(sg_solver* h, int reps, char* report, size_t len) {
             s.mom_tile_first = best_f;
             std::snprintf(line, sizeof line, "best first-sweep tile: %s (%d)\n",
                           tile_name(best_f).c_str(), best_f);
+            rep += line;
+        }
+        if (mom_dc_ok(s.p)) {
+            // FP32 correction sweep tile: one middle u sweep per candidate shape.
+            double best_t = 1e30;
+            int best_k = s.dc_tile;
+            for (int k = 0; k < N_DC_TILES; ++k) {
+                s.dc_tile = k;
+                launch_dc_sweep_probe(s, true, s.never);
+                SG_HIP(hipEventRecord(s.kt0, s.stream));
+                for (int r = 0; r < reps; ++r)
+                    launch_dc_sweep_probe(s, true, s.never);
+                SG_HIP(hipEventRecord(s.kt1, s.stream));
+                sync(s);
+                float ms = 0.f;
+                SG_HIP(hipEventElapsedTime(&ms, s.kt0, s.kt1));
+                const double t = 1e-3 * ms / reps;
+                char line[160];
+                std::snprintf(line, sizeof line, "correction sweep tile %-10s : %9.2f us\n",
+                              dc_tile_name(k).c_str(), 1e6 * t);
+                rep += line;
+                if (t < best_t)
+                    best_t = t, best_k = k;
+            }
+            s.dc_tile = best_k;
+            char line[160];
+            std::snprintf(line, sizeof line, "best correction sweep tile: %s (%d)\n",
+                          dc_tile_name(best_k).c_str(), best_k);
             rep += line;
         }
         if (s.p.pressure_solver != SG_P_MG) {

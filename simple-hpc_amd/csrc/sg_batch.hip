@@ -75,9 +75,11 @@ void build_plan(Batch& b) {
         Solver& s = *b.cav[c];
         // Recording runs nothing: the coefficient form and pressure-term flags stay.
         const bool f0 = s.face[0], f1 = s.face[1], p0 = s.pt_stale[0], p1 = s.pt_stale[1];
+        const bool d0 = s.dc_stale[0], d1 = s.dc_stale[1];
         const bool lean = s.p_lean, pcs = s.pc_in_scratch, braw = s.b_raw;
         auto restore = [&] {
             s.face[0] = f0, s.face[1] = f1, s.pt_stale[0] = p0, s.pt_stale[1] = p1;
+            s.dc_stale[0] = d0, s.dc_stale[1] = d1;
             s.p_lean = lean, s.pc_in_scratch = pcs, s.b_raw = braw;
             s.fuse_pc_mean = s.pc_mean_pending = s.fuse_b_mean = s.b_mean_pending = false;
         };
@@ -391,6 +393,7 @@ int sg_batch_create(const sg_params* ps, int n, sg_batch** out) {
                 p.pressure_solver == p0.pressure_solver && p.mg_cycles == p0.mg_cycles &&
                 p.mg_pre == p0.mg_pre && p.mg_post == p0.mg_post &&
                 p.mg_precision == p0.mg_precision && p.mg_dc_maxit == p0.mg_dc_maxit &&
+                p.momentum_precision == p0.momentum_precision &&
                 p.residual_normalisation == p0.residual_normalisation;
             if (!same)
                 throw ArgError("batch: cavities may differ only in N_cells_x, N_cells_y, Re, "
@@ -460,14 +463,8 @@ int sg_batch_advance(sg_batch* h, int max_iters, int* iters, int* status, double
             const int k = std::min(left, chunk);
             // Cavities still running assemble in this chunk, in their iterations' form.
             for (int c = 0; c < n; ++c)
-                if (!b.done[c]) {
-                    Solver& s = *b.cav[c];
-                    s.face[0] = face_ok(s, true), s.face[1] = face_ok(s, false);
-                    s.pt_stale[0] = s.pt_stale[1] = true;
-                    s.p_lean = mg_lean_ok(s);
-                    s.pc_in_scratch = s.p.pressure_solver == SG_P_MG;
-                    s.b_raw = mg_fuse_b_ok(s);
-                }
+                if (!b.done[c])
+                    mark_iterated(*b.cav[c]);
             begin_chunk(b);
             replay(b, k);
             read_state(b);

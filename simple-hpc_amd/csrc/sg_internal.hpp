@@ -63,6 +63,19 @@ inline std::string tile_name(int k) {
     return std::to_string(MOM_TILES[k].ti) + "x" + std::to_string(MOM_TILES[k].tj);
 }
 
+// Tile shapes of the FP32 correction sweeps (momentum_precision = mixed): rows x columns of cells
+// and threads, indexed by SG_TUNE_DC_TILE.
+struct DcTile {
+    int ti, tj, nt;
+};
+constexpr DcTile DC_TILES[] = {{32, 64, 256}, {32, 64, 512}, {64, 64, 512}, {32, 128, 512},
+                               {48, 64, 512}, {16, 128, 256}, {24, 96, 512}, {40, 56, 512}};
+constexpr int N_DC_TILES = sizeof(DC_TILES) / sizeof(DC_TILES[0]);
+inline std::string dc_tile_name(int k) {
+    return std::to_string(DC_TILES[k].ti) + "x" + std::to_string(DC_TILES[k].tj) + "/" +
+           std::to_string(DC_TILES[k].nt);
+}
+
 // Device control block of the outer loop.
 struct Ctl {
     int stop;       // nonzero: skip all work (set by the stop rule)
@@ -119,6 +132,13 @@ struct Solver {
     std::vector<void*> owned;          // allocations to free
     Pack pku{}, pkv{};                 // packed momentum coefficients
     Pack fku{}, fkv{};                 // face packs: ue, vn, pressure term (face form)
+    // momentum_precision = mixed: FP32 packs of u and v (8 neighbour coefficients, a / AP, r0;
+    // geometry of pku / pkv) and, per component, the two FP32 correction arrays its sweeps
+    // alternate between (u: 0, 1; v: 2, 3). Each component's pair is zero outside its own sweep
+    // range, which the sweeps read as the fixed boundary values' zero correction.
+    float* dcf[2] = {};
+    float* dd[4] = {};
+    bool dc_stale[2] = {};             // u, v: the last outer iteration kept no FP64 coefficients
     bool face[2] = {};                 // u, v: the face pack holds the current coefficients (the
                                        // coefficient pack is stale until ensure_packed)
     bool pt_stale[2] = {};             // u, v: p changed since the pressure terms were evaluated
@@ -136,6 +156,8 @@ struct Solver {
     bool fuse_b_mean = false, b_mean_pending = false, b_raw = false;
     int mom_tile = 0;                  // momentum sweep tile shape, an index of MOM_TILES (tunable)
     int mom_tile_first = -1;           // the first sweep's tile shape (-1: mom_tile)
+    int dc_tile = 6;                   // FP32 correction sweep tile, an index of DC_TILES (24x96:
+                                       // the fastest at 1024^2)
     C9 diff2{}, diff4{};               // diffusive coefficients: order 2 (and band), order 4
     hipGraphExec_t graph = nullptr;    // one captured outer iteration
     hipGraphExec_t graph_n = nullptr;  // GRAPH_ITERS of them (fewer launch gaps)
@@ -200,6 +222,16 @@ void ensure_packed(Solver& s, bool is_u);
 void refresh_pterm(Solver& s, bool is_u);
 Pack make_face_pack(const sg_params& p, bool is_u);
 void launch_bc(Solver& s, bool is_u, const int* stop);
+// momentum_precision = mixed (FP32 sweeps inside FP64 defect correction) applies to these params
+bool mom_dc_ok(const sg_params& p);
+// Throws if the component's FP64 coefficients were not kept (after mixed-momentum iterations).
+void need_coefs(const Solver& s, bool is_u);
+// The assembly with the FP64 residual r0 and the FP32 pack (and u's ghost rows with with_bc).
+void launch_asm_dc(Solver& s, bool is_u, const int* stop, bool with_bc);
+// nsweeps FP32 multicolour sweeps of the correction; the last writes x0 + d into the field.
+void launch_dc_sweeps(Solver& s, bool is_u, int nsweeps, const int* stop);
+// One middle sweep of the correction (neither first nor last), correction array 0 -> 1: probes.
+void launch_dc_sweep_probe(Solver& s, bool is_u, const int* stop);
 void launch_mom_residual(Solver& s, bool is_u, double* slot, const int* stop, bool defer);
 // bcv (v, multicolour, at least one sweep): v's ghost columns are set by the first sweep, which
 // loads them as the boundary condition gives them, instead of a launch_bc before the sweeps.
@@ -302,6 +334,8 @@ __device__ __forceinline__ void stop_rule(const StopArgs& a) {
 // shapes and pointers are baked in).
 constexpr int GRAPH_ITERS = 8;
 void drop_graphs(Solver& s);
+// The arrays' state flags after outer iterations ran as graph replays.
+void mark_iterated(Solver& s);
 
 // Enqueue every deferred sum: one launch for all their rows, one for their totals (with rule:
 // the totals and then the stopping rule in that second launch).

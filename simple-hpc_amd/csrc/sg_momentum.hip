@@ -24,6 +24,8 @@
 // pressure term (p(i,j) - p(i,j+1)) * dy is evaluated once, at assembly, with the same operands.
 #include "sg_internal.hpp"
 
+#include <stdexcept>
+#include <string>
 #include <utility>
 
 namespace sg {
@@ -372,6 +374,242 @@ __device__ __forceinline__ void k_assemble_face(const AsmArgs& a, const Blk& bk)
     }
     if (IS_U && a.ubc && i - fk.ai0 < 2)
         ghost_rows_u(a, i - fk.ai0, jj);
+}
+
+// ---- FP32 momentum smoothing inside FP64 defect correction (momentum_precision = mixed) ----
+//
+// A multicolour sweep is linear in the field: with x = x0 + d, the update
+// x = a (nb(x) + pterm) / AP + (1 - a) x of src/momentum_u.cpp:123-143 becomes
+// d = a (r0 + nb(d)) / AP + (1 - a) d, r0 = nb(x0) + pterm - AP x0 the residual of x0
+// (src/momentum_u.cpp:43-59), with d = 0 to start. So the outer iteration's assembly also evaluates
+// r0 in FP64, exactly as the reference's residual does (its |r0| is res_u), the sweeps run on the
+// correction d in FP32, and the last sweep writes x0 + d in FP64. The fixed point of the outer
+// loop is still that of the FP64 residual: d = 0 where r0 = 0.
+//
+// The FP32 pack holds, colour-packed like the coefficient pack (array a, colour c, row i, k-th
+// cell of the colour), the 8 neighbour coefficients (AE, AW, AN, AS, AEE, AWW, ANN, ASS), a / AP
+// and r0 of every cell of the sweep range: DC_NARR arrays of floats.
+constexpr int DC_W = 8, DC_R = 9, DC_NARR = 10;
+
+struct AsmDcArgs {
+    AsmArgs a;       // the assembly (row-major AP; ubc: u's ghost rows set in the same launch)
+    float* dc;       // FP32 pack over the geometry of a.pk (sweep-range cells only)
+    double alpha;    // relaxation factor of the component
+    double* cell;    // |r0| of the sweep range (the residual's cells), row-major
+    int i0, i1, j0, j1; // sweep range
+};
+
+// The field after the boundary conditions of this phase: u's ghost rows (src/momentum_u.cpp:24-34)
+// and v's ghost columns (src/momentum_v.cpp:24-34) as bc_u / bc_v store them, from the inner
+// values, so the residual sees what the reference's residual sees after bc.
+template <bool IS_U>
+__device__ __forceinline__ double bc_val(const AsmArgs& a, int i, int j) {
+    const long ld = a.ld;
+    const int H = a.cv.H, NX = a.cv.NX, NY = a.cv.NY;
+    const double* x = IS_U ? a.u : a.v;
+    if (IS_U) {
+        if (j >= H + 1 && j <= NY + H) {
+            if (i == H - 1)
+                return 2 - x[static_cast<size_t>(H) * ld + j];
+            if (i == NX + H)
+                return -x[static_cast<size_t>(NX + H - 1) * ld + j];
+        }
+    } else if (i >= H && i <= NX + H - 1) {
+        if (j == H)
+            return -x[static_cast<size_t>(i) * ld + H + 1];
+        if (j == NY + H + 1)
+            return -x[static_cast<size_t>(i) * ld + NY + H];
+    }
+    return x[static_cast<size_t>(i) * ld + j];
+}
+
+// The assembly of k_assemble (coefficients from the faces, AP row-major; u's ghost rows) plus, for
+// every cell of the sweep range, r0 = ((E + W + N + S) + (EE + WW + NN + SS)) + pterm - AP x0
+// (src/momentum_u.cpp:48-50, the same operands and order) into the |r0| cells, and the FP32 pack.
+template <bool IS_U> __device__ __forceinline__ void k_asm_dc(const AsmDcArgs& d, const Blk& bk) {
+    const AsmArgs& a = d.a;
+    const Pack& pk = a.pk;
+    const int kk = bk.x * FBX + threadIdx.x;
+    const int i = pk.ai0 + bk.y * FAY + threadIdx.y;
+    if (i > pk.ai1)
+        return;
+    const long ld = a.ld;
+    constexpr int MC = 5;
+    double fue[MC], fuw[MC], fvn[MC], fvs[MC], fpt[MC];
+    int jj[MC];
+#pragma unroll
+    for (int c = 0; c < MC; ++c) {
+        const int j = jfirst(pk, c < pk.ncol ? c : 0, i) + pk.ncol * kk;
+        jj[c] = (c < pk.ncol && j <= pk.aj1) ? j : -1;
+        if (jj[c] < 0)
+            continue;
+        const size_t q = static_cast<size_t>(i) * ld + j;
+        if (IS_U) {
+            fue[c] = (a.u[q] + a.u[q + 1]) / 2;
+            fuw[c] = (a.u[q] + a.u[q - 1]) / 2;
+            fvn[c] = (a.v[q] + a.v[q + 1]) / 2;
+            fvs[c] = (a.v[q + ld] + a.v[q + ld + 1]) / 2;
+        } else {
+            fue[c] = (a.u[q - ld] + a.u[q]) / 2;
+            fuw[c] = (a.u[q - 1] + a.u[q - ld - 1]) / 2;
+            fvn[c] = (a.v[q] + a.v[q - ld]) / 2;
+            fvs[c] = (a.v[q] + a.v[q + ld]) / 2;
+        }
+        fpt[c] = (a.pf[q] - a.pf[q + a.poff]) * a.h;
+    }
+    const size_t as = pk.astride;
+    const int rw = d.j1 - d.j0 + 1;
+#pragma unroll
+    for (int c = 0; c < MC; ++c) {
+        const int j = jj[c];
+        if (j < 0)
+            continue;
+        const size_t q = static_cast<size_t>(i) * ld + j;
+        const C9 co = mom_coefs(a.cv, i, j, fue[c], fuw[c], fvn[c], fvs[c]);
+        a.APn[q] = co.v[8];
+        if (i < d.i0 || i > d.i1 || j < d.j0 || j > d.j1)
+            continue;
+        const double nb =
+            (co.v[0] * bc_val<IS_U>(a, i, j + 1) + co.v[1] * bc_val<IS_U>(a, i, j - 1) +
+             co.v[2] * bc_val<IS_U>(a, i - 1, j) + co.v[3] * bc_val<IS_U>(a, i + 1, j)) +
+            (co.v[4] * bc_val<IS_U>(a, i, j + 2) + co.v[5] * bc_val<IS_U>(a, i, j - 2) +
+             co.v[6] * bc_val<IS_U>(a, i - 2, j) + co.v[7] * bc_val<IS_U>(a, i + 2, j));
+        const double r0 = nb + fpt[c] - co.v[8] * bc_val<IS_U>(a, i, j);
+        d.cell[static_cast<size_t>(i - d.i0) * rw + (j - d.j0)] = fabs(r0);
+        float* b = d.dc + pidx(pk, c, i, kk);
+#pragma unroll
+        for (int k = 0; k < DC_W; ++k)
+            b[k * as] = static_cast<float>(co.v[k]);
+        b[DC_W * as] = static_cast<float>(d.alpha / co.v[8]);
+        b[DC_R * as] = static_cast<float>(r0);
+    }
+    if (IS_U && a.ubc && i - pk.ai0 < 2)
+        ghost_rows_u(a, i - pk.ai0, jj);
+}
+
+// One multicolour sweep of the correction d on a TI x TJ tile in LDS (FP32), the colour passes as
+// in k_mom_sweep (pass k over the tile grown by R (NCOL - 1 - k) cells, so the result equals the
+// colour-by-colour sweep): d = (a / AP) (r0 + nb(d)) + (1 - a) d. FIRST: d starts at 0 (nothing
+// loaded). LAST: the tile's cells of the sweep range get x = x0 + d (FP64) instead of d.
+struct Dc32Args {
+    const float* dc; // the FP32 pack
+    Pack g;          // its geometry (base unused)
+    const float* src; // d before the sweep (unused when FIRST)
+    float* dst;      // d after the sweep (unless LAST)
+    double* x;       // LAST: the field, x0 read and x0 + d written
+    int ld, nr, nc;  // d arrays: nr x ld floats, indexed like the fields
+    int i0, i1, j0, j1; // sweep range
+    float oma;       // 1 - a
+};
+
+template <int NCOL, int TI, int TJ, bool FIRST, bool LAST, int NT>
+__device__ __forceinline__ void k_dc_sweep(const Dc32Args& sa, const Blk& bk) {
+    constexpr int R = NCOL == 2 ? 1 : 2;
+    constexpr int HL = R * NCOL;
+    constexpr int LW = TJ + 2 * HL, LH = TI + 2 * HL;
+    __shared__ float su[LH * LW];
+    int tx, ty;
+    xcd_tile(bk, tx, ty);
+    const int ti0 = ty * TI, tj0 = tx * TJ;
+    const int gi0 = ti0 - HL, gj0 = tj0 - HL;
+    const long ld = sa.ld;
+    constexpr int NL = (LH * LW + NT - 1) / NT;
+    if (FIRST) {
+#pragma unroll
+        for (int k = 0; k < NL; ++k) {
+            const int e = k * NT + threadIdx.x;
+            if (e < LH * LW)
+                su[e] = 0.0f;
+        }
+    } else {
+        float v[NL];
+#pragma unroll
+        for (int k = 0; k < NL; ++k) {
+            const int e = k * NT + threadIdx.x, r = e / LW, c = e - r * LW;
+            const int i = gi0 + r, j = gj0 + c;
+            v[k] = (e < LH * LW && i >= 0 && i < sa.nr && j >= 0 && j < sa.nc)
+                       ? sa.src[static_cast<size_t>(i) * ld + j]
+                       : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < NL; ++k) {
+            const int e = k * NT + threadIdx.x;
+            if (e < LH * LW)
+                su[e] = v[k];
+        }
+    }
+    __syncthreads();
+#define SD(i, j) su[((i) - gi0) * LW + ((j) - gj0)]
+    const Pack& g = sa.g;
+    const size_t as = g.astride;
+    constexpr int DM = R * (NCOL - 1);
+    constexpr int MU = ((TI + 2 * DM) * ((TJ + 2 * DM + NCOL) / NCOL) + NT - 1) / NT;
+    float cf[2][MU][DC_NARR];
+    int fi[2][MU], fj[2][MU];
+    // The pack values of this thread's cells of pass k (loaded one pass ahead: they do not depend
+    // on d). Cell u of pass k is the pass's (u * NT + tid)-th.
+    auto fetch = [&](int k, float (&f)[MU][DC_NARR], int (&ci)[MU], int (&cj)[MU]) {
+        const int dd = R * (NCOL - 1 - k);
+        const int ia = max(sa.i0, ti0 - dd), ib = min(sa.i1, ti0 + TI - 1 + dd);
+        const int ja = max(sa.j0, tj0 - dd), jb = min(sa.j1, tj0 + TJ - 1 + dd);
+        const int cpr = (jb - ja + NCOL) / NCOL;
+        const int n = ia <= ib && ja <= jb ? (ib - ia + 1) * cpr : 0;
+        const float rcp = 1.0f / static_cast<float>(max(cpr, 1)); // exact split, as in k_mom_sweep
+#pragma unroll
+        for (int u = 0; u < MU; ++u) {
+            const int e = u * NT + threadIdx.x;
+            cj[u] = -1;
+            if (e >= n)
+                continue;
+            const int r = static_cast<int>((static_cast<float>(e) + 0.5f) * rcp);
+            const int t = e - r * cpr, i = ia + r;
+            const int c0 = colour_of(NCOL, i, ja);
+            const int j = ja + (NCOL == 2 ? ((k - c0) & 1) : (2 * (k - c0) + 10) % 5) + NCOL * t;
+            if (j > jb)
+                continue;
+            ci[u] = i;
+            cj[u] = j;
+            const float* b = sa.dc + (k * g.rows + (i - g.ai0)) * g.kp + (j - g.aj0) / NCOL;
+#pragma unroll
+            for (int a = 0; a < DC_NARR; ++a)
+                if (NCOL != 2 || a < 4 || a >= DC_W) // two colours: no far neighbours
+                    f[u][a] = b[a * as];
+        }
+    };
+    fetch(0, cf[0], fi[0], fj[0]);
+#pragma unroll
+    for (int k = 0; k < NCOL; ++k) {
+        if (k + 1 < NCOL)
+            fetch(k + 1, cf[(k + 1) & 1], fi[(k + 1) & 1], fj[(k + 1) & 1]);
+#pragma unroll
+        for (int u = 0; u < MU; ++u) {
+            const int j = fj[k & 1][u];
+            if (j < 0)
+                continue;
+            const int i = fi[k & 1][u];
+            const float* f = cf[k & 1][u];
+            float nb = f[0] * SD(i, j + 1) + f[1] * SD(i, j - 1) + f[2] * SD(i - 1, j) +
+                       f[3] * SD(i + 1, j);
+            if (NCOL != 2)
+                nb += f[4] * SD(i, j + 2) + f[5] * SD(i, j - 2) + f[6] * SD(i - 2, j) +
+                      f[7] * SD(i + 2, j);
+            SD(i, j) = f[DC_W] * (f[DC_R] + nb) + sa.oma * SD(i, j);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < (TI * TJ + NT - 1) / NT; ++k) {
+        const int e = k * NT + threadIdx.x, r = e / TJ, c = e - r * TJ;
+        const int i = ti0 + r, j = tj0 + c;
+        if (e >= TI * TJ || i < sa.i0 || i > sa.i1 || j < sa.j0 || j > sa.j1)
+            continue;
+        const size_t q = static_cast<size_t>(i) * ld + j;
+        if (LAST)
+            sa.x[q] = sa.x[q] + static_cast<double>(SD(i, j));
+        else
+            sa.dst[q] = SD(i, j);
+    }
+#undef SD
 }
 
 // Face form -> the coefficient pack (and AP row-major) over the assembly range.
@@ -837,6 +1075,7 @@ void launch_assemble(Solver& s, bool is_u, const int* stop, bool with_bc, bool f
     }
     s.face[is_u ? 0 : 1] = face;
     s.pt_stale[is_u ? 0 : 1] = false;
+    s.dc_stale[is_u ? 0 : 1] = false;
 }
 
 bool face_ok(const Solver& s, bool is_u) {
@@ -873,6 +1112,7 @@ void launch_bc(Solver& s, bool is_u, const int* stop) {
 // residual_u / residual_v (src/momentum_u.cpp:43-59): |R| per cell, then the ordered sum into
 // *slot (deferred to flush_sums if defer).
 void launch_mom_residual(Solver& s, bool is_u, double* slot, const int* stop, bool defer) {
+    need_coefs(s, is_u);
     refresh_pterm(s, is_u);
     ensure_packed(s, is_u);
     const MomArgs m = mom_args(s, is_u);
@@ -955,6 +1195,7 @@ bool bcv_in_sweep(const Solver& s) {
 void launch_mom_sweeps(Solver& s, bool is_u, int nsweeps, const int* stop, double* res_slot,
                        bool defer, bool bcv) {
     const sg_params& p = s.p;
+    need_coefs(s, is_u);
     refresh_pterm(s, is_u);
     const MomArgs m = mom_args(s, is_u);
     const bool fuse = res_slot && p.momentum_solver == SG_MOM_GS_MULTICOLOUR && nsweeps > 0;
@@ -1009,8 +1250,10 @@ void launch_mom_sweeps(Solver& s, bool is_u, int nsweeps, const int* stop, doubl
 void sync_packed(Solver& s, int id, bool to_packed) {
     const bool cu = id >= SG_F_CU_AE && id <= SG_F_CU_AP;
     const bool cv = id >= SG_F_CV_AE && id <= SG_F_CV_AP;
-    if (cu || cv)
+    if (cu || cv) {
+        need_coefs(s, cu);
         ensure_packed(s, cu);
+    }
     if (cu || cv) {
         const Pack& pk = cu ? s.pku : s.pkv;
         const int a = id - (cu ? SG_F_CU_AE : SG_F_CV_AE);
@@ -1019,6 +1262,123 @@ void sync_packed(Solver& s, int id, bool to_packed) {
         SG_HIP(hipGetLastError());
     } else if (id == SG_F_P && to_packed) {
         s.pt_stale[0] = s.pt_stale[1] = true;
+    }
+}
+
+bool mom_dc_ok(const sg_params& p) {
+    return p.momentum_precision == SG_MOMP_MIXED && p.momentum_solver == SG_MOM_GS_MULTICOLOUR &&
+           p.maxit_inner_u > 0 && p.maxit_inner_v > 0;
+}
+
+void need_coefs(const Solver& s, bool is_u) {
+    if (s.dc_stale[is_u ? 0 : 1])
+        throw std::runtime_error(
+            std::string("the ") + (is_u ? "u" : "v") +
+            "-momentum coefficients of the last outer iteration are not kept with "
+            "momentum_precision = mixed (its sweeps use an FP32 copy); call sg_assemble_" +
+            (is_u ? "u" : "v") + " first");
+}
+
+void launch_asm_dc(Solver& s, bool is_u, const int* stop, bool with_bc) {
+    const sg_params& p = s.p;
+    AsmDcArgs d{};
+    AsmArgs& a = d.a;
+    a.u = s.F(SG_F_U);
+    a.v = s.F(SG_F_V);
+    a.pf = s.F(SG_F_P);
+    a.APn = s.F(is_u ? SG_F_CU_AP : SG_F_CV_AP);
+    a.pk = is_u ? s.pku : s.pkv; // geometry only: the coefficient pack itself is not written
+    a.ld = s.ld;
+    a.h = is_u ? p.dy : p.dx;
+    a.poff = is_u ? 1 : -static_cast<long>(s.ld);
+    a.ubc = is_u && with_bc ? s.F(SG_F_U) : nullptr;
+    a.cv = mom_conv(s);
+    const MomArgs m = mom_args(s, is_u);
+    d.dc = s.dcf[is_u ? 0 : 1];
+    d.alpha = m.alpha;
+    d.cell = s.rcell[is_u ? 0 : 1];
+    d.i0 = m.i0, d.i1 = m.i1, d.j0 = m.j0, d.j1 = m.j1;
+    const dim3 grid(cdiv(a.pk.kp, FBX), cdiv(a.pk.rows, FAY));
+    if (is_u)
+        emit<AsmDcArgs, k_asm_dc<true>, FBX * FAY>(s, grid, dim3(FBX, FAY), 0, d, stop);
+    else
+        emit<AsmDcArgs, k_asm_dc<false>, FBX * FAY>(s, grid, dim3(FBX, FAY), 0, d, stop);
+    const int k = is_u ? 0 : 1;
+    s.dc_stale[k] = true;
+    s.face[k] = false;
+    s.pt_stale[k] = false;
+}
+
+namespace {
+template <int NCOL, bool FIRST, bool LAST, int K>
+void dc_sweep_k(Solver& s, const Dc32Args& a, const int* stop) {
+    constexpr int TI = DC_TILES[K].ti, TJ = DC_TILES[K].tj, NT = DC_TILES[K].nt;
+    emit<Dc32Args, k_dc_sweep<NCOL, TI, TJ, FIRST, LAST, NT>, NT>(
+        s, dim3(cdiv(s.nc, TJ), cdiv(s.nr, TI)), dim3(NT), 0, a, stop);
+}
+
+template <int NCOL, bool FIRST, bool LAST, int... K>
+void dc_sweep(Solver& s, const Dc32Args& a, const int* stop, std::integer_sequence<int, K...>) {
+    static_assert(sizeof...(K) == N_DC_TILES);
+    ((s.dc_tile == K ? dc_sweep_k<NCOL, FIRST, LAST, K>(s, a, stop) : void()), ...);
+}
+
+template <int NCOL> void dc_sweep_any(Solver& s, const Dc32Args& a, bool first, bool last,
+                                      const int* stop) {
+    constexpr auto shapes = std::make_integer_sequence<int, N_DC_TILES>{};
+    if (first && last)
+        dc_sweep<NCOL, true, true>(s, a, stop, shapes);
+    else if (first)
+        dc_sweep<NCOL, true, false>(s, a, stop, shapes);
+    else if (last)
+        dc_sweep<NCOL, false, true>(s, a, stop, shapes);
+    else
+        dc_sweep<NCOL, false, false>(s, a, stop, shapes);
+}
+} // namespace
+
+namespace {
+Dc32Args dc_args(Solver& s, bool is_u) {
+    const MomArgs m = mom_args(s, is_u);
+    Dc32Args a{};
+    a.dc = s.dcf[is_u ? 0 : 1];
+    a.g = is_u ? s.pku : s.pkv;
+    a.x = m.x;
+    a.ld = s.ld, a.nr = s.nr, a.nc = s.nc;
+    a.i0 = m.i0, a.i1 = m.i1, a.j0 = m.j0, a.j1 = m.j1;
+    a.oma = static_cast<float>(m.one_minus_alpha);
+    return a;
+}
+} // namespace
+
+void launch_dc_sweep_probe(Solver& s, bool is_u, const int* stop) {
+    Dc32Args a = dc_args(s, is_u);
+    float* const* dd = s.dd + (is_u ? 0 : 2);
+    a.src = dd[0];
+    a.dst = dd[1];
+    if (a.g.ncol == 2)
+        dc_sweep_any<2>(s, a, false, false, stop);
+    else
+        dc_sweep_any<5>(s, a, false, false, stop);
+}
+
+void launch_dc_sweeps(Solver& s, bool is_u, int nsweeps, const int* stop) {
+    const MomArgs m = mom_args(s, is_u);
+    Dc32Args a{};
+    a.dc = s.dcf[is_u ? 0 : 1];
+    a.g = is_u ? s.pku : s.pkv;
+    a.x = m.x;
+    a.ld = s.ld, a.nr = s.nr, a.nc = s.nc;
+    a.i0 = m.i0, a.i1 = m.i1, a.j0 = m.j0, a.j1 = m.j1;
+    a.oma = static_cast<float>(m.one_minus_alpha);
+    float* const* dd = s.dd + (is_u ? 0 : 2); // this component's pair
+    for (int k = 0; k < nsweeps; ++k) {
+        a.src = k > 0 ? dd[(k - 1) & 1] : nullptr;
+        a.dst = dd[k & 1];
+        if (m.pk.ncol == 2)
+            dc_sweep_any<2>(s, a, k == 0, k == nsweeps - 1, stop);
+        else
+            dc_sweep_any<5>(s, a, k == 0, k == nsweeps - 1, stop);
     }
 }
 

@@ -160,6 +160,8 @@ void write_outputs(const Params& P, const std::vector<double>& u, const std::vec
     std::fprintf(fp, "  \"mg_omega\": %.17g,\n", p.mg_omega);
     std::fprintf(fp, "  \"mg_precision\": \"%s\",\n  \"mg_coarsest\": %d,\n",
                  p.mg_precision == SG_MG_MIXED ? "mixed" : "fp64", p.mg_coarsest);
+    std::fprintf(fp, "  \"momentum_precision\": \"%s\",\n",
+                 p.momentum_precision == SG_MOMP_MIXED ? "mixed" : "fp64");
     std::fprintf(fp, "  \"pressure_warm_start\": \"%s\",\n",
                  p.pressure_warm_start == SG_WARM_LEARNED ? "learned" : "none");
     std::fprintf(fp, "  \"cg_iterations_total\": %lld,\n  \"cg_iterations_mean\": %.6f,\n",

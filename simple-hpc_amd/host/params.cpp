@@ -62,7 +62,8 @@ const std::set<std::string>& known_keys() {
         "mpi_grid", "halo_overlap", "write_fields",
         // additions
         "mg_precision", "mg_coarsest", "mg_dc_maxit", "pressure_warm_start",
-        "warm_start_weights", "warm_start_level", "check_every", "field_format"};
+        "warm_start_weights", "warm_start_level", "check_every", "field_format",
+        "momentum_precision"};
     return k;
 }
 } // namespace
@@ -149,6 +150,9 @@ Params make_params(const ConfigMap& cfg) {
         }
         else if (k == "mg_precision")
             p.mg_precision = as_enum(k, v, {{"fp64", SG_MG_FP64}, {"mixed", SG_MG_MIXED}});
+        else if (k == "momentum_precision")
+            p.momentum_precision =
+                as_enum(k, v, {{"fp64", SG_MOMP_FP64}, {"mixed", SG_MOMP_MIXED}});
         else if (k == "mg_coarsest") p.mg_coarsest = as_int(k, v);
         else if (k == "mg_dc_maxit") p.mg_dc_maxit = as_int(k, v);
         else if (k == "pressure_warm_start")
@@ -194,6 +198,7 @@ void print_params(std::ostream& os, const Params& P) {
        << "\nwrite_fields = " << (p.write_fields ? "True" : "False")
        << "\nfield_format = " << P.field_format
        << "\nmg_precision = " << (p.mg_precision == SG_MG_MIXED ? "mixed" : "fp64")
+       << "\nmomentum_precision = " << (p.momentum_precision == SG_MOMP_MIXED ? "mixed" : "fp64")
        << "\nmg_coarsest = " << p.mg_coarsest
        << "\npressure_warm_start = " << (p.pressure_warm_start == SG_WARM_LEARNED ? "learned" : "none")
        << "\n";

@@ -16,6 +16,7 @@ BACKEND_SERIAL, BACKEND_OPENMP, BACKEND_GPU = 0, 1, 2
 MOM_GS_LEX, MOM_JACOBI, MOM_GS_MULTICOLOUR = 0, 1, 2
 P_GS_LEX, P_JACOBI, P_GS_REDBLACK, P_CG, P_MG = 0, 1, 2, 3, 4
 MG_FP64, MG_MIXED = 0, 1
+MOMP_FP64, MOMP_MIXED = 0, 1
 WARM_NONE, WARM_LEARNED = 0, 1
 
 FIELDS = ["u", "v", "p",
@@ -27,15 +28,19 @@ NPHASES = 11
 PHASES = ["assemble_u", "bc_u", "residual_u", "sweep_u", "assemble_v", "bc_v", "residual_v",
           "sweep_v", "assemble_p", "solve_p", "correct"]
 (K_SWEEP_U, K_SWEEP_V, K_COPY, K_ASSEMBLE_U, K_PSOLVE, K_SWEEP_U_FIRST, K_ASSEMBLE_V,
- K_SWEEP_V_FIRST) = range(8)
+ K_SWEEP_V_FIRST, K_SWEEP_U_DC, K_ASSEMBLE_U_DC) = range(10)
 TUNE_TAIL_THREADS, TUNE_MID_CELLS, TUNE_LDS_CELLS, TUNE_MOM_TILE, TUNE_MG_STAGED = 0, 1, 2, 3, 4
 TUNE_MOM_TILE_FIRST = 5
+TUNE_DC_TILE = 6
+# Values of TUNE_DC_TILE: FP32 correction sweep tiles as (rows, columns, threads).
+DC_TILES = ((32, 64, 256), (32, 64, 512), (64, 64, 512), (32, 128, 512), (48, 64, 512),
+            (16, 128, 256), (24, 96, 512), (40, 56, 512))
 # Values of TUNE_MOM_TILE / TUNE_MOM_TILE_FIRST: sweep tile shapes as (rows, columns).
 MOM_TILES = ((32, 64), (64, 64), (32, 128), (16, 128), (24, 96), (48, 96), (24, 64), (40, 128),
              (32, 72), (28, 80), (36, 64), (40, 56))
 # Launch-shape knobs in the order of bench.py's --knobs "tile,threads,mid,lds[,staged[,first]]".
 TUNE_KEYS = (TUNE_MOM_TILE, TUNE_TAIL_THREADS, TUNE_MID_CELLS, TUNE_LDS_CELLS, TUNE_MG_STAGED,
-             TUNE_MOM_TILE_FIRST)
+             TUNE_MOM_TILE_FIRST, TUNE_DC_TILE)
 
 
 def tuning_of(obj):
@@ -71,6 +76,7 @@ class SgParams(C.Structure):
         ("dx", C.c_double), ("dy", C.c_double), ("nu", C.c_double),
         ("mg_precision", C.c_int), ("mg_coarsest", C.c_int), ("mg_dc_maxit", C.c_int),
         ("pressure_warm_start", C.c_int), ("check_every", C.c_int),
+        ("momentum_precision", C.c_int),
     ]
 
     def copy(self):
