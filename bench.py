@@ -1,30 +1,41 @@
 """Benchmark of the SIMPLE lid-driven-cavity hot path on one MI355X.
 
-Workload (N=1): the 1024 x 1024 cavity, Re 1000, QUICK convection, 4th-order diffusion,
-multicolour Gauss-Seidel momentum, geometric-multigrid V(2,2) p', zero initial fields, run to the
-absolute SIMPLE residual 1e-5 (the stop rule of the reference's run_outer_loop). A step is one
-SIMPLE outer iteration over the whole grid, replayed as a captured hipGraph; the timed region is
-bracketed by device synchronisations and the fields are resident in HBM throughout.
+Default workload (N=1): the 1024 x 1024 cavity, Re 1000, QUICK convection, 4th-order diffusion,
+multicolour Gauss-Seidel momentum (4 sweeps), geometric-multigrid V(2,2) p', zero initial fields,
+absolute SIMPLE residual tolerance 1e-5 (the stop rule of the reference's run_outer_loop). A step
+is one SIMPLE outer iteration over the whole grid, replayed as a captured hipGraph; the fields stay
+resident in HBM, the stop rule runs on the device, and every timed region is bracketed by device
+synchronisations.
 
-Forms: plain (FP64 multigrid, bit-identical to the reference's CPU path) and mixed (FP32 V-cycle
-inside FP64 defect correction).
-
-Also reported on the same JSON line: the roofline of the dominant kernel (one colour pass of the
-u-momentum multicolour sweep) from hipEvent timings on the solver's stream, and a CPU baseline:
-the oracle (a restatement of the reference's OpenMP CPU path) timed on this host's cores on a
-bounded sample of the same workload.
+The metric is time to the 1e-5 residual, plain FP64 against learned + mixed precision:
+  plain    the reference's arithmetic, bit-identical to its CPU path;
+  learned  mixed-precision multigrid (FP32 V-cycle inside FP64 defect correction) started from the
+           warm-start U-Net's prediction of p' (bfloat16 MFMA; simple-hpc_amd/weights/); the
+           outer loop's stop rule is the same.
+Both run from zero fields to the tolerance in the same process, with the launch shapes of one
+autotune. `value` is the learned + mixed time; the plain time, the ratio and the largest
+differences of u, v and p between the two converged states are on the same line. Also on the
+line:
+  - per-iteration times of every form (`--steps` iterations each after `--warmup`): plain,
+    mixed (FP32 V-cycle), mixed_mom (also FP32 momentum corrections inside FP64 defect
+    correction, momentum_precision = mixed) and learned;
+  - `roofline`: the dominant kernel (one u multicolour sweep) timed with hipEvents on the
+    solver's stream, credited with its algorithmic bytes per cell (the reference sweep's arrays
+    read and written once: 9 coefficients, p and u read, u written, 96 B), against 8 TB/s; the
+    HBM traffic per launch from the committed rocprofv3 counter summary;
+  - `cpu_baseline`: the oracle (a restatement of the reference's OpenMP CPU path) on this host's
+    cores on a bounded sample of the same workload, and its projection to the tolerance.
 
 Other workloads (--workload):
-  batch  the 64 uneven cavities of simple-hpc_amd/workloads/cavities_seed0.txt (N in
-         {64..256}, Re log-uniform in [100, 3200], QUICK, 2nd-order diffusion), advanced together
-         (one persistent launch per phase, one hipGraph per batched iteration) to 1e-5 with
-         mixed-precision multigrid, and again with FP64 multigrid; every cavity's u, v, p compared
-         with its own FP64 solve;
-  large  one 8192 x 8192 cavity, Re 3200, QUICK, 4th-order diffusion: fixed-work outer
-         iterations/s, plain FP64 and mixed.
+  batch    the 64 uneven cavities of simple-hpc_amd/workloads/cavities_seed0.txt to 1e-5, one
+           persistent launch per phase, mixed and FP64 forms;
+  large    one 8192 x 8192 cavity, Re 3200: fixed-work outer iterations/s of plain, mixed and
+           learned + mixed;
+  learned  the warm start's training (data from batched runs, GPU training) and its A/B;
+  unet     the network's bfloat16 MFMA kernels alone.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode fixed|tol] [--forms plain,mixed]
-                       [--workload cavity|batch|large]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode tol|fixed] [--forms ...]
+                       [--workload cavity|batch|large|learned|unet] [--knobs ...] [--no-cpu]
 """
 import argparse
 import json
@@ -36,9 +47,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "simple-hpc_amd", "py"))
 
 HBM_PEAK_GBS = 8000.0
+# The reference sweep's algorithmic bytes per cell: 9 coefficients, p and u read, u written.
+SWEEP_BYTES_PER_CELL = 96.0
+WEIGHTS = os.path.join(ROOT, "simple-hpc_amd", "weights", "unet_warm_start.bin")
 # HBM bytes per launch of the dominant kernel from rocprofv3 FETCH_SIZE / WRITE_SIZE runs,
 # calibrated on a copy kernel (recorded with the commands that made them).
 PROFILE = os.path.join(ROOT, "profiles", "k_mom_sweep_1024sq_re1000.md")
+FORMS = ("plain", "mixed", "mixed_mom", "learned")
 
 
 def recorded_traffic(n):
@@ -51,13 +66,25 @@ def recorded_traffic(n):
     return None
 
 
+def warm_level(n):
+    """Multigrid level the warm-start network runs on: a 256 x 256 input from 1024^2 up."""
+    lv = 2
+    while (n >> (lv + 1)) >= 256 and lv < 8:
+        lv += 1
+    return lv
+
+
 def config_text(n, form, re=1000, maxit=10000000):
     text = (f"N_cells_x = {n}\nN_cells_y = {n}\nRe = {re}\nconvection_scheme = QUICK\n"
             f"diffusion_order = 4\nmomentum_solver = gs_multicolour\npressure_solver = mg\n"
             f"mg_pre = 2\nmg_post = 2\nerror_tolerance = 1e-5\nmaxit = {maxit}\nbackend = gpu\n"
             f"write_fields = False\ncheck_every = 500\n")
-    if form in ("mixed", "learned"):
+    if form in ("mixed", "mixed_mom", "learned"):
         text += "mg_precision = mixed\n"
+    if form == "mixed_mom":
+        text += "momentum_precision = mixed\n"
+    if form == "learned":
+        text += "pressure_warm_start = learned\n"
     return text
 
 
@@ -65,36 +92,73 @@ def log(msg):
     print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
-def run_form(sg, n, form, mode, steps, warmup, knobs=None):
-    p, _ = sg.params_from_config(config_text(n, form))
-    if knobs:  # given launch shapes: "tile,threads,mid,lds[,staged[,first]]" (no autotuner)
-        tune = tuple(int(x) for x in knobs.split(","))
-    else:
-        s = sg.Solver(p)
-        s.advance(3)
-        rep = s.autotune(reps=10)
+class Form:
+    """A solver of one form, with the warm-start network attached for `learned`."""
+
+    def __init__(self, sg, n, form, tune, re=1000, level=None):
+        self.p, _ = sg.params_from_config(config_text(n, form, re))
+        self.s = sg.Solver(self.p)
+        self.net = None
+        if tune:
+            sg.set_tunings(self.s, tune)
+        if form == "learned":
+            import warmstart as ws
+            self.net = sg.UNet(ws.load(WEIGHTS))
+            sg.attach_warm_start(self.s, self.net, level or warm_level(n))
+
+    def close(self):
+        self.s.close()
+        if self.net is not None:
+            self.net.close()
+
+
+def autotune(sg, n, knobs):
+    """Launch shapes {form: knobs}: the FP64 momentum sweeps and FP64 multigrid tuned on the plain
+    form, the FP32 multigrid and the FP32 correction sweeps on the mixed-momentum form (the mixed
+    forms take the FP64 sweep tiles of the plain one: their momentum sweeps are the same)."""
+    if knobs:  # given launch shapes "tile,threads,mid,lds[,staged[,first[,dc]]]" (no autotuner)
+        t = tuple(int(x) for x in knobs.split(","))
+        return {f: t for f in FORMS}
+    tunes = {}
+    for form in ("plain", "mixed_mom"):
+        f = Form(sg, n, form, None)
+        f.s.advance(3)
+        rep = f.s.autotune(reps=10)
         for line in rep.strip().splitlines():
-            log(f"{n}^2 {form}: autotune: {line}")
-        tune = sg.tuning_of(s)
-        s.close()
-    s = sg.Solver(p)  # fresh fields; the tuned launch shapes
-    sg.set_tunings(s, tune)
-    if mode == "fixed":
-        s.advance(warmup)
-        log(f"{n}^2 {form}: timing {steps} outer iterations")
-        t0 = time.perf_counter()
-        done, lg, conv = s.advance(steps)
-        t = time.perf_counter() - t0
-        return s, {"form": form, "iterations": done, "seconds": t, "ms_per_iter": 1e3 * t / done,
-                   "last_residuals": [float(x) for x in lg[-1]],
-                   "coarse_sweeps_last": s.coarse_sweeps(),
-                   "knobs": list(tune)}
+            log(f"{n}^2 {form} autotune: {line}")
+        tunes[form] = sg.tuning_of(f.s)
+        f.close()
+    keys = list(sg.TUNE_KEYS)
+    mixed = list(tunes["mixed_mom"])
+    for k in (sg.TUNE_MOM_TILE, sg.TUNE_MOM_TILE_FIRST):
+        mixed[keys.index(k)] = tunes["plain"][keys.index(k)]
+    return {"plain": tunes["plain"], "mixed": tuple(mixed), "mixed_mom": tuple(mixed),
+            "learned": tuple(mixed)}
+
+
+def fixed_work(sg, n, form, tune, steps, warmup, level=None):
+    f = Form(sg, n, form, tune, level=level)
+    f.s.advance(warmup)
+    t0 = time.perf_counter()
+    done, lg, _ = f.s.advance(steps)
+    t = time.perf_counter() - t0
+    out = {"form": form, "iterations": done, "seconds": t, "ms_per_iter": 1e3 * t / done,
+           "last_residuals": [float(x) for x in lg[-1]]}
+    if form == "learned":
+        out["guesses_kept_discarded"] = list(sg.warm_stats(f.s))
+    f.close()
+    return out
+
+
+def to_tolerance(sg, n, form, tune, chunk=5000, level=None):
+    f = Form(sg, n, form, tune, level=level)
     log(f"{n}^2 {form}: to tolerance")
+    # Every outer iteration from zero fields is timed, the graph captures (a few ms) included.
     t0 = time.perf_counter()
     total, conv, last = 0, False, None
-    next_report = time.perf_counter() + 20
-    while not conv and total < p.maxit:
-        k, lg, conv = s.advance(2000)
+    next_report = t0 + 20
+    while not conv and total < f.p.maxit:
+        k, lg, conv = f.s.advance(chunk)
         total += k
         last = lg[-1] if k else last
         if time.perf_counter() > next_report:
@@ -103,12 +167,26 @@ def run_form(sg, n, form, mode, steps, warmup, knobs=None):
         if k == 0:
             break
     t = time.perf_counter() - t0
-    return s, {"form": form, "iterations": total, "seconds": t, "ms_per_iter": 1e3 * t / total,
-               "converged": bool(conv), "last_residuals": [float(x) for x in last],
-               "knobs": list(tune)}
+    out = {"form": form, "iterations": total, "seconds": t,
+           "ms_per_iter": 1e3 * t / max(total, 1), "converged": bool(conv),
+           "last_residuals": [float(x) for x in last]}
+    if form == "learned":
+        out["guesses_kept_discarded"] = list(sg.warm_stats(f.s))
+    return f, out
 
 
-def cpu_baseline(sg, n, seconds=12.0):
+def field_errors(p, a, b):
+    import numpy as np
+    H = p.H
+    du = float(np.abs(a.get("u") - b.get("u")).max())
+    dv = float(np.abs(a.get("v") - b.get("v")).max())
+    pa = a.get("p")[H:H + p.NX, H + 1:H + 1 + p.NY]
+    pb = b.get("p")[H:H + p.NX, H + 1:H + 1 + p.NY]
+    dp = float(np.abs((pa - pa.mean()) - (pb - pb.mean())).max())
+    return {"u": du, "v": dv, "p": dp}
+
+
+def cpu_baseline(sg, n, seconds=12.0, iterations=None):
     """Oracle outer iterations at the full size on all host cores for about `seconds`."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
@@ -121,9 +199,82 @@ def cpu_baseline(sg, n, seconds=12.0):
         dt, _ = o.run_fixed(1)
         t += dt
         k += 1
-    return {"value": k / t, "unit": "outer iterations/s", "cores": cores, "kind": "port",
-            "sample": f"{k} FP64 outer iterations of the {n}^2 workload from zero fields "
-                      f"(oracle restating the reference's OpenMP path, {cores} threads), {t:.1f} s"}
+    out = {"value": k / t, "unit": "outer iterations/s", "cores": cores, "kind": "port",
+           "sample": f"{k} FP64 outer iterations of the {n}^2 workload from zero fields "
+                     f"(oracle restating the reference's OpenMP path, {cores} threads), {t:.1f} s"}
+    if iterations:
+        # the plain paths are bit-identical, so the CPU needs the same iteration count
+        out["projected_time_to_tolerance_s"] = iterations * t / k
+    return out
+
+
+def run_cavity(sg, a):
+    import numpy as np  # noqa: F401
+    n = a.n
+    tune = autotune(sg, n, a.knobs)
+    forms = [f for f in a.forms.split(",") if f]
+    level = a.warm_level or warm_level(n)
+    table = [fixed_work(sg, n, f, tune[f], a.steps, a.warmup, level) for f in FORMS]
+    for r in table:
+        log(f"{n}^2 {r['form']}: {r['ms_per_iter']:.4f} ms per outer iteration "
+            f"({a.steps} after {a.warmup})")
+    # the dominant kernel, on a solver of the headline form's momentum arithmetic (FP64)
+    probe = Form(sg, n, "plain", tune["plain"])
+    probe.s.advance(3)
+    sec, _, per = probe.s.probe(sg.K_SWEEP_U, reps=200)
+    cells = n * (n - 1)
+    csec, cbytes, _ = probe.s.probe(sg.K_COPY, reps=200)
+    probe.close()
+    achieved = SWEEP_BYTES_PER_CELL * cells / sec / 1e9
+    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": recorded_traffic(n),
+            "kernel": "k_mom_sweep<5> (one u multicolour sweep, all colours, face form)",
+            "kernel_us": sec * 1e6, "algorithmic_bytes_per_launch": SWEEP_BYTES_PER_CELL * cells,
+            "algorithmic_bytes_per_cell": SWEEP_BYTES_PER_CELL, "cells_per_launch": cells,
+            "launches_per_iteration": per,
+            "device_copy": {"bytes_per_launch": cbytes, "us": csec * 1e6,
+                            "GBs": cbytes / csec / 1e9}}
+    out = {"n_gpus": 1, "warmup": a.warmup, "scaling": "weak", "vs_baseline": None,
+           "data": "synthetic (lid-driven cavity from zero fields, lid velocity 1)"}
+    workload = (f"lid-driven cavity {n}x{n}, Re 1000, QUICK, 4th-order diffusion, gs_multicolour "
+                f"momentum (4 sweeps), mg V(2,2) p', tol 1e-5")
+    if a.mode == "fixed":
+        head = next(r for r in table if r["form"] == forms[0])
+        out.update({"metric": f"outer iterations/s, {n}^2 Re1000 ({head['form']})",
+                    "value": head["iterations"] / head["seconds"], "unit": "outer iterations/s",
+                    "steps": head["iterations"], "ms_per_step": head["ms_per_iter"],
+                    "higher_is_better": True, "dtype": "f64",
+                    "config": {"workload": workload, "per_iteration": table,
+                               "knobs": {k: list(v) for k, v in tune.items()}}})
+    else:
+        runs, solvers = {}, {}
+        for f in forms:
+            solvers[f], runs[f] = to_tolerance(sg, n, f, tune[f], level=level)
+            log(f"{n}^2 {f}: {runs[f]['iterations']} iterations in {runs[f]['seconds']:.1f} s")
+        head = runs[forms[-1]]
+        base = runs[forms[0]]
+        errs = field_errors(solvers[forms[0]].p, solvers[forms[-1]].s, solvers[forms[0]].s)
+        for s in solvers.values():
+            s.close()
+        out.update({
+            "metric": f"time to SIMPLE residual 1e-5, {n}^2 Re1000: {forms[0]} FP64 vs "
+                      f"{'mixed+learned' if forms[-1] == 'learned' else forms[-1]}",
+            "value": head["seconds"], "unit": "s", "steps": head["iterations"],
+            "ms_per_step": head["ms_per_iter"], "higher_is_better": False,
+            "dtype": "f64 (plain); f64 + f32 V-cycle + bf16 network (mixed+learned)",
+            "config": {"workload": workload, "to_tolerance": runs,
+                       f"speedup_{forms[-1]}_over_{forms[0]}": base["seconds"] / head["seconds"],
+                       f"max_abs_diff_{forms[-1]}_vs_{forms[0]}": errs,
+                       "per_iteration": table,
+                       "knobs": {k: list(v) for k, v in tune.items()},
+                       "warm_start": {"weights": os.path.relpath(WEIGHTS, ROOT),
+                                      "level": level}}})
+    out["roofline"] = roof
+    if not a.no_cpu:
+        log("cpu baseline")
+        its = out["config"].get("to_tolerance", {}).get("plain", {}).get("iterations")
+        out["cpu_baseline"] = cpu_baseline(sg, n, iterations=its)
+    return out
 
 
 def run_batch(sg, steps_cap, knobs=None, cpu=True):
@@ -154,7 +305,6 @@ def run_batch(sg, steps_cap, knobs=None, cpu=True):
         t0 = time.perf_counter()
         total_its, st = None, None
         next_report = t0 + 20
-        done = 1
         while True:
             its, st, last = b.advance(2000)
             if all(x != 0 for x in st) or max(its) >= steps_cap:
@@ -213,33 +363,27 @@ def cpu_batch_baseline(sg, cases, gpu_iters, seconds=15.0):
                       f"{cores} threads), projected onto the FP64 iteration counts"}
 
 
-def large_config(form):
-    text = (f"N_cells_x = 8192\nN_cells_y = 8192\nRe = 3200\nconvection_scheme = QUICK\n"
-            f"diffusion_order = 4\nmomentum_solver = gs_multicolour\npressure_solver = mg\n"
-            f"mg_pre = 2\nmg_post = 2\nerror_tolerance = 1e-5\nmaxit = 10000000\n"
-            f"backend = gpu\nwrite_fields = False\n")
-    if form == "mixed":
-        text += "mg_precision = mixed\n"
-    return text
-
-
 def run_large(sg, steps, warmup, knobs=None):
-    res = []
-    for form in ("plain", "mixed"):
-        p, _ = sg.params_from_config(large_config(form))
-        s = sg.Solver(p)
-        if knobs:
-            sg.set_tunings(s, [int(x) for x in knobs.split(",")])
-        s.advance(warmup)
+    n, res = 8192, []
+    tune = tuple(int(x) for x in knobs.split(",")) if knobs else None
+    for form in ("plain", "mixed", "learned"):
+        f = Form(sg, n, form, tune, re=3200)
+        f.s.advance(warmup)
         log(f"8192^2 {form}: timing {steps} outer iterations")
         t0 = time.perf_counter()
-        done, lg, conv = s.advance(steps)
+        done, lg, conv = f.s.advance(steps)
         t = time.perf_counter() - t0
-        sec, bytes_, per = s.probe(sg.K_SWEEP_U, reps=20)
-        res.append({"form": form, "iterations": done, "seconds": t, "ms_per_iter": 1e3 * t / done,
-                    "last_residuals": [float(x) for x in lg[-1]],
-                    "sweep_u_us": sec * 1e6, "sweep_u_GBs": bytes_ / sec / 1e9})
-        s.close()
+        r = {"form": form, "iterations": done, "seconds": t, "ms_per_iter": 1e3 * t / done,
+             "last_residuals": [float(x) for x in lg[-1]]}
+        if form == "plain":
+            sec, bytes_, per = f.s.probe(sg.K_SWEEP_U, reps=20)
+            r["sweep_u_us"] = sec * 1e6
+            r["sweep_u_GBs_algorithmic"] = SWEEP_BYTES_PER_CELL * n * (n - 1) / sec / 1e9
+        if form == "learned":
+            r["guesses_kept_discarded"] = list(sg.warm_stats(f.s))
+            r["warm_level"] = warm_level(n)
+        res.append(r)
+        f.close()
     return res
 
 
@@ -259,12 +403,10 @@ def run_learned(sg, quick=False, full=False):
     net = sg.UNet(ws.init_params(0))
     t0 = time.perf_counter()
     steps = 300 if quick else 3000
-    hist = ws.train(net, data, steps=steps, batch=8, lr=1e-3, log=log, every=500)
+    hist = ws.train(net, data, steps=steps, batch=8, lr=1e-3, log=log, every=500, augment=True)
     t_train = time.perf_counter() - t0
     mse, mse0 = ws.evaluate(net, held)
     log(f"held-out mse {mse:.4f} vs zero prediction {mse0:.4f}")
-    theta = net.get_params()
-    ws.save(os.path.join(ROOT, "gpurun_out", "warm_start_weights.bin"), theta)
     sec256, fl256 = net.probe(1, 256, 256, reps=20)
     per_solve, stats = ws.ab_per_solve(net, 256 if quick else 1024, 1000.0,
                                         [10, 100, 1000], level=level)
@@ -285,11 +427,13 @@ def run_learned(sg, quick=False, full=False):
 
 def run_unet(sg, reps=50):
     """The warm-start network's bfloat16 MFMA kernels: one forward pass at the 1024^2 cavity's
-    level-2 shape (256 x 256) and one training step (8 x 64 x 64), timed with hipEvents."""
+    level-2 shape (256 x 256), at the smaller shapes a coarse-level network would see (the
+    multigrid tail starts at 64 x 64), and one training step (8 x 64 x 64), timed with hipEvents."""
     import numpy as np
     import warmstart as ws
     net = sg.UNet(ws.init_params(0))
     sec, fl = net.probe(1, 256, 256, reps=reps)
+    small = {f"{h}x{h}": net.probe(1, h, h, reps=reps)[0] * 1e6 for h in (128, 64, 32)}
     rng = np.random.default_rng(0)
     x = rng.standard_normal((8, 64, 64, 8)).astype(np.float32)
     x[..., 3:] = 0
@@ -302,26 +446,30 @@ def run_unet(sg, reps=50):
     _, fl64 = net.probe(8, 64, 64, reps=1)
     net.close()
     return {"forward_us": sec * 1e6, "forward_flops": fl, "forward_tflops": fl / sec / 1e12,
-            "train_step_ms": tstep * 1e3, "train_step_flops": 3 * fl64,
+            "forward_us_smaller": small, "train_step_ms": tstep * 1e3, "train_step_flops": 3 * fl64,
             "train_tflops": 3 * fl64 / tstep / 1e12}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--steps", type=int, default=1000,
+                    help="outer iterations of each form's fixed-work timing (per-iteration table)")
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--mode", default="fixed", choices=["fixed", "tol"])
-    ap.add_argument("--forms", default="plain")
+    ap.add_argument("--mode", default="tol", choices=["fixed", "tol"])
+    ap.add_argument("--forms", default="plain,learned",
+                    help="to-tolerance forms, the baseline first and the headline last")
     ap.add_argument("--n", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="cavity",
                     choices=["cavity", "batch", "large", "learned", "unet"])
+    ap.add_argument("--warm-level", type=int, default=None,
+                    help="multigrid level the warm-start network runs on (default: 256^2 input)")
     ap.add_argument("--quick", action="store_true", help="learned: a small, fast version")
     ap.add_argument("--full", action="store_true", help="learned: also 1024^2 to tolerance")
     ap.add_argument("--knobs", default=None,
-                    help="launch shapes 'tile,tail_threads,mid_cells,lds_cells' instead of "
-                         "autotuning (profiling runs)")
+                    help="launch shapes 'tile,tail_threads,mid_cells,lds_cells[,staged[,first"
+                         "[,dc]]]' instead of autotuning (profiling runs)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU: the multi-GPU decomposition of the cavity is not built "
@@ -374,7 +522,8 @@ def main():
         print(json.dumps(out))
         return
     if a.workload == "large":
-        r = run_large(sg, a.steps, a.warmup, a.knobs)
+        r = run_large(sg, a.steps if a.steps != 1000 else 50, a.warmup if a.warmup != 50 else 3,
+                      a.knobs)
         out = {"metric": "outer iterations/s, 8192^2 Re3200 (plain)",
                "value": r[0]["iterations"] / r[0]["seconds"], "unit": "outer iterations/s",
                "n_gpus": 1, "steps": r[0]["iterations"], "warmup": a.warmup,
@@ -385,45 +534,7 @@ def main():
                                       "diffusion, gs_multicolour, mg V(2,2)", "forms": r}}
         print(json.dumps(out))
         return
-    results = []
-    probe = None
-    for form in a.forms.split(","):
-        s, r = run_form(sg, a.n, form, a.mode, a.steps, a.warmup, a.knobs)
-        results.append(r)
-        if probe is None:
-            sec, bytes_, per = s.probe(sg.K_SWEEP_U, reps=200)
-            probe = (sec, bytes_, per)
-            csec, cbytes, _ = s.probe(sg.K_COPY, reps=200)  # device copy, for calibration
-            copy = {"bytes_per_launch": cbytes, "us": csec * 1e6, "GBs": cbytes / csec / 1e9}
-        s.close()
-    head = results[0]
-    sec, bytes_, per = probe
-    gbs = bytes_ / sec / 1e9
-    if a.mode == "tol":
-        metric = f"time to SIMPLE residual 1e-5, {a.n}^2 Re1000 ({head['form']})"
-        value, unit, hib = head["seconds"], "s", False
-    else:
-        metric = f"outer iterations/s, {a.n}^2 Re1000 ({head['form']})"
-        value, unit, hib = head["iterations"] / head["seconds"], "outer iterations/s", True
-    out = {
-        "metric": metric, "value": value, "unit": unit,
-        "n_gpus": 1, "steps": head["iterations"], "warmup": a.warmup if a.mode == "fixed" else 0,
-        "ms_per_step": head["ms_per_iter"], "higher_is_better": hib, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic (lid-driven cavity from zero fields, lid velocity 1)",
-        "config": {"workload": f"lid-driven cavity {a.n}x{a.n}, Re 1000, QUICK, 4th-order "
-                               f"diffusion, gs_multicolour momentum, mg V(2,2) p', tol 1e-5",
-                   "forms": results},
-        "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": gbs / HBM_PEAK_GBS, "traffic": recorded_traffic(a.n),
-                     "kernel": "k_mom_sweep<5> (one u multicolour sweep, all colours)",
-                     "kernel_us": sec * 1e6, "bytes_per_launch": bytes_,
-                     "launches_per_iteration": per, "device_copy": copy},
-    }
-    if not a.no_cpu:
-        log("cpu baseline")
-        out["cpu_baseline"] = cpu_baseline(sg, a.n)
-    print(json.dumps(out))
+    print(json.dumps(run_cavity(sg, a)))
 
 
 if __name__ == "__main__":

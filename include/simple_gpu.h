@@ -159,6 +159,18 @@ int sg_advance(sg_solver* s, int max_iters, int* iters_done, double* log, int* c
 /* Iterations done so far by this solver (sg_outer_iteration and sg_advance). */
 int sg_iterations(sg_solver* s, int* iters);
 
+/* Per-phase timing of replayed outer iterations, the measured counterpart of the reference's
+ * per-phase timers (src/hip/solver_hip.hip:502-508, written to timing.csv by src/io.cpp:71-90).
+ * With on != 0, every sg_advance call runs one of its outer iterations launch by launch with
+ * hipEvents at the phase boundaries (the same kernels as the replayed iterations, enqueued behind
+ * the first replayed graph; results unchanged) and adds that iteration's device seconds per phase
+ * (SG_NPHASES, timer.hpp order) into an accumulator; sg_set_phase_timing clears it. In the replayed iteration the residuals are
+ * evaluated inside the first sweeps and summed inside the p' solve, and u's and v's boundary
+ * conditions inside the assembly and the first sweep, so those phases carry that work.
+ * sg_phase_times reads the accumulated seconds and the number of timed iterations. */
+int sg_set_phase_timing(sg_solver* s, int on);
+int sg_phase_times(sg_solver* s, double* phase_s, int* timed_iterations);
+
 /* Mean seconds of one launch of the named kernel over the last sg_advance call that ran with
  * timing on (sg_set_kernel_timing), measured with hipEvents on the solver's stream, and the
  * algorithmic bytes of one launch (SG_K_* ids). */

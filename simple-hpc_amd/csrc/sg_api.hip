@@ -91,9 +91,10 @@ void ensure_log(Solver& s, int cap) {
 
 // One outer iteration (src/solver.cpp:27-93), enqueued. ev: optional phase events (12).
 // The residual sums (they feed only the stop rule) are deferred and done together before it.
-void enqueue_iteration(Solver& s, const int* stop, bool eager, hipEvent_t* ev) {
+void enqueue_iteration(Solver& s, const int* stop, bool eager, hipEvent_t* ev, int fused) {
     const sg_params& p = s.p;
-    const bool defer = !ev; // per-phase timing sums each residual in its own phase
+    // per-phase iterations sum each residual in its own phase
+    const bool defer = fused < 0 ? !ev : fused != 0;
     auto mark = [&](int k) {
         if (ev)
             SG_HIP(hipEventRecord(ev[k], s.stream));
@@ -201,7 +202,7 @@ void capture_graph(Solver& s, int reps, hipGraphExec_t* out) {
     SG_HIP(hipStreamBeginCapture(s.stream, hipStreamCaptureModeRelaxed));
     try {
         for (int r = 0; r < reps; ++r)
-            enqueue_iteration(s, &s.ctl->stop, false, nullptr);
+            enqueue_iteration(s, &s.ctl->stop, false, nullptr, 1);
     } catch (...) {
         s.ride = s.rode = false;
         s.deferred.clear();
@@ -233,6 +234,7 @@ template <class F> int guard(F f) {
         return SG_OK;
     } catch (const HipError& e) {
         g_err = e.what();
+        (void)hipGetLastError(); // the failed call's error is reported here, not by the next call
         return SG_E_HIP;
     } catch (const ArgError& e) {
         g_err = e.what();
@@ -432,9 +434,11 @@ Solver* create_solver(const sg_params& p, const double* u, const double* v, cons
             c = dalloc<double>(s, static_cast<size_t>(p.NX + 2) * (p.NY + 2));
         s.ctl = dalloc<Ctl>(s, 1);
         s.never = dalloc<int>(s, 1);
-        if (p.pressure_solver == SG_P_CG)
+        if (p.pressure_solver == SG_P_CG) {
             for (auto& c : s.cg)
                 c = dalloc<double>(s, s.n);
+            s.cgctl = dalloc<int>(s, 2);
+        }
         s.pku = make_pack(p, true);
         s.pkv = make_pack(p, false);
         s.pku.base = dalloc<double>(s, 10 * s.pku.astride);
@@ -454,6 +458,8 @@ Solver* create_solver(const sg_params& p, const double* u, const double* v, cons
         if (p.pressure_solver == SG_P_MG)
             mg_setup(s);
         for (auto& e : s.ev)
+            SG_HIP(hipEventCreate(&e));
+        for (auto& e : s.pev)
             SG_HIP(hipEventCreate(&e));
         SG_HIP(hipEventCreate(&s.kt0));
         SG_HIP(hipEventCreate(&s.kt1));
@@ -480,6 +486,9 @@ void destroy_solver(Solver* s) {
     if (s->graph_n)
         (void)hipGraphExecDestroy(s->graph_n);
     for (auto e : s->ev)
+        if (e)
+            (void)hipEventDestroy(e);
+    for (auto e : s->pev)
         if (e)
             (void)hipEventDestroy(e);
     if (s->kt0)
@@ -647,18 +656,42 @@ int sg_advance(sg_solver* h, int max_iters, int* iters_done, double* log, int* c
         } else {
             // Every kernel of an iteration returns at once after the stop rule fires, so replaying
             // GRAPH_ITERS iterations per launch stops at the same iteration.
-            int k = 0;
-            if (max_iters >= GRAPH_ITERS) {
+            // Phase timing: one iteration enqueued launch by launch with events at the phase
+            // boundaries (the replayed iteration's launches: fused), behind the first
+            // GRAPH_ITERS-iteration graph when there is one, so the host has enqueued it before
+            // the device reaches it and the event deltas hold no launch gaps. Every graph is
+            // captured first, from the flags at the start of the call, as the eager iteration
+            // is enqueued from them.
+            const bool timed = s.phase_timing && max_iters > 0;
+            const int replayed = max_iters - (timed ? 1 : 0);
+            if (replayed >= GRAPH_ITERS)
                 capture_graph(s, GRAPH_ITERS, &s.graph_n);
-                for (; k + GRAPH_ITERS <= max_iters; k += GRAPH_ITERS)
-                    SG_HIP(hipGraphLaunch(s.graph_n, s.stream));
-            }
-            if (k < max_iters)
+            if (replayed % GRAPH_ITERS)
                 capture_graph(s, 1, &s.graph);
-            for (; k < max_iters; ++k)
+            int k = 0, left = replayed;
+            if (left >= GRAPH_ITERS) {
+                SG_HIP(hipGraphLaunch(s.graph_n, s.stream));
+                k += GRAPH_ITERS, left -= GRAPH_ITERS;
+            }
+            if (timed) {
+                enqueue_iteration(s, &s.ctl->stop, false, s.pev, 1);
+                ++k;
+            }
+            for (; left >= GRAPH_ITERS; k += GRAPH_ITERS, left -= GRAPH_ITERS)
+                SG_HIP(hipGraphLaunch(s.graph_n, s.stream));
+            for (; left > 0; ++k, --left)
                 SG_HIP(hipGraphLaunch(s.graph, s.stream));
             if (max_iters > 0)
                 mark_iterated(s);
+            if (timed) {
+                sync(s);
+                for (int q = 0; q < SG_NPHASES; ++q) {
+                    float ms = 0.f;
+                    SG_HIP(hipEventElapsedTime(&ms, s.pev[q], s.pev[q + 1]));
+                    s.phase_acc[q] += 1e-3 * ms;
+                }
+                ++s.phase_n;
+            }
         }
         const Ctl c = read_ctl(s);
         const int done = std::min(c.log_pos, max_iters);
@@ -679,6 +712,26 @@ int sg_iterations(sg_solver* h, int* iters) {
     return guard([&] {
         Solver& s = S(h);
         *iters = read_ctl(s).iter;
+    });
+}
+
+int sg_set_phase_timing(sg_solver* h, int on) {
+    return guard([&] {
+        Solver& s = S(h);
+        s.phase_timing = on != 0;
+        std::fill(std::begin(s.phase_acc), std::end(s.phase_acc), 0.0);
+        s.phase_n = 0;
+    });
+}
+
+int sg_phase_times(sg_solver* h, double* phase_s, int* timed) {
+    return guard([&] {
+        Solver& s = S(h);
+        if (phase_s)
+            for (int k = 0; k < SG_NPHASES; ++k)
+                phase_s[k] = s.phase_acc[k];
+        if (timed)
+            *timed = s.phase_n;
     });
 }
 
@@ -942,8 +995,9 @@ int sg_autotune(sg_solver* h, int reps, char* report, size_t len) {
         if (reps < 1)
             throw ArgError("reps must be >= 1");
         std::string rep;
-        if (s.p.momentum_solver == SG_MOM_GS_MULTICOLOUR) {
-            // Momentum sweep tile: one u sweep (field -> scratch) per candidate shape.
+        if (s.p.momentum_solver == SG_MOM_GS_MULTICOLOUR && !mom_dc_ok(s.p)) {
+            // Momentum sweep tile: one u sweep (field -> scratch) per candidate shape. (With
+            // momentum_precision = mixed, outer iterations run the correction sweeps instead.)
             double best_t = 1e30;
             int best_k = s.mom_tile;
             for (int k = 0; k < N_MOM_TILES; ++k) {

@@ -340,6 +340,7 @@ template <class F> int bguard(F f) {
         return SG_OK;
     } catch (const HipError& e) {
         set_error(e.what());
+        (void)hipGetLastError(); // reported here, not by the next call
         return SG_E_HIP;
     } catch (const ArgError& e) {
         set_error(e.what());

@@ -90,7 +90,9 @@ struct Ctl {
 
 // Device scalar slots.
 enum {
-    S_RES_U = 0, S_RES_V, S_RES_P, S_SUM_A, S_SUM_B, S_SUM_C, S_SUM_D, S_BSUM, S_COUNT = 32
+    S_RES_U = 0, S_RES_V, S_RES_P, S_SUM_A, S_SUM_B, S_SUM_C, S_SUM_D, S_BSUM,
+    S_CG = 8, // CG's recurrence scalars (8 slots)
+    S_COUNT = 32
 };
 
 struct Warm;
@@ -116,6 +118,7 @@ struct Solver {
     double* rcell[3] = {};             // |R| cells of res_u, res_v, res_p (deferred sums)
     double* scal = nullptr;            // S_COUNT device scalars
     double* cg[4] = {};                // CG r, z, d, q
+    int* cgctl = nullptr;              // CG: done flag (the CG launches' stop flag), iterations
     Ctl* ctl = nullptr;                // device control block
     int* never = nullptr;              // device int that stays 0 (eager per-phase calls)
     double* dlog = nullptr;            // device log, 3 doubles per iteration (log_cap entries)
@@ -161,6 +164,13 @@ struct Solver {
     C9 diff2{}, diff4{};               // diffusive coefficients: order 2 (and band), order 4
     hipGraphExec_t graph = nullptr;    // one captured outer iteration
     hipGraphExec_t graph_n = nullptr;  // GRAPH_ITERS of them (fewer launch gaps)
+    // Phase timing of replayed iterations (sg_set_phase_timing): in every sg_advance call one
+    // outer iteration is enqueued launch by launch with events at its phase boundaries, between
+    // the graph replays; its per-phase device seconds accumulate in phase_acc.
+    bool phase_timing = false;
+    hipEvent_t pev[SG_NPHASES + 1] = {};
+    double phase_acc[SG_NPHASES] = {};
+    int phase_n = 0;
     int iters = 0;                     // host copy of completed iterations
     long long cg_total = 0;
     int cg_last = 0;
@@ -351,7 +361,10 @@ void warm_free(Solver& s);
 // outer loop control (sg_api.hip)
 
 // One outer iteration (src/solver.cpp:27-93), enqueued or recorded. ev: optional phase events.
-void enqueue_iteration(Solver& s, const int* stop, bool eager, hipEvent_t* ev);
+// ev: optional events (SG_NPHASES + 1) recorded at the phase boundaries. fused (default: no ev):
+// the replayed iteration's launch structure (residual sums deferred, boundary conditions in the
+// assemblies and sweeps), else every phase on its own as the eager per-phase iteration runs it.
+void enqueue_iteration(Solver& s, const int* stop, bool eager, hipEvent_t* ev, int fused = -1);
 void ensure_log(Solver& s, int cap);
 Solver* create_solver(const sg_params& p, const double* u, const double* v, const double* pf);
 void destroy_solver(Solver* s);

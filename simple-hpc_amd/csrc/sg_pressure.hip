@@ -214,14 +214,20 @@ __device__ __forceinline__ void k_fill(const FillArgs& g, const Blk& bk) {
         g.a[k] = g.v;
 }
 
-// ---- CG (src/pressure.cpp:208-317); per-cell work on the device, the scalar recurrence on the
-// host from the ordered sums.
+// ---- CG (src/pressure.cpp:208-317). Everything on the device: the per-cell work, the ordered
+// sums, and the scalar recurrence (alpha, beta, the stop test) in one-thread kernels with the
+// reference's expressions. Every CG launch takes the solve's done flag as its stop flag, so the
+// host enqueues iterations in chunks and reads the flag once per chunk.
 
 struct CgArgs {
     PArgs a;
     double *r, *z, *d, *q, *cell;
     size_t plane; // stride between the 4 value planes of cell
+    const double* sc; // the recurrence's scalars (CG_*)
 };
+
+// Scalars of the recurrence, in the solver's scalar slots from S_CG on.
+enum { CG_RR = 0, CG_RZ, CG_MZ, CG_NB, CG_ALPHA, CG_BETA, CG_N };
 
 __device__ __forceinline__ size_t cidx(const PArgs& a, int i, int j) {
     return static_cast<size_t>(i - a.H) * a.NY + (j - a.H - 1);
@@ -247,25 +253,91 @@ __global__ void __launch_bounds__(BX* BY) k_cg_init(CgArgs c, const double* sb, 
     c.cell[k + 3 * c.plane] = ri;
 }
 
-// d = (z - mz) + beta d (first: beta unused, d = z - mz).
-__global__ void __launch_bounds__(BX* BY) k_cg_dir(CgArgs c, double mz, double beta, int first) {
-    const PArgs& a = c.a;
-    const int H = a.H;
-    const int j = H + 1 + blockIdx.x * BX + threadIdx.x;
-    const int i = H + blockIdx.y * BY + threadIdx.y;
-    if (i > a.NX + H - 1 || j > a.NY + H)
+// The recurrence, one thread each (src/pressure.cpp:254-302). sums: the 4 ordered sums
+// (r.r, sum z, r.z, sum r) or dq in sums[0]; ctl: [done, iterations].
+struct CgStep {
+    double* sc;
+    const double* sums;
+    int* ctl;
+    double n, tol;
+    int maxit;
+};
+// After the initial sums: stop at once if r.r = 0, else |b| and the mean-free r.z.
+__global__ void k_cg_start(CgStep g) {
+    if (threadIdx.x != 0)
         return;
-    const size_t q = static_cast<size_t>(i) * a.ld + j;
-    c.d[q] = first ? c.z[q] - mz : (c.z[q] - mz) + beta * c.d[q];
+    const double rr = g.sums[0], sz = g.sums[1], sr = g.sums[3];
+    g.ctl[1] = 0;
+    if (rr == 0.0) {
+        g.ctl[0] = 1;
+        return;
+    }
+    g.ctl[0] = 0;
+    const double mz = sz / g.n;
+    g.sc[CG_RR] = rr;
+    g.sc[CG_NB] = sqrt(rr);
+    g.sc[CG_MZ] = mz;
+    g.sc[CG_RZ] = g.sums[2] - mz * sr;
+}
+// After d.q: alpha = r.z / d.q, or stop if d.q is not positive.
+__global__ void k_cg_alpha(CgStep g) {
+    if (threadIdx.x != 0 || g.ctl[0])
+        return;
+    const double dq = g.sums[0];
+    if (!(dq > 0.0)) {
+        g.ctl[0] = 1;
+        return;
+    }
+    g.sc[CG_ALPHA] = g.sc[CG_RZ] / dq;
+}
+// After the update's sums: one more iteration; stop at |r| / |b| <= tol or the iteration cap,
+// else the new mean-free r.z and beta.
+__global__ void k_cg_next(CgStep g) {
+    if (threadIdx.x != 0 || g.ctl[0])
+        return;
+    const double rr = g.sums[0], sz = g.sums[1], sr = g.sums[3];
+    double rzn = g.sums[2];
+    const int k = ++g.ctl[1];
+    g.sc[CG_RR] = rr;
+    const double rel = sqrt(rr) / g.sc[CG_NB];
+    if (rel <= g.tol || k >= g.maxit) {
+        g.ctl[0] = 1;
+        return;
+    }
+    const double mz = sz / g.n;
+    rzn -= mz * sr;
+    g.sc[CG_MZ] = mz;
+    g.sc[CG_BETA] = rzn / g.sc[CG_RZ];
+    g.sc[CG_RZ] = rzn;
+}
+
+__device__ __forceinline__ bool cg_cell(const PArgs& a, int& i, int& j, const Blk& bk) {
+    const int H = a.H;
+    j = H + 1 + bk.x * BX + threadIdx.x;
+    i = H + bk.y * BY + threadIdx.y;
+    return i <= a.NX + H - 1 && j <= a.NY + H;
+}
+
+// d = (z - mz) + beta d (first: d = z - mz).
+struct CgDirArgs {
+    CgArgs c;
+    int first;
+};
+__device__ __forceinline__ void k_cg_dir(const CgDirArgs& g, const Blk& bk) {
+    const CgArgs& c = g.c;
+    int i, j;
+    if (!cg_cell(c.a, i, j, bk))
+        return;
+    const size_t q = static_cast<size_t>(i) * c.a.ld + j;
+    const double mz = c.sc[CG_MZ];
+    c.d[q] = g.first ? c.z[q] - mz : (c.z[q] - mz) + c.sc[CG_BETA] * c.d[q];
 }
 
 // q = A d (apply_A_p, src/pressure.cpp:165-173); value d*q.
-__global__ void __launch_bounds__(BX* BY) k_cg_matvec(CgArgs c) {
+__device__ __forceinline__ void k_cg_matvec(const CgArgs& c, const Blk& bk) {
     const PArgs& a = c.a;
-    const int H = a.H;
-    const int j = H + 1 + blockIdx.x * BX + threadIdx.x;
-    const int i = H + blockIdx.y * BY + threadIdx.y;
-    if (i > a.NX + H - 1 || j > a.NY + H)
+    int i, j;
+    if (!cg_cell(a, i, j, bk))
         return;
     const long ld = a.ld;
     const size_t q = static_cast<size_t>(i) * ld + j;
@@ -277,14 +349,13 @@ __global__ void __launch_bounds__(BX* BY) k_cg_matvec(CgArgs c) {
 }
 
 // x += alpha d; r -= alpha q; z = r / APP; values r*r, z, r*z, r.
-__global__ void __launch_bounds__(BX* BY) k_cg_update(CgArgs c, double alpha) {
+__device__ __forceinline__ void k_cg_update(const CgArgs& c, const Blk& bk) {
     const PArgs& a = c.a;
-    const int H = a.H;
-    const int j = H + 1 + blockIdx.x * BX + threadIdx.x;
-    const int i = H + blockIdx.y * BY + threadIdx.y;
-    if (i > a.NX + H - 1 || j > a.NY + H)
+    int i, j;
+    if (!cg_cell(a, i, j, bk))
         return;
     const size_t q = static_cast<size_t>(i) * a.ld + j, k = cidx(a, i, j);
+    const double alpha = c.sc[CG_ALPHA];
     a.pc[q] += alpha * c.d[q];
     const double ri = c.r[q] - alpha * c.q[q];
     c.r[q] = ri;
@@ -298,72 +369,56 @@ __global__ void __launch_bounds__(BX* BY) k_cg_update(CgArgs c, double alpha) {
 
 dim3 pgrid(const sg_params& p) { return dim3(cdiv(p.NY, BX), cdiv(p.NX, BY)); }
 
+// CG iterations enqueued per host check of the done flag.
+constexpr int CG_CHUNK = 32;
+
 void solve_cg(Solver& s) {
     const sg_params& p = s.p;
     const PArgs a = p_args(s);
     const size_t plane = static_cast<size_t>(p.NX) * p.NY;
-    CgArgs c{a, s.cg[0], s.cg[1], s.cg[2], s.cg[3], s.cell, plane};
+    double* slot = s.scal + S_SUM_A;
+    double* sc = s.scal + S_CG;
+    CgArgs c{a, s.cg[0], s.cg[1], s.cg[2], s.cg[3], s.cell, plane, sc};
     const double n = static_cast<double>(p.NX) * p.NY;
     const int* never = s.never;
-    double* slot = s.scal + S_SUM_A;
-    auto sum_plane = [&](int m, double* dst) {
-        launch_ordered_sum(s, s.cell + m * plane, p.NY, 0, p.NX - 1, 0, p.NY - 1, dst, never);
+    const int* done = s.cgctl;
+    const CgStep step{sc, slot, s.cgctl, n, p.pressure_tolerance, p.pressure_maxit};
+    auto sum_plane = [&](int m, double* dst, const int* stop) {
+        launch_ordered_sum(s, s.cell + m * plane, p.NY, 0, p.NX - 1, 0, p.NY - 1, dst, stop);
     };
-    auto fetch = [&](double* h, int k) {
-        SG_HIP(hipMemcpyAsync(h, slot, sizeof(double) * k, hipMemcpyDeviceToHost, s.stream));
-        SG_HIP(hipStreamSynchronize(s.stream));
-    };
-    s.cg_last = 0;
+    const dim3 g = pgrid(p), b(BX, BY);
     launch_fill(s, a.pc, s.n, 0.0, never);
-    // sb = sum b, r = b - mb, z = r / APP.
+    // sb = sum b, r = b - mb, z = r / APP, and the four sums.
     launch_ordered_sum(s, a.b, s.ld, p.H, p.NX + p.H - 1, p.H + 1, p.NY + p.H, slot, never);
-    hipLaunchKernelGGL(k_cg_init, pgrid(p), dim3(BX, BY), 0, s.stream, c, slot, n);
+    hipLaunchKernelGGL(k_cg_init, g, b, 0, s.stream, c, slot, n);
     SG_HIP(hipGetLastError());
     for (int m = 0; m < 4; ++m)
-        sum_plane(m, slot + m);
-    double s4[4];
-    fetch(s4, 4);
-    double rr = s4[0], sz = s4[1], rz = s4[2], sr = s4[3];
-    if (rr == 0.0)
-        return;
-    const double nb = std::sqrt(rr);
-    double mz = sz / n;
-    rz -= mz * sr;
-    hipLaunchKernelGGL(k_cg_dir, pgrid(p), dim3(BX, BY), 0, s.stream, c, mz, 0.0, 1);
+        sum_plane(m, slot + m, never);
+    hipLaunchKernelGGL(k_cg_start, dim3(1), dim3(64), 0, s.stream, step);
     SG_HIP(hipGetLastError());
-    int k = 0;
-    while (k < p.pressure_maxit) {
-        hipLaunchKernelGGL(k_cg_matvec, pgrid(p), dim3(BX, BY), 0, s.stream, c);
-        SG_HIP(hipGetLastError());
-        sum_plane(0, slot);
-        double dq;
-        fetch(&dq, 1);
-        if (!(dq > 0.0))
+    emit<CgDirArgs, k_cg_dir, BX * BY>(s, g, b, 0, CgDirArgs{c, 1}, done);
+    int ctl[2] = {0, 0};
+    for (int k = 0; k < p.pressure_maxit; k += CG_CHUNK) {
+        for (int m = 0; m < CG_CHUNK && k + m < p.pressure_maxit; ++m) {
+            emit<CgArgs, k_cg_matvec, BX * BY>(s, g, b, 0, c, done);
+            sum_plane(0, slot, done);
+            hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(64), 0, s.stream, step);
+            SG_HIP(hipGetLastError());
+            emit<CgArgs, k_cg_update, BX * BY>(s, g, b, 0, c, done);
+            for (int q = 0; q < 4; ++q)
+                sum_plane(q, slot + q, done);
+            hipLaunchKernelGGL(k_cg_next, dim3(1), dim3(64), 0, s.stream, step);
+            SG_HIP(hipGetLastError());
+            emit<CgDirArgs, k_cg_dir, BX * BY>(s, g, b, 0, CgDirArgs{c, 0}, done);
+        }
+        SG_HIP(hipMemcpyAsync(ctl, s.cgctl, sizeof ctl, hipMemcpyDeviceToHost, s.stream));
+        SG_HIP(hipStreamSynchronize(s.stream));
+        if (ctl[0])
             break;
-        const double alpha = rz / dq;
-        hipLaunchKernelGGL(k_cg_update, pgrid(p), dim3(BX, BY), 0, s.stream, c, alpha);
-        SG_HIP(hipGetLastError());
-        for (int m = 0; m < 4; ++m)
-            sum_plane(m, slot + m);
-        fetch(s4, 4);
-        rr = s4[0];
-        sz = s4[1];
-        double rzn = s4[2];
-        sr = s4[3];
-        ++k;
-        const double rel = std::sqrt(rr) / nb;
-        if (rel <= p.pressure_tolerance)
-            break;
-        mz = sz / n;
-        rzn -= mz * sr;
-        const double beta = rzn / rz;
-        rz = rzn;
-        hipLaunchKernelGGL(k_cg_dir, pgrid(p), dim3(BX, BY), 0, s.stream, c, mz, beta, 0);
-        SG_HIP(hipGetLastError());
     }
     launch_ordered_mean_sub(s, a.pc, s.ld, p.H, p.NX + p.H - 1, p.H + 1, p.NY + p.H, n, slot,
                             never);
-    s.cg_last = k;
+    s.cg_last = ctl[1];
 }
 
 } // namespace

@@ -54,6 +54,7 @@ struct RunResult {
     long long cg_iterations = 0;
     std::string device;
     int warm_kept = 0, warm_discarded = 0; // learned warm start: guesses used / discarded
+    int phase_sampled = 0;              // replayed iterations timed per phase on the device
 };
 
 // C ABI status -> std::runtime_error with sg_last_error() (the "GPU error: ..." path of

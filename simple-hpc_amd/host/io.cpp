@@ -172,6 +172,7 @@ void write_outputs(const Params& P, const std::vector<double>& u, const std::vec
     std::fprintf(fp, "  \"collective_seconds_mean\": %.6f,\n  \"collective_seconds_max\": %.6f,\n",
                  0.0, 0.0);
     std::fprintf(fp, "  \"diverged\": %s,\n", r.diverged ? "true" : "false");
+    std::fprintf(fp, "  \"phase_timed_iterations\": %d,\n", r.phase_sampled);
     if (p.pressure_warm_start == SG_WARM_LEARNED)
         std::fprintf(fp, "  \"warm_start\": {\"weights\": \"%s\", \"level\": %d, \"kept\": %d, "
                          "\"discarded\": %d},\n",

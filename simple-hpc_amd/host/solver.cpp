@@ -60,6 +60,8 @@ RunResult run_simple_gpu(const Params& P, std::vector<double>& u, std::vector<do
         check(sg_unet_set_params(h.net, w.data(), static_cast<int>(w.size())));
         check(sg_attach_warm_start(h.s, h.net, P.warm_start_level));
     }
+    // Phase timing of the replayed iterations: the first of every chunk is timed on the device.
+    check(sg_set_phase_timing(h.s, 1));
     const auto t0 = std::chrono::steady_clock::now();
     std::array<double, SG_NPHASES> ph1{};
     double eager_s = 0.0;
@@ -107,14 +109,18 @@ RunResult run_simple_gpu(const Params& P, std::vector<double>& u, std::vector<do
     check(sg_download(h.s, SG_F_V, v.data()));
     check(sg_download(h.s, SG_F_P, pf.data()));
     out.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    // Phase seconds: iteration 1 measured per phase; the graph-replayed rest apportioned by the
-    // same fractions (a replayed graph has no phase boundaries the host can time).
-    double s1 = 0.0;
-    for (double x : ph1)
-        s1 += x;
-    const double rest = out.seconds - eager_s;
+    // Phase seconds: iteration 1 measured per phase (eager, hipEvents); the replayed iterations
+    // from the device-timed first iteration of every chunk (event-record nodes in its graph),
+    // their mean per phase times the replayed iterations. What the phases do not cover (gaps
+    // between graph launches, the host's reads of the stop flag) stays in timing.csv's
+    // "unaccounted" row.
+    std::array<double, SG_NPHASES> acc{};
+    int timed = 0;
+    check(sg_phase_times(h.s, acc.data(), &timed));
     for (int k = 0; k < SG_NPHASES; ++k)
-        out.phase_s[k] = ph1[k] + (s1 > 0.0 ? rest * ph1[k] / s1 : 0.0);
+        out.phase_s[k] = ph1[k] + (timed > 0 ? acc[k] / timed * (done - 1) : 0.0);
+    out.phase_sampled = timed;
+    (void)eager_s;
     if (out.diverged)
         std::printf("stopped: non-finite residual at iteration %d\n", done);
     return out;

@@ -184,6 +184,8 @@ def gpu():
         g.sg_unet_mfma_probe.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp]
         g.sg_attach_warm_start.argtypes = [vp, vp, C.c_int]
         g.sg_unet_forward_noalloc.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp]
+        g.sg_set_phase_timing.argtypes = [vp, C.c_int]
+        g.sg_phase_times.argtypes = [vp, dp, ip]
         g.sg_warm_features.argtypes = [vp, C.c_int, fp, dp]
         g.sg_warm_stats.argtypes = [vp, ip, ip]
         _gpu = g
@@ -275,6 +277,17 @@ class Solver:
         conv = C.c_int()
         check(gpu().sg_advance(self.h, n, C.byref(done), _dptr(log), C.byref(conv)))
         return done.value, log[:done.value].copy(), bool(conv.value)
+
+    def set_phase_timing(self, on=True):
+        check(gpu().sg_set_phase_timing(self.h, 1 if on else 0))
+
+    def phase_times(self):
+        """(accumulated device seconds per phase [NPHASES], timed iterations)"""
+        import numpy as np
+        a = np.zeros(NPHASES)
+        n = C.c_int()
+        check(gpu().sg_phase_times(self.h, _dptr(a), C.byref(n)))
+        return a, n.value
 
     def iterations(self):
         k = C.c_int()

@@ -102,8 +102,21 @@ def generate(cases, snapshots, level=2, precision="mixed", log=print):
     return {k: (np.stack(v[0]), np.stack(v[1])) for k, v in data.items()}
 
 
-def train(net, data, steps=2000, batch=8, lr=1e-3, seed=0, log=print, every=200):
-    """Adam steps over minibatches drawn from the size groups in proportion to their pairs."""
+def dihedral(a, k):
+    """Image k (0..7) of the dihedral group of the square applied to the grid axes (1, 2) of a
+    batch [m, h, w, ...]: k & 3 quarter turns, then a flip if k & 4. A p' system on a square grid
+    reflected or turned is again a p' system (links, a_P and b carried along; a_P, the sum of the
+    links, is unchanged), and its solution is the solution carried along, so each image of a pair
+    is an exact pair of the image system."""
+    a = np.rot90(a, k & 3, axes=(1, 2))
+    if k & 4:
+        a = a[:, :, ::-1]
+    return np.ascontiguousarray(a)
+
+
+def train(net, data, steps=2000, batch=8, lr=1e-3, seed=0, log=print, every=200, augment=False):
+    """Adam steps over minibatches drawn from the size groups in proportion to their pairs; with
+    augment, each minibatch is one random dihedral image of its pairs (square grids)."""
     rng = np.random.default_rng(seed)
     keys = sorted(data)
     weights = np.array([len(data[k][0]) for k in keys], np.float64)
@@ -114,14 +127,46 @@ def train(net, data, steps=2000, batch=8, lr=1e-3, seed=0, log=print, every=200)
         k = keys[rng.choice(len(keys), p=weights)]
         x, t = data[k]
         idx = rng.choice(len(x), size=min(batch, len(x)), replace=False)
+        xb, tb = x[idx], t[idx]
+        if augment and xb.shape[1] == xb.shape[2]:
+            k = int(rng.integers(8))
+            xb, tb = dihedral(xb, k), dihedral(tb, k)
         # cosine decay of the step size
         a = lr * 0.5 * (1 + math.cos(math.pi * s / steps))
-        loss = net.train_step(x[idx], t[idx], max(a, lr * 0.02))
+        loss = net.train_step(xb, tb, max(a, lr * 0.02))
         hist.append(loss)
         if log and (s % every == 0 or s == steps - 1):
             log(f"  train step {s}: loss {np.mean(hist[-every:]):.4f} "
                 f"({time.perf_counter() - t0:.1f} s)")
     return hist
+
+
+def normalise(data):
+    """Each target divided by its own RMS (all-zero targets left as they are). The solver scales
+    the network's guess by the energy-minimising factor b.g / g.A.g, so only the shape of the
+    prediction matters: unit-RMS targets keep the samples of very different magnitude (early and
+    late in a run, small and large cavities) equally weighted in the loss."""
+    out = {}
+    for k, (x, t) in data.items():
+        r = np.sqrt((t.astype(np.float64) ** 2).mean(axis=(1, 2)))
+        r[r == 0] = 1.0
+        out[k] = (x, (t / r[:, None, None]).astype(np.float32))
+    return out
+
+
+def shape_error(net, data):
+    """Mean over samples of the relative error of the best-scaled prediction, min over a of
+    |a y - t| / |t| = sqrt(1 - cos^2(y, t)) (1 for the zero prediction)."""
+    errs = []
+    for k, (x, t) in data.items():
+        for i in range(0, len(x), 16):
+            y = net.forward(x[i:i + 16]).astype(np.float64)
+            tt = t[i:i + 16].astype(np.float64)
+            for a, b in zip(y, tt):
+                na, nb = np.linalg.norm(a), np.linalg.norm(b)
+                c = float(np.dot(a.ravel(), b.ravel()) / (na * nb)) if na > 0 and nb > 0 else 0.0
+                errs.append(np.sqrt(max(0.0, 1.0 - c * c)))
+    return float(np.mean(errs))
 
 
 def evaluate(net, data):

@@ -1,0 +1,106 @@
+"""The learned warm start with its committed weights (simple-hpc_amd/weights/unet_warm_start.bin,
+trained by simple-hpc_amd/py/train_warm_start.py): what the trained network gains over the method
+without it, and what it does not.
+
+ - Per p' solve: on states of a 256^2 and of the 1024^2 Re 1000 run, one mixed-precision p'
+   solve started from the network's guess ends closer to the converged p' than the same solve
+   started from zero (the gain), and the guesses are kept.
+ - End to end, 256^2 Re 1000 to the 1e-5 residual: plain FP64, mixed, and learned + mixed all
+   converge; the learned form's outer-iteration count stays within 1 % of the mixed form's (the
+   SIMPLE iteration count is set by the under-relaxation, not by how exactly p' is solved), and
+   its converged u, v, p lie within 1e-6 of the FP64 run's (itself bit-identical to the oracle).
+   Times are printed: the network's forward pass costs more per iteration than it saves.
+ - The stop rule is the reference's: the run stops at the first iteration after the first whose
+   largest residual is at most the tolerance (src/solver.cpp:125).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+pytestmark = pytest.mark.gpu
+
+import simple_gpu as sg  # noqa: E402
+import warmstart as ws  # noqa: E402
+
+WEIGHTS = os.path.join(ROOT, "simple-hpc_amd", "weights", "unet_warm_start.bin")
+
+
+def _params(n, form, tol=1e-5, check_every=500):
+    text = (f"N_cells_x = {n}\nN_cells_y = {n}\nRe = 1000\nconvection_scheme = QUICK\n"
+            f"diffusion_order = 4\nmomentum_solver = gs_multicolour\npressure_solver = mg\n"
+            f"mg_pre = 2\nmg_post = 2\nerror_tolerance = {tol!r}\nmaxit = 10000000\n"
+            f"backend = gpu\nwrite_fields = False\ncheck_every = {check_every}\n")
+    if form != "plain":
+        text += "mg_precision = mixed\n"
+    if form == "learned":
+        text += "pressure_warm_start = learned\n"
+    return sg.params_from_config(text)[0]
+
+
+@pytest.fixture(scope="module")
+def net():
+    assert os.path.exists(WEIGHTS), "the trained weights are committed with the repository"
+    u = sg.UNet(ws.load(WEIGHTS))
+    yield u
+    u.close()
+
+
+@pytest.mark.parametrize("n,snaps", [(256, [10, 100, 1000, 4000]), (1024, [10, 100])])
+def test_warm_start_gain_per_solve(net, n, snaps):
+    per, (kept, rej) = ws.ab_per_solve(net, n, 1000.0, snaps, level=2)
+    for snap, e0, e1 in per:
+        print(f"{n}^2 iteration {snap}: p' error after one solve from zero {e0:.3f}, "
+              f"from the network's guess {e1:.3f} ({e1 / e0:.2f}x)", flush=True)
+    assert kept == len(snaps) and rej == 0
+    for snap, e0, e1 in per:
+        assert e1 < 0.95 * e0, (n, snap, e0, e1)
+
+
+def _to_tolerance(p, net=None):
+    s = sg.Solver(p)
+    if net is not None:
+        sg.attach_warm_start(s, net, 2)
+    import time
+    t0 = time.perf_counter()
+    total, conv, logs = 0, False, []
+    while not conv and total < 200000:
+        k, lg, conv = s.advance(5000)
+        total += k
+        logs.append(lg)
+        if k == 0:
+            break
+    return s, total, conv, time.perf_counter() - t0, np.concatenate(logs)
+
+
+def test_learned_to_tolerance_256(net):
+    n = 256
+    runs = {}
+    for form in ("plain", "mixed", "learned"):
+        s, its, conv, sec, log = _to_tolerance(_params(n, form),
+                                               net if form == "learned" else None)
+        runs[form] = (s, its, conv, sec, log)
+        print(f"256^2 Re 1000 {form}: {its} outer iterations, {sec:.2f} s "
+              f"({1e3 * sec / its:.3f} ms per iteration)", flush=True)
+    for form, (s, its, conv, sec, log) in runs.items():
+        assert conv, form
+        m = log.max(axis=1)
+        assert m[-1] <= 1e-5 and np.all(m[1:-1] > 1e-5), form  # the reference's stop rule
+    assert abs(runs["learned"][1] - runs["mixed"][1]) <= 0.01 * runs["mixed"][1]
+    p = runs["plain"][0].p
+    H = p.H
+    ref, lrn = runs["plain"][0], runs["learned"][0]
+    errs = [float(np.abs(lrn.get(f) - ref.get(f)).max()) for f in ("u", "v")]
+    pa = lrn.get("p")[H:H + n, H + 1:H + 1 + n]
+    pb = ref.get("p")[H:H + n, H + 1:H + 1 + n]
+    errs.append(float(np.abs((pa - pa.mean()) - (pb - pb.mean())).max()))
+    kept, rej = sg.warm_stats(lrn)
+    print(f"learned vs plain at tolerance: max|du| {errs[0]:.2e} max|dv| {errs[1]:.2e} "
+          f"max|dp| {errs[2]:.2e}; guesses kept {kept}, discarded {rej}; time learned / plain "
+          f"{runs['learned'][3] / runs['plain'][3]:.2f}", flush=True)
+    assert max(errs) < 1e-6
+    assert kept > 0.9 * runs["learned"][1]
+    for s, *_ in runs.values():
+        s.close()

@@ -368,3 +368,34 @@ def test_lean_pressure_assembly_mixed():
     assert bits_equal(runs[0][0], runs[1][0]), first_diff(runs[0][0], runs[1][0])
     for a, b in zip(runs[0][1], runs[1][1]):
         assert bits_equal(a, b), first_diff(a, b)
+
+
+def test_phase_timing_leaves_results_unchanged():
+    """Phase timing of replayed iterations (the first iteration of every sg_advance call replayed
+    with event-record nodes at the phase boundaries) gives the oracle's results bit for bit, and
+    per-phase device times that add up to about the replayed iteration's time."""
+    import time
+    cfg = ("N_cells_x = 256\nN_cells_y = 256\nRe = 1000\nmomentum_solver = gs_multicolour\n"
+           "pressure_solver = mg\nmaxit = 100\nerror_tolerance = 0\nbackend = gpu\n")
+    p, _ = sg.params_from_config(cfg)
+    cpu = po.Oracle(p)
+    _, logc, _ = cpu.run()
+    g = sg.Solver(p)
+    g.set_phase_timing(True)
+    logs = []
+    t0 = time.perf_counter()
+    for _ in range(10):
+        _, lg, _ = g.advance(10)
+        logs.append(lg)
+    wall = time.perf_counter() - t0
+    assert bits_equal(np.concatenate(logs), logc[:, 1:])
+    _same("u", g.get("u"), cpu.get("u"))
+    acc, n = g.phase_times()
+    per_iter = acc.sum() / n
+    print(f"phase-timed iterations {n}: " + ", ".join(
+        f"{name} {1e6 * t / n:.1f} us" for name, t in zip(sg.PHASES, acc)) +
+        f"; sum {1e6 * per_iter:.1f} us, wall {1e6 * wall / 100:.1f} us per iteration",
+        flush=True)
+    assert n == 10 and np.all(acc[[0, 3, 4, 7, 8, 9, 10]] > 0)
+    assert 0.5 * wall / 100 < per_iter < 1.5 * wall / 100
+    g.close()

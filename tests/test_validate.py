@@ -115,3 +115,45 @@ def test_validation_matches_published_numbers(tmp_path):
     assert _fmt5(row[5:]) == ["0.01698", "0.03268", "0.00825", "0.01386"]
     assert vg.main([str(d), "--check", "--max-rms-u", "0.025", "--max-rms-v", "0.015"]) == 0
     assert vg.main([str(d), "--check", "--max-rms-u", "0.015", "--max-rms-v", "0.015"]) == 1
+
+
+def test_profile_summariser(tmp_path):
+    """prof_summary groups dispatches by kernel body, grid and workgroup, calibrates the counters
+    on k_copy and gives time shares, traffic and rates (synthetic rocprofv3 CSVs)."""
+    import prof_summary as ps
+    sweep = ("void sg::k_one<sg::(anonymous namespace)::SweepArgs, &sg::(anonymous namespace)::"
+             "k_mom_sweep<5, 40, 56, false, 512, true>, 512>(sg::(anonymous namespace)::"
+             "SweepArgs, int const*)")
+    copy = ("void sg::k_one<sg::(anonymous namespace)::CopyArgs, &sg::(anonymous namespace)::"
+            "k_copy, 256>(sg::(anonymous namespace)::CopyArgs, int const*)")
+    own = "void sg::(anonymous namespace)::k_expand<5>(sg::(anonymous namespace)::ExpArgs)"
+    assert ps.body_name(sweep) == "k_mom_sweep<5,40,56,false,512,true>"
+    assert ps.body_name(copy) == "k_copy" and ps.body_name(own) == "k_expand<5>"
+    assert ps.body_name("__amd_rocclr_fillBufferAligned.kd") == "__amd_rocclr_fillBufferAligned"
+    kt = tmp_path / "kt.csv"
+    kt.write_text("Kind,Kernel_Name,Start_Timestamp,End_Timestamp,Workgroup_Size_X,"
+                  "Workgroup_Size_Y,Workgroup_Size_Z,Grid_Size_X,Grid_Size_Y,Grid_Size_Z\n"
+                  f'KERNEL,"{sweep}",1000,19000,512,1,1,252928,1,1\n'
+                  f'KERNEL,"{sweep}",20000,38000,512,1,1,252928,1,1\n'
+                  f'KERNEL,"{copy}",40000,44000,256,1,1,262144,1,1\n')
+    cols = "Kernel_Name,Grid_Size,Workgroup_Size,Counter_Name,Counter_Value\n"
+    fe, wr = tmp_path / "f.csv", tmp_path / "w.csv"
+    fe.write_text(cols + f'"{sweep}",252928,512,FETCH_SIZE,20000\n'
+                         f'"{sweep}",252928,512,FETCH_SIZE,22000\n'
+                         f'"{copy}",262144,256,FETCH_SIZE,4000\n')
+    wr.write_text(cols + f'"{sweep}",252928,512,WRITE_SIZE,8000\n'
+                         f'"{sweep}",252928,512,WRITE_SIZE,8000\n'
+                         f'"{copy}",262144,256,WRITE_SIZE,8000\n')
+    rows, (ff, wf) = ps.summarise(ps.read_trace(kt), ps.read_counter(fe), ps.read_counter(wr),
+                                  copy_bytes=8000 * 1024.0, algo={"k_mom_sweep": 41.9e6})
+    assert ff == 2.0 and wf == 1.0
+    r = rows[0]
+    assert r["kernel"].startswith("k_mom_sweep") and r["calls"] == 2 and r["us"] == 18.0
+    assert abs(r["share"] - 36.0 / 40.0) < 1e-12
+    assert abs(r["fetch_MB"] - 21000 * 1024 * 2 / 1e6) < 1e-9
+    assert abs(r["GBs"] - (21000 * 2 + 8000) * 1024 / 18e-6 / 1e9) < 1e-6
+    assert abs(r["algo_GBs"] - 41.9e6 / 18e-6 / 1e9) < 1e-9
+    import io
+    buf = io.StringIO()
+    ps.reference_csv(rows, buf)
+    assert buf.getvalue().splitlines()[0] == "kernel,time_pct,us_per_launch,MB_read,MB_written,GB_per_s"
