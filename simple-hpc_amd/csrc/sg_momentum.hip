@@ -12,12 +12,12 @@
 // when the host downloads them.
 //
 // Sweeps. One launch per multicolour sweep (all colours): a workgroup loads a tile of the field
-// plus a halo of 2*ncol cells into LDS and runs the colour passes there, pass k over the tile
-// grown by 2*(ncol-1-k) cells (the cells whose new values later passes of the tile still read),
-// so it reproduces the colour-by-colour sweep exactly; the tile's cells are then written to a
-// second array (tiles read their halo from the first), and sweeps alternate between the field and
-// a scratch copy. The first sweep of an outer iteration also evaluates the residual from the
-// loaded field before any update.
+// plus a halo into LDS and runs the colour passes there, pass k over the tile grown by the cells
+// whose new values later passes of the tile still read (SweepReach, below), so it reproduces the
+// colour-by-colour sweep exactly; the tile's cells are then written to a second array (tiles read
+// their halo from the first), and sweeps alternate between the field and a scratch copy. The
+// first sweep of an outer iteration also evaluates the residual from the loaded field before any
+// update.
 //
 // Arithmetic is the reference's expression for expression (compiled with -ffp-contract=off, IEEE
 // division), so every value is bit-identical to src/momentum_u.cpp / momentum_v.cpp; the
@@ -50,6 +50,70 @@ __device__ __forceinline__ int jfirst(const Pack& k, int c, int i) {
 __device__ __forceinline__ size_t pidx(const Pack& k, int c, int i, int kk) {
     return static_cast<size_t>(c * k.rows + (i - k.ai0)) * k.kp + kk;
 }
+
+// Reach of a temporally blocked multicolour sweep: how far beyond its tile (north = smaller i,
+// south, west = smaller j, east) colour pass k must update cells, and which cells the tile loads.
+// Pass k (colour k) must cover the tile and every cell of colour k that a later pass m > k reads
+// inside its own region: a cell x of colour m reads x + o for the stencil offsets o (E, W, N, S
+// and, with 5 colours, EE, WW, NN, SS) whose colour is k, i.e. (o_i + 3 o_j) mod 5 = k - m (mod 2
+// of o_i + o_j with 2 colours). Then every cell of every pass region sees the values the
+// colour-by-colour sweep gives it: a neighbour of an earlier colour c lies in pass c's region by
+// construction, a neighbour of a later colour still holds its loaded, pre-sweep value, and no
+// stencil offset has colour 0. The loaded cells are the pass regions grown by the stencil reach.
+// With 5 colours the regions come out lopsided (pass 0: 4 north, 2 south, 8 west, 2 east), not
+// the 2 (4 - k) cells on every side of the symmetric cone: 1.17 instead of 1.39 updates per tile
+// cell at 40 x 56, and 23 % fewer cells in LDS.
+struct Ext {
+    int n, s, w, e;
+};
+template <int NCOL> struct SweepReach {
+    static constexpr int R = NCOL == 2 ? 1 : 2; // stencil reach
+    Ext pass[NCOL];
+    Ext load;
+    constexpr SweepReach() : pass{}, load{0, 0, 0, 0} {
+        constexpr int NS = NCOL == 2 ? 4 : 8;
+        constexpr int oi[8] = {0, 0, -1, 1, 0, 0, -2, 2}, oj[8] = {1, -1, 0, 0, 2, -2, 0, 0};
+        for (int k = NCOL - 1; k >= 0; --k) {
+            Ext x{0, 0, 0, 0};
+            for (int m = k + 1; m < NCOL; ++m)
+                for (int t = 0; t < NS; ++t) {
+                    const int c = NCOL == 2 ? ((oi[t] + oj[t]) % 2 + 2) % 2
+                                            : ((oi[t] + 3 * oj[t]) % 5 + 5) % 5;
+                    if (c != ((k - m) % NCOL + NCOL) % NCOL)
+                        continue;
+                    const Ext& p = pass[m];
+                    x.n = x.n > p.n - oi[t] ? x.n : p.n - oi[t];
+                    x.s = x.s > p.s + oi[t] ? x.s : p.s + oi[t];
+                    x.w = x.w > p.w - oj[t] ? x.w : p.w - oj[t];
+                    x.e = x.e > p.e + oj[t] ? x.e : p.e + oj[t];
+                }
+            pass[k] = x;
+            load.n = load.n > x.n + R ? load.n : x.n + R;
+            load.s = load.s > x.s + R ? load.s : x.s + R;
+            load.w = load.w > x.w + R ? load.w : x.w + R;
+            load.e = load.e > x.e + R ? load.e : x.e + R;
+        }
+    }
+    // Cells of colour k per thread of an NT-thread workgroup on a TI x TJ tile, over all passes.
+    template <int TI, int TJ, int NT> constexpr int per_thread() const {
+        int mu = 0;
+        for (int k = 0; k < NCOL; ++k) {
+            const int c = (TI + pass[k].n + pass[k].s) *
+                          ((TJ + pass[k].w + pass[k].e + NCOL) / NCOL);
+            mu = mu > c ? mu : c;
+        }
+        return (mu + NT - 1) / NT;
+    }
+};
+template <int NCOL> constexpr SweepReach<NCOL> REACH{};
+static_assert(REACH<5>.pass[0].n == 4 && REACH<5>.pass[0].s == 2 && REACH<5>.pass[0].w == 8 &&
+                  REACH<5>.pass[0].e == 2 && REACH<5>.pass[3].w == 2 && REACH<5>.pass[4].w == 0,
+              "5-colour pass regions");
+static_assert(REACH<5>.load.n == 6 && REACH<5>.load.s == 4 && REACH<5>.load.w == 10 &&
+                  REACH<5>.load.e == 4,
+              "5-colour load halo");
+static_assert(REACH<2>.pass[0].n == 1 && REACH<2>.pass[0].e == 1 && REACH<2>.load.w == 2,
+              "2-colour regions");
 
 // What a control volume's coefficients depend on besides its face velocities.
 struct MomConv {
@@ -311,7 +375,9 @@ template <bool IS_U> __device__ __forceinline__ void k_assemble(const AsmArgs& a
 
 // Workgroups of the face form's assembly: FBX cells of each colour by FAY rows (32 x 16 measured
 // 18.8 / 22.2 us for u / v at 1024^2, against 21.3 / 27.1 with 64 x 16: the v faces read three rows
-// of v and two of u and p, and a narrower block keeps more of them in L1 and L2).
+// of v and two of u and p, and a narrower block keeps more of them in L1 and L2). Blocks are
+// handed to the XCDs in contiguous bands (xcd_tile), so the rows two vertically adjacent blocks
+// both read are fetched into one L2.
 constexpr int FBX = 32, FAY = 16;
 
 // Face form of the assembly: ue, vn and the pressure term of every face-pack cell (the extra
@@ -321,8 +387,10 @@ constexpr int FBX = 32, FAY = 16;
 template <bool IS_U>
 __device__ __forceinline__ void k_assemble_face(const AsmArgs& a, const Blk& bk) {
     const Pack& fk = a.pk;
-    const int kk = bk.x * FBX + threadIdx.x;
-    const int i = fk.ai0 + bk.y * FAY + threadIdx.y;
+    int tx, ty;
+    xcd_tile(bk, tx, ty);
+    const int kk = tx * FBX + threadIdx.x;
+    const int i = fk.ai0 + ty * FAY + threadIdx.y;
     if (i > fk.ai1)
         return;
     const long ld = a.ld;
@@ -429,8 +497,10 @@ __device__ __forceinline__ double bc_val(const AsmArgs& a, int i, int j) {
 template <bool IS_U> __device__ __forceinline__ void k_asm_dc(const AsmDcArgs& d, const Blk& bk) {
     const AsmArgs& a = d.a;
     const Pack& pk = a.pk;
-    const int kk = bk.x * FBX + threadIdx.x;
-    const int i = pk.ai0 + bk.y * FAY + threadIdx.y;
+    int tx, ty;
+    xcd_tile(bk, tx, ty);
+    const int kk = tx * FBX + threadIdx.x;
+    const int i = pk.ai0 + ty * FAY + threadIdx.y;
     if (i > pk.ai1)
         return;
     const long ld = a.ld;
@@ -488,7 +558,7 @@ template <bool IS_U> __device__ __forceinline__ void k_asm_dc(const AsmDcArgs& d
 }
 
 // One multicolour sweep of the correction d on a TI x TJ tile in LDS (FP32), the colour passes as
-// in k_mom_sweep (pass k over the tile grown by R (NCOL - 1 - k) cells, so the result equals the
+// in k_mom_sweep (pass k over the tile grown by SweepReach's region, so the result equals the
 // colour-by-colour sweep): d = (a / AP) (r0 + nb(d)) + (1 - a) d. FIRST: d starts at 0 (nothing
 // loaded). LAST: the tile's cells of the sweep range get x = x0 + d (FP64) instead of d.
 struct Dc32Args {
@@ -504,14 +574,14 @@ struct Dc32Args {
 
 template <int NCOL, int TI, int TJ, bool FIRST, bool LAST, int NT>
 __device__ __forceinline__ void k_dc_sweep(const Dc32Args& sa, const Blk& bk) {
-    constexpr int R = NCOL == 2 ? 1 : 2;
-    constexpr int HL = R * NCOL;
-    constexpr int LW = TJ + 2 * HL, LH = TI + 2 * HL;
+    constexpr SweepReach<NCOL> RC{};
+    constexpr Ext L = RC.load;
+    constexpr int LW = TJ + L.w + L.e, LH = TI + L.n + L.s;
     __shared__ float su[LH * LW];
     int tx, ty;
     xcd_tile(bk, tx, ty);
     const int ti0 = ty * TI, tj0 = tx * TJ;
-    const int gi0 = ti0 - HL, gj0 = tj0 - HL;
+    const int gi0 = ti0 - L.n, gj0 = tj0 - L.w;
     const long ld = sa.ld;
     constexpr int NL = (LH * LW + NT - 1) / NT;
     if (FIRST) {
@@ -542,16 +612,15 @@ __device__ __forceinline__ void k_dc_sweep(const Dc32Args& sa, const Blk& bk) {
 #define SD(i, j) su[((i) - gi0) * LW + ((j) - gj0)]
     const Pack& g = sa.g;
     const size_t as = g.astride;
-    constexpr int DM = R * (NCOL - 1);
-    constexpr int MU = ((TI + 2 * DM) * ((TJ + 2 * DM + NCOL) / NCOL) + NT - 1) / NT;
+    constexpr int MU = RC.template per_thread<TI, TJ, NT>();
     float cf[2][MU][DC_NARR];
     int fi[2][MU], fj[2][MU];
     // The pack values of this thread's cells of pass k (loaded one pass ahead: they do not depend
     // on d). Cell u of pass k is the pass's (u * NT + tid)-th.
     auto fetch = [&](int k, float (&f)[MU][DC_NARR], int (&ci)[MU], int (&cj)[MU]) {
-        const int dd = R * (NCOL - 1 - k);
-        const int ia = max(sa.i0, ti0 - dd), ib = min(sa.i1, ti0 + TI - 1 + dd);
-        const int ja = max(sa.j0, tj0 - dd), jb = min(sa.j1, tj0 + TJ - 1 + dd);
+        const Ext P = RC.pass[k];
+        const int ia = max(sa.i0, ti0 - P.n), ib = min(sa.i1, ti0 + TI - 1 + P.s);
+        const int ja = max(sa.j0, tj0 - P.w), jb = min(sa.j1, tj0 + TJ - 1 + P.e);
         const int cpr = (jb - ja + NCOL) / NCOL;
         const int n = ia <= ib && ja <= jb ? (ib - ia + 1) * cpr : 0;
         const float rcp = 1.0f / static_cast<float>(max(cpr, 1)); // exact split, as in k_mom_sweep
@@ -750,9 +819,10 @@ __device__ __forceinline__ void k_mom_sweep(const SweepArgs& sa, const Blk& bk) 
     double* dst = sa.dst;
     double* cell = sa.cell;
     const int nr = sa.nr, nc = sa.nc;
-    constexpr int R = NCOL == 2 ? 1 : 2; // stencil reach
-    constexpr int HL = R * NCOL;          // halo read by the first pass
-    constexpr int LW = TJ + 2 * HL, LH = TI + 2 * HL;
+    constexpr SweepReach<NCOL> RC{};
+    constexpr int R = RC.R;     // stencil reach
+    constexpr Ext L = RC.load;  // halo loaded around the tile
+    constexpr int LW = TJ + L.w + L.e, LH = TI + L.n + L.s;
     constexpr int W0 = TJ + 2 * R, H0 = TI + 2 * R; // pre-sweep copy: tile and stencil ring
     __shared__ double su[LH * LW];
     __shared__ double su0[FIRST ? H0 * W0 : 1];
@@ -760,7 +830,7 @@ __device__ __forceinline__ void k_mom_sweep(const SweepArgs& sa, const Blk& bk) 
     int tx, ty;
     xcd_tile(bk, tx, ty);
     const int ti0 = ty * TI, tj0 = tx * TJ;
-    const int gi0 = ti0 - HL, gj0 = tj0 - HL; // array cell of su[0]
+    const int gi0 = ti0 - L.n, gj0 = tj0 - L.w; // array cell of su[0]
     const long ld = m.ld;
     // Batched loads: every load of a chunk is in flight before the first LDS store.
     constexpr int NL = (LH * LW + NT - 1) / NT;
@@ -786,7 +856,7 @@ __device__ __forceinline__ void k_mom_sweep(const SweepArgs& sa, const Blk& bk) 
             if (e < LH * LW) {
                 su[e] = v[k];
                 if (FIRST) {
-                    const int a = e / LW - (HL - R), b = e % LW - (HL - R);
+                    const int a = e / LW - (L.n - R), b = e % LW - (L.w - R);
                     if (a >= 0 && a < H0 && b >= 0 && b < W0)
                         su0[a * W0 + b] = v[k];
                 }
@@ -820,14 +890,13 @@ __device__ __forceinline__ void k_mom_sweep(const SweepArgs& sa, const Blk& bk) 
         // The face values do not depend on the field, so each thread loads those of its cells of
         // pass k + 1 before it updates its cells of pass k: their latency hides behind pass k's
         // arithmetic and barrier. Cell u of pass k is the thread's (u * NT + tid)-th of the pass.
-        constexpr int DM = R * (NCOL - 1);
-        constexpr int MU = ((TI + 2 * DM) * ((TJ + 2 * DM + NCOL) / NCOL) + NT - 1) / NT;
+        constexpr int MU = RC.template per_thread<TI, TJ, NT>();
         double fv[2][MU][5];
         int fi[2][MU], fj[2][MU];
         auto fetch = [&](int k, double (&f)[MU][5], int (&ci)[MU], int (&cj)[MU]) {
-            const int d = R * (NCOL - 1 - k);
-            const int ia = max(m.i0, ti0 - d), ib = min(m.i1, ti0 + TI - 1 + d);
-            const int ja = max(m.j0, tj0 - d), jb = min(m.j1, tj0 + TJ - 1 + d);
+            const Ext P = RC.pass[k];
+            const int ia = max(m.i0, ti0 - P.n), ib = min(m.i1, ti0 + TI - 1 + P.s);
+            const int ja = max(m.j0, tj0 - P.w), jb = min(m.j1, tj0 + TJ - 1 + P.e);
             const int cpr = (jb - ja + NCOL) / NCOL;
             const int n = ia <= ib && ja <= jb ? (ib - ia + 1) * cpr : 0;
             // e / cpr in FP32: e < 2^16 and cpr < 2^12, so (e + 0.5) / cpr is at least 1 / (2 cpr)
@@ -869,9 +938,9 @@ __device__ __forceinline__ void k_mom_sweep(const SweepArgs& sa, const Blk& bk) 
         }
     }
     for (int k = 0; k < (FACE ? 0 : NCOL); ++k) {
-        const int d = R * (NCOL - 1 - k);
-        const int ia = max(m.i0, ti0 - d), ib = min(m.i1, ti0 + TI - 1 + d);
-        const int ja = max(m.j0, tj0 - d), jb = min(m.j1, tj0 + TJ - 1 + d);
+        const Ext P = RC.pass[k];
+        const int ia = max(m.i0, ti0 - P.n), ib = min(m.i1, ti0 + TI - 1 + P.s);
+        const int ja = max(m.j0, tj0 - P.w), jb = min(m.j1, tj0 + TJ - 1 + P.e);
         if (ia <= ib && ja <= jb) {
             const int cpr = (jb - ja + NCOL) / NCOL;
             const int n = (ib - ia + 1) * cpr;
