@@ -332,8 +332,13 @@ int sg_device_name(char* buf, size_t len) {
         hipDeviceProp_t prop;
         SG_HIP(hipGetDeviceProperties(&prop, 0));
         if (buf && len) {
-            std::snprintf(buf, len, "%s (%s, %d CUs)", prop.name, prop.gcnArchName,
-                          prop.multiProcessorCount);
+            // Some boxes report an empty marketing name: then the architecture leads.
+            if (prop.name[0] != '\0')
+                std::snprintf(buf, len, "%s (%s, %d CUs)", prop.name, prop.gcnArchName,
+                              prop.multiProcessorCount);
+            else
+                std::snprintf(buf, len, "%s, %d CUs", prop.gcnArchName,
+                              prop.multiProcessorCount);
         }
     });
 }

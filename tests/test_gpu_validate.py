@@ -53,3 +53,29 @@ def test_ghia_validation_gpu(tmp_path, name, extra, iters, published):
         assert (f"{row[5]:.5f}", f"{row[7]:.5f}") == published
     max_u, max_v = vg.LIMITS[row[1]]
     assert row[5] <= max_u and row[7] <= max_v, row
+
+
+def test_bench_runner_on_gpu(tmp_path):
+    """The fixed-work runner (restating bench/run_bench.py) on the GPU CLI: one row per run with
+    every phase of timing.csv, the solver time of run_info.json and no convergence at tolerance 0.
+    The phase times are measured on sampled eager iterations, so their sum is compared with the
+    solver time only loosely."""
+    import csv
+
+    import run_bench as rb
+    out = tmp_path / "gpu.csv"
+    rows = rb.run([64, 96], 30, 1, [1], os.path.join(ROOT, "build", "simple"), "gpu",
+                  {"mg_precision": "mixed"}, out)
+    with open(out) as fh:
+        got = list(csv.DictReader(fh))
+    assert [int(r["grid"]) for r in got] == [64, 96]
+    for r in got:
+        print({k: r[k] for k in ("grid", "iters", "seconds_per_iter", "solve_p", "cpu")},
+              flush=True)
+        assert int(r["iters"]) == 30 and r["converged"] == "False"
+        assert float(r["seconds_per_iter"]) > 0 and float(r["solve_p"]) > 0
+        assert r["cpu"] and r["compiler"] != "unknown"
+        phases = sum(float(r[p]) for p in ("assemble_u", "sweep_u", "assemble_p", "solve_p",
+                                           "correct"))
+        assert phases < 3 * float(r["total_seconds"])
+    assert len(rows) == 2

@@ -130,6 +130,15 @@ def test_profile_summariser(tmp_path):
     assert ps.body_name(sweep) == "k_mom_sweep<5,40,56,false,512,true>"
     assert ps.body_name(copy) == "k_copy" and ps.body_name(own) == "k_expand<5>"
     assert ps.body_name("__amd_rocclr_fillBufferAligned.kd") == "__amd_rocclr_fillBufferAligned"
+    # the symbols as rocprofv3 reports them (left mangled: a function-pointer template parameter)
+    assert ps.body_name("_ZN2sg5k_oneINS_12_GLOBAL__N_19SweepArgsETnPFvRKT_RKNS_3BlkEEXadL_ZNS1_"
+                        "11k_mom_sweepILi5ELi40ELi56ELb0ELi512ELb1EEEvRKS2_S8_EELi512EEEvS3_PKi"
+                        ) == "k_mom_sweep<5,40,56,false,512,true>"
+    assert ps.body_name("_ZN2sg5k_oneINS_12_GLOBAL__N_12DUIdEETnPFvRKT_RKNS_3BlkEEXadL_ZNS1_11k_"
+                        "mg_down_sIdLi32ELi24EEEvRKNS2_IS4_EES9_EELi512EEEvS4_PKi"
+                        ) == "k_mg_down_s<double,32,24>"
+    assert ps.body_name("_ZN2sg5k_oneINS_8StopArgsETnPFvRKT_RKNS_3BlkEEXadL_ZNS_12_GLOBAL__N_111k"
+                        "_stop_ruleES1_S4_EELi64EEEvS1_PKi") == "k_stop_rule"
     kt = tmp_path / "kt.csv"
     kt.write_text("Kind,Kernel_Name,Start_Timestamp,End_Timestamp,Workgroup_Size_X,"
                   "Workgroup_Size_Y,Workgroup_Size_Z,Grid_Size_X,Grid_Size_Y,Grid_Size_Z\n"
@@ -153,7 +162,56 @@ def test_profile_summariser(tmp_path):
     assert abs(r["fetch_MB"] - 21000 * 1024 * 2 / 1e6) < 1e-9
     assert abs(r["GBs"] - (21000 * 2 + 8000) * 1024 / 18e-6 / 1e9) < 1e-6
     assert abs(r["algo_GBs"] - 41.9e6 / 18e-6 / 1e9) < 1e-9
+    # exact read bytes from the request-size counters: 32 x n32 + 64 x n64 + 128 x n128
+    rq = tmp_path / "r.csv"
+    rq.write_text("Dispatch_Id,Kernel_Name,Grid_Size,Workgroup_Size,Counter_Name,Counter_Value\n"
+                  f'1,"{sweep}",252928,512,TCC_EA0_RDREQ_32B_sum,10\n'
+                  f'1,"{sweep}",252928,512,TCC_EA0_RDREQ_64B_sum,100\n'
+                  f'1,"{sweep}",252928,512,TCC_EA0_RDREQ_128B_sum,1000\n'
+                  f'2,"{sweep}",252928,512,TCC_EA0_RDREQ_128B_sum,3000\n')
+    reads = ps.read_requests(rq)
+    assert sorted(reads[("k_mom_sweep<5,40,56,false,512,true>", 252928, 512)]) == \
+        [320 + 6400 + 128000, 384000]
+    rows2, (ff2, wf2) = ps.summarise(ps.read_trace(kt), None, ps.read_counter(wr), reads=reads)
+    assert ff2 is None and wf2 == 1.0
+    assert abs(rows2[0]["fetch_MB"] - (134720 + 384000) / 2 / 1e6) < 1e-12
+    assert abs(rows2[0]["write_MB"] - 8000 * 1024 / 1e6) < 1e-12
     import io
     buf = io.StringIO()
     ps.reference_csv(rows, buf)
     assert buf.getvalue().splitlines()[0] == "kernel,time_pct,us_per_launch,MB_read,MB_written,GB_per_s"
+
+
+def test_bench_runner_rows(tmp_path):
+    """The bench runner's config and CSV rows (bench/run_bench.py:33-55, 150-185), with a stand-in
+    for the CLI run."""
+    import run_bench as rb
+    text = rb.make_config(96, 40, tmp_path / "o", {"mg_precision": "mixed", "Re": "400"})
+    kv = dict(ln.split(" = ") for ln in text.strip().splitlines())
+    assert kv["N_cells_x"] == kv["N_cells_y"] == "96" and kv["maxit"] == "40"
+    assert kv["error_tolerance"] == "0" and kv["convection_scheme"] == "QUICK"
+    assert kv["diffusion_order"] == "4" and kv["Re"] == "400"  # --set wins over the overrides
+    assert kv["backend"] == "gpu" and kv["pressure_solver"] == "mg"
+    assert kv["relaxation_factor_p"] == "0.2" and kv["mg_precision"] == "mixed"
+    assert kv["output_dir"] == str(tmp_path / "o")
+    calls = []
+
+    def fake(exe, grid, threads, iters, extra):
+        calls.append((grid, threads))
+        phases = [("assemble_u", 0.5 * grid), ("solve_p", 1.0 * grid), ("total", 2.0 * grid),
+                  ("unaccounted", 0.1)]
+        return phases, {"seconds": 1.5 * grid, "iterations": iters, "device": "gfx950 test",
+                        "compiler": "c", "git_hash": "h", "converged": False}
+
+    out = tmp_path / "r.csv"
+    rows = rb.run([64, 128], 10, 2, [1, 4], "simple", "t", {}, out, runner=fake)
+    # repeat-major, the thread list rotated each repeat
+    assert calls == [(64, 1), (64, 4), (128, 1), (128, 4), (64, 4), (64, 1), (128, 4), (128, 1)]
+    with open(out) as fh:
+        head = fh.readline().strip().split(",")
+    assert head == rb.FIXED + ["assemble_u", "solve_p"] + rb.TRAILER
+    r = rows[2]
+    assert r["grid"] == 128 and r["total_seconds"] == f"{192.0:.9f}"
+    assert float(r["seconds_per_iter"]) == pytest.approx(19.2)
+    assert float(r["cell_updates_per_s"]) == pytest.approx(128 * 128 * 10 / 192.0, rel=1e-6)
+    assert r["cpu"] == "gfx950 test" and r["solve_p"] == f"{128.0:.9f}"
