@@ -197,47 +197,105 @@ struct WgArgs {
     int n, h, wd, chunk;
 };
 
-// grid: x = cdiv(9*cin, 64), y = cdiv(cout, 16), z = pixel splits.
+// Weight gradient of one convolution: dW[co][k] = sum over pixels q of dz[q][co] * x(q moved by
+// tap(k))[ci(k)], k = tap * cin + ci, a GEMM whose reduction runs over pixels. The MFMA wants 8
+// consecutive pixels per lane for each operand, while the activations are stored channel-
+// contiguous (NHWC), so a workgroup stages each batch of WG_PX pixels in LDS transposed:
+//  - loads are 16 bytes (8 channels of one pixel; a group of 8 k lies in one tap, cin being a
+//    multiple of 8), four pixels per thread, transposed in registers and stored as 8-byte runs of
+//    four pixels of one channel;
+//  - the pixel -> (image, row, column) split is done once per pixel and batch, not per element;
+//  - each wave then runs WG_PX / 32 MFMAs (16 x 16 x 32) on its 16 k, A and B one 16-byte LDS
+//    read each.
+// A workgroup computes 16 output channels x 64 k over one pixel split (its partial sums in fixed
+// pixel order); k_wreduce adds the splits in order, so training stays deterministic.
+// grid: x = cdiv(9*cin, 64), y = cout / 16, z = pixel splits. Needs x, dz 16-byte aligned and
+// ldx, ldz, cin multiples of 8 (checked on the host).
+constexpr int WG_PX = 128;       // pixels per batch
+constexpr int WG_LD = WG_PX + 8; // LDS row (u16): rows start 16-byte aligned
+
+// Four pixels' 8 channels (v[t]: channels 0..7 of pixel t) to 8 rows of 4 pixels: row c of the
+// LDS tile gets (v[0].c, v[1].c, v[2].c, v[3].c) at column col.
+__device__ __forceinline__ void wg_store_quad(u16* tile, int row0, int col, const uint4 (&v)[4]) {
+    const unsigned* w0 = reinterpret_cast<const unsigned*>(&v[0]);
+    const unsigned* w1 = reinterpret_cast<const unsigned*>(&v[1]);
+    const unsigned* w2 = reinterpret_cast<const unsigned*>(&v[2]);
+    const unsigned* w3 = reinterpret_cast<const unsigned*>(&v[3]);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        // channel 2e: low halves; channel 2e + 1: high halves
+        const uint2 lo = make_uint2((w0[e] & 0xFFFFu) | (w1[e] << 16), (w2[e] & 0xFFFFu) | (w3[e] << 16));
+        const uint2 hi = make_uint2((w0[e] >> 16) | (w1[e] & 0xFFFF0000u),
+                                    (w2[e] >> 16) | (w3[e] & 0xFFFF0000u));
+        *reinterpret_cast<uint2*>(tile + (row0 + 2 * e) * WG_LD + col) = lo;
+        *reinterpret_cast<uint2*>(tile + (row0 + 2 * e + 1) * WG_LD + col) = hi;
+    }
+}
+
 __global__ void __launch_bounds__(256) k_wgrad(WgArgs a) {
+    __shared__ __align__(16) u16 sdz[16 * WG_LD];
+    __shared__ __align__(16) u16 sx[64 * WG_LD];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int m = lane & 15, oct = lane >> 4;
     const int K = 9 * a.cin;
-    const int co = blockIdx.y * 16 + m;
-    const int kc = blockIdx.x * 64 + wave * 16 + m;
-    int tap = 0, ci = 0, dy = 0, dx = 0;
-    if (kc < K) {
-        tap = kc / a.cin;
-        ci = kc - tap * a.cin;
+    const int co0 = blockIdx.y * 16, kc0 = blockIdx.x * 64;
+    const int npix = a.n * a.h * a.wd, hw = a.h * a.wd;
+    const int q0b = blockIdx.z * a.chunk, q1 = min(npix, q0b + a.chunk);
+    // Staging roles. x: thread = (pixel quad xq 0..31, k group xg 0..7); dz (wave 0 only):
+    // thread = (pixel quad 0..31, channel half 0..1).
+    const int xq = threadIdx.x & 31, xg = threadIdx.x >> 5;
+    const int kg = kc0 + 8 * xg;
+    const bool kval = kg < K;
+    int ci = 0, dy = 0, dx = 0;
+    if (kval) {
+        const int tap = kg / a.cin;
+        ci = kg - tap * a.cin;
         dy = tap / 3 - 1;
         dx = tap % 3 - 1;
     }
-    const int npix = a.n * a.h * a.wd;
-    const int q0b = blockIdx.z * a.chunk, q1 = min(npix, q0b + a.chunk);
     f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-    for (int q0 = q0b; q0 < q1; q0 += 32) {
-        u16 av[8], bv[8];
+    for (int qb = q0b; qb < q1; qb += WG_PX) {
+        {
+            uint4 v[4];
 #pragma unroll
-        for (int t = 0; t < 8; ++t) {
-            const int q = q0 + oct * 8 + t;
-            av[t] = 0;
-            bv[t] = 0;
-            if (q < q1) {
-                if (co < a.cout)
-                    av[t] = a.dz[static_cast<size_t>(q) * a.ldz + co];
-                if (kc < K) {
-                    const int j = q % a.wd, r = q / a.wd, i = r % a.h, nn = r / a.h;
+            for (int t = 0; t < 4; ++t) {
+                const int q = qb + 4 * xq + t;
+                v[t] = make_uint4(0, 0, 0, 0);
+                if (kval && q < q1) {
+                    const int nn = q / hw, r = q - nn * hw, i = r / a.wd, j = r - i * a.wd;
                     const int ii = i + dy, jj = j + dx;
                     if (ii >= 0 && ii < a.h && jj >= 0 && jj < a.wd)
-                        bv[t] = a.x[(static_cast<size_t>(nn * a.h + ii) * a.wd + jj) * a.ldx + ci];
+                        v[t] = *reinterpret_cast<const uint4*>(
+                            a.x + (static_cast<size_t>(nn) * hw + ii * a.wd + jj) * a.ldx + ci);
                 }
             }
+            wg_store_quad(sx, 8 * xg, 4 * xq, v);
         }
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(pack8(av), pack8(bv), acc, 0, 0, 0);
+        if (threadIdx.x < 64) {
+            const int dq = threadIdx.x >> 1, half = threadIdx.x & 1;
+            uint4 v[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int q = qb + 4 * dq + t;
+                v[t] = q < q1 ? *reinterpret_cast<const uint4*>(a.dz + static_cast<size_t>(q) * a.ldz +
+                                                               co0 + 8 * half)
+                              : make_uint4(0, 0, 0, 0);
+            }
+            wg_store_quad(sdz, 8 * half, 4 * dq, v);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int g = 0; g < WG_PX / 32; ++g) {
+            const bf16x8 A = ld8(sdz + m * WG_LD + g * 32 + oct * 8);
+            const bf16x8 B = ld8(sx + (wave * 16 + m) * WG_LD + g * 32 + oct * 8);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc, 0, 0, 0);
+        }
+        __syncthreads();
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        const int cr = blockIdx.y * 16 + 4 * oct + i;
-        const int kr = blockIdx.x * 64 + wave * 16 + m;
+        const int cr = co0 + 4 * oct + i;
+        const int kr = kc0 + wave * 16 + m;
         if (cr < a.cout && kr < K)
             a.part[(static_cast<size_t>(blockIdx.z) * a.cout + cr) * K + kr] = acc[i];
     }
@@ -709,7 +767,11 @@ struct UNet {
         int chunk = static_cast<int>((npix + splits - 1) / splits);
         chunk = (chunk + 31) / 32 * 32;
         WgArgs a{x, ldx, s.cin, dz, ldz, s.cout, c.wpart, n, h, w, chunk};
-        hipLaunchKernelGGL(k_wgrad, dim3(cdiv(9 * s.cin, 64), cdiv(s.cout, 16), splits),
+        if ((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(dz)) % 16 != 0 ||
+            ldx % 8 != 0 || ldz % 8 != 0 || s.cin % 8 != 0 || s.cout % 16 != 0)
+            throw ArgError("weight gradient: views must be 16-byte aligned with channel counts "
+                           "in multiples of 8 (16 for outputs)");
+        hipLaunchKernelGGL(k_wgrad, dim3(cdiv(9 * s.cin, 64), s.cout / 16, splits),
                            dim3(256), 0, st, a);
         SG_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_wreduce, dim3(g1(static_cast<long>(s.cout) * 9 * s.cin)), dim3(256),
@@ -980,10 +1042,10 @@ __device__ __forceinline__ void k_guess_rows(const GuessArgs& a, const Blk& bk) 
 // solution than zero in the A-norm. It is discarded (p' = 0) when g.A.g is not positive or alpha is
 // not finite. Rows added in order by one lane, from LDS: the wave first loads a chunk of the row
 // partials at once, so the chain waits on the adds, not on one global load after another (a
-// lane reading them itself took 83 us at 1024 rows).
+// lane reading them itself took 83 us at 1024 rows, one LDS read per add 19 us).
 constexpr int GD_CHUNK = 1024;
 __device__ __forceinline__ void k_guess_decide(const GuessArgs& a, const Blk&) {
-    __shared__ double sp[2 * GD_CHUNK];
+    __shared__ __align__(16) double sp[2 * GD_CHUNK];
     double gag = 0.0, bg = 0.0;
     for (int c0 = 0; c0 < a.nx; c0 += GD_CHUNK) {
         const int n = min(GD_CHUNK, a.nx - c0);
@@ -991,8 +1053,22 @@ __device__ __forceinline__ void k_guess_decide(const GuessArgs& a, const Blk&) {
             sp[e] = a.part[2 * static_cast<size_t>(c0) + e];
         __syncthreads();
         if (threadIdx.x == 0) {
-#pragma unroll 16
-            for (int i = 0; i < n; ++i) {
+            // 16 (g.A.g, b.g) pairs at a time into registers, then added in order.
+            constexpr int B = 16;
+            const double2* sp2 = reinterpret_cast<const double2*>(sp);
+            int i = 0;
+            for (; i + B <= n; i += B) {
+                double2 v[B];
+#pragma unroll
+                for (int q = 0; q < B; ++q)
+                    v[q] = sp2[i + q];
+#pragma unroll
+                for (int q = 0; q < B; ++q) {
+                    gag += v[q].x;
+                    bg += v[q].y;
+                }
+            }
+            for (; i < n; ++i) {
                 gag += sp[2 * i];
                 bg += sp[2 * i + 1];
             }
