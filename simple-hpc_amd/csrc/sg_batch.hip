@@ -300,6 +300,11 @@ void set_knob(Batch& b, int key, long v) {
                 throw ArgError("cell counts must be >= 0");
             *knob_of(*s, key) = v;
             break;
+        case SG_TUNE_DC_TILE:
+            if (v < 0 || v >= N_DC_TILES)
+                throw ArgError("correction sweep tile must be 0.." + std::to_string(N_DC_TILES - 1));
+            s->dc_tile = static_cast<int>(v);
+            break;
         default:
             throw ArgError("unknown tuning key");
         }
@@ -322,6 +327,8 @@ long get_knob(Batch& b, int key) {
         return s.mid_cells;
     case SG_TUNE_LDS_CELLS:
         return s.lds_cells;
+    case SG_TUNE_DC_TILE:
+        return s.dc_tile;
     default:
         throw ArgError("unknown tuning key");
     }

@@ -1200,12 +1200,22 @@ void launch_mom_residual(Solver& s, bool is_u, double* slot, const int* stop, bo
 }
 
 namespace {
-// Threads of a regular sweep's workgroup (the first sweep, with its extra LDS, uses ST).
+// Threads of a sweep's workgroup: 512 (two workgroups per CU, the VGPR limit), except a first
+// sweep whose two LDS copies and residual buffer would not let two workgroups share a CU: that
+// one takes ST threads, the CU's share in one workgroup.
 constexpr int SW_NT = 512;
+template <int NCOL, int TI, int TJ> constexpr int first_sweep_threads() {
+    constexpr SweepReach<NCOL> RC{};
+    constexpr int R = RC.R;
+    constexpr size_t lds = sizeof(double) * ((TI + RC.load.n + RC.load.s) * (TJ + RC.load.w + RC.load.e) +
+                                             (TI + 2 * R) * (TJ + 2 * R) + TI * TJ);
+    return 2 * lds <= 160 * 1024 ? SW_NT : ST;
+}
 
 template <int NCOL, bool FIRST, bool FACE, int K>
 void sweep_shape(Solver& s, const SweepArgs& a, const int* stop) {
-    constexpr int NT = FIRST ? ST : SW_NT, TI = MOM_TILES[K].ti, TJ = MOM_TILES[K].tj;
+    constexpr int TI = MOM_TILES[K].ti, TJ = MOM_TILES[K].tj;
+    constexpr int NT = FIRST ? first_sweep_threads<NCOL, TI, TJ>() : SW_NT;
     emit<SweepArgs, k_mom_sweep<NCOL, TI, TJ, FIRST, NT, FACE>, NT>(
         s, dim3(cdiv(s.nc, TJ), cdiv(s.nr, TI)), dim3(NT), 0, a, stop);
 }

@@ -113,6 +113,51 @@ __device__ __forceinline__ void k_pc_rb(const RbArgs& g, const Blk& bk) {
     pc_update(a, a.pc, a.pc, static_cast<size_t>(i) * a.ld + j);
 }
 
+// The whole red-black solve of a small grid in one workgroup (config-1 sizes, where the
+// per-colour launches above are pure latency): p' of the interior and its ring of zeros in LDS,
+// (NX + 2) x (NY + 2) doubles; the operator read from global memory (it stays in the caches);
+// maxit_inner_p sweeps of colour 0 then colour 1 with a barrier after every colour pass; the
+// interior written back at the end (p' outside it is zero, as the fill before this launch left
+// it). Every update is pc_update's expression on the same neighbour values in the same colour
+// order, so the result is the per-colour launches' bit for bit.
+constexpr int RB1_T = 1024;
+constexpr size_t RB1_MAX_LDS = 65536; // bytes: within the default dynamic-LDS limit
+struct Rb1Args {
+    PArgs a;
+    int sweeps;
+};
+
+__device__ __forceinline__ void k_pc_rb_one(const Rb1Args& g, const Blk&) {
+    extern __shared__ __align__(16) double sp[];
+    const PArgs& a = g.a;
+    const int NX = a.NX, NY = a.NY, H = a.H, W = NY + 2;
+    const long ld = a.ld;
+    const int n = (NX + 2) * W, half = (NY + 1) / 2;
+    for (int e = threadIdx.x; e < n; e += RB1_T)
+        sp[e] = 0.0;
+    __syncthreads();
+    for (int k = 0; k < g.sweeps; ++k)
+        for (int c = 0; c < 2; ++c) {
+            for (int e = threadIdx.x; e < NX * half; e += RB1_T) {
+                const int r = e / half, i = H + r;
+                const int j = H + 1 + ((i + H + 1 + c) & 1) + 2 * (e - r * half);
+                if (j > NY + H)
+                    continue;
+                const size_t q = static_cast<size_t>(i) * ld + j;
+                double* x = sp + (r + 1) * W + (j - H);
+                double t = a.b[q] / a.AP[q];
+                t += (a.AE[q] * x[1] + a.AW[q] * x[-1] + a.AS[q] * x[W] + a.AN[q] * x[-W]) /
+                     a.AP[q];
+                *x = t;
+            }
+            __syncthreads();
+        }
+    for (int e = threadIdx.x; e < NX * NY; e += RB1_T) {
+        const int r = e / NY, cj = e - r * NY;
+        a.pc[static_cast<size_t>(H + r) * ld + H + 1 + cj] = sp[(r + 1) * W + cj + 1];
+    }
+}
+
 // Damped Jacobi pass, weight 0.8 (src/pressure.cpp:98-104).
 struct PJacArgs {
     PArgs a;
@@ -503,6 +548,14 @@ void launch_remove_mean0(Solver& s, double* x, const int* stop, double* zero) {
                             zero ? s.n : 0);
 }
 
+// One workgroup pays off while a colour pass is at most one update per thread: at 32^2 (config 1)
+// an outer iteration took 0.131 -> 0.109 ms, at 80^2 0.142 -> 0.153 ms (one CU doing 3 updates
+// per thread per pass loses to 16 launches spread over the chip).
+static bool rb_one_fits(const sg_params& p) {
+    return static_cast<long>(p.NX) * p.NY <= 2L * RB1_T &&
+           sizeof(double) * (p.NX + 2) * static_cast<size_t>(p.NY + 2) <= RB1_MAX_LDS;
+}
+
 // solve_p for every GPU pressure solver. CG needs the host between iterations (eager only).
 void launch_solve_p(Solver& s, const int* stop, bool eager) {
     const sg_params& p = s.p;
@@ -510,6 +563,12 @@ void launch_solve_p(Solver& s, const int* stop, bool eager) {
     switch (p.pressure_solver) {
     case SG_P_GS_REDBLACK:
         launch_fill(s, a.pc, s.n, 0.0, stop);
+        if (rb_one_fits(p)) {
+            const size_t lds = sizeof(double) * (p.NX + 2) * (p.NY + 2);
+            emit<Rb1Args, k_pc_rb_one, RB1_T>(s, dim3(1), dim3(RB1_T), lds,
+                                              Rb1Args{a, p.maxit_inner_p}, stop);
+            return;
+        }
         for (int k = 0; k < p.maxit_inner_p; ++k)
             for (int c = 0; c < 2; ++c) {
                 emit<RbArgs, k_pc_rb, BX * BY>(s, dim3(cdiv(cdiv(p.NY, 2), BX), cdiv(p.NX, BY)),

@@ -105,6 +105,19 @@ def test_batch_autotune_keeps_results():
     b.close()
 
 
+def test_batch_tuning_keys_round_trip():
+    """Every launch-shape knob bench.py passes (sg.TUNE_KEYS, the FP32 correction tile included)
+    reads and writes on a batch as on a solver."""
+    b = sg.Batch(_params(SMALL[:2], "mixed", "mg"))
+    t = sg.tuning_of(b)
+    assert len(t) == len(sg.TUNE_KEYS)
+    sg.set_tunings(b, t)
+    assert sg.tuning_of(b) == t
+    b.set_tuning(sg.TUNE_DC_TILE, 2)
+    assert b.get_tuning(sg.TUNE_DC_TILE) == 2
+    b.close()
+
+
 def test_batch_mixed_close_to_fp64_at_tolerance():
     # The workload's first cavities at a quarter of their size (same Re draws), run to 1e-5.
     cases = [(n // 4, re) for n, re in sg.cavities()[:6]]
