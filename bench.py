@@ -66,14 +66,6 @@ def recorded_traffic(n):
     return None
 
 
-def warm_level(n):
-    """Multigrid level the warm-start network runs on: a 256 x 256 input from 1024^2 up."""
-    lv = 2
-    while (n >> (lv + 1)) >= 256 and lv < 8:
-        lv += 1
-    return lv
-
-
 def config_text(n, form, re=1000, maxit=10000000):
     text = (f"N_cells_x = {n}\nN_cells_y = {n}\nRe = {re}\nconvection_scheme = QUICK\n"
             f"diffusion_order = 4\nmomentum_solver = gs_multicolour\npressure_solver = mg\n"
@@ -104,7 +96,10 @@ class Form:
         if form == "learned":
             import warmstart as ws
             self.net = sg.UNet(ws.load(WEIGHTS))
-            sg.attach_warm_start(self.s, self.net, level or warm_level(n))
+            # -1: the library picks the coarsest multigrid level whose grid is still at least
+            # 64 x 64, the size of the coarse grids the network was trained on
+            sg.attach_warm_start(self.s, self.net, -1 if level is None else level)
+            self.level = sg.warm_level(self.s)
 
     def close(self):
         self.s.close()
@@ -146,6 +141,7 @@ def fixed_work(sg, n, form, tune, steps, warmup, level=None):
            "last_residuals": [float(x) for x in lg[-1]]}
     if form == "learned":
         out["guesses_kept_discarded"] = list(sg.warm_stats(f.s))
+        out["warm_level"] = f.level
     f.close()
     return out
 
@@ -172,6 +168,7 @@ def to_tolerance(sg, n, form, tune, chunk=5000, level=None):
            "last_residuals": [float(x) for x in last]}
     if form == "learned":
         out["guesses_kept_discarded"] = list(sg.warm_stats(f.s))
+        out["warm_level"] = f.level
     return f, out
 
 
@@ -213,7 +210,7 @@ def run_cavity(sg, a):
     n = a.n
     tune = autotune(sg, n, a.knobs)
     forms = [f for f in a.forms.split(",") if f]
-    level = a.warm_level or warm_level(n)
+    level = a.warm_level
     table = [fixed_work(sg, n, f, tune[f], a.steps, a.warmup, level) for f in FORMS]
     for r in table:
         log(f"{n}^2 {r['form']}: {r['ms_per_iter']:.4f} ms per outer iteration "
@@ -268,7 +265,8 @@ def run_cavity(sg, a):
                        "per_iteration": table,
                        "knobs": {k: list(v) for k, v in tune.items()},
                        "warm_start": {"weights": os.path.relpath(WEIGHTS, ROOT),
-                                      "level": level}}})
+                                      "level": solvers[forms[-1]].level
+                                      if forms[-1] == "learned" else None}}})
     out["roofline"] = roof
     if not a.no_cpu:
         log("cpu baseline")
@@ -381,7 +379,7 @@ def run_large(sg, steps, warmup, knobs=None):
             r["sweep_u_GBs_algorithmic"] = SWEEP_BYTES_PER_CELL * n * (n - 1) / sec / 1e9
         if form == "learned":
             r["guesses_kept_discarded"] = list(sg.warm_stats(f.s))
-            r["warm_level"] = warm_level(n)
+            r["warm_level"] = f.level
         res.append(r)
         f.close()
     return res
@@ -464,7 +462,8 @@ def main():
     ap.add_argument("--workload", default="cavity",
                     choices=["cavity", "batch", "large", "learned", "unet"])
     ap.add_argument("--warm-level", type=int, default=None,
-                    help="multigrid level the warm-start network runs on (default: 256^2 input)")
+                    help="multigrid level the warm-start network runs on (default: the coarsest "
+                         "level with at least 64 x 64 cells)")
     ap.add_argument("--quick", action="store_true", help="learned: a small, fast version")
     ap.add_argument("--full", action="store_true", help="learned: also 1024^2 to tolerance")
     ap.add_argument("--knobs", default=None,

@@ -288,13 +288,17 @@ int sg_unet_mfma_probe(sg_unet* net, int nimg, int h, int w, int reps, double* s
                        double* flops);
 /* Use net (not owned) ahead of every p' solve of s when s's params say pressure_warm_start =
  * learned (pressure_solver = mg, mg_precision = mixed), on multigrid level `level` (2^level
- * blocks; N_cells_x and N_cells_y multiples of 8 * 2^level). The guess g enters scaled by
+ * blocks; N_cells_x and N_cells_y multiples of 8 * 2^level; -1: the coarsest level whose grid
+ * is at least 64 x 64, the size the network was trained on). The guess g enters scaled by
  * b.g / g.A.g (the energy minimiser along g: never farther from the solution than zero in the
  * A-norm); a degenerate guess is discarded (p' starts from zero). Do not train net while s runs. */
 int sg_attach_warm_start(sg_solver* s, sg_unet* net, int level);
 /* The network input of s's current p' system at `level` (x: (NX/2^level)*(NY/2^level)*8 floats)
  * and the scale that turns a prediction into p'. */
 int sg_warm_features(sg_solver* s, int level, float* x, double* scale);
+/* The level the attached warm start runs on (-1 when none is attached). No counterpart in the
+ * reference (the warm start is an addition); reports what level -1 resolved to. */
+int sg_warm_level(sg_solver* s, int* level);
 /* Guesses kept and discarded so far. */
 int sg_warm_stats(sg_solver* s, int* accepted, int* rejected);
 

@@ -119,20 +119,31 @@ struct ConvArgs {
     int ldy;         // output view
     int n, h, wd, cin, cout;
     int relu;
+    const u16* zero; // 16 zero bytes: the operand of taps outside the image and steps past K
 };
 
 constexpr int CV_T = 256; // 4 waves; a workgroup covers 64 output channels
 
-// CPW = column blocks (16 output channels each) per wave. CPW = 4: a wave computes 16 pixels x 64
-// channels and a workgroup 64 pixels (large images: the A fragment is loaded once for 4 MFMAs).
-// CPW = 1: a wave computes 16 pixels x 16 channels and a workgroup 16 pixels x 64 channels (small
-// images: 4x more waves, 4x shorter dependent MFMA chains).
-template <int CPW> __global__ void __launch_bounds__(CV_T) k_conv(ConvArgs a) {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+// NB = column blocks (16 output channels each) per wave, PIXW = pixel blocks (16 pixels each) per
+// workgroup. PIXW = 4 (large images): each wave computes 16 pixels x 16 NB channels and the
+// workgroup 64 pixels (the A fragment is loaded once for NB MFMAs). PIXW = 1 (small images): the
+// four waves of a workgroup take 16 pixels x 16 channels each (4x more waves, 4x shorter
+// dependent MFMA chains). The host picks NB so that 16 NB divides the output channels.
+// The K loop runs in groups of D steps, double-buffered in registers: the loads of group g + 1
+// are issued before the MFMAs of group g, so a wave waits on one load latency per group instead
+// of one per step. Every load is unconditional (an operand outside the image or past K is read
+// from a zero buffer: the choice is made on the address, so nothing waits on the data before its
+// MFMA). The MFMAs run in the same order on the same operands as one step at a time: the same
+// accumulated bits.
+template <int NB, int PIXW> __global__ void __launch_bounds__(CV_T) k_conv(ConvArgs a) {
+    static_assert((PIXW == 4 && (NB == 1 || NB == 2 || NB == 4)) || (PIXW == 1 && NB == 1),
+                  "launch forms");
+    constexpr int D = NB == 4 ? 2 : 4;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int m = lane & 15, oct = lane >> 4;
     const int npix = a.n * a.h * a.wd;
-    const int pix0 = CPW == 4 ? (blockIdx.x * 4 + wave) * 16 : blockIdx.x * 16;
-    const int co0 = blockIdx.y * 64 + (CPW == 4 ? 0 : wave * 16);
+    const int pix0 = PIXW == 4 ? (blockIdx.x * 4 + wave) * 16 : blockIdx.x * 16;
+    const int co0 = PIXW == 4 ? blockIdx.y * 16 * NB : blockIdx.y * 64 + wave * 16;
     if (co0 >= a.cout)
         return;
     const int p = pix0 + m;
@@ -144,34 +155,65 @@ template <int CPW> __global__ void __launch_bounds__(CV_T) k_conv(ConvArgs a) {
         pi = r % a.h;
         pn = r / a.h;
     }
-    f4 acc[CPW];
+    f4 acc[NB];
 #pragma unroll
-    for (int c = 0; c < CPW; ++c)
+    for (int c = 0; c < NB; ++c)
         acc[c] = f4{0.0f, 0.0f, 0.0f, 0.0f};
-    const int K = 9 * a.cin;
-    for (int k0 = 0; k0 < a.kpad; k0 += 32) {
-        const int kk = k0 + oct * 8;
-        bf16x8 A = zero8();
-        if (pv && kk < K) {
-            const int tap = kk / a.cin, ci = kk - tap * a.cin;
-            const int ii = pi + tap / 3 - 1, jj = pj + tap % 3 - 1;
-            if (ii >= 0 && ii < a.h && jj >= 0 && jj < a.wd)
-                A = ld8(a.x + (static_cast<size_t>(pn * a.h + ii) * a.wd + jj) * a.ldx + ci);
-        }
+    const int K = 9 * a.cin, nk = a.kpad >> 5;
+    const u16* wrow[NB];
 #pragma unroll
-        for (int c = 0; c < CPW; ++c) {
-            if (co0 + c * 16 >= a.cout) // wave-uniform: no MFMA for absent channel blocks
-                break;
-            const int co = co0 + c * 16 + m;
-            const bf16x8 B = co < a.cout ? ld8(a.w + static_cast<size_t>(co) * a.kpad + kk) : zero8();
-            acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[c], 0, 0, 0);
+    for (int c = 0; c < NB; ++c)
+        wrow[c] = a.w + static_cast<size_t>(co0 + c * 16 + m) * a.kpad + oct * 8;
+    // kk / cin in FP32: kk < 2^12 and cin <= 2^8, so (kk + 0.5) / cin lies at least 1 / (2 cin)
+    // from an integer, far more than the rounding error.
+    const float rc = 1.0f / static_cast<float>(a.cin);
+    // Element offsets fit 32 bits (the host checks npix * ldx < 2^31).
+    const int img = pn * a.h;
+    auto load_a = [&](int s) {
+        const int kk = s * 32 + oct * 8;
+        const int tap = static_cast<int>((static_cast<float>(kk) + 0.5f) * rc);
+        const int ci = kk - tap * a.cin;
+        const int dr = (tap * 11) >> 5; // tap / 3 for tap < 9
+        const int ii = pi + dr - 1, jj = pj + (tap - 3 * dr) - 1;
+        const bool ok = pv & (kk < K) & (ii >= 0) & (ii < a.h) & (jj >= 0) & (jj < a.wd);
+        return ld8(ok ? a.x + ((img + ii) * a.wd + jj) * a.ldx + ci : a.zero);
+    };
+    bf16x8 A0[D], A1[D], B0[D][NB], B1[D][NB];
+    auto load_group = [&](int g, bf16x8 (&A)[D], bf16x8 (&B)[D][NB]) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const int s = min(g * D + d, nk - 1);
+            A[d] = load_a(s);
+#pragma unroll
+            for (int c = 0; c < NB; ++c)
+                B[d][c] = ld8(wrow[c] + s * 32);
         }
+    };
+    auto mma_group = [&](int g, const bf16x8 (&A)[D], const bf16x8 (&B)[D][NB]) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            if (g * D + d >= nk)
+                break;
+#pragma unroll
+            for (int c = 0; c < NB; ++c)
+                acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[d], B[d][c], acc[c], 0, 0, 0);
+        }
+    };
+    // The next group is loaded whether or not it exists (its steps clamp to the last one), so
+    // every path into an MFMA has the same loads in flight and the waits count only its own.
+    const int ng = (nk + D - 1) / D;
+    load_group(0, A0, B0);
+    for (int g = 0; g < ng; g += 2) {
+        load_group(g + 1, A1, B1);
+        mma_group(g, A0, B0);
+        if (g + 1 >= ng)
+            break;
+        load_group(g + 2, A0, B0);
+        mma_group(g + 1, A1, B1);
     }
 #pragma unroll
-    for (int c = 0; c < CPW; ++c) {
+    for (int c = 0; c < NB; ++c) {
         const int co = co0 + c * 16 + m;
-        if (co >= a.cout)
-            continue;
         const float b = a.bias ? a.bias[co] : 0.0f;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -501,6 +543,7 @@ struct UNet {
     int kpad[NCONV] = {}, kpadt[NCONV] = {};
     float *master = nullptr, *grad = nullptr, *m1 = nullptr, *m2 = nullptr;
     u16 *wf[NCONV] = {}, *wt[NCONV] = {};
+    u16* zero16 = nullptr; // 16 zero bytes (alloc zero-fills): the convolution's padding operand
     int step = 0;
     int device = 0; // the device the network's arrays live on (current at creation)
     hipStream_t stream = nullptr;
@@ -559,6 +602,7 @@ struct UNet {
             wf[k] = alloc<u16>(owned, static_cast<size_t>(SPEC[k].cout) * kpad[k]);
             wt[k] = alloc<u16>(owned, static_cast<size_t>(SPEC[k].cin) * kpadt[k]);
         }
+        zero16 = alloc<u16>(owned, 8);
     }
     ~UNet() {
         if (stream)
@@ -712,13 +756,28 @@ struct UNet {
             a.relu = 0;
         }
         const long npix = static_cast<long>(n) * h * w;
-        // 64 pixels x 64 channels per workgroup when that fills the GPU, else 16 x 64.
-        if (static_cast<long>(g1(npix, 64)) * cdiv(a.cout, 64) >= 1024)
-            hipLaunchKernelGGL(k_conv<4>, dim3(g1(npix, 64), cdiv(a.cout, 64)), dim3(CV_T), 0, st,
-                               a);
-        else
-            hipLaunchKernelGGL(k_conv<1>, dim3(g1(npix, 16), cdiv(a.cout, 64)), dim3(CV_T), 0, st,
-                               a);
+        a.zero = zero16;
+        if (a.cout % 16 != 0 || a.cin % 8 != 0 || ldx % 8 != 0 || ldy % 8 != 0 || a.kpad % 32 != 0 ||
+            9 * a.cin > a.kpad || reinterpret_cast<uintptr_t>(x) % 16 != 0 ||
+            npix * std::max(ldx, ldy) >= (1L << 31))
+            throw ArgError("convolution: 16-byte aligned views, input channels in multiples of 8, "
+                           "output channels in multiples of 16");
+        // 64 pixels per workgroup (each wave up to 64 channels) when that fills the GPU, else 16
+        // pixels x 64 channels.
+        if (static_cast<long>(g1(npix, 64)) * cdiv(a.cout, 64) >= 1024) {
+            if (a.cout % 64 == 0)
+                hipLaunchKernelGGL((k_conv<4, 4>), dim3(g1(npix, 64), a.cout / 64), dim3(CV_T), 0,
+                                   st, a);
+            else if (a.cout % 32 == 0)
+                hipLaunchKernelGGL((k_conv<2, 4>), dim3(g1(npix, 64), a.cout / 32), dim3(CV_T), 0,
+                                   st, a);
+            else
+                hipLaunchKernelGGL((k_conv<1, 4>), dim3(g1(npix, 64), a.cout / 16), dim3(CV_T), 0,
+                                   st, a);
+        } else {
+            hipLaunchKernelGGL((k_conv<1, 1>), dim3(g1(npix, 16), cdiv(a.cout, 64)), dim3(CV_T), 0,
+                               st, a);
+        }
         SG_HIP(hipGetLastError());
     }
 
@@ -903,9 +962,41 @@ struct FeatArgs {
     int ld, nx, ny, bs; // fine interior size, block side
     float* cb;          // coarse block sums of b [nxc*nyc]
     float* ca;          // coarse block means of a_P
+    double* rows;       // block row segment sums of b and a_P, interleaved [nx*nyc][2]
 };
 
-// Coarse block sums / means (one thread per coarse cell, rows then columns of the block).
+// Coarse block sums of b and block means of a_P in two launches, each with many threads: the
+// sum of every block row segment (one thread per fine row and coarse column), then the sum of a
+// block's row sums in row order (one thread per coarse cell). Loads go 16 at a time before they
+// are added in order, so a large block (16 x 16 at level 4) waits on one load latency per 16
+// values.
+__global__ void k_feat_rows(FeatArgs a) {
+    const int nyc = a.ny / a.bs;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= a.nx * nyc)
+        return;
+    const int i = e / nyc, J = e - i * nyc;
+    const size_t q0 = static_cast<size_t>(i) * a.ld + J * a.bs;
+    double sb = 0.0, sa = 0.0;
+    for (int j0 = 0; j0 < a.bs; j0 += 16) {
+        double vb[16], va[16];
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            const bool in = j0 + t < a.bs;
+            vb[t] = in ? a.b[q0 + j0 + t] : 0.0;
+            va[t] = in ? a.ap[q0 + j0 + t] : 0.0;
+        }
+#pragma unroll
+        for (int t = 0; t < 16; ++t)
+            if (j0 + t < a.bs) {
+                sb += vb[t];
+                sa += va[t];
+            }
+    }
+    a.rows[2 * static_cast<size_t>(e)] = sb;
+    a.rows[2 * static_cast<size_t>(e) + 1] = sa;
+}
+
 __global__ void k_feat_blocks(FeatArgs a) {
     const int nyc = a.ny / a.bs, nxc = a.nx / a.bs;
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -913,12 +1004,20 @@ __global__ void k_feat_blocks(FeatArgs a) {
         return;
     const int I = e / nyc, J = e - I * nyc;
     double sb = 0.0, sa = 0.0;
-    for (int i = 0; i < a.bs; ++i)
-        for (int j = 0; j < a.bs; ++j) {
-            const size_t q = static_cast<size_t>(I * a.bs + i) * a.ld + J * a.bs + j;
-            sb += a.b[q];
-            sa += a.ap[q];
+    for (int i0 = 0; i0 < a.bs; i0 += 16) {
+        double2 v[16];
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            const size_t r = static_cast<size_t>(I * a.bs + i0 + t) * nyc + J;
+            v[t] = i0 + t < a.bs ? reinterpret_cast<const double2*>(a.rows)[r] : double2{0.0, 0.0};
         }
+#pragma unroll
+        for (int t = 0; t < 16; ++t)
+            if (i0 + t < a.bs) {
+                sb += v[t].x;
+                sa += v[t].y;
+            }
+    }
     a.cb[e] = static_cast<float>(sb);
     a.ca[e] = static_cast<float>(sa / (a.bs * a.bs));
 }
@@ -928,7 +1027,22 @@ __global__ void __launch_bounds__(1024) k_feat_stats(const float* cb, const floa
                                                      double* out) {
     __shared__ double s1[1024], s2[1024];
     double a = 0.0, b = 0.0;
-    for (int e = threadIdx.x; e < n; e += 1024) {
+    // Each thread adds its elements in index order; eight loads are in flight at a time.
+    int e = threadIdx.x;
+    for (; e + 7 * 1024 < n; e += 8 * 1024) {
+        float vb[8], va[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            vb[q] = cb[e + q * 1024];
+            va[q] = ca[e + q * 1024];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            a += static_cast<double>(vb[q]) * vb[q];
+            b += va[q];
+        }
+    }
+    for (; e < n; e += 1024) {
         a += static_cast<double>(cb[e]) * cb[e];
         b += ca[e];
     }
@@ -987,12 +1101,13 @@ struct GuessArgs {
     Ctl* ctl;
 };
 
-// p' = scale * the prediction interpolated bilinearly from the coarse cell centres (clamped at
-// the walls) to the fine cell centres; the halo stays zero.
-__device__ __forceinline__ void k_guess_write(const GuessArgs& a, const Blk& bk) {
-    const int j = bk.x * 64 + threadIdx.x, i = bk.y * 4 + threadIdx.y;
-    if (i >= a.nx || j >= a.ny)
-        return;
+// The guess g at interior cell (i, j): scale * the prediction interpolated bilinearly from the
+// coarse cell centres (clamped at the walls) to the fine cell centre; 0 outside the interior (p''s
+// halo stays zero). Evaluated where it is needed (the energy sums and the final write), so it is
+// never stored and read back.
+__device__ __forceinline__ double guess_at(const GuessArgs& a, int i, int j) {
+    if (i < 0 || j < 0 || i >= a.nx || j >= a.ny)
+        return 0.0;
     const int nxc = a.nx / a.bs, nyc = a.ny / a.bs;
     const float fi = (i + 0.5f) / a.bs - 0.5f, fj = (j + 0.5f) / a.bs - 0.5f;
     const float ci = fminf(fmaxf(fi, 0.0f), static_cast<float>(nxc - 1));
@@ -1003,7 +1118,7 @@ __device__ __forceinline__ void k_guess_write(const GuessArgs& a, const Blk& bk)
     const float* y = a.y;
     const float v = (1.0f - wi) * ((1.0f - wj) * y[i0 * nyc + j0] + wj * y[i0 * nyc + j1]) +
                     wi * ((1.0f - wj) * y[i1 * nyc + j0] + wj * y[i1 * nyc + j1]);
-    a.pc[static_cast<size_t>(i) * a.ld + j] = static_cast<double>(v) * a.st[2];
+    return static_cast<double>(v) * a.st[2];
 }
 
 // Per row: sums of g (A g) and b g for the guess g (one workgroup per row, fixed-order tree).
@@ -1014,11 +1129,13 @@ __device__ __forceinline__ void k_guess_rows(const GuessArgs& a, const Blk& bk) 
     double d2 = 0.0, b2 = 0.0;
     for (int j = threadIdx.x; j < a.ny; j += 256) {
         const size_t q = static_cast<size_t>(i) * s + j;
-        const double* x = a.pc;
-        const double ag = a.AP[q] * x[q] - (a.AE[q] * x[q + 1] + a.AW[q] * x[q - 1] +
-                                            a.AS[q] * x[q + s] + a.AN[q] * x[q - s]);
-        d2 += x[q] * ag;
-        b2 += a.b[q] * x[q];
+        const double x = guess_at(a, i, j);
+        const double ag = a.AP[q] * x - (a.AE[q] * guess_at(a, i, j + 1) +
+                                         a.AW[q] * guess_at(a, i, j - 1) +
+                                         a.AS[q] * guess_at(a, i + 1, j) +
+                                         a.AN[q] * guess_at(a, i - 1, j));
+        d2 += x * ag;
+        b2 += a.b[q] * x;
     }
     sd[threadIdx.x] = d2;
     sb[threadIdx.x] = b2;
@@ -1049,8 +1166,19 @@ __device__ __forceinline__ void k_guess_decide(const GuessArgs& a, const Blk&) {
     double gag = 0.0, bg = 0.0;
     for (int c0 = 0; c0 < a.nx; c0 += GD_CHUNK) {
         const int n = min(GD_CHUNK, a.nx - c0);
-        for (int e = threadIdx.x; e < 2 * n; e += blockDim.x)
-            sp[e] = a.part[2 * static_cast<size_t>(c0) + e];
+        // The chunk's loads are all in flight before the first LDS store (64 lanes, 32 each).
+        {
+            constexpr int PER = 2 * GD_CHUNK / 64;
+            double v[PER];
+#pragma unroll
+            for (int q = 0; q < PER; ++q) {
+                const int e = q * 64 + threadIdx.x;
+                v[q] = e < 2 * n ? a.part[2 * static_cast<size_t>(c0) + e] : 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < PER; ++q)
+                sp[q * 64 + threadIdx.x] = v[q];
+        }
         __syncthreads();
         if (threadIdx.x == 0) {
             // 16 (g.A.g, b.g) pairs at a time into registers, then added in order.
@@ -1087,13 +1215,12 @@ __device__ __forceinline__ void k_guess_decide(const GuessArgs& a, const Blk&) {
         ++a.ctl->warm_rejected;
 }
 
-// p' = alpha g (0 if the guess was discarded).
+// p' = alpha g over the interior (0 if the guess was discarded: it may hold non-finite values).
 __device__ __forceinline__ void k_guess_scale(const GuessArgs& a, const Blk& bk) {
     const int j = bk.x * 64 + threadIdx.x, i = bk.y * 4 + threadIdx.y;
     if (i >= a.nx || j >= a.ny)
         return;
-    double& x = a.pc[static_cast<size_t>(i) * a.ld + j];
-    x = *a.flag ? x * a.part[2 * a.nx] : 0.0; // a discarded guess may hold non-finite values
+    a.pc[static_cast<size_t>(i) * a.ld + j] = *a.flag ? guess_at(a, i, j) * a.part[2 * a.nx] : 0.0;
 }
 
 } // namespace
@@ -1105,6 +1232,7 @@ struct Warm {
     float *cb = nullptr, *ca = nullptr;
     double* st = nullptr;
     double* part = nullptr;
+    double* rows = nullptr;
     int* flag = nullptr;
     float* xf = nullptr;
     std::vector<void*> owned;
@@ -1122,8 +1250,11 @@ void warm_free(Solver& s) {
 static void feat_launch(Solver& s, hipStream_t st) {
     Warm& w = *s.warm;
     const MgLevel& L0 = s.mg[0];
-    FeatArgs f{L0.b, L0.AP, s.ld, s.p.NX, s.p.NY, w.bs, w.cb, w.ca};
+    FeatArgs f{L0.b, L0.AP, s.ld, s.p.NX, s.p.NY, w.bs, w.cb, w.ca, w.rows};
     const int nc = w.nxc * w.nyc;
+    hipLaunchKernelGGL(k_feat_rows, dim3(g1(static_cast<long>(s.p.NX) * w.nyc)), dim3(256), 0, st,
+                       f);
+    SG_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_feat_blocks, dim3(g1(nc)), dim3(256), 0, st, f);
     SG_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_feat_stats, dim3(1), dim3(1024), 0, st, w.cb, w.ca, nc, w.st);
@@ -1133,8 +1264,17 @@ static void feat_launch(Solver& s, hipStream_t st) {
 void warm_attach(Solver& s, UNet* net, int level) {
     if (s.p.pressure_solver != SG_P_MG)
         throw ArgError("the learned warm start needs pressure_solver = mg");
-    if (level < 0 || level > 8)
-        throw ArgError("warm-start level must be in 0..8");
+    if (level < -1 || level > 8)
+        throw ArgError("warm-start level must be -1 (automatic) or in 0..8");
+    if (level < 0) {
+        // The coarsest level whose grid is still at least 64 x 64 on both sides: the size of the
+        // coarse grids the network was trained on (32^2 to 128^2).
+        level = 0;
+        const int nmin = std::min(s.p.NX, s.p.NY);
+        while (level < 8 && (nmin >> (level + 1)) >= 64 && s.p.NX % (8 << (level + 1)) == 0 &&
+               s.p.NY % (8 << (level + 1)) == 0)
+            ++level;
+    }
     const int bs = 1 << level;
     if (s.p.NX % (8 * bs) || s.p.NY % (8 * bs))
         throw ArgError("warm start: N_cells_x and N_cells_y must be multiples of 8 * 2^level");
@@ -1159,6 +1299,7 @@ void warm_attach(Solver& s, UNet* net, int level) {
     w.st = static_cast<double*>(al(4 * sizeof(double)));
     w.part = static_cast<double*>(al((2 * static_cast<size_t>(s.p.NX) + 1) * sizeof(double)));
     w.flag = static_cast<int*>(al(sizeof(int)));
+    w.rows = static_cast<double*>(al(2 * sizeof(double) * s.p.NX * static_cast<size_t>(w.nyc)));
     w.xf = static_cast<float*>(al(nc * IN_PAD * sizeof(float)));
     if (net)
         net->get_ctx(1, w.nxc, w.nyc); // workspace exists before any capture
@@ -1186,7 +1327,6 @@ void launch_warm_guess(Solver& s, const int* stop) {
     GuessArgs g{c.y,  w.st, L0.x,    L0.AE,   L0.AW, L0.AN, L0.AS, L0.AP, L0.b,
                 s.ld, s.p.NX, s.p.NY, w.bs, w.part, w.flag, s.ctl};
     const dim3 grid(cdiv(s.p.NY, 64), cdiv(s.p.NX, 4));
-    emit<GuessArgs, k_guess_write, 256>(s, grid, dim3(64, 4), 0, g, stop);
     emit<GuessArgs, k_guess_rows, 256>(s, dim3(s.p.NX), dim3(256), 0, g, stop);
     emit<GuessArgs, k_guess_decide, 64>(s, dim3(1), dim3(64), 0, g, stop);
     emit<GuessArgs, k_guess_scale, 256>(s, grid, dim3(64, 4), 0, g, stop);
@@ -1439,6 +1579,15 @@ int sg_warm_features(sg_solver* sh, int level, float* x, double* scale) {
         SG_HIP(hipStreamSynchronize(s.stream));
         if (scale)
             *scale = st[2];
+    });
+}
+
+int sg_warm_level(sg_solver* sh, int* level) {
+    return uguard([&] {
+        if (!sh || !level)
+            throw ArgError("null pointer");
+        const Solver& s = *reinterpret_cast<Solver*>(sh);
+        *level = s.warm ? s.warm->level : -1;
     });
 }
 

@@ -27,7 +27,7 @@ struct Params {
     std::string mpi_grid = "auto";      // accepted for config compatibility; one GPU runs 1x1
     bool halo_overlap = true;
     std::string warm_start_weights;     // pressure_warm_start = learned: network weights file
-    int warm_start_level = 2;           // multigrid level the network runs on (2^level blocks)
+    int warm_start_level = -1; // multigrid level the network runs on (2^level blocks; -1: automatic)
     std::string field_format = "csv";   // field files: csv (the reference's), or npy (binary)
 };
 
@@ -54,6 +54,7 @@ struct RunResult {
     long long cg_iterations = 0;
     std::string device;
     int warm_kept = 0, warm_discarded = 0; // learned warm start: guesses used / discarded
+    int warm_level = -1;                   // the level it ran on
     int phase_sampled = 0;              // replayed iterations timed per phase on the device
 };
 

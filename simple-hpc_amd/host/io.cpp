@@ -176,7 +176,7 @@ void write_outputs(const Params& P, const std::vector<double>& u, const std::vec
     if (p.pressure_warm_start == SG_WARM_LEARNED)
         std::fprintf(fp, "  \"warm_start\": {\"weights\": \"%s\", \"level\": %d, \"kept\": %d, "
                          "\"discarded\": %d},\n",
-                     json_escape(P.warm_start_weights).c_str(), P.warm_start_level, r.warm_kept,
+                     json_escape(P.warm_start_weights).c_str(), r.warm_level, r.warm_kept,
                      r.warm_discarded);
     std::fprintf(fp,
                  "  \"iterations\": %d,\n  \"converged\": %s,\n  \"seconds\": %.6f,\n"

@@ -103,8 +103,10 @@ RunResult run_simple_gpu(const Params& P, std::vector<double>& u, std::vector<do
             break;
     }
     out.iterations = done;
-    if (h.net)
+    if (h.net) {
         check(sg_warm_stats(h.s, &out.warm_kept, &out.warm_discarded));
+        check(sg_warm_level(h.s, &out.warm_level));
+    }
     check(sg_download(h.s, SG_F_U, u.data()));
     check(sg_download(h.s, SG_F_V, v.data()));
     check(sg_download(h.s, SG_F_P, pf.data()));

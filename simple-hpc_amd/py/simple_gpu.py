@@ -188,6 +188,7 @@ def gpu():
         g.sg_phase_times.argtypes = [vp, dp, ip]
         g.sg_warm_features.argtypes = [vp, C.c_int, fp, dp]
         g.sg_warm_stats.argtypes = [vp, ip, ip]
+        g.sg_warm_level.argtypes = [vp, ip]
         _gpu = g
     return _gpu
 
@@ -404,7 +405,7 @@ class UNet:
         return sec.value, fl.value
 
 
-def attach_warm_start(solver, net, level=2):
+def attach_warm_start(solver, net, level=-1):
     check(gpu().sg_attach_warm_start(solver.h, net.h if net is not None else None, level))
 
 
@@ -416,6 +417,13 @@ def warm_features(solver, level=2):
     sc = C.c_double()
     check(gpu().sg_warm_features(solver.h, level, _fptr(x), C.byref(sc)))
     return x, sc.value
+
+
+def warm_level(solver):
+    """The multigrid level the attached warm start runs on (-1: none attached)."""
+    lv = C.c_int(0)
+    check(gpu().sg_warm_level(solver.h, C.byref(lv)))
+    return lv.value
 
 
 def warm_stats(solver):

@@ -211,7 +211,7 @@ def _mixed_params(n, re, learned, cycles=1, tol=1e-5, check_every=200):
     return sg.params_from_config(text)[0]
 
 
-def ab_per_solve(net, n, re, snapshots, level=2):
+def ab_per_solve(net, n, re, snapshots, level=-1):
     """Error of p' after one mixed-precision p' solve, started from zero and from the network's
     guess, against the converged p' of the same system, on states of an n x n run at Re.
     Returns a list of (iteration, error zero start, error warm start, guess kept)."""
@@ -241,7 +241,7 @@ def ab_per_solve(net, n, re, snapshots, level=2):
     return out, (acc, rej)
 
 
-def ab_to_tolerance(net, n, re, level=2, cap=10000000, log=print):
+def ab_to_tolerance(net, n, re, level=-1, cap=10000000, log=print):
     """Outer iterations and seconds to the tolerance, mixed vs learned + mixed."""
     res = {}
     for form in ("mixed", "learned"):

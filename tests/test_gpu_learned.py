@@ -48,15 +48,18 @@ def net():
     u.close()
 
 
-@pytest.mark.parametrize("n,snaps", [(256, [10, 100, 1000, 4000]), (1024, [10, 100])])
+@pytest.mark.parametrize("n,snaps", [(256, [10, 100, 1000, 4000]), (1024, [10, 100, 1000])])
 def test_warm_start_gain_per_solve(net, n, snaps):
-    per, (kept, rej) = ws.ab_per_solve(net, n, 1000.0, snaps, level=2)
+    """The network on the automatic level (a 64 x 64 coarse grid: level 2 at 256^2, level 4 at
+    1024^2) cuts the p' error of one solve to at most a quarter of the zero start's (measured:
+    0.05-0.11x at 256^2, 0.12-0.16x at 1024^2)."""
+    per, (kept, rej) = ws.ab_per_solve(net, n, 1000.0, snaps)
     for snap, e0, e1 in per:
         print(f"{n}^2 iteration {snap}: p' error after one solve from zero {e0:.3f}, "
               f"from the network's guess {e1:.3f} ({e1 / e0:.2f}x)", flush=True)
     assert kept == len(snaps) and rej == 0
     for snap, e0, e1 in per:
-        assert e1 < 0.95 * e0, (n, snap, e0, e1)
+        assert e1 < 0.25 * e0, (n, snap, e0, e1)
 
 
 def _to_tolerance(p, net=None):
