@@ -399,6 +399,21 @@ int sg_params_finalize(sg_params* p) {
 
 namespace sg {
 
+// Launch shapes before any autotuning (results do not depend on them): what the autotuner chose
+// on an MI355X for the 1024^2 benchmark workload and at 64^2 to 512^2. Every size kept the same
+// multigrid knobs (a 1024-thread tail holding only the levels that fit in LDS, no
+// intermediate levels in that workgroup); the sweep tile grows with the grid: 16 x 128 at 64^2,
+// 24 x 64 up to 512^2, 40 x 56 (first sweep 28 x 80) at 1024^2.
+void default_launch_shapes(Solver& s) {
+    s.tail_threads = 1024;
+    s.mid_cells = 0;
+    s.lds_cells = 4096;
+    s.mg_staged = 0;
+    const long cells = static_cast<long>(s.p.NX) * s.p.NY;
+    s.mom_tile = cells <= 96L * 96 ? 3 : cells < 768L * 768 ? 6 : 11;
+    s.mom_tile_first = cells < 768L * 768 ? -1 : 9;
+}
+
 Solver* create_solver(const sg_params& p, const double* u, const double* v, const double* pf) {
     validate(p);
     if (p.nrows != p.NX + 2 * p.H + 1 || p.ncols != p.NY + 2 * p.H + 1)
@@ -422,6 +437,7 @@ Solver* create_solver(const sg_params& p, const double* u, const double* v, cons
             SG_HIP(hipGetDeviceProperties(&prop, s.device));
             s.lds_budget = std::min<size_t>(prop.sharedMemPerBlock, 160 * 1024) - 1024;
         }
+        default_launch_shapes(s);
         s.nr = p.nrows;
         s.nc = p.ncols;
         s.ld = (p.ncols + 15) / 16 * 16;

@@ -415,6 +415,17 @@ int sg_batch_create(const sg_params* ps, int n, sg_batch** out) {
             b.cav.push_back(create_solver(ps[c], nullptr, nullptr, nullptr));
             ensure_log(*b.cav.back(), std::max(256, ps[c].check_every));
         }
+        // The batched launches share one set of launch shapes: the largest cavity's defaults.
+        const Solver* big = b.cav[0];
+        for (const Solver* s : b.cav)
+            if (static_cast<long>(s->p.NX) * s->p.NY > static_cast<long>(big->p.NX) * big->p.NY)
+                big = s;
+        for (Solver* s : b.cav) {
+            s->tail_threads = big->tail_threads, s->mid_cells = big->mid_cells;
+            s->lds_cells = big->lds_cells, s->mg_staged = big->mg_staged;
+            s->mom_tile = big->mom_tile, s->mom_tile_first = big->mom_tile_first;
+            s->dc_tile = big->dc_tile;
+        }
         int dev = 0;
         SG_HIP(hipGetDevice(&dev));
         SG_HIP(hipDeviceGetAttribute(&b.cus, hipDeviceAttributeMultiprocessorCount, dev));

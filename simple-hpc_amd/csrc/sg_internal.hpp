@@ -127,8 +127,9 @@ struct Solver {
     double* mg_r0 = nullptr;           // level-0 residual
     // Multigrid tail (tunable; sg_tune): levels with at most mid_cells cells run in one workgroup
     // of tail_threads threads, those with at most lds_cells cells (and fitting lds_budget) in LDS.
-    long mid_cells = 16384, lds_cells = 4096;
-    int tail_threads = 256;
+    // Defaults for every launch-shape knob: default_launch_shapes.
+    long mid_cells = 0, lds_cells = 4096;
+    int tail_threads = 1024;
     bool mg_tiled = true;              // multigrid levels above the tail: down/up tile kernels
     int mg_staged = 0;                 // first level whose tiles stage the operator in LDS too
     size_t lds_budget = 65536 - 1024;  // dynamic LDS of the multigrid tail (set from the device)
@@ -357,6 +358,7 @@ void launch_mg_solve(Solver& s, const int* stop, bool has_guess);
 void launch_remove_mean0(Solver& s, double* a, const int* stop, double* zero = nullptr);
 // learned warm start (sg_unet.hip): p' guess into the level-0 x, kept only if it lowers the defect
 void launch_warm_guess(Solver& s, const int* stop);
+void default_launch_shapes(Solver& s);
 void warm_free(Solver& s);
 // outer loop control (sg_api.hip)
 

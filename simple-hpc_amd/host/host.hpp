@@ -29,6 +29,8 @@ struct Params {
     std::string warm_start_weights;     // pressure_warm_start = learned: network weights file
     int warm_start_level = -1; // multigrid level the network runs on (2^level blocks; -1: automatic)
     std::string field_format = "csv";   // field files: csv (the reference's), or npy (binary)
+    bool autotune = true;               // time the launch shapes on the running state after
+                                        // iteration 1 (results do not depend on them)
 };
 
 // Known keys only, typed parsing, validation and derived sizes (src/params.cpp:50-180), plus this
@@ -56,6 +58,7 @@ struct RunResult {
     int warm_kept = 0, warm_discarded = 0; // learned warm start: guesses used / discarded
     int warm_level = -1;                   // the level it ran on
     int phase_sampled = 0;              // replayed iterations timed per phase on the device
+    double autotune_s = -1.0;           // seconds the autotuner took (-1: not run)
 };
 
 // C ABI status -> std::runtime_error with sg_last_error() (the "GPU error: ..." path of

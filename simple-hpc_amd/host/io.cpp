@@ -173,6 +173,7 @@ void write_outputs(const Params& P, const std::vector<double>& u, const std::vec
                  0.0, 0.0);
     std::fprintf(fp, "  \"diverged\": %s,\n", r.diverged ? "true" : "false");
     std::fprintf(fp, "  \"phase_timed_iterations\": %d,\n", r.phase_sampled);
+    std::fprintf(fp, "  \"autotune_seconds\": %.6f,\n", r.autotune_s);
     if (p.pressure_warm_start == SG_WARM_LEARNED)
         std::fprintf(fp, "  \"warm_start\": {\"weights\": \"%s\", \"level\": %d, \"kept\": %d, "
                          "\"discarded\": %d},\n",

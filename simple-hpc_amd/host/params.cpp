@@ -62,7 +62,7 @@ const std::set<std::string>& known_keys() {
         "mpi_grid", "halo_overlap", "write_fields",
         // additions
         "mg_precision", "mg_coarsest", "mg_dc_maxit", "pressure_warm_start",
-        "warm_start_weights", "warm_start_level", "check_every", "field_format",
+        "warm_start_weights", "warm_start_level", "check_every", "field_format", "autotune",
         "momentum_precision"};
     return k;
 }
@@ -161,6 +161,7 @@ Params make_params(const ConfigMap& cfg) {
         else if (k == "warm_start_weights") P.warm_start_weights = v;
         else if (k == "warm_start_level") P.warm_start_level = as_int(k, v);
         else if (k == "check_every") p.check_every = as_int(k, v);
+        else if (k == "autotune") P.autotune = as_bool(k, v);
         else if (k == "field_format") {
             if (v != "csv" && v != "npy")
                 throw std::runtime_error("field_format must be csv or npy: " + v);
@@ -201,7 +202,7 @@ void print_params(std::ostream& os, const Params& P) {
        << "\nmomentum_precision = " << (p.momentum_precision == SG_MOMP_MIXED ? "mixed" : "fp64")
        << "\nmg_coarsest = " << p.mg_coarsest
        << "\npressure_warm_start = " << (p.pressure_warm_start == SG_WARM_LEARNED ? "learned" : "none")
-       << "\n";
+       << "\nautotune = " << (P.autotune ? "True" : "False") << "\n";
 }
 
 const char* phase_name(int k) {
