@@ -23,8 +23,9 @@ line:
     solver's stream, credited with its algorithmic bytes per cell (the reference sweep's arrays
     read and written once: 9 coefficients, p and u read, u written, 96 B), against 8 TB/s; the
     HBM traffic per launch from the committed rocprofv3 counter summary;
-  - `cpu_baseline`: the oracle (a restatement of the reference's OpenMP CPU path) on this host's
-    cores on a bounded sample of the same workload, and its projection to the tolerance.
+  - `cpu_baseline`: the reference's own OpenMP CPU code (built from its sources into oracle/_ref;
+    the oracle restating it where that build is absent) on this host's cores, on a bounded
+    sample of the same workload, and its projection to the tolerance.
 
 Other workloads (--workload):
   batch    the 64 uneven cavities of simple-hpc_amd/workloads/cavities_seed0.txt to 1e-5, one
@@ -183,22 +184,61 @@ def field_errors(p, a, b):
     return {"u": du, "v": dv, "p": dp}
 
 
+# This backend's config keys that the reference's parser does not know (it rejects unknown keys).
+ADDED_KEYS = {"mg_precision", "mg_coarsest", "mg_dc_maxit", "pressure_warm_start",
+              "warm_start_weights", "warm_start_level", "check_every", "field_format", "autotune",
+              "momentum_precision"}
+
+
+def reference_config(text):
+    """A config text of this backend turned into the reference's: the added keys dropped and
+    backend = openmp (the reference's CPU path)."""
+    out = []
+    for line in text.splitlines():
+        key = line.split("=")[0].strip()
+        if key in ADDED_KEYS or key == "backend":
+            continue
+        out.append(line)
+    return "\n".join(out + ["backend = openmp"]) + "\n"
+
+
+class CpuSolver:
+    """FP64 outer iterations on the host: the reference's own OpenMP CPU code built from its
+    sources into oracle/_ref (kind "reference") or, where that build is absent, the oracle
+    restating it (kind "port"). step() runs one outer iteration and returns its seconds."""
+
+    def __init__(self, sg, text, cores):
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        if pyoracle.ref_available():
+            pyoracle.ref_lib().ref_set_threads(cores)
+            self.o = pyoracle.Reference(reference_config(text))
+            self.kind, self.what = "reference", "the reference's own CPU code (oracle/_ref)"
+        else:
+            pyoracle.set_threads(cores)
+            self.o = pyoracle.Oracle(sg.params_from_config(text)[0])
+            self.kind, self.what = "port", "the oracle restating the reference's OpenMP path"
+
+    def step(self):
+        if self.kind == "port":
+            return self.o.run_fixed(1)[0]
+        t0 = time.perf_counter()
+        self.o.outer_iteration()
+        return time.perf_counter() - t0
+
+
 def cpu_baseline(sg, n, seconds=12.0, iterations=None):
-    """Oracle outer iterations at the full size on all host cores for about `seconds`."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import pyoracle
+    """FP64 outer iterations of the workload at its full size on all host cores for about
+    `seconds` (CpuSolver: the reference's own CPU code where its build is present)."""
     cores = os.cpu_count() or 1
-    pyoracle.set_threads(cores)
-    p, _ = sg.params_from_config(config_text(n, "plain"))
-    o = pyoracle.Oracle(p)
+    o = CpuSolver(sg, config_text(n, "plain"), cores)
     k, t = 0, 0.0
-    while t < seconds and k < 500:
-        dt, _ = o.run_fixed(1)
-        t += dt
+    while t < seconds and k < 2000:
+        t += o.step()
         k += 1
-    out = {"value": k / t, "unit": "outer iterations/s", "cores": cores, "kind": "port",
+    out = {"value": k / t, "unit": "outer iterations/s", "cores": cores, "kind": o.kind,
            "sample": f"{k} FP64 outer iterations of the {n}^2 workload from zero fields "
-                     f"(oracle restating the reference's OpenMP path, {cores} threads), {t:.1f} s"}
+                     f"({o.what}, {cores} OpenMP threads), {t:.1f} s"}
     if iterations:
         # the plain paths are bit-identical, so the CPU needs the same iteration count
         out["projected_time_to_tolerance_s"] = iterations * t / k
@@ -334,30 +374,28 @@ def run_batch(sg, steps_cap, knobs=None, cpu=True):
 
 
 def cpu_batch_baseline(sg, cases, gpu_iters, seconds=15.0):
-    """Oracle s/iteration per distinct cavity size (all host cores), projected onto the FP64
-    iteration counts (bit-identical paths, so the CPU would need the same counts)."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import pyoracle
+    """CPU s/iteration per distinct cavity size (all host cores; CpuSolver: the reference's own
+    CPU code where its build is present), projected onto the FP64 iteration counts
+    (bit-identical paths, so the CPU would need the same counts)."""
     cores = os.cpu_count() or 1
-    pyoracle.set_threads(cores)
     per = {}
     budget = seconds / len(set(n for n, _ in cases))
+    kind = what = None
     for n in sorted(set(n for n, _ in cases)):
         re = [r for m, r in cases if m == n][0]
-        p, _ = sg.params_from_config(sg.batch_config(n, re, precision="fp64"))
-        o = pyoracle.Oracle(p)
+        o = CpuSolver(sg, sg.batch_config(n, re, precision="fp64"), cores)
+        kind, what = o.kind, o.what
         k, t = 0, 0.0
         while t < budget and k < 200:
-            dt, _ = o.run_fixed(1)
-            t += dt
+            t += o.step()
             k += 1
         per[n] = t / k
     total = sum(per[n] * (its - 1) for (n, _), its in zip(cases, gpu_iters))
     work = sum(its - 1 for its in gpu_iters)
     return {"value": work / total, "unit": "cavity outer iterations/s", "cores": cores,
-            "kind": "port", "projected_time_to_tolerance_s": total,
+            "kind": kind, "projected_time_to_tolerance_s": total,
             "s_per_iteration": {str(k): v for k, v in per.items()},
-            "sample": f"oracle outer iterations per cavity size ({budget:.1f} s each, "
+            "sample": f"outer iterations per cavity size ({what}, {budget:.1f} s each, "
                       f"{cores} threads), projected onto the FP64 iteration counts"}
 
 
