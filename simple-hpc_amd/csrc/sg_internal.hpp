@@ -357,7 +357,7 @@ void launch_mg_solve(Solver& s, const int* stop, bool has_guess);
 // With zero, the same launches also fill zero[0..s.n) with 0.0.
 void launch_remove_mean0(Solver& s, double* a, const int* stop, double* zero = nullptr);
 // learned warm start (sg_unet.hip): p' guess into the level-0 x, kept only if it lowers the defect
-void launch_warm_guess(Solver& s, const int* stop);
+void launch_warm_guess(Solver& s, const int* stop, bool bfused);
 void default_launch_shapes(Solver& s);
 void warm_free(Solver& s);
 // outer loop control (sg_api.hip)

@@ -1414,7 +1414,7 @@ void launch_mg_solve(Solver& s, const int* stop, bool has_guess) {
         launch_remove_mean0(s, s.F(SG_F_B), stop,
                             !has_guess && !warm ? s.F(SG_F_PC) : nullptr); // and p' = 0
     if (warm) {
-        launch_warm_guess(s, stop);
+        launch_warm_guess(s, stop, bfused);
         has_guess = true;
     }
     emit<Mix, k_mx_init, BX * BY>(s, g, dim3(BX, BY), 0, m, stop);
@@ -1432,11 +1432,11 @@ void launch_mg_solve(Solver& s, const int* stop, bool has_guess) {
 }
 
 // The same conditions as launch_vcycles' use of staged tiles on level 0, the tail starting at
-// level min_tm or below; no learned warm start (it reads level 0's operator and b).
+// level min_tm or below. (The learned warm start reads level 0 as the mixed conversions do: the
+// lean operator's derived links and b - mean.)
 static bool staged_from0(Solver& s, int min_tm) {
     const sg_params& p = s.p;
-    if (p.pressure_solver != SG_P_MG || s.mg.empty() || !tiled_ok(s) || s.mg_staged != 0 ||
-        p.pressure_warm_start == SG_WARM_LEARNED)
+    if (p.pressure_solver != SG_P_MG || s.mg.empty() || !tiled_ok(s) || s.mg_staged != 0)
         return false;
     const bool mixed = p.mg_precision == SG_MG_MIXED;
     const int hdn = 2 * p.mg_pre + 1, hup = 2 * p.mg_post;

@@ -956,10 +956,31 @@ unsigned char* scratch_here() {
 
 namespace {
 
+// Level 0 of the p' system as the warm start reads it: AW, AS and AP derived from AE and AN when
+// the assembly was lean, and b - mean when b's mean removal is pending (the values the full
+// assembly and k_total_sub would have stored, by the same expressions as the mixed form's
+// conversions).
+struct L0View {
+    const double *AE, *AW, *AN, *AS, *AP, *b; // interior cell (0, 0) at index 0, stride ld
+    int ld, nx, ny;
+    int lean;             // AW, AS, AP not assembled
+    const double* bmean;  // b's total when its mean removal is pending, else null
+    double bn;
+    __device__ __forceinline__ double bv(size_t q) const { return bmean ? b[q] - *bmean / bn : b[q]; }
+    __device__ __forceinline__ double aw(int j, size_t q) const {
+        return lean ? (j == 0 ? 0.0 : AE[q - 1]) : AW[q];
+    }
+    __device__ __forceinline__ double as(int i, size_t q) const {
+        return lean ? (i == nx - 1 ? 0.0 : AN[q + ld]) : AS[q];
+    }
+    __device__ __forceinline__ double ap(int i, int j, size_t q) const {
+        return lean ? AE[q] + aw(j, q) + AN[q] + as(i, q) : AP[q];
+    }
+};
+
 struct FeatArgs {
-    const double* b;    // level-0 b (interior cell (0,0) at b[0], stride ld)
-    const double* ap;   // level-0 APP, same layout
-    int ld, nx, ny, bs; // fine interior size, block side
+    L0View l;           // level 0
+    int bs;             // block side
     float* cb;          // coarse block sums of b [nxc*nyc]
     float* ca;          // coarse block means of a_P
     double* rows;       // block row segment sums of b and a_P, interleaved [nx*nyc][2]
@@ -971,20 +992,22 @@ struct FeatArgs {
 // are added in order, so a large block (16 x 16 at level 4) waits on one load latency per 16
 // values.
 __global__ void k_feat_rows(FeatArgs a) {
-    const int nyc = a.ny / a.bs;
+    const L0View& l = a.l;
+    const int nyc = l.ny / a.bs;
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= a.nx * nyc)
+    if (e >= l.nx * nyc)
         return;
     const int i = e / nyc, J = e - i * nyc;
-    const size_t q0 = static_cast<size_t>(i) * a.ld + J * a.bs;
+    const size_t q0 = static_cast<size_t>(i) * l.ld + J * a.bs;
     double sb = 0.0, sa = 0.0;
     for (int j0 = 0; j0 < a.bs; j0 += 16) {
         double vb[16], va[16];
 #pragma unroll
         for (int t = 0; t < 16; ++t) {
             const bool in = j0 + t < a.bs;
-            vb[t] = in ? a.b[q0 + j0 + t] : 0.0;
-            va[t] = in ? a.ap[q0 + j0 + t] : 0.0;
+            const size_t q = q0 + j0 + t;
+            vb[t] = in ? l.bv(q) : 0.0;
+            va[t] = in ? l.ap(i, J * a.bs + j0 + t, q) : 0.0;
         }
 #pragma unroll
         for (int t = 0; t < 16; ++t)
@@ -998,7 +1021,7 @@ __global__ void k_feat_rows(FeatArgs a) {
 }
 
 __global__ void k_feat_blocks(FeatArgs a) {
-    const int nyc = a.ny / a.bs, nxc = a.nx / a.bs;
+    const int nyc = a.l.ny / a.bs, nxc = a.l.nx / a.bs;
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nxc * nyc)
         return;
@@ -1094,7 +1117,7 @@ struct GuessArgs {
     const float* y;     // prediction on the coarse grid
     const double* st;   // statistics (st[2] = scale)
     double* pc;         // level-0 p' (interior cell (0,0) at pc[0], stride ld)
-    const double *AE, *AW, *AN, *AS, *AP, *b;
+    L0View l;           // level 0 of the p' system
     int ld, nx, ny, bs;
     double* part;       // per-row partials: [nx][2] (defect^2, b^2)
     int* flag;          // 1: keep the guess
@@ -1129,13 +1152,14 @@ __device__ __forceinline__ void k_guess_rows(const GuessArgs& a, const Blk& bk) 
     double d2 = 0.0, b2 = 0.0;
     for (int j = threadIdx.x; j < a.ny; j += 256) {
         const size_t q = static_cast<size_t>(i) * s + j;
+        const L0View& l = a.l;
         const double x = guess_at(a, i, j);
-        const double ag = a.AP[q] * x - (a.AE[q] * guess_at(a, i, j + 1) +
-                                         a.AW[q] * guess_at(a, i, j - 1) +
-                                         a.AS[q] * guess_at(a, i + 1, j) +
-                                         a.AN[q] * guess_at(a, i - 1, j));
+        const double ag = l.ap(i, j, q) * x - (l.AE[q] * guess_at(a, i, j + 1) +
+                                               l.aw(j, q) * guess_at(a, i, j - 1) +
+                                               l.as(i, q) * guess_at(a, i + 1, j) +
+                                               l.AN[q] * guess_at(a, i - 1, j));
         d2 += x * ag;
-        b2 += a.b[q] * x;
+        b2 += l.bv(q) * x;
     }
     sd[threadIdx.x] = d2;
     sb[threadIdx.x] = b2;
@@ -1157,54 +1181,41 @@ __device__ __forceinline__ void k_guess_rows(const GuessArgs& a, const Blk& bk) 
 // E(alpha g) = alpha^2 g.A.g / 2 - alpha b.g, whose minimum -(b.g)^2 / (2 g.A.g) is never above
 // E(0) = 0: since E(x) - E(p') = |x - p'|_A^2 / 2, the scaled guess is never farther from the
 // solution than zero in the A-norm. It is discarded (p' = 0) when g.A.g is not positive or alpha is
-// not finite. Rows added in order by one lane, from LDS: the wave first loads a chunk of the row
-// partials at once, so the chain waits on the adds, not on one global load after another (a
-// lane reading them itself took 83 us at 1024 rows, one LDS read per add 19 us).
-constexpr int GD_CHUNK = 1024;
+// not finite. The row partials are added by a fixed tree (each thread its consecutive share in
+// order, then pairwise in LDS): deterministic, and latency-bound on log2 of the thread count
+// rather than on a chain as long as the grid (the learned form is pinned by tolerance, not bits).
+constexpr int GD_T = 1024;
 __device__ __forceinline__ void k_guess_decide(const GuessArgs& a, const Blk&) {
-    __shared__ __align__(16) double sp[2 * GD_CHUNK];
+    __shared__ double sg[GD_T], sb[GD_T];
+    const int t = threadIdx.x, per = (a.nx + GD_T - 1) / GD_T;
     double gag = 0.0, bg = 0.0;
-    for (int c0 = 0; c0 < a.nx; c0 += GD_CHUNK) {
-        const int n = min(GD_CHUNK, a.nx - c0);
-        // The chunk's loads are all in flight before the first LDS store (64 lanes, 32 each).
-        {
-            constexpr int PER = 2 * GD_CHUNK / 64;
-            double v[PER];
+    for (int q0 = t * per; q0 < min(a.nx, (t + 1) * per); q0 += 8) {
+        double2 v[8];
 #pragma unroll
-            for (int q = 0; q < PER; ++q) {
-                const int e = q * 64 + threadIdx.x;
-                v[q] = e < 2 * n ? a.part[2 * static_cast<size_t>(c0) + e] : 0.0;
-            }
+        for (int q = 0; q < 8; ++q)
+            v[q] = q0 + q < min(a.nx, (t + 1) * per)
+                       ? reinterpret_cast<const double2*>(a.part)[q0 + q]
+                       : double2{0.0, 0.0};
 #pragma unroll
-            for (int q = 0; q < PER; ++q)
-                sp[q * 64 + threadIdx.x] = v[q];
+        for (int q = 0; q < 8; ++q) {
+            gag += v[q].x;
+            bg += v[q].y;
         }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            // 16 (g.A.g, b.g) pairs at a time into registers, then added in order.
-            constexpr int B = 16;
-            const double2* sp2 = reinterpret_cast<const double2*>(sp);
-            int i = 0;
-            for (; i + B <= n; i += B) {
-                double2 v[B];
-#pragma unroll
-                for (int q = 0; q < B; ++q)
-                    v[q] = sp2[i + q];
-#pragma unroll
-                for (int q = 0; q < B; ++q) {
-                    gag += v[q].x;
-                    bg += v[q].y;
-                }
-            }
-            for (; i < n; ++i) {
-                gag += sp[2 * i];
-                bg += sp[2 * i + 1];
-            }
+    }
+    sg[t] = gag;
+    sb[t] = bg;
+    __syncthreads();
+    for (int w = GD_T / 2; w > 0; w >>= 1) {
+        if (t < w) {
+            sg[t] += sg[t + w];
+            sb[t] += sb[t + w];
         }
         __syncthreads();
     }
-    if (threadIdx.x != 0)
+    if (t != 0)
         return;
+    gag = sg[0];
+    bg = sb[0];
     const double alpha = gag > 0.0 ? bg / gag : 0.0;
     const int keep = (gag > 0.0 && isfinite(alpha) && alpha != 0.0) ? 1 : 0;
     *a.flag = keep;
@@ -1247,10 +1258,18 @@ void warm_free(Solver& s) {
     s.warm = nullptr;
 }
 
-static void feat_launch(Solver& s, hipStream_t st) {
-    Warm& w = *s.warm;
+// Level 0 as it stands: lean when assemble_p wrote only AE, AN and b; b - mean when bfused (b's
+// mean removal carried by the multigrid solve: its total is in S_BSUM).
+static L0View l0_view(Solver& s, bool bfused) {
     const MgLevel& L0 = s.mg[0];
-    FeatArgs f{L0.b, L0.AP, s.ld, s.p.NX, s.p.NY, w.bs, w.cb, w.ca, w.rows};
+    return {L0.AE,  L0.AW,  L0.AN, L0.AS, L0.AP,           L0.b,
+            s.ld,   s.p.NX, s.p.NY, s.p_lean ? 1 : 0, bfused ? s.scal + S_BSUM : nullptr,
+            static_cast<double>(s.p.NX) * s.p.NY};
+}
+
+static void feat_launch(Solver& s, hipStream_t st, bool bfused) {
+    Warm& w = *s.warm;
+    FeatArgs f{l0_view(s, bfused), w.bs, w.cb, w.ca, w.rows};
     const int nc = w.nxc * w.nyc;
     hipLaunchKernelGGL(k_feat_rows, dim3(g1(static_cast<long>(s.p.NX) * w.nyc)), dim3(256), 0, st,
                        f);
@@ -1308,7 +1327,7 @@ void warm_attach(Solver& s, UNet* net, int level) {
 
 // The guess: features, network, p' = prediction * scale, acceptance test (enqueued on s.stream;
 // every kernel is stream-ordered, so the sequence can be captured).
-void launch_warm_guess(Solver& s, const int* stop) {
+void launch_warm_guess(Solver& s, const int* stop, bool bfused) {
     Warm& w = *s.warm;
     if (!w.net)
         throw ArgError("pressure_warm_start = learned but no network is attached");
@@ -1317,18 +1336,18 @@ void launch_warm_guess(Solver& s, const int* stop) {
     UNet& net = *w.net;
     UNet::Ctx& c = net.get_ctx(1, w.nxc, w.nyc);
     const int nc = w.nxc * w.nyc;
-    feat_launch(s, s.stream);
+    feat_launch(s, s.stream, bfused);
     const float lre = static_cast<float>(std::log10(s.p.Re / 1000.0));
     hipLaunchKernelGGL(k_feat_input, dim3(g1(nc)), dim3(256), 0, s.stream, w.cb, w.ca, nc, w.st,
                        lre, c.x_in);
     SG_HIP(hipGetLastError());
     net.forward(s.stream, c);
     const MgLevel& L0 = s.mg[0];
-    GuessArgs g{c.y,  w.st, L0.x,    L0.AE,   L0.AW, L0.AN, L0.AS, L0.AP, L0.b,
-                s.ld, s.p.NX, s.p.NY, w.bs, w.part, w.flag, s.ctl};
+    GuessArgs g{c.y,   w.st,   L0.x, l0_view(s, bfused), s.ld, s.p.NX, s.p.NY,
+                w.bs,  w.part, w.flag, s.ctl};
     const dim3 grid(cdiv(s.p.NY, 64), cdiv(s.p.NX, 4));
     emit<GuessArgs, k_guess_rows, 256>(s, dim3(s.p.NX), dim3(256), 0, g, stop);
-    emit<GuessArgs, k_guess_decide, 64>(s, dim3(1), dim3(64), 0, g, stop);
+    emit<GuessArgs, k_guess_decide, GD_T>(s, dim3(1), dim3(GD_T), 0, g, stop);
     emit<GuessArgs, k_guess_scale, 256>(s, grid, dim3(64, 4), 0, g, stop);
 }
 
@@ -1567,7 +1586,7 @@ int sg_warm_features(sg_solver* sh, int level, float* x, double* scale) {
         }
         Warm& w = *s.warm;
         const int nc = w.nxc * w.nyc;
-        feat_launch(s, s.stream);
+        feat_launch(s, s.stream, false); // b materialized above
         const float lre = static_cast<float>(std::log10(s.p.Re / 1000.0));
         hipLaunchKernelGGL(k_feat_input_f32, dim3(g1(nc)), dim3(256), 0, s.stream, w.cb, w.ca, nc,
                            w.st, lre, w.xf);
