@@ -986,38 +986,42 @@ struct FeatArgs {
     double* rows;       // block row segment sums of b and a_P, interleaved [nx*nyc][2]
 };
 
-// Coarse block sums of b and block means of a_P in two launches, each with many threads: the
-// sum of every block row segment (one thread per fine row and coarse column), then the sum of a
-// block's row sums in row order (one thread per coarse cell). Loads go 16 at a time before they
-// are added in order, so a large block (16 x 16 at level 4) waits on one load latency per 16
-// values.
-__global__ void k_feat_rows(FeatArgs a) {
+// Coarse block sums of b and block means of a_P in two launches: the sum of every block row
+// segment, then the sum of a block's row sums in row order (one thread per coarse cell). Rows: a
+// workgroup takes FC consecutive columns of one fine row, every thread loads its share of them
+// (consecutive threads, consecutive columns) into LDS, then one lane per segment adds its bs
+// values in column order from LDS.
+constexpr int FC = 1024; // columns per k_feat_rows workgroup (a multiple of every block side)
+constexpr int FT = 256;  // its threads
+__global__ void __launch_bounds__(FT) k_feat_rows(FeatArgs a) {
+    __shared__ double sb[FC], sa[FC];
     const L0View& l = a.l;
     const int nyc = l.ny / a.bs;
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= l.nx * nyc)
-        return;
-    const int i = e / nyc, J = e - i * nyc;
-    const size_t q0 = static_cast<size_t>(i) * l.ld + J * a.bs;
-    double sb = 0.0, sa = 0.0;
-    for (int j0 = 0; j0 < a.bs; j0 += 16) {
-        double vb[16], va[16];
+    const int i = blockIdx.y, c0 = blockIdx.x * FC, w = min(FC, l.ny - c0);
+    const size_t q0 = static_cast<size_t>(i) * l.ld + c0;
+    double vb[FC / FT], va[FC / FT];
 #pragma unroll
-        for (int t = 0; t < 16; ++t) {
-            const bool in = j0 + t < a.bs;
-            const size_t q = q0 + j0 + t;
-            vb[t] = in ? l.bv(q) : 0.0;
-            va[t] = in ? l.ap(i, J * a.bs + j0 + t, q) : 0.0;
-        }
-#pragma unroll
-        for (int t = 0; t < 16; ++t)
-            if (j0 + t < a.bs) {
-                sb += vb[t];
-                sa += va[t];
-            }
+    for (int k = 0; k < FC / FT; ++k) {
+        const int c = k * FT + threadIdx.x;
+        vb[k] = c < w ? l.bv(q0 + c) : 0.0;
+        va[k] = c < w ? l.ap(i, c0 + c, q0 + c) : 0.0;
     }
-    a.rows[2 * static_cast<size_t>(e)] = sb;
-    a.rows[2 * static_cast<size_t>(e) + 1] = sa;
+#pragma unroll
+    for (int k = 0; k < FC / FT; ++k) {
+        sb[k * FT + threadIdx.x] = vb[k];
+        sa[k * FT + threadIdx.x] = va[k];
+    }
+    __syncthreads();
+    for (int s = threadIdx.x; s < w / a.bs; s += FT) {
+        double tb = 0.0, ta = 0.0;
+        for (int c = s * a.bs; c < (s + 1) * a.bs; ++c) {
+            tb += sb[c];
+            ta += sa[c];
+        }
+        const size_t e = static_cast<size_t>(i) * nyc + c0 / a.bs + s;
+        a.rows[2 * e] = tb;
+        a.rows[2 * e + 1] = ta;
+    }
 }
 
 __global__ void k_feat_blocks(FeatArgs a) {
@@ -1271,8 +1275,9 @@ static void feat_launch(Solver& s, hipStream_t st, bool bfused) {
     Warm& w = *s.warm;
     FeatArgs f{l0_view(s, bfused), w.bs, w.cb, w.ca, w.rows};
     const int nc = w.nxc * w.nyc;
-    hipLaunchKernelGGL(k_feat_rows, dim3(g1(static_cast<long>(s.p.NX) * w.nyc)), dim3(256), 0, st,
-                       f);
+    if (w.bs > FC || FC % w.bs != 0)
+        throw ArgError("warm start: the block side must divide the feature row chunk");
+    hipLaunchKernelGGL(k_feat_rows, dim3(cdiv(s.p.NY, FC), s.p.NX), dim3(FT), 0, st, f);
     SG_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_feat_blocks, dim3(g1(nc)), dim3(256), 0, st, f);
     SG_HIP(hipGetLastError());
