@@ -118,6 +118,23 @@ def _mixed_params(n, re, learned):
     return sg.params_from_config(text)[0]
 
 
+def test_automatic_warm_start_level():
+    """Level -1 resolves to the coarsest multigrid level whose grid is still at least 64 x 64 (the
+    coarse-grid size the network was trained on); an explicit level is kept."""
+    net = sg.UNet(np.zeros(U.n_params(), np.float32))
+    for n, want in ((64, 0), (128, 1), (256, 2), (1024, 4)):
+        s = sg.Solver(_mixed_params(n, 400.0, True))
+        assert sg.warm_level(s) == -1  # none attached yet
+        sg.attach_warm_start(s, net, -1)
+        assert sg.warm_level(s) == want, (n, sg.warm_level(s))
+        sg.attach_warm_start(s, net, 1)
+        assert sg.warm_level(s) == 1
+        with pytest.raises(sg.GpuError, match="warm-start level"):
+            sg.attach_warm_start(s, net, 9)
+        s.close()
+    net.close()
+
+
 def test_rejected_guess_is_exactly_the_plain_solve():
     net = sg.UNet(np.zeros(U.n_params(), np.float32))  # predicts 0: a degenerate guess
     plain = sg.Solver(_mixed_params(64, 400.0, False))

@@ -163,6 +163,9 @@ def test_cli_error_paths(tmp_path):
     assert rc == 1 and "this build runs backend = gpu only" in err
     rc, _, err = _cli(tmp_path, "backend = gpu\nmomentum_solver = gs_lex\n")
     assert rc == 1 and err.startswith("error (rank 0): ")
+    # the added autotune key is a bool with the reference's bool parsing
+    rc, _, err = _cli(tmp_path, "N_cells_x = 8\nautotune = maybe\n")
+    assert rc == 1 and err.startswith("error (rank 0): ") and "autotune" in err
 
 
 def test_abi_exports_every_declared_symbol():

@@ -167,3 +167,21 @@ def test_params_derivation_matches_reference():
         p, _ = sg.params_from_config(cfg)
         r = po.Reference(cfg)
         assert r.derived == (p.nrows, p.ncols, p.dx, p.dy, p.nu)
+
+
+@pytest.mark.skipif(not po.ref_available(), reason="the reference build oracle/_ref is absent")
+def test_bench_cpu_baseline_drives_the_reference():
+    """bench.py's cpu_baseline times the reference's own CPU code: the workload's config text with
+    this backend's added keys dropped parses in the reference's parser (which rejects unknown
+    keys), and its outer iterations give the oracle's residuals bit for bit."""
+    import sys
+    from conftest import ROOT
+    sys.path.insert(0, ROOT)
+    import bench
+    o = bench.CpuSolver(sg, bench.config_text(32, "learned"), 2)  # every key bench writes
+    assert o.kind == "reference"
+    _, ref = _oracle(bench.config_text(32, "plain"))
+    for _ in range(3):
+        a, b = o.o.outer_iteration(), ref.outer_iteration()
+        assert np.array_equal(a, b), (a, b)
+    assert o.step() > 0.0
