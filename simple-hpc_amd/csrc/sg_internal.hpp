@@ -354,6 +354,9 @@ void flush_sums(Solver& s, const int* stop, const StopArgs* rule = nullptr);
 // multigrid (sg_mg.hip)
 void mg_setup(Solver& s);
 void launch_mg_solve(Solver& s, const int* stop, bool has_guess);
+void launch_rb_staged(Solver& s, const int* stop);
+// The red-black solve of a grid this small runs in one workgroup (k_pc_rb_one).
+bool rb_one_fits(const sg_params& p);
 // With zero, the same launches also fill zero[0..s.n) with 0.0.
 void launch_remove_mean0(Solver& s, double* a, const int* stop, double* zero = nullptr);
 // learned warm start (sg_unet.hip): p' guess into the level-0 x, kept only if it lowers the defect

@@ -443,11 +443,12 @@ template <class T> __device__ __forceinline__ Stage<T> stage_view(int gi0, int g
 // x of the tile and its halo (zero outside the level's padded range; prolonged if C), and the
 // staged operator inside the level (zero elsewhere: the walls' AW and AS read there).
 // bmean: b - *bmean / bn is staged, the value b's mean removal (k_total_sub) would store, by the
-// same subtraction; xzero: the iterate is zero everywhere, so it is not read.
+// same subtraction; xzero: the iterate is zero everywhere, so it is not read; xring = 0: the ring
+// around the level is zero too but not read (an iterate array whose ring holds other data).
 template <class T, int NTH>
 __device__ __forceinline__ void stage_load(const Stage<T>& st, const Lv<T>& L, const T* x, int LH,
                                            const Lv<T>* C, T omega, const double* bmean = nullptr,
-                                           double bn = 0.0, int xzero = 0) {
+                                           double bn = 0.0, int xzero = 0, int xring = 1) {
     constexpr int B = 4;
     const int n = LH * st.LW;
     const T bm = bmean ? static_cast<T>(*bmean / bn) : T(0);
@@ -460,8 +461,9 @@ __device__ __forceinline__ void stage_load(const Stage<T>& st, const Lv<T>& L, c
             vx[k] = ve[k] = vn[k] = vb[k] = vp[k] = T(0);
             if (e < n && I >= -1 && I <= L.nx && J >= -1 && J <= L.ny) {
                 const long q = static_cast<long>(I) * L.stride + J;
-                vx[k] = xzero ? T(0) : x[q];
-                if (I >= 0 && I < L.nx && J >= 0 && J < L.ny) {
+                const bool inside = I >= 0 && I < L.nx && J >= 0 && J < L.ny;
+                vx[k] = xzero || (!xring && !inside) ? T(0) : x[q];
+                if (inside) {
                     if (C)
                         vx[k] = vx[k] + omega * C->x[static_cast<size_t>(I / 2) * C->stride + J / 2];
                     ve[k] = L.AE[q];
@@ -581,6 +583,46 @@ template <class T, int TS, int TI = TS> __device__ __forceinline__ void k_mg_up_
         const int d = 2 * post - 1 - p;
         stage_pass<T, NTH>(st, p & 1, max(0, ti0 - d), min(F.nx - 1, ti0 + TI - 1 + d), max(0, tj0 - d),
                    min(F.ny - 1, tj0 + TS - 1 + d));
+    }
+    for (int e = threadIdx.x; e < TI * TS; e += NTH) {
+        const int I = ti0 + e / TS, J = tj0 + e % TS;
+        if (I < F.nx && J < F.ny)
+            a.xout[static_cast<size_t>(I) * F.stride + J] = st.x[st.at(I, J)];
+    }
+}
+
+// ---- red-black p' in staged tiles ----
+//
+// solve_pc_redblack (src/pressure.cpp:114-134): p' = 0, then maxit_inner_p sweeps of colour 0
+// then colour 1 of pc_update (:70-75), which is the multigrid smoother's two-statement update
+// (smooth, :62-64) on the level-0 operator. So the staged tiles above run it too: a launch loads
+// p' (or nothing, the first one: p' = 0), AE, AN and b of a tile and a halo of 2 * sweeps cells
+// into LDS and runs 2 * sweeps colour passes there, each over the tile grown by the passes still
+// to come, instead of one launch per colour pass that streams every array for half its cells.
+// The red-black solve's colour c is the cells with I + J + 1 + c even in level-0 indices
+// (I = i - H, J = j - H - 1), the smoother's colour (c + 1) mod 2. AW, AS and AP are derived
+// from AE and AN bit for bit (the p' operator's symmetry, as above). Launches alternate between
+// p' and a second array, so every launch reads the previous launch's whole result and the last
+// one writes p'; the ring around the interior is never read (it is zero by definition).
+struct RbS {
+    Lv<double> F;      // level 0: operator, b
+    const double* xin; // p' before this launch's sweeps (unread when xzero)
+    double* xout;
+    int sweeps, xzero;
+};
+template <int TS, int TI> __device__ __forceinline__ void k_pc_rb_s(const RbS& a, const Blk& bk) {
+    constexpr int NTH = sg_n<double, TS, TI>();
+    const Lv<double>& F = a.F;
+    const int H = 2 * a.sweeps, LW = TS + 2 * H, LH = TI + 2 * H;
+    int tx, ty;
+    xcd_tile(bk, tx, ty);
+    const int ti0 = ty * TI, tj0 = tx * TS;
+    const Stage<double> st = stage_view<double>(ti0 - H, tj0 - H, LH, LW);
+    stage_load<double, NTH>(st, F, a.xin, LH, nullptr, 0.0, nullptr, 0.0, a.xzero, 0);
+    for (int p = 0; p < 2 * a.sweeps; ++p) {
+        const int d = 2 * a.sweeps - 1 - p;
+        stage_pass<double, NTH>(st, (p + 1) & 1, max(0, ti0 - d), min(F.nx - 1, ti0 + TI - 1 + d),
+                                max(0, tj0 - d), min(F.ny - 1, tj0 + TS - 1 + d));
     }
     for (int e = threadIdx.x; e < TI * TS; e += NTH) {
         const int I = ti0 + e / TS, J = tj0 + e % TS;
@@ -1431,6 +1473,32 @@ void launch_mg_solve(Solver& s, const int* stop, bool has_guess) {
     finish_pc_mean(s, stop);
 }
 
+// Red-black p' in staged tiles (k_pc_rb_s): RB_SWEEPS sweeps per launch (a halo of 8 cells: the
+// four staged arrays of a 24 x 32 tile fit 61 KB of LDS, the default dynamic limit, and two
+// workgroups share a CU), p' last in the F(SG_F_PC) array. p' itself need not be zero beforehand
+// except on its ring, which the caller's fill leaves zero.
+constexpr int RB_SWEEPS = 4;
+void launch_rb_staged(Solver& s, const int* stop) {
+    const sg_params& p = s.p;
+    const size_t o = static_cast<size_t>(p.H) * s.ld + p.H + 1;
+    const Lv<double> F{p.NX,        p.NY,        static_cast<int>(s.ld), s.F(SG_F_AEP) + o,
+                       s.F(SG_F_AWP) + o, s.F(SG_F_ANP) + o, s.F(SG_F_ASP) + o, s.F(SG_F_APP) + o,
+                       s.F(SG_F_B) + o,   s.F(SG_F_PC) + o,  nullptr,           nullptr};
+    const int n = p.maxit_inner_p, nl = (n + RB_SWEEPS - 1) / RB_SWEEPS;
+    double* buf[2] = {s.F(SG_F_PC) + o, s.scratch + o};
+    const double* xin = nullptr;
+    constexpr int TS = 32, TI = BR<double>;
+    for (int k = 0; k < nl; ++k) {
+        const int sw = std::min(RB_SWEEPS, n - k * RB_SWEEPS);
+        double* xout = buf[(nl - 1 - k) % 2]; // the last launch writes p'
+        const RbS a{F, xin, xout, sw, k == 0 ? 1 : 0};
+        emit<RbS, k_pc_rb_s<TS, TI>, sg_n<double, TS, TI>()>(
+            s, dim3(cdiv(p.NY, TS), cdiv(p.NX, TI)), dim3(sg_n<double, TS, TI>()),
+            staged_bytes<double>(TS, 2 * sw, TI), a, stop);
+        xin = xout;
+    }
+}
+
 // The same conditions as launch_vcycles' use of staged tiles on level 0, the tail starting at
 // level min_tm or below. (The learned warm start reads level 0 as the mixed conversions do: the
 // lean operator's derived links and b - mean.)
@@ -1448,8 +1516,13 @@ static bool staged_from0(Solver& s, int min_tm) {
 
 // The tail builds its first level from the level above it in global memory, so level 0's AW, AS
 // and AP go unread only if the tail starts at level 2 or below. The mixed form's FP64 conversion
-// and defect kernels derive the three themselves (mix_op).
-bool mg_lean_ok(Solver& s) { return staged_from0(s, 2); }
+// and defect kernels derive the three themselves (mix_op), and so does the red-black solve in
+// staged tiles.
+bool mg_lean_ok(Solver& s) {
+    if (s.p.pressure_solver == SG_P_GS_REDBLACK)
+        return !rb_one_fits(s.p);
+    return staged_from0(s, 2);
+}
 
 // Level 0 above the tail: only level 0's staged kernels (or the mixed form's conversions) read b
 // and the initial p'.

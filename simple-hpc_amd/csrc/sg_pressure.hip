@@ -94,32 +94,15 @@ __device__ __forceinline__ void pc_update(const PArgs& a, double* dst, const dou
     dst[q] = t;
 }
 
-// Red-black pass, colour (i + j) & 1 (src/pressure.cpp:114-134).
-struct RbArgs {
-    PArgs a;
-    int col;
-};
-
-__device__ __forceinline__ void k_pc_rb(const RbArgs& g, const Blk& bk) {
-    const PArgs& a = g.a;
-    const int col = g.col;
-    const int H = a.H;
-    const int i = H + bk.y * BY + threadIdx.y;
-    if (i > a.NX + H - 1)
-        return;
-    const int j = H + 1 + ((i + H + 1 + col) & 1) + 2 * (bk.x * BX + threadIdx.x);
-    if (j > a.NY + H)
-        return;
-    pc_update(a, a.pc, a.pc, static_cast<size_t>(i) * a.ld + j);
-}
-
-// The whole red-black solve of a small grid in one workgroup (config-1 sizes, where the
-// per-colour launches above are pure latency): p' of the interior and its ring of zeros in LDS,
-// (NX + 2) x (NY + 2) doubles; the operator read from global memory (it stays in the caches);
-// maxit_inner_p sweeps of colour 0 then colour 1 with a barrier after every colour pass; the
-// interior written back at the end (p' outside it is zero, as the fill before this launch left
-// it). Every update is pc_update's expression on the same neighbour values in the same colour
-// order, so the result is the per-colour launches' bit for bit.
+// Red-black solve (solve_pc_redblack, src/pressure.cpp:114-134): colour c is the cells with
+// (i + j) mod 2 = c, i and j the padded array's indices. A grid up to 2048 cells (the 32 x 32
+// cavity's size) runs the whole solve in one workgroup, where launches per colour pass would be
+// pure latency: p' of the interior and its ring of zeros in LDS, (NX + 2) x (NY + 2) doubles; the
+// operator read from global memory (it stays in the caches); maxit_inner_p sweeps of colour 0
+// then colour 1 with a barrier after every colour pass; the interior written back at the end (p'
+// outside it is zero, as the fill before this launch left it). Every update is pc_update's
+// expression on the same neighbour values in the same colour order as the reference's loops, so
+// the result is theirs bit for bit. Larger grids run in staged tiles (launch_rb_staged).
 constexpr int RB1_T = 1024;
 constexpr size_t RB1_MAX_LDS = 65536; // bytes: within the default dynamic-LDS limit
 struct Rb1Args {
@@ -551,7 +534,7 @@ void launch_remove_mean0(Solver& s, double* x, const int* stop, double* zero) {
 // One workgroup pays off while a colour pass is at most one update per thread: at 32^2 (config 1)
 // an outer iteration took 0.131 -> 0.109 ms, at 80^2 0.142 -> 0.153 ms (one CU doing 3 updates
 // per thread per pass loses to 16 launches spread over the chip).
-static bool rb_one_fits(const sg_params& p) {
+bool rb_one_fits(const sg_params& p) {
     return static_cast<long>(p.NX) * p.NY <= 2L * RB1_T &&
            sizeof(double) * (p.NX + 2) * static_cast<size_t>(p.NY + 2) <= RB1_MAX_LDS;
 }
@@ -569,11 +552,7 @@ void launch_solve_p(Solver& s, const int* stop, bool eager) {
                                               Rb1Args{a, p.maxit_inner_p}, stop);
             return;
         }
-        for (int k = 0; k < p.maxit_inner_p; ++k)
-            for (int c = 0; c < 2; ++c) {
-                emit<RbArgs, k_pc_rb, BX * BY>(s, dim3(cdiv(cdiv(p.NY, 2), BX), cdiv(p.NX, BY)),
-                                               dim3(BX, BY), 0, RbArgs{a, c}, stop);
-            }
+        launch_rb_staged(s, stop);
         return;
     case SG_P_JACOBI: {
         const int n = p.maxit_inner_p;

@@ -67,15 +67,6 @@ __device__ __forceinline__ bf16x8 ld8(const u16* p) {
     const uint4 v = *reinterpret_cast<const uint4*>(p);
     return __builtin_bit_cast(bf16x8, v);
 }
-__device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, make_uint4(0, 0, 0, 0)); }
-__device__ __forceinline__ bf16x8 pack8(const u16* v) {
-    uint4 r;
-    r.x = v[0] | (static_cast<unsigned>(v[1]) << 16);
-    r.y = v[2] | (static_cast<unsigned>(v[3]) << 16);
-    r.z = v[4] | (static_cast<unsigned>(v[5]) << 16);
-    r.w = v[6] | (static_cast<unsigned>(v[7]) << 16);
-    return __builtin_bit_cast(bf16x8, r);
-}
 
 // ---- weights ----
 

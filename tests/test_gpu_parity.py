@@ -157,6 +157,45 @@ def test_running_max_normalisation_stop():
     _same("u", gpu.get("u"), cpu.get("u"))
 
 
+@pytest.mark.parametrize("nx,ny,inner", [
+    (50, 41, 1), (77, 130, 3), (96, 96, 4), (130, 128, 5), (257, 96, 9), (200, 301, 8),
+])
+def test_redblack_staged_tiles_bit_identical(nx, ny, inner):
+    """The red-black p' solve on grids too large for one workgroup: tiles of 24 x 32 cells and a
+    halo of eight in LDS, four sweeps per launch, on ragged sizes and sweep counts that are not
+    multiples of four. After 3 oracle iterations, one assemble_p + solve_p on each side (p' and
+    the p' operator, its derived AW, AS and AP included, bit for bit), then a 12-iteration run
+    (graph replays) against the oracle's."""
+    cfg = with_keys(golden_cfg("converged_32x40"), N_cells_x=nx, N_cells_y=ny,
+                    momentum_solver="gs_multicolour", pressure_solver="gs_redblack",
+                    maxit_inner_p=inner, maxit=3, error_tolerance=0, backend="gpu")
+    p, _ = sg.params_from_config(cfg)
+    cpu = po.Oracle(p)
+    for _ in range(3):
+        cpu.outer_iteration()
+    gpu = sg.Solver(p)
+    for name in sg.FIELDS:
+        gpu.set(name, cpu.get(name))
+    print(f"red-black staged {nx}x{ny}, {inner} sweeps", flush=True)
+    rc, rg = cpu.assemble_p(), gpu.assemble_p()
+    assert _rel_close(rg, rc), (rg, rc)
+    cpu.solve_p(); gpu.solve_p()
+    _same("solve_p pc", gpu.get("pc"), cpu.get("pc"))
+    for n in ["AEP", "AWP", "ANP", "ASP", "APP", "b"]:
+        _same("assemble_p " + n, gpu.get(n), cpu.get(n))
+    gpu.close()
+    p, _ = sg.params_from_config(with_keys(cfg, maxit=12))
+    cpu = po.Oracle(p)
+    nc, logc, _ = cpu.run()
+    gpu = sg.Solver(p)
+    done, logg, _ = gpu.advance(12)
+    assert done == 12
+    assert bits_equal(logg, logc[:, 1:]), first_diff(logg, logc[:, 1:])
+    for f in ["u", "v", "p", "pc"]:
+        _same(f, gpu.get(f), cpu.get(f))
+    gpu.close()
+
+
 @pytest.mark.parametrize("nx,ny,pres,scheme,order,iters", [
     (33, 47, "mg", "QUICK", 4, 30),       # no coarsening: the whole p' solve in the LDS tail
     (64, 96, "mg", "QUICK", 4, 30),       # non-power-of-two hierarchy (coarsest 4 x 6)
