@@ -100,6 +100,17 @@ __global__ void k_wprep(const float* master, int cin, int cout, int ci_real, int
 
 // ---- convolution (forward and input gradient) ----
 
+// Where a convolution's input operand comes from. X: a stored activation (x, ldx). The other
+// forms compute the input of an inference pass on the fly from what the preceding layers stored,
+// by the same expressions as the kernels they replace (so the same bits), saving the launch and
+// the store:
+//  - POOL: the 2x2 average pooling of x, an image of twice the sides (k_pool's sums and scale);
+//  - CAT: the concatenation of the nearest 2x upsampling of x2 (ld2, channels [0, csplit), from an
+//    image of half the sides: k_up's copy) and x (channels [csplit, cin), offset by csplit);
+//  - FEAT: the network input of the warm start from the coarse features (k_feat_input_f32's
+//    values, rounded to bfloat16).
+enum { SRC_X = 0, SRC_POOL = 1, SRC_CAT = 2, SRC_FEAT = 3 };
+
 struct ConvArgs {
     const u16* x;
     int ldx;         // input view: pixel stride (channels of the buffer)
@@ -111,7 +122,104 @@ struct ConvArgs {
     int n, h, wd, cin, cout;
     int relu;
     const u16* zero; // 16 zero bytes: the operand of taps outside the image and steps past K
+    const u16* x2;   // SRC_CAT: the upsampled channels' source
+    int ld2, csplit;
+    const float *cb, *ca; // SRC_FEAT: coarse block sums of b, block means of a_P
+    const double* st;     // SRC_FEAT: their statistics
+    float lre;            // SRC_FEAT: log10(Re / 1000)
+    const float *hw, *hb; // head fused into the epilogue (hw null: none)
+    float* yh;
 };
+
+// One lane's raw operand of one K step (8 input channels of one pixel and tap) as loaded; the
+// arithmetic of the pooled and feature forms happens at use, so the loads of a group stay in
+// flight while the previous group's MFMAs run.
+template <int SRC> struct Raw {
+    bf16x8 v;
+};
+template <> struct Raw<SRC_POOL> {
+    uint4 v[4]; // the 2x2 source pixels: (2i, 2j), (2i, 2j+1), (2i+1, 2j), (2i+1, 2j+1)
+};
+template <> struct Raw<SRC_FEAT> {
+    float b, a;
+    int ok;
+};
+
+struct PixPos {
+    int pn, pi, pj; // image, row, column of the lane's output pixel
+    bool pv;        // the pixel exists
+};
+
+template <int SRC>
+__device__ __forceinline__ Raw<SRC> fetch(const ConvArgs& a, const PixPos& px, int s, int oct,
+                                          int K, float rc) {
+    // kk / cin in FP32: kk < 2^12 and cin <= 2^8, so (kk + 0.5) / cin lies at least 1 / (2 cin)
+    // from an integer, far more than the rounding error.
+    const int kk = s * 32 + oct * 8;
+    const int tap = static_cast<int>((static_cast<float>(kk) + 0.5f) * rc);
+    const int ci = kk - tap * a.cin;
+    const int dr = (tap * 11) >> 5; // tap / 3 for tap < 9
+    const int ii = px.pi + dr - 1, jj = px.pj + (tap - 3 * dr) - 1;
+    const bool ok = px.pv & (kk < K) & (ii >= 0) & (ii < a.h) & (jj >= 0) & (jj < a.wd);
+    // Element offsets fit 32 bits (the host checks the sources' sizes against 2^31).
+    Raw<SRC> r;
+    if constexpr (SRC == SRC_X) {
+        r.v = ld8(ok ? a.x + ((px.pn * a.h + ii) * a.wd + jj) * a.ldx + ci : a.zero);
+    } else if constexpr (SRC == SRC_CAT) {
+        const u16* p =
+            ci < a.csplit
+                ? a.x2 + ((px.pn * (a.h >> 1) + (ii >> 1)) * (a.wd >> 1) + (jj >> 1)) * a.ld2 + ci
+                : a.x + ((px.pn * a.h + ii) * a.wd + jj) * a.ldx + (ci - a.csplit);
+        r.v = ld8(ok ? p : a.zero);
+    } else if constexpr (SRC == SRC_POOL) {
+        const int w2 = 2 * a.wd;
+        const u16* p = a.x + ((px.pn * 2 * a.h + 2 * ii) * w2 + 2 * jj) * a.ldx + ci;
+        const int o[4] = {0, a.ldx, w2 * a.ldx, (w2 + 1) * a.ldx};
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            r.v[k] = *reinterpret_cast<const uint4*>(ok ? p + o[k] : a.zero);
+    } else {
+        const int e = (px.pn * a.h + ii) * a.wd + jj;
+        const float* zf = reinterpret_cast<const float*>(a.zero);
+        r.b = *(ok ? a.cb + e : zf);
+        r.a = *(ok ? a.ca + e : zf);
+        r.ok = ok;
+    }
+    return r;
+}
+
+__device__ __forceinline__ unsigned pool2(unsigned q0, unsigned q1, unsigned q2, unsigned q3) {
+    auto one = [](unsigned a, unsigned b, unsigned c, unsigned d) {
+        const float s = (__uint_as_float(a) + __uint_as_float(b)) +
+                        (__uint_as_float(c) + __uint_as_float(d));
+        return static_cast<unsigned>(f2bf(s * 0.25f));
+    };
+    const unsigned lo = one(q0 << 16, q1 << 16, q2 << 16, q3 << 16);
+    const unsigned hi = one(q0 & 0xFFFF0000u, q1 & 0xFFFF0000u, q2 & 0xFFFF0000u, q3 & 0xFFFF0000u);
+    return lo | (hi << 16);
+}
+
+template <int SRC>
+__device__ __forceinline__ bf16x8 value(const Raw<SRC>& r, double st0, double st1, float lre) {
+    if constexpr (SRC == SRC_X || SRC == SRC_CAT) {
+        return r.v;
+    } else if constexpr (SRC == SRC_POOL) {
+        const uint4 o{pool2(r.v[0].x, r.v[1].x, r.v[2].x, r.v[3].x),
+                      pool2(r.v[0].y, r.v[1].y, r.v[2].y, r.v[3].y),
+                      pool2(r.v[0].z, r.v[1].z, r.v[2].z, r.v[3].z),
+                      pool2(r.v[0].w, r.v[1].w, r.v[2].w, r.v[3].w)};
+        return __builtin_bit_cast(bf16x8, o);
+    } else {
+        uint4 o{0u, 0u, 0u, 0u};
+        if (r.ok) {
+            const float v0 = static_cast<float>(static_cast<double>(r.b) / st0);
+            const float v1 = static_cast<float>(static_cast<double>(r.a) / st1 - 1.0);
+            o.x = static_cast<unsigned>(f2bf(v0)) | (static_cast<unsigned>(f2bf(v1)) << 16);
+            o.y = static_cast<unsigned>(f2bf(lre));
+        }
+        return __builtin_bit_cast(bf16x8, o);
+    }
+}
 
 constexpr int CV_T = 256; // 4 waves; a workgroup covers 64 output channels
 
@@ -125,82 +233,103 @@ constexpr int CV_T = 256; // 4 waves; a workgroup covers 64 output channels
 // of one per step. Every load is unconditional (an operand outside the image or past K is read
 // from a zero buffer: the choice is made on the address, so nothing waits on the data before its
 // MFMA). The MFMAs run in the same order on the same operands as one step at a time: the same
-// accumulated bits.
-template <int NB, int PIXW> __global__ void __launch_bounds__(CV_T) k_conv(ConvArgs a) {
-    static_assert((PIXW == 4 && (NB == 1 || NB == 2 || NB == 4)) || (PIXW == 1 && NB == 1),
+// accumulated bits, whatever the operand's source form (SRC).
+// HEAD (NB = 1 and 16 output channels, so a wave holds every channel of its 16 pixels): the
+// network's head, yh = sum over channels c in order of bf16(out_c) * hw[c], plus hb, is formed in
+// the epilogue from the stored values (k_head's sum), gathered across the 16 lanes of a pixel
+// row by shuffles.
+// KS = 4 (PIXW = 1, inference on small images, where a wave's chain of K steps is a chain of
+// memory latencies): the four waves of a workgroup share one 16 x 16 output tile and take a
+// quarter of the K steps each; the quarters' sums are added in wave order through LDS. The same
+// products summed in another order: not the bits of KS = 1.
+template <int NB, int PIXW, int SRC, bool HEAD, int KS>
+__global__ void __launch_bounds__(CV_T) k_conv(ConvArgs a) {
+    static_assert((PIXW == 4 && (NB == 1 || NB == 2 || NB == 4) && KS == 1) ||
+                      (PIXW == 1 && NB == 1 && (KS == 1 || KS == 4)),
                   "launch forms");
-    constexpr int D = NB == 4 ? 2 : 4;
+    static_assert(!HEAD || NB == 1, "the head needs all 16 channels of a pixel in one wave");
+    constexpr int D = (NB == 4 || SRC == SRC_POOL) ? 2 : 4;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int m = lane & 15, oct = lane >> 4;
     const int npix = a.n * a.h * a.wd;
     const int pix0 = PIXW == 4 ? (blockIdx.x * 4 + wave) * 16 : blockIdx.x * 16;
-    const int co0 = PIXW == 4 ? blockIdx.y * 16 * NB : blockIdx.y * 64 + wave * 16;
-    if (co0 >= a.cout)
+    const int co0 = PIXW == 4   ? blockIdx.y * 16 * NB
+                    : KS == 4 ? blockIdx.y * 16
+                              : blockIdx.y * 64 + wave * 16;
+    if (KS == 1 && co0 >= a.cout)
         return;
-    const int p = pix0 + m;
-    const bool pv = p < npix;
-    int pn = 0, pi = 0, pj = 0;
-    if (pv) {
-        pj = p % a.wd;
+    PixPos px{0, 0, 0, pix0 + m < npix};
+    if (px.pv) {
+        const int p = pix0 + m;
+        px.pj = p % a.wd;
         const int r = p / a.wd;
-        pi = r % a.h;
-        pn = r / a.h;
+        px.pi = r % a.h;
+        px.pn = r / a.h;
+    }
+    double st0 = 1.0, st1 = 1.0;
+    if constexpr (SRC == SRC_FEAT) {
+        st0 = a.st[0];
+        st1 = a.st[1];
     }
     f4 acc[NB];
 #pragma unroll
     for (int c = 0; c < NB; ++c)
         acc[c] = f4{0.0f, 0.0f, 0.0f, 0.0f};
     const int K = 9 * a.cin, nk = a.kpad >> 5;
+    // This wave's steps [s0, s0 + len).
+    const int s0 = KS == 1 ? 0 : wave * nk / KS, len = KS == 1 ? nk : (wave + 1) * nk / KS - s0;
     const u16* wrow[NB];
 #pragma unroll
     for (int c = 0; c < NB; ++c)
         wrow[c] = a.w + static_cast<size_t>(co0 + c * 16 + m) * a.kpad + oct * 8;
-    // kk / cin in FP32: kk < 2^12 and cin <= 2^8, so (kk + 0.5) / cin lies at least 1 / (2 cin)
-    // from an integer, far more than the rounding error.
     const float rc = 1.0f / static_cast<float>(a.cin);
-    // Element offsets fit 32 bits (the host checks npix * ldx < 2^31).
-    const int img = pn * a.h;
-    auto load_a = [&](int s) {
-        const int kk = s * 32 + oct * 8;
-        const int tap = static_cast<int>((static_cast<float>(kk) + 0.5f) * rc);
-        const int ci = kk - tap * a.cin;
-        const int dr = (tap * 11) >> 5; // tap / 3 for tap < 9
-        const int ii = pi + dr - 1, jj = pj + (tap - 3 * dr) - 1;
-        const bool ok = pv & (kk < K) & (ii >= 0) & (ii < a.h) & (jj >= 0) & (jj < a.wd);
-        return ld8(ok ? a.x + ((img + ii) * a.wd + jj) * a.ldx + ci : a.zero);
-    };
-    bf16x8 A0[D], A1[D], B0[D][NB], B1[D][NB];
-    auto load_group = [&](int g, bf16x8 (&A)[D], bf16x8 (&B)[D][NB]) {
+    Raw<SRC> A0[D], A1[D];
+    bf16x8 B0[D][NB], B1[D][NB];
+    auto load_group = [&](int g, Raw<SRC> (&A)[D], bf16x8 (&B)[D][NB]) {
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-            const int s = min(g * D + d, nk - 1);
-            A[d] = load_a(s);
+            const int s = s0 + min(g * D + d, len - 1);
+            A[d] = fetch<SRC>(a, px, s, oct, K, rc);
 #pragma unroll
             for (int c = 0; c < NB; ++c)
                 B[d][c] = ld8(wrow[c] + s * 32);
         }
     };
-    auto mma_group = [&](int g, const bf16x8 (&A)[D], const bf16x8 (&B)[D][NB]) {
+    auto mma_group = [&](int g, const Raw<SRC> (&A)[D], const bf16x8 (&B)[D][NB]) {
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-            if (g * D + d >= nk)
+            if (g * D + d >= len)
                 break;
+            const bf16x8 av = value<SRC>(A[d], st0, st1, a.lre);
 #pragma unroll
             for (int c = 0; c < NB; ++c)
-                acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[d], B[d][c], acc[c], 0, 0, 0);
+                acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, B[d][c], acc[c], 0, 0, 0);
         }
     };
     // The next group is loaded whether or not it exists (its steps clamp to the last one), so
     // every path into an MFMA has the same loads in flight and the waits count only its own.
-    const int ng = (nk + D - 1) / D;
-    load_group(0, A0, B0);
-    for (int g = 0; g < ng; g += 2) {
-        load_group(g + 1, A1, B1);
-        mma_group(g, A0, B0);
-        if (g + 1 >= ng)
-            break;
-        load_group(g + 2, A0, B0);
-        mma_group(g + 1, A1, B1);
+    if (len > 0) {
+        const int ng = (len + D - 1) / D;
+        load_group(0, A0, B0);
+        for (int g = 0; g < ng; g += 2) {
+            load_group(g + 1, A1, B1);
+            mma_group(g, A0, B0);
+            if (g + 1 >= ng)
+                break;
+            load_group(g + 2, A0, B0);
+            mma_group(g + 1, A1, B1);
+        }
+    }
+    if constexpr (KS > 1) {
+        __shared__ f4 red[KS - 1][64];
+        if (wave > 0)
+            red[wave - 1][lane] = acc[0];
+        __syncthreads();
+        if (wave > 0)
+            return;
+#pragma unroll
+        for (int w = 0; w < KS - 1; ++w)
+            acc[0] += red[w][lane];
     }
 #pragma unroll
     for (int c = 0; c < NB; ++c) {
@@ -209,12 +338,22 @@ template <int NB, int PIXW> __global__ void __launch_bounds__(CV_T) k_conv(ConvA
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int q = pix0 + 4 * oct + i;
-            if (q >= npix)
-                continue;
             float z = acc[c][i] + b;
             if (a.relu)
                 z = fmaxf(z, 0.0f);
-            a.y[static_cast<size_t>(q) * a.ldy + co] = f2bf(z);
+            const u16 zb = f2bf(z);
+            if (q < npix)
+                a.y[static_cast<size_t>(q) * a.ldy + co] = zb;
+            if constexpr (HEAD) {
+                // Every lane of the 16 sharing oct forms the sum of pixel q; lane m == i stores it.
+                const float zf = bf2f(zb);
+                float h = 0.0f;
+#pragma unroll
+                for (int k = 0; k < 16; ++k)
+                    h += __shfl(zf, 16 * oct + k) * a.hw[k];
+                if (m == i && q < npix)
+                    a.yh[q] = h + *a.hb;
+            }
         }
     }
 }
@@ -720,12 +859,63 @@ struct UNet {
     // Pixel splits of the weight gradient at npix pixels (each split a multiple of 32 pixels).
     static int wsplits(size_t npix) { return static_cast<int>(std::min<size_t>(64, std::max<size_t>(1, npix / 1024))); }
 
-    void conv(hipStream_t st, int k, const u16* x, int ldx, u16* y, int ldy, int n, int h, int w,
-              bool dgrad) {
+    // A convolution's input operand: a stored view (x, ldx) or one of the computed forms (SRC_*).
+    struct In {
+        const u16* x = nullptr;
+        int ldx = 0;
+        int src = SRC_X;
+        const u16* x2 = nullptr; // SRC_CAT
+        int ld2 = 0, csplit = 0;
+        const float *cb = nullptr, *ca = nullptr; // SRC_FEAT
+        const double* st = nullptr;
+        float lre = 0.0f;
+    };
+    static In view(const u16* x, int ldx) { return In{x, ldx}; }
+    static In pooled(const u16* x, int ldx) { return In{x, ldx, SRC_POOL}; }
+    static In cat(const u16* up, int ldup, int cup, const u16* x, int ldx) {
+        return In{x, ldx, SRC_CAT, up, ldup, cup};
+    }
+
+    // split: small images take the split-K form (inference only: training keeps one order), where
+    // the unsplit form would have fewer workgroups than two per CU (at 64 x 64 and below; larger
+    // images fill the GPU without it and measured slower with it).
+    template <int SRC, bool HEAD>
+    static void launch_conv(hipStream_t st, const ConvArgs& a, long npix, bool split) {
+        // 64 pixels per workgroup (each wave up to 64 channels) when that fills the GPU, else 16
+        // pixels x 64 channels (or, split, 16 pixels x 16 channels with K split over the waves).
+        if (static_cast<long>(g1(npix, 64)) * cdiv(a.cout, 64) >= 1024) {
+            if constexpr (!HEAD) {
+                if (a.cout % 64 == 0) {
+                    hipLaunchKernelGGL((k_conv<4, 4, SRC, false, 1>),
+                                       dim3(g1(npix, 64), a.cout / 64), dim3(CV_T), 0, st, a);
+                    return;
+                }
+                if (a.cout % 32 == 0) {
+                    hipLaunchKernelGGL((k_conv<2, 4, SRC, false, 1>),
+                                       dim3(g1(npix, 64), a.cout / 32), dim3(CV_T), 0, st, a);
+                    return;
+                }
+            }
+            hipLaunchKernelGGL((k_conv<1, 4, SRC, HEAD, 1>), dim3(g1(npix, 64), a.cout / 16),
+                               dim3(CV_T), 0, st, a);
+        } else if (split && static_cast<long>(g1(npix, 16)) * cdiv(a.cout, 64) < 512) {
+            hipLaunchKernelGGL((k_conv<1, 1, SRC, HEAD, 4>), dim3(g1(npix, 16), a.cout / 16),
+                               dim3(CV_T), 0, st, a);
+        } else {
+            hipLaunchKernelGGL((k_conv<1, 1, SRC, HEAD, 1>), dim3(g1(npix, 16), cdiv(a.cout, 64)),
+                               dim3(CV_T), 0, st, a);
+        }
+    }
+
+    // Convolution k (dgrad: its input gradient) on the n x h x w image of `in`, output view
+    // (y, ldy); yh: the head fused into the epilogue (the last convolution of an inference pass);
+    // split: the split-K form on small images (inference).
+    void conv(hipStream_t st, int k, const In& in, u16* y, int ldy, int n, int h, int w, bool dgrad,
+              float* yh = nullptr, bool split = false) {
         const Spec& s = SPEC[k];
         ConvArgs a{};
-        a.x = x;
-        a.ldx = ldx;
+        a.x = in.x;
+        a.ldx = in.ldx;
         a.y = y;
         a.ldy = ldy;
         a.n = n;
@@ -748,31 +938,70 @@ struct UNet {
         }
         const long npix = static_cast<long>(n) * h * w;
         a.zero = zero16;
-        if (a.cout % 16 != 0 || a.cin % 8 != 0 || ldx % 8 != 0 || ldy % 8 != 0 || a.kpad % 32 != 0 ||
-            9 * a.cin > a.kpad || reinterpret_cast<uintptr_t>(x) % 16 != 0 ||
-            npix * std::max(ldx, ldy) >= (1L << 31))
+        a.x2 = in.x2;
+        a.ld2 = in.ld2;
+        a.csplit = in.csplit;
+        a.cb = in.cb;
+        a.ca = in.ca;
+        a.st = in.st;
+        a.lre = in.lre;
+        auto al16 = [](const void* p) { return reinterpret_cast<uintptr_t>(p) % 16 == 0; };
+        const long lim = 1L << 31;
+        bool ok = a.cout % 16 == 0 && a.cin % 8 == 0 && ldy % 8 == 0 && a.kpad % 32 == 0 &&
+                  9 * a.cin <= a.kpad && npix * ldy < lim;
+        switch (in.src) {
+        case SRC_X:
+            ok = ok && in.x && al16(in.x) && in.ldx % 8 == 0 && npix * in.ldx < lim;
+            break;
+        case SRC_POOL: // the source image has twice the sides
+            ok = ok && in.x && al16(in.x) && in.ldx % 8 == 0 && 4 * npix * in.ldx < lim;
+            break;
+        case SRC_CAT: // channels [0, csplit) from the half-size image x2
+            ok = ok && in.x && in.x2 && al16(in.x) && al16(in.x2) && in.ldx % 8 == 0 &&
+                 in.ld2 % 8 == 0 && in.csplit % 8 == 0 && in.csplit > 0 && in.csplit < a.cin &&
+                 h % 2 == 0 && w % 2 == 0 && npix * in.ldx < lim && npix * in.ld2 < lim;
+            break;
+        case SRC_FEAT:
+            ok = ok && in.cb && in.ca && in.st && a.cin == IN_PAD && !dgrad;
+            break;
+        default:
+            ok = false;
+        }
+        if (!ok)
             throw ArgError("convolution: 16-byte aligned views, input channels in multiples of 8, "
                            "output channels in multiples of 16");
-        // 64 pixels per workgroup (each wave up to 64 channels) when that fills the GPU, else 16
-        // pixels x 64 channels.
-        if (static_cast<long>(g1(npix, 64)) * cdiv(a.cout, 64) >= 1024) {
-            if (a.cout % 64 == 0)
-                hipLaunchKernelGGL((k_conv<4, 4>), dim3(g1(npix, 64), a.cout / 64), dim3(CV_T), 0,
-                                   st, a);
-            else if (a.cout % 32 == 0)
-                hipLaunchKernelGGL((k_conv<2, 4>), dim3(g1(npix, 64), a.cout / 32), dim3(CV_T), 0,
-                                   st, a);
-            else
-                hipLaunchKernelGGL((k_conv<1, 4>), dim3(g1(npix, 64), a.cout / 16), dim3(CV_T), 0,
-                                   st, a);
+        if (yh) {
+            if (a.cout != 16 || in.src != SRC_X)
+                throw ArgError("convolution: the fused head needs 16 output channels");
+            a.hw = master + hwoff;
+            a.hb = master + hboff;
+            a.yh = yh;
+            launch_conv<SRC_X, true>(st, a, npix, split);
         } else {
-            hipLaunchKernelGGL((k_conv<1, 1>), dim3(g1(npix, 16), cdiv(a.cout, 64)), dim3(CV_T), 0,
-                               st, a);
+            switch (in.src) {
+            case SRC_X:
+                launch_conv<SRC_X, false>(st, a, npix, split);
+                break;
+            case SRC_POOL:
+                launch_conv<SRC_POOL, false>(st, a, npix, split);
+                break;
+            case SRC_CAT:
+                launch_conv<SRC_CAT, false>(st, a, npix, split);
+                break;
+            default:
+                launch_conv<SRC_FEAT, false>(st, a, npix, split);
+            }
         }
         SG_HIP(hipGetLastError());
     }
+    void conv(hipStream_t st, int k, const u16* x, int ldx, u16* y, int ldy, int n, int h, int w,
+              bool dgrad) {
+        conv(st, k, view(x, ldx), y, ldy, n, h, w, dgrad);
+    }
 
-    void forward(hipStream_t st, Ctx& c) {
+    // The forward pass for training: every layer's input is stored (the backward pass reads
+    // them), pooling and upsampling are launches of their own.
+    void forward_train(hipStream_t st, Ctx& c) {
         const int n = c.n, h = c.h, w = c.w;
         const int h1 = h / 2, w1 = w / 2, h2 = h / 4, w2 = w / 4, h3 = h / 8, w3 = w / 8;
         auto ew = [&](long cnt) { return dim3(g1(cnt)); };
@@ -806,6 +1035,33 @@ struct UNet {
         hipLaunchKernelGGL(k_head, dim3(g1(P0)), dim3(256), 0, st, c.d0, 16, master + hwoff,
                            master + hboff, c.y, static_cast<int>(P0));
         SG_HIP(hipGetLastError());
+    }
+
+    // The forward pass for inference in 14 launches. Pooling and upsampling are folded into the
+    // next convolution's operand loads (the pooled and upsampled images are never stored) and the
+    // head into the last convolution's epilogue, by the same expressions; small images take the
+    // split-K convolutions (their sums in another order than the training pass's). With `in` of
+    // form SRC_FEAT the input is computed from the coarse features instead of read from x_in.
+    void forward(hipStream_t st, Ctx& c, const In* in = nullptr) {
+        const int n = c.n, h = c.h, w = c.w;
+        const int h1 = h / 2, w1 = w / 2, h2 = h / 4, w2 = w / 4, h3 = h / 8, w3 = w / 8;
+        auto cv = [&](int k, const In& x, u16* y, int ldy, int hh, int ww, float* yh = nullptr) {
+            conv(st, k, x, y, ldy, n, hh, ww, false, yh, true);
+        };
+        cv(0, in ? *in : view(c.x_in, IN_PAD), c.t0, 16, h, w);
+        cv(1, view(c.t0, 16), c.cat0 + 32, 48, h, w); // e0
+        cv(2, pooled(c.cat0 + 32, 48), c.t1, 32, h1, w1);
+        cv(3, view(c.t1, 32), c.cat1 + 64, 96, h1, w1); // e1
+        cv(4, pooled(c.cat1 + 64, 96), c.t2, 64, h2, w2);
+        cv(5, view(c.t2, 64), c.cat2 + 128, 192, h2, w2); // e2
+        cv(6, pooled(c.cat2 + 128, 192), c.t3, 128, h3, w3);
+        cv(7, view(c.t3, 128), c.bt, 128, h3, w3);
+        cv(8, cat(c.bt, 128, 128, c.cat2 + 128, 192), c.d2a, 64, h2, w2);
+        cv(9, view(c.d2a, 64), c.d2, 64, h2, w2);
+        cv(10, cat(c.d2, 64, 64, c.cat1 + 64, 96), c.d1a, 32, h1, w1);
+        cv(11, view(c.d1a, 32), c.d1, 32, h1, w1);
+        cv(12, cat(c.d1, 32, 32, c.cat0 + 32, 48), c.d0a, 16, h, w);
+        cv(13, view(c.d0a, 16), c.d0, 16, h, w, c.y);
     }
 
     // Weight and bias gradients of conv k (input view x, masked output gradient dz).
@@ -1084,19 +1340,9 @@ __global__ void __launch_bounds__(1024) k_feat_stats(const float* cb, const floa
     }
 }
 
-// Network input (bf16 NHWC, 8 channels) from the coarse fields and the statistics.
-__global__ void k_feat_input(const float* cb, const float* ca, int n, const double* st, float lre,
-                             u16* x) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n)
-        return;
-    const float v[IN_PAD] = {static_cast<float>(cb[e] / st[0]),
-                             static_cast<float>(ca[e] / st[1] - 1.0), lre, 0, 0, 0, 0, 0};
-    for (int c = 0; c < IN_PAD; ++c)
-        x[static_cast<size_t>(e) * IN_PAD + c] = f2bf(v[c]);
-}
-
-// Same input as FP32 (for the host: training data).
+// The network input (8 channels) from the coarse fields and the statistics, in FP32 for the host
+// (training data); an inference pass computes the same values in its first convolution's operand
+// loads (SRC_FEAT) and rounds them to bfloat16 there.
 __global__ void k_feat_input_f32(const float* cb, const float* ca, int n, const double* st,
                                  float lre, float* x) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1122,54 +1368,43 @@ struct GuessArgs {
 // The guess g at interior cell (i, j): scale * the prediction interpolated bilinearly from the
 // coarse cell centres (clamped at the walls) to the fine cell centre; 0 outside the interior (p''s
 // halo stays zero). Evaluated where it is needed (the energy sums and the final write), so it is
-// never stored and read back.
-__device__ __forceinline__ double guess_at(const GuessArgs& a, int i, int j) {
-    if (i < 0 || j < 0 || i >= a.nx || j >= a.ny)
+// never stored and read back. The interpolation is separable: each axis gives its two coarse
+// neighbours and weight (GAxis, computed once per row or column where a kernel reuses it), and
+// gval combines them. The block side bs is a power of two, so (k + 1/2) / bs is computed as a
+// product by 1 / bs, exactly.
+struct GAxis {
+    int o0, o1; // offsets of the two coarse neighbours (rows: times the coarse row length)
+    float w;    // weight of the second
+    bool in;    // the fine index lies in the interior
+};
+__device__ __forceinline__ GAxis gaxis(int k, int n, int nc, float inv, int stride) {
+    GAxis r;
+    r.in = k >= 0 && k < n;
+    const float f = (k + 0.5f) * inv - 0.5f;
+    const float c = fminf(fmaxf(f, 0.0f), static_cast<float>(nc - 1));
+    const int k0 = min(static_cast<int>(c), nc - 1), k1 = min(k0 + 1, nc - 1);
+    r.o0 = k0 * stride;
+    r.o1 = k1 * stride;
+    r.w = c - k0;
+    return r;
+}
+__device__ __forceinline__ double gval(const GuessArgs& a, const GAxis& r, const GAxis& c) {
+    if (!(r.in && c.in))
         return 0.0;
-    const int nxc = a.nx / a.bs, nyc = a.ny / a.bs;
-    const float fi = (i + 0.5f) / a.bs - 0.5f, fj = (j + 0.5f) / a.bs - 0.5f;
-    const float ci = fminf(fmaxf(fi, 0.0f), static_cast<float>(nxc - 1));
-    const float cj = fminf(fmaxf(fj, 0.0f), static_cast<float>(nyc - 1));
-    const int i0 = min(static_cast<int>(ci), nxc - 1), j0 = min(static_cast<int>(cj), nyc - 1);
-    const int i1 = min(i0 + 1, nxc - 1), j1 = min(j0 + 1, nyc - 1);
-    const float wi = ci - i0, wj = cj - j0;
     const float* y = a.y;
-    const float v = (1.0f - wi) * ((1.0f - wj) * y[i0 * nyc + j0] + wj * y[i0 * nyc + j1]) +
-                    wi * ((1.0f - wj) * y[i1 * nyc + j0] + wj * y[i1 * nyc + j1]);
+    const float wi = r.w, wj = c.w;
+    const float v = (1.0f - wi) * ((1.0f - wj) * y[r.o0 + c.o0] + wj * y[r.o0 + c.o1]) +
+                    wi * ((1.0f - wj) * y[r.o1 + c.o0] + wj * y[r.o1 + c.o1]);
     return static_cast<double>(v) * a.st[2];
 }
-
-// Per row: sums of g (A g) and b g for the guess g (one workgroup per row, fixed-order tree).
-__device__ __forceinline__ void k_guess_rows(const GuessArgs& a, const Blk& bk) {
-    __shared__ double sd[256], sb[256];
-    const int i = bk.x;
-    const long s = a.ld;
-    double d2 = 0.0, b2 = 0.0;
-    for (int j = threadIdx.x; j < a.ny; j += 256) {
-        const size_t q = static_cast<size_t>(i) * s + j;
-        const L0View& l = a.l;
-        const double x = guess_at(a, i, j);
-        const double ag = l.ap(i, j, q) * x - (l.AE[q] * guess_at(a, i, j + 1) +
-                                               l.aw(j, q) * guess_at(a, i, j - 1) +
-                                               l.as(i, q) * guess_at(a, i + 1, j) +
-                                               l.AN[q] * guess_at(a, i - 1, j));
-        d2 += x * ag;
-        b2 += l.bv(q) * x;
-    }
-    sd[threadIdx.x] = d2;
-    sb[threadIdx.x] = b2;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) {
-            sd[threadIdx.x] += sd[threadIdx.x + w];
-            sb[threadIdx.x] += sb[threadIdx.x + w];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        a.part[2 * i] = sd[0];
-        a.part[2 * i + 1] = sb[0];
-    }
+__device__ __forceinline__ GAxis grow(const GuessArgs& a, int i) {
+    return gaxis(i, a.nx, a.nx / a.bs, 1.0f / a.bs, a.ny / a.bs);
+}
+__device__ __forceinline__ GAxis gcol(const GuessArgs& a, int j) {
+    return gaxis(j, a.ny, a.ny / a.bs, 1.0f / a.bs, 1);
+}
+__device__ __forceinline__ double guess_at(const GuessArgs& a, int i, int j) {
+    return gval(a, grow(a, i), gcol(a, j));
 }
 
 // The guess enters scaled by alpha = b.g / g.A.g, the minimiser of the energy
@@ -1219,6 +1454,58 @@ __device__ __forceinline__ void k_guess_decide(const GuessArgs& a, const Blk&) {
         ++a.ctl->warm_ok;
     else
         ++a.ctl->warm_rejected;
+}
+
+// Per row: sums of g (A g) and b g for the guess g (one workgroup per row, fixed-order tree). A
+// thread takes the columns j = t, t + 256, ... in order, four at a time with all their operator
+// loads issued before the arithmetic.
+__device__ __forceinline__ void k_guess_rows(const GuessArgs& a, const Blk& bk) {
+    __shared__ double sd[256], sb[256];
+    const int i = bk.x;
+    const long s = a.ld;
+    const L0View& l = a.l;
+    const GAxis rm = grow(a, i - 1), ri = grow(a, i), rp = grow(a, i + 1); // uniform
+    double d2 = 0.0, b2 = 0.0;
+    for (int c0 = 0; c0 < a.ny; c0 += 4 * 256) {
+        double ap[4], ae[4], aw[4], as[4], an[4], bv[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int j = c0 + k * 256 + threadIdx.x;
+            const size_t q = static_cast<size_t>(i) * s + min(j, a.ny - 1);
+            ap[k] = l.ap(i, min(j, a.ny - 1), q);
+            ae[k] = l.AE[q];
+            aw[k] = l.aw(min(j, a.ny - 1), q);
+            as[k] = l.as(i, q);
+            an[k] = l.AN[q];
+            bv[k] = l.bv(q);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int j = c0 + k * 256 + threadIdx.x;
+            if (j < a.ny) {
+                const GAxis cm = gcol(a, j - 1), cj = gcol(a, j), cp = gcol(a, j + 1);
+                const double x = gval(a, ri, cj);
+                const double ag = ap[k] * x - (ae[k] * gval(a, ri, cp) + aw[k] * gval(a, ri, cm) +
+                                               as[k] * gval(a, rp, cj) + an[k] * gval(a, rm, cj));
+                d2 += x * ag;
+                b2 += bv[k] * x;
+            }
+        }
+    }
+    sd[threadIdx.x] = d2;
+    sb[threadIdx.x] = b2;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            sd[threadIdx.x] += sd[threadIdx.x + w];
+            sb[threadIdx.x] += sb[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        a.part[2 * i] = sd[0];
+        a.part[2 * i + 1] = sb[0];
+    }
 }
 
 // p' = alpha g over the interior (0 if the guess was discarded: it may hold non-finite values).
@@ -1331,13 +1618,14 @@ void launch_warm_guess(Solver& s, const int* stop, bool bfused) {
         throw ArgError("the learned warm start does not run in a batch");
     UNet& net = *w.net;
     UNet::Ctx& c = net.get_ctx(1, w.nxc, w.nyc);
-    const int nc = w.nxc * w.nyc;
     feat_launch(s, s.stream, bfused);
-    const float lre = static_cast<float>(std::log10(s.p.Re / 1000.0));
-    hipLaunchKernelGGL(k_feat_input, dim3(g1(nc)), dim3(256), 0, s.stream, w.cb, w.ca, nc, w.st,
-                       lre, c.x_in);
-    SG_HIP(hipGetLastError());
-    net.forward(s.stream, c);
+    UNet::In in;
+    in.src = SRC_FEAT;
+    in.cb = w.cb;
+    in.ca = w.ca;
+    in.st = w.st;
+    in.lre = static_cast<float>(std::log10(s.p.Re / 1000.0));
+    net.forward(s.stream, c, &in);
     const MgLevel& L0 = s.mg[0];
     GuessArgs g{c.y,   w.st,   L0.x, l0_view(s, bfused), s.ld, s.p.NX, s.p.NY,
                 w.bs,  w.part, w.flag, s.ctl};
@@ -1488,7 +1776,7 @@ int sg_unet_loss_grad(sg_unet* h, int nimg, int ht, int wd, const float* x, cons
         u.upload_x(c, x);
         SG_HIP(hipMemcpyAsync(c.t, t, sizeof(float) * nimg * ht * wd, hipMemcpyHostToDevice,
                               u.stream));
-        u.forward(u.stream, c);
+        u.forward_train(u.stream, c);
         u.backward(u.stream, c);
         if (loss)
             *loss = u.loss_of(c);
@@ -1510,7 +1798,7 @@ int sg_unet_train_step(sg_unet* h, int nimg, int ht, int wd, const float* x, con
         u.upload_x(c, x);
         SG_HIP(hipMemcpyAsync(c.t, t, sizeof(float) * nimg * ht * wd, hipMemcpyHostToDevice,
                               u.stream));
-        u.forward(u.stream, c);
+        u.forward_train(u.stream, c);
         u.backward(u.stream, c);
         ++u.step;
         const float b1 = 0.9f, b2 = 0.999f;

@@ -18,6 +18,10 @@ What it restates:
 
 Usage: python run_bench.py [--grids 256 512 1024] [--iters 200] [--repeats 3] [--threads 1]
            [--exe build/simple] [--tag gpu] [--set KEY=VALUE ...] [--out results.csv]
+           [--write-config DIR]
+
+--write-config DIR writes the config of each grid's run to DIR/g<grid>.cfg (output_dir
+DIR/g<grid>) and runs nothing: the same run under a profiler is then `simple DIR/g<grid>.cfg`.
 """
 import argparse
 import csv
@@ -151,6 +155,8 @@ def main(argv=None):
     ap.add_argument("--tag", default="gpu")
     ap.add_argument("--set", action="append", default=[], metavar="KEY=VALUE")
     ap.add_argument("--out", default=None, help="CSV path (default: <tag>_<stamp>.csv here)")
+    ap.add_argument("--write-config", default=None, metavar="DIR",
+                    help="write DIR/g<grid>.cfg for each grid and run nothing")
     a = ap.parse_args(argv)
     extra = {}
     for item in a.set:
@@ -158,6 +164,14 @@ def main(argv=None):
             sys.exit(f"--set expects KEY=VALUE, got: {item}")
         k, v = item.split("=", 1)
         extra[k.strip()] = v.strip()
+    if a.write_config:
+        d = Path(a.write_config).resolve()
+        d.mkdir(parents=True, exist_ok=True)
+        for grid in a.grids:
+            path = d / f"g{grid}.cfg"
+            path.write_text(make_config(grid, a.iters, d / f"g{grid}", extra))
+            print(path)
+        return 0
     exe = Path(a.exe).resolve()
     if not exe.exists():
         sys.exit(f"executable not found: {exe}")

@@ -215,3 +215,16 @@ def test_bench_runner_rows(tmp_path):
     assert float(r["seconds_per_iter"]) == pytest.approx(19.2)
     assert float(r["cell_updates_per_s"]) == pytest.approx(128 * 128 * 10 / 192.0, rel=1e-6)
     assert r["cpu"] == "gfx950 test" and r["solve_p"] == f"{128.0:.9f}"
+
+
+def test_bench_runner_write_config(tmp_path):
+    """--write-config writes each grid's run config (the run's output_dir beside it) and runs
+    nothing, so the same run can be started under a profiler."""
+    import run_bench as rb
+    d = tmp_path / "cfgs"
+    assert rb.main(["--grids", "64", "2048", "--iters", "100", "--write-config", str(d),
+                    "--set", "pressure_solver=gs_redblack", "--exe", "/nonexistent"]) == 0
+    assert sorted(p.name for p in d.iterdir()) == ["g2048.cfg", "g64.cfg"]
+    kv = dict(ln.split(" = ") for ln in (d / "g2048.cfg").read_text().strip().splitlines())
+    assert kv["N_cells_x"] == "2048" and kv["maxit"] == "100"
+    assert kv["pressure_solver"] == "gs_redblack" and kv["output_dir"] == str(d / "g2048")
