@@ -689,18 +689,16 @@ int sg_advance(sg_solver* h, int max_iters, int* iters_done, double* log, int* c
                 capture_graph(s, GRAPH_ITERS, &s.graph_n);
             if (replayed % GRAPH_ITERS)
                 capture_graph(s, 1, &s.graph);
-            int k = 0, left = replayed;
+            int left = replayed;
             if (left >= GRAPH_ITERS) {
                 SG_HIP(hipGraphLaunch(s.graph_n, s.stream));
-                k += GRAPH_ITERS, left -= GRAPH_ITERS;
+                left -= GRAPH_ITERS;
             }
-            if (timed) {
+            if (timed)
                 enqueue_iteration(s, &s.ctl->stop, false, s.pev, 1);
-                ++k;
-            }
-            for (; left >= GRAPH_ITERS; k += GRAPH_ITERS, left -= GRAPH_ITERS)
+            for (; left >= GRAPH_ITERS; left -= GRAPH_ITERS)
                 SG_HIP(hipGraphLaunch(s.graph_n, s.stream));
-            for (; left > 0; ++k, --left)
+            for (; left > 0; --left)
                 SG_HIP(hipGraphLaunch(s.graph, s.stream));
             if (max_iters > 0)
                 mark_iterated(s);

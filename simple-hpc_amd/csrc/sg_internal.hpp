@@ -359,8 +359,10 @@ void launch_rb_staged(Solver& s, const int* stop);
 bool rb_one_fits(const sg_params& p);
 // With zero, the same launches also fill zero[0..s.n) with 0.0.
 void launch_remove_mean0(Solver& s, double* a, const int* stop, double* zero = nullptr);
-// learned warm start (sg_unet.hip): p' guess into the level-0 x, kept only if it lowers the defect
-void launch_warm_guess(Solver& s, const int* stop, bool bfused);
+// learned warm start (sg_unet.hip): the network's guess of p' and its decision (kept only if it
+// lowers the energy); the caller applies it to the level-0 x
+struct GuessArgs;
+GuessArgs launch_warm_guess(Solver& s, const int* stop, bool bfused);
 void default_launch_shapes(Solver& s);
 void warm_free(Solver& s);
 // outer loop control (sg_api.hip)

@@ -20,6 +20,7 @@
 //  - the tail (every level with at most tail_cells cells, down to the coarsest, and its coarse
 //    solve) runs in one workgroup, one launch per V-cycle, synchronising with barriers.
 #include "sg_internal.hpp"
+#include "sg_warm.hpp"
 
 #include <algorithm>
 
@@ -311,6 +312,10 @@ template <class T> struct DU {
     const double* bmean = nullptr; // level 0, b's mean not removed: b - *bmean / bn is read
     double bn = 0.0;
     int xzero = 0;                 // the iterate is zero (first V-cycle from p' = 0): not read
+    // Mixed form, level 0 of the last V-cycle: the result is not stored in FP32 but added to the
+    // FP64 p' (k_mx_update's update: x64 = (first64 ? 0 : x64) + e), stride ld64.
+    double* x64 = nullptr;
+    int ld64 = 0, first64 = 0;
 };
 
 template <class T> __device__ __forceinline__ void k_mg_down(const DU<T>& a, const Blk& bk) {
@@ -586,8 +591,15 @@ template <class T, int TS, int TI = TS> __device__ __forceinline__ void k_mg_up_
     }
     for (int e = threadIdx.x; e < TI * TS; e += NTH) {
         const int I = ti0 + e / TS, J = tj0 + e % TS;
-        if (I < F.nx && J < F.ny)
-            a.xout[static_cast<size_t>(I) * F.stride + J] = st.x[st.at(I, J)];
+        if (I < F.nx && J < F.ny) {
+            const T v = st.x[st.at(I, J)];
+            if (a.x64) {
+                double& x = a.x64[static_cast<size_t>(I) * a.ld64 + J];
+                x = (a.first64 ? 0.0 : x) + static_cast<double>(v);
+            } else {
+                a.xout[static_cast<size_t>(I) * F.stride + J] = v;
+            }
+        }
     }
 }
 
@@ -1143,6 +1155,41 @@ __device__ __forceinline__ void k_mx_defect(const Mix& m, const Blk& bk) {
     m.f.x[g] = 0.0f;
 }
 
+// The learned guess applied, in place of k_mx_init and the first k_mx_defect of a solve that starts
+// from it: p' = x = alpha g, or 0 if the guess was discarded (x at the neighbours evaluated the
+// same way, so they are the values p' holds); the FP32 copies of the operator; the FP32 right-hand
+// side the FP64 defect b - A x (k_mx_defect's expression); FP32 x = 0.
+struct GuessMix {
+    GuessArgs g;
+    Mix m;
+};
+__device__ __forceinline__ void k_mx_guess(const GuessMix& a, const Blk& bk) {
+    const Mix& m = a.m;
+    const GuessArgs& g = a.g;
+    const int J = bk.x * BX + threadIdx.x, I = bk.y * BY + threadIdx.y;
+    if (I >= m.nx || J >= m.ny)
+        return;
+    const int keep = *g.flag;
+    const double alpha = g.part[2 * g.nx];
+    const GAxis rm = grow(g, I - 1), ri = grow(g, I), rp = grow(g, I + 1);
+    const GAxis cm = gcol(g, J - 1), cj = gcol(g, J), cp = gcol(g, J + 1);
+    auto xat = [&](const GAxis& r, const GAxis& c) { return keep ? gval(g, r, c) * alpha : 0.0; };
+    const size_t q = static_cast<size_t>(I) * m.ld + J, f = static_cast<size_t>(I) * m.f.stride + J;
+    const double x = xat(ri, cj);
+    m.x[q] = x;
+    double aw, as, ap;
+    mix_op(m, I, J, q, aw, as, ap);
+    m.f.AE[f] = static_cast<float>(m.AE[q]);
+    m.f.AW[f] = static_cast<float>(aw);
+    m.f.AN[f] = static_cast<float>(m.AN[q]);
+    m.f.AS[f] = static_cast<float>(as);
+    m.f.AP[f] = static_cast<float>(ap);
+    const double r = mix_b(m, q) - (ap * x - (m.AE[q] * xat(ri, cp) + aw * xat(ri, cm) +
+                                              as * xat(rp, cj) + m.AN[q] * xat(rm, cj)));
+    m.f.b[f] = static_cast<float>(r);
+    m.f.x[f] = 0.0f;
+}
+
 // p' += e (FP64 update).
 __device__ __forceinline__ void k_mx_update(const Mix& m, const Blk& bk) {
     const int J = bk.x * BX + threadIdx.x, I = bk.y * BY + threadIdx.y;
@@ -1246,7 +1293,11 @@ template <class T> size_t staged_bytes(int ts, int h, int ti = 0) {
 }
 
 // mg_cycles V-cycles (mg_vcycle, :299-323) on level 0 of t.
-template <class T> void launch_vcycles(Solver& s, const Tail<T>& t, const int* stop) {
+// x64 (mixed form): the FP64 p' that level 0's last up-kernel adds its result to when it is a
+// staged one (first64: from p' = 0, not read); returns whether it did, else the caller updates.
+template <class T>
+bool launch_vcycles(Solver& s, const Tail<T>& t, const int* stop, double* x64 = nullptr,
+                    int first64 = 0) {
     const sg_params& p = s.p;
     const size_t lds = lds_bytes<T>(s, t.t0);
     const bool tiled = tiled_ok(s);
@@ -1262,6 +1313,7 @@ template <class T> void launch_vcycles(Solver& s, const Tail<T>& t, const int* s
     const double bn = static_cast<double>(p.NX) * p.NY;
     if (bfused && !(staged && s.mg_staged == 0 && t.tm >= 1))
         throw HipError("fused mean removal without staged level-0 tiles");
+    bool updated = false;
     for (int cyc = 0; cyc < p.mg_cycles; ++cyc) {
         const int v = s.vcount++;
         for (int l = 0; l < t.tm; ++l) {
@@ -1302,6 +1354,10 @@ template <class T> void launch_vcycles(Solver& s, const Tail<T>& t, const int* s
             DU<T> du{F, C, F.xt, F.x, p.mg_post, 0, t.omega};
             if (bfused && l == 0)
                 du.bmean = s.scal + S_BSUM, du.bn = bn;
+            if (x64 && l == 0 && cyc + 1 == p.mg_cycles && staged && l >= s.mg_staged) {
+                du.x64 = x64, du.ld64 = static_cast<int>(s.ld), du.first64 = first64;
+                updated = true;
+            }
             if (staged && l >= s.mg_staged && big(F)) {
                 emit<DU<T>, k_mg_up_s<T, 32, BR<T>>, sg_n<T, 32, BR<T>>()>(
                     s, dim3(cdiv(F.ny, 32), cdiv(F.nx, BR<T>)), dim3(sg_n<T, 32, BR<T>>()),
@@ -1326,6 +1382,7 @@ template <class T> void launch_vcycles(Solver& s, const Tail<T>& t, const int* s
         }
     }
     s.tag = 0;
+    return updated;
 }
 
 template <class T> Tail<T> make_tail(Solver& s, Lv<T> (*view)(const MgLevel&)) {
@@ -1455,19 +1512,20 @@ void launch_mg_solve(Solver& s, const int* stop, bool has_guess) {
     else
         launch_remove_mean0(s, s.F(SG_F_B), stop,
                             !has_guess && !warm ? s.F(SG_F_PC) : nullptr); // and p' = 0
-    if (warm) {
-        launch_warm_guess(s, stop, bfused);
-        has_guess = true;
-    }
-    emit<Mix, k_mx_init, BX * BY>(s, g, dim3(BX, BY), 0, m, stop);
+    if (warm)
+        emit<GuessMix, k_mx_guess, BX * BY>(
+            s, g, dim3(BX, BY), 0, GuessMix{launch_warm_guess(s, stop, bfused), m}, stop);
+    else
+        emit<Mix, k_mx_init, BX * BY>(s, g, dim3(BX, BY), 0, m, stop);
+    has_guess = has_guess || warm;
     launch_builds(s, t, stop);
     for (int k = 0; k < p.mg_dc_maxit; ++k) {
-        if (k > 0 || has_guess) {
+        if (k > 0 || (has_guess && !warm)) { // the guess's defect is formed by k_mx_guess
             emit<Mix, k_mx_defect, BX * BY>(s, g, dim3(BX, BY), 0, m, stop);
         }
-        launch_vcycles(s, t, stop);
         m.first = bfused && k == 0 && !has_guess ? 1 : 0;
-        emit<Mix, k_mx_update, BX * BY>(s, g, dim3(BX, BY), 0, m, stop);
+        if (!launch_vcycles(s, t, stop, m.x, m.first))
+            emit<Mix, k_mx_update, BX * BY>(s, g, dim3(BX, BY), 0, m, stop);
     }
     s.b_mean_pending = false;
     finish_pc_mean(s, stop);

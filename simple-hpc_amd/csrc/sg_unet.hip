@@ -25,6 +25,7 @@
 // and the mixed-precision multigrid solve then starts from it. The stopping rule of the outer loop is untouched: a run ends only when the
 // SIMPLE residuals meet the tolerance.
 #include "sg_internal.hpp"
+#include "sg_warm.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -1203,28 +1204,6 @@ unsigned char* scratch_here() {
 
 namespace {
 
-// Level 0 of the p' system as the warm start reads it: AW, AS and AP derived from AE and AN when
-// the assembly was lean, and b - mean when b's mean removal is pending (the values the full
-// assembly and k_total_sub would have stored, by the same expressions as the mixed form's
-// conversions).
-struct L0View {
-    const double *AE, *AW, *AN, *AS, *AP, *b; // interior cell (0, 0) at index 0, stride ld
-    int ld, nx, ny;
-    int lean;             // AW, AS, AP not assembled
-    const double* bmean;  // b's total when its mean removal is pending, else null
-    double bn;
-    __device__ __forceinline__ double bv(size_t q) const { return bmean ? b[q] - *bmean / bn : b[q]; }
-    __device__ __forceinline__ double aw(int j, size_t q) const {
-        return lean ? (j == 0 ? 0.0 : AE[q - 1]) : AW[q];
-    }
-    __device__ __forceinline__ double as(int i, size_t q) const {
-        return lean ? (i == nx - 1 ? 0.0 : AN[q + ld]) : AS[q];
-    }
-    __device__ __forceinline__ double ap(int i, int j, size_t q) const {
-        return lean ? AE[q] + aw(j, q) + AN[q] + as(i, q) : AP[q];
-    }
-};
-
 struct FeatArgs {
     L0View l;           // level 0
     int bs;             // block side
@@ -1354,59 +1333,6 @@ __global__ void k_feat_input_f32(const float* cb, const float* ca, int n, const 
         x[static_cast<size_t>(e) * IN_PAD + c] = v[c];
 }
 
-struct GuessArgs {
-    const float* y;     // prediction on the coarse grid
-    const double* st;   // statistics (st[2] = scale)
-    double* pc;         // level-0 p' (interior cell (0,0) at pc[0], stride ld)
-    L0View l;           // level 0 of the p' system
-    int ld, nx, ny, bs;
-    double* part;       // per-row partials: [nx][2] (defect^2, b^2)
-    int* flag;          // 1: keep the guess
-    Ctl* ctl;
-};
-
-// The guess g at interior cell (i, j): scale * the prediction interpolated bilinearly from the
-// coarse cell centres (clamped at the walls) to the fine cell centre; 0 outside the interior (p''s
-// halo stays zero). Evaluated where it is needed (the energy sums and the final write), so it is
-// never stored and read back. The interpolation is separable: each axis gives its two coarse
-// neighbours and weight (GAxis, computed once per row or column where a kernel reuses it), and
-// gval combines them. The block side bs is a power of two, so (k + 1/2) / bs is computed as a
-// product by 1 / bs, exactly.
-struct GAxis {
-    int o0, o1; // offsets of the two coarse neighbours (rows: times the coarse row length)
-    float w;    // weight of the second
-    bool in;    // the fine index lies in the interior
-};
-__device__ __forceinline__ GAxis gaxis(int k, int n, int nc, float inv, int stride) {
-    GAxis r;
-    r.in = k >= 0 && k < n;
-    const float f = (k + 0.5f) * inv - 0.5f;
-    const float c = fminf(fmaxf(f, 0.0f), static_cast<float>(nc - 1));
-    const int k0 = min(static_cast<int>(c), nc - 1), k1 = min(k0 + 1, nc - 1);
-    r.o0 = k0 * stride;
-    r.o1 = k1 * stride;
-    r.w = c - k0;
-    return r;
-}
-__device__ __forceinline__ double gval(const GuessArgs& a, const GAxis& r, const GAxis& c) {
-    if (!(r.in && c.in))
-        return 0.0;
-    const float* y = a.y;
-    const float wi = r.w, wj = c.w;
-    const float v = (1.0f - wi) * ((1.0f - wj) * y[r.o0 + c.o0] + wj * y[r.o0 + c.o1]) +
-                    wi * ((1.0f - wj) * y[r.o1 + c.o0] + wj * y[r.o1 + c.o1]);
-    return static_cast<double>(v) * a.st[2];
-}
-__device__ __forceinline__ GAxis grow(const GuessArgs& a, int i) {
-    return gaxis(i, a.nx, a.nx / a.bs, 1.0f / a.bs, a.ny / a.bs);
-}
-__device__ __forceinline__ GAxis gcol(const GuessArgs& a, int j) {
-    return gaxis(j, a.ny, a.ny / a.bs, 1.0f / a.bs, 1);
-}
-__device__ __forceinline__ double guess_at(const GuessArgs& a, int i, int j) {
-    return gval(a, grow(a, i), gcol(a, j));
-}
-
 // The guess enters scaled by alpha = b.g / g.A.g, the minimiser of the energy
 // E(alpha g) = alpha^2 g.A.g / 2 - alpha b.g, whose minimum -(b.g)^2 / (2 g.A.g) is never above
 // E(0) = 0: since E(x) - E(p') = |x - p'|_A^2 / 2, the scaled guess is never farther from the
@@ -1508,14 +1434,6 @@ __device__ __forceinline__ void k_guess_rows(const GuessArgs& a, const Blk& bk) 
     }
 }
 
-// p' = alpha g over the interior (0 if the guess was discarded: it may hold non-finite values).
-__device__ __forceinline__ void k_guess_scale(const GuessArgs& a, const Blk& bk) {
-    const int j = bk.x * 64 + threadIdx.x, i = bk.y * 4 + threadIdx.y;
-    if (i >= a.nx || j >= a.ny)
-        return;
-    a.pc[static_cast<size_t>(i) * a.ld + j] = *a.flag ? guess_at(a, i, j) * a.part[2 * a.nx] : 0.0;
-}
-
 } // namespace
 
 // Per-solver state of the warm start.
@@ -1608,9 +1526,10 @@ void warm_attach(Solver& s, UNet* net, int level) {
     drop_graphs(s);
 }
 
-// The guess: features, network, p' = prediction * scale, acceptance test (enqueued on s.stream;
-// every kernel is stream-ordered, so the sequence can be captured).
-void launch_warm_guess(Solver& s, const int* stop, bool bfused) {
+// The guess up to its decision: features, network, the energy sums and the acceptance test
+// (enqueued on s.stream; every kernel is stream-ordered, so the sequence can be captured). The
+// caller applies it (p' = alpha g or 0) with its own level-0 conversions (k_mx_guess).
+GuessArgs launch_warm_guess(Solver& s, const int* stop, bool bfused) {
     Warm& w = *s.warm;
     if (!w.net)
         throw ArgError("pressure_warm_start = learned but no network is attached");
@@ -1629,10 +1548,9 @@ void launch_warm_guess(Solver& s, const int* stop, bool bfused) {
     const MgLevel& L0 = s.mg[0];
     GuessArgs g{c.y,   w.st,   L0.x, l0_view(s, bfused), s.ld, s.p.NX, s.p.NY,
                 w.bs,  w.part, w.flag, s.ctl};
-    const dim3 grid(cdiv(s.p.NY, 64), cdiv(s.p.NX, 4));
     emit<GuessArgs, k_guess_rows, 256>(s, dim3(s.p.NX), dim3(256), 0, g, stop);
     emit<GuessArgs, k_guess_decide, GD_T>(s, dim3(1), dim3(GD_T), 0, g, stop);
-    emit<GuessArgs, k_guess_scale, 256>(s, grid, dim3(64, 4), 0, g, stop);
+    return g;
 }
 
 } // namespace sg
