@@ -22,7 +22,9 @@ line:
   - `roofline`: the dominant kernel (one u multicolour sweep) timed with hipEvents on the
     solver's stream, credited with its algorithmic bytes per cell (the reference sweep's arrays
     read and written once: 9 coefficients, p and u read, u written, 96 B), against 8 TB/s; the
-    HBM traffic per launch from the committed rocprofv3 counter summary;
+    HBM traffic per launch from the committed rocprofv3 counter summary; beside it
+    (`outer_iteration`) the plain form's whole outer iteration credited the same way (about
+    1851 B per cell);
   - `cpu_baseline`: the reference's own OpenMP CPU code (built from its sources into oracle/_ref;
     the oracle restating it where that build is absent) on this host's cores, on a bounded
     sample of the same workload, and its projection to the tolerance.
@@ -50,6 +52,12 @@ sys.path.insert(0, os.path.join(ROOT, "simple-hpc_amd", "py"))
 HBM_PEAK_GBS = 8000.0
 # The reference sweep's algorithmic bytes per cell: 9 coefficients, p and u read, u written.
 SWEEP_BYTES_PER_CELL = 96.0
+# A whole FP64 outer iteration with multigrid V(2,2) p', each array of the reference's passes read
+# and written once per logical pass, per cell: assemble_u 88, residual_u 88, 4 u sweeps 4 x 96,
+# the same for v, assemble_p 80, the p' solve about 579 (level 0: smoothing 4 x 64, residual 64,
+# restriction 12, prolongation 18, operator build 42, times 4/3 for the coarse levels; mean
+# projections 48, zero fill 8), correct 72.
+ITERATION_BYTES_PER_CELL = 88 + 88 + 4 * 96 + 88 + 88 + 4 * 96 + 80 + (4 * 64 + 64 + 12 + 18 + 42) * 4 / 3 + 48 + 8 + 72
 WEIGHTS = os.path.join(ROOT, "simple-hpc_amd", "weights", "unet_warm_start.bin")
 # HBM bytes per launch of the dominant kernel from rocprofv3 FETCH_SIZE / WRITE_SIZE runs,
 # calibrated on a copy kernel (recorded with the commands that made them).
@@ -245,6 +253,15 @@ def cpu_baseline(sg, n, seconds=12.0, iterations=None):
     return out
 
 
+def iteration_roofline(n, form, ms):
+    """One whole outer iteration of `form` credited with ITERATION_BYTES_PER_CELL, against 8 TB/s."""
+    b = ITERATION_BYTES_PER_CELL * n * n
+    achieved = b / (ms * 1e-3) / 1e9
+    return {"form": form, "algorithmic_bytes_per_cell": ITERATION_BYTES_PER_CELL,
+            "algorithmic_bytes": b, "ms": ms, "achieved": achieved, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS}
+
+
 def run_cavity(sg, a):
     import numpy as np  # noqa: F401
     n = a.n
@@ -270,7 +287,9 @@ def run_cavity(sg, a):
             "algorithmic_bytes_per_cell": SWEEP_BYTES_PER_CELL, "cells_per_launch": cells,
             "launches_per_iteration": per,
             "device_copy": {"bytes_per_launch": cbytes, "us": csec * 1e6,
-                            "GBs": cbytes / csec / 1e9}}
+                            "GBs": cbytes / csec / 1e9},
+            "outer_iteration": iteration_roofline(
+                n, "plain", next(r for r in table if r["form"] == "plain")["ms_per_iter"])}
     out = {"n_gpus": 1, "warmup": a.warmup, "scaling": "weak", "vs_baseline": None,
            "data": "synthetic (lid-driven cavity from zero fields, lid velocity 1)"}
     workload = (f"lid-driven cavity {n}x{n}, Re 1000, QUICK, 4th-order diffusion, gs_multicolour "
@@ -415,6 +434,7 @@ def run_large(sg, steps, warmup, knobs=None):
             sec, bytes_, per = f.s.probe(sg.K_SWEEP_U, reps=20)
             r["sweep_u_us"] = sec * 1e6
             r["sweep_u_GBs_algorithmic"] = SWEEP_BYTES_PER_CELL * n * (n - 1) / sec / 1e9
+            r["outer_iteration_roofline"] = iteration_roofline(n, form, r["ms_per_iter"])
         if form == "learned":
             r["guesses_kept_discarded"] = list(sg.warm_stats(f.s))
             r["warm_level"] = f.level
