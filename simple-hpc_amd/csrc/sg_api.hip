@@ -132,7 +132,7 @@ void enqueue_iteration(Solver& s, const int* stop, bool eager, hipEvent_t* ev, i
     } else {
         // Without per-phase events, u's ghost rows are set by the assembly launch itself.
         const bool fuse_bc = defer && s.pku.rows >= 2;
-        launch_assemble(s, true, stop, fuse_bc, face_ok(s, true));
+        launch_assemble(s, true, stop, fuse_bc, face_ok(s, true), true);
         mark(1);
         if (!fuse_bc)
             launch_bc(s, true, stop);
@@ -140,7 +140,7 @@ void enqueue_iteration(Solver& s, const int* stop, bool eager, hipEvent_t* ev, i
         mark(3);
         launch_mom_sweeps(s, true, p.maxit_inner_u, stop, s.scal + S_RES_U, defer);
         mark(4);
-        launch_assemble(s, false, stop, false, face_ok(s, false));
+        launch_assemble(s, false, stop, false, face_ok(s, false), true);
         mark(5);
         // Without per-phase events, v's ghost columns are set by its first sweep.
         const bool bcv = defer && bcv_in_sweep(s);
@@ -195,6 +195,7 @@ void capture_graph(Solver& s, int reps, hipGraphExec_t* out) {
             s.dc_stale[0] = d0, s.dc_stale[1] = d1;
             s.p_lean = lean, s.pc_in_scratch = pcs, s.b_raw = braw;
             s.fuse_pc_mean = s.pc_mean_pending = s.fuse_b_mean = s.b_mean_pending = false;
+            s.ap_in_sweep[0] = s.ap_in_sweep[1] = false;
         }
     } restore{s,           s.face[0],   s.face[1], s.pt_stale[0],    s.pt_stale[1],
               s.dc_stale[0], s.dc_stale[1], s.p_lean, s.pc_in_scratch, s.b_raw};

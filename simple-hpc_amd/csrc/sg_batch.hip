@@ -82,6 +82,7 @@ void build_plan(Batch& b) {
             s.dc_stale[0] = d0, s.dc_stale[1] = d1;
             s.p_lean = lean, s.pc_in_scratch = pcs, s.b_raw = braw;
             s.fuse_pc_mean = s.pc_mean_pending = s.fuse_b_mean = s.b_mean_pending = false;
+            s.ap_in_sweep[0] = s.ap_in_sweep[1] = false;
         };
         s.rec = &b.recs[c];
         try {

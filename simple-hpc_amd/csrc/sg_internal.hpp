@@ -146,6 +146,8 @@ struct Solver {
     bool face[2] = {};                 // u, v: the face pack holds the current coefficients (the
                                        // coefficient pack is stale until ensure_packed)
     bool pt_stale[2] = {};             // u, v: p changed since the pressure terms were evaluated
+    bool ap_in_sweep[2] = {};          // u, v: row-major AP of the sweep range not yet written:
+                                       // the next first (residual) face-form sweep writes it
     bool p_lean = false;               // AWP, ASP, APP not written by the last assemble_p (stale
                                        // until ensure_p_full)
     // p' mean removal carried by correct (outer iterations with the multigrid p' solve): fuse_pc_mean
@@ -222,8 +224,10 @@ void emit(Solver& s, dim3 grid, dim3 block, size_t lds, const A& a, const int* s
 // momentum (sg_momentum.hip)
 // with_bc (u only): the same launch also sets u's ghost rows (launch_bc).
 // face: the face form (face pack and AP only; the sweeps then recompute the coefficients).
+// ap_in_sweep (face form, the outer iteration): AP of the sweep range is left to the first sweep,
+// which evaluates every coefficient of those cells for the residual anyway (ap_in_sweep flag).
 void launch_assemble(Solver& s, bool is_u, const int* stop, bool with_bc = false,
-                     bool face = false);
+                     bool face = false, bool ap_in_sweep = false);
 // Whether outer iterations use the face form for this component (multicolour sweeps, at least one).
 bool face_ok(const Solver& s, bool is_u);
 // Bring the coefficient pack up to date if the face pack holds the current coefficients.

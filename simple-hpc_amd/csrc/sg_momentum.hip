@@ -290,6 +290,9 @@ struct AsmArgs {
     long poff;   // second pressure cell (+1 for u, -ld for v)
     double* ubc; // u only: also set u's ghost rows (bc_u) in this launch, or null
     MomConv cv;
+    // Face form: AP of cells in rows ap_i0..ap_i1, columns ap_j0..ap_j1 is left to the first sweep
+    // (an empty range: AP of every cell here).
+    int ap_i0, ap_i1, ap_j0, ap_j1;
 };
 
 // One control volume, written at packed entry b (and AP row-major at q).
@@ -437,12 +440,66 @@ __device__ __forceinline__ void k_assemble_face(const AsmArgs& a, const Blk& bk)
             b[F_VN * fs] = fvn[c];
         if (!xrow && !xcol) {
             b[F_PT * fs] = fpt[c];
-            a.APn[static_cast<size_t>(i) * ld + j] =
-                mom_coefs(a.cv, i, j, fue[c], fuw[c], fvn[c], fvs[c]).v[8];
+            if (i < a.ap_i0 || i > a.ap_i1 || j < a.ap_j0 || j > a.ap_j1)
+                a.APn[static_cast<size_t>(i) * ld + j] =
+                    mom_coefs(a.cv, i, j, fue[c], fuw[c], fvn[c], fvs[c]).v[8];
         }
     }
     if (IS_U && a.ubc && i - fk.ai0 < 2)
         ghost_rows_u(a, i - fk.ai0, jj);
+}
+
+// The face form's assembly with one thread per cell in row-major order (FRX consecutive columns
+// by FRY rows per workgroup): the u, v and p reads and the row-major AP writes are coalesced; each
+// cell's face values go to its colour's plane of the face pack. The same values as
+// k_assemble_face.
+constexpr int FRX = 64, FRY = 4;
+template <bool IS_U, int NCOL>
+__device__ __forceinline__ void k_assemble_face_rm(const AsmArgs& a, const Blk& bk) {
+    const Pack& fk = a.pk;
+    int tx, ty;
+    xcd_tile(bk, tx, ty);
+    const int j = fk.aj0 + tx * FRX + threadIdx.x;
+    const int i = fk.ai0 + ty * FRY + threadIdx.y;
+    if (i > fk.ai1 || j > fk.aj1)
+        return;
+    const long ld = a.ld;
+    const size_t fs = fk.astride;
+    const bool xrow = i == fk.ai1, xcol = j == fk.aj0; // the extra row and column
+    const size_t q = static_cast<size_t>(i) * ld + j;
+    double* b = fk.base + fidx<NCOL>(fk, colour_of(NCOL, i, j), i, j);
+    double ue = 0.0, vn = 0.0;
+    if (!xcol) {
+        vn = face_vn<IS_U>(a.v, q, ld);
+        b[F_VN * fs] = vn;
+    }
+    if (!xrow) {
+        ue = face_ue<IS_U>(a.u, q, ld);
+        b[F_UE * fs] = ue;
+    }
+    if (!xrow && !xcol) {
+        b[F_PT * fs] = (a.pf[q] - a.pf[q + a.poff]) * a.h;
+        if (i < a.ap_i0 || i > a.ap_i1 || j < a.ap_j0 || j > a.ap_j1) {
+            double uw, vs;
+            if (IS_U) {
+                uw = (a.u[q] + a.u[q - 1]) / 2;
+                vs = (a.v[q + ld] + a.v[q + ld + 1]) / 2;
+            } else {
+                uw = (a.u[q - 1] + a.u[q - ld - 1]) / 2;
+                vs = (a.v[q] + a.v[q + ld]) / 2;
+            }
+            a.APn[q] = mom_coefs(a.cv, i, j, ue, uw, vn, vs).v[8];
+        }
+    }
+    if (IS_U && a.ubc && i - fk.ai0 < 2 && j >= a.cv.H + 1 && j <= a.cv.NY + a.cv.H) {
+        double* u = a.ubc;
+        const int H = a.cv.H;
+        if (i == fk.ai0)
+            u[static_cast<size_t>(H - 1) * ld + j] = 2 - u[static_cast<size_t>(H) * ld + j];
+        else
+            u[static_cast<size_t>(a.cv.NX + H) * ld + j] =
+                -u[static_cast<size_t>(a.cv.NX + H - 1) * ld + j];
+    }
 }
 
 // ---- FP32 momentum smoothing inside FP64 defect correction (momentum_precision = mixed) ----
@@ -804,6 +861,9 @@ struct SweepArgs {
     // FIRST only: v's ghost columns (src/momentum_v.cpp:24-34). Cells (i, bjw) and (i, bje) of
     // rows bi0..bi1 are loaded as minus their inner neighbour; none when bi1 < bi0.
     int bi0, bi1, bjw, bje;
+    // FIRST only: row-major AP of the core cells, from the coefficients the residual uses (the
+    // face assembly left it out: AsmArgs::ap_*), or null.
+    double* apn;
 };
 
 // FIRST: a second LDS copy keeps the loaded (pre-sweep) values of the tile and the ring of R
@@ -880,6 +940,8 @@ __device__ __forceinline__ void k_mom_sweep(const SweepArgs& sa, const Blk& bk) 
                                (cee * S0(i, j + 2) + cww * S0(i, j - 2) + cnn * S0(i - 2, j) +
                                 css * S0(i + 2, j));
             res[(i - ti0) * TJ + (j - tj0)] = fabs(nb0 + ct - cp * S0(i, j));
+            if (sa.apn)
+                sa.apn[static_cast<size_t>(i) * ld + j] = cp;
         }
         double nb = ce * SU(i, j + 1) + cw * SU(i, j - 1) + cn * SU(i - 1, j) + cs * SU(i + 1, j);
         if (NCOL != 2)
@@ -1118,7 +1180,8 @@ Pack make_pack(const sg_params& p, bool is_u) {
     return k;
 }
 
-void launch_assemble(Solver& s, bool is_u, const int* stop, bool with_bc, bool face) {
+void launch_assemble(Solver& s, bool is_u, const int* stop, bool with_bc, bool face,
+                     bool ap_in_sweep) {
     const sg_params& p = s.p;
     AsmArgs a{};
     a.u = s.F(SG_F_U);
@@ -1131,13 +1194,26 @@ void launch_assemble(Solver& s, bool is_u, const int* stop, bool with_bc, bool f
     a.poff = is_u ? 1 : -static_cast<long>(s.ld);
     a.ubc = is_u && with_bc ? s.F(SG_F_U) : nullptr;
     a.cv = mom_conv(s);
+    a.ap_i0 = 0, a.ap_i1 = -1, a.ap_j0 = 0, a.ap_j1 = -1;
+    const bool skip = face && ap_in_sweep;
+    if (skip) { // the sweep range (mom_args), whose first sweep writes AP
+        const MomArgs m = mom_args(s, is_u);
+        a.ap_i0 = m.i0, a.ap_i1 = m.i1, a.ap_j0 = m.j0, a.ap_j1 = m.j1;
+    }
+    s.ap_in_sweep[is_u ? 0 : 1] = skip;
     const dim3 grid(cdiv(a.pk.kp, BX), cdiv(a.pk.rows, AY));
     if (face) {
-        const dim3 fg(cdiv(a.pk.kp, FBX), cdiv(a.pk.rows, FAY)), fb(FBX, FAY);
-        if (is_u)
-            emit<AsmArgs, k_assemble_face<true>, FBX * FAY>(s, fg, fb, 0, a, stop);
-        else
-            emit<AsmArgs, k_assemble_face<false>, FBX * FAY>(s, fg, fb, 0, a, stop);
+        const dim3 fg(cdiv(a.pk.aj1 - a.pk.aj0 + 1, FRX), cdiv(a.pk.rows, FRY)), fb(FRX, FRY);
+        if (a.pk.ncol == 2) {
+            if (is_u)
+                emit<AsmArgs, k_assemble_face_rm<true, 2>, FRX * FRY>(s, fg, fb, 0, a, stop);
+            else
+                emit<AsmArgs, k_assemble_face_rm<false, 2>, FRX * FRY>(s, fg, fb, 0, a, stop);
+        } else if (is_u) {
+            emit<AsmArgs, k_assemble_face_rm<true, 5>, FRX * FRY>(s, fg, fb, 0, a, stop);
+        } else {
+            emit<AsmArgs, k_assemble_face_rm<false, 5>, FRX * FRY>(s, fg, fb, 0, a, stop);
+        }
     } else if (is_u) {
         emit<AsmArgs, k_assemble<true>, BX * AY>(s, grid, dim3(BX, AY), 0, a, stop);
     } else {
@@ -1237,9 +1313,9 @@ void sweep_tile_f(Solver& s, const SweepArgs& a, bool face, const int* stop,
 
 template <int NCOL>
 void sweep_tile(Solver& s, const MomArgs& m, const double* src, double* dst, double* cell,
-                int first, bool face, const int* stop, bool bcv) {
+                int first, bool face, const int* stop, bool bcv, double* apn) {
     const sg_params& p = s.p;
-    SweepArgs a{m, src, dst, cell, s.nr, s.nc, first, 0, -1, p.H, p.NY + p.H + 1};
+    SweepArgs a{m, src, dst, cell, s.nr, s.nc, first, 0, -1, p.H, p.NY + p.H + 1, apn};
     if (bcv) // the rows of launch_bc's v ghost columns
         a.bi0 = p.H, a.bi1 = p.NX + p.H - 1;
     constexpr auto shapes = std::make_integer_sequence<int, N_MOM_TILES>{};
@@ -1259,10 +1335,15 @@ void launch_mom_sweep_once(Solver& s, bool is_u, const double* src, double* dst,
     const bool face = s.face[is_u ? 0 : 1];
     if (bcv && (is_u || !first))
         throw HipError("v's boundary condition rides only on v's first sweep");
+    // The AP the face assembly left out, written by this sweep's residual pass.
+    bool& pend = s.ap_in_sweep[is_u ? 0 : 1];
+    double* apn = first && face && pend ? s.F(is_u ? SG_F_CU_AP : SG_F_CV_AP) : nullptr;
+    if (apn)
+        pend = false;
     if (m.pk.ncol == 2)
-        sweep_tile<2>(s, m, src, dst, cell, first ? 1 : 0, face, stop, bcv);
+        sweep_tile<2>(s, m, src, dst, cell, first ? 1 : 0, face, stop, bcv, apn);
     else
-        sweep_tile<5>(s, m, src, dst, cell, first ? 1 : 0, face, stop, bcv);
+        sweep_tile<5>(s, m, src, dst, cell, first ? 1 : 0, face, stop, bcv, apn);
 }
 
 bool bcv_in_sweep(const Solver& s) {

@@ -27,11 +27,13 @@ def _interior(p, a):
     return a[p.H:p.NX + p.H, p.H + 1:p.NY + p.H + 1]
 
 
-@pytest.mark.parametrize("dc", [1, 2])
-def test_mixed_pc_solve_close_to_fp64(dc):
-    # dc defect-correction steps of one FP32 V-cycle each are dc V-cycles: compare with FP64
-    # mg_cycles = dc (a V-cycle is a linear stationary iteration x += B (b - A x)).
-    p64 = _system(128, 1000, mg_cycles=dc)
+@pytest.mark.parametrize("dc,cycles", [(1, 1), (2, 1), (1, 2)])
+def test_mixed_pc_solve_close_to_fp64(dc, cycles):
+    # dc defect-correction steps of `cycles` FP32 V-cycles each are dc * cycles V-cycles: compare
+    # with FP64 mg_cycles = dc * cycles (a V-cycle is a linear stationary iteration
+    # x += B (b - A x)). With two cycles per step the first one's result stays in FP32 for the
+    # second, whose level-0 up-kernel adds the step's correction to the FP64 p'.
+    p64 = _system(128, 1000, mg_cycles=dc * cycles)
     cpu = po.Oracle(p64)
     for _ in range(5):
         cpu.outer_iteration()
@@ -41,15 +43,16 @@ def test_mixed_pc_solve_close_to_fp64(dc):
     state = {n: cpu.get(n) for n in sg.FIELDS}
     cpu.solve_p()
     x64 = _interior(p64, cpu.get("pc"))
-    pm = _system(128, 1000, mg_precision="mixed", mg_dc_maxit=dc)
+    pm = _system(128, 1000, mg_precision="mixed", mg_dc_maxit=dc, mg_cycles=cycles)
     g = sg.Solver(pm)
     for n, a in state.items():
         g.set(n, a)
     g.solve_p()
     xm = _interior(pm, g.get("pc"))
     err = np.abs(xm - x64).max() / np.abs(x64).max()
-    print(f"mixed p' (dc={dc}) vs fp64 p': rel max err {err:.3e}", flush=True)
-    # about 10x the measured 2.6e-7 (one step) and 7.3e-8 (two steps)
+    print(f"mixed p' (dc={dc}, cycles={cycles}) vs fp64 p': rel max err {err:.3e}", flush=True)
+    # about 10x the measured 2.6e-7 (one step) and 7.3e-8 (two steps); two FP32 cycles in one
+    # step are bounded as one step (the FP32 rounding of its correction)
     assert err < (3e-6 if dc == 1 else 1e-6)
     # FP64 pieces: b is made mean-free in FP64 exactly as in the reference.
     assert np.array_equal(g.get("b"), cpu.get("b"))
