@@ -376,83 +376,15 @@ template <bool IS_U> __device__ __forceinline__ void k_assemble(const AsmArgs& a
         ghost_rows_u(a, i - pk.ai0, jj);
 }
 
-// Workgroups of the face form's assembly: FBX cells of each colour by FAY rows (32 x 16 measured
-// 18.8 / 22.2 us for u / v at 1024^2, against 21.3 / 27.1 with 64 x 16: the v faces read three rows
-// of v and two of u and p, and a narrower block keeps more of them in L1 and L2). Blocks are
-// handed to the XCDs in contiguous bands (xcd_tile), so the rows vertically adjacent blocks both
-// read are fetched into one L2: at 1024^2 the v assembly's fetched bytes fell from 42.9 to 28.0 MB
-// per launch and its time from 22.3 to 20.0 us (u: 18.9 -> 18.3 us) against row-major order.
-constexpr int FBX = 32, FAY = 16;
-
 // Face form of the assembly: ue, vn and the pressure term of every face-pack cell (the extra
 // column needs only ue and the extra row only vn: they are the west faces of the first column and
-// the south faces of the last row), and AP of every assembly-range cell, row-major (assemble_p and
-// correct read it). With ubc, u's ghost rows as in k_assemble.
-template <bool IS_U>
-__device__ __forceinline__ void k_assemble_face(const AsmArgs& a, const Blk& bk) {
-    const Pack& fk = a.pk;
-    int tx, ty;
-    xcd_tile(bk, tx, ty);
-    const int kk = tx * FBX + threadIdx.x;
-    const int i = fk.ai0 + ty * FAY + threadIdx.y;
-    if (i > fk.ai1)
-        return;
-    const long ld = a.ld;
-    const size_t fs = fk.astride;
-    const bool xrow = i == fk.ai1; // the extra row
-    constexpr int MC = 5;
-    double fue[MC], fuw[MC], fvn[MC], fvs[MC], fpt[MC];
-    int jj[MC];
-#pragma unroll
-    for (int c = 0; c < MC; ++c) {
-        const int j = jfirst(fk, c < fk.ncol ? c : 0, i) + fk.ncol * kk;
-        jj[c] = (c < fk.ncol && j <= fk.aj1) ? j : -1;
-        fue[c] = fuw[c] = fvn[c] = fvs[c] = fpt[c] = 0.0;
-        if (jj[c] < 0)
-            continue;
-        const size_t q = static_cast<size_t>(i) * ld + j;
-        const bool xcol = j == fk.aj0; // the extra column
-        if (!xcol)
-            fvn[c] = face_vn<IS_U>(a.v, q, ld);
-        if (!xrow)
-            fue[c] = face_ue<IS_U>(a.u, q, ld);
-        if (!xrow && !xcol) {
-            if (IS_U) {
-                fuw[c] = (a.u[q] + a.u[q - 1]) / 2;
-                fvs[c] = (a.v[q + ld] + a.v[q + ld + 1]) / 2;
-            } else {
-                fuw[c] = (a.u[q - 1] + a.u[q - ld - 1]) / 2;
-                fvs[c] = (a.v[q] + a.v[q + ld]) / 2;
-            }
-            fpt[c] = (a.pf[q] - a.pf[q + a.poff]) * a.h;
-        }
-    }
-#pragma unroll
-    for (int c = 0; c < MC; ++c) {
-        const int j = jj[c];
-        if (j < 0)
-            continue;
-        const bool xcol = j == fk.aj0;
-        double* b = fk.base + pidx(fk, c, i, kk);
-        if (!xrow)
-            b[F_UE * fs] = fue[c];
-        if (!xcol)
-            b[F_VN * fs] = fvn[c];
-        if (!xrow && !xcol) {
-            b[F_PT * fs] = fpt[c];
-            if (i < a.ap_i0 || i > a.ap_i1 || j < a.ap_j0 || j > a.ap_j1)
-                a.APn[static_cast<size_t>(i) * ld + j] =
-                    mom_coefs(a.cv, i, j, fue[c], fuw[c], fvn[c], fvs[c]).v[8];
-        }
-    }
-    if (IS_U && a.ubc && i - fk.ai0 < 2)
-        ghost_rows_u(a, i - fk.ai0, jj);
-}
-
-// The face form's assembly with one thread per cell in row-major order (FRX consecutive columns
-// by FRY rows per workgroup): the u, v and p reads and the row-major AP writes are coalesced; each
-// cell's face values go to its colour's plane of the face pack. The same values as
-// k_assemble_face.
+// the south faces of the last row), and AP, row-major (assemble_p and correct read it), of the
+// assembly-range cells outside [ap_i0, ap_i1] x [ap_j0, ap_j1] (the first sweep stores the rest).
+// With ubc, u's ghost rows as in k_assemble. One thread per cell in row-major order, FRX
+// consecutive columns by FRY rows per workgroup (the workgroups handed to the XCDs in contiguous
+// bands, xcd_tile): the u, v and p reads and the AP writes are coalesced, and each cell's face
+// values go to its colour's plane of the face pack. (One thread per five cells of one colour each
+// read and wrote five columns apart: 18.3 / 20.0 us for u / v at 1024^2 then.)
 constexpr int FRX = 64, FRY = 4;
 template <bool IS_U, int NCOL>
 __device__ __forceinline__ void k_assemble_face_rm(const AsmArgs& a, const Blk& bk) {
@@ -548,6 +480,10 @@ __device__ __forceinline__ double bc_val(const AsmArgs& a, int i, int j) {
     }
     return x[static_cast<size_t>(i) * ld + j];
 }
+
+// Workgroups of k_asm_dc: FBX cells of each colour (one thread per five cells, one of each colour)
+// by FAY rows.
+constexpr int FBX = 32, FAY = 16;
 
 // The assembly of k_assemble (coefficients from the faces, AP row-major; u's ghost rows) plus, for
 // every cell of the sweep range, r0 = ((E + W + N + S) + (EE + WW + NN + SS)) + pterm - AP x0
