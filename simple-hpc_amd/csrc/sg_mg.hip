@@ -1163,17 +1163,31 @@ struct GuessMix {
     GuessArgs g;
     Mix m;
 };
-__device__ __forceinline__ void k_mx_guess(const GuessMix& a, const Blk& bk) {
+// LY (one row per workgroup): the prediction's rows read from an LDS copy (gy_stage).
+template <bool LY> __device__ __forceinline__ void k_mx_guess(const GuessMix& a, const Blk& bk) {
+    static_assert(!LY || BY == 1, "one fine row per workgroup");
+    __shared__ float sy[LY ? GY_ROWS * GY_MAX : 1];
     const Mix& m = a.m;
     const GuessArgs& g = a.g;
     const int J = bk.x * BX + threadIdx.x, I = bk.y * BY + threadIdx.y;
-    if (I >= m.nx || J >= m.ny)
+    if (I >= m.nx)
+        return;
+    const float* y = g.y;
+    int k0 = 0;
+    if constexpr (LY) {
+        k0 = gy_stage(g, I, sy);
+        y = sy;
+    }
+    if (J >= m.ny)
         return;
     const int keep = *g.flag;
-    const double alpha = g.part[2 * g.nx];
-    const GAxis rm = grow(g, I - 1), ri = grow(g, I), rp = grow(g, I + 1);
+    const double alpha = g.part[2 * g.nx], scale = g.st[2];
+    auto row = [&](int r) { return LY ? gy_row(g, r, k0) : grow(g, r); };
+    const GAxis rm = row(I - 1), ri = row(I), rp = row(I + 1);
     const GAxis cm = gcol(g, J - 1), cj = gcol(g, J), cp = gcol(g, J + 1);
-    auto xat = [&](const GAxis& r, const GAxis& c) { return keep ? gval(g, r, c) * alpha : 0.0; };
+    auto xat = [&](const GAxis& r, const GAxis& c) {
+        return keep ? gval(y, scale, r, c) * alpha : 0.0;
+    };
     const size_t q = static_cast<size_t>(I) * m.ld + J, f = static_cast<size_t>(I) * m.f.stride + J;
     const double x = xat(ri, cj);
     m.x[q] = x;
@@ -1512,9 +1526,13 @@ void launch_mg_solve(Solver& s, const int* stop, bool has_guess) {
     else
         launch_remove_mean0(s, s.F(SG_F_B), stop,
                             !has_guess && !warm ? s.F(SG_F_PC) : nullptr); // and p' = 0
-    if (warm)
-        emit<GuessMix, k_mx_guess, BX * BY>(
-            s, g, dim3(BX, BY), 0, GuessMix{launch_warm_guess(s, stop, bfused), m}, stop);
+    if (warm) {
+        const GuessMix gm{launch_warm_guess(s, stop, bfused), m};
+        if (p.NY / gm.g.bs <= GY_MAX)
+            emit<GuessMix, k_mx_guess<true>, BX * BY>(s, g, dim3(BX, BY), 0, gm, stop);
+        else
+            emit<GuessMix, k_mx_guess<false>, BX * BY>(s, g, dim3(BX, BY), 0, gm, stop);
+    }
     else
         emit<Mix, k_mx_init, BX * BY>(s, g, dim3(BX, BY), 0, m, stop);
     has_guess = has_guess || warm;

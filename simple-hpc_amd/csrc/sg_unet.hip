@@ -1384,13 +1384,23 @@ __device__ __forceinline__ void k_guess_decide(const GuessArgs& a, const Blk&) {
 
 // Per row: sums of g (A g) and b g for the guess g (one workgroup per row, fixed-order tree). A
 // thread takes the columns j = t, t + 256, ... in order, four at a time with all their operator
-// loads issued before the arithmetic.
-__device__ __forceinline__ void k_guess_rows(const GuessArgs& a, const Blk& bk) {
+// loads issued before the arithmetic. LY: the prediction's rows this row interpolates from are
+// read from an LDS copy (gy_stage).
+template <bool LY> __device__ __forceinline__ void k_guess_rows(const GuessArgs& a, const Blk& bk) {
     __shared__ double sd[256], sb[256];
+    __shared__ float sy[LY ? GY_ROWS * GY_MAX : 1];
     const int i = bk.x;
     const long s = a.ld;
     const L0View& l = a.l;
-    const GAxis rm = grow(a, i - 1), ri = grow(a, i), rp = grow(a, i + 1); // uniform
+    const float* y = a.y;
+    int k0 = 0;
+    if constexpr (LY) {
+        k0 = gy_stage(a, i, sy);
+        y = sy;
+    }
+    const double scale = a.st[2];
+    auto row = [&](int r) { return LY ? gy_row(a, r, k0) : grow(a, r); };
+    const GAxis rm = row(i - 1), ri = row(i), rp = row(i + 1); // uniform
     double d2 = 0.0, b2 = 0.0;
     for (int c0 = 0; c0 < a.ny; c0 += 4 * 256) {
         double ap[4], ae[4], aw[4], as[4], an[4], bv[4];
@@ -1410,9 +1420,11 @@ __device__ __forceinline__ void k_guess_rows(const GuessArgs& a, const Blk& bk) 
             const int j = c0 + k * 256 + threadIdx.x;
             if (j < a.ny) {
                 const GAxis cm = gcol(a, j - 1), cj = gcol(a, j), cp = gcol(a, j + 1);
-                const double x = gval(a, ri, cj);
-                const double ag = ap[k] * x - (ae[k] * gval(a, ri, cp) + aw[k] * gval(a, ri, cm) +
-                                               as[k] * gval(a, rp, cj) + an[k] * gval(a, rm, cj));
+                const double x = gval(y, scale, ri, cj);
+                const double ag = ap[k] * x - (ae[k] * gval(y, scale, ri, cp) +
+                                               aw[k] * gval(y, scale, ri, cm) +
+                                               as[k] * gval(y, scale, rp, cj) +
+                                               an[k] * gval(y, scale, rm, cj));
                 d2 += x * ag;
                 b2 += bv[k] * x;
             }
@@ -1548,7 +1560,10 @@ GuessArgs launch_warm_guess(Solver& s, const int* stop, bool bfused) {
     const MgLevel& L0 = s.mg[0];
     GuessArgs g{c.y,   w.st,   L0.x, l0_view(s, bfused), s.ld, s.p.NX, s.p.NY,
                 w.bs,  w.part, w.flag, s.ctl};
-    emit<GuessArgs, k_guess_rows, 256>(s, dim3(s.p.NX), dim3(256), 0, g, stop);
+    if (w.nyc <= GY_MAX)
+        emit<GuessArgs, k_guess_rows<true>, 256>(s, dim3(s.p.NX), dim3(256), 0, g, stop);
+    else
+        emit<GuessArgs, k_guess_rows<false>, 256>(s, dim3(s.p.NX), dim3(256), 0, g, stop);
     emit<GuessArgs, k_guess_decide, GD_T>(s, dim3(1), dim3(GD_T), 0, g, stop);
     return g;
 }

@@ -64,14 +64,19 @@ __device__ __forceinline__ GAxis gaxis(int k, int n, int nc, float inv, int stri
     r.w = c - k0;
     return r;
 }
-__device__ __forceinline__ double gval(const GuessArgs& a, const GAxis& r, const GAxis& c) {
+// y: the prediction (a.y), or the workgroup's copy of some of its rows (gy_stage) with row
+// offsets taken relative to the first of them (gy_row).
+__device__ __forceinline__ double gval(const float* y, double scale, const GAxis& r,
+                                       const GAxis& c) {
     if (!(r.in && c.in))
         return 0.0;
-    const float* y = a.y;
     const float wi = r.w, wj = c.w;
     const float v = (1.0f - wi) * ((1.0f - wj) * y[r.o0 + c.o0] + wj * y[r.o0 + c.o1]) +
                     wi * ((1.0f - wj) * y[r.o1 + c.o0] + wj * y[r.o1 + c.o1]);
-    return static_cast<double>(v) * a.st[2];
+    return static_cast<double>(v) * scale;
+}
+__device__ __forceinline__ double gval(const GuessArgs& a, const GAxis& r, const GAxis& c) {
+    return gval(a.y, a.st[2], r, c);
 }
 __device__ __forceinline__ GAxis grow(const GuessArgs& a, int i) {
     return gaxis(i, a.nx, a.nx / a.bs, 1.0f / a.bs, a.ny / a.bs);
@@ -81,6 +86,29 @@ __device__ __forceinline__ GAxis gcol(const GuessArgs& a, int j) {
 }
 __device__ __forceinline__ double guess_at(const GuessArgs& a, int i, int j) {
     return gval(a, grow(a, i), gcol(a, j));
+}
+
+// A workgroup whose cells lie in fine rows i - 1 .. i + 1 interpolates from at most GY_ROWS
+// consecutive coarse rows (from the first row of fine row i - 1 to the second of fine row i + 1:
+// the rows of gaxis do not decrease with the fine row; four at bs = 1, three above). gy_stage
+// copies them to sy (GY_ROWS x nyc floats, nyc <= GY_MAX) and returns the first one's index;
+// gy_row(a, i, k0) is grow with its offsets relative to that row.
+constexpr int GY_ROWS = 4, GY_MAX = 1024;
+__device__ __forceinline__ int gy_stage(const GuessArgs& a, int i, float* sy) {
+    const int nyc = a.ny / a.bs;
+    const int k0 = grow(a, max(i - 1, 0)).o0 / nyc, k1 = grow(a, min(i + 1, a.nx - 1)).o1 / nyc;
+    const int t = threadIdx.y * blockDim.x + threadIdx.x;
+    for (int e = t; e < (k1 - k0 + 1) * nyc; e += blockDim.x * blockDim.y)
+        sy[e] = a.y[static_cast<size_t>(k0) * nyc + e];
+    __syncthreads();
+    return k0;
+}
+__device__ __forceinline__ GAxis gy_row(const GuessArgs& a, int i, int k0) {
+    GAxis r = grow(a, i);
+    const int off = k0 * (a.ny / a.bs);
+    r.o0 -= off;
+    r.o1 -= off;
+    return r;
 }
 
 } // namespace sg
