@@ -108,8 +108,8 @@ def test_training_reduces_loss():
     net.close()
 
 
-def _mixed_params(n, re, learned):
-    text = (f"N_cells_x = {n}\nN_cells_y = {n}\nRe = {re}\nconvection_scheme = QUICK\n"
+def _mixed_params(n, re, learned, ny=None):
+    text = (f"N_cells_x = {n}\nN_cells_y = {ny or n}\nRe = {re}\nconvection_scheme = QUICK\n"
             f"diffusion_order = 2\nmomentum_solver = gs_multicolour\npressure_solver = mg\n"
             f"mg_precision = mixed\nerror_tolerance = 0\nmaxit = 100000\nbackend = gpu\n"
             f"write_fields = False\n")
@@ -146,6 +146,26 @@ def test_rejected_guess_is_exactly_the_plain_solve():
         assert bits_equal(warm.get(f), plain.get(f)), f
     acc, rej = sg.warm_stats(warm)
     assert acc == 0 and rej == 25, (acc, rej)
+    plain.close()
+    warm.close()
+    net.close()
+
+
+def test_rejected_guess_wide_grid_is_exactly_the_plain_solve():
+    """Level 0 on a 64 x 2048 grid: the prediction's rows are 2048 wide, more than the guess
+    kernels copy to LDS, so they interpolate from global memory instead; the zero network's guess
+    is rejected every time and the run equals the plain mixed one bit for bit."""
+    net = sg.UNet(np.zeros(U.n_params(), np.float32))
+    plain = sg.Solver(_mixed_params(64, 400.0, False, ny=2048))
+    warm = sg.Solver(_mixed_params(64, 400.0, True, ny=2048))
+    sg.attach_warm_start(warm, net, level=0)
+    assert sg.warm_level(warm) == 0
+    plain.advance(12)
+    warm.advance(12)
+    for f in ("u", "v", "p"):
+        assert bits_equal(warm.get(f), plain.get(f)), f
+    acc, rej = sg.warm_stats(warm)
+    assert acc == 0 and rej == 12, (acc, rej)
     plain.close()
     warm.close()
     net.close()
